@@ -1,0 +1,206 @@
+/*
+ * wipems_fdtd.h -- C ABI of the MI355X (gfx950) FDTD engine.
+ *
+ * This is the boundary an openEMS-style host plugs a device engine into.  The
+ * entry points replace the hot-path pieces of the reference:
+ *
+ *   fdtd_create / fdtd_destroy      Engine::New / Engine::Init / Engine::Reset
+ *                                   (FDTD/engine.cpp:25-57,98-106), allocation
+ *                                   of the volt/curr arrays
+ *   fdtd_set_coeffs                 Operator vv/vi/ii/iv arrays
+ *                                   (FDTD/operator.h:63-78, FDTD/operator_sse.h:76-80)
+ *   fdtd_set_coeffs_compressed      Operator_SSE_Compressed::m_Op_index and
+ *                                   f4_{vv,vi,iv,ii}_Compressed
+ *                                   (FDTD/operator_sse_compressed.h:76-80)
+ *   fdtd_set_excitation             Operator_Ext_Excitation arrays + Excitation signals
+ *                                   (FDTD/extensions/operator_ext_excitation.h:68-81,
+ *                                    FDTD/excitation.cpp:151-177), consumed by
+ *                                   Engine_Ext_Excitation (engine_ext_excitation.cpp:33-94)
+ *   fdtd_add_upml                   Operator_Ext_UPML vv/vvfo/vvfn/ii/iifo/iifn
+ *                                   (FDTD/extensions/operator_ext_upml.h:98-103),
+ *                                   consumed by Engine_Ext_UPML (engine_ext_upml.cpp:57-234)
+ *   fdtd_add_ade                    Operator_Ext_LorentzMaterial per-order lists
+ *                                   (FDTD/extensions/operator_ext_lorentzmaterial.h:49-59),
+ *                                   consumed by Engine_Ext_LorentzMaterial / _Dispersive
+ *                                   (engine_ext_lorentzmaterial.cpp:79-168, engine_ext_dispersive.cpp:76-127)
+ *   fdtd_set_tfsf                   Operator_Ext_TFSF delay/amp tables
+ *                                   (FDTD/extensions/operator_ext_tfsf.h:71-78),
+ *                                   consumed by Engine_Ext_TFSF (engine_ext_tfsf.cpp:36-215)
+ *   fdtd_iterate                    Engine::IterateTS (FDTD/engine.cpp:255-274)
+ *   fdtd_get_num_timesteps          Engine::GetNumberOfTimesteps (FDTD/engine.h:48)
+ *   fdtd_get_value / fdtd_set_value Engine::GetVolt/GetCurr/SetVolt/SetCurr (FDTD/engine.h:52-61)
+ *   fdtd_get_field / fdtd_set_field bulk form of the same accessors (field dumps)
+ *   fdtd_add_probe, fdtd_read_probe ProcessVoltage / ProcessCurrent integrals and
+ *                                   ProcessIntegral::Process sampling + FD accumulation
+ *                                   (Common/processintegral.cpp:134-171,
+ *                                    FDTD/engine_interface_fdtd.cpp:200-226,
+ *                                    Common/processcurrent.cpp:93-168)
+ *   fdtd_calc_energy                Engine_Interface_SSE_FDTD::CalcFastEnergy
+ *                                   (FDTD/engine_interface_sse_fdtd.cpp:37-69), in FP64
+ *
+ * Conventions: every function returns 0 on success and a negative FDTD_ERR_*
+ * code on failure; fdtd_last_error() returns the message of the last failure
+ * on the calling thread.  No C++ types and no exceptions cross this boundary.
+ * The caller owns every host buffer it passes; the library owns device memory
+ * and copies what it is given at call time.  One handle is used by one host
+ * thread at a time; work is ordered on the handle's own HIP stream.
+ *
+ * Array layout at this boundary is the reference's basic layout:
+ * field/coefficient arrays are [3][Nx][Ny][Nz] row-major (z fastest),
+ * matching volt[n][x][y][z] of FDTD/engine.h:95-96.
+ */
+#ifndef WIPEMS_FDTD_H
+#define WIPEMS_FDTD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define FDTD_OK            0
+#define FDTD_ERR_ARG      -1   /* invalid argument                          */
+#define FDTD_ERR_HIP      -2   /* HIP runtime error                         */
+#define FDTD_ERR_STATE    -3   /* call not valid in the current state       */
+#define FDTD_ERR_RANGE    -4   /* index or size out of range                */
+#define FDTD_ERR_NOMEM    -5   /* device or host allocation failed          */
+#define FDTD_ERR_NODEVICE -6   /* no usable gfx950 device                   */
+
+/* ---- field storage precision of a handle ---- */
+#define FDTD_PREC_FP32 0       /* float fields, float coefficients (the drop-in)      */
+#define FDTD_PREC_FP64 1       /* double fields and coefficients (FP64 oracle engine)  */
+#define FDTD_PREC_FP16 2       /* half fields with per-component scales, FP32 math     */
+#define FDTD_PREC_BF16 3       /* bfloat16 fields with per-component scales, FP32 math */
+
+/* ---- element type tag of host arrays passed in or out ---- */
+#define FDTD_DTYPE_F32 0
+#define FDTD_DTYPE_F64 1
+
+/* ---- which field ---- */
+#define FDTD_VOLT 0
+#define FDTD_CURR 1
+
+/* ---- probe kinds ---- */
+#define FDTD_PROBE_VOLTAGE 0   /* ProcessVoltage: line integral of volt           */
+#define FDTD_PROBE_CURRENT 1   /* ProcessCurrent: loop integral of curr, dual time */
+
+typedef struct fdtd_engine fdtd_engine;
+
+typedef struct fdtd_grid_desc {
+    uint32_t num_lines[3];     /* mesh lines per axis (Operator::GetNumberOfLines) */
+    double   dT;               /* timestep in seconds (Operator::GetTimestep)      */
+    int32_t  precision;        /* FDTD_PREC_*                                       */
+    uint32_t flags;            /* reserved, pass 0                                  */
+} fdtd_grid_desc;
+
+/* Excitation: signals and the sparse soft-source edge lists.  Arrays of
+ * positions are given as x[], y[], z[] blocks of length count ([3][count]). */
+typedef struct fdtd_excitation {
+    uint32_t       length;      /* Excitation::GetLength                          */
+    uint32_t       period;      /* signal period in timesteps, 0 = not periodic   */
+    const double*  signal_volt; /* [length] Excitation::GetVoltageSignal          */
+    const double*  signal_curr; /* [length] Excitation::GetCurrentSignal          */
+    uint32_t       volt_count;
+    const uint32_t* volt_pos;   /* [3][volt_count]                                */
+    const uint32_t* volt_dir;   /* [volt_count]                                   */
+    const double*  volt_amp;    /* [volt_count]                                   */
+    const uint32_t* volt_delay; /* [volt_count] in timesteps                      */
+    uint32_t       curr_count;
+    const uint32_t* curr_pos;
+    const uint32_t* curr_dir;
+    const double*  curr_amp;
+    const uint32_t* curr_delay;
+} fdtd_excitation;
+
+/* ---- lifetime ---- */
+int  fdtd_create(const fdtd_grid_desc* desc, fdtd_engine** out);
+int  fdtd_destroy(fdtd_engine* h);
+int  fdtd_last_error(char* buf, size_t len);
+/* Library version string and the device name (for reports). */
+const char* fdtd_version(void);
+int  fdtd_device_name(char* buf, size_t len);
+
+/* ---- operator (set once, before the first fdtd_iterate) ---- */
+/* Full coefficient arrays, each [3][Nx][Ny][Nz]. */
+int  fdtd_set_coeffs(fdtd_engine* h, int dtype,
+                     const void* vv, const void* vi, const void* ii, const void* iv);
+/* Compressed coefficients: per-cell index [Nx][Ny][Nz] into a table of
+ * n_entries rows of 12 values {vv0,vv1,vv2, vi0,vi1,vi2, ii0,ii1,ii2, iv0,iv1,iv2}. */
+int  fdtd_set_coeffs_compressed(fdtd_engine* h, const uint32_t* index,
+                                int dtype, const void* table, uint32_t n_entries);
+/* Per-component storage scales for FP16/BF16 handles (volt, curr); stored
+ * value = physical value / scale.  Ignored for FP32/FP64 handles. */
+int  fdtd_set_storage_scale(fdtd_engine* h, const double volt_scale[3], const double curr_scale[3]);
+
+int  fdtd_set_excitation(fdtd_engine* h, const fdtd_excitation* exc);
+
+/* One UPML slab: lines start..start+num_lines-1 per axis; six coefficient
+ * arrays, each [3][nx][ny][nz].  Slabs must not overlap. */
+int  fdtd_add_upml(fdtd_engine* h, const uint32_t start[3], const uint32_t num_lines[3], int dtype,
+                   const void* vv, const void* vvfo, const void* vvfn,
+                   const void* ii, const void* iifo, const void* iifn);
+
+/* One dispersive (Drude/Lorentz/Debye ADE) order: positions [3][count] and
+ * coefficient lists [3][count].  Pass NULL for the lists of a disabled part;
+ * flags bit0 = volt ADE on, bit1 = curr ADE on, bit2 = volt Lorentz on,
+ * bit3 = curr Lorentz on (operator_ext_lorentzmaterial.h m_*_ADE_On). */
+int  fdtd_add_ade(fdtd_engine* h, uint32_t count, const uint32_t* pos, int dtype,
+                  const void* v_int, const void* v_ext, const void* v_lor,
+                  const void* i_int, const void* i_ext, const void* i_lor, uint32_t flags);
+
+/* Total-field/scattered-field injection list.  which = FDTD_VOLT (added after
+ * the voltage update using the current signal) or FDTD_CURR (added after the
+ * current update using the voltage signal).  Each entry adds
+ *   (1-delta)*amp*s[L[delay]] + delta*amp*s[L[delay+1]]
+ * to component comp at pos, L[n] being the delay lookup of engine_ext_tfsf.cpp:43-52. */
+int  fdtd_set_tfsf(fdtd_engine* h, int which, uint32_t count, const uint32_t* pos,
+                   const uint32_t* comp, const uint32_t* delay, const double* delta,
+                   const double* amp, uint32_t max_delay);
+
+/* ---- probes ---- */
+/* Voltage: line from start to stop (one axis differs).  Current: rectangle
+ * start..stop with normal norm_dir; inside_mask bits 0-2 = start inside per
+ * axis, bits 3-5 = stop inside per axis (ProcessIntegral m_start_inside /
+ * m_stop_inside), pass 0x3f for a probe inside the domain.
+ * td_interval: time-domain sampling interval in timesteps (0 = none).
+ * fd_interval: FD accumulation interval (0 = none) for n_freq frequencies. */
+int  fdtd_add_probe(fdtd_engine* h, int kind, const uint32_t start[3], const uint32_t stop[3],
+                    int norm_dir, uint32_t inside_mask, double weight,
+                    uint32_t td_interval, uint32_t fd_interval,
+                    const double* freqs, uint32_t n_freq, int* id);
+int  fdtd_probe_num_samples(fdtd_engine* h, int id, uint32_t* n);
+/* Copies min(n, max_n) samples (time in s, weighted value). */
+int  fdtd_read_probe(fdtd_engine* h, int id, double* t, double* val, uint32_t max_n);
+/* FD accumulators (weighted, FP64): re[n_freq], im[n_freq]. */
+int  fdtd_read_probe_fd(fdtd_engine* h, int id, double* re, double* im);
+
+/* ---- time stepping ---- */
+int  fdtd_iterate(fdtd_engine* h, uint32_t nsteps);
+int  fdtd_synchronize(fdtd_engine* h);
+int  fdtd_get_num_timesteps(fdtd_engine* h, uint32_t* numTS);
+/* Use a captured HIP graph for the step loop (default on). */
+int  fdtd_set_use_graph(fdtd_engine* h, int enable);
+/* Kernel timing: while enabled, fdtd_iterate runs eagerly and records HIP
+ * events around each main update kernel on the handle's stream.
+ * fdtd_get_kernel_times returns the summed milliseconds and launch counts of
+ * the voltage and current update kernels since the last reset. */
+int  fdtd_set_kernel_timing(fdtd_engine* h, int enable);
+int  fdtd_get_kernel_times(fdtd_engine* h, double* volt_ms, double* curr_ms,
+                           uint32_t* volt_launches, uint32_t* curr_launches);
+
+/* ---- field access ---- */
+int  fdtd_calc_energy(fdtd_engine* h, double* energy);
+int  fdtd_get_field(fdtd_engine* h, int which, int dtype, void* out);
+int  fdtd_set_field(fdtd_engine* h, int which, int dtype, const void* in);
+int  fdtd_get_value(fdtd_engine* h, int which, uint32_t n, uint32_t x, uint32_t y, uint32_t z, double* v);
+int  fdtd_set_value(fdtd_engine* h, int which, uint32_t n, uint32_t x, uint32_t y, uint32_t z, double v);
+
+/* Device bytes held by the handle (fields, coefficients, extensions). */
+int  fdtd_device_bytes(fdtd_engine* h, uint64_t* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WIPEMS_FDTD_H */

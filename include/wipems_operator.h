@@ -1,0 +1,121 @@
+/*
+ * wipems_operator.h -- C ABI of the host-side operator builder.
+ *
+ * This library plays the part of openEMS's Operator for the scenarios this
+ * project runs (it replaces CSXCAD geometry with boxes and spheres).  It
+ * builds the equivalent-circuit coefficients and the extension tables that
+ * the engine ABI (wipems_fdtd.h) consumes:
+ *
+ *   wop_build         Operator::CalcECOperator (FDTD/operator.cpp:973-1084):
+ *                     Calc_EC / AverageMatQuarterCell (:1406-1518) / Calc_ECPos (:1176-1224),
+ *                     CalcTimestep_Var3 (:1897-1971), Calc_ECOperatorPos (:943-971),
+ *                     ApplyElectricBC (:1086-1125), CalcPEC (:1973-2020),
+ *                     Calc_LumpedElements (:1535-1712), ApplyMagneticBC (:1127-1174),
+ *                     then the extensions' BuildExtension in creation order:
+ *                     Operator_Ext_Excitation (operator_ext_excitation.cpp:101-285),
+ *                     Operator_Ext_UPML (operator_ext_upml.cpp:77-466),
+ *                     Operator_Ext_LorentzMaterial (operator_ext_lorentzmaterial.cpp:117-438),
+ *                     and the excitation signal (FDTD/excitation.cpp:96-177).
+ *   wop_compress      Operator_SSE_Compressed::CompressOperator
+ *                     (FDTD/operator_sse_compressed.cpp:120-176), per cell instead
+ *                     of per 4-lane vector.
+ *
+ * The build runs either with FDTD_FLOAT = float semantics (every EC value and
+ * coefficient rounded to float where the reference stores float: prec 0) or
+ * fully in double (prec 1, the FP64 oracle operator).
+ *
+ * Status: 0 = OK, negative = error; wop_last_error() gives the message.
+ */
+#ifndef WIPEMS_OPERATOR_H
+#define WIPEMS_OPERATOR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct wop_operator wop_operator;
+
+/* boundary condition codes (openems.cpp BoundaryCond): 0 PEC, 1 PMC, 2 MUR, 3 PML, -1 none */
+#define WOP_BC_PEC  0
+#define WOP_BC_PMC  1
+#define WOP_BC_MUR  2
+#define WOP_BC_PML  3
+
+/* excitation signal types (FDTD/excitation.h) */
+#define WOP_EXC_GAUSS   0
+#define WOP_EXC_SINE    1
+#define WOP_EXC_DIRAC   2
+#define WOP_EXC_STEP    3
+
+int  wop_create(wop_operator** out);
+int  wop_destroy(wop_operator* op);
+int  wop_last_error(char* buf, size_t len);
+
+/* mesh lines in drawing units; unit = metres per drawing unit (gridDelta) */
+int  wop_set_mesh(wop_operator* op, int axis, const double* lines, uint32_t n);
+int  wop_set_grid_delta(wop_operator* op, double unit);
+int  wop_set_boundary(wop_operator* op, const int bc[6], const uint32_t pml_size[6]);
+int  wop_set_timestep_factor(wop_operator* op, double factor);
+/* optional forced timestep (openems.cpp TimeStep): 0 = calculate */
+int  wop_set_timestep(wop_operator* op, double dT);
+int  wop_set_threads(wop_operator* op, int nthreads);
+
+/* properties (CSXCAD stand-ins); each returns its id */
+int  wop_add_material(wop_operator* op, double eps_r, double kappa, double mue_r, double sigma, int* id);
+int  wop_add_metal(wop_operator* op, int* id);
+/* Drude/Lorentz material of order 1: plasma frequency (Hz), relaxation time (s),
+ * Lorentz pole frequency (Hz, 0 = Drude) for the electric response. */
+int  wop_add_lorentz_material(wop_operator* op, double eps_r, double kappa, double mue_r, double sigma,
+                              double eps_plasma_freq, double eps_relax_time, double eps_lor_pole_freq, int* id);
+/* lumped parallel RC (R or C <= 0 means not given), direction ny, metal caps */
+int  wop_add_lumped_element(wop_operator* op, int ny, double R, double C, int caps, int* id);
+/* excitation property: type 0 soft E, 1 hard E, 2 soft H, 3 hard H; value vector; delay (s) */
+int  wop_add_excitation(wop_operator* op, int type, const double value[3], double delay, int* id);
+int  wop_add_box(wop_operator* op, int prop, const double start[3], const double stop[3], int priority);
+int  wop_add_sphere(wop_operator* op, int prop, const double center[3], double radius, int priority);
+
+/* excitation time signal */
+int  wop_set_excite_signal(wop_operator* op, int type, double f0, double fc);
+
+/* build: prec 0 = float storage semantics, 1 = double; nr_ts = max timesteps
+ * (caps the signal length as Excitation::CalcGaussianPulsExcitation does) */
+int  wop_build(wop_operator* op, int prec, uint32_t nr_ts);
+
+/* results */
+int  wop_get_num_lines(wop_operator* op, uint32_t n[3]);
+int  wop_get_timestep(wop_operator* op, double* dT);
+/* Nyquist number of the excitation (Excitation::GetNyquistNum) and the
+ * step of maximum excitation amplitude (GetMaxExcitationTimestep) */
+int  wop_get_excite_info(wop_operator* op, uint32_t* length, uint32_t* nyquist, uint32_t* max_step);
+int  wop_get_signal(wop_operator* op, double* volt, double* curr);
+/* coefficients as dense [3][Nx][Ny][Nz] arrays in dtype (0 f32, 1 f64) */
+int  wop_get_coeffs(wop_operator* op, int dtype, void* vv, void* vi, void* ii, void* iv);
+/* EC values after Calc_EC (before the operator build), [3][N] each, double */
+int  wop_get_ec(wop_operator* op, double* C, double* G, double* L, double* R);
+/* compression of the coefficients: returns the number of unique 12-tuples and
+ * (if index/table are non-null) the per-cell index and the table [n][12] */
+int  wop_compress(wop_operator* op, uint32_t* n_entries, uint32_t* index, double* table, uint32_t max_entries);
+
+int  wop_get_excitation_counts(wop_operator* op, uint32_t* volt_count, uint32_t* curr_count);
+int  wop_get_excitation(wop_operator* op, uint32_t* volt_pos, uint32_t* volt_dir, double* volt_amp, uint32_t* volt_delay,
+                        uint32_t* curr_pos, uint32_t* curr_dir, double* curr_amp, uint32_t* curr_delay);
+
+int  wop_get_num_upml(wop_operator* op, uint32_t* n);
+int  wop_get_upml_range(wop_operator* op, uint32_t k, uint32_t start[3], uint32_t num_lines[3]);
+int  wop_get_upml(wop_operator* op, uint32_t k, int dtype, void* vv, void* vvfo, void* vvfn, void* ii, void* iifo, void* iifn);
+
+int  wop_get_num_ade(wop_operator* op, uint32_t* orders);
+int  wop_get_ade_info(wop_operator* op, uint32_t order, uint32_t* count, uint32_t* flags);
+int  wop_get_ade(wop_operator* op, uint32_t order, uint32_t* pos, int dtype, void* v_int, void* v_ext, void* v_lor,
+                 void* i_int, void* i_ext, void* i_lor);
+
+/* snapping helpers (Operator::SnapToMeshLine, FDTD/operator.cpp:244-273) */
+int  wop_snap_to_mesh_line(wop_operator* op, int axis, double coord, int dual, uint32_t* idx, int* inside);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WIPEMS_OPERATOR_H */

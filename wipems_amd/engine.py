@@ -1,0 +1,260 @@
+"""Python mirror of the openEMS Engine interface over the HIP engine C ABI.
+
+``Engine`` wraps one ``fdtd_engine`` handle (include/wipems_fdtd.h).  Method
+names follow FDTD/engine.h: ``IterateTS``, ``GetNumberOfTimesteps``,
+``GetVolt``/``GetCurr``/``SetVolt``/``SetCurr``; probes follow
+ProcessVoltage/ProcessCurrent.  All compute runs on the GPU; there is no CPU
+fallback (constructing an Engine without the built library or without a GPU
+raises FdtdError).
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, byref, c_double, c_int, c_uint32, c_uint64, c_void_p
+
+import numpy as np
+
+from ._native import ExcitationDesc, FdtdError, GridDesc, check, engine_lib
+from .operator import OperatorData
+
+PREC_FP32, PREC_FP64, PREC_FP16, PREC_BF16 = 0, 1, 2, 3
+_PREC_NAMES = {"fp32": 0, "fp64": 1, "fp16": 2, "bf16": 3}
+VOLT, CURR = 0, 1
+
+
+def _dp(a):
+    return a.ctypes.data_as(POINTER(c_double))
+
+
+def _up(a):
+    return a.ctypes.data_as(POINTER(c_uint32))
+
+
+def _vp(a):
+    return a.ctypes.data_as(c_void_p)
+
+
+def _dtype_code(a):
+    if a.dtype == np.float64:
+        return 1
+    if a.dtype == np.float32:
+        return 0
+    raise FdtdError("arrays must be float32 or float64")
+
+
+def device_name():
+    lib = engine_lib()
+    buf = ctypes.create_string_buffer(256)
+    check(lib, lib.fdtd_device_name(buf, 256), "fdtd_last_error")
+    return buf.value.decode()
+
+
+class Engine:
+    """One FDTD domain on the GPU (Engine_HIP)."""
+
+    def __init__(self, num_lines, dT, precision="fp32"):
+        self._lib = engine_lib()
+        prec = _PREC_NAMES[precision] if isinstance(precision, str) else int(precision)
+        d = GridDesc((c_uint32 * 3)(*num_lines), float(dT), prec, 0)
+        h = c_void_p()
+        self._chk(self._lib.fdtd_create(byref(d), byref(h)))
+        self._h = h
+        self.num_lines = tuple(int(x) for x in num_lines)
+        self.dT = float(dT)
+        self.precision = prec
+        self.probe_ids = []
+        self._keep = []
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.fdtd_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        return check(self._lib, rc, "fdtd_last_error")
+
+    # ---- operator ----
+    def set_coeffs(self, vv, vi, ii, iv):
+        arrs = [np.ascontiguousarray(a) for a in (vv, vi, ii, iv)]
+        code = _dtype_code(arrs[0])
+        arrs = [a.astype(arrs[0].dtype, copy=False) for a in arrs]
+        self._chk(self._lib.fdtd_set_coeffs(self._h, code, *[_vp(a) for a in arrs]))
+
+    def set_coeffs_compressed(self, index, table):
+        idx = np.ascontiguousarray(index, dtype=np.uint32)
+        tab = np.ascontiguousarray(table, dtype=np.float64)
+        self._chk(self._lib.fdtd_set_coeffs_compressed(self._h, _up(idx), 1, _vp(tab), int(tab.shape[0])))
+
+    def set_storage_scale(self, volt_scale, curr_scale):
+        v = (c_double * 3)(*volt_scale)
+        c = (c_double * 3)(*curr_scale)
+        self._chk(self._lib.fdtd_set_storage_scale(self._h, v, c))
+
+    def set_excitation(self, exc):
+        keep = []
+
+        def arr(a, dt):
+            b = np.ascontiguousarray(a, dtype=dt)
+            if b.size == 0:
+                b = np.zeros(1, dtype=dt)
+            keep.append(b)
+            return b
+
+        sv, si = arr(exc.signal_volt, np.float64), arr(exc.signal_curr, np.float64)
+        vp, vd, va, vl = arr(exc.volt_pos, np.uint32), arr(exc.volt_dir, np.uint32), arr(exc.volt_amp, np.float64), arr(exc.volt_delay, np.uint32)
+        cp, cd, ca, cl = arr(exc.curr_pos, np.uint32), arr(exc.curr_dir, np.uint32), arr(exc.curr_amp, np.float64), arr(exc.curr_delay, np.uint32)
+        d = ExcitationDesc(exc.length, exc.period, _dp(sv), _dp(si),
+                           len(exc.volt_dir), _up(vp), _up(vd), _dp(va), _up(vl),
+                           len(exc.curr_dir), _up(cp), _up(cd), _dp(ca), _up(cl))
+        self._chk(self._lib.fdtd_set_excitation(self._h, byref(d)))
+
+    def add_upml(self, slab):
+        arrs = [np.ascontiguousarray(a) for a in slab.arrays]
+        code = _dtype_code(arrs[0])
+        st = (c_uint32 * 3)(*[int(x) for x in slab.start])
+        nl = (c_uint32 * 3)(*[int(x) for x in slab.num_lines])
+        self._chk(self._lib.fdtd_add_upml(self._h, st, nl, code, *[_vp(a) for a in arrs]))
+
+    def add_ade(self, order):
+        pos = np.ascontiguousarray(order.pos, dtype=np.uint32)
+        lists = [np.ascontiguousarray(a) for a in order.lists]
+        code = _dtype_code(lists[0])
+        self._chk(self._lib.fdtd_add_ade(self._h, pos.shape[1], _up(pos), code, *[_vp(a) for a in lists], int(order.flags)))
+
+    def set_tfsf(self, which, pos, comp, delay, delta, amp, max_delay):
+        pos = np.ascontiguousarray(pos, dtype=np.uint32)
+        comp = np.ascontiguousarray(comp, dtype=np.uint32)
+        delay = np.ascontiguousarray(delay, dtype=np.uint32)
+        delta = np.ascontiguousarray(delta, dtype=np.float64)
+        amp = np.ascontiguousarray(amp, dtype=np.float64)
+        self._chk(self._lib.fdtd_set_tfsf(self._h, int(which), len(comp), _up(pos), _up(comp), _up(delay), _dp(delta), _dp(amp), int(max_delay)))
+
+    def add_probe(self, p):
+        fr = np.ascontiguousarray(p.freqs if len(p.freqs) else [0.0], dtype=np.float64)
+        st = (c_uint32 * 3)(*p.start)
+        sp = (c_uint32 * 3)(*p.stop)
+        i = c_int()
+        self._chk(self._lib.fdtd_add_probe(self._h, int(p.kind), st, sp, int(p.norm_dir), int(p.inside_mask), float(p.weight),
+                                           int(p.td_interval), int(p.fd_interval), _dp(fr), len(p.freqs), byref(i)))
+        self.probe_ids.append(i.value)
+        return i.value
+
+    # ---- stepping (Engine::IterateTS) ----
+    def IterateTS(self, n):
+        self._chk(self._lib.fdtd_iterate(self._h, int(n)))
+
+    iterate = IterateTS
+
+    def synchronize(self):
+        self._chk(self._lib.fdtd_synchronize(self._h))
+
+    def GetNumberOfTimesteps(self):
+        t = c_uint32()
+        self._chk(self._lib.fdtd_get_num_timesteps(self._h, byref(t)))
+        return t.value
+
+    def set_use_graph(self, on):
+        self._chk(self._lib.fdtd_set_use_graph(self._h, int(bool(on))))
+
+    def set_kernel_timing(self, on):
+        self._chk(self._lib.fdtd_set_kernel_timing(self._h, int(bool(on))))
+
+    def kernel_times(self):
+        a, b = c_double(), c_double()
+        na, nb = c_uint32(), c_uint32()
+        self._chk(self._lib.fdtd_get_kernel_times(self._h, byref(a), byref(b), byref(na), byref(nb)))
+        return a.value, b.value, na.value, nb.value
+
+    # ---- field access ----
+    def energy(self):
+        e = c_double()
+        self._chk(self._lib.fdtd_calc_energy(self._h, byref(e)))
+        return e.value
+
+    CalcFastEnergy = energy
+
+    def get_field(self, which, dtype=np.float64):
+        out = np.empty((3,) + self.num_lines, dtype=dtype)
+        self._chk(self._lib.fdtd_get_field(self._h, int(which), 1 if dtype == np.float64 else 0, _vp(out)))
+        return out
+
+    def set_field(self, which, arr):
+        a = np.ascontiguousarray(arr)
+        self._chk(self._lib.fdtd_set_field(self._h, int(which), _dtype_code(a), _vp(a)))
+
+    def GetVolt(self, n, x, y, z):
+        v = c_double()
+        self._chk(self._lib.fdtd_get_value(self._h, VOLT, n, x, y, z, byref(v)))
+        return v.value
+
+    def GetCurr(self, n, x, y, z):
+        v = c_double()
+        self._chk(self._lib.fdtd_get_value(self._h, CURR, n, x, y, z, byref(v)))
+        return v.value
+
+    def SetVolt(self, n, x, y, z, value):
+        self._chk(self._lib.fdtd_set_value(self._h, VOLT, n, x, y, z, float(value)))
+
+    def SetCurr(self, n, x, y, z, value):
+        self._chk(self._lib.fdtd_set_value(self._h, CURR, n, x, y, z, float(value)))
+
+    def read_probe(self, pid):
+        n = c_uint32()
+        self._chk(self._lib.fdtd_probe_num_samples(self._h, pid, byref(n)))
+        t = np.empty(max(n.value, 1))
+        v = np.empty(max(n.value, 1))
+        self._chk(self._lib.fdtd_read_probe(self._h, pid, _dp(t), _dp(v), n.value))
+        return t[:n.value], v[:n.value]
+
+    def read_probe_fd(self, pid, n_freq):
+        re = np.zeros(max(n_freq, 1))
+        im = np.zeros(max(n_freq, 1))
+        self._chk(self._lib.fdtd_read_probe_fd(self._h, pid, _dp(re), _dp(im)))
+        return re[:n_freq] + 1j * im[:n_freq]
+
+    def device_bytes(self):
+        b = c_uint64()
+        self._chk(self._lib.fdtd_device_bytes(self._h, byref(b)))
+        return b.value
+
+
+def engine_from_operator(data: OperatorData, precision="fp32", compressed=None, max_entries=65536, scales=None):
+    """Create an Engine and load the operator, extensions and probes into it.
+
+    compressed: True -> per-cell index + table (fails if more than max_entries
+    unique tuples), False -> full coefficient arrays, None -> compress when the
+    table fits in max_entries."""
+    eng = Engine(data.num_lines, data.dT, precision)
+    use_cmp = compressed
+    if compressed is None or compressed:
+        if data.index is None:
+            res = data.compress(max_entries if compressed is None else 0)
+        else:
+            res = (data.index, data.table)
+        use_cmp = res is not None
+    if use_cmp:
+        eng.set_coeffs_compressed(data.index, data.table)
+    else:
+        eng.set_coeffs(data.vv, data.vi, data.ii, data.iv)
+    eng.compressed = bool(use_cmp)
+    if scales is not None:
+        eng.set_storage_scale(*scales)
+    if data.excitation is not None and data.excitation.length:
+        eng.set_excitation(data.excitation)
+    for s in data.upml:
+        eng.add_upml(s)
+    for a in data.ade:
+        eng.add_ade(a)
+    if data.tfsf:
+        for which in (0, 1):
+            t = data.tfsf[which]
+            eng.set_tfsf(which, t["pos"], t["comp"], t["delay"], t["delta"], t["amp"], data.tfsf["max_delay"])
+    for p in data.probes:
+        eng.add_probe(p)
+    return eng

@@ -1,0 +1,1345 @@
+// operator.cpp -- host-side equivalent-circuit operator builder (C ABI in
+// include/wipems_operator.h).
+//
+// It restates openEMS's Cartesian Operator build for geometry made of boxes
+// and spheres (the CSXCAD stand-in), so that the device engine can be fed the
+// same vv/vi/ii/iv arrays and extension tables the reference engine would use.
+// The template parameter F is the reference's FDTD_FLOAT: float reproduces the
+// reference's rounding points (EC_* and the coefficient arrays are float),
+// double gives the FP64 oracle operator.
+//
+// Compiled with -ffp-contract=off so that double expressions round like the
+// reference's (baseline x86-64, no FMA).
+#include <algorithm>
+#include <array>
+#include <atomic>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "wipems_operator.h"
+
+namespace {
+
+const double EPS0 = 8.85418781762e-12;  // tools/constants.h:22
+const double MUE0 = 1.256637062e-6;     // tools/constants.h:23
+const double C0 = 299792458;            // tools/constants.h:24
+const double Z0 = 376.730313461;        // tools/constants.h:25
+const double PI_ = 3.141592653589793238462643383279;  // tools/constants.h:26
+
+thread_local std::string g_err;
+int fail(int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+enum PropKind { P_MATERIAL = 1, P_METAL = 2, P_LUMPED = 4, P_EXCITATION = 8 };
+
+struct Property {
+    int kind = 0;
+    bool lorentz = false;
+    double eps = 1, kappa = 0, mue = 1, sigma = 0;
+    double eps_plasma_f = 0, eps_relax_t = 0, eps_lor_f = 0;
+    int ny = 0;
+    double R = -1, C = -1;
+    bool caps = false;
+    int exc_type = 0;
+    double exc_val[3] = {0, 0, 0};
+    double delay = 0;
+};
+
+struct Primitive {
+    int shape = 0;            // 0 box, 1 sphere
+    double a[3] = {0, 0, 0};  // box min / sphere centre
+    double b[3] = {0, 0, 0};  // box max / (radius, -, -)
+    double bb0[3], bb1[3];    // bounding box
+    int priority = 0;
+    int prop = 0;
+    // CSPrimBox::IsInside is inclusive on both faces; spheres use dist <= r
+    bool inside(const double c[3]) const {
+        if (shape == 0) {
+            for (int n = 0; n < 3; ++n)
+                if (c[n] < a[n] || c[n] > b[n]) return false;
+            return true;
+        }
+        double d2 = 0;
+        for (int n = 0; n < 3; ++n) { double t = c[n] - a[n]; d2 += t * t; }
+        return d2 <= b[0] * b[0];
+    }
+};
+
+}  // namespace
+
+struct wop_operator {
+    std::vector<double> lines[3];
+    uint32_t N[3] = {0, 0, 0};
+    double gridDelta = 1;
+    int BC[6] = {0, 0, 0, 0, 0, 0};
+    uint32_t pml_size[6] = {0, 0, 0, 0, 0, 0};
+    double ts_factor = 1;
+    double forced_dT = 0;
+    int nthreads = 0;
+
+    std::vector<Property> props;
+    std::vector<Primitive> prims;   // kept sorted by descending priority (stable)
+
+    int exc_sig_type = WOP_EXC_GAUSS;
+    double f0 = 0, fc = 0;
+
+    // ---- results ----
+    bool built = false;
+    int prec = 0;
+    double dT = 0;
+    std::vector<double> ec[4];             // C,G,L,R as stored (F rounded), [3][N]
+    std::vector<double> coef[4];           // vv, vi, ii, iv as stored (F rounded), [3][N]
+    std::vector<double> sig_v, sig_i;
+    uint32_t exc_length = 0, nyquist = 0, max_step = 0;
+
+    struct Exc { uint32_t pos[3]; uint32_t dir; double amp; uint32_t delay; };
+    std::vector<Exc> exc_v, exc_i;
+
+    struct Upml {
+        uint32_t start[3], n[3];
+        std::vector<double> a[6];          // vv, vvfo, vvfn, ii, iifo, iifn [3][n]
+    };
+    std::vector<Upml> upml;
+
+    struct Ade {
+        std::vector<uint32_t> pos[3];
+        std::vector<double> lists[6];      // v_int, v_ext, v_lor, i_int, i_ext, i_lor [3][count]
+        uint32_t flags = 0;
+    };
+    std::vector<Ade> ade;
+
+    // ---- mesh helpers (FDTD/operator.cpp:136-242) ----
+    size_t idx(uint32_t x, uint32_t y, uint32_t z) const { return ((size_t)x * N[1] + y) * N[2] + z; }
+    size_t nd() const { return (size_t)N[0] * N[1] * N[2]; }
+
+    double DiscLine(int n, uint32_t pos, bool dual) const {
+        if (pos >= N[n]) return 0.0;
+        if (!dual) return lines[n][pos];
+        if (pos < N[n] - 1) return 0.5 * (lines[n][pos] + lines[n][pos + 1]);
+        return lines[n][pos] + 0.5 * (lines[n][pos] - lines[n][pos - 1]);
+    }
+    double DiscDelta(int n, uint32_t pos, bool dual) const {
+        if (pos >= N[n]) return 0.0;
+        if (!dual) {
+            if (pos < N[n] - 1) return DiscLine(n, pos + 1, false) - DiscLine(n, pos, false);
+            return DiscLine(n, pos, false) - DiscLine(n, pos - 1, false);
+        }
+        if (pos > 0) return DiscLine(n, pos, true) - DiscLine(n, pos - 1, true);
+        return DiscLine(n, 1, false) - DiscLine(n, 0, false);
+    }
+    double RawDiscDelta(int n, int pos) const {
+        if (pos < 0) return lines[n][0] - lines[n][1];
+        if (pos >= (int)N[n] - 1) return lines[n][N[n] - 2] - lines[n][N[n] - 1];
+        return lines[n][pos + 1] - lines[n][pos];
+    }
+    double EdgeLength(int n, const uint32_t* pos, bool dual = false) const { return DiscDelta(n, pos[n], dual) * gridDelta; }
+    double NodeWidth(int n, const uint32_t* pos, bool dual) const { return EdgeLength(n, pos, !dual); }
+    double NodeWidthI(int n, const int* pos, bool dual) const {
+        if (pos[0] < 0 || pos[1] < 0 || pos[2] < 0) return 0.0;
+        uint32_t u[3] = {(uint32_t)pos[0], (uint32_t)pos[1], (uint32_t)pos[2]};
+        return NodeWidth(n, u, dual);
+    }
+    double NodeArea(int n, const uint32_t* pos, bool dual) const {
+        return NodeWidth((n + 1) % 3, pos, dual) * NodeWidth((n + 2) % 3, pos, dual);
+    }
+    double NodeAreaI(int n, const int* pos, bool dual) const {
+        if (pos[0] < 0 || pos[1] < 0 || pos[2] < 0) return 0.0;
+        uint32_t u[3] = {(uint32_t)pos[0], (uint32_t)pos[1], (uint32_t)pos[2]};
+        return NodeArea(n, u, dual);
+    }
+    double EdgeArea(int n, const uint32_t* pos, bool dual = false) const { return NodeArea(n, pos, dual); }
+    // GetYeeCoords, FDTD/operator.cpp:174-194
+    bool YeeCoords(int ny, const uint32_t* pos, double* c, bool dual) const {
+        for (int n = 0; n < 3; ++n) c[n] = DiscLine(n, pos[n], dual);
+        c[ny] = DiscLine(ny, pos[ny], !dual);
+        if (!dual) return pos[ny] < N[ny] - 1;
+        int nP = (ny + 1) % 3, nPP = (ny + 2) % 3;
+        return !(pos[nP] >= N[nP] - 1 || pos[nPP] >= N[nPP] - 1);
+    }
+    uint32_t SnapToMeshLine(int ny, double coord, bool& inside, bool dual) const {
+        inside = false;
+        if (coord < DiscLine(ny, 0, false)) return 0;
+        if (coord > DiscLine(ny, N[ny] - 1, false)) return N[ny] - 1;
+        inside = true;
+        if (!dual) {
+            for (uint32_t n = 0; n < N[ny]; ++n)
+                if (coord <= DiscLine(ny, n, true)) return n;
+        } else {
+            for (uint32_t n = 1; n < N[ny]; ++n)
+                if (coord <= DiscLine(ny, n, false)) return n - 1;
+        }
+        return 0;
+    }
+
+    // ---- geometry lookup (CSX->GetPropertyByCoordPriority) ----
+    // primitives are kept in descending priority order; the first primitive
+    // of a matching kind that contains the point wins
+    int PropertyAt(const double* c, int kinds, const std::vector<int>* cand = nullptr) const {
+        if (cand) {
+            for (int i : *cand) {
+                const Primitive& p = prims[i];
+                if ((props[p.prop].kind & kinds) && p.inside(c)) return p.prop;
+            }
+            return -1;
+        }
+        for (const Primitive& p : prims)
+            if ((props[p.prop].kind & kinds) && p.inside(c)) return p.prop;
+        return -1;
+    }
+    // candidate primitives whose bounding box touches [lo,hi] (drawing units)
+    std::vector<int> Candidates(const double lo[3], const double hi[3], int kinds) const {
+        std::vector<int> r;
+        for (size_t i = 0; i < prims.size(); ++i) {
+            const Primitive& p = prims[i];
+            if (!(props[p.prop].kind & kinds)) continue;
+            bool ok = true;
+            for (int n = 0; n < 3; ++n)
+                if (p.bb1[n] < lo[n] || p.bb0[n] > hi[n]) ok = false;
+            if (ok) r.push_back((int)i);
+        }
+        return r;
+    }
+    std::vector<int> RowCandidates(int x, int y, int kinds) const {
+        double lo[3], hi[3];
+        int pos[2] = {x, y};
+        for (int n = 0; n < 2; ++n) {
+            lo[n] = lines[n][std::max(0, pos[n] - 1)];
+            hi[n] = lines[n][std::min((int)N[n] - 1, pos[n] + 1)];
+            // quarter-cell and dual sample points stay within one cell of the row
+            double d = (hi[n] - lo[n]);
+            lo[n] -= d; hi[n] += d;
+        }
+        lo[2] = -std::numeric_limits<double>::infinity();
+        hi[2] = std::numeric_limits<double>::infinity();
+        return Candidates(lo, hi, kinds);
+    }
+
+    // GetMaterial (FDTD/operator.cpp:1251-1298)
+    double GetMaterial(const double* c, int type, const std::vector<int>& cand) const {
+        int p = PropertyAt(c, P_MATERIAL, &cand);
+        if (p >= 0) {
+            const Property& q = props[p];
+            switch (type) {
+            case 0: return q.eps;
+            case 1: return q.kappa;
+            case 2: return q.mue;
+            default: return q.sigma;
+            }
+        }
+        switch (type) {
+        case 0: return 1.0;
+        case 1: return 0.0;
+        case 2: return 1.0;
+        default: return 0.0;
+        }
+    }
+
+    // AverageMatQuarterCell (FDTD/operator.cpp:1406-1518)
+    void AverageMatQuarterCell(int ny, const uint32_t* pos, double* EffMat, const std::vector<int>& cand) const {
+        const int n = ny, nP = (n + 1) % 3, nPP = (n + 2) % 3;
+        double coord[3] = {lines[0][pos[0]], lines[1][pos[1]], lines[2][pos[2]]};
+        double shiftCoord[3];
+        const double delta = RawDiscDelta(n, pos[n]);
+        const double deltaP = RawDiscDelta(nP, pos[nP]);
+        const double deltaPP = RawDiscDelta(nPP, pos[nPP]);
+        const double delta_M = RawDiscDelta(n, (int)pos[n] - 1);
+        const double deltaP_M = RawDiscDelta(nP, (int)pos[nP] - 1);
+        const double deltaPP_M = RawDiscDelta(nPP, (int)pos[nPP] - 1);
+        int loc_pos[3] = {(int)pos[0], (int)pos[1], (int)pos[2]};
+        double A_n, area = 0;
+
+        shiftCoord[n] = coord[n] + delta * 0.5;
+        shiftCoord[nP] = coord[nP] + deltaP * 0.25;
+        shiftCoord[nPP] = coord[nPP] + deltaPP * 0.25;
+        A_n = NodeAreaI(ny, loc_pos, true);
+        EffMat[0] = GetMaterial(shiftCoord, 0, cand) * A_n;
+        EffMat[1] = GetMaterial(shiftCoord, 1, cand) * A_n;
+        area += A_n;
+
+        shiftCoord[n] = coord[n] + delta * 0.5;
+        shiftCoord[nP] = coord[nP] - deltaP_M * 0.25;
+        shiftCoord[nPP] = coord[nPP] + deltaPP * 0.25;
+        --loc_pos[nP];
+        A_n = NodeAreaI(ny, loc_pos, true);
+        EffMat[0] += GetMaterial(shiftCoord, 0, cand) * A_n;
+        EffMat[1] += GetMaterial(shiftCoord, 1, cand) * A_n;
+        area += A_n;
+
+        shiftCoord[n] = coord[n] + delta * 0.5;
+        shiftCoord[nP] = coord[nP] + deltaP * 0.25;
+        shiftCoord[nPP] = coord[nPP] - deltaPP_M * 0.25;
+        ++loc_pos[nP];
+        --loc_pos[nPP];
+        A_n = NodeAreaI(ny, loc_pos, true);
+        EffMat[0] += GetMaterial(shiftCoord, 0, cand) * A_n;
+        EffMat[1] += GetMaterial(shiftCoord, 1, cand) * A_n;
+        area += A_n;
+
+        shiftCoord[n] = coord[n] + delta * 0.5;
+        shiftCoord[nP] = coord[nP] - deltaP_M * 0.25;
+        shiftCoord[nPP] = coord[nPP] - deltaPP_M * 0.25;
+        --loc_pos[nP];
+        A_n = NodeAreaI(ny, loc_pos, true);
+        EffMat[0] += GetMaterial(shiftCoord, 0, cand) * A_n;
+        EffMat[1] += GetMaterial(shiftCoord, 1, cand) * A_n;
+        area += A_n;
+
+        EffMat[0] *= EPS0 / area;
+        EffMat[1] /= area;
+
+        loc_pos[0] = pos[0]; loc_pos[1] = pos[1]; loc_pos[2] = pos[2];
+        double length = 0;
+        shiftCoord[n] = coord[n] - delta_M * 0.25;
+        shiftCoord[nP] = coord[nP] + deltaP * 0.5;
+        shiftCoord[nPP] = coord[nPP] + deltaPP * 0.5;
+        --loc_pos[n];
+        double delta_ny = NodeWidthI(n, loc_pos, true);
+        EffMat[2] = delta_ny / GetMaterial(shiftCoord, 2, cand);
+        double sigma = GetMaterial(shiftCoord, 3, cand);
+        if (sigma) EffMat[3] = delta_ny / sigma;
+        else EffMat[3] = 0;
+        length = delta_ny;
+
+        shiftCoord[n] = coord[n] + delta * 0.25;
+        shiftCoord[nP] = coord[nP] + deltaP * 0.5;
+        shiftCoord[nPP] = coord[nPP] + deltaPP * 0.5;
+        ++loc_pos[n];
+        delta_ny = NodeWidthI(n, loc_pos, true);
+        EffMat[2] += delta_ny / GetMaterial(shiftCoord, 2, cand);
+        sigma = GetMaterial(shiftCoord, 3, cand);
+        if (sigma) EffMat[3] += delta_ny / sigma;
+        else EffMat[3] = 0;
+        length += delta_ny;
+
+        EffMat[2] = length * MUE0 / EffMat[2];
+        if (EffMat[3]) EffMat[3] = length / EffMat[3];
+    }
+
+    template <typename F>
+    F rnd(double v) const { return (F)v; }
+
+    int threads() const {
+        int t = nthreads > 0 ? nthreads : (int)std::thread::hardware_concurrency();
+        return std::max(1, std::min(t, 64));
+    }
+
+    template <typename Fn>
+    void parallel_x(uint32_t nx, Fn fn) const {
+        const int T = threads();
+        std::atomic<uint32_t> next(0);
+        std::vector<std::thread> th;
+        for (int t = 0; t < T; ++t)
+            th.emplace_back([&]() {
+                for (;;) {
+                    uint32_t x = next.fetch_add(1);
+                    if (x >= nx) break;
+                    fn(x);
+                }
+            });
+        for (auto& t : th) t.join();
+    }
+
+    template <typename F>
+    int build(uint32_t nr_ts);
+};
+
+namespace {
+
+// ---- coefficient helpers with F rounding at the reference's storage points ----
+template <typename F>
+struct Builder {
+    wop_operator* op;
+    std::vector<F> EC[4][3];   // C,G,L,R per direction
+    std::vector<F> co[4][3];   // vv, vi, ii, iv per direction
+
+    explicit Builder(wop_operator* o) : op(o) {}
+
+    F& VV(int n, size_t i) { return co[0][n][i]; }
+    F& VI(int n, size_t i) { return co[1][n][i]; }
+    F& II(int n, size_t i) { return co[2][n][i]; }
+    F& IV(int n, size_t i) { return co[3][n][i]; }
+
+    // Calc_ECOperatorPos, FDTD/operator.cpp:943-971
+    void ECOperatorPos(int n, size_t i) {
+        const double dT = op->dT;
+        double C = EC[0][n][i], G = EC[1][n][i];
+        if (C > 0) {
+            VV(n, i) = (F)((1.0 - dT * G / 2.0 / C) / (1.0 + dT * G / 2.0 / C));
+            VI(n, i) = (F)((dT / C) / (1.0 + dT * G / 2.0 / C));
+        } else {
+            VV(n, i) = 0;
+            VI(n, i) = 0;
+        }
+        double L = EC[2][n][i], R = EC[3][n][i];
+        if (L > 0) {
+            II(n, i) = (F)((1.0 - dT * R / 2.0 / L) / (1.0 + dT * R / 2.0 / L));
+            IV(n, i) = (F)((dT / L) / (1.0 + dT * R / 2.0 / L));
+        } else {
+            II(n, i) = 0;
+            IV(n, i) = 0;
+        }
+    }
+
+    // Calc_EC_Range + Calc_ECPos (FDTD/operator.cpp:1784-1809,1176-1224)
+    void CalcEC() {
+        const size_t nd = op->nd();
+        for (int k = 0; k < 4; ++k)
+            for (int n = 0; n < 3; ++n) EC[k][n].assign(nd, (F)0);
+        op->parallel_x(op->N[0], [&](uint32_t x) {
+            uint32_t pos[3];
+            pos[0] = x;
+            double EffMat[4];
+            for (pos[1] = 0; pos[1] < op->N[1]; ++pos[1]) {
+                std::vector<int> cand = op->RowCandidates((int)pos[0], (int)pos[1], P_MATERIAL);
+                for (pos[2] = 0; pos[2] < op->N[2]; ++pos[2]) {
+                    const size_t i = op->idx(pos[0], pos[1], pos[2]);
+                    for (int n = 0; n < 3; ++n) {
+                        op->AverageMatQuarterCell(n, pos, EffMat, cand);
+                        double delta = op->EdgeLength(n, pos);
+                        double area = op->EdgeArea(n, pos);
+                        double e0, e1, e2, e3;
+                        if (delta) { e0 = EffMat[0] * area / delta; e1 = EffMat[1] * area / delta; }
+                        else { e0 = 0; e1 = 0; }
+                        delta = op->EdgeLength(n, pos, true);
+                        area = op->EdgeArea(n, pos, true);
+                        if (delta) { e2 = EffMat[2] * area / delta; e3 = EffMat[3] * area / delta; }
+                        else { e2 = 0; e3 = 0; }
+                        EC[0][n][i] = (F)e0; EC[1][n][i] = (F)e1; EC[2][n][i] = (F)e2; EC[3][n][i] = (F)e3;
+                    }
+                }
+            }
+        });
+    }
+
+    // CalcTimestep_Var3 (FDTD/operator.cpp:1897-1971), AdrOp with
+    // SetReflection2Cell: a shift of -1 at index 0 or +1 at the last index
+    // stays on the same cell (tools/AdrOp.cpp:183-220)
+    double CalcTimestep() {
+        const uint32_t* N = op->N;
+        auto sh = [&](const uint32_t* p, int dir, int step) -> size_t {
+            int q[3] = {(int)p[0], (int)p[1], (int)p[2]};
+            q[dir] += step;
+            if (q[dir] < 0) q[dir] = -q[dir] - 1;
+            if (q[dir] > (int)N[dir] - 1) q[dir] = 2 * ((int)N[dir] - 1) - q[dir] + 1;
+            return op->idx(q[0], q[1], q[2]);
+        };
+        auto sh2 = [&](const uint32_t* p, int d1, int s1, int d2, int s2) -> size_t {
+            // GetShiftedPos after a Shift: both offsets applied, then reflected per axis
+            int q[3] = {(int)p[0], (int)p[1], (int)p[2]};
+            q[d1] += s1;
+            q[d2] += s2;
+            for (int d = 0; d < 3; ++d) {
+                if (q[d] < 0) q[d] = -q[d] - 1;
+                if (q[d] > (int)N[d] - 1) q[d] = 2 * ((int)N[d] - 1) - q[d] + 1;
+            }
+            return op->idx(q[0], q[1], q[2]);
+        };
+        double dT = 1e200;
+        for (int n = 0; n < 3; ++n) {
+            const int nP = (n + 1) % 3, nPP = (n + 2) % 3;
+            const std::vector<F>& Cn = EC[0][n];
+            const std::vector<F>& CnP = EC[0][nP];
+            const std::vector<F>& CnPP = EC[0][nPP];
+            const std::vector<F>& LnP = EC[2][nP];
+            const std::vector<F>& LnPP = EC[2][nPP];
+            // per-z-plane minima computed in parallel, combined in the reference's loop order
+            std::vector<double> zmin(N[2], 1e200);
+            op->parallel_x(N[2], [&](uint32_t z) {
+                uint32_t pos[3];
+                pos[2] = z;
+                double best = 1e200;
+                for (pos[1] = 0; pos[1] < N[1]; ++pos[1]) {
+                    for (pos[0] = 0; pos[0] < N[0]; ++pos[0]) {
+                        // EC_* are FDTD_FLOAT: each 1/(L*C) term and the pair sums are
+                        // evaluated in F, then accumulated in double (operator.cpp:1920-1951)
+                        const F one = (F)1;
+                        size_t ipos = op->idx(pos[0], pos[1], pos[2]);
+                        double wqp;
+                        wqp = (double)(one / (LnPP[ipos] * Cn[sh(pos, nP, 1)]) + one / (LnPP[ipos] * Cn[ipos]));
+                        wqp += (double)(one / (LnP[ipos] * Cn[sh(pos, nPP, 1)]) + one / (LnP[ipos] * Cn[ipos]));
+                        // Shift(nP,-1): GetShiftedPos(nP,1) from there is the original cell
+                        size_t ip1 = sh(pos, nP, -1);
+                        wqp += (double)(one / (LnPP[ip1] * Cn[sh2(pos, nP, -1, nP, 1)]) + one / (LnPP[ip1] * Cn[ip1]));
+                        // Shift(nPP,-1) keeps the nP shift of the previous Shift call
+                        size_t ip2 = sh2(pos, nP, -1, nPP, -1);
+                        wqp += (double)(one / (LnP[ip2] * Cn[sh2(pos, nP, -1, nPP, 0)]) + one / (LnP[ip2] * Cn[ip2]));
+
+                        double wt_4[4];
+                        wt_4[0] = (double)(one / (LnPP[ipos] * CnP[ipos]));
+                        wt_4[1] = (double)(one / (LnPP[sh(pos, nP, -1)] * CnP[ipos]));
+                        wt_4[2] = (double)(one / (LnP[ipos] * CnPP[ipos]));
+                        wt_4[3] = (double)(one / (LnP[sh(pos, nPP, -1)] * CnPP[ipos]));
+                        double wt1 = wt_4[0] + wt_4[1] + wt_4[2] + wt_4[3] - 2 * std::min(std::min(wt_4[0], wt_4[1]), std::min(wt_4[2], wt_4[3]));
+
+                        size_t in1 = sh(pos, n, 1);
+                        wt_4[0] = (double)(one / (LnPP[ipos] * CnP[in1]));
+                        wt_4[1] = (double)(one / (LnPP[sh(pos, nP, -1)] * CnP[in1]));
+                        wt_4[2] = (double)(one / (LnP[ipos] * CnPP[in1]));
+                        wt_4[3] = (double)(one / (LnP[sh(pos, nPP, -1)] * CnPP[in1]));
+                        double wt2 = wt_4[0] + wt_4[1] + wt_4[2] + wt_4[3] - 2 * std::min(std::min(wt_4[0], wt_4[1]), std::min(wt_4[2], wt_4[3]));
+
+                        double w_total = wqp + wt1 + wt2;
+                        double newT = 2 / sqrt(w_total);
+                        if ((newT < best) && (newT > 0.0)) best = newT;
+                    }
+                }
+                zmin[z] = best;
+            });
+            for (uint32_t z = 0; z < N[2]; ++z)
+                if (zmin[z] < dT && zmin[z] > 0.0) dT = zmin[z];
+        }
+        return dT;
+    }
+
+    void ApplyElectricBC() {
+        bool dirs[6];
+        for (int k = 0; k < 6; ++k) dirs[k] = op->BC[k] != -1;
+        const uint32_t* N = op->N;
+        uint32_t pos[3];
+        for (int n = 0; n < 3; ++n) {
+            int nP = (n + 1) % 3, nPP = (n + 2) % 3;
+            for (pos[nP] = 0; pos[nP] < N[nP]; ++pos[nP])
+                for (pos[nPP] = 0; pos[nPP] < N[nPP]; ++pos[nPP]) {
+                    if (dirs[2 * n]) {
+                        pos[n] = 0;
+                        size_t i = op->idx(pos[0], pos[1], pos[2]);
+                        VV(nP, i) = 0; VI(nP, i) = 0; VV(nPP, i) = 0; VI(nPP, i) = 0;
+                    }
+                    if (dirs[2 * n + 1]) {
+                        pos[n] = N[n] - 1;
+                        size_t i = op->idx(pos[0], pos[1], pos[2]);
+                        VV(n, i) = 0; VI(n, i) = 0;
+                        VV(nP, i) = 0; VI(nP, i) = 0; VV(nPP, i) = 0; VI(nPP, i) = 0;
+                    }
+                }
+        }
+    }
+
+    void ApplyMagneticBC() {
+        bool dirs[6];
+        for (int k = 0; k < 6; ++k) dirs[k] = op->BC[k] == 1;
+        const uint32_t* N = op->N;
+        uint32_t pos[3];
+        for (int n = 0; n < 3; ++n) {
+            int nP = (n + 1) % 3, nPP = (n + 2) % 3;
+            for (pos[nP] = 0; pos[nP] < N[nP]; ++pos[nP])
+                for (pos[nPP] = 0; pos[nPP] < N[nPP]; ++pos[nPP]) {
+                    if (dirs[2 * n]) {
+                        pos[n] = 0;
+                        size_t i = op->idx(pos[0], pos[1], pos[2]);
+                        II(n, i) = 0; IV(n, i) = 0; II(nP, i) = 0; IV(nP, i) = 0; II(nPP, i) = 0; IV(nPP, i) = 0;
+                    }
+                    if (dirs[2 * n + 1]) {
+                        pos[n] = N[n] - 2;
+                        size_t i = op->idx(pos[0], pos[1], pos[2]);
+                        II(nP, i) = 0; IV(nP, i) = 0; II(nPP, i) = 0; IV(nPP, i) = 0;
+                    }
+                    pos[n] = N[n] - 1;
+                    size_t i = op->idx(pos[0], pos[1], pos[2]);
+                    II(n, i) = 0; IV(n, i) = 0; II(nP, i) = 0; IV(nP, i) = 0; II(nPP, i) = 0; IV(nPP, i) = 0;
+                }
+        }
+    }
+
+    // CalcPEC_Range (FDTD/operator.cpp:1986-2020): edges whose Yee point lies in metal
+    void CalcPEC() {
+        op->parallel_x(op->N[0], [&](uint32_t x) {
+            uint32_t pos[3];
+            pos[0] = x;
+            double coord[3];
+            for (pos[1] = 0; pos[1] < op->N[1]; ++pos[1]) {
+                std::vector<int> cand = op->RowCandidates((int)pos[0], (int)pos[1], P_MATERIAL | P_METAL);
+                bool any_metal = false;
+                for (int c : cand) if (op->props[op->prims[c].prop].kind & P_METAL) any_metal = true;
+                if (!any_metal) continue;
+                for (pos[2] = 0; pos[2] < op->N[2]; ++pos[2])
+                    for (int n = 0; n < 3; ++n) {
+                        op->YeeCoords(n, pos, coord, false);
+                        int p = op->PropertyAt(coord, P_MATERIAL | P_METAL, &cand);
+                        if (p >= 0 && (op->props[p].kind & P_METAL)) {
+                            size_t i = op->idx(pos[0], pos[1], pos[2]);
+                            VV(n, i) = 0;
+                            VI(n, i) = 0;
+                        }
+                    }
+            }
+        });
+    }
+
+    // Calc_LumpedElements (FDTD/operator.cpp:1535-1712), parallel RC boxes only
+    int LumpedElements() {
+        for (const Primitive& pr : op->prims) {
+            const Property& P = op->props[pr.prop];
+            if (!(P.kind & P_LUMPED) || pr.shape != 0) continue;
+            double C = P.C;
+            if (C <= 0) C = NAN;
+            double R = P.R;
+            if (R < 0) R = NAN;
+            if (std::isnan(R) && std::isnan(C)) continue;
+            int ny = P.ny;
+            if (ny < 0 || ny > 2) continue;
+            int nyP = (ny + 1) % 3, nyPP = (ny + 2) % 3;
+            uint32_t s[3], e[3];
+            bool in;
+            for (int n = 0; n < 3; ++n) {
+                s[n] = op->SnapToMeshLine(n, pr.a[n], in, false);
+                e[n] = op->SnapToMeshLine(n, pr.b[n], in, false);
+            }
+            if (s[ny] == e[ny]) continue;
+            uint32_t pos[3];
+            double unitGC = 0;
+            for (pos[ny] = s[ny]; pos[ny] < e[ny]; ++pos[ny]) {
+                double plane = 0;
+                for (pos[nyP] = s[nyP]; pos[nyP] <= e[nyP]; ++pos[nyP])
+                    for (pos[nyPP] = s[nyPP]; pos[nyPP] <= e[nyPP]; ++pos[nyPP])
+                        plane += op->EdgeArea(ny, pos) / op->EdgeLength(ny, pos);
+                unitGC += 1 / plane;
+            }
+            unitGC = 1 / unitGC;
+            double kappa = 0, epsilon = 0;
+            if (R > 0) kappa = 1 / R / unitGC;
+            if (C > 0) {
+                epsilon = C / unitGC;
+                if (epsilon < EPS0) C = 0;
+            }
+            for (pos[ny] = s[ny]; pos[ny] < e[ny]; ++pos[ny])
+                for (pos[nyP] = s[nyP]; pos[nyP] <= e[nyP]; ++pos[nyP])
+                    for (pos[nyPP] = s[nyPP]; pos[nyPP] <= e[nyPP]; ++pos[nyPP]) {
+                        size_t i = op->idx(pos[0], pos[1], pos[2]);
+                        if (C > 0) EC[0][ny][i] = (F)(epsilon * op->EdgeArea(ny, pos) / op->EdgeLength(ny, pos));
+                        if (R > 0) EC[1][ny][i] = (F)(kappa * op->EdgeArea(ny, pos) / op->EdgeLength(ny, pos));
+                        if (R == 0) { VV(ny, i) = 0; VI(ny, i) = 0; }
+                        else ECOperatorPos(ny, i);
+                    }
+            if (P.caps) {
+                for (pos[nyP] = s[nyP]; pos[nyP] <= e[nyP]; ++pos[nyP])
+                    for (pos[nyPP] = s[nyPP]; pos[nyPP] <= e[nyPP]; ++pos[nyPP]) {
+                        for (int side = 0; side < 2; ++side) {
+                            pos[ny] = side ? e[ny] : s[ny];
+                            size_t i = op->idx(pos[0], pos[1], pos[2]);
+                            if (pos[nyP] < e[nyP]) { VV(nyP, i) = 0; VI(nyP, i) = 0; }
+                            if (pos[nyPP] < e[nyPP]) { VV(nyPP, i) = 0; VI(nyPP, i) = 0; }
+                        }
+                    }
+            }
+        }
+        return 0;
+    }
+
+    // Operator_Ext_Excitation::BuildExtension (operator_ext_excitation.cpp:101-222), box primitives
+    void Excitation() {
+        op->exc_v.clear();
+        op->exc_i.clear();
+        const uint32_t* N = op->N;
+        const double dT = op->dT;
+        bool any = false;
+        for (auto& p : op->prims) if (op->props[p.prop].kind & P_EXCITATION) any = true;
+        if (!any) return;
+        uint32_t pos[3];
+        double c[3];
+        for (pos[2] = 0; pos[2] < N[2]; ++pos[2])
+            for (pos[1] = 0; pos[1] < N[1]; ++pos[1])
+                for (pos[0] = 0; pos[0] < N[0]; ++pos[0]) {
+                    for (int n = 0; n < 3; ++n) {
+                        if (!op->YeeCoords(n, pos, c, false)) continue;
+                        int p = op->PropertyAt(c, P_EXCITATION);
+                        if (p < 0) continue;
+                        const Property& P = op->props[p];
+                        if (P.exc_val[n] != 0 && (P.exc_type == 0 || P.exc_type == 1)) {
+                            double amp = P.exc_val[n] * op->EdgeLength(n, pos);
+                            if (amp != 0) {
+                                wop_operator::Exc e;
+                                for (int k = 0; k < 3; ++k) e.pos[k] = pos[k];
+                                e.dir = n;
+                                e.amp = (double)(F)amp;
+                                e.delay = (uint32_t)(P.delay / dT);
+                                op->exc_v.push_back(e);
+                            }
+                            if (P.exc_type == 1) {
+                                size_t i = op->idx(pos[0], pos[1], pos[2]);
+                                VV(n, i) = 0; VI(n, i) = 0;
+                            }
+                        }
+                    }
+                    for (int n = 0; n < 3; ++n) {
+                        if (pos[0] >= N[0] - 1 || pos[1] >= N[1] - 1 || pos[2] >= N[2] - 1) continue;
+                        if (!op->YeeCoords(n, pos, c, true)) continue;
+                        int p = op->PropertyAt(c, P_EXCITATION);
+                        if (p < 0) continue;
+                        const Property& P = op->props[p];
+                        if (P.exc_val[n] != 0 && (P.exc_type == 2 || P.exc_type == 3)) {
+                            double amp = P.exc_val[n] * op->EdgeLength(n, pos, true);
+                            if (amp != 0) {
+                                wop_operator::Exc e;
+                                for (int k = 0; k < 3; ++k) e.pos[k] = pos[k];
+                                e.dir = n;
+                                e.amp = (double)(F)amp;
+                                e.delay = (uint32_t)(P.delay / dT);
+                                op->exc_i.push_back(e);
+                            }
+                            if (P.exc_type == 3) {
+                                size_t i = op->idx(pos[0], pos[1], pos[2]);
+                                II(n, i) = 0; IV(n, i) = 0;
+                            }
+                        }
+                    }
+                }
+    }
+
+    // default grading function of Operator_Ext_UPML (operator_ext_upml.cpp:31):
+    // -log(1e-6)*log(2.5)/(2*dl*Z*(pow(2.5,W/dl)-1)) * pow(2.5, D/dl)
+    static double Grading(double D, double dl, double W, double Z, double) {
+        return -log(1e-6) * log(2.5) / (2 * dl * Z * (pow(2.5, W / dl) - 1)) * pow(2.5, D / dl);
+    }
+
+    // Operator_Ext_UPML::CalcGradingKappa (operator_ext_upml.cpp:289-363)
+    void GradingKappa(int ny, const uint32_t* pos, double Zm, double* kv, double* ki) {
+        const int* BC = op->BC;
+        const uint32_t* S = op->pml_size;
+        for (int n = 0; n < 3; ++n) {
+            const uint32_t Nn = op->N[n];
+            if (pos[n] <= S[2 * n] && BC[2 * n] == 3) {
+                double width = (op->lines[n][S[2 * n]] - op->lines[n][0]) * op->gridDelta;
+                double depth = width - (op->lines[n][pos[n]] - op->lines[n][0]) * op->gridDelta;
+                if (n == ny) depth -= op->EdgeLength(n, pos) / 2;
+                double v1 = width / S[2 * n], v4 = (double)S[2 * n];
+                kv[n] = depth > 0 ? Grading(depth, v1, width, Zm, v4) : 0;
+                if (n == ny) depth += op->EdgeLength(n, pos) / 2;
+                if (n != ny) depth -= op->EdgeLength(n, pos) / 2;
+                if (depth < 0) depth = 0;
+                ki[n] = depth > 0 ? Grading(depth, v1, width, Zm, v4) : 0;
+            } else if (pos[n] >= Nn - 1 - S[2 * n + 1] && BC[2 * n + 1] == 3) {
+                double width = (op->lines[n][Nn - 1] - op->lines[n][Nn - S[2 * n + 1] - 1]) * op->gridDelta;
+                double depth = width - (op->lines[n][Nn - 1] - op->lines[n][pos[n]]) * op->gridDelta;
+                if (n == ny) depth += op->EdgeLength(n, pos) / 2;
+                // the reference passes the lower-side size here (operator_ext_upml.cpp:338)
+                double v1 = width / (S[2 * n]), v4 = (double)S[2 * n];
+                kv[n] = depth > 0 ? Grading(depth, v1, width, Zm, v4) : 0;
+                if (n == ny) depth -= op->EdgeLength(n, pos) / 2;
+                if (n != ny) depth += op->EdgeLength(n, pos) / 2;
+                if (depth > width) depth = 0;
+                ki[n] = depth > 0 ? Grading(depth, v1, width, Zm, v4) : 0;
+            } else {
+                kv[n] = 0;
+                ki[n] = 0;
+            }
+        }
+    }
+
+    // Operator_Ext_UPML::Create_UPML (slab ranges, operator_ext_upml.cpp:77-209) and
+    // BuildExtension (:365-466)
+    void Upml() {
+        op->upml.clear();
+        int* BC = op->BC;
+        uint32_t* S = op->pml_size;
+        const uint32_t* N = op->N;
+        for (int n = 0; n < 3; ++n)
+            if ((S[2 * n] * (BC[2 * n] == 3) + S[2 * n + 1] * (BC[2 * n + 1] == 3)) >= N[n]) {
+                BC[2 * n] = 0; S[2 * n] = 0; BC[2 * n + 1] = 0; S[2 * n + 1] = 0;
+            }
+        struct R { uint32_t s[3], e[3]; };
+        std::vector<R> ranges;
+        uint32_t start[3] = {0, 0, 0}, stop[3] = {N[0] - 1, N[1] - 1, N[2] - 1};
+        if (BC[0] == 3) { R r; start[0] = 0; stop[0] = S[0]; memcpy(r.s, start, 12); memcpy(r.e, stop, 12); ranges.push_back(r); }
+        if (BC[1] == 3) { R r; start[0] = N[0] - 1 - S[1]; stop[0] = N[0] - 1; memcpy(r.s, start, 12); memcpy(r.e, stop, 12); ranges.push_back(r); }
+        start[0] = (S[0] + 1) * (BC[0] == 3);
+        stop[0] = N[0] - 1 - (S[0] + 1) * (BC[1] == 3);
+        if (BC[2] == 3) { R r; start[1] = 0; stop[1] = S[2]; memcpy(r.s, start, 12); memcpy(r.e, stop, 12); ranges.push_back(r); }
+        if (BC[3] == 3) { R r; start[1] = N[1] - 1 - S[3]; stop[1] = N[1] - 1; memcpy(r.s, start, 12); memcpy(r.e, stop, 12); ranges.push_back(r); }
+        start[1] = (S[2] + 1) * (BC[2] == 3);
+        stop[1] = N[1] - 1 - (S[3] + 1) * (BC[3] == 3);
+        if (BC[4] == 3) { R r; start[2] = 0; stop[2] = S[4]; memcpy(r.s, start, 12); memcpy(r.e, stop, 12); ranges.push_back(r); }
+        if (BC[5] == 3) { R r; start[2] = N[2] - 1 - S[5]; stop[2] = N[2] - 1; memcpy(r.s, start, 12); memcpy(r.e, stop, 12); ranges.push_back(r); }
+
+        const double dT = op->dT;
+        for (const R& r : ranges) {
+            wop_operator::Upml u;
+            size_t cnt = 1;
+            for (int n = 0; n < 3; ++n) { u.start[n] = r.s[n]; u.n[n] = r.e[n] - r.s[n] + 1; cnt *= u.n[n]; }
+            for (int k = 0; k < 6; ++k) u.a[k].assign(3 * cnt, 0.0);
+            std::vector<F> ext[6];
+            for (int k = 0; k < 6; ++k) ext[k].assign(3 * cnt, (F)0);
+            op->parallel_x(u.n[0], [&](uint32_t lx) {
+                uint32_t loc[3], pos[3];
+                loc[0] = lx;
+                pos[0] = lx + u.start[0];
+                double eff[4], kv[3], ki[3];
+                for (loc[1] = 0; loc[1] < u.n[1]; ++loc[1]) {
+                    pos[1] = loc[1] + u.start[1];
+                    std::vector<int> cand = op->RowCandidates((int)pos[0], (int)pos[1], P_MATERIAL);
+                    for (loc[2] = 0; loc[2] < u.n[2]; ++loc[2]) {
+                        pos[2] = loc[2] + u.start[2];
+                        const size_t i = op->idx(pos[0], pos[1], pos[2]);
+                        for (int n = 0; n < 3; ++n) {
+                            const size_t li = (size_t)n * cnt + ((size_t)loc[0] * u.n[1] + loc[1]) * u.n[2] + loc[2];
+                            op->AverageMatQuarterCell(n, pos, eff, cand);
+                            GradingKappa(n, pos, Z0, kv, ki);
+                            const int nP = (n + 1) % 3, nPP = (n + 2) % 3;
+                            if (((kv[0] + kv[1] + kv[2]) != 0) && (eff[1] < 1e3)) {
+                                if ((VV(n, i) + VI(n, i)) != 0) {
+                                    VV(n, i) = (F)((2 * EPS0 - kv[nP] * dT) / (2 * EPS0 + kv[nP] * dT));
+                                    VI(n, i) = (F)((2 * EPS0 * dT) / (2 * EPS0 + kv[nP] * dT) * op->EdgeLength(n, pos) / op->EdgeArea(n, pos));
+                                    ext[0][li] = (F)((2 * EPS0 - kv[nPP] * dT) / (2 * EPS0 + kv[nPP] * dT));
+                                    ext[2][li] = (F)((2 * EPS0 + kv[n] * dT) / (2 * EPS0 + kv[nPP] * dT) / eff[0]);
+                                    ext[1][li] = (F)((2 * EPS0 - kv[n] * dT) / (2 * EPS0 + kv[nPP] * dT) / eff[0]);
+                                }
+                            } else {
+                                ext[0][li] = VV(n, i);
+                                VV(n, i) = 0;
+                                ext[1][li] = 0;
+                                ext[2][li] = 1;
+                            }
+                            if ((ki[0] + ki[1] + ki[2]) != 0) {
+                                if ((II(n, i) + IV(n, i)) != 0) {
+                                    II(n, i) = (F)((2 * EPS0 - ki[nP] * dT) / (2 * EPS0 + ki[nP] * dT));
+                                    IV(n, i) = (F)((2 * EPS0 * dT) / (2 * EPS0 + ki[nP] * dT) * op->EdgeLength(n, pos, true) / op->EdgeArea(n, pos, true));
+                                    ext[3][li] = (F)((2 * EPS0 - ki[nPP] * dT) / (2 * EPS0 + ki[nPP] * dT));
+                                    ext[5][li] = (F)((2 * EPS0 + ki[n] * dT) / (2 * EPS0 + ki[nPP] * dT) / eff[2]);
+                                    ext[4][li] = (F)((2 * EPS0 - ki[n] * dT) / (2 * EPS0 + ki[nPP] * dT) / eff[2]);
+                                }
+                            } else {
+                                ext[3][li] = II(n, i);
+                                II(n, i) = 0;
+                                ext[4][li] = 0;
+                                ext[5][li] = 1;
+                            }
+                        }
+                    }
+                }
+            });
+            for (int k = 0; k < 6; ++k)
+                for (size_t i = 0; i < 3 * cnt; ++i) u.a[k][i] = (double)ext[k][i];
+            op->upml.push_back(std::move(u));
+        }
+    }
+
+    // Operator_Ext_LorentzMaterial::BuildExtension (operator_ext_lorentzmaterial.cpp:117-438),
+    // order 1, electric Drude/Lorentz poles
+    void Lorentz() {
+        op->ade.clear();
+        bool any = false;
+        for (auto& p : op->props) if (p.lorentz) any = true;
+        if (!any) return;
+        const uint32_t* N = op->N;
+        const double dT = op->dT;
+        wop_operator::Ade a;
+        bool volt_on = false, volt_lor_on = false;
+        std::vector<double> v_int[3], v_ext[3], v_lor[3];
+        uint32_t pos[3];
+        double c[3];
+        for (pos[0] = 0; pos[0] < N[0]; ++pos[0])
+            for (pos[1] = 0; pos[1] < N[1]; ++pos[1]) {
+                std::vector<int> cand = op->RowCandidates((int)pos[0], (int)pos[1], P_MATERIAL | P_METAL);
+                for (pos[2] = 0; pos[2] < N[2]; ++pos[2]) {
+                    const size_t index = op->idx(pos[0], pos[1], pos[2]);
+                    bool on = false;
+                    double L_D[3], R_D[3], C_L[3];
+                    for (int n = 0; n < 3; ++n) {
+                        L_D[n] = 0; R_D[n] = 0; C_L[n] = 0;
+                        if (!op->YeeCoords(n, pos, c, false)) continue;
+                        if (VI(n, index) == 0) continue;
+                        int p = op->PropertyAt(c, P_MATERIAL | P_METAL, &cand);
+                        if (p < 0 || !op->props[p].lorentz) continue;
+                        const Property& P = op->props[p];
+                        double w_plasma = P.eps_plasma_f * 2 * PI_;
+                        if ((w_plasma > 0) && (EC[0][n][index] > 0)) {
+                            on = true;
+                            volt_on = true;
+                            L_D[n] = 1 / (w_plasma * w_plasma * (double)EC[0][n][index]);
+                        }
+                        double t_relax = P.eps_relax_t;
+                        if ((t_relax > 0) && volt_on) R_D[n] = L_D[n] / t_relax;
+                        double w_lor = P.eps_lor_f * 2 * PI_;
+                        if ((w_lor > 0) && (L_D[n] > 0)) {
+                            volt_lor_on = true;
+                            C_L[n] = 1 / (w_lor * w_lor * L_D[n]);
+                        }
+                    }
+                    if (!on) continue;
+                    for (int n = 0; n < 3; ++n) {
+                        a.pos[n].push_back(pos[n]);
+                        const double vi = (double)VI(n, index);
+                        if (L_D[n] > 0) {
+                            v_int[n].push_back((double)(F)((2.0 * L_D[n] - dT * R_D[n]) / (2.0 * L_D[n] + dT * R_D[n])));
+                            v_ext[n].push_back((double)(F)(dT / (L_D[n] + dT * R_D[n] / 2.0) * vi));
+                        } else if ((R_D[n] > 0) && (C_L[n] > 0)) {
+                            v_int[n].push_back((double)(F)((2.0 * dT - R_D[n] * C_L[n]) / (C_L[n] * R_D[n])));
+                            v_ext[n].push_back((double)(F)(2.0 / R_D[n] * vi));
+                        } else {
+                            v_int[n].push_back(1);
+                            v_ext[n].push_back(0);
+                        }
+                        if (C_L[n] > 0) v_lor[n].push_back((double)(F)(dT / C_L[n] / vi));
+                        else v_lor[n].push_back(0);
+                    }
+                }
+            }
+        const size_t cnt = a.pos[0].size();
+        if (!cnt) return;
+        for (int k = 0; k < 6; ++k) a.lists[k].assign(3 * cnt, 0.0);
+        for (int n = 0; n < 3; ++n)
+            for (size_t i = 0; i < cnt; ++i) {
+                a.lists[0][n * cnt + i] = v_int[n][i];
+                a.lists[1][n * cnt + i] = v_ext[n][i];
+                a.lists[2][n * cnt + i] = v_lor[n][i];
+                a.lists[3][n * cnt + i] = 1;
+                a.lists[4][n * cnt + i] = 0;
+            }
+        a.flags = (volt_on ? 1u : 0u) | (volt_lor_on ? 4u : 0u);
+        op->ade.push_back(std::move(a));
+    }
+
+    // Excitation::CalcGaussianPulsExcitation etc. (FDTD/excitation.cpp:151-277)
+    void Signal(uint32_t nr_ts) {
+        const double dT = op->dT;
+        std::vector<F> sv, si;
+        uint32_t L = 0;
+        double fmax = 0;
+        switch (op->exc_sig_type) {
+        case WOP_EXC_GAUSS: {
+            const double f0 = op->f0, fc = op->fc;
+            L = (uint32_t)ceil(2.0 * 9.0 / (2.0 * PI_ * fc) / dT);
+            if (L > nr_ts) L = nr_ts;
+            sv.resize(L); si.resize(L);
+            for (uint32_t n = 0; n < L; ++n) {
+                double t = n * dT;
+                sv[n] = (F)(cos(2.0 * PI_ * f0 * (t - 9.0 / (2.0 * PI_ * fc))) * exp(-1 * pow(2.0 * PI_ * fc * t / 3.0 - 3, 2)));
+                t += 0.5 * dT;
+                si[n] = (F)(cos(2.0 * PI_ * f0 * (t - 9.0 / (2.0 * PI_ * fc))) * exp(-1 * pow(2.0 * PI_ * fc * t / 3.0 - 3, 2)));
+            }
+            fmax = f0 + fc;
+            break;
+        }
+        case WOP_EXC_SINE: {
+            const double f0 = op->f0;
+            L = (uint32_t)round(2.0 / f0 / dT);
+            sv.assign(L, (F)0); si.assign(L, (F)0);
+            for (uint32_t n = 1; n < L; ++n) {
+                double t = n * dT;
+                sv[n] = (F)sin(2.0 * PI_ * f0 * t);
+                t += 0.5 * dT;
+                si[n] = (F)sin(2.0 * PI_ * f0 * t);
+            }
+            fmax = f0;
+            break;
+        }
+        case WOP_EXC_DIRAC:
+            L = 2; sv = {(F)0, (F)1}; si = {(F)0, (F)1}; fmax = -1;
+            break;
+        default:
+            L = 2; sv = {(F)1, (F)1}; si = {(F)1, (F)1}; fmax = -1;
+            break;
+        }
+        op->sig_v.assign(sv.begin(), sv.end());
+        op->sig_i.assign(si.begin(), si.end());
+        op->exc_length = L;
+        // CalcNyquistNum, tools/useful.cpp:30-36
+        if (fmax < 0) op->nyquist = 1;
+        else if (fmax == 0) op->nyquist = 0xffffffffu;
+        else op->nyquist = (uint32_t)floor(1 / fmax / 2 / dT);
+        // GetMaxExcitationTimestep (FDTD/excitation.cpp:136-149)
+        F maxAmp = 0;
+        uint32_t maxStep = 0;
+        for (uint32_t n = 0; n < L; ++n)
+            if (std::fabs(sv[n]) > maxAmp) { maxAmp = std::fabs(sv[n]); maxStep = n; }
+        op->max_step = maxStep;
+    }
+};
+
+}  // namespace
+
+template <typename F>
+int wop_operator::build(uint32_t nr_ts) {
+    Builder<F> b(this);
+    b.CalcEC();
+    // CalcECOperator timestep handling (FDTD/operator.cpp:981-1007)
+    if (forced_dT > 0) dT = forced_dT;
+    else dT = b.CalcTimestep();
+    dT *= ts_factor;
+    if (exc_sig_type == WOP_EXC_SINE && f0 > 0) {
+        double period = 1 / f0;
+        unsigned TS = (unsigned)ceil(period / dT);
+        dT = period / TS;
+    }
+    const size_t n = nd();
+    for (int k = 0; k < 4; ++k)
+        for (int d = 0; d < 3; ++d) b.co[k][d].assign(n, (F)0);
+    parallel_x(N[0], [&](uint32_t x) {
+        for (int d = 0; d < 3; ++d)
+            for (size_t i = (size_t)x * N[1] * N[2]; i < (size_t)(x + 1) * N[1] * N[2]; ++i) b.ECOperatorPos(d, i);
+    });
+    b.ApplyElectricBC();
+    b.CalcPEC();
+    b.LumpedElements();
+    b.ApplyMagneticBC();
+    // extensions in creation order: excitation, (TFSF), UPML, Lorentz (openems.cpp:1041-1092)
+    b.Excitation();
+    b.Upml();
+    b.Lorentz();
+    b.Signal(nr_ts);
+    for (int k = 0; k < 4; ++k) {
+        ec[k].resize(3 * n);
+        coef[k].resize(3 * n);
+        for (int d = 0; d < 3; ++d)
+            for (size_t i = 0; i < n; ++i) {
+                ec[k][d * n + i] = (double)b.EC[k][d][i];
+                coef[k][d * n + i] = (double)b.co[k][d][i];
+            }
+    }
+    built = true;
+    return 0;
+}
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int wop_create(wop_operator** out) {
+    if (!out) return fail(-1, "null argument");
+    *out = new wop_operator();
+    return 0;
+}
+
+int wop_destroy(wop_operator* op) {
+    delete op;
+    return 0;
+}
+
+int wop_last_error(char* buf, size_t len) {
+    if (!buf || !len) return -1;
+    snprintf(buf, len, "%s", g_err.c_str());
+    return 0;
+}
+
+int wop_set_mesh(wop_operator* op, int axis, const double* lines, uint32_t n) {
+    if (!op || !lines || axis < 0 || axis > 2) return fail(-1, "bad argument");
+    if (n < 2) return fail(-1, "need at least 2 mesh lines");
+    for (uint32_t i = 1; i < n; ++i)
+        if (!(lines[i] > lines[i - 1])) return fail(-1, "mesh lines must increase");
+    op->lines[axis].assign(lines, lines + n);
+    op->N[axis] = n;
+    op->built = false;
+    return 0;
+}
+
+int wop_set_grid_delta(wop_operator* op, double unit) {
+    if (!op || !(unit > 0)) return fail(-1, "bad argument");
+    op->gridDelta = unit;
+    return 0;
+}
+
+int wop_set_boundary(wop_operator* op, const int bc[6], const uint32_t pml_size[6]) {
+    if (!op || !bc) return fail(-1, "bad argument");
+    for (int k = 0; k < 6; ++k) {
+        if (bc[k] < -1 || bc[k] > 3) return fail(-1, "bad boundary code %d", bc[k]);
+        if (bc[k] == WOP_BC_MUR) return fail(-1, "Mur ABC is not supported by this engine");
+        op->BC[k] = bc[k];
+        op->pml_size[k] = (bc[k] == WOP_BC_PML && pml_size) ? pml_size[k] : 0;
+    }
+    return 0;
+}
+
+int wop_set_timestep_factor(wop_operator* op, double f) {
+    if (!op || f <= 0 || f > 1) return fail(-1, "invalid timestep factor");
+    op->ts_factor = f;
+    return 0;
+}
+
+int wop_set_timestep(wop_operator* op, double dT) {
+    if (!op || dT < 0) return fail(-1, "bad argument");
+    op->forced_dT = dT;
+    return 0;
+}
+
+int wop_set_threads(wop_operator* op, int n) {
+    if (!op) return fail(-1, "bad argument");
+    op->nthreads = n;
+    return 0;
+}
+
+static int add_prop(wop_operator* op, const Property& p, int* id) {
+    op->props.push_back(p);
+    if (id) *id = (int)op->props.size() - 1;
+    return 0;
+}
+
+int wop_add_material(wop_operator* op, double eps, double kappa, double mue, double sigma, int* id) {
+    if (!op) return fail(-1, "bad argument");
+    Property p;
+    p.kind = P_MATERIAL; p.eps = eps; p.kappa = kappa; p.mue = mue; p.sigma = sigma;
+    return add_prop(op, p, id);
+}
+
+int wop_add_metal(wop_operator* op, int* id) {
+    if (!op) return fail(-1, "bad argument");
+    Property p;
+    p.kind = P_METAL;
+    return add_prop(op, p, id);
+}
+
+int wop_add_lorentz_material(wop_operator* op, double eps, double kappa, double mue, double sigma, double fp, double tr, double fl, int* id) {
+    if (!op) return fail(-1, "bad argument");
+    Property p;
+    p.kind = P_MATERIAL; p.lorentz = true;
+    p.eps = eps; p.kappa = kappa; p.mue = mue; p.sigma = sigma;
+    p.eps_plasma_f = fp; p.eps_relax_t = tr; p.eps_lor_f = fl;
+    return add_prop(op, p, id);
+}
+
+int wop_add_lumped_element(wop_operator* op, int ny, double R, double C, int caps, int* id) {
+    if (!op || ny < 0 || ny > 2) return fail(-1, "bad argument");
+    Property p;
+    p.kind = P_LUMPED; p.ny = ny; p.R = R; p.C = C; p.caps = caps != 0;
+    return add_prop(op, p, id);
+}
+
+int wop_add_excitation(wop_operator* op, int type, const double value[3], double delay, int* id) {
+    if (!op || !value || type < 0 || type > 3) return fail(-1, "bad argument");
+    Property p;
+    p.kind = P_EXCITATION; p.exc_type = type; p.delay = delay;
+    for (int n = 0; n < 3; ++n) p.exc_val[n] = value[n];
+    return add_prop(op, p, id);
+}
+
+static int add_prim(wop_operator* op, Primitive pr) {
+    // descending priority, first added wins among equals
+    auto it = std::upper_bound(op->prims.begin(), op->prims.end(), pr,
+                               [](const Primitive& a, const Primitive& b) { return a.priority > b.priority; });
+    op->prims.insert(it, pr);
+    return 0;
+}
+
+int wop_add_box(wop_operator* op, int prop, const double start[3], const double stop[3], int priority) {
+    if (!op || !start || !stop || prop < 0 || prop >= (int)op->props.size()) return fail(-1, "bad argument");
+    Primitive pr;
+    pr.shape = 0; pr.prop = prop; pr.priority = priority;
+    for (int n = 0; n < 3; ++n) {
+        pr.a[n] = std::min(start[n], stop[n]);
+        pr.b[n] = std::max(start[n], stop[n]);
+        pr.bb0[n] = pr.a[n]; pr.bb1[n] = pr.b[n];
+    }
+    return add_prim(op, pr);
+}
+
+int wop_add_sphere(wop_operator* op, int prop, const double center[3], double radius, int priority) {
+    if (!op || !center || prop < 0 || prop >= (int)op->props.size() || !(radius > 0)) return fail(-1, "bad argument");
+    Primitive pr;
+    pr.shape = 1; pr.prop = prop; pr.priority = priority;
+    for (int n = 0; n < 3; ++n) {
+        pr.a[n] = center[n];
+        pr.bb0[n] = center[n] - radius; pr.bb1[n] = center[n] + radius;
+    }
+    pr.b[0] = radius;
+    return add_prim(op, pr);
+}
+
+int wop_set_excite_signal(wop_operator* op, int type, double f0, double fc) {
+    if (!op || type < 0 || type > 3) return fail(-1, "bad argument");
+    if (type == WOP_EXC_GAUSS && !(fc > 0)) return fail(-1, "Gaussian pulse needs fc > 0");
+    if (type == WOP_EXC_SINE && !(f0 > 0)) return fail(-1, "sine needs f0 > 0");
+    op->exc_sig_type = type; op->f0 = f0; op->fc = fc;
+    return 0;
+}
+
+int wop_build(wop_operator* op, int prec, uint32_t nr_ts) {
+    if (!op) return fail(-1, "bad argument");
+    for (int n = 0; n < 3; ++n)
+        if (op->N[n] < 2) return fail(-1, "mesh not set on axis %d", n);
+    op->prec = prec;
+    if (prec == 0) return op->build<float>(nr_ts);
+    return op->build<double>(nr_ts);
+}
+
+int wop_get_num_lines(wop_operator* op, uint32_t n[3]) {
+    if (!op || !n) return fail(-1, "bad argument");
+    for (int k = 0; k < 3; ++k) n[k] = op->N[k];
+    return 0;
+}
+
+int wop_get_timestep(wop_operator* op, double* dT) {
+    if (!op || !dT || !op->built) return fail(-1, "operator not built");
+    *dT = op->dT;
+    return 0;
+}
+
+int wop_get_excite_info(wop_operator* op, uint32_t* length, uint32_t* nyquist, uint32_t* max_step) {
+    if (!op || !op->built) return fail(-1, "operator not built");
+    if (length) *length = op->exc_length;
+    if (nyquist) *nyquist = op->nyquist;
+    if (max_step) *max_step = op->max_step;
+    return 0;
+}
+
+int wop_get_signal(wop_operator* op, double* v, double* i) {
+    if (!op || !op->built || !v || !i) return fail(-1, "operator not built");
+    memcpy(v, op->sig_v.data(), op->sig_v.size() * 8);
+    memcpy(i, op->sig_i.data(), op->sig_i.size() * 8);
+    return 0;
+}
+
+static void put(const std::vector<double>& src, int dtype, void* dst) {
+    if (!dst) return;
+    if (dtype == 1) memcpy(dst, src.data(), src.size() * 8);
+    else {
+        float* o = (float*)dst;
+        for (size_t i = 0; i < src.size(); ++i) o[i] = (float)src[i];
+    }
+}
+
+int wop_get_coeffs(wop_operator* op, int dtype, void* vv, void* vi, void* ii, void* iv) {
+    if (!op || !op->built) return fail(-1, "operator not built");
+    put(op->coef[0], dtype, vv);
+    put(op->coef[1], dtype, vi);
+    put(op->coef[2], dtype, ii);
+    put(op->coef[3], dtype, iv);
+    return 0;
+}
+
+int wop_get_ec(wop_operator* op, double* C, double* G, double* L, double* R) {
+    if (!op || !op->built) return fail(-1, "operator not built");
+    put(op->ec[0], 1, C);
+    put(op->ec[1], 1, G);
+    put(op->ec[2], 1, L);
+    put(op->ec[3], 1, R);
+    return 0;
+}
+
+namespace {
+struct TupleHash {
+    size_t operator()(const std::array<uint64_t, 12>& t) const {
+        uint64_t h = 1469598103934665603ull;
+        for (uint64_t v : t) { h ^= v; h *= 1099511628211ull; h ^= h >> 29; }
+        return (size_t)h;
+    }
+};
+}  // namespace
+
+}  // extern "C"
+
+extern "C" int wop_compress(wop_operator* op, uint32_t* n_entries, uint32_t* index, double* table, uint32_t max_entries) {
+    if (!op || !op->built || !n_entries) return fail(-1, "operator not built");
+    const size_t n = op->nd();
+    std::unordered_map<std::array<uint64_t, 12>, uint32_t, TupleHash> map;
+    std::vector<std::array<uint64_t, 12>> uniq;
+    for (size_t i = 0; i < n; ++i) {
+        std::array<uint64_t, 12> t;
+        for (int k = 0; k < 4; ++k)
+            for (int d = 0; d < 3; ++d) {
+                double v = op->coef[k][d * n + i];
+                uint64_t bits;
+                memcpy(&bits, &v, 8);
+                t[k * 3 + d] = bits;
+            }
+        auto it = map.find(t);
+        uint32_t id;
+        if (it == map.end()) {
+            id = (uint32_t)uniq.size();
+            map.emplace(t, id);
+            uniq.push_back(t);
+            if (max_entries && uniq.size() > max_entries) {
+                *n_entries = (uint32_t)uniq.size();
+                return fail(-2, "more than %u unique coefficient tuples", max_entries);
+            }
+        } else id = it->second;
+        if (index) index[i] = id;
+    }
+    *n_entries = (uint32_t)uniq.size();
+    if (table)
+        for (size_t r = 0; r < uniq.size(); ++r)
+            for (int k = 0; k < 12; ++k) memcpy(&table[r * 12 + k], &uniq[r][k], 8);
+    return 0;
+}
+
+extern "C" {
+
+int wop_get_excitation_counts(wop_operator* op, uint32_t* vc, uint32_t* cc) {
+    if (!op || !op->built) return fail(-1, "operator not built");
+    if (vc) *vc = (uint32_t)op->exc_v.size();
+    if (cc) *cc = (uint32_t)op->exc_i.size();
+    return 0;
+}
+
+int wop_get_excitation(wop_operator* op, uint32_t* vpos, uint32_t* vdir, double* vamp, uint32_t* vdel,
+                       uint32_t* cpos, uint32_t* cdir, double* camp, uint32_t* cdel) {
+    if (!op || !op->built) return fail(-1, "operator not built");
+    auto fill = [](const std::vector<wop_operator::Exc>& l, uint32_t* pos, uint32_t* dir, double* amp, uint32_t* del) {
+        const size_t c = l.size();
+        for (size_t i = 0; i < c; ++i) {
+            if (pos) for (int n = 0; n < 3; ++n) pos[n * c + i] = l[i].pos[n];
+            if (dir) dir[i] = l[i].dir;
+            if (amp) amp[i] = l[i].amp;
+            if (del) del[i] = l[i].delay;
+        }
+    };
+    fill(op->exc_v, vpos, vdir, vamp, vdel);
+    fill(op->exc_i, cpos, cdir, camp, cdel);
+    return 0;
+}
+
+int wop_get_num_upml(wop_operator* op, uint32_t* n) {
+    if (!op || !op->built || !n) return fail(-1, "operator not built");
+    *n = (uint32_t)op->upml.size();
+    return 0;
+}
+
+int wop_get_upml_range(wop_operator* op, uint32_t k, uint32_t start[3], uint32_t num_lines[3]) {
+    if (!op || !op->built || k >= op->upml.size()) return fail(-1, "bad UPML index");
+    for (int n = 0; n < 3; ++n) { start[n] = op->upml[k].start[n]; num_lines[n] = op->upml[k].n[n]; }
+    return 0;
+}
+
+int wop_get_upml(wop_operator* op, uint32_t k, int dtype, void* vv, void* vvfo, void* vvfn, void* ii, void* iifo, void* iifn) {
+    if (!op || !op->built || k >= op->upml.size()) return fail(-1, "bad UPML index");
+    void* dst[6] = {vv, vvfo, vvfn, ii, iifo, iifn};
+    for (int j = 0; j < 6; ++j) put(op->upml[k].a[j], dtype, dst[j]);
+    return 0;
+}
+
+int wop_get_num_ade(wop_operator* op, uint32_t* orders) {
+    if (!op || !op->built || !orders) return fail(-1, "operator not built");
+    *orders = (uint32_t)op->ade.size();
+    return 0;
+}
+
+int wop_get_ade_info(wop_operator* op, uint32_t o, uint32_t* count, uint32_t* flags) {
+    if (!op || !op->built || o >= op->ade.size()) return fail(-1, "bad ADE order");
+    if (count) *count = (uint32_t)op->ade[o].pos[0].size();
+    if (flags) *flags = op->ade[o].flags;
+    return 0;
+}
+
+int wop_get_ade(wop_operator* op, uint32_t o, uint32_t* pos, int dtype, void* v_int, void* v_ext, void* v_lor,
+                void* i_int, void* i_ext, void* i_lor) {
+    if (!op || !op->built || o >= op->ade.size()) return fail(-1, "bad ADE order");
+    const auto& a = op->ade[o];
+    const size_t c = a.pos[0].size();
+    if (pos) for (int n = 0; n < 3; ++n) memcpy(pos + n * c, a.pos[n].data(), c * 4);
+    void* dst[6] = {v_int, v_ext, v_lor, i_int, i_ext, i_lor};
+    for (int j = 0; j < 6; ++j) put(a.lists[j], dtype, dst[j]);
+    return 0;
+}
+
+int wop_snap_to_mesh_line(wop_operator* op, int axis, double coord, int dual, uint32_t* idx, int* inside) {
+    if (!op || axis < 0 || axis > 2 || !idx || op->N[axis] < 2) return fail(-1, "bad argument");
+    bool in = false;
+    *idx = op->SnapToMeshLine(axis, coord, in, dual != 0);
+    if (inside) *inside = in ? 1 : 0;
+    return 0;
+}
+
+}  // extern "C"

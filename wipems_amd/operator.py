@@ -1,0 +1,313 @@
+"""Python mirror of the openEMS Operator interface over libwipems_operator.so.
+
+``Operator`` collects the mesh, boundary conditions, materials and sources of
+a simulation (what openEMS reads from CSXCAD + the FDTD XML), runs the native
+coefficient build and returns an ``OperatorData`` with everything an engine
+needs: the coefficient arrays (Operator::GetVV/VI/II/IV), the excitation
+lists (Operator_Ext_Excitation), UPML slabs (Operator_Ext_UPML) and ADE
+orders (Operator_Ext_LorentzMaterial).
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, byref, c_double, c_int, c_uint32, c_void_p
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from ._native import FdtdError, check, operator_lib
+
+BC_PEC, BC_PMC, BC_MUR, BC_PML = 0, 1, 2, 3
+EXC_GAUSS, EXC_SINE, EXC_DIRAC, EXC_STEP = 0, 1, 2, 3
+
+
+def _dp(a):
+    return a.ctypes.data_as(POINTER(c_double))
+
+
+def _up(a):
+    return a.ctypes.data_as(POINTER(c_uint32))
+
+
+def _vp(a):
+    return a.ctypes.data_as(c_void_p)
+
+
+@dataclass
+class UpmlSlab:
+    start: np.ndarray              # uint32[3]
+    num_lines: np.ndarray          # uint32[3]
+    arrays: List[np.ndarray]       # vv, vvfo, vvfn, ii, iifo, iifn, each [3][nx][ny][nz]
+
+
+@dataclass
+class AdeOrder:
+    pos: np.ndarray                # uint32 [3][count]
+    lists: List[np.ndarray]        # v_int, v_ext, v_lor, i_int, i_ext, i_lor, each [3][count]
+    flags: int
+
+
+@dataclass
+class Excitation:
+    length: int
+    period: int
+    signal_volt: np.ndarray
+    signal_curr: np.ndarray
+    volt_pos: np.ndarray           # uint32 [3][n]
+    volt_dir: np.ndarray
+    volt_amp: np.ndarray           # float64 (values are float-rounded for a float build)
+    volt_delay: np.ndarray
+    curr_pos: np.ndarray
+    curr_dir: np.ndarray
+    curr_amp: np.ndarray
+    curr_delay: np.ndarray
+
+
+@dataclass
+class ProbeDef:
+    kind: int                      # 0 voltage, 1 current
+    start: tuple
+    stop: tuple
+    weight: float = 1.0
+    norm_dir: int = -1
+    inside_mask: int = 0x3F
+    td_interval: int = 0
+    fd_interval: int = 0
+    freqs: tuple = ()
+    name: str = ""
+
+
+@dataclass
+class OperatorData:
+    num_lines: tuple
+    dT: float
+    prec: int                      # 0 float build, 1 double build
+    vv: np.ndarray                 # [3][Nx][Ny][Nz]
+    vi: np.ndarray
+    ii: np.ndarray
+    iv: np.ndarray
+    excitation: Optional[Excitation]
+    upml: List[UpmlSlab] = field(default_factory=list)
+    ade: List[AdeOrder] = field(default_factory=list)
+    nyquist: int = 0
+    max_excite_step: int = 0
+    probes: List[ProbeDef] = field(default_factory=list)
+    tfsf: Optional[dict] = None
+    index: Optional[np.ndarray] = None      # compressed form, filled by compress()
+    table: Optional[np.ndarray] = None
+
+    @property
+    def cells(self):
+        n = self.num_lines
+        return int(n[0]) * int(n[1]) * int(n[2])
+
+    def compress(self, max_entries=0):
+        """Per-cell coefficient dedup (Operator_SSE_Compressed::CompressOperator)."""
+        return compress_coeffs(self, max_entries)
+
+
+def compress_coeffs(data: OperatorData, max_entries=0):
+    """Deduplicate the per-cell 12-tuples {vv,vi,ii,iv}x3 bitwise, first-seen order.
+
+    Returns (index uint32 [Nx][Ny][Nz], table float64 [n][12]) or None when the
+    number of unique tuples exceeds max_entries (> 0)."""
+    nd = data.cells
+    dt = np.float64 if data.prec == 1 else np.float32
+    stack = np.empty((nd, 12), dtype=dt)
+    for k, arr in enumerate((data.vv, data.vi, data.ii, data.iv)):
+        stack[:, 3 * k:3 * k + 3] = arr.reshape(3, nd).T
+    view = stack.view(np.dtype((np.void, stack.dtype.itemsize * 12)))
+    uniq, first, inv = np.unique(view, return_index=True, return_inverse=True)
+    if max_entries and len(uniq) > max_entries:
+        return None
+    # renumber by first occurrence (the reference's insertion order)
+    order = np.argsort(first, kind="stable")
+    rank = np.empty_like(order)
+    rank[order] = np.arange(len(order))
+    index = rank[inv.reshape(-1)].astype(np.uint32).reshape(tuple(data.num_lines))
+    table = stack[first[order]].astype(np.float64)
+    data.index, data.table = index, table
+    return index, table
+
+
+class Operator:
+    """Scenario description + native EC operator build."""
+
+    def __init__(self):
+        self._lib = operator_lib()
+        h = c_void_p()
+        self._chk(self._lib.wop_create(byref(h)))
+        self._h = h
+        self.num_lines = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                self._lib.wop_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        return check(self._lib, rc, "wop_last_error")
+
+    # ---- setup ----
+    def set_mesh(self, lines_xyz, unit):
+        n = []
+        for ax, lines in enumerate(lines_xyz):
+            a = np.ascontiguousarray(lines, dtype=np.float64)
+            self._chk(self._lib.wop_set_mesh(self._h, ax, _dp(a), len(a)))
+            n.append(len(a))
+        self._chk(self._lib.wop_set_grid_delta(self._h, float(unit)))
+        self.num_lines = tuple(n)
+        self.lines = [np.asarray(l, dtype=np.float64) for l in lines_xyz]
+        self.unit = unit
+
+    def set_boundary(self, bc, pml_size=(8,) * 6):
+        b = (c_int * 6)(*bc)
+        s = (c_uint32 * 6)(*pml_size)
+        self._chk(self._lib.wop_set_boundary(self._h, b, s))
+
+    def set_timestep_factor(self, f):
+        self._chk(self._lib.wop_set_timestep_factor(self._h, float(f)))
+
+    def set_timestep(self, dT):
+        self._chk(self._lib.wop_set_timestep(self._h, float(dT)))
+
+    def set_threads(self, n):
+        self._chk(self._lib.wop_set_threads(self._h, int(n)))
+
+    def add_material(self, eps_r=1.0, kappa=0.0, mue_r=1.0, sigma=0.0):
+        i = c_int()
+        self._chk(self._lib.wop_add_material(self._h, eps_r, kappa, mue_r, sigma, byref(i)))
+        return i.value
+
+    def add_metal(self):
+        i = c_int()
+        self._chk(self._lib.wop_add_metal(self._h, byref(i)))
+        return i.value
+
+    def add_lorentz_material(self, eps_r=1.0, kappa=0.0, mue_r=1.0, sigma=0.0, plasma_freq=0.0, relax_time=0.0, lor_pole_freq=0.0):
+        i = c_int()
+        self._chk(self._lib.wop_add_lorentz_material(self._h, eps_r, kappa, mue_r, sigma, plasma_freq, relax_time, lor_pole_freq, byref(i)))
+        return i.value
+
+    def add_lumped_element(self, ny, R=-1.0, C=-1.0, caps=True):
+        i = c_int()
+        self._chk(self._lib.wop_add_lumped_element(self._h, int(ny), float(R), float(C), int(bool(caps)), byref(i)))
+        return i.value
+
+    def add_excitation(self, exc_type, value, delay=0.0):
+        i = c_int()
+        v = (c_double * 3)(*value)
+        self._chk(self._lib.wop_add_excitation(self._h, int(exc_type), v, float(delay), byref(i)))
+        return i.value
+
+    def add_box(self, prop, start, stop, priority=0):
+        a = (c_double * 3)(*start)
+        b = (c_double * 3)(*stop)
+        self._chk(self._lib.wop_add_box(self._h, int(prop), a, b, int(priority)))
+
+    def add_sphere(self, prop, center, radius, priority=0):
+        c = (c_double * 3)(*center)
+        self._chk(self._lib.wop_add_sphere(self._h, int(prop), c, float(radius), int(priority)))
+
+    def set_gauss_excite(self, f0, fc):
+        self._chk(self._lib.wop_set_excite_signal(self._h, EXC_GAUSS, float(f0), float(fc)))
+
+    def set_excite_signal(self, kind, f0=0.0, fc=0.0):
+        self._chk(self._lib.wop_set_excite_signal(self._h, int(kind), float(f0), float(fc)))
+
+    def snap(self, axis, coord, dual=False):
+        """Operator::SnapToMeshLine -> (index, inside)."""
+        idx = c_uint32()
+        ins = c_int()
+        self._chk(self._lib.wop_snap_to_mesh_line(self._h, int(axis), float(coord), int(bool(dual)), byref(idx), byref(ins)))
+        return idx.value, bool(ins.value)
+
+    # ---- build ----
+    def build(self, prec=0, nr_ts=1_000_000_000) -> OperatorData:
+        lib, h = self._lib, self._h
+        self._chk(lib.wop_build(h, int(prec), int(nr_ts)))
+        n = (c_uint32 * 3)()
+        self._chk(lib.wop_get_num_lines(h, n))
+        shape = (3, n[0], n[1], n[2])
+        dT = c_double()
+        self._chk(lib.wop_get_timestep(h, byref(dT)))
+        dt = np.float64 if prec == 1 else np.float32
+        arrs = [np.empty(shape, dtype=dt) for _ in range(4)]
+        self._chk(lib.wop_get_coeffs(h, 1 if prec == 1 else 0, *[_vp(a) for a in arrs]))
+        length, nyq, mstep = c_uint32(), c_uint32(), c_uint32()
+        self._chk(lib.wop_get_excite_info(h, byref(length), byref(nyq), byref(mstep)))
+        sv = np.empty(length.value, dtype=np.float64)
+        si = np.empty(length.value, dtype=np.float64)
+        if length.value:
+            self._chk(lib.wop_get_signal(h, _dp(sv), _dp(si)))
+        nv, nc = c_uint32(), c_uint32()
+        self._chk(lib.wop_get_excitation_counts(h, byref(nv), byref(nc)))
+        nv, nc = nv.value, nc.value
+        vpos = np.zeros((3, max(nv, 1)), np.uint32); vdir = np.zeros(max(nv, 1), np.uint32)
+        vamp = np.zeros(max(nv, 1)); vdel = np.zeros(max(nv, 1), np.uint32)
+        cpos = np.zeros((3, max(nc, 1)), np.uint32); cdir = np.zeros(max(nc, 1), np.uint32)
+        camp = np.zeros(max(nc, 1)); cdel = np.zeros(max(nc, 1), np.uint32)
+        self._chk(lib.wop_get_excitation(h, _up(vpos), _up(vdir), _dp(vamp), _up(vdel), _up(cpos), _up(cdir), _dp(camp), _up(cdel)))
+        exc = Excitation(length.value, 0, sv, si, np.ascontiguousarray(vpos[:, :nv]), vdir[:nv].copy(), vamp[:nv].copy(), vdel[:nv].copy(),
+                         np.ascontiguousarray(cpos[:, :nc]), cdir[:nc].copy(), camp[:nc].copy(), cdel[:nc].copy())
+        nu = c_uint32()
+        self._chk(lib.wop_get_num_upml(h, byref(nu)))
+        slabs = []
+        for k in range(nu.value):
+            st = (c_uint32 * 3)(); nl = (c_uint32 * 3)()
+            self._chk(lib.wop_get_upml_range(h, k, st, nl))
+            sh = (3, nl[0], nl[1], nl[2])
+            ua = [np.empty(sh, dtype=dt) for _ in range(6)]
+            self._chk(lib.wop_get_upml(h, k, 1 if prec == 1 else 0, *[_vp(a) for a in ua]))
+            slabs.append(UpmlSlab(np.array(list(st), np.uint32), np.array(list(nl), np.uint32), ua))
+        na = c_uint32()
+        self._chk(lib.wop_get_num_ade(h, byref(na)))
+        ades = []
+        for o in range(na.value):
+            cnt, fl = c_uint32(), c_uint32()
+            self._chk(lib.wop_get_ade_info(h, o, byref(cnt), byref(fl)))
+            pos = np.empty((3, cnt.value), np.uint32)
+            lists = [np.empty((3, cnt.value), dtype=dt) for _ in range(6)]
+            self._chk(lib.wop_get_ade(h, o, _up(pos), 1 if prec == 1 else 0, *[_vp(a) for a in lists]))
+            ades.append(AdeOrder(pos, lists, fl.value))
+        return OperatorData(tuple(int(x) for x in n), dT.value, prec, *arrs, exc, slabs, ades, nyq.value, mstep.value)
+
+    def get_ec(self):
+        n = self.num_lines
+        shape = (3,) + tuple(n)
+        out = [np.empty(shape) for _ in range(4)]
+        self._chk(self._lib.wop_get_ec(self._h, *[_dp(a) for a in out]))
+        return out
+
+    # ---- probe helpers (ProcessIntegral::DefineStartStopCoord snapping) ----
+    def voltage_probe(self, start, stop, weight=1.0, **kw):
+        """Voltage probe on a line between two coordinates (snapped to the main mesh)."""
+        s = tuple(self.snap(n, start[n], False)[0] for n in range(3))
+        e = tuple(self.snap(n, stop[n], False)[0] for n in range(3))
+        return ProbeDef(0, s, e, weight=weight, **kw)
+
+    def current_probe(self, start, stop, norm_dir, weight=1.0, **kw):
+        """Current probe on a (possibly degenerate) box; snapped to the dual mesh
+        and widened to a loop as in ProcessCurrent::DefineStartStopCoord
+        (Common/processcurrent.cpp:39-90)."""
+        s, e, ins_s, ins_e = [], [], [], []
+        for n in range(3):
+            a, ia = self.snap(n, min(start[n], stop[n]), True)
+            b, ib = self.snap(n, max(start[n], stop[n]), True)
+            s.append(a); e.append(b); ins_s.append(ia); ins_e.append(ib)
+        for n in range(3):
+            if n == norm_dir:
+                continue
+            if start[n] == stop[n]:
+                dual = lambda i: 0.5 * (self.lines[n][i] + self.lines[n][i + 1]) if i < len(self.lines[n]) - 1 else \
+                    self.lines[n][i] + 0.5 * (self.lines[n][i] - self.lines[n][i - 1])
+                if dual(s[n]) > start[n] and s[n] > 0:
+                    s[n] -= 1
+                if dual(s[n]) < start[n] and e[n] < len(self.lines[n]) - 1:
+                    e[n] += 1
+        mask = sum((1 << n) for n in range(3) if ins_s[n]) | sum((1 << (3 + n)) for n in range(3) if ins_e[n])
+        return ProbeDef(1, tuple(s), tuple(e), weight=weight, norm_dir=norm_dir, inside_mask=mask, **kw)

@@ -1,0 +1,159 @@
+"""Scenario builders for the benchmark configurations and the parity tests.
+
+Each builder describes a simulation through ``Operator`` (mesh, boundary
+conditions, materials, sources, probes) and returns the built
+``OperatorData``.  Geometry is drawn from ``numpy.random.default_rng`` with
+the stated seeds, so every run sees the same inputs.
+
+Mesh sizes are counted in mesh lines, as openEMS counts cells
+(FDTD/operator.cpp:502-507): "64x64x64 cells" is 65 lines per axis.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from .operator import BC_PEC, BC_PML, Operator, OperatorData, ProbeDef
+
+SEED_BASE = 20261203
+
+
+@dataclass
+class Scenario:
+    name: str
+    data: OperatorData
+    steps: int
+    op: Operator
+    port: dict = None          # lumped port description for S11 post-processing
+    meta: dict = None
+
+
+def _finish(op, data, probes):
+    data.probes = list(probes)
+    return data
+
+
+def cavity_point_source(n_cells=64, steps=2000, prec=0, delta_mm=1.0, f0=0.0, fc=10e9, bc=BC_PEC, factor=0.99):
+    """Plain vacuum cube, PEC walls, one soft Ez edge at the centre driven by a
+    Gaussian pulse; voltage probe on that edge, current probe around it.
+    With the defaults this is the 64x64x64-cell / 1 mm / 2000-step case."""
+    op = Operator()
+    lines = np.arange(n_cells + 1, dtype=np.float64) * delta_mm
+    op.set_mesh([lines, lines, lines], 1e-3)
+    op.set_boundary([bc] * 6, [8] * 6)
+    op.set_timestep_factor(factor)
+    op.set_gauss_excite(f0, fc)
+    c = lines[n_cells // 2]
+    d = lines[n_cells // 2 + 1]
+    e = op.add_excitation(0, (0.0, 0.0, 1.0))
+    op.add_box(e, (c, c, c), (c, c, d), priority=10)
+    data = op.build(prec, nr_ts=steps)
+    interval = max(1, data.nyquist // 4)
+    probes = [
+        op.voltage_probe((c, c, c), (c, c, d), 1.0, td_interval=interval, name="ut"),
+        op.current_probe((c, c, 0.5 * (c + d)), (c, c, 0.5 * (c + d)), 2, 1.0, td_interval=interval, name="it"),
+    ]
+    _finish(op, data, probes)
+    return Scenario("cavity_%d" % n_cells, data, steps, op, meta={"interval": interval})
+
+
+def graded_lines(rng, n_cells, lo=0.5, hi=2.0):
+    gaps = rng.uniform(lo, hi, n_cells)
+    return np.concatenate([[0.0], np.cumsum(gaps)])
+
+
+def add_lumped_port(op, lines, port_idx, z_span, R=50.0, excite=1.0, priority=100):
+    """openEMS LumpedPort (python/openEMS/ports.py:143-194): a z-directed
+    parallel-R lumped element with metal caps, a soft E excitation with value
+    -direction*excite, a voltage probe along the port (weight -1) and a
+    current probe at the mid plane (weight = direction)."""
+    xi, yi, zi = port_idx
+    x, y = lines[0][xi], lines[1][yi]
+    z0, z1 = lines[2][zi], lines[2][zi + z_span]
+    start, stop = (x, y, z0), (x, y, z1)
+    ny = 2
+    direction = 1.0 if stop[ny] > start[ny] else -1.0
+    le = op.add_lumped_element(ny, R=R, caps=True)
+    op.add_box(le, start, stop, priority=priority)
+    exc = op.add_excitation(0, (0.0, 0.0, -direction * excite))
+    op.add_box(exc, start, stop, priority=priority)
+    zmid = 0.5 * (z0 + z1)
+    return {"start": start, "stop": stop, "R": R, "direction": direction, "zmid": zmid}
+
+
+def port_probes(op, port, interval, fd_interval=0, freqs=()):
+    s, e = port["start"], port["stop"]
+    u_start = (0.5 * (s[0] + e[0]), 0.5 * (s[1] + e[1]), s[2])
+    u_stop = (0.5 * (s[0] + e[0]), 0.5 * (s[1] + e[1]), e[2])
+    i_pt = (s[0], s[1], port["zmid"])
+    return [
+        op.voltage_probe(u_start, u_stop, -1.0, td_interval=interval, fd_interval=fd_interval, freqs=tuple(freqs), name="port_ut"),
+        op.current_probe(i_pt, i_pt, 2, port["direction"], td_interval=interval, fd_interval=fd_interval, freqs=tuple(freqs), name="port_it"),
+    ]
+
+
+def random_boxes(rng, op, lines, count, pml, eps_range=(1.0, 12.0), kappa_log=(-3.0, 0.0), min_frac=0.04, max_frac=0.25,
+                 avoid=None, priority=1):
+    """Random axis-aligned dielectric boxes inside the non-PML interior."""
+    boxes = []
+    for k in range(count):
+        start, stop = [], []
+        for n in range(3):
+            lo, hi = lines[n][pml], lines[n][-1 - pml]
+            span = hi - lo
+            size = rng.uniform(min_frac, max_frac) * span
+            a = rng.uniform(lo, hi - size)
+            start.append(a)
+            stop.append(a + size)
+        eps = rng.uniform(*eps_range)
+        kappa = 10 ** rng.uniform(*kappa_log) if kappa_log is not None else 0.0
+        m = op.add_material(eps_r=eps, kappa=kappa)
+        op.add_box(m, tuple(start), tuple(stop), priority=priority)
+        boxes.append({"start": start, "stop": stop, "eps_r": eps, "kappa": kappa})
+    return boxes
+
+
+def graded_port_scenario(n_cells=256, steps=10000, prec=0, seed=SEED_BASE + 2, n_boxes=32, pml=8, f0=5e9, fc=5e9,
+                         kappa_log=(-3.0, 0.0), eps_range=(1.0, 12.0)):
+    """Graded mesh (cell sizes U[0.5,2] mm per axis), 8-cell UPML on all faces,
+    random lossy dielectric boxes (eps_r ~ U[1,12], kappa ~ LogU[1e-3,1] S/m)
+    and a 50 Ohm lumped port with a Gaussian excitation.  Defaults give the
+    256^3-cell, 32-box, 10000-step case."""
+    rng = np.random.default_rng(seed)
+    lines = [graded_lines(rng, n_cells) for _ in range(3)]
+    op = Operator()
+    op.set_mesh(lines, 1e-3)
+    op.set_boundary([BC_PML] * 6, [pml] * 6)
+    op.set_gauss_excite(f0, fc)
+    boxes = random_boxes(rng, op, lines, n_boxes, pml, eps_range=eps_range, kappa_log=kappa_log)
+    c = n_cells // 2
+    port = add_lumped_port(op, lines, (c, c, c - 2), 4)
+    data = op.build(prec, nr_ts=steps)
+    interval = max(1, data.nyquist // 4)
+    probes = port_probes(op, port, interval)
+    _finish(op, data, probes)
+    return Scenario("graded_port_%d" % n_cells, data, steps, op, port=port,
+                    meta={"interval": interval, "boxes": boxes, "seed": seed, "f0": f0, "fc": fc})
+
+
+def uniform_bench(n_lines=512, steps=1000, prec=0):
+    """Uniform vacuum cube of n_lines^3 mesh lines, PEC walls, one soft Ez edge
+    at the centre (Gaussian f0=0, fc=10 GHz), 0.99 Courant."""
+    return cavity_point_source(n_cells=n_lines - 1, steps=steps, prec=prec)
+
+
+def s11_from_probes(t_u, u, t_i, i, freqs, Z_ref=50.0):
+    """S11 of a lumped port from its U/I time series, as python/openEMS/ports.py
+    Port.CalcPort (:103-141) with DFT_time2freq (python/openEMS/utilities.py:21-35)."""
+    def dft(t, val, f):
+        out = np.empty(len(f), dtype=np.complex128)
+        for k, fk in enumerate(f):
+            out[k] = np.sum(val * np.exp(-1j * 2 * np.pi * fk * t))
+        return 2 * out * (t[1] - t[0])
+    uf = dft(np.asarray(t_u), np.asarray(u), freqs)
+    itf = dft(np.asarray(t_i), np.asarray(i), freqs)
+    uf_inc = 0.5 * (uf + itf * Z_ref)
+    uf_ref = uf - uf_inc
+    return uf_ref / uf_inc
