@@ -1,0 +1,173 @@
+"""Benchmark of the MI355X FDTD engine (one GPU).
+
+    python bench.py [--steps K] [--warmup W] [--workload NAME] [--precision fp32]
+
+A step is one full FDTD timestep (Engine::IterateTS(1): voltage and current
+updates with the fused UPML, excitation and probe hooks) over the whole
+domain.  The default workload is the graded-mesh lumped-port case: 256^3
+cells (257^3 mesh lines) with cell sizes U[0.5,2] mm per axis, 8-cell UPML on
+all faces, 32 random lossy dielectric boxes, a 50 Ohm lumped port with a
+Gaussian (f0 = fc = 5 GHz) excitation, 10000 steps.  MCells/s counts mesh-line
+products per second (openems.cpp:1337-1338).
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "MCells/s + % HBM peak at 1 GPU; FP16-store & batched vs plain FP32 at stated err"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+WORKLOADS = {
+    # name: (builder, kwargs, description)
+    "graded_port_256": ("graded_port_scenario", {"n_cells": 256, "steps": 10000},
+                        "256^3 cells (257^3 lines), graded 0.5-2 mm, UPML-8 on 6 faces, 32 lossy eps boxes, 50 ohm lumped port, 10000 steps"),
+    "uniform_512": ("uniform_bench", {"n_lines": 512, "steps": 1000},
+                    "512^3 lines uniform 1 mm vacuum, PEC, soft Ez Gaussian at centre, 1000 steps"),
+    "cavity_64": ("cavity_point_source", {"n_cells": 64, "steps": 2000},
+                  "64^3 cells (65^3 lines) 1 mm vacuum, PEC, soft Ez Gaussian, 2000 steps"),
+}
+
+
+def kernel_bytes(eng, data):
+    """Algorithmic HBM bytes per cell of one launch of the voltage (or current)
+    update kernel: read 3 own + 3 other field components, write 3 own, plus
+    the coefficients (12 floats in full mode, a 2- or 4-byte index with the
+    compressed table)."""
+    fb = 8 if eng.precision == 1 else (4 if eng.precision == 0 else 2)
+    cb = 8 if eng.precision == 1 else 4
+    field = 9 * fb
+    if eng.compressed:
+        coef = 2 if (data.table is not None and len(data.table) <= 65536) else 4
+    else:
+        coef = 6 * cb
+    return field + coef
+
+
+def read_traffic(kernel, workload):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary, if any."""
+    path = os.path.join(ROOT, "profiles", "%s_%s_pmc.md" % (kernel, workload))
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            try:
+                return float(line.split(":", 1)[1].strip())
+            except ValueError:
+                return None
+    return None
+
+
+def cpu_baseline_leg(data, interval, cells, threads, steps):
+    from oracle.cpu_baseline import CpuBaseline
+    t0 = time.time()
+    b = CpuBaseline(data, threads)
+    log("cpu baseline: operator compressed to %d unique 4-lane tuples in %.1f s, %d threads" % (b.unique, time.time() - t0, b.nthreads))
+    dt = b.run(steps, interval)
+    val = cells * steps / dt / 1e6
+    return {"value": round(val, 2), "unit": "MCells/s", "cores": int(b.nthreads), "kind": "port",
+            "sample": "%d steps of the same workload (restated Engine_SSE_Compressed + Engine_Multithread, %d threads, %.1f s)" % (steps, b.nthreads, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--workload", default="graded_port_256", choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64", "fp16", "bf16"])
+    ap.add_argument("--coeffs", default="auto", choices=["auto", "full", "compressed"])
+    ap.add_argument("--cpu-steps", type=int, default=None, help="steps of the CPU-baseline sample (0 = skip)")
+    ap.add_argument("--cpu-threads", type=int, default=None)
+    ap.add_argument("--no-timing-pass", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: every box here has one MI355X and the engine has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    from wipems_amd import scenarios
+    from wipems_amd.engine import device_name, engine_from_operator
+
+    builder, kw, desc = WORKLOADS[args.workload]
+    kw = dict(kw)
+    steps = args.steps if args.steps is not None else kw["steps"]
+    kw["steps"] = max(steps, kw["steps"])
+    t0 = time.time()
+    sc = getattr(scenarios, builder)(**kw)
+    data = sc.data
+    cells = data.cells
+    log("workload %s: %d cells, dT=%.4e s, operator build %.1f s" % (args.workload, cells, data.dT, time.time() - t0))
+
+    compressed = {"auto": None, "full": False, "compressed": True}[args.coeffs]
+    eng = engine_from_operator(data, precision=args.precision, compressed=compressed)
+    log("device %s, coefficients %s, device bytes %.2f GB" % (device_name(), "compressed" if eng.compressed else "full", eng.device_bytes() / 1e9))
+
+    eng.iterate(args.warmup)
+    eng.synchronize()
+    t1 = time.perf_counter()
+    eng.iterate(steps)
+    eng.synchronize()
+    t2 = time.perf_counter()
+    wall = t2 - t1
+    value = cells * steps / wall / 1e6
+    log("timed %d steps in %.3f s -> %.1f MCells/s (%.3f ms/step)" % (steps, wall, value, 1e3 * wall / steps))
+
+    roofline = None
+    if not args.no_timing_pass:
+        # second pass of the same length with HIP events around every launch of
+        # the two update kernels, on the engine's stream
+        eng.set_kernel_timing(True)
+        eng.iterate(steps)
+        eng.synchronize()
+        tv, tc, nv, nc = eng.kernel_times()
+        eng.set_kernel_timing(False)
+        avg_v, avg_c = tv / nv, tc / nc
+        kern, avg = ("volt_update", avg_v) if tv >= tc else ("curr_update", avg_c)
+        bpc = kernel_bytes(eng, data)
+        alg = bpc * cells
+        achieved = alg / (avg * 1e-3) / 1e9
+        traffic = read_traffic(kern, args.workload)
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                    "kernel": kern, "avg_ms": round(avg, 5), "bytes_per_cell": bpc,
+                    "volt_avg_ms": round(avg_v, 5), "curr_avg_ms": round(avg_c, 5)}
+        log("kernel avg: volt %.4f ms, curr %.4f ms -> %s %.1f GB/s (%.1f%% of %.0f)" % (avg_v, avg_c, kern, achieved, 100 * achieved / HBM_PEAK_GBS, HBM_PEAK_GBS))
+
+    cpu = None
+    cpu_steps = args.cpu_steps
+    if cpu_steps is None:
+        cpu_steps = max(10, int(1.5e9 / cells))   # about 10-30 s of host work at ~100 MCells/s
+    if cpu_steps > 0 and args.precision == "fp32" and not data.ade and not data.tfsf:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline_leg(data, sc.meta.get("interval", 0), cells, threads, cpu_steps)
+        log("cpu baseline: %.1f MCells/s on %d threads" % (cpu["value"], cpu["cores"]))
+
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "MCells/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * wall / steps, 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": {"fp32": "f32", "fp64": "f64", "fp16": "f16-store/f32", "bf16": "bf16-store/f32"}[args.precision],
+        "data": "synthetic (seeded geometry)",
+        "config": {"workload": args.workload, "description": desc, "cells": cells,
+                   "coefficients": "compressed" if eng.compressed else "full", "precision": args.precision},
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

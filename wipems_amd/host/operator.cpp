@@ -79,6 +79,16 @@ struct Primitive {
     }
 };
 
+template <typename F>
+struct Store {
+    std::vector<F> EC[4][3];   // C, G, L, R per direction, as stored by the reference (FDTD_FLOAT)
+    std::vector<F> co[4][3];   // vv, vi, ii, iv per direction
+    void clear() {
+        for (int k = 0; k < 4; ++k)
+            for (int n = 0; n < 3; ++n) { std::vector<F>().swap(EC[k][n]); std::vector<F>().swap(co[k][n]); }
+    }
+};
+
 }  // namespace
 
 struct wop_operator {
@@ -101,8 +111,10 @@ struct wop_operator {
     bool built = false;
     int prec = 0;
     double dT = 0;
-    std::vector<double> ec[4];             // C,G,L,R as stored (F rounded), [3][N]
-    std::vector<double> coef[4];           // vv, vi, ii, iv as stored (F rounded), [3][N]
+    Store<float> sf;                       // float build results
+    Store<double> sd;                      // double build results
+    template <typename F> Store<F>& store();
+    double coef_at(int k, int n, size_t i) const { return prec == 0 ? (double)sf.co[k][n][i] : sd.co[k][n][i]; }
     std::vector<double> sig_v, sig_i;
     uint32_t exc_length = 0, nyquist = 0, max_step = 0;
 
@@ -363,10 +375,10 @@ namespace {
 template <typename F>
 struct Builder {
     wop_operator* op;
-    std::vector<F> EC[4][3];   // C,G,L,R per direction
-    std::vector<F> co[4][3];   // vv, vi, ii, iv per direction
+    std::vector<F> (&EC)[4][3];   // C,G,L,R per direction
+    std::vector<F> (&co)[4][3];   // vv, vi, ii, iv per direction
 
-    explicit Builder(wop_operator* o) : op(o) {}
+    explicit Builder(wop_operator* o) : op(o), EC(o->store<F>().EC), co(o->store<F>().co) {}
 
     F& VV(int n, size_t i) { return co[0][n][i]; }
     F& VI(int n, size_t i) { return co[1][n][i]; }
@@ -962,8 +974,13 @@ struct Builder {
 
 }  // namespace
 
+template <> Store<float>& wop_operator::store<float>() { return sf; }
+template <> Store<double>& wop_operator::store<double>() { return sd; }
+
 template <typename F>
 int wop_operator::build(uint32_t nr_ts) {
+    sf.clear();
+    sd.clear();
     Builder<F> b(this);
     b.CalcEC();
     // CalcECOperator timestep handling (FDTD/operator.cpp:981-1007)
@@ -991,17 +1008,23 @@ int wop_operator::build(uint32_t nr_ts) {
     b.Upml();
     b.Lorentz();
     b.Signal(nr_ts);
-    for (int k = 0; k < 4; ++k) {
-        ec[k].resize(3 * n);
-        coef[k].resize(3 * n);
-        for (int d = 0; d < 3; ++d)
-            for (size_t i = 0; i < n; ++i) {
-                ec[k][d * n + i] = (double)b.EC[k][d][i];
-                coef[k][d * n + i] = (double)b.co[k][d][i];
-            }
-    }
     built = true;
     return 0;
+}
+
+template <typename F>
+static void put3(const std::vector<F> (&src)[3], int dtype, void* dst) {
+    if (!dst) return;
+    const size_t n = src[0].size();
+    for (int d = 0; d < 3; ++d) {
+        if (dtype == 1) {
+            double* o = (double*)dst + d * n;
+            for (size_t i = 0; i < n; ++i) o[i] = (double)src[d][i];
+        } else {
+            float* o = (float*)dst + d * n;
+            for (size_t i = 0; i < n; ++i) o[i] = (float)src[d][i];
+        }
+    }
 }
 
 // ===========================================================================
@@ -1203,19 +1226,21 @@ static void put(const std::vector<double>& src, int dtype, void* dst) {
 
 int wop_get_coeffs(wop_operator* op, int dtype, void* vv, void* vi, void* ii, void* iv) {
     if (!op || !op->built) return fail(-1, "operator not built");
-    put(op->coef[0], dtype, vv);
-    put(op->coef[1], dtype, vi);
-    put(op->coef[2], dtype, ii);
-    put(op->coef[3], dtype, iv);
+    void* dst[4] = {vv, vi, ii, iv};
+    for (int k = 0; k < 4; ++k) {
+        if (op->prec == 0) put3(op->sf.co[k], dtype, dst[k]);
+        else put3(op->sd.co[k], dtype, dst[k]);
+    }
     return 0;
 }
 
 int wop_get_ec(wop_operator* op, double* C, double* G, double* L, double* R) {
     if (!op || !op->built) return fail(-1, "operator not built");
-    put(op->ec[0], 1, C);
-    put(op->ec[1], 1, G);
-    put(op->ec[2], 1, L);
-    put(op->ec[3], 1, R);
+    void* dst[4] = {C, G, L, R};
+    for (int k = 0; k < 4; ++k) {
+        if (op->prec == 0) put3(op->sf.EC[k], 1, dst[k]);
+        else put3(op->sd.EC[k], 1, dst[k]);
+    }
     return 0;
 }
 
@@ -1240,7 +1265,7 @@ extern "C" int wop_compress(wop_operator* op, uint32_t* n_entries, uint32_t* ind
         std::array<uint64_t, 12> t;
         for (int k = 0; k < 4; ++k)
             for (int d = 0; d < 3; ++d) {
-                double v = op->coef[k][d * n + i];
+                double v = op->coef_at(k, d, i);
                 uint64_t bits;
                 memcpy(&bits, &v, 8);
                 t[k * 3 + d] = bits;

@@ -96,6 +96,7 @@ class OperatorData:
     tfsf: Optional[dict] = None
     index: Optional[np.ndarray] = None      # compressed form, filled by compress()
     table: Optional[np.ndarray] = None
+    source: object = None                   # the Operator that built this data (native compression)
 
     @property
     def cells(self):
@@ -112,6 +113,11 @@ def compress_coeffs(data: OperatorData, max_entries=0):
 
     Returns (index uint32 [Nx][Ny][Nz], table float64 [n][12]) or None when the
     number of unique tuples exceeds max_entries (> 0)."""
+    if data.source is not None and getattr(data.source, "_built_prec", None) == data.prec:
+        res = data.source.compress(max_entries)
+        if res is not None:
+            data.index, data.table = res
+        return res
     nd = data.cells
     dt = np.float64 if data.prec == 1 else np.float32
     stack = np.empty((nd, 12), dtype=dt)
@@ -274,7 +280,24 @@ class Operator:
             lists = [np.empty((3, cnt.value), dtype=dt) for _ in range(6)]
             self._chk(lib.wop_get_ade(h, o, _up(pos), 1 if prec == 1 else 0, *[_vp(a) for a in lists]))
             ades.append(AdeOrder(pos, lists, fl.value))
-        return OperatorData(tuple(int(x) for x in n), dT.value, prec, *arrs, exc, slabs, ades, nyq.value, mstep.value)
+        self._built_prec = prec
+        return OperatorData(tuple(int(x) for x in n), dT.value, prec, *arrs, exc, slabs, ades, nyq.value, mstep.value,
+                            source=self)
+
+    def compress(self, max_entries=0):
+        """Native per-cell compression of the last build (wop_compress); None if
+        more than max_entries unique tuples exist."""
+        nu = c_uint32()
+        rc = self._lib.wop_compress(self._h, byref(nu), None, None, int(max_entries))
+        if rc != 0:
+            if max_entries and nu.value > max_entries:
+                return None
+            self._chk(rc)
+        n = self.num_lines
+        index = np.empty(n, dtype=np.uint32)
+        table = np.empty((nu.value, 12), dtype=np.float64)
+        self._chk(self._lib.wop_compress(self._h, byref(nu), _up(index), _dp(table), 0))
+        return index, table
 
     def get_ec(self):
         n = self.num_lines
