@@ -93,6 +93,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=None, help="steps of the CPU-baseline sample (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=None)
     ap.add_argument("--no-timing-pass", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=0,
+                    help="counter-collection mode: run this many steps of the workload, then a 1 GiB device copy "
+                         "of known size (the calibration kernel for FETCH_SIZE/WRITE_SIZE), and exit")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -101,7 +104,7 @@ def main():
         sys.exit(2)
 
     from wipems_amd import scenarios
-    from wipems_amd.engine import device_name, engine_from_operator
+    from wipems_amd.engine import copy_bandwidth, device_name, engine_from_operator
 
     builder, kw, desc = WORKLOADS[args.workload]
     kw = dict(kw)
@@ -116,6 +119,13 @@ def main():
     compressed = {"auto": None, "full": False, "compressed": True}[args.coeffs]
     eng = engine_from_operator(data, precision=args.precision, compressed=compressed)
     log("device %s, coefficients %s, device bytes %.2f GB" % (device_name(), "compressed" if eng.compressed else "full", eng.device_bytes() / 1e9))
+
+    if args.profile_steps:
+        eng.iterate(args.profile_steps)
+        eng.synchronize()
+        gb = copy_bandwidth(1 << 30, 3)
+        log("profile mode: %d steps + 4 launches of the 1 GiB copy kernel (%.0f GB/s)" % (args.profile_steps, gb))
+        return
 
     eng.iterate(args.warmup)
     eng.synchronize()
@@ -156,6 +166,11 @@ def main():
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         cpu = cpu_baseline_leg(data, sc.meta.get("interval", 0), cells, threads, cpu_steps)
         log("cpu baseline: %.1f MCells/s on %d threads" % (cpu["value"], cpu["cores"]))
+
+    copy_gbs = copy_bandwidth(1 << 30, 10)
+    if roofline is not None:
+        roofline["copy_kernel_GBs"] = round(copy_gbs, 1)
+    log("plain copy kernel: %.1f GB/s" % copy_gbs)
 
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "MCells/s", "n_gpus": 1, "steps": steps,

@@ -180,6 +180,11 @@ int  fdtd_read_probe_fd(fdtd_engine* h, int id, double* re, double* im);
 int  fdtd_iterate(fdtd_engine* h, uint32_t nsteps);
 int  fdtd_synchronize(fdtd_engine* h);
 int  fdtd_get_num_timesteps(fdtd_engine* h, uint32_t* numTS);
+/* Kernel shape: variant 0 = one thread per cell in hardware block order,
+ * 1 = x-marching columns (chunk_x planes per block), 2 = one thread per cell
+ * in XCD-banded order (chunk_x = y tiles per band); tile_y = rows per 64-lane
+ * tile (1, 2 or 4).  Results do not depend on the shape, only speed does. */
+int  fdtd_set_tuning(fdtd_engine* h, int variant, int tile_y, int chunk_x);
 /* Use a captured HIP graph for the step loop (default on). */
 int  fdtd_set_use_graph(fdtd_engine* h, int enable);
 /* Kernel timing: while enabled, fdtd_iterate runs eagerly and records HIP
@@ -196,6 +201,11 @@ int  fdtd_get_field(fdtd_engine* h, int which, int dtype, void* out);
 int  fdtd_set_field(fdtd_engine* h, int which, int dtype, const void* in);
 int  fdtd_get_value(fdtd_engine* h, int which, uint32_t n, uint32_t x, uint32_t y, uint32_t z, double* v);
 int  fdtd_set_value(fdtd_engine* h, int which, uint32_t n, uint32_t x, uint32_t y, uint32_t z, double v);
+
+/* Streaming copy of `bytes` bytes, `reps` times, on a private stream:
+ * achieved read+write bandwidth in GB/s.  Reference for the roofline and the
+ * calibration kernel (k_copy) for HBM byte counters. */
+int  fdtd_copy_bench(uint64_t bytes, int reps, double* gbps);
 
 /* Device bytes held by the handle (fields, coefficients, extensions). */
 int  fdtd_device_bytes(fdtd_engine* h, uint64_t* bytes);

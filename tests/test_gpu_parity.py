@@ -95,3 +95,33 @@ def test_graded_upml_port_fp32_compressed(gpu):
     eng, orc = run_pair(sc, 200, compressed=True)
     assert eng.compressed or sc.data.index is None
     compare(eng, orc)
+
+
+@pytest.mark.parametrize("variant,tile_y,chunk", [(0, 4, 64), (1, 1, 7), (1, 2, 16), (1, 4, 64), (1, 4, 1000), (2, 4, 1), (2, 2, 3), (2, 4, 8), (2, 1, 64)])
+def test_kernel_shapes_bitwise(gpu, variant, tile_y, chunk):
+    """Every kernel shape gives the same bits (graded mesh, UPML, port)."""
+    sc = scenarios.graded_port_scenario(n_cells=36, steps=150, prec=0, n_boxes=5, pml=8)
+    d = sc.data
+    eng = engine_from_operator(d, precision="fp32", compressed=False)
+    eng.set_tuning(variant, tile_y, chunk)
+    _kinds[id(eng)] = [p.kind for p in d.probes]
+    orc = oracle_from_operator(d, ftz=True)
+    eng.iterate(150)
+    orc.iterate(150)
+    compare(eng, orc)
+
+
+def test_cavity64_matches_golden_fixture(gpu):
+    """Full-size 64^3-cell / 2000-step case on the GPU against the committed
+    oracle fixture (tests/golden/cavity64.npz)."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cavity64.npz"), allow_pickle=False)
+    sc = scenarios.cavity_point_source(64, 2000, prec=0)
+    eng = engine_from_operator(sc.data, precision="fp32", compressed=True)
+    eng.iterate(2000)
+    tu, u = eng.read_probe(0)
+    ti, i = eng.read_probe(1)
+    np.testing.assert_array_equal(tu, g["t_u"])
+    np.testing.assert_array_equal(u, g["u"])
+    assert rel_l2(i, g["i"]) <= 1e-6
+    assert eng.energy() == pytest.approx(float(g["energy"]), rel=1e-12)

@@ -1,0 +1,159 @@
+"""Checks of the host operator build (wipems_amd/host/operator.cpp) -- CPU only.
+
+Closed forms and invariants that fix the coefficient build
+(Operator::CalcECOperator, FDTD/operator.cpp:973-1084):
+  * uniform vacuum mesh: C = eps0*d, L = mu0*d (float), vv = ii = 1,
+    vi = dT/C, iv = dT/L, and dT = 2/sqrt(12/(L*C)) * factor with the float
+    products of CalcTimestep_Var3 -- i.e. d/(c0*sqrt(3)) up to float rounding
+  * PEC walls: tangential vv/vi zero on all six faces, everything zero on the
+    upper faces, ii/iv zero on the last line (ApplyElectricBC/ApplyMagneticBC)
+  * quarter-cell averaging: a dielectric filling the domain scales C by eps_r;
+    an interface on a mesh plane gives the mean permittivity for E parallel to
+    it and the one-sided value for E normal to it
+  * lumped R: the series/parallel sum of the element's conductances is 1/R
+  * mirror symmetry of geometry gives mirrored coefficients
+  * compression round trip is exact and the unique count matches a set
+  * Gaussian signal and its constants (FDTD/excitation.cpp:151-177)
+"""
+import math
+
+import numpy as np
+import pytest
+
+from wipems_amd import scenarios
+from wipems_amd.operator import BC_PEC, Operator
+
+EPS0 = 8.85418781762e-12
+MUE0 = 1.256637062e-6
+
+
+def uniform_op(n=8, delta=1.0, unit=1e-3, factor=1.0):
+    op = Operator()
+    lines = np.arange(n + 1, dtype=np.float64) * delta
+    op.set_mesh([lines] * 3, unit)
+    op.set_boundary([BC_PEC] * 6)
+    op.set_timestep_factor(factor)
+    op.set_gauss_excite(0, 10e9)
+    return op
+
+
+def test_uniform_vacuum_closed_forms():
+    op = uniform_op(10, factor=0.99)
+    d = op.build(0, 1000)
+    C, G, L, R = op.get_ec()
+    dl = 1e-3
+    Cf = np.float32(EPS0 * dl)
+    Lf = np.float32(MUE0 * dl)
+    inner = (slice(None), slice(1, -1), slice(1, -1), slice(1, -1))
+    assert np.all(C[inner] == float(Cf))
+    assert np.all(L[inner] == float(Lf))
+    assert np.all(G == 0) and np.all(R == 0)
+    t = np.float32(1) / (Lf * Cf)
+    dT = 2 / math.sqrt(12 * float(t)) * 0.99
+    assert d.dT == dT
+    assert d.dT == pytest.approx(dl / (299792458 * math.sqrt(3)) * 0.99, rel=1e-6)
+    core = (slice(None), slice(1, -2), slice(1, -2), slice(1, -2))
+    assert np.all(d.vv[core] == 1) and np.all(d.ii[core] == 1)
+    assert np.all(d.vi[core] == np.float32(d.dT / float(Cf)))
+    assert np.all(d.iv[core] == np.float32(d.dT / float(Lf)))
+
+
+def test_pec_boundary_masks():
+    op = uniform_op(6)
+    d = op.build(0, 100)
+    for n in range(3):
+        nP, nPP = (n + 1) % 3, (n + 2) % 3
+        lo = [slice(None)] * 3
+        lo[n] = 0
+        hi = [slice(None)] * 3
+        hi[n] = -1
+        for c in (nP, nPP):
+            assert np.all(d.vv[(c,) + tuple(lo)] == 0) and np.all(d.vi[(c,) + tuple(lo)] == 0)
+        for c in range(3):
+            assert np.all(d.vv[(c,) + tuple(hi)] == 0) and np.all(d.vi[(c,) + tuple(hi)] == 0)
+            assert np.all(d.ii[(c,) + tuple(hi)] == 0) and np.all(d.iv[(c,) + tuple(hi)] == 0)
+        # the normal component on the lower face is a regular edge
+        assert np.all(d.vi[(n,) + tuple(lo)][1:-1, 1:-1] != 0)
+
+
+def test_filled_dielectric_scales_C():
+    op = uniform_op(6)
+    m = op.add_material(eps_r=4.0)
+    op.add_box(m, (-1, -1, -1), (10, 10, 10))
+    op.build(1, 100)
+    C4 = op.get_ec()[0]
+    op0 = uniform_op(6)
+    op0.build(1, 100)
+    C1 = op0.get_ec()[0]
+    np.testing.assert_allclose(C4, 4 * C1, rtol=1e-14, atol=0)
+
+
+def test_interface_averaging():
+    """Half space eps_r=5 for x < 3 (interface on mesh plane x=3)."""
+    op = uniform_op(6)
+    m = op.add_material(eps_r=5.0)
+    op.add_box(m, (-1, -1, -1), (3, 10, 10))
+    op.build(1, 100)
+    C = op.get_ec()[0]
+    op0 = uniform_op(6)
+    op0.build(1, 100)
+    C1 = op0.get_ec()[0]
+    # E_y on the interface plane x=3: two quarter cells on each side -> mean
+    np.testing.assert_allclose(C[1, 3, 2, 2], 3.0 * C1[1, 3, 2, 2], rtol=1e-14)
+    # E_x sample points sit at x+dx/2: normal component on one side only
+    np.testing.assert_allclose(C[0, 2, 2, 2], 5.0 * C1[0, 2, 2, 2], rtol=1e-14)
+    np.testing.assert_allclose(C[0, 3, 2, 2], C1[0, 3, 2, 2], rtol=1e-14)
+
+
+def test_lumped_resistor_total_conductance():
+    rng = np.random.default_rng(11)
+    lines = [np.concatenate([[0], np.cumsum(rng.uniform(0.5, 2, 10))]) for _ in range(3)]
+    op = Operator()
+    op.set_mesh(lines, 1e-3)
+    op.set_boundary([BC_PEC] * 6)
+    op.set_gauss_excite(0, 5e9)
+    le = op.add_lumped_element(2, R=50.0, caps=False)
+    op.add_box(le, (lines[0][3], lines[1][4], lines[2][2]), (lines[0][5], lines[1][6], lines[2][7]))
+    op.build(1, 100)
+    G = op.get_ec()[1]
+    total = 0.0
+    for k in range(2, 7):
+        total += 1.0 / G[2, 3:6, 4:7, k].sum()     # planes in series, edges in parallel
+    assert 1.0 / total == pytest.approx(1 / 50.0, rel=1e-12)
+
+
+def test_mirror_symmetry():
+    op = uniform_op(8)
+    m = op.add_material(eps_r=3.0, kappa=0.1)
+    op.add_box(m, (1.5, 2, 2), (3, 6, 6))
+    m2 = op.add_material(eps_r=3.0, kappa=0.1)
+    op.add_box(m2, (5, 2, 2), (6.5, 6, 6))
+    d = op.build(1, 100)
+    # x -> 8 - x maps the geometry to itself; E_y, E_z edges at x map to 8-x
+    vy = d.vi[1]
+    np.testing.assert_allclose(vy[1:8], vy[7:0:-1], rtol=1e-13)
+
+
+def test_compression_roundtrip_and_count():
+    sc = scenarios.graded_port_scenario(n_cells=20, steps=50, prec=0, n_boxes=3, pml=4)
+    d = sc.data
+    idx, tab = d.compress()
+    stack = np.stack([d.vv, d.vi, d.ii, d.iv]).transpose(2, 3, 4, 0, 1).reshape(-1, 12)
+    rebuilt = tab[idx.reshape(-1)]
+    np.testing.assert_array_equal(rebuilt.astype(np.float32), stack)
+    assert len(tab) == len({tuple(r) for r in stack.view(np.uint32).tolist()})
+
+
+def test_gaussian_signal_constants():
+    sc = scenarios.cavity_point_source(64, 2000, prec=0)
+    d = sc.data
+    assert d.excitation.length == 151
+    assert d.nyquist == 26
+    assert sc.meta["interval"] == 6
+    dT = d.dT
+    n = np.arange(151)
+    fc = 10e9
+    ref = np.cos(0 * (n * dT)) * np.exp(-((2 * math.pi * fc * n * dT / 3 - 3) ** 2))
+    np.testing.assert_array_equal(d.excitation.signal_volt, ref.astype(np.float32).astype(np.float64))
+    assert d.max_excite_step == int(np.argmax(np.abs(ref.astype(np.float32))))
+    assert d.excitation.signal_volt[0] == np.float32(math.exp(-9))

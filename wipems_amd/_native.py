@@ -54,6 +54,7 @@ _ENGINE_SIGS = {
     "fdtd_synchronize": [c_void_p],
     "fdtd_get_num_timesteps": [c_void_p, POINTER(c_uint32)],
     "fdtd_set_use_graph": [c_void_p, c_int],
+    "fdtd_set_tuning": [c_void_p, c_int, c_int, c_int],
     "fdtd_set_kernel_timing": [c_void_p, c_int],
     "fdtd_get_kernel_times": [c_void_p, POINTER(c_double), POINTER(c_double), POINTER(c_uint32), POINTER(c_uint32)],
     "fdtd_calc_energy": [c_void_p, POINTER(c_double)],
@@ -62,6 +63,7 @@ _ENGINE_SIGS = {
     "fdtd_get_value": [c_void_p, c_int, c_uint32, c_uint32, c_uint32, c_uint32, POINTER(c_double)],
     "fdtd_set_value": [c_void_p, c_int, c_uint32, c_uint32, c_uint32, c_uint32, c_double],
     "fdtd_device_bytes": [c_void_p, POINTER(c_uint64)],
+    "fdtd_copy_bench": [c_uint64, c_int, POINTER(c_double)],
 }
 
 _OPERATOR_SIGS = {

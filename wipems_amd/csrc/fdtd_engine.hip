@@ -174,15 +174,11 @@ __device__ __forceinline__ void load_coeffs(const CoeffPtrs<A>& cf, long long p,
 //   post: h = F ; F = V ; V = h + fn*F
 // ---------------------------------------------------------------------------
 template <typename S, int CM, bool PML>
-__global__ void __launch_bounds__(256)
-k_volt(S* __restrict__ V, const S* __restrict__ I, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
-       Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp) {
+__device__ __forceinline__ void volt_cell(S* __restrict__ V, const S* __restrict__ I, const CoeffPtrs<typename StoreTraits<S>::A>& cf,
+                                          const Dims& d, const Scales<typename StoreTraits<S>::A>& sc, const PmlSet& ps,
+                                          const PmlPtrs<typename StoreTraits<S>::A>& pp, int x, int y, int z) {
     typedef StoreTraits<S> T;
     typedef typename T::A A;
-    const int z = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    const int x = blockIdx.z;
-    if (z >= d.nz || y >= d.ny) return;
     const long long p = x * d.sx + y * d.sy + z;
     const long long c1 = d.sc, c2 = 2 * d.sc;
     const long long ox = x > 0 ? d.sx : 0;   // x-1 neighbour, self at x=0 (shift[0])
@@ -230,20 +226,27 @@ k_volt(S* __restrict__ V, const S* __restrict__ I, CoeffPtrs<typename StoreTrait
     }
 }
 
+template <typename S, int CM, bool PML>
+__global__ void __launch_bounds__(256)
+k_volt(S* __restrict__ V, const S* __restrict__ I, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
+       Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp) {
+    const int z = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int x = blockIdx.z;
+    if (z >= d.nz || y >= d.ny) return;
+    volt_cell<S, CM, PML>(V, I, cf, d, sc, ps, pp, x, y, z);
+}
+
 // ---------------------------------------------------------------------------
 // main current update (Engine::UpdateCurrents, FDTD/engine.cpp:163-210) over
 // x<Nx-1, y<Ny-1, z<Nz-1 with UPML fused (engine_ext_upml.cpp:141-234).
 // ---------------------------------------------------------------------------
 template <typename S, int CM, bool PML>
-__global__ void __launch_bounds__(256)
-k_curr(S* __restrict__ I, const S* __restrict__ V, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
-       Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp) {
+__device__ __forceinline__ void curr_cell(S* __restrict__ I, const S* __restrict__ V, const CoeffPtrs<typename StoreTraits<S>::A>& cf,
+                                          const Dims& d, const Scales<typename StoreTraits<S>::A>& sc, const PmlSet& ps,
+                                          const PmlPtrs<typename StoreTraits<S>::A>& pp, int x, int y, int z) {
     typedef StoreTraits<S> T;
     typedef typename T::A A;
-    const int z = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    const int x = blockIdx.z;
-    if (z >= d.nz - 1 || y >= d.ny - 1 || x >= d.nx - 1) return;
     const long long p = x * d.sx + y * d.sy + z;
     const long long c1 = d.sc, c2 = 2 * d.sc;
 
@@ -286,6 +289,226 @@ k_curr(S* __restrict__ I, const S* __restrict__ V, CoeffPtrs<typename StoreTrait
             c = c + iv[n] * curl[n];
         }
         T::store(ip, c, sc.ic[n]);
+    }
+}
+
+template <typename S, int CM, bool PML>
+__global__ void __launch_bounds__(256)
+k_curr(S* __restrict__ I, const S* __restrict__ V, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
+       Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp) {
+    const int z = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int x = blockIdx.z;
+    if (z >= d.nz - 1 || y >= d.ny - 1 || x >= d.nx - 1) return;
+    curr_cell<S, CM, PML>(I, V, cf, d, sc, ps, pp, x, y, z);
+}
+
+// ---------------------------------------------------------------------------
+// banded launch order: one thread per cell like k_volt/k_curr, but block ids
+// are remapped so that each XCD (hardware places block b on XCD b % 8) walks
+// a contiguous run of logical tiles ordered (y band, x, y tile in band, z
+// tile).  A band plane (band_tiles x tile_y rows x Nz) is small, so the x-1
+// plane and the y-1 rows a tile needs were just fetched into the same XCD's
+// L2 by the tiles before it.
+// ---------------------------------------------------------------------------
+struct BandGrid {
+    int tiles_z, tiles_y, band_tiles, nbands, nx;
+    int nblocks, per_xcd;
+};
+
+__device__ __forceinline__ bool band_tile(const BandGrid& g, int& tz, int& ty, int& x) {
+    const int b = blockIdx.x;
+    const int t = (b & 7) * g.per_xcd + (b >> 3);
+    if (t >= g.nblocks) return false;
+    tz = t % g.tiles_z;
+    int r = t / g.tiles_z;
+    const int tyb = r % g.band_tiles;
+    r /= g.band_tiles;
+    x = r % g.nx;
+    const int band = r / g.nx;
+    ty = band * g.band_tiles + tyb;
+    return ty < g.tiles_y;
+}
+
+template <typename S, int CM, bool PML>
+__global__ void __launch_bounds__(256)
+k_volt_band(S* __restrict__ V, const S* __restrict__ I, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
+            Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp, BandGrid g) {
+    int tz, ty, x;
+    if (!band_tile(g, tz, ty, x)) return;
+    const int z = tz * 64 + threadIdx.x;
+    const int y = ty * blockDim.y + threadIdx.y;
+    if (z >= d.nz || y >= d.ny) return;
+    volt_cell<S, CM, PML>(V, I, cf, d, sc, ps, pp, x, y, z);
+}
+
+template <typename S, int CM, bool PML>
+__global__ void __launch_bounds__(256)
+k_curr_band(S* __restrict__ I, const S* __restrict__ V, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
+            Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp, BandGrid g) {
+    int tz, ty, x;
+    if (!band_tile(g, tz, ty, x)) return;
+    const int z = tz * 64 + threadIdx.x;
+    const int y = ty * blockDim.y + threadIdx.y;
+    if (z >= d.nz - 1 || y >= d.ny - 1) return;
+    curr_cell<S, CM, PML>(I, V, cf, d, sc, ps, pp, x, y, z);
+}
+
+// ---------------------------------------------------------------------------
+// x-marching versions of the two updates (2.5-D blocking).  A block is a
+// 64(z) x TY(y) tile of columns; each thread walks its (y,z) column through an
+// x chunk and keeps the x-neighbour values of the previous plane in
+// registers, so every field element is fetched from HBM once per kernel; the
+// y and z neighbours are re-read by adjacent lanes/waves of the same block
+// and hit L1/L2.  The linear block id is remapped so that consecutive tiles
+// (which share their y halo rows) run on the same XCD and share its L2.
+// Arithmetic is identical to k_volt / k_curr.
+// ---------------------------------------------------------------------------
+struct MarchGrid {
+    int tiles_z, tiles_y, chunks_x;    // logical tile grid
+    int chunk;                         // x planes per chunk
+    int nblocks;                       // tiles_z * tiles_y * chunks_x
+    int per_xcd;                       // ceil(nblocks / 8)
+};
+
+__device__ __forceinline__ bool march_tile(const MarchGrid& g, int& tz, int& ty, int& cx) {
+    // hardware dispatch places block b on XCD b % 8; give each XCD a
+    // contiguous run of logical tiles (z fastest, then y, then x chunk)
+    const int b = blockIdx.x;
+    const int t = (b & 7) * g.per_xcd + (b >> 3);
+    if (t >= g.nblocks) return false;
+    tz = t % g.tiles_z;
+    const int r = t / g.tiles_z;
+    ty = r % g.tiles_y;
+    cx = r / g.tiles_y;
+    return true;
+}
+
+template <typename S, int CM, bool PML>
+__global__ void __launch_bounds__(256)
+k_volt_march(S* __restrict__ V, const S* __restrict__ I, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
+             Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp, MarchGrid g) {
+    typedef StoreTraits<S> T;
+    typedef typename T::A A;
+    int tz, ty, cx;
+    if (!march_tile(g, tz, ty, cx)) return;
+    const int z = tz * 64 + threadIdx.x;
+    const int y = ty * blockDim.y + threadIdx.y;
+    if (z >= d.nz || y >= d.ny) return;
+    const int x0 = cx * g.chunk;
+    const int x1 = min(d.nx, x0 + g.chunk);
+    const long long c1 = d.sc, c2 = 2 * d.sc;
+    const long long oy = y > 0 ? d.sy : 0;
+    long long p = x0 * d.sx + y * d.sy + z;
+    // x-1 neighbours of the first plane (the cell itself at x = 0)
+    A i1_prev, i2_prev;
+    if (x0 > 0) {
+        i1_prev = T::load(I + c1 + p - d.sx, sc.c[1]);
+        i2_prev = T::load(I + c2 + p - d.sx, sc.c[2]);
+    } else {
+        i1_prev = T::load(I + c1 + p, sc.c[1]);
+        i2_prev = T::load(I + c2 + p, sc.c[2]);
+    }
+    for (int x = x0; x < x1; ++x, p += d.sx) {
+        const A i0 = T::load(I + p, sc.c[0]);
+        const A i1 = T::load(I + c1 + p, sc.c[1]);
+        const A i2 = T::load(I + c2 + p, sc.c[2]);
+        const A i2_ym = T::load(I + c2 + p - oy, sc.c[2]);
+        const A i0_ym = T::load(I + p - oy, sc.c[0]);
+        const A i1_zm = z > 0 ? T::load(I + c1 + p - 1, sc.c[1]) : A(0);
+        const A i0_zm = z > 0 ? T::load(I + p - 1, sc.c[0]) : A(0);
+        A curl[3];
+        curl[0] = ((i2 - i2_ym) - i1) + i1_zm;
+        curl[1] = ((i0 - i0_zm) - i2) + i2_prev;
+        curl[2] = ((i1 - i1_prev) - i0) + i0_ym;
+        i1_prev = i1;
+        i2_prev = i2;
+
+        A vv[3], vi[3];
+        load_coeffs<A, CM>(cf, p, d.sc, vv, vi);
+        long long lp = 0;
+        int slab = PML ? find_slab(ps, x, y, z, lp) : -1;
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+            S* vp = V + n * d.sc + p;
+            A v = T::load(vp, sc.v[n]);
+            if (PML && slab >= 0) {
+                const PmlSlab& b = ps.slab[slab];
+                const long long q = lp + n * b.sc;
+                A F = pp.vflux[q];
+                A h = pp.vv[q] * v - pp.vvfo[q] * F;
+                A t = F;
+                t = t * vv[n];
+                t = t + vi[n] * curl[n];
+                pp.vflux[q] = t;
+                v = h + pp.vvfn[q] * t;
+            } else {
+                v = v * vv[n];
+                v = v + vi[n] * curl[n];
+            }
+            T::store(vp, v, sc.iv[n]);
+        }
+    }
+}
+
+template <typename S, int CM, bool PML>
+__global__ void __launch_bounds__(256)
+k_curr_march(S* __restrict__ I, const S* __restrict__ V, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
+             Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp, MarchGrid g) {
+    typedef StoreTraits<S> T;
+    typedef typename T::A A;
+    int tz, ty, cx;
+    if (!march_tile(g, tz, ty, cx)) return;
+    const int z = tz * 64 + threadIdx.x;
+    const int y = ty * blockDim.y + threadIdx.y;
+    if (z >= d.nz - 1 || y >= d.ny - 1) return;
+    const int x0 = cx * g.chunk;
+    const int x1 = min(d.nx - 1, x0 + g.chunk);
+    if (x0 >= x1) return;
+    const long long c1 = d.sc, c2 = 2 * d.sc;
+    long long p = x0 * d.sx + y * d.sy + z;
+    // V1, V2 of the current plane come from the previous iteration's x+1 loads
+    A v1 = T::load(V + c1 + p, sc.v[1]);
+    A v2 = T::load(V + c2 + p, sc.v[2]);
+    for (int x = x0; x < x1; ++x, p += d.sx) {
+        const A v0 = T::load(V + p, sc.v[0]);
+        const A v2_xp = T::load(V + c2 + p + d.sx, sc.v[2]);
+        const A v1_xp = T::load(V + c1 + p + d.sx, sc.v[1]);
+        const A v2_yp = T::load(V + c2 + p + d.sy, sc.v[2]);
+        const A v0_yp = T::load(V + p + d.sy, sc.v[0]);
+        const A v1_zp = T::load(V + c1 + p + 1, sc.v[1]);
+        const A v0_zp = T::load(V + p + 1, sc.v[0]);
+        A curl[3];
+        curl[0] = ((v2 - v2_yp) - v1) + v1_zp;
+        curl[1] = ((v0 - v0_zp) - v2) + v2_xp;
+        curl[2] = ((v1 - v1_xp) - v0) + v0_yp;
+        v1 = v1_xp;
+        v2 = v2_xp;
+
+        A ii[3], iv[3];
+        load_coeffs<A, CM>(cf, p, d.sc, ii, iv);
+        long long lp = 0;
+        int slab = PML ? find_slab(ps, x, y, z, lp) : -1;
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+            S* ip = I + n * d.sc + p;
+            A c = T::load(ip, sc.c[n]);
+            if (PML && slab >= 0) {
+                const PmlSlab& b = ps.slab[slab];
+                const long long q = lp + n * b.sc;
+                A F = pp.cflux[q];
+                A h = pp.ii[q] * c - pp.iifo[q] * F;
+                A t = F;
+                t = t * ii[n];
+                t = t + iv[n] * curl[n];
+                pp.cflux[q] = t;
+                c = h + pp.iifn[q] * t;
+            } else {
+                c = c * ii[n];
+                c = c + iv[n] * curl[n];
+            }
+            T::store(ip, c, sc.ic[n]);
+        }
     }
 }
 
@@ -545,6 +768,13 @@ __global__ void k_energy_final(const double* part, int n, double* out) {
     if (threadIdx.x == 0) { out[0] = se[0]; out[1] = sh[0]; }
 }
 
+// plain streaming copy: the calibration kernel for HBM byte counters and the
+// achievable-bandwidth reference
+__global__ void __launch_bounds__(256) k_copy(const float4* __restrict__ a, float4* __restrict__ b, long long n) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+
 // gather/scatter between the padded device layout and the dense host layout
 template <typename S, typename H>
 __global__ void k_pack(const S* __restrict__ F, H* __restrict__ out, Dims d, Scales<typename StoreTraits<S>::A> sc, int is_curr) {
@@ -673,6 +903,11 @@ struct fdtd_engine {
     int graph_steps = 0;
     hipGraphExec_t gexec1 = nullptr;
     hipGraph_t graph1 = nullptr;
+
+    // kernel selection: 0 = one thread per cell (default), 1 = x-march
+    int variant = 0;
+    int tile_y = 4;
+    int chunk_x = 64;
 
     // timing
     bool timing = false;
@@ -878,6 +1113,81 @@ void launch_curr_cm(fdtd_engine* h, dim3 g, dim3 b, CoeffPtrs<typename StoreTrai
     }
 }
 
+MarchGrid make_march(const fdtd_engine* h, int ny_eff, int nx_eff) {
+    MarchGrid g;
+    g.tiles_z = (h->N[2] + 63) / 64;
+    g.tiles_y = (ny_eff + h->tile_y - 1) / h->tile_y;
+    g.chunk = std::max(1, std::min(h->chunk_x, nx_eff));
+    g.chunks_x = (nx_eff + g.chunk - 1) / g.chunk;
+    g.nblocks = g.tiles_z * g.tiles_y * g.chunks_x;
+    g.per_xcd = (g.nblocks + 7) / 8;
+    return g;
+}
+
+template <typename S, bool PML>
+void launch_volt_march(fdtd_engine* h, CoeffPtrs<typename StoreTraits<S>::A> cf, PmlPtrs<typename StoreTraits<S>::A> pp) {
+    typedef typename StoreTraits<S>::A A;
+    Scales<A> sc = make_scales<A>(h);
+    MarchGrid g = make_march(h, h->N[1], h->N[0]);
+    const dim3 b(64, h->tile_y, 1), gr(g.per_xcd * 8);
+    switch (h->cf_mode) {
+    case CF_FULL: hipLaunchKernelGGL((k_volt_march<S, CF_FULL, PML>), gr, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, sc, h->pml, pp, g); break;
+    case CF_IDX16: hipLaunchKernelGGL((k_volt_march<S, CF_IDX16, PML>), gr, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, sc, h->pml, pp, g); break;
+    default: hipLaunchKernelGGL((k_volt_march<S, CF_IDX32, PML>), gr, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, sc, h->pml, pp, g); break;
+    }
+}
+
+template <typename S, bool PML>
+void launch_curr_march(fdtd_engine* h, CoeffPtrs<typename StoreTraits<S>::A> cf, PmlPtrs<typename StoreTraits<S>::A> pp) {
+    typedef typename StoreTraits<S>::A A;
+    Scales<A> sc = make_scales<A>(h);
+    MarchGrid g = make_march(h, h->N[1] - 1, h->N[0] - 1);
+    const dim3 b(64, h->tile_y, 1), gr(g.per_xcd * 8);
+    switch (h->cf_mode) {
+    case CF_FULL: hipLaunchKernelGGL((k_curr_march<S, CF_FULL, PML>), gr, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, sc, h->pml, pp, g); break;
+    case CF_IDX16: hipLaunchKernelGGL((k_curr_march<S, CF_IDX16, PML>), gr, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, sc, h->pml, pp, g); break;
+    default: hipLaunchKernelGGL((k_curr_march<S, CF_IDX32, PML>), gr, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, sc, h->pml, pp, g); break;
+    }
+}
+
+BandGrid make_band(const fdtd_engine* h, int ny_eff, int nx_eff) {
+    BandGrid g;
+    g.tiles_z = (h->N[2] + 63) / 64;
+    g.tiles_y = (ny_eff + h->tile_y - 1) / h->tile_y;
+    g.band_tiles = std::max(1, std::min(h->chunk_x, g.tiles_y));
+    g.nbands = (g.tiles_y + g.band_tiles - 1) / g.band_tiles;
+    g.nx = nx_eff;
+    g.nblocks = g.tiles_z * g.band_tiles * g.nx * g.nbands;
+    g.per_xcd = (g.nblocks + 7) / 8;
+    return g;
+}
+
+template <typename S, bool PML>
+void launch_volt_band(fdtd_engine* h, CoeffPtrs<typename StoreTraits<S>::A> cf, PmlPtrs<typename StoreTraits<S>::A> pp) {
+    typedef typename StoreTraits<S>::A A;
+    Scales<A> sc = make_scales<A>(h);
+    BandGrid g = make_band(h, h->N[1], h->N[0]);
+    const dim3 b(64, h->tile_y, 1), gr(g.per_xcd * 8);
+    switch (h->cf_mode) {
+    case CF_FULL: hipLaunchKernelGGL((k_volt_band<S, CF_FULL, PML>), gr, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, sc, h->pml, pp, g); break;
+    case CF_IDX16: hipLaunchKernelGGL((k_volt_band<S, CF_IDX16, PML>), gr, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, sc, h->pml, pp, g); break;
+    default: hipLaunchKernelGGL((k_volt_band<S, CF_IDX32, PML>), gr, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, sc, h->pml, pp, g); break;
+    }
+}
+
+template <typename S, bool PML>
+void launch_curr_band(fdtd_engine* h, CoeffPtrs<typename StoreTraits<S>::A> cf, PmlPtrs<typename StoreTraits<S>::A> pp) {
+    typedef typename StoreTraits<S>::A A;
+    Scales<A> sc = make_scales<A>(h);
+    BandGrid g = make_band(h, h->N[1] - 1, h->N[0] - 1);
+    const dim3 b(64, h->tile_y, 1), gr(g.per_xcd * 8);
+    switch (h->cf_mode) {
+    case CF_FULL: hipLaunchKernelGGL((k_curr_band<S, CF_FULL, PML>), gr, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, sc, h->pml, pp, g); break;
+    case CF_IDX16: hipLaunchKernelGGL((k_curr_band<S, CF_IDX16, PML>), gr, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, sc, h->pml, pp, g); break;
+    default: hipLaunchKernelGGL((k_curr_band<S, CF_IDX32, PML>), gr, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, sc, h->pml, pp, g); break;
+    }
+}
+
 template <typename S>
 int launch_step_kernels(fdtd_engine* h, bool timed) {
     typedef typename StoreTraits<S>::A A;
@@ -926,7 +1236,9 @@ int launch_step_kernels(fdtd_engine* h, bool timed) {
     const size_t evi = timed ? h->ev_used : 0;
     if (timed) h->ev_used += 4;
     if (timed) HIPCHK(hipEventRecord(h->ev[evi], h->stream));
-    if (pml) launch_volt_cm<S, true>(h, grd, blk, cfe, pp); else launch_volt_cm<S, false>(h, grd, blk, cfe, pp);
+    if (h->variant == 2) { if (pml) launch_volt_band<S, true>(h, cfe, pp); else launch_volt_band<S, false>(h, cfe, pp); }
+    else if (h->variant == 1) { if (pml) launch_volt_march<S, true>(h, cfe, pp); else launch_volt_march<S, false>(h, cfe, pp); }
+    else if (pml) launch_volt_cm<S, true>(h, grd, blk, cfe, pp); else launch_volt_cm<S, false>(h, grd, blk, cfe, pp);
     if (timed) HIPCHK(hipEventRecord(h->ev[evi + 1], h->stream));
     if (h->tfsf_v.count)
         hipLaunchKernelGGL((k_tfsf<S>), dim3((h->tfsf_v.count + 255) / 256), dim3(256), 0, h->stream, h->V.as<S>(), stp, h->tfsf_v, sc, 0);
@@ -941,7 +1253,9 @@ int launch_step_kernels(fdtd_engine* h, bool timed) {
         if ((a->flags & 2) && a->count)
             hipLaunchKernelGGL((k_ade_pre<S>), dim3((a->count + 255) / 256), dim3(256), 0, h->stream, h->I.as<S>(), ade_dev(a), h->d.sc, sc, 1);
     if (timed) HIPCHK(hipEventRecord(h->ev[evi + 2], h->stream));
-    if (pml) launch_curr_cm<S, true>(h, grdc, blk, cfh, pp); else launch_curr_cm<S, false>(h, grdc, blk, cfh, pp);
+    if (h->variant == 2) { if (pml) launch_curr_band<S, true>(h, cfh, pp); else launch_curr_band<S, false>(h, cfh, pp); }
+    else if (h->variant == 1) { if (pml) launch_curr_march<S, true>(h, cfh, pp); else launch_curr_march<S, false>(h, cfh, pp); }
+    else if (pml) launch_curr_cm<S, true>(h, grdc, blk, cfh, pp); else launch_curr_cm<S, false>(h, grdc, blk, cfh, pp);
     if (timed) HIPCHK(hipEventRecord(h->ev[evi + 3], h->stream));
     if (h->tfsf_c.count)
         hipLaunchKernelGGL((k_tfsf<S>), dim3((h->tfsf_c.count + 255) / 256), dim3(256), 0, h->stream, h->I.as<S>(), stp, h->tfsf_c, sc, 1);
@@ -1496,6 +1810,18 @@ int fdtd_read_probe_fd(fdtd_engine* h, int id, double* re, double* im) {
     return FDTD_OK;
 }
 
+int fdtd_set_tuning(fdtd_engine* h, int variant, int tile_y, int chunk_x) {
+    if (!h) return fail(FDTD_ERR_ARG, "null handle");
+    if (variant < 0 || variant > 2) return fail(FDTD_ERR_ARG, "unknown kernel variant %d", variant);
+    if (tile_y != 1 && tile_y != 2 && tile_y != 4) return fail(FDTD_ERR_ARG, "tile_y must be 1, 2 or 4");
+    if (chunk_x < 1) return fail(FDTD_ERR_ARG, "chunk_x must be >= 1");
+    h->variant = variant;
+    h->tile_y = tile_y;
+    h->chunk_x = chunk_x;
+    invalidate_graph(h);
+    return FDTD_OK;
+}
+
 int fdtd_set_use_graph(fdtd_engine* h, int enable) {
     if (!h) return fail(FDTD_ERR_ARG, "null handle");
     h->use_graph = enable != 0;
@@ -1649,6 +1975,33 @@ int fdtd_set_value(fdtd_engine* h, int which, uint32_t n, uint32_t x, uint32_t y
     DevBuf& F = which == FDTD_VOLT ? h->V : h->I;
     HIPCHK(hipMemcpyAsync((char*)F.p + off * h->elem, raw, h->elem, hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
+    return FDTD_OK;
+}
+
+int fdtd_copy_bench(uint64_t bytes, int reps, double* gbps) {
+    if (!gbps || reps < 1 || bytes < 4096) return fail(FDTD_ERR_ARG, "bad argument");
+    const long long n = (long long)(bytes / 16);
+    DevBuf a, b;
+    int rc;
+    if ((rc = a.alloc((size_t)n * 16)) || (rc = b.alloc((size_t)n * 16))) return rc;
+    hipStream_t st;
+    HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipMemsetAsync(a.p, 0, a.bytes, st));
+    const int blocks = 256 * 16;
+    hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, st, a.as<float4>(), b.as<float4>(), n);   // warm-up
+    HIPCHK(hipEventRecord(e0, st));
+    for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, st, a.as<float4>(), b.as<float4>(), n);
+    HIPCHK(hipEventRecord(e1, st));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(st);
+    *gbps = 2.0 * (double)n * 16 * reps / (ms * 1e-3) / 1e9;
     return FDTD_OK;
 }
 

@@ -49,6 +49,14 @@ def device_name():
     return buf.value.decode()
 
 
+def copy_bandwidth(nbytes=1 << 30, reps=10):
+    """Achieved read+write GB/s of a plain device copy of nbytes (k_copy)."""
+    lib = engine_lib()
+    g = c_double()
+    check(lib, lib.fdtd_copy_bench(int(nbytes), int(reps), byref(g)), "fdtd_last_error")
+    return g.value
+
+
 class Engine:
     """One FDTD domain on the GPU (Engine_HIP)."""
 
@@ -161,6 +169,9 @@ class Engine:
 
     def set_use_graph(self, on):
         self._chk(self._lib.fdtd_set_use_graph(self._h, int(bool(on))))
+
+    def set_tuning(self, variant=1, tile_y=4, chunk_x=64):
+        self._chk(self._lib.fdtd_set_tuning(self._h, int(variant), int(tile_y), int(chunk_x)))
 
     def set_kernel_timing(self, on):
         self._chk(self._lib.fdtd_set_kernel_timing(self._h, int(bool(on))))
