@@ -48,6 +48,7 @@ def kernel_bytes(eng, data):
     the coefficients (12 floats in full mode, a 2- or 4-byte index with the
     compressed table)."""
     fb = 8 if eng.precision == 1 else (4 if eng.precision == 0 else 2)
+    # coefficients are FP32 for the reduced-precision storage modes
     cb = 8 if eng.precision == 1 else 4
     field = 9 * fb
     if eng.compressed:
@@ -104,7 +105,7 @@ def main():
         sys.exit(2)
 
     from wipems_amd import scenarios
-    from wipems_amd.engine import copy_bandwidth, device_name, engine_from_operator
+    from wipems_amd.engine import copy_bandwidth, device_name, engine_from_operator, pilot_scales
 
     builder, kw, desc = WORKLOADS[args.workload]
     kw = dict(kw)
@@ -117,7 +118,12 @@ def main():
     log("workload %s: %d cells, dT=%.4e s, operator build %.1f s" % (args.workload, cells, data.dT, time.time() - t0))
 
     compressed = {"auto": None, "full": False, "compressed": True}[args.coeffs]
-    eng = engine_from_operator(data, precision=args.precision, compressed=compressed)
+    scales = None
+    if args.precision in ("fp16", "bf16"):
+        t0 = time.time()
+        scales = pilot_scales(data, compressed=compressed)
+        log("storage scales (pilot FP32 run, %.1f s): volt %s curr %s" % (time.time() - t0, scales[0], scales[1]))
+    eng = engine_from_operator(data, precision=args.precision, compressed=compressed, scales=scales)
     log("device %s, coefficients %s, device bytes %.2f GB" % (device_name(), "compressed" if eng.compressed else "full", eng.device_bytes() / 1e9))
 
     if args.profile_steps:

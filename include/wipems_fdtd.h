@@ -180,10 +180,11 @@ int  fdtd_read_probe_fd(fdtd_engine* h, int id, double* re, double* im);
 int  fdtd_iterate(fdtd_engine* h, uint32_t nsteps);
 int  fdtd_synchronize(fdtd_engine* h);
 int  fdtd_get_num_timesteps(fdtd_engine* h, uint32_t* numTS);
-/* Kernel shape: variant 0 = one thread per cell in hardware block order,
+/* Kernel shape (default 3): variant 0 = one thread per cell in hardware block order,
  * 1 = x-marching columns (chunk_x planes per block), 2 = one thread per cell
- * in XCD-banded order (chunk_x = y tiles per band); tile_y = rows per 64-lane
- * tile (1, 2 or 4).  Results do not depend on the shape, only speed does. */
+ * in XCD-banded order (chunk_x = y tiles per band), 3 = 16-byte vectors of
+ * consecutive z cells per lane; tile_y = rows per 64-lane tile (1, 2 or 4).
+ * Results do not depend on the shape, only speed does. */
 int  fdtd_set_tuning(fdtd_engine* h, int variant, int tile_y, int chunk_x);
 /* Use a captured HIP graph for the step loop (default on). */
 int  fdtd_set_use_graph(fdtd_engine* h, int enable);

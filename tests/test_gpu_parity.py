@@ -97,7 +97,7 @@ def test_graded_upml_port_fp32_compressed(gpu):
     compare(eng, orc)
 
 
-@pytest.mark.parametrize("variant,tile_y,chunk", [(0, 4, 64), (1, 1, 7), (1, 2, 16), (1, 4, 64), (1, 4, 1000), (2, 4, 1), (2, 2, 3), (2, 4, 8), (2, 1, 64)])
+@pytest.mark.parametrize("variant,tile_y,chunk", [(0, 4, 64), (1, 1, 7), (1, 2, 16), (1, 4, 64), (1, 4, 1000), (2, 4, 1), (2, 2, 3), (2, 4, 8), (2, 1, 64), (3, 1, 1), (3, 2, 1), (3, 4, 1)])
 def test_kernel_shapes_bitwise(gpu, variant, tile_y, chunk):
     """Every kernel shape gives the same bits (graded mesh, UPML, port)."""
     sc = scenarios.graded_port_scenario(n_cells=36, steps=150, prec=0, n_boxes=5, pml=8)
@@ -125,3 +125,31 @@ def test_cavity64_matches_golden_fixture(gpu):
     np.testing.assert_array_equal(u, g["u"])
     assert rel_l2(i, g["i"]) <= 1e-6
     assert eng.energy() == pytest.approx(float(g["energy"]), rel=1e-12)
+
+
+@pytest.mark.parametrize("variant", [0, 3])
+@pytest.mark.parametrize("n_cells", [21, 30, 33])
+def test_vector_kernel_odd_sizes(gpu, variant, n_cells):
+    """Vector kernels with Nz not a multiple of the vector width, compressed
+    and full coefficients, against the oracle."""
+    sc = scenarios.graded_port_scenario(n_cells=n_cells, steps=120, prec=0, n_boxes=4, pml=5)
+    d = sc.data
+    for compressed in (False, True):
+        eng = engine_from_operator(d, precision="fp32", compressed=compressed)
+        eng.set_tuning(variant, 4, 1)
+        _kinds[id(eng)] = [p.kind for p in d.probes]
+        orc = oracle_from_operator(d, ftz=True)
+        eng.iterate(120)
+        orc.iterate(120)
+        compare(eng, orc)
+
+
+def test_vector_kernel_fp64(gpu):
+    sc = scenarios.graded_port_scenario(n_cells=27, steps=100, prec=1, n_boxes=4, pml=5)
+    eng = engine_from_operator(sc.data, precision="fp64", compressed=False)
+    eng.set_tuning(3, 2, 1)
+    _kinds[id(eng)] = [p.kind for p in sc.data.probes]
+    orc = oracle_from_operator(sc.data)
+    eng.iterate(100)
+    orc.iterate(100)
+    compare(eng, orc)

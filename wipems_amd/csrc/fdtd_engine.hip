@@ -110,11 +110,17 @@ struct Scales {            // per-component storage scales (1 for FP32/FP64)
 };
 
 #define MAX_PML_SLABS 6
+// A slab's arrays are stored [3][nx][ny][nzp] with the z range widened to
+// start at za = s[2] rounded down to a multiple of 16 and padded to nzp, so
+// a 16-byte-aligned vector of consecutive z cells of the main arrays maps to
+// an aligned vector of the slab arrays.  Padding cells hold zeros.
 struct PmlSlab {
     int s[3];              // first line per axis
     int n[3];              // lines per axis
+    int za;                // aligned first z of the slab arrays
+    int nzp;               // padded z extent of the slab arrays
     long long off;         // element offset of this slab's component 0
-    long long sc;          // component stride inside the slab (nx*ny*nz)
+    long long sc;          // component stride inside the slab (nx*ny*nzp)
 };
 struct PmlSet {
     int count;
@@ -143,9 +149,27 @@ __device__ __forceinline__ int find_slab(const PmlSet& ps, int x, int y, int z, 
         const PmlSlab& b = ps.slab[s];
         int lx = x - b.s[0], ly = y - b.s[1], lz = z - b.s[2];
         if ((unsigned)lx < (unsigned)b.n[0] && (unsigned)ly < (unsigned)b.n[1] && (unsigned)lz < (unsigned)b.n[2]) {
-            local = b.off + ((long long)lx * b.n[1] + ly) * b.n[2] + lz;
+            local = b.off + ((long long)lx * b.n[1] + ly) * b.nzp + (z - b.za);
             return s;
         }
+    }
+    return -1;
+}
+
+// slab holding any of the cells z0..z0+vz-1 at (x,y); base = slab-array
+// offset of z0 (aligned like z0), mask bit k = cell z0+k lies in the slab
+__device__ __forceinline__ int find_slab_vec(const PmlSet& ps, int x, int y, int z0, int vz, long long& base, unsigned& mask) {
+    for (int s = 0; s < ps.count; ++s) {
+        const PmlSlab& b = ps.slab[s];
+        const int lx = x - b.s[0], ly = y - b.s[1];
+        if ((unsigned)lx >= (unsigned)b.n[0] || (unsigned)ly >= (unsigned)b.n[1]) continue;
+        unsigned m = 0;
+        for (int k = 0; k < vz; ++k)
+            if ((unsigned)(z0 + k - b.s[2]) < (unsigned)b.n[2]) m |= 1u << k;
+        if (!m) continue;
+        base = b.off + ((long long)lx * b.n[1] + ly) * b.nzp + (z0 - b.za);
+        mask = m;
+        return s;
     }
     return -1;
 }
@@ -301,6 +325,227 @@ k_curr(S* __restrict__ I, const S* __restrict__ V, CoeffPtrs<typename StoreTrait
     const int x = blockIdx.z;
     if (z >= d.nz - 1 || y >= d.ny - 1 || x >= d.nx - 1) return;
     curr_cell<S, CM, PML>(I, V, cf, d, sc, ps, pp, x, y, z);
+}
+
+// ---------------------------------------------------------------------------
+// vectorised updates: each lane updates VZ consecutive z cells with one wide
+// load/store per field component (16 bytes for FP32, FP16/BF16 and FP64), so
+// the number of memory instructions per cell drops VZ-fold; the z-1 (E) and
+// z+1 (H) neighbours come from the neighbouring element in the same vector,
+// and only the first/last element needs one scalar load.  Per-cell arithmetic
+// is the same as volt_cell/curr_cell.
+// ---------------------------------------------------------------------------
+template <typename S, int VZ>
+struct alignas(sizeof(S) * VZ) SVec {
+    S v[VZ];
+};
+
+template <typename S, int VZ>
+__device__ __forceinline__ void vload(const S* p, typename StoreTraits<S>::A* out, typename StoreTraits<S>::A s) {
+    const SVec<S, VZ> t = *reinterpret_cast<const SVec<S, VZ>*>(p);
+#pragma unroll
+    for (int k = 0; k < VZ; ++k) out[k] = StoreTraits<S>::load(&t.v[k], s);
+}
+
+template <typename S, int VZ>
+__device__ __forceinline__ void vstore(S* p, const typename StoreTraits<S>::A* in, typename StoreTraits<S>::A inv_s, int valid) {
+    SVec<S, VZ> t;
+#pragma unroll
+    for (int k = 0; k < VZ; ++k) StoreTraits<S>::store(&t.v[k], in[k], inv_s);
+    if (valid >= VZ) {
+        *reinterpret_cast<SVec<S, VZ>*>(p) = t;
+    } else {
+        for (int k = 0; k < valid; ++k) p[k] = t.v[k];
+    }
+}
+
+template <typename A, int CM, int VZ>
+__device__ __forceinline__ void vload_coeffs(const CoeffPtrs<A>& cf, long long p, long long sc, A a[3][VZ], A b[3][VZ]) {
+    if (CM == CF_FULL) {
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+            const SVec<A, VZ> ta = *reinterpret_cast<const SVec<A, VZ>*>(cf.a + n * sc + p);
+            const SVec<A, VZ> tb = *reinterpret_cast<const SVec<A, VZ>*>(cf.b + n * sc + p);
+#pragma unroll
+            for (int k = 0; k < VZ; ++k) { a[n][k] = ta.v[k]; b[n][k] = tb.v[k]; }
+        }
+    } else {
+        unsigned c[VZ];
+        if (CM == CF_IDX16) {
+            const SVec<unsigned short, VZ> ti = *reinterpret_cast<const SVec<unsigned short, VZ>*>((const unsigned short*)cf.idx + p);
+#pragma unroll
+            for (int k = 0; k < VZ; ++k) c[k] = ti.v[k];
+        } else {
+            const SVec<unsigned, VZ> ti = *reinterpret_cast<const SVec<unsigned, VZ>*>((const unsigned*)cf.idx + p);
+#pragma unroll
+            for (int k = 0; k < VZ; ++k) c[k] = ti.v[k];
+        }
+#pragma unroll
+        for (int k = 0; k < VZ; ++k) {
+            const A* row = cf.a + (size_t)c[k] * 8;
+            const SVec<A, 4> r0 = *reinterpret_cast<const SVec<A, 4>*>(row);
+            const SVec<A, 4> r1 = *reinterpret_cast<const SVec<A, 4>*>(row + 4);
+#pragma unroll
+            for (int n = 0; n < 3; ++n) { a[n][k] = r0.v[n]; b[n][k] = r1.v[n]; }
+        }
+    }
+}
+
+template <typename S, int CM, bool PML, int VZ>
+__global__ void __launch_bounds__(256)
+k_volt_vec(S* __restrict__ V, const S* __restrict__ I, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
+           Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp) {
+    typedef StoreTraits<S> T;
+    typedef typename T::A A;
+    // flat (y, z) index over the padded rows of one x plane
+    const long long q0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * VZ;
+    const int y = (int)(q0 / d.sy);
+    const int z0 = (int)(q0 - (long long)y * d.sy);
+    const int x = blockIdx.y;
+    if (y >= d.ny || z0 >= d.nz) return;
+    const long long p = x * d.sx + q0;
+    const long long c1 = d.sc, c2 = 2 * d.sc;
+    const long long ox = x > 0 ? d.sx : 0;
+    const long long oy = y > 0 ? d.sy : 0;
+    A i0[VZ], i1[VZ], i2[VZ], i2_ym[VZ], i0_ym[VZ], i2_xm[VZ], i1_xm[VZ];
+    vload<S, VZ>(I + p, i0, sc.c[0]);
+    vload<S, VZ>(I + c1 + p, i1, sc.c[1]);
+    vload<S, VZ>(I + c2 + p, i2, sc.c[2]);
+    vload<S, VZ>(I + c2 + p - oy, i2_ym, sc.c[2]);
+    vload<S, VZ>(I + p - oy, i0_ym, sc.c[0]);
+    vload<S, VZ>(I + c2 + p - ox, i2_xm, sc.c[2]);
+    vload<S, VZ>(I + c1 + p - ox, i1_xm, sc.c[1]);
+    const A i1_zm0 = z0 > 0 ? T::load(I + c1 + p - 1, sc.c[1]) : A(0);
+    const A i0_zm0 = z0 > 0 ? T::load(I + p - 1, sc.c[0]) : A(0);
+    A curl[3][VZ];
+#pragma unroll
+    for (int k = 0; k < VZ; ++k) {
+        const A i1_zm = k == 0 ? i1_zm0 : i1[k - 1];
+        const A i0_zm = k == 0 ? i0_zm0 : i0[k - 1];
+        curl[0][k] = ((i2[k] - i2_ym[k]) - i1[k]) + i1_zm;
+        curl[1][k] = ((i0[k] - i0_zm) - i2[k]) + i2_xm[k];
+        curl[2][k] = ((i1[k] - i1_xm[k]) - i0[k]) + i0_ym[k];
+    }
+    A vv[3][VZ], vi[3][VZ];
+    vload_coeffs<A, CM, VZ>(cf, p, d.sc, vv, vi);
+    const int valid = min(VZ, d.nz - z0);
+    long long pb = 0;
+    unsigned pm = 0;
+    const int slab = PML ? find_slab_vec(ps, x, y, z0, VZ, pb, pm) : -1;
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+        A v[VZ];
+        vload<S, VZ>(V + n * d.sc + p, v, sc.v[n]);
+        if (PML && slab >= 0) {
+            const long long q = pb + n * ps.slab[slab].sc;
+            A F[VZ], kv[VZ], kfo[VZ], kfn[VZ];
+            vload<A, VZ>(pp.vflux + q, F, A(1));
+            vload<A, VZ>(pp.vv + q, kv, A(1));
+            vload<A, VZ>(pp.vvfo + q, kfo, A(1));
+            vload<A, VZ>(pp.vvfn + q, kfn, A(1));
+#pragma unroll
+            for (int k = 0; k < VZ; ++k) {
+                if (pm & (1u << k)) {
+                    A h = kv[k] * v[k] - kfo[k] * F[k];
+                    A t = F[k];
+                    t = t * vv[n][k];
+                    t = t + vi[n][k] * curl[n][k];
+                    F[k] = t;
+                    v[k] = h + kfn[k] * t;
+                } else {
+                    v[k] = v[k] * vv[n][k];
+                    v[k] = v[k] + vi[n][k] * curl[n][k];
+                }
+            }
+            vstore<A, VZ>(pp.vflux + q, F, A(1), VZ);
+        } else {
+#pragma unroll
+            for (int k = 0; k < VZ; ++k) {
+                v[k] = v[k] * vv[n][k];
+                v[k] = v[k] + vi[n][k] * curl[n][k];
+            }
+        }
+        vstore<S, VZ>(V + n * d.sc + p, v, sc.iv[n], valid);
+    }
+}
+
+template <typename S, int CM, bool PML, int VZ>
+__global__ void __launch_bounds__(256)
+k_curr_vec(S* __restrict__ I, const S* __restrict__ V, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
+           Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp) {
+    typedef StoreTraits<S> T;
+    typedef typename T::A A;
+    const long long q0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * VZ;
+    const int y = (int)(q0 / d.sy);
+    const int z0 = (int)(q0 - (long long)y * d.sy);
+    const int x = blockIdx.y;
+    if (y >= d.ny - 1 || z0 >= d.nz - 1) return;
+    const long long p = x * d.sx + q0;
+    const long long c1 = d.sc, c2 = 2 * d.sc;
+    A v0[VZ], v1[VZ], v2[VZ], v2_yp[VZ], v0_yp[VZ], v2_xp[VZ], v1_xp[VZ];
+    vload<S, VZ>(V + p, v0, sc.v[0]);
+    vload<S, VZ>(V + c1 + p, v1, sc.v[1]);
+    vload<S, VZ>(V + c2 + p, v2, sc.v[2]);
+    vload<S, VZ>(V + c2 + p + d.sy, v2_yp, sc.v[2]);
+    vload<S, VZ>(V + p + d.sy, v0_yp, sc.v[0]);
+    vload<S, VZ>(V + c2 + p + d.sx, v2_xp, sc.v[2]);
+    vload<S, VZ>(V + c1 + p + d.sx, v1_xp, sc.v[1]);
+    // z+1 neighbour of the last element: only needed when it is a valid line
+    const bool tail = z0 + VZ < d.nz;
+    const A v1_zpl = tail ? T::load(V + c1 + p + VZ, sc.v[1]) : A(0);
+    const A v0_zpl = tail ? T::load(V + p + VZ, sc.v[0]) : A(0);
+    A curl[3][VZ];
+#pragma unroll
+    for (int k = 0; k < VZ; ++k) {
+        const A v1_zp = k == VZ - 1 ? v1_zpl : v1[k + 1];
+        const A v0_zp = k == VZ - 1 ? v0_zpl : v0[k + 1];
+        curl[0][k] = ((v2[k] - v2_yp[k]) - v1[k]) + v1_zp;
+        curl[1][k] = ((v0[k] - v0_zp) - v2[k]) + v2_xp[k];
+        curl[2][k] = ((v1[k] - v1_xp[k]) - v0[k]) + v0_yp[k];
+    }
+    A ii[3][VZ], iv[3][VZ];
+    vload_coeffs<A, CM, VZ>(cf, p, d.sc, ii, iv);
+    const int valid = min(VZ, d.nz - 1 - z0);
+    long long pb = 0;
+    unsigned pm = 0;
+    const int slab = PML ? find_slab_vec(ps, x, y, z0, VZ, pb, pm) : -1;
+    // cells at z >= nz-1 are not updated: keep them out of the slab mask too
+    if (valid < VZ) pm &= (1u << valid) - 1u;
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+        A c[VZ];
+        vload<S, VZ>(I + n * d.sc + p, c, sc.c[n]);
+        if (PML && slab >= 0) {
+            const long long q = pb + n * ps.slab[slab].sc;
+            A F[VZ], ki[VZ], kfo[VZ], kfn[VZ];
+            vload<A, VZ>(pp.cflux + q, F, A(1));
+            vload<A, VZ>(pp.ii + q, ki, A(1));
+            vload<A, VZ>(pp.iifo + q, kfo, A(1));
+            vload<A, VZ>(pp.iifn + q, kfn, A(1));
+#pragma unroll
+            for (int k = 0; k < VZ; ++k) {
+                if (pm & (1u << k)) {
+                    A h = ki[k] * c[k] - kfo[k] * F[k];
+                    A t = F[k];
+                    t = t * ii[n][k];
+                    t = t + iv[n][k] * curl[n][k];
+                    F[k] = t;
+                    c[k] = h + kfn[k] * t;
+                } else {
+                    c[k] = c[k] * ii[n][k];
+                    c[k] = c[k] + iv[n][k] * curl[n][k];
+                }
+            }
+            vstore<A, VZ>(pp.cflux + q, F, A(1), VZ);
+        } else {
+#pragma unroll
+            for (int k = 0; k < VZ; ++k) {
+                c[k] = c[k] * ii[n][k];
+                c[k] = c[k] + iv[n][k] * curl[n][k];
+            }
+        }
+        vstore<S, VZ>(I + n * d.sc + p, c, sc.ic[n], valid);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -771,8 +1016,14 @@ __global__ void k_energy_final(const double* part, int n, double* out) {
 // plain streaming copy: the calibration kernel for HBM byte counters and the
 // achievable-bandwidth reference
 __global__ void __launch_bounds__(256) k_copy(const float4* __restrict__ a, float4* __restrict__ b, long long n) {
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-        b[i] = a[i];
+    // four independent 16-byte loads in flight per lane, then four stores
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        const float4 r0 = a[i], r1 = a[i + stride], r2 = a[i + 2 * stride], r3 = a[i + 3 * stride];
+        b[i] = r0; b[i + stride] = r1; b[i + 2 * stride] = r2; b[i + 3 * stride] = r3;
+    }
+    for (; i < n; i += stride) b[i] = a[i];
 }
 
 // gather/scatter between the padded device layout and the dense host layout
@@ -904,8 +1155,8 @@ struct fdtd_engine {
     hipGraphExec_t gexec1 = nullptr;
     hipGraph_t graph1 = nullptr;
 
-    // kernel selection: 0 = one thread per cell (default), 1 = x-march
-    int variant = 0;
+    // kernel selection: 0 = one thread per cell, 1 = x-march, 2 = banded, 3 = vectors (default)
+    int variant = 3;
     int tile_y = 4;
     int chunk_x = 64;
 
@@ -1150,6 +1401,38 @@ void launch_curr_march(fdtd_engine* h, CoeffPtrs<typename StoreTraits<S>::A> cf,
     }
 }
 
+template <typename S> struct VecWidth { enum { value = 16 / sizeof(S) }; };
+
+template <typename S, bool PML>
+void launch_volt_vec(fdtd_engine* h, CoeffPtrs<typename StoreTraits<S>::A> cf, PmlPtrs<typename StoreTraits<S>::A> pp) {
+    typedef typename StoreTraits<S>::A A;
+    constexpr int VZ = VecWidth<S>::value;
+    Scales<A> sc = make_scales<A>(h);
+    const long long plane = (long long)h->N[1] * h->d.sy;
+    const dim3 b(256, 1, 1);
+    const dim3 g((unsigned)((plane / VZ + 255) / 256), h->N[0], 1);
+    switch (h->cf_mode) {
+    case CF_FULL: hipLaunchKernelGGL((k_volt_vec<S, CF_FULL, PML, VZ>), g, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, sc, h->pml, pp); break;
+    case CF_IDX16: hipLaunchKernelGGL((k_volt_vec<S, CF_IDX16, PML, VZ>), g, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, sc, h->pml, pp); break;
+    default: hipLaunchKernelGGL((k_volt_vec<S, CF_IDX32, PML, VZ>), g, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, sc, h->pml, pp); break;
+    }
+}
+
+template <typename S, bool PML>
+void launch_curr_vec(fdtd_engine* h, CoeffPtrs<typename StoreTraits<S>::A> cf, PmlPtrs<typename StoreTraits<S>::A> pp) {
+    typedef typename StoreTraits<S>::A A;
+    constexpr int VZ = VecWidth<S>::value;
+    Scales<A> sc = make_scales<A>(h);
+    const long long plane = (long long)(h->N[1] - 1) * h->d.sy;
+    const dim3 b(256, 1, 1);
+    const dim3 g((unsigned)((plane / VZ + 255) / 256), h->N[0] > 1 ? h->N[0] - 1 : 1, 1);
+    switch (h->cf_mode) {
+    case CF_FULL: hipLaunchKernelGGL((k_curr_vec<S, CF_FULL, PML, VZ>), g, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, sc, h->pml, pp); break;
+    case CF_IDX16: hipLaunchKernelGGL((k_curr_vec<S, CF_IDX16, PML, VZ>), g, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, sc, h->pml, pp); break;
+    default: hipLaunchKernelGGL((k_curr_vec<S, CF_IDX32, PML, VZ>), g, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, sc, h->pml, pp); break;
+    }
+}
+
 BandGrid make_band(const fdtd_engine* h, int ny_eff, int nx_eff) {
     BandGrid g;
     g.tiles_z = (h->N[2] + 63) / 64;
@@ -1236,7 +1519,8 @@ int launch_step_kernels(fdtd_engine* h, bool timed) {
     const size_t evi = timed ? h->ev_used : 0;
     if (timed) h->ev_used += 4;
     if (timed) HIPCHK(hipEventRecord(h->ev[evi], h->stream));
-    if (h->variant == 2) { if (pml) launch_volt_band<S, true>(h, cfe, pp); else launch_volt_band<S, false>(h, cfe, pp); }
+    if (h->variant == 3) { if (pml) launch_volt_vec<S, true>(h, cfe, pp); else launch_volt_vec<S, false>(h, cfe, pp); }
+    else if (h->variant == 2) { if (pml) launch_volt_band<S, true>(h, cfe, pp); else launch_volt_band<S, false>(h, cfe, pp); }
     else if (h->variant == 1) { if (pml) launch_volt_march<S, true>(h, cfe, pp); else launch_volt_march<S, false>(h, cfe, pp); }
     else if (pml) launch_volt_cm<S, true>(h, grd, blk, cfe, pp); else launch_volt_cm<S, false>(h, grd, blk, cfe, pp);
     if (timed) HIPCHK(hipEventRecord(h->ev[evi + 1], h->stream));
@@ -1253,7 +1537,8 @@ int launch_step_kernels(fdtd_engine* h, bool timed) {
         if ((a->flags & 2) && a->count)
             hipLaunchKernelGGL((k_ade_pre<S>), dim3((a->count + 255) / 256), dim3(256), 0, h->stream, h->I.as<S>(), ade_dev(a), h->d.sc, sc, 1);
     if (timed) HIPCHK(hipEventRecord(h->ev[evi + 2], h->stream));
-    if (h->variant == 2) { if (pml) launch_curr_band<S, true>(h, cfh, pp); else launch_curr_band<S, false>(h, cfh, pp); }
+    if (h->variant == 3) { if (pml) launch_curr_vec<S, true>(h, cfh, pp); else launch_curr_vec<S, false>(h, cfh, pp); }
+    else if (h->variant == 2) { if (pml) launch_curr_band<S, true>(h, cfh, pp); else launch_curr_band<S, false>(h, cfh, pp); }
     else if (h->variant == 1) { if (pml) launch_curr_march<S, true>(h, cfh, pp); else launch_curr_march<S, false>(h, cfh, pp); }
     else if (pml) launch_curr_cm<S, true>(h, grdc, blk, cfh, pp); else launch_curr_cm<S, false>(h, grdc, blk, cfh, pp);
     if (timed) HIPCHK(hipEventRecord(h->ev[evi + 3], h->stream));
@@ -1592,16 +1877,23 @@ int fdtd_add_upml(fdtd_engine* h, const uint32_t start[3], const uint32_t num_li
     }
     PmlSlab b{};
     for (int n = 0; n < 3; ++n) { b.s[n] = (int)start[n]; b.n[n] = (int)num_lines[n]; }
-    b.sc = (long long)num_lines[0] * num_lines[1] * num_lines[2];
+    b.za = b.s[2] / 16 * 16;
+    b.nzp = (b.s[2] + b.n[2] - b.za + 15) / 16 * 16;
+    b.sc = (long long)num_lines[0] * num_lines[1] * b.nzp;
     b.off = h->pml_total;
     const size_t cnt = (size_t)3 * b.sc;
-    std::vector<char> all((size_t)6 * cnt * h->acc);
+    const size_t dense = (size_t)3 * num_lines[0] * num_lines[1] * num_lines[2];
+    std::vector<char> all((size_t)6 * cnt * h->acc, 0);
     const void* src[6] = {vv, vvfo, vvfn, ii, iifo, iifn};
     std::vector<char> tmp;
+    const size_t a = h->acc;
     for (int k = 0; k < 6; ++k) {
-        int rc = convert_host(h, dtype, src[k], cnt, tmp);
+        int rc = convert_host(h, dtype, src[k], dense, tmp);
         if (rc) return rc;
-        memcpy(all.data() + (size_t)k * cnt * h->acc, tmp.data(), cnt * h->acc);
+        // dense [3][nx][ny][nz] -> padded [3][nx][ny][nzp] at z offset s[2]-za
+        const size_t rows = (size_t)3 * num_lines[0] * num_lines[1];
+        for (size_t r = 0; r < rows; ++r)
+            memcpy(all.data() + ((size_t)k * cnt + r * b.nzp + (b.s[2] - b.za)) * a, tmp.data() + r * num_lines[2] * a, num_lines[2] * a);
     }
     // offsets are in elements per array; each array covers all slabs, comp-major per slab
     h->pml.slab[h->pml.count++] = b;
@@ -1812,7 +2104,7 @@ int fdtd_read_probe_fd(fdtd_engine* h, int id, double* re, double* im) {
 
 int fdtd_set_tuning(fdtd_engine* h, int variant, int tile_y, int chunk_x) {
     if (!h) return fail(FDTD_ERR_ARG, "null handle");
-    if (variant < 0 || variant > 2) return fail(FDTD_ERR_ARG, "unknown kernel variant %d", variant);
+    if (variant < 0 || variant > 3) return fail(FDTD_ERR_ARG, "unknown kernel variant %d", variant);
     if (tile_y != 1 && tile_y != 2 && tile_y != 4) return fail(FDTD_ERR_ARG, "tile_y must be 1, 2 or 4");
     if (chunk_x < 1) return fail(FDTD_ERR_ARG, "chunk_x must be >= 1");
     h->variant = variant;

@@ -170,7 +170,7 @@ class Engine:
     def set_use_graph(self, on):
         self._chk(self._lib.fdtd_set_use_graph(self._h, int(bool(on))))
 
-    def set_tuning(self, variant=1, tile_y=4, chunk_x=64):
+    def set_tuning(self, variant=3, tile_y=4, chunk_x=1):
         self._chk(self._lib.fdtd_set_tuning(self._h, int(variant), int(tile_y), int(chunk_x)))
 
     def set_kernel_timing(self, on):
@@ -233,6 +233,35 @@ class Engine:
         b = c_uint64()
         self._chk(self._lib.fdtd_device_bytes(self._h, byref(b)))
         return b.value
+
+
+def pilot_scales(data: OperatorData, steps=None, headroom=1024.0, compressed=None):
+    """Per-component power-of-two storage scales for FP16/BF16 handles.
+
+    Runs the FP32 engine for `steps` (default: twice the excitation length)
+    and records max |V_n| and |I_n|; the scale of a component is the power of
+    two nearest to max/headroom, so stored values peak near `headroom`
+    (FP16 max 65504) and scaling itself is exact."""
+    if steps is None:
+        steps = max(64, 2 * (data.excitation.length if data.excitation is not None else 64))
+    eng = engine_from_operator(data, precision="fp32", compressed=compressed)
+    peak_v = np.zeros(3)
+    peak_i = np.zeros(3)
+    chunk = max(1, steps // 8)
+    done = 0
+    while done < steps:
+        eng.iterate(chunk)
+        done += chunk
+        V = eng.get_field(VOLT, np.float32)
+        I = eng.get_field(CURR, np.float32)
+        peak_v = np.maximum(peak_v, np.abs(V).reshape(3, -1).max(axis=1))
+        peak_i = np.maximum(peak_i, np.abs(I).reshape(3, -1).max(axis=1))
+    eng.close()
+    top_v = max(peak_v.max(), 1e-30)
+    top_i = max(peak_i.max(), 1e-30)
+    sv = [2.0 ** round(np.log2(max(p, top_v * 1e-3) / headroom)) for p in peak_v]
+    si = [2.0 ** round(np.log2(max(p, top_i * 1e-3) / headroom)) for p in peak_i]
+    return sv, si
 
 
 def engine_from_operator(data: OperatorData, precision="fp32", compressed=None, max_entries=65536, scales=None):

@@ -125,7 +125,7 @@ def graded_port_scenario(n_cells=256, steps=10000, prec=0, seed=SEED_BASE + 2, n
     lines = [graded_lines(rng, n_cells) for _ in range(3)]
     op = Operator()
     op.set_mesh(lines, 1e-3)
-    op.set_boundary([BC_PML] * 6, [pml] * 6)
+    op.set_boundary([BC_PML if pml > 0 else BC_PEC] * 6, [pml] * 6)
     op.set_gauss_excite(f0, fc)
     boxes = random_boxes(rng, op, lines, n_boxes, pml, eps_range=eps_range, kappa_log=kappa_log)
     c = n_cells // 2
