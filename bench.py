@@ -39,6 +39,9 @@ WORKLOADS = {
                     "512^3 lines uniform 1 mm vacuum, PEC, soft Ez Gaussian at centre, 1000 steps"),
     "cavity_64": ("cavity_point_source", {"n_cells": 64, "steps": 2000},
                   "64^3 cells (65^3 lines) 1 mm vacuum, PEC, soft Ez Gaussian, 2000 steps"),
+    "batched_64": ("batched_port_scenarios", {"count": 64, "steps": 5000},
+                   "64 independent domains, per-axis cells U{48..160} at 1 mm, UPML-8, 8 eps boxes, 50 ohm port (3/3 GHz), "
+                   "5000 steps, one launch per update phase for all 64 (graph-captured), vs 64 sequential single-domain runs"),
 }
 
 
@@ -83,6 +86,93 @@ def cpu_baseline_leg(data, interval, cells, threads, steps):
             "sample": "%d steps of the same workload (restated Engine_SSE_Compressed + Engine_Multithread, %d threads, %.1f s)" % (steps, b.nthreads, dt)}
 
 
+def run_batched(args, desc):
+    from wipems_amd import scenarios
+    from wipems_amd.autotune import tune_batch
+    from wipems_amd.engine import Batch, copy_bandwidth, device_name, engine_from_operator
+
+    steps = args.steps if args.steps is not None else 5000
+    t0 = time.time()
+    scs = scenarios.batched_port_scenarios(count=64, steps=steps)
+    datas = [sc.data for sc in scs]
+    cells = sum(d.cells for d in datas)
+    log("workload batched_64: %d domains, %d cells in total, operator builds %.1f s" % (len(datas), cells, time.time() - t0))
+    best, trials = tune_batch(datas, precision=args.precision)
+    log("autotuned batch block size: %d threads (ms/step %s)" % (best, {k: round(v, 4) for k, v in trials.items()}))
+    engines = [engine_from_operator(d, precision=args.precision) for d in datas]
+    b = Batch(engines)
+    b.set_block(best)
+    b.iterate(args.warmup)
+    b.synchronize()
+    t1 = time.perf_counter()
+    b.iterate(steps)
+    b.synchronize()
+    wall = time.perf_counter() - t1
+    value = cells * steps / wall / 1e6
+    log("batched: %d steps in %.3f s -> %.1f MCells/s (%.3f ms/step), device %s" % (steps, wall, value, 1e3 * wall / steps, device_name()))
+    roofline = None
+    if not args.no_timing_pass:
+        b.set_kernel_timing(True)
+        nt = min(steps, 1000)
+        b.iterate(nt)
+        tv, tc, n = b.kernel_times()
+        b.set_kernel_timing(False)
+        avg_v, avg_c = tv / n, tc / n
+        # algorithmic bytes of one batched volt launch: per domain, its cells
+        # times its per-cell bytes (compressed index or full coefficients)
+        alg = 0.0
+        for e, d in zip(engines, datas):
+            alg += kernel_bytes(e, d) * d.cells
+        achieved = alg / (avg_v * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic("volt_batch", "batched_64"),
+                    "kernel": "volt_batch", "avg_ms": round(avg_v, 5), "curr_avg_ms": round(avg_c, 5)}
+        log("batched kernel avg: volt %.4f ms, curr %.4f ms -> %.1f GB/s" % (avg_v, avg_c, achieved))
+    b.close()
+    for e in engines:
+        e.close()
+    # the same 64 domains run one after another as plain single-domain engines
+    seq_steps = min(steps, 500)
+    seq_time = 0.0
+    for d in datas:
+        e = engine_from_operator(d, precision=args.precision)
+        e.iterate(5)
+        e.synchronize()
+        ts = time.perf_counter()
+        e.iterate(seq_steps)
+        e.synchronize()
+        seq_time += time.perf_counter() - ts
+        e.close()
+    seq_value = cells * seq_steps / seq_time / 1e6
+    log("sequential single-domain runs: %.1f MCells/s (%d steps each) -> batched speedup %.2fx" % (seq_value, seq_steps, value / seq_value))
+    cpu = None
+    cpu_steps = args.cpu_steps if args.cpu_steps is not None else 20
+    if cpu_steps > 0 and args.precision == "fp32":
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        from oracle.cpu_baseline import CpuBaseline
+        tot, tcells = 0.0, 0
+        for d, sc in list(zip(datas, scs))[:8]:
+            cb = CpuBaseline(d, threads)
+            tot += cb.run(cpu_steps, sc.meta["interval"])
+            tcells += d.cells
+        cpu = {"value": round(tcells * cpu_steps / tot / 1e6, 2), "unit": "MCells/s", "cores": threads, "kind": "port",
+               "sample": "domains 0-7 of the batch, %d steps each, sequentially (restated Engine_SSE_Compressed + Engine_Multithread)" % cpu_steps}
+        log("cpu baseline: %.1f MCells/s" % cpu["value"])
+    copy_gbs = copy_bandwidth(1 << 30, 10)
+    if roofline is not None:
+        roofline["copy_kernel_GBs"] = round(copy_gbs, 1)
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "MCells/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * wall / steps, 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": {"fp32": "f32", "fp64": "f64", "fp16": "f16-store/f32", "bf16": "bf16-store/f32"}[args.precision],
+        "data": "synthetic (seeded geometry)",
+        "config": {"workload": "batched_64", "description": desc, "cells": cells, "domains": len(datas), "block": best,
+                   "sequential_value": round(seq_value, 2), "batched_speedup": round(value / seq_value, 3), "precision": args.precision},
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,6 +198,8 @@ def main():
     from wipems_amd.engine import copy_bandwidth, device_name, engine_from_operator, pilot_scales
 
     builder, kw, desc = WORKLOADS[args.workload]
+    if args.workload == "batched_64":
+        return run_batched(args, desc)
     kw = dict(kw)
     steps = args.steps if args.steps is not None else kw["steps"]
     kw["steps"] = max(steps, kw["steps"])

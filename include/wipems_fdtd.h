@@ -208,6 +208,26 @@ int  fdtd_set_value(fdtd_engine* h, int which, uint32_t n, uint32_t x, uint32_t 
  * calibration kernel (k_copy) for HBM byte counters. */
 int  fdtd_copy_bench(uint64_t bytes, int reps, double* gbps);
 
+/* ---- batches of independent domains ----
+ * A batch groups engine handles configured with the calls above (same
+ * storage precision; operator, excitation, UPML and probes; all at the same
+ * timestep) so that one launch per update phase advances every domain.
+ * The handles stay owned by the caller and must outlive the batch; results
+ * are read with the per-handle calls after fdtd_batch_synchronize.  While a
+ * handle belongs to a batch it must not be stepped with fdtd_iterate.
+ * No counterpart in the reference, which runs one simulation per process;
+ * this is the batched-sweep pattern (parameter sweeps, per-port excitations). */
+typedef struct fdtd_batch fdtd_batch;
+int  fdtd_batch_create(fdtd_engine* const* engines, int count, fdtd_batch** out);
+int  fdtd_batch_destroy(fdtd_batch* b);
+/* threads per block of the batched update kernels: 64, 128 or 256 (tuning) */
+int  fdtd_batch_set_block(fdtd_batch* b, int threads);
+int  fdtd_batch_set_use_graph(fdtd_batch* b, int enable);
+int  fdtd_batch_iterate(fdtd_batch* b, uint32_t nsteps);
+int  fdtd_batch_synchronize(fdtd_batch* b);
+int  fdtd_batch_set_kernel_timing(fdtd_batch* b, int enable);
+int  fdtd_batch_get_kernel_times(fdtd_batch* b, double* volt_ms, double* curr_ms, uint32_t* launches);
+
 /* Device bytes held by the handle (fields, coefficients, extensions). */
 int  fdtd_device_bytes(fdtd_engine* h, uint64_t* bytes);
 

@@ -64,6 +64,14 @@ _ENGINE_SIGS = {
     "fdtd_set_value": [c_void_p, c_int, c_uint32, c_uint32, c_uint32, c_uint32, c_double],
     "fdtd_device_bytes": [c_void_p, POINTER(c_uint64)],
     "fdtd_copy_bench": [c_uint64, c_int, POINTER(c_double)],
+    "fdtd_batch_create": [POINTER(c_void_p), c_int, POINTER(c_void_p)],
+    "fdtd_batch_destroy": [c_void_p],
+    "fdtd_batch_set_block": [c_void_p, c_int],
+    "fdtd_batch_set_use_graph": [c_void_p, c_int],
+    "fdtd_batch_iterate": [c_void_p, c_uint32],
+    "fdtd_batch_synchronize": [c_void_p],
+    "fdtd_batch_set_kernel_timing": [c_void_p, c_int],
+    "fdtd_batch_get_kernel_times": [c_void_p, POINTER(c_double), POINTER(c_double), POINTER(c_uint32)],
 }
 
 _OPERATOR_SIGS = {

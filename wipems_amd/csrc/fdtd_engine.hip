@@ -25,979 +25,9 @@
 // with contraction disabled (no FMA) and, for FP32, denormals flushed (the
 // SSE engines set FTZ|DAZ, tools/denormal.h:19-31), so an FP32 handle
 // reproduces the reference FP32 engine bit for bit.
-#include <hip/hip_runtime.h>
-#include <hip/hip_fp16.h>
-#include <hip/hip_bf16.h>
+#include "fdtd_internal.h"
 
-#include <cmath>
-#include <cstdarg>
-#include <cstdio>
-#include <cstring>
-#include <string>
-#include <vector>
-#include <algorithm>
-
-#include "wipems_fdtd.h"
-
-#pragma clang fp contract(off)
-
-#define FDTD_VERSION_STR "wipems-fdtd 0.1 (gfx950)"
-
-static const double kEPS0 = 8.85418781762e-12;   // tools/constants.h:22
-static const double kMUE0 = 1.256637062e-6;      // tools/constants.h:23
-
-// ---------------------------------------------------------------------------
-// error handling
-// ---------------------------------------------------------------------------
-static thread_local std::string g_last_error;
-
-static int fail(int code, const char* fmt, ...) {
-    char buf[1024];
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(buf, sizeof(buf), fmt, ap);
-    va_end(ap);
-    g_last_error = buf;
-    return code;
-}
-
-#define HIPCHK(expr)                                                                 \
-    do {                                                                             \
-        hipError_t e_ = (expr);                                                      \
-        if (e_ != hipSuccess)                                                        \
-            return fail(FDTD_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
-    } while (0)
-
-// ---------------------------------------------------------------------------
-// storage types: FP32/FP64 store the arithmetic type; FP16/BF16 store a
-// reduced type and compute in float with per-component scales.
-// ---------------------------------------------------------------------------
-template <typename S> struct StoreTraits;
-template <> struct StoreTraits<float> {
-    typedef float A;
-    __device__ static inline float load(const float* p, float) { return *p; }
-    __device__ static inline void store(float* p, float v, float) { *p = v; }
-};
-template <> struct StoreTraits<double> {
-    typedef double A;
-    __device__ static inline double load(const double* p, double) { return *p; }
-    __device__ static inline void store(double* p, double v, double) { *p = v; }
-};
-template <> struct StoreTraits<__half> {
-    typedef float A;
-    __device__ static inline float load(const __half* p, float s) { return __half2float(*p) * s; }
-    __device__ static inline void store(__half* p, float v, float inv_s) { *p = __float2half_rn(v * inv_s); }
-};
-template <> struct StoreTraits<__hip_bfloat16> {
-    typedef float A;
-    __device__ static inline float load(const __hip_bfloat16* p, float s) { return __bfloat162float(*p) * s; }
-    __device__ static inline void store(__hip_bfloat16* p, float v, float inv_s) { *p = __float2bfloat16(v * inv_s); }
-};
-
-// ---------------------------------------------------------------------------
-// kernel argument blocks
-// ---------------------------------------------------------------------------
-struct Dims {
-    int nx, ny, nz;        // mesh lines
-    long long sx, sy;      // element strides of x and y (z stride is 1)
-    long long sc;          // component stride
-};
-
-template <typename A>
-struct Scales {            // per-component storage scales (1 for FP32/FP64)
-    A v[3], iv[3];         // scale and its inverse, voltage
-    A c[3], ic[3];         // scale and its inverse, current
-};
-
-#define MAX_PML_SLABS 6
-// A slab's arrays are stored [3][nx][ny][nzp] with the z range widened to
-// start at za = s[2] rounded down to a multiple of 16 and padded to nzp, so
-// a 16-byte-aligned vector of consecutive z cells of the main arrays maps to
-// an aligned vector of the slab arrays.  Padding cells hold zeros.
-struct PmlSlab {
-    int s[3];              // first line per axis
-    int n[3];              // lines per axis
-    int za;                // aligned first z of the slab arrays
-    int nzp;               // padded z extent of the slab arrays
-    long long off;         // element offset of this slab's component 0
-    long long sc;          // component stride inside the slab (nx*ny*nzp)
-};
-struct PmlSet {
-    int count;
-    PmlSlab slab[MAX_PML_SLABS];
-};
-
-template <typename A>
-struct PmlPtrs {
-    const A* vv; const A* vvfo; const A* vvfn;   // voltage side
-    const A* ii; const A* iifo; const A* iifn;   // current side
-    A* vflux; A* cflux;
-};
-
-// coefficient access modes
-enum { CF_FULL = 0, CF_IDX16 = 1, CF_IDX32 = 2 };
-
-template <typename A>
-struct CoeffPtrs {
-    const A* a;            // full: vv (E) or ii (H) [3][cells]; table: [U][8] rows {a0,a1,a2,_,b0,b1,b2,_}
-    const A* b;            // full: vi (E) or iv (H)
-    const void* idx;       // per-cell index (compressed modes)
-};
-
-__device__ __forceinline__ int find_slab(const PmlSet& ps, int x, int y, int z, long long& local) {
-    for (int s = 0; s < ps.count; ++s) {
-        const PmlSlab& b = ps.slab[s];
-        int lx = x - b.s[0], ly = y - b.s[1], lz = z - b.s[2];
-        if ((unsigned)lx < (unsigned)b.n[0] && (unsigned)ly < (unsigned)b.n[1] && (unsigned)lz < (unsigned)b.n[2]) {
-            local = b.off + ((long long)lx * b.n[1] + ly) * b.nzp + (z - b.za);
-            return s;
-        }
-    }
-    return -1;
-}
-
-// slab holding any of the cells z0..z0+vz-1 at (x,y); base = slab-array
-// offset of z0 (aligned like z0), mask bit k = cell z0+k lies in the slab
-__device__ __forceinline__ int find_slab_vec(const PmlSet& ps, int x, int y, int z0, int vz, long long& base, unsigned& mask) {
-    for (int s = 0; s < ps.count; ++s) {
-        const PmlSlab& b = ps.slab[s];
-        const int lx = x - b.s[0], ly = y - b.s[1];
-        if ((unsigned)lx >= (unsigned)b.n[0] || (unsigned)ly >= (unsigned)b.n[1]) continue;
-        unsigned m = 0;
-        for (int k = 0; k < vz; ++k)
-            if ((unsigned)(z0 + k - b.s[2]) < (unsigned)b.n[2]) m |= 1u << k;
-        if (!m) continue;
-        base = b.off + ((long long)lx * b.n[1] + ly) * b.nzp + (z0 - b.za);
-        mask = m;
-        return s;
-    }
-    return -1;
-}
-
-template <typename A, int CM>
-__device__ __forceinline__ void load_coeffs(const CoeffPtrs<A>& cf, long long p, long long sc, A a[3], A b[3]) {
-    if (CM == CF_FULL) {
-#pragma unroll
-        for (int n = 0; n < 3; ++n) { a[n] = cf.a[n * sc + p]; b[n] = cf.b[n * sc + p]; }
-    } else {
-        unsigned c = (CM == CF_IDX16) ? (unsigned)((const unsigned short*)cf.idx)[p]
-                                      : ((const unsigned*)cf.idx)[p];
-        const A* row = cf.a + (size_t)c * 8;
-#pragma unroll
-        for (int n = 0; n < 3; ++n) { a[n] = row[n]; b[n] = row[4 + n]; }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// main voltage update (Engine::UpdateVoltages, FDTD/engine.cpp:108-161, with the
-// z-boundary of Engine_SSE_Compressed, FDTD/engine_sse_compressed.cpp:95-153:
-// the z-1 neighbour at z=0 is a zero-filled lane) with UPML Pre/Post fused
-// (FDTD/extensions/engine_ext_upml.cpp:57-135):
-//   pre : h = pv*V - fo*F ; V = F ; F = h
-//   main: V = V*vv + vi*curl
-//   post: h = F ; F = V ; V = h + fn*F
-// ---------------------------------------------------------------------------
-template <typename S, int CM, bool PML>
-__device__ __forceinline__ void volt_cell(S* __restrict__ V, const S* __restrict__ I, const CoeffPtrs<typename StoreTraits<S>::A>& cf,
-                                          const Dims& d, const Scales<typename StoreTraits<S>::A>& sc, const PmlSet& ps,
-                                          const PmlPtrs<typename StoreTraits<S>::A>& pp, int x, int y, int z) {
-    typedef StoreTraits<S> T;
-    typedef typename T::A A;
-    const long long p = x * d.sx + y * d.sy + z;
-    const long long c1 = d.sc, c2 = 2 * d.sc;
-    const long long ox = x > 0 ? d.sx : 0;   // x-1 neighbour, self at x=0 (shift[0])
-    const long long oy = y > 0 ? d.sy : 0;   // y-1 neighbour, self at y=0 (shift[1])
-
-    const A i0 = T::load(I + p, sc.c[0]);
-    const A i1 = T::load(I + c1 + p, sc.c[1]);
-    const A i2 = T::load(I + c2 + p, sc.c[2]);
-    const A i2_ym = T::load(I + c2 + p - oy, sc.c[2]);
-    const A i0_ym = T::load(I + p - oy, sc.c[0]);
-    const A i2_xm = T::load(I + c2 + p - ox, sc.c[2]);
-    const A i1_xm = T::load(I + c1 + p - ox, sc.c[1]);
-    const A i1_zm = z > 0 ? T::load(I + c1 + p - 1, sc.c[1]) : A(0);
-    const A i0_zm = z > 0 ? T::load(I + p - 1, sc.c[0]) : A(0);
-
-    A curl[3];
-    curl[0] = ((i2 - i2_ym) - i1) + i1_zm;
-    curl[1] = ((i0 - i0_zm) - i2) + i2_xm;
-    curl[2] = ((i1 - i1_xm) - i0) + i0_ym;
-
-    A vv[3], vi[3];
-    load_coeffs<A, CM>(cf, p, d.sc, vv, vi);
-
-    long long lp = 0;
-    int slab = PML ? find_slab(ps, x, y, z, lp) : -1;
-#pragma unroll
-    for (int n = 0; n < 3; ++n) {
-        S* vp = V + n * d.sc + p;
-        A v = T::load(vp, sc.v[n]);
-        if (PML && slab >= 0) {
-            const PmlSlab& b = ps.slab[slab];
-            const long long q = lp + n * b.sc;
-            A F = pp.vflux[q];
-            A h = pp.vv[q] * v - pp.vvfo[q] * F;
-            A t = F;
-            t = t * vv[n];
-            t = t + vi[n] * curl[n];
-            pp.vflux[q] = t;
-            v = h + pp.vvfn[q] * t;
-        } else {
-            v = v * vv[n];
-            v = v + vi[n] * curl[n];
-        }
-        T::store(vp, v, sc.iv[n]);
-    }
-}
-
-template <typename S, int CM, bool PML>
-__global__ void __launch_bounds__(256)
-k_volt(S* __restrict__ V, const S* __restrict__ I, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
-       Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp) {
-    const int z = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    const int x = blockIdx.z;
-    if (z >= d.nz || y >= d.ny) return;
-    volt_cell<S, CM, PML>(V, I, cf, d, sc, ps, pp, x, y, z);
-}
-
-// ---------------------------------------------------------------------------
-// main current update (Engine::UpdateCurrents, FDTD/engine.cpp:163-210) over
-// x<Nx-1, y<Ny-1, z<Nz-1 with UPML fused (engine_ext_upml.cpp:141-234).
-// ---------------------------------------------------------------------------
-template <typename S, int CM, bool PML>
-__device__ __forceinline__ void curr_cell(S* __restrict__ I, const S* __restrict__ V, const CoeffPtrs<typename StoreTraits<S>::A>& cf,
-                                          const Dims& d, const Scales<typename StoreTraits<S>::A>& sc, const PmlSet& ps,
-                                          const PmlPtrs<typename StoreTraits<S>::A>& pp, int x, int y, int z) {
-    typedef StoreTraits<S> T;
-    typedef typename T::A A;
-    const long long p = x * d.sx + y * d.sy + z;
-    const long long c1 = d.sc, c2 = 2 * d.sc;
-
-    const A v0 = T::load(V + p, sc.v[0]);
-    const A v1 = T::load(V + c1 + p, sc.v[1]);
-    const A v2 = T::load(V + c2 + p, sc.v[2]);
-    const A v2_yp = T::load(V + c2 + p + d.sy, sc.v[2]);
-    const A v0_yp = T::load(V + p + d.sy, sc.v[0]);
-    const A v2_xp = T::load(V + c2 + p + d.sx, sc.v[2]);
-    const A v1_xp = T::load(V + c1 + p + d.sx, sc.v[1]);
-    const A v1_zp = T::load(V + c1 + p + 1, sc.v[1]);
-    const A v0_zp = T::load(V + p + 1, sc.v[0]);
-
-    A curl[3];
-    curl[0] = ((v2 - v2_yp) - v1) + v1_zp;
-    curl[1] = ((v0 - v0_zp) - v2) + v2_xp;
-    curl[2] = ((v1 - v1_xp) - v0) + v0_yp;
-
-    A ii[3], iv[3];
-    load_coeffs<A, CM>(cf, p, d.sc, ii, iv);
-
-    long long lp = 0;
-    int slab = PML ? find_slab(ps, x, y, z, lp) : -1;
-#pragma unroll
-    for (int n = 0; n < 3; ++n) {
-        S* ip = I + n * d.sc + p;
-        A c = T::load(ip, sc.c[n]);
-        if (PML && slab >= 0) {
-            const PmlSlab& b = ps.slab[slab];
-            const long long q = lp + n * b.sc;
-            A F = pp.cflux[q];
-            A h = pp.ii[q] * c - pp.iifo[q] * F;
-            A t = F;
-            t = t * ii[n];
-            t = t + iv[n] * curl[n];
-            pp.cflux[q] = t;
-            c = h + pp.iifn[q] * t;
-        } else {
-            c = c * ii[n];
-            c = c + iv[n] * curl[n];
-        }
-        T::store(ip, c, sc.ic[n]);
-    }
-}
-
-template <typename S, int CM, bool PML>
-__global__ void __launch_bounds__(256)
-k_curr(S* __restrict__ I, const S* __restrict__ V, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
-       Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp) {
-    const int z = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    const int x = blockIdx.z;
-    if (z >= d.nz - 1 || y >= d.ny - 1 || x >= d.nx - 1) return;
-    curr_cell<S, CM, PML>(I, V, cf, d, sc, ps, pp, x, y, z);
-}
-
-// ---------------------------------------------------------------------------
-// vectorised updates: each lane updates VZ consecutive z cells with one wide
-// load/store per field component (16 bytes for FP32, FP16/BF16 and FP64), so
-// the number of memory instructions per cell drops VZ-fold; the z-1 (E) and
-// z+1 (H) neighbours come from the neighbouring element in the same vector,
-// and only the first/last element needs one scalar load.  Per-cell arithmetic
-// is the same as volt_cell/curr_cell.
-// ---------------------------------------------------------------------------
-template <typename S, int VZ>
-struct alignas(sizeof(S) * VZ) SVec {
-    S v[VZ];
-};
-
-template <typename S, int VZ>
-__device__ __forceinline__ void vload(const S* p, typename StoreTraits<S>::A* out, typename StoreTraits<S>::A s) {
-    const SVec<S, VZ> t = *reinterpret_cast<const SVec<S, VZ>*>(p);
-#pragma unroll
-    for (int k = 0; k < VZ; ++k) out[k] = StoreTraits<S>::load(&t.v[k], s);
-}
-
-template <typename S, int VZ>
-__device__ __forceinline__ void vstore(S* p, const typename StoreTraits<S>::A* in, typename StoreTraits<S>::A inv_s, int valid) {
-    SVec<S, VZ> t;
-#pragma unroll
-    for (int k = 0; k < VZ; ++k) StoreTraits<S>::store(&t.v[k], in[k], inv_s);
-    if (valid >= VZ) {
-        *reinterpret_cast<SVec<S, VZ>*>(p) = t;
-    } else {
-        for (int k = 0; k < valid; ++k) p[k] = t.v[k];
-    }
-}
-
-template <typename A, int CM, int VZ>
-__device__ __forceinline__ void vload_coeffs(const CoeffPtrs<A>& cf, long long p, long long sc, A a[3][VZ], A b[3][VZ]) {
-    if (CM == CF_FULL) {
-#pragma unroll
-        for (int n = 0; n < 3; ++n) {
-            const SVec<A, VZ> ta = *reinterpret_cast<const SVec<A, VZ>*>(cf.a + n * sc + p);
-            const SVec<A, VZ> tb = *reinterpret_cast<const SVec<A, VZ>*>(cf.b + n * sc + p);
-#pragma unroll
-            for (int k = 0; k < VZ; ++k) { a[n][k] = ta.v[k]; b[n][k] = tb.v[k]; }
-        }
-    } else {
-        unsigned c[VZ];
-        if (CM == CF_IDX16) {
-            const SVec<unsigned short, VZ> ti = *reinterpret_cast<const SVec<unsigned short, VZ>*>((const unsigned short*)cf.idx + p);
-#pragma unroll
-            for (int k = 0; k < VZ; ++k) c[k] = ti.v[k];
-        } else {
-            const SVec<unsigned, VZ> ti = *reinterpret_cast<const SVec<unsigned, VZ>*>((const unsigned*)cf.idx + p);
-#pragma unroll
-            for (int k = 0; k < VZ; ++k) c[k] = ti.v[k];
-        }
-#pragma unroll
-        for (int k = 0; k < VZ; ++k) {
-            const A* row = cf.a + (size_t)c[k] * 8;
-            const SVec<A, 4> r0 = *reinterpret_cast<const SVec<A, 4>*>(row);
-            const SVec<A, 4> r1 = *reinterpret_cast<const SVec<A, 4>*>(row + 4);
-#pragma unroll
-            for (int n = 0; n < 3; ++n) { a[n][k] = r0.v[n]; b[n][k] = r1.v[n]; }
-        }
-    }
-}
-
-template <typename S, int CM, bool PML, int VZ>
-__global__ void __launch_bounds__(256)
-k_volt_vec(S* __restrict__ V, const S* __restrict__ I, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
-           Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp) {
-    typedef StoreTraits<S> T;
-    typedef typename T::A A;
-    // flat (y, z) index over the padded rows of one x plane
-    const long long q0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * VZ;
-    const int y = (int)(q0 / d.sy);
-    const int z0 = (int)(q0 - (long long)y * d.sy);
-    const int x = blockIdx.y;
-    if (y >= d.ny || z0 >= d.nz) return;
-    const long long p = x * d.sx + q0;
-    const long long c1 = d.sc, c2 = 2 * d.sc;
-    const long long ox = x > 0 ? d.sx : 0;
-    const long long oy = y > 0 ? d.sy : 0;
-    A i0[VZ], i1[VZ], i2[VZ], i2_ym[VZ], i0_ym[VZ], i2_xm[VZ], i1_xm[VZ];
-    vload<S, VZ>(I + p, i0, sc.c[0]);
-    vload<S, VZ>(I + c1 + p, i1, sc.c[1]);
-    vload<S, VZ>(I + c2 + p, i2, sc.c[2]);
-    vload<S, VZ>(I + c2 + p - oy, i2_ym, sc.c[2]);
-    vload<S, VZ>(I + p - oy, i0_ym, sc.c[0]);
-    vload<S, VZ>(I + c2 + p - ox, i2_xm, sc.c[2]);
-    vload<S, VZ>(I + c1 + p - ox, i1_xm, sc.c[1]);
-    const A i1_zm0 = z0 > 0 ? T::load(I + c1 + p - 1, sc.c[1]) : A(0);
-    const A i0_zm0 = z0 > 0 ? T::load(I + p - 1, sc.c[0]) : A(0);
-    A curl[3][VZ];
-#pragma unroll
-    for (int k = 0; k < VZ; ++k) {
-        const A i1_zm = k == 0 ? i1_zm0 : i1[k - 1];
-        const A i0_zm = k == 0 ? i0_zm0 : i0[k - 1];
-        curl[0][k] = ((i2[k] - i2_ym[k]) - i1[k]) + i1_zm;
-        curl[1][k] = ((i0[k] - i0_zm) - i2[k]) + i2_xm[k];
-        curl[2][k] = ((i1[k] - i1_xm[k]) - i0[k]) + i0_ym[k];
-    }
-    A vv[3][VZ], vi[3][VZ];
-    vload_coeffs<A, CM, VZ>(cf, p, d.sc, vv, vi);
-    const int valid = min(VZ, d.nz - z0);
-    long long pb = 0;
-    unsigned pm = 0;
-    const int slab = PML ? find_slab_vec(ps, x, y, z0, VZ, pb, pm) : -1;
-#pragma unroll
-    for (int n = 0; n < 3; ++n) {
-        A v[VZ];
-        vload<S, VZ>(V + n * d.sc + p, v, sc.v[n]);
-        if (PML && slab >= 0) {
-            const long long q = pb + n * ps.slab[slab].sc;
-            A F[VZ], kv[VZ], kfo[VZ], kfn[VZ];
-            vload<A, VZ>(pp.vflux + q, F, A(1));
-            vload<A, VZ>(pp.vv + q, kv, A(1));
-            vload<A, VZ>(pp.vvfo + q, kfo, A(1));
-            vload<A, VZ>(pp.vvfn + q, kfn, A(1));
-#pragma unroll
-            for (int k = 0; k < VZ; ++k) {
-                if (pm & (1u << k)) {
-                    A h = kv[k] * v[k] - kfo[k] * F[k];
-                    A t = F[k];
-                    t = t * vv[n][k];
-                    t = t + vi[n][k] * curl[n][k];
-                    F[k] = t;
-                    v[k] = h + kfn[k] * t;
-                } else {
-                    v[k] = v[k] * vv[n][k];
-                    v[k] = v[k] + vi[n][k] * curl[n][k];
-                }
-            }
-            vstore<A, VZ>(pp.vflux + q, F, A(1), VZ);
-        } else {
-#pragma unroll
-            for (int k = 0; k < VZ; ++k) {
-                v[k] = v[k] * vv[n][k];
-                v[k] = v[k] + vi[n][k] * curl[n][k];
-            }
-        }
-        vstore<S, VZ>(V + n * d.sc + p, v, sc.iv[n], valid);
-    }
-}
-
-template <typename S, int CM, bool PML, int VZ>
-__global__ void __launch_bounds__(256)
-k_curr_vec(S* __restrict__ I, const S* __restrict__ V, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
-           Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp) {
-    typedef StoreTraits<S> T;
-    typedef typename T::A A;
-    const long long q0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * VZ;
-    const int y = (int)(q0 / d.sy);
-    const int z0 = (int)(q0 - (long long)y * d.sy);
-    const int x = blockIdx.y;
-    if (y >= d.ny - 1 || z0 >= d.nz - 1) return;
-    const long long p = x * d.sx + q0;
-    const long long c1 = d.sc, c2 = 2 * d.sc;
-    A v0[VZ], v1[VZ], v2[VZ], v2_yp[VZ], v0_yp[VZ], v2_xp[VZ], v1_xp[VZ];
-    vload<S, VZ>(V + p, v0, sc.v[0]);
-    vload<S, VZ>(V + c1 + p, v1, sc.v[1]);
-    vload<S, VZ>(V + c2 + p, v2, sc.v[2]);
-    vload<S, VZ>(V + c2 + p + d.sy, v2_yp, sc.v[2]);
-    vload<S, VZ>(V + p + d.sy, v0_yp, sc.v[0]);
-    vload<S, VZ>(V + c2 + p + d.sx, v2_xp, sc.v[2]);
-    vload<S, VZ>(V + c1 + p + d.sx, v1_xp, sc.v[1]);
-    // z+1 neighbour of the last element: only needed when it is a valid line
-    const bool tail = z0 + VZ < d.nz;
-    const A v1_zpl = tail ? T::load(V + c1 + p + VZ, sc.v[1]) : A(0);
-    const A v0_zpl = tail ? T::load(V + p + VZ, sc.v[0]) : A(0);
-    A curl[3][VZ];
-#pragma unroll
-    for (int k = 0; k < VZ; ++k) {
-        const A v1_zp = k == VZ - 1 ? v1_zpl : v1[k + 1];
-        const A v0_zp = k == VZ - 1 ? v0_zpl : v0[k + 1];
-        curl[0][k] = ((v2[k] - v2_yp[k]) - v1[k]) + v1_zp;
-        curl[1][k] = ((v0[k] - v0_zp) - v2[k]) + v2_xp[k];
-        curl[2][k] = ((v1[k] - v1_xp[k]) - v0[k]) + v0_yp[k];
-    }
-    A ii[3][VZ], iv[3][VZ];
-    vload_coeffs<A, CM, VZ>(cf, p, d.sc, ii, iv);
-    const int valid = min(VZ, d.nz - 1 - z0);
-    long long pb = 0;
-    unsigned pm = 0;
-    const int slab = PML ? find_slab_vec(ps, x, y, z0, VZ, pb, pm) : -1;
-    // cells at z >= nz-1 are not updated: keep them out of the slab mask too
-    if (valid < VZ) pm &= (1u << valid) - 1u;
-#pragma unroll
-    for (int n = 0; n < 3; ++n) {
-        A c[VZ];
-        vload<S, VZ>(I + n * d.sc + p, c, sc.c[n]);
-        if (PML && slab >= 0) {
-            const long long q = pb + n * ps.slab[slab].sc;
-            A F[VZ], ki[VZ], kfo[VZ], kfn[VZ];
-            vload<A, VZ>(pp.cflux + q, F, A(1));
-            vload<A, VZ>(pp.ii + q, ki, A(1));
-            vload<A, VZ>(pp.iifo + q, kfo, A(1));
-            vload<A, VZ>(pp.iifn + q, kfn, A(1));
-#pragma unroll
-            for (int k = 0; k < VZ; ++k) {
-                if (pm & (1u << k)) {
-                    A h = ki[k] * c[k] - kfo[k] * F[k];
-                    A t = F[k];
-                    t = t * ii[n][k];
-                    t = t + iv[n][k] * curl[n][k];
-                    F[k] = t;
-                    c[k] = h + kfn[k] * t;
-                } else {
-                    c[k] = c[k] * ii[n][k];
-                    c[k] = c[k] + iv[n][k] * curl[n][k];
-                }
-            }
-            vstore<A, VZ>(pp.cflux + q, F, A(1), VZ);
-        } else {
-#pragma unroll
-            for (int k = 0; k < VZ; ++k) {
-                c[k] = c[k] * ii[n][k];
-                c[k] = c[k] + iv[n][k] * curl[n][k];
-            }
-        }
-        vstore<S, VZ>(I + n * d.sc + p, c, sc.ic[n], valid);
-    }
-}
-
-// ---------------------------------------------------------------------------
-// banded launch order: one thread per cell like k_volt/k_curr, but block ids
-// are remapped so that each XCD (hardware places block b on XCD b % 8) walks
-// a contiguous run of logical tiles ordered (y band, x, y tile in band, z
-// tile).  A band plane (band_tiles x tile_y rows x Nz) is small, so the x-1
-// plane and the y-1 rows a tile needs were just fetched into the same XCD's
-// L2 by the tiles before it.
-// ---------------------------------------------------------------------------
-struct BandGrid {
-    int tiles_z, tiles_y, band_tiles, nbands, nx;
-    int nblocks, per_xcd;
-};
-
-__device__ __forceinline__ bool band_tile(const BandGrid& g, int& tz, int& ty, int& x) {
-    const int b = blockIdx.x;
-    const int t = (b & 7) * g.per_xcd + (b >> 3);
-    if (t >= g.nblocks) return false;
-    tz = t % g.tiles_z;
-    int r = t / g.tiles_z;
-    const int tyb = r % g.band_tiles;
-    r /= g.band_tiles;
-    x = r % g.nx;
-    const int band = r / g.nx;
-    ty = band * g.band_tiles + tyb;
-    return ty < g.tiles_y;
-}
-
-template <typename S, int CM, bool PML>
-__global__ void __launch_bounds__(256)
-k_volt_band(S* __restrict__ V, const S* __restrict__ I, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
-            Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp, BandGrid g) {
-    int tz, ty, x;
-    if (!band_tile(g, tz, ty, x)) return;
-    const int z = tz * 64 + threadIdx.x;
-    const int y = ty * blockDim.y + threadIdx.y;
-    if (z >= d.nz || y >= d.ny) return;
-    volt_cell<S, CM, PML>(V, I, cf, d, sc, ps, pp, x, y, z);
-}
-
-template <typename S, int CM, bool PML>
-__global__ void __launch_bounds__(256)
-k_curr_band(S* __restrict__ I, const S* __restrict__ V, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
-            Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp, BandGrid g) {
-    int tz, ty, x;
-    if (!band_tile(g, tz, ty, x)) return;
-    const int z = tz * 64 + threadIdx.x;
-    const int y = ty * blockDim.y + threadIdx.y;
-    if (z >= d.nz - 1 || y >= d.ny - 1) return;
-    curr_cell<S, CM, PML>(I, V, cf, d, sc, ps, pp, x, y, z);
-}
-
-// ---------------------------------------------------------------------------
-// x-marching versions of the two updates (2.5-D blocking).  A block is a
-// 64(z) x TY(y) tile of columns; each thread walks its (y,z) column through an
-// x chunk and keeps the x-neighbour values of the previous plane in
-// registers, so every field element is fetched from HBM once per kernel; the
-// y and z neighbours are re-read by adjacent lanes/waves of the same block
-// and hit L1/L2.  The linear block id is remapped so that consecutive tiles
-// (which share their y halo rows) run on the same XCD and share its L2.
-// Arithmetic is identical to k_volt / k_curr.
-// ---------------------------------------------------------------------------
-struct MarchGrid {
-    int tiles_z, tiles_y, chunks_x;    // logical tile grid
-    int chunk;                         // x planes per chunk
-    int nblocks;                       // tiles_z * tiles_y * chunks_x
-    int per_xcd;                       // ceil(nblocks / 8)
-};
-
-__device__ __forceinline__ bool march_tile(const MarchGrid& g, int& tz, int& ty, int& cx) {
-    // hardware dispatch places block b on XCD b % 8; give each XCD a
-    // contiguous run of logical tiles (z fastest, then y, then x chunk)
-    const int b = blockIdx.x;
-    const int t = (b & 7) * g.per_xcd + (b >> 3);
-    if (t >= g.nblocks) return false;
-    tz = t % g.tiles_z;
-    const int r = t / g.tiles_z;
-    ty = r % g.tiles_y;
-    cx = r / g.tiles_y;
-    return true;
-}
-
-template <typename S, int CM, bool PML>
-__global__ void __launch_bounds__(256)
-k_volt_march(S* __restrict__ V, const S* __restrict__ I, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
-             Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp, MarchGrid g) {
-    typedef StoreTraits<S> T;
-    typedef typename T::A A;
-    int tz, ty, cx;
-    if (!march_tile(g, tz, ty, cx)) return;
-    const int z = tz * 64 + threadIdx.x;
-    const int y = ty * blockDim.y + threadIdx.y;
-    if (z >= d.nz || y >= d.ny) return;
-    const int x0 = cx * g.chunk;
-    const int x1 = min(d.nx, x0 + g.chunk);
-    const long long c1 = d.sc, c2 = 2 * d.sc;
-    const long long oy = y > 0 ? d.sy : 0;
-    long long p = x0 * d.sx + y * d.sy + z;
-    // x-1 neighbours of the first plane (the cell itself at x = 0)
-    A i1_prev, i2_prev;
-    if (x0 > 0) {
-        i1_prev = T::load(I + c1 + p - d.sx, sc.c[1]);
-        i2_prev = T::load(I + c2 + p - d.sx, sc.c[2]);
-    } else {
-        i1_prev = T::load(I + c1 + p, sc.c[1]);
-        i2_prev = T::load(I + c2 + p, sc.c[2]);
-    }
-    for (int x = x0; x < x1; ++x, p += d.sx) {
-        const A i0 = T::load(I + p, sc.c[0]);
-        const A i1 = T::load(I + c1 + p, sc.c[1]);
-        const A i2 = T::load(I + c2 + p, sc.c[2]);
-        const A i2_ym = T::load(I + c2 + p - oy, sc.c[2]);
-        const A i0_ym = T::load(I + p - oy, sc.c[0]);
-        const A i1_zm = z > 0 ? T::load(I + c1 + p - 1, sc.c[1]) : A(0);
-        const A i0_zm = z > 0 ? T::load(I + p - 1, sc.c[0]) : A(0);
-        A curl[3];
-        curl[0] = ((i2 - i2_ym) - i1) + i1_zm;
-        curl[1] = ((i0 - i0_zm) - i2) + i2_prev;
-        curl[2] = ((i1 - i1_prev) - i0) + i0_ym;
-        i1_prev = i1;
-        i2_prev = i2;
-
-        A vv[3], vi[3];
-        load_coeffs<A, CM>(cf, p, d.sc, vv, vi);
-        long long lp = 0;
-        int slab = PML ? find_slab(ps, x, y, z, lp) : -1;
-#pragma unroll
-        for (int n = 0; n < 3; ++n) {
-            S* vp = V + n * d.sc + p;
-            A v = T::load(vp, sc.v[n]);
-            if (PML && slab >= 0) {
-                const PmlSlab& b = ps.slab[slab];
-                const long long q = lp + n * b.sc;
-                A F = pp.vflux[q];
-                A h = pp.vv[q] * v - pp.vvfo[q] * F;
-                A t = F;
-                t = t * vv[n];
-                t = t + vi[n] * curl[n];
-                pp.vflux[q] = t;
-                v = h + pp.vvfn[q] * t;
-            } else {
-                v = v * vv[n];
-                v = v + vi[n] * curl[n];
-            }
-            T::store(vp, v, sc.iv[n]);
-        }
-    }
-}
-
-template <typename S, int CM, bool PML>
-__global__ void __launch_bounds__(256)
-k_curr_march(S* __restrict__ I, const S* __restrict__ V, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
-             Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp, MarchGrid g) {
-    typedef StoreTraits<S> T;
-    typedef typename T::A A;
-    int tz, ty, cx;
-    if (!march_tile(g, tz, ty, cx)) return;
-    const int z = tz * 64 + threadIdx.x;
-    const int y = ty * blockDim.y + threadIdx.y;
-    if (z >= d.nz - 1 || y >= d.ny - 1) return;
-    const int x0 = cx * g.chunk;
-    const int x1 = min(d.nx - 1, x0 + g.chunk);
-    if (x0 >= x1) return;
-    const long long c1 = d.sc, c2 = 2 * d.sc;
-    long long p = x0 * d.sx + y * d.sy + z;
-    // V1, V2 of the current plane come from the previous iteration's x+1 loads
-    A v1 = T::load(V + c1 + p, sc.v[1]);
-    A v2 = T::load(V + c2 + p, sc.v[2]);
-    for (int x = x0; x < x1; ++x, p += d.sx) {
-        const A v0 = T::load(V + p, sc.v[0]);
-        const A v2_xp = T::load(V + c2 + p + d.sx, sc.v[2]);
-        const A v1_xp = T::load(V + c1 + p + d.sx, sc.v[1]);
-        const A v2_yp = T::load(V + c2 + p + d.sy, sc.v[2]);
-        const A v0_yp = T::load(V + p + d.sy, sc.v[0]);
-        const A v1_zp = T::load(V + c1 + p + 1, sc.v[1]);
-        const A v0_zp = T::load(V + p + 1, sc.v[0]);
-        A curl[3];
-        curl[0] = ((v2 - v2_yp) - v1) + v1_zp;
-        curl[1] = ((v0 - v0_zp) - v2) + v2_xp;
-        curl[2] = ((v1 - v1_xp) - v0) + v0_yp;
-        v1 = v1_xp;
-        v2 = v2_xp;
-
-        A ii[3], iv[3];
-        load_coeffs<A, CM>(cf, p, d.sc, ii, iv);
-        long long lp = 0;
-        int slab = PML ? find_slab(ps, x, y, z, lp) : -1;
-#pragma unroll
-        for (int n = 0; n < 3; ++n) {
-            S* ip = I + n * d.sc + p;
-            A c = T::load(ip, sc.c[n]);
-            if (PML && slab >= 0) {
-                const PmlSlab& b = ps.slab[slab];
-                const long long q = lp + n * b.sc;
-                A F = pp.cflux[q];
-                A h = pp.ii[q] * c - pp.iifo[q] * F;
-                A t = F;
-                t = t * ii[n];
-                t = t + iv[n] * curl[n];
-                pp.cflux[q] = t;
-                c = h + pp.iifn[q] * t;
-            } else {
-                c = c * ii[n];
-                c = c + iv[n] * curl[n];
-            }
-            T::store(ip, c, sc.ic[n]);
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// sparse hooks
-// ---------------------------------------------------------------------------
-struct ExcDev {
-    int count;
-    int length;
-    int period;                    // timesteps, 0 = none
-    const long long* off;          // element offset incl. component
-    const int* comp;
-    const int* delay;
-    const void* amp;               // A[count]
-    const void* sig;               // A[length]
-    int sequential;                // duplicate edges present: apply in list order
-};
-
-// Engine_Ext_Excitation::Apply2{Voltages,Current}Impl, engine_ext_excitation.cpp:33-94
-template <typename S>
-__global__ void k_excite(S* F, const unsigned* __restrict__ step, ExcDev e, Scales<typename StoreTraits<S>::A> sc, int is_curr) {
-    typedef StoreTraits<S> T;
-    typedef typename T::A A;
-    const int numTS = (int)*step;
-    const A* amp = (const A*)e.amp;
-    const A* sig = (const A*)e.sig;
-    int p = numTS + 1;
-    if (e.period > 0) p = e.period;
-    int begin = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
-    if (e.sequential) { if (begin != 0) return; stride = 1; }
-    for (int i = begin; i < e.count; i += stride) {
-        int k = numTS - e.delay[i];
-        k *= (k > 0);
-        k %= p;
-        k *= (k < e.length);
-        const int n = e.comp[i];
-        const A s = is_curr ? sc.c[n] : sc.v[n];
-        const A is = is_curr ? sc.ic[n] : sc.iv[n];
-        S* fp = F + e.off[i];
-        A v = T::load(fp, s);
-        v = v + amp[i] * sig[k];
-        T::store(fp, v, is);
-    }
-}
-
-struct AdeDev {
-    int count;
-    const long long* off;          // [count] element offset of position (component 0)
-    const void* v_int; const void* v_ext; const void* v_lor;   // [3][count]
-    const void* i_int; const void* i_ext; const void* i_lor;
-    void* volt_ade; void* volt_lor; void* curr_ade; void* curr_lor;  // state [3][count]
-    int volt_on, curr_on, volt_lor_on, curr_lor_on;
-};
-
-// Engine_Ext_LorentzMaterial::DoPre{Voltage,Current}UpdatesImpl, engine_ext_lorentzmaterial.cpp:79-168
-template <typename S>
-__global__ void k_ade_pre(const S* __restrict__ F, AdeDev a, long long sc_field, Scales<typename StoreTraits<S>::A> sc, int is_curr) {
-    typedef StoreTraits<S> T;
-    typedef typename T::A A;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.count) return;
-    const A* kint = (const A*)(is_curr ? a.i_int : a.v_int);
-    const A* kext = (const A*)(is_curr ? a.i_ext : a.v_ext);
-    const A* klor = (const A*)(is_curr ? a.i_lor : a.v_lor);
-    A* J = (A*)(is_curr ? a.curr_ade : a.volt_ade);
-    A* JL = (A*)(is_curr ? a.curr_lor : a.volt_lor);
-    const int lor_on = is_curr ? a.curr_lor_on : a.volt_lor_on;
-#pragma unroll
-    for (int n = 0; n < 3; ++n) {
-        const long long q = (long long)n * a.count + i;
-        const A f = T::load(F + n * sc_field + a.off[i], is_curr ? sc.c[n] : sc.v[n]);
-        if (lor_on) {
-            A jl = JL[q];
-            A j = J[q];
-            jl = jl + klor[q] * j;
-            j = j * kint[q];
-            j = j + kext[q] * (f - jl);
-            JL[q] = jl;
-            J[q] = j;
-        } else {
-            A j = J[q];
-            j = j * kint[q];
-            j = j + kext[q] * f;
-            J[q] = j;
-        }
-    }
-}
-
-// Engine_Ext_Dispersive::Apply2{Voltages,Current}Impl, engine_ext_dispersive.cpp:76-127
-template <typename S>
-__global__ void k_ade_apply(S* F, AdeDev a, long long sc_field, Scales<typename StoreTraits<S>::A> sc, int is_curr) {
-    typedef StoreTraits<S> T;
-    typedef typename T::A A;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.count) return;
-    const A* J = (const A*)(is_curr ? a.curr_ade : a.volt_ade);
-#pragma unroll
-    for (int n = 0; n < 3; ++n) {
-        S* fp = F + n * sc_field + a.off[i];
-        const A s = is_curr ? sc.c[n] : sc.v[n];
-        A f = T::load(fp, s);
-        f = f - J[(long long)n * a.count + i];
-        T::store(fp, f, is_curr ? sc.ic[n] : sc.iv[n]);
-    }
-}
-
-struct TfsfDev {
-    int count;
-    int length, period, max_delay;
-    const long long* off;   // element offset incl. component
-    const int* comp;
-    const int* delay;
-    const void* delta;      // A[count]
-    const void* amp;        // A[count]
-    const void* sig;        // A[length] (curr signal for volt injection and vice versa)
-};
-
-// Engine_Ext_TFSF::DoPost{Voltage,Current}Updates, engine_ext_tfsf.cpp:36-215:
-// F += (1-delta)*amp*s[L[d]] + delta*amp*s[L[d+1]] evaluated in double and
-// rounded once to the storage type.
-template <typename S>
-__global__ void k_tfsf(S* F, const unsigned* __restrict__ step, TfsfDev t, Scales<typename StoreTraits<S>::A> sc, int is_curr) {
-    typedef StoreTraits<S> T;
-    typedef typename T::A A;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= t.count) return;
-    const unsigned numTS = *step;
-    const A* sig = (const A*)t.sig;
-    const A* amp = (const A*)t.amp;
-    const A* delta = (const A*)t.delta;
-    auto lookup = [&](unsigned n) -> unsigned {
-        unsigned L;
-        if (numTS < n) L = 0;
-        else if ((numTS - n >= (unsigned)t.length) && (t.period == 0)) L = 0;
-        else L = numTS - n;
-        if (t.period > 0) L = L % (unsigned)t.period;
-        return L;
-    };
-    const unsigned dl = (unsigned)t.delay[i];
-    const int n = t.comp[i];
-    S* fp = F + t.off[i];
-    const A s = is_curr ? sc.c[n] : sc.v[n];
-    const double f = (double)T::load(fp, s);
-    // (1.0-delta)*amp*s is double arithmetic; delta*amp*s is evaluated in the
-    // storage float type, as in the reference expression
-    const A second = delta[i] * amp[i] * sig[lookup(dl + 1)];
-    double r = f + (1.0 - (double)delta[i]) * (double)amp[i] * (double)sig[lookup(dl)] + (double)second;
-    T::store(fp, (A)r, is_curr ? sc.ic[n] : sc.iv[n]);
-}
-
-// ---------------------------------------------------------------------------
-// step end: ++numTS, then probe integrals (FP64) and FD accumulation,
-// ProcessIntegral::Process (Common/processintegral.cpp:134-171) at the new
-// timestep, time = (numTS + 0.5*dual)*dT (FDTD/engine_interface_fdtd.h:52).
-// ---------------------------------------------------------------------------
-struct ProbeDev {
-    int kind;              // 0 volt, 1 curr
-    int first, count;      // edge range
-    int td_interval, fd_interval;
-    int n_freq;
-    double weight;
-    double t_shift;        // 0.5 for dual time
-    double* td;            // [slots][2] (t, value)
-    long long td_slots;
-    double* fd;            // [n_freq][2]
-    const double* freqs;
-};
-
-template <typename S>
-__global__ void k_step_end(const S* __restrict__ V, const S* __restrict__ I, unsigned* step, int advance,
-                           const ProbeDev* __restrict__ probes, int nprobes, const long long* __restrict__ e_off,
-                           const int* __restrict__ e_comp, const signed char* __restrict__ e_sign,
-                           Scales<typename StoreTraits<S>::A> sc, double dT) {
-    typedef StoreTraits<S> T;
-    __shared__ unsigned s_ts;
-    if (threadIdx.x == 0) s_ts = *step + (unsigned)advance;
-    __syncthreads();
-    const unsigned ts = s_ts;
-    for (int pi = threadIdx.x; pi < nprobes; pi += blockDim.x) {
-        const ProbeDev& pr = probes[pi];
-        const bool td = pr.td_interval > 0 && (ts % (unsigned)pr.td_interval) == 0;
-        const bool fd = pr.fd_interval > 0 && (ts % (unsigned)pr.fd_interval) == 0;
-        if (!td && !fd) continue;
-        const S* F = pr.kind == 0 ? V : I;
-        double val = 0.0;
-        for (int e = pr.first; e < pr.first + pr.count; ++e) {
-            const int n = e_comp[e];
-            const double f = (double)T::load(F + e_off[e], pr.kind == 0 ? sc.v[n] : sc.c[n]);
-            if (e_sign[e] > 0) val += f; else val -= f;
-        }
-        const double t = ((double)ts + pr.t_shift) * dT;
-        const double w = val * pr.weight;
-        if (td) {
-            long long slot = ts / (unsigned)pr.td_interval;
-            if (slot < pr.td_slots) { pr.td[2 * slot] = t; pr.td[2 * slot + 1] = w; }
-        }
-        if (fd) {
-            for (int k = 0; k < pr.n_freq; ++k) {
-                double sn, cs;
-                sincos(-2.0 * M_PI * pr.freqs[k] * t, &sn, &cs);
-                const double g = w * 2.0 * dT * (double)pr.fd_interval;
-                pr.fd[2 * k] += g * cs;
-                pr.fd[2 * k + 1] += g * sn;
-            }
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) *step = ts;
-}
-
-// ---------------------------------------------------------------------------
-// energy: eps0*sum(V^2) + mu0*sum(I^2) over x<Nx-1, y<Ny-1, all z
-// (Engine_Interface_SSE_FDTD::CalcFastEnergy, engine_interface_sse_fdtd.cpp:37-69),
-// accumulated in FP64, deterministic two-pass reduction.
-// ---------------------------------------------------------------------------
-template <typename S>
-__global__ void __launch_bounds__(256)
-k_energy_partial(const S* __restrict__ V, const S* __restrict__ I, Dims d, Scales<typename StoreTraits<S>::A> sc, double* part) {
-    typedef StoreTraits<S> T;
-    __shared__ double se[256], sh[256];
-    double e = 0.0, h = 0.0;
-    const long long rows = (long long)(d.nx - 1) * (d.ny - 1);
-    for (long long r = blockIdx.x; r < rows; r += gridDim.x) {
-        const int x = (int)(r / (d.ny - 1)), y = (int)(r % (d.ny - 1));
-        const long long base = x * d.sx + y * d.sy;
-        for (int z = threadIdx.x; z < d.nz; z += blockDim.x) {
-#pragma unroll
-            for (int n = 0; n < 3; ++n) {
-                const double v = (double)T::load(V + n * d.sc + base + z, sc.v[n]);
-                const double c = (double)T::load(I + n * d.sc + base + z, sc.c[n]);
-                e += v * v;
-                h += c * c;
-            }
-        }
-    }
-    se[threadIdx.x] = e;
-    sh[threadIdx.x] = h;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) { se[threadIdx.x] += se[threadIdx.x + s]; sh[threadIdx.x] += sh[threadIdx.x + s]; }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) { part[2 * blockIdx.x] = se[0]; part[2 * blockIdx.x + 1] = sh[0]; }
-}
+thread_local std::string g_last_error;
 
 __global__ void k_energy_final(const double* part, int n, double* out) {
     __shared__ double se[256], sh[256];
@@ -1026,160 +56,8 @@ __global__ void __launch_bounds__(256) k_copy(const float4* __restrict__ a, floa
     for (; i < n; i += stride) b[i] = a[i];
 }
 
-// gather/scatter between the padded device layout and the dense host layout
-template <typename S, typename H>
-__global__ void k_pack(const S* __restrict__ F, H* __restrict__ out, Dims d, Scales<typename StoreTraits<S>::A> sc, int is_curr) {
-    typedef StoreTraits<S> T;
-    const long long total = 3LL * d.nx * d.ny * d.nz;
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total; i += (long long)gridDim.x * blockDim.x) {
-        long long r = i;
-        const int z = (int)(r % d.nz); r /= d.nz;
-        const int y = (int)(r % d.ny); r /= d.ny;
-        const int x = (int)(r % d.nx); const int n = (int)(r / d.nx);
-        out[i] = (H)T::load(F + n * d.sc + x * d.sx + y * d.sy + z, is_curr ? sc.c[n] : sc.v[n]);
-    }
-}
 
-template <typename S, typename H>
-__global__ void k_unpack(S* __restrict__ F, const H* __restrict__ in, Dims d, Scales<typename StoreTraits<S>::A> sc, int is_curr) {
-    typedef StoreTraits<S> T;
-    typedef typename T::A A;
-    const long long total = 3LL * d.nx * d.ny * d.nz;
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total; i += (long long)gridDim.x * blockDim.x) {
-        long long r = i;
-        const int z = (int)(r % d.nz); r /= d.nz;
-        const int y = (int)(r % d.ny); r /= d.ny;
-        const int x = (int)(r % d.nx); const int n = (int)(r / d.nx);
-        T::store(F + n * d.sc + x * d.sx + y * d.sy + z, (A)in[i], is_curr ? sc.ic[n] : sc.iv[n]);
-    }
-}
-
-// ---------------------------------------------------------------------------
-// host side
-// ---------------------------------------------------------------------------
-namespace {
-
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    ~DevBuf() { if (p) (void)hipFree(p); }
-    int alloc(size_t n) {
-        if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
-        if (n == 0) return FDTD_OK;
-        hipError_t e = hipMalloc(&p, n);
-        if (e != hipSuccess) { p = nullptr; return fail(FDTD_ERR_NOMEM, "hipMalloc(%zu) failed: %s", n, hipGetErrorString(e)); }
-        bytes = n;
-        return FDTD_OK;
-    }
-    template <typename X> X* as() const { return (X*)p; }
-};
-
-struct ProbeHost {
-    int kind;
-    int norm_dir;
-    double weight;
-    uint32_t td_interval, fd_interval;
-    std::vector<double> freqs;
-    std::vector<long long> off;
-    std::vector<int> comp;
-    std::vector<signed char> sign;
-    DevBuf td, fd, dfreq;
-    long long td_slots = 0;
-};
-
-struct AdeHost {
-    uint32_t count = 0;
-    uint32_t flags = 0;
-    DevBuf off, coef, state;   // coef: 6 lists [3][count]; state: 4 lists [3][count]
-};
-
-} // namespace
-
-struct fdtd_engine {
-    int prec = FDTD_PREC_FP32;
-    int dev = 0;
-    uint32_t N[3] = {0, 0, 0};
-    Dims d{};
-    double dT = 0;
-    size_t elem = 4;              // bytes per stored field value
-    size_t acc = 4;               // bytes per arithmetic value (coefficients)
-    hipStream_t stream = nullptr;
-
-    DevBuf V, I;                  // fields
-    DevBuf step;                  // unsigned numTS on device
-    uint32_t numTS = 0;
-
-    int cf_mode = -1;             // CF_* once set
-    DevBuf cf_vv, cf_vi, cf_ii, cf_iv;   // full mode
-    DevBuf cf_idx, cf_tabE, cf_tabH;     // compressed mode
-    uint32_t n_entries = 0;
-
-    double vscale[3] = {1, 1, 1}, cscale[3] = {1, 1, 1};
-
-    // UPML
-    PmlSet pml{};
-    DevBuf pml_coef;              // 6 arrays over all slabs
-    DevBuf pml_flux;              // 2 arrays over all slabs
-    std::vector<std::vector<char>> pml_host_coef;   // staged per slab until finalized
-    long long pml_total = 0;
-    bool pml_dirty = false;
-    bool pml_final = false;
-
-    // excitation
-    bool has_exc = false;
-    ExcDev exc_v{}, exc_c{};
-    DevBuf exc_buf;
-
-    // ADE orders
-    std::vector<AdeHost*> ade;
-
-    // TFSF
-    TfsfDev tfsf_v{}, tfsf_c{};
-    DevBuf tfsf_buf_v, tfsf_buf_c;
-
-    // probes
-    std::vector<ProbeHost*> probes;
-    DevBuf probe_dev, e_off, e_comp, e_sign;
-    bool probes_dirty = true;
-    bool sampled0 = false;
-
-    // energy
-    DevBuf e_part, e_out;
-    int e_blocks = 0;
-
-    // graph
-    bool use_graph = true;
-    hipGraphExec_t gexec = nullptr;
-    hipGraph_t graph = nullptr;
-    int graph_steps = 0;
-    hipGraphExec_t gexec1 = nullptr;
-    hipGraph_t graph1 = nullptr;
-
-    // kernel selection: 0 = one thread per cell, 1 = x-march, 2 = banded, 3 = vectors (default)
-    int variant = 3;
-    int tile_y = 4;
-    int chunk_x = 64;
-
-    // timing
-    bool timing = false;
-    std::vector<hipEvent_t> ev;
-    size_t ev_used = 0;
-    double t_volt = 0, t_curr = 0;
-    uint32_t n_volt = 0, n_curr = 0;
-
-    ~fdtd_engine() {
-        if (gexec) (void)hipGraphExecDestroy(gexec);
-        if (graph) (void)hipGraphDestroy(graph);
-        if (gexec1) (void)hipGraphExecDestroy(gexec1);
-        if (graph1) (void)hipGraphDestroy(graph1);
-        for (auto e : ev) (void)hipEventDestroy(e);
-        for (auto p : probes) delete p;
-        for (auto a : ade) delete a;
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-};
-
-namespace {
+namespace wipems {
 
 void invalidate_graph(fdtd_engine* h) {
     if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
@@ -1189,15 +67,6 @@ void invalidate_graph(fdtd_engine* h) {
     h->graph_steps = 0;
 }
 
-template <typename A>
-Scales<A> make_scales(const fdtd_engine* h) {
-    Scales<A> s;
-    for (int n = 0; n < 3; ++n) {
-        s.v[n] = (A)h->vscale[n]; s.iv[n] = (A)(1.0 / h->vscale[n]);
-        s.c[n] = (A)h->cscale[n]; s.ic[n] = (A)(1.0 / h->cscale[n]);
-    }
-    return s;
-}
 
 long long cell_off(const fdtd_engine* h, uint32_t x, uint32_t y, uint32_t z) {
     return (long long)x * h->d.sx + (long long)y * h->d.sy + z;
@@ -1400,8 +269,6 @@ void launch_curr_march(fdtd_engine* h, CoeffPtrs<typename StoreTraits<S>::A> cf,
     default: hipLaunchKernelGGL((k_curr_march<S, CF_IDX32, PML>), gr, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, sc, h->pml, pp, g); break;
     }
 }
-
-template <typename S> struct VecWidth { enum { value = 16 / sizeof(S) }; };
 
 template <typename S, bool PML>
 void launch_volt_vec(fdtd_engine* h, CoeffPtrs<typename StoreTraits<S>::A> cf, PmlPtrs<typename StoreTraits<S>::A> pp) {
@@ -1651,7 +518,9 @@ double decode_elem(const fdtd_engine* h, const void* raw, double scale) {
     }
 }
 
-} // namespace
+} // namespace wipems
+
+using namespace wipems;
 
 // ===========================================================================
 // C ABI

@@ -1,0 +1,170 @@
+// fdtd_internal.h -- host-side state of an engine handle, shared by the
+// single-domain (fdtd_engine.hip) and batched (fdtd_batch.hip) C ABI code.
+#pragma once
+#include "fdtd_device.h"
+
+// ---------------------------------------------------------------------------
+// error handling
+// ---------------------------------------------------------------------------
+extern thread_local std::string g_last_error;
+
+inline int fail(int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                 \
+    do {                                                                             \
+        hipError_t e_ = (expr);                                                      \
+        if (e_ != hipSuccess)                                                        \
+            return fail(FDTD_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+
+namespace wipems {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    int alloc(size_t n) {
+        if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
+        if (n == 0) return FDTD_OK;
+        hipError_t e = hipMalloc(&p, n);
+        if (e != hipSuccess) { p = nullptr; return fail(FDTD_ERR_NOMEM, "hipMalloc(%zu) failed: %s", n, hipGetErrorString(e)); }
+        bytes = n;
+        return FDTD_OK;
+    }
+    template <typename X> X* as() const { return (X*)p; }
+};
+
+struct ProbeHost {
+    int kind;
+    int norm_dir;
+    double weight;
+    uint32_t td_interval, fd_interval;
+    std::vector<double> freqs;
+    std::vector<long long> off;
+    std::vector<int> comp;
+    std::vector<signed char> sign;
+    DevBuf td, fd, dfreq;
+    long long td_slots = 0;
+};
+
+struct AdeHost {
+    uint32_t count = 0;
+    uint32_t flags = 0;
+    DevBuf off, coef, state;   // coef: 6 lists [3][count]; state: 4 lists [3][count]
+};
+
+} // namespace wipems
+
+using wipems::DevBuf;
+using wipems::ProbeHost;
+using wipems::AdeHost;
+
+struct fdtd_engine {
+    int prec = FDTD_PREC_FP32;
+    int dev = 0;
+    uint32_t N[3] = {0, 0, 0};
+    Dims d{};
+    double dT = 0;
+    size_t elem = 4;              // bytes per stored field value
+    size_t acc = 4;               // bytes per arithmetic value (coefficients)
+    hipStream_t stream = nullptr;
+
+    DevBuf V, I;                  // fields
+    DevBuf step;                  // unsigned numTS on device
+    uint32_t numTS = 0;
+
+    int cf_mode = -1;             // CF_* once set
+    DevBuf cf_vv, cf_vi, cf_ii, cf_iv;   // full mode
+    DevBuf cf_idx, cf_tabE, cf_tabH;     // compressed mode
+    uint32_t n_entries = 0;
+
+    double vscale[3] = {1, 1, 1}, cscale[3] = {1, 1, 1};
+
+    // UPML
+    PmlSet pml{};
+    DevBuf pml_coef;              // 6 arrays over all slabs
+    DevBuf pml_flux;              // 2 arrays over all slabs
+    std::vector<std::vector<char>> pml_host_coef;   // staged per slab until finalized
+    long long pml_total = 0;
+    bool pml_dirty = false;
+    bool pml_final = false;
+
+    // excitation
+    bool has_exc = false;
+    ExcDev exc_v{}, exc_c{};
+    DevBuf exc_buf;
+
+    // ADE orders
+    std::vector<AdeHost*> ade;
+
+    // TFSF
+    TfsfDev tfsf_v{}, tfsf_c{};
+    DevBuf tfsf_buf_v, tfsf_buf_c;
+
+    // probes
+    std::vector<ProbeHost*> probes;
+    DevBuf probe_dev, e_off, e_comp, e_sign;
+    bool probes_dirty = true;
+    bool sampled0 = false;
+
+    // energy
+    DevBuf e_part, e_out;
+    int e_blocks = 0;
+
+    // graph
+    bool use_graph = true;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    int graph_steps = 0;
+    hipGraphExec_t gexec1 = nullptr;
+    hipGraph_t graph1 = nullptr;
+
+    // kernel selection: 0 = one thread per cell, 1 = x-march, 2 = banded, 3 = vectors (default)
+    int variant = 3;
+    int tile_y = 4;
+    int chunk_x = 64;
+
+    // timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev;
+    size_t ev_used = 0;
+    double t_volt = 0, t_curr = 0;
+    uint32_t n_volt = 0, n_curr = 0;
+
+    ~fdtd_engine() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (gexec1) (void)hipGraphExecDestroy(gexec1);
+        if (graph1) (void)hipGraphDestroy(graph1);
+        for (auto e : ev) (void)hipEventDestroy(e);
+        for (auto p : probes) delete p;
+        for (auto a : ade) delete a;
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace wipems {
+void invalidate_graph(fdtd_engine* h);
+int finalize_pml(fdtd_engine* h);
+int finalize_probes(fdtd_engine* h, uint32_t upto_ts);
+int ready_check(fdtd_engine* h);
+
+template <typename A>
+Scales<A> make_scales(const fdtd_engine* h) {
+    Scales<A> s;
+    for (int n = 0; n < 3; ++n) {
+        s.v[n] = (A)h->vscale[n]; s.iv[n] = (A)(1.0 / h->vscale[n]);
+        s.c[n] = (A)h->cscale[n]; s.ic[n] = (A)(1.0 / h->cscale[n]);
+    }
+    return s;
+}
+} // namespace wipems

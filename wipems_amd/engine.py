@@ -298,3 +298,52 @@ def engine_from_operator(data: OperatorData, precision="fp32", compressed=None, 
     for p in data.probes:
         eng.add_probe(p)
     return eng
+
+
+class Batch:
+    """Many independent domains advanced with one launch per update phase
+    (fdtd_batch_* in include/wipems_fdtd.h).  Results are read from the member
+    Engines after synchronize()."""
+
+    def __init__(self, engines):
+        self._lib = engine_lib()
+        self.engines = list(engines)
+        arr = (c_void_p * len(self.engines))(*[e._h.value for e in self.engines])
+        h = c_void_p()
+        check(self._lib, self._lib.fdtd_batch_create(arr, len(self.engines), byref(h)), "fdtd_last_error")
+        self._h = h
+
+    def _chk(self, rc):
+        return check(self._lib, rc, "fdtd_last_error")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.fdtd_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_block(self, threads):
+        self._chk(self._lib.fdtd_batch_set_block(self._h, int(threads)))
+
+    def set_use_graph(self, on):
+        self._chk(self._lib.fdtd_batch_set_use_graph(self._h, int(bool(on))))
+
+    def iterate(self, n):
+        self._chk(self._lib.fdtd_batch_iterate(self._h, int(n)))
+
+    def synchronize(self):
+        self._chk(self._lib.fdtd_batch_synchronize(self._h))
+
+    def set_kernel_timing(self, on):
+        self._chk(self._lib.fdtd_batch_set_kernel_timing(self._h, int(bool(on))))
+
+    def kernel_times(self):
+        a, b = c_double(), c_double()
+        n = c_uint32()
+        self._chk(self._lib.fdtd_batch_get_kernel_times(self._h, byref(a), byref(b), byref(n)))
+        return a.value, b.value, n.value

@@ -157,3 +157,29 @@ def s11_from_probes(t_u, u, t_i, i, freqs, Z_ref=50.0):
     uf_inc = 0.5 * (uf + itf * Z_ref)
     uf_ref = uf - uf_inc
     return uf_ref / uf_inc
+
+
+def batched_port_scenarios(count=64, lo=48, hi=160, steps=5000, prec=0, seed=SEED_BASE + 4, pml=8, n_boxes=8, f0=3e9, fc=3e9):
+    """`count` independent domains with per-axis cell counts drawn from
+    U{lo..hi} (1 mm uniform mesh), 8-cell UPML on all faces, `n_boxes` random
+    lossless dielectric boxes (eps_r ~ U[1,10]) and one 50 Ohm lumped port
+    with a Gaussian excitation (f0 = fc = 3 GHz).  Defaults give the 64-domain,
+    48..160-cell, 5000-step case."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        n = [int(v) for v in rng.integers(lo, hi + 1, size=3)]
+        lines = [np.arange(m + 1, dtype=np.float64) for m in n]
+        op = Operator()
+        op.set_mesh(lines, 1e-3)
+        op.set_boundary([BC_PML] * 6, [pml] * 6)
+        op.set_gauss_excite(f0, fc)
+        boxes = random_boxes(rng, op, lines, n_boxes, pml, eps_range=(1.0, 10.0), kappa_log=None)
+        c = [m // 2 for m in n]
+        port = add_lumped_port(op, lines, (c[0], c[1], c[2] - 2), 4)
+        data = op.build(prec, nr_ts=steps)
+        interval = max(1, data.nyquist // 4)
+        _finish(op, data, port_probes(op, port, interval))
+        out.append(Scenario("batch_%d_%dx%dx%d" % (k, n[0], n[1], n[2]), data, steps, op, port=port,
+                            meta={"interval": interval, "boxes": boxes, "cells": n}))
+    return out
