@@ -61,18 +61,23 @@ def kernel_bytes(eng, data):
     return field + coef
 
 
-def read_traffic(kernel, workload):
-    """HBM bytes per launch from the committed rocprofv3 PMC summary, if any."""
+def read_traffic(kernel, workload, precision="fp32"):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary of this
+    kernel, workload and storage precision, if there is one."""
     path = os.path.join(ROOT, "profiles", "%s_%s_pmc.md" % (kernel, workload))
     if not os.path.exists(path):
         return None
+    fields = {}
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
-            try:
-                return float(line.split(":", 1)[1].strip())
-            except ValueError:
-                return None
-    return None
+        if ":" in line and not line.startswith(("|", "#", " ")):
+            k, v = line.split(":", 1)
+            fields[k.strip()] = v.strip()
+    if fields.get("precision", "fp32") != precision:
+        return None
+    try:
+        return float(fields["traffic_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
 
 
 def cpu_baseline_leg(data, interval, cells, threads, steps):
@@ -125,7 +130,7 @@ def run_batched(args, desc):
             alg += kernel_bytes(e, d) * d.cells
         achieved = alg / (avg_v * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic("volt_batch", "batched_64"),
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic("volt_batch", "batched_64", args.precision),
                     "kernel": "volt_batch", "avg_ms": round(avg_v, 5), "curr_avg_ms": round(avg_c, 5)}
         log("batched kernel avg: volt %.4f ms, curr %.4f ms -> %.1f GB/s" % (avg_v, avg_c, achieved))
     b.close()
@@ -184,6 +189,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=None, help="steps of the CPU-baseline sample (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=None)
     ap.add_argument("--no-timing-pass", action="store_true")
+    ap.add_argument("--no-fp64-check", action="store_true", help="skip the FP64-engine error check")
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="counter-collection mode: run this many steps of the workload, then a 1 GiB device copy "
                          "of known size (the calibration kernel for FETCH_SIZE/WRITE_SIZE), and exit")
@@ -249,7 +255,7 @@ def main():
         bpc = kernel_bytes(eng, data)
         alg = bpc * cells
         achieved = alg / (avg * 1e-3) / 1e9
-        traffic = read_traffic(kern, args.workload)
+        traffic = read_traffic(kern, args.workload, args.precision)
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                     "kernel": kern, "avg_ms": round(avg, 5), "bytes_per_cell": bpc,
@@ -270,6 +276,10 @@ def main():
         roofline["copy_kernel_GBs"] = round(copy_gbs, 1)
     log("plain copy kernel: %.1f GB/s" % copy_gbs)
 
+    err = None
+    if not args.no_fp64_check and args.precision != "fp64":
+        err = fp64_check(builder, kw, eng, sc, steps, args)
+
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "MCells/s", "n_gpus": 1, "steps": steps,
         "warmup": args.warmup, "ms_per_step": round(1e3 * wall / steps, 5), "higher_is_better": True,
@@ -277,9 +287,43 @@ def main():
         "data": "synthetic (seeded geometry)",
         "config": {"workload": args.workload, "description": desc, "cells": cells,
                    "coefficients": "compressed" if eng.compressed else "full", "precision": args.precision},
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, "error_vs_fp64": err,
     }
     print(json.dumps(out), flush=True)
+
+
+def fp64_check(builder, kw, eng, sc, steps, args):
+    """Run the same workload on an FP64 handle (double operator build, double
+    fields and coefficients) and compare the probe series of the timed engine;
+    for a lumped port also max |S11 - S11_fp64| over the excited band."""
+    import numpy as np
+    from wipems_amd import scenarios
+    from wipems_amd.engine import engine_from_operator
+    t0 = time.time()
+    kw64 = dict(kw)
+    kw64["prec"] = 1
+    sc64 = getattr(scenarios, builder)(**kw64)
+    e64 = engine_from_operator(sc64.data, precision="fp64", compressed=False)
+    total = eng.GetNumberOfTimesteps()
+    e64.iterate(total)
+    e64.synchronize()
+    out = {"steps": total}
+    for pid, p in enumerate(sc.data.probes):
+        t, v = eng.read_probe(pid)
+        t64, v64 = e64.read_probe(pid)
+        out["probe_%s_rel_l2" % (p.name or pid)] = float(np.linalg.norm(v - v64) / max(np.linalg.norm(v64), 1e-300))
+    if sc.port is not None:
+        f0, fc = sc.meta["f0"], sc.meta["fc"]
+        freqs = np.linspace(max(f0 - fc, 0.0) + 0.05 * fc, f0 + fc, 96)
+        tu, u = eng.read_probe(0); ti, i = eng.read_probe(1)
+        tu6, u6 = e64.read_probe(0); ti6, i6 = e64.read_probe(1)
+        s = scenarios.s11_from_probes(tu, u, ti, i, freqs)
+        s6 = scenarios.s11_from_probes(tu6, u6, ti6, i6, freqs)
+        out["s11_max_abs_err"] = float(np.max(np.abs(s - s6)))
+        out["band_GHz"] = [round(freqs[0] / 1e9, 3), round(freqs[-1] / 1e9, 3)]
+    e64.close()
+    log("fp64 check (%.1f s): %s" % (time.time() - t0, out))
+    return out
 
 
 if __name__ == "__main__":
