@@ -37,6 +37,11 @@
  *                                    Common/processcurrent.cpp:93-168)
  *   fdtd_calc_energy                Engine_Interface_SSE_FDTD::CalcFastEnergy
  *                                   (FDTD/engine_interface_sse_fdtd.cpp:37-69), in FP64
+ *   fdtd_set_edge_lengths,          ProcessFieldsFD (Common/processfields_fd.cpp:72-107) over
+ *   fdtd_add_dft_box, fdtd_read_dft a box of mesh nodes with NODE_INTERPOLATE
+ *                                   (FDTD/engine_interface_fdtd.cpp:63-158), FP64 values
+ *                                   and accumulators; six of them make the nf2ff box
+ *                                   (python/openEMS/nf2ff.py:61-77)
  *
  * Conventions: every function returns 0 on success and a negative FDTD_ERR_*
  * code on failure; fdtd_last_error() returns the message of the last failure
@@ -175,6 +180,22 @@ int  fdtd_probe_num_samples(fdtd_engine* h, int id, uint32_t* n);
 int  fdtd_read_probe(fdtd_engine* h, int id, double* t, double* val, uint32_t max_n);
 /* FD accumulators (weighted, FP64): re[n_freq], im[n_freq]. */
 int  fdtd_read_probe_fd(fdtd_engine* h, int id, double* re, double* im);
+
+/* ---- frequency-domain field boxes ---- */
+/* Edge lengths in metres per axis, concatenated x, y, z ([Nx+Ny+Nz] each):
+ * primary = Operator::GetEdgeLength(n, pos, false), dual = (..., true)
+ * (FDTD/operator.cpp:151-207).  Needed before fdtd_add_dft_box. */
+int  fdtd_set_edge_lengths(fdtd_engine* h, const double* primary, const double* dual);
+/* Accumulate E (field = FDTD_VOLT) or H (FDTD_CURR) node-interpolated
+ * fields of the mesh nodes start..stop (inclusive) every fd_interval steps
+ * at n_freq (<= 64) frequencies: X += F(T) * exp(-2 pi i f T) * 2 * dT * fd_interval,
+ * T = numTS*dT for E and (numTS+0.5)*dT for H.  Samples are taken at every
+ * numTS divisible by fd_interval, numTS = 0 included. */
+int  fdtd_add_dft_box(fdtd_engine* h, int field, const uint32_t start[3], const uint32_t stop[3],
+                      const double* freqs, uint32_t n_freq, uint32_t fd_interval, int* id);
+int  fdtd_dft_num_samples(fdtd_engine* h, int id, uint64_t* n);
+/* Copies the accumulators, [n_freq][3][nx][ny][nz][2] doubles (re, im), z fastest. */
+int  fdtd_read_dft(fdtd_engine* h, int id, double* out);
 
 /* ---- time stepping ---- */
 int  fdtd_iterate(fdtd_engine* h, uint32_t nsteps);

@@ -86,6 +86,9 @@ int  wop_build(wop_operator* op, int prec, uint32_t nr_ts);
 
 /* results */
 int  wop_get_num_lines(wop_operator* op, uint32_t n[3]);
+/* Operator::GetEdgeLength(n, pos, dual) in metres for every line of each axis,
+ * concatenated x, y, z: out[Nx+Ny+Nz] (FDTD/operator.cpp:151-207). */
+int  wop_get_edge_lengths(wop_operator* op, int dual, double* out);
 int  wop_get_timestep(wop_operator* op, double* dT);
 /* Nyquist number of the excitation (Excitation::GetNyquistNum) and the
  * step of maximum excitation amplitude (GetMaxExcitationTimestep) */

@@ -50,6 +50,11 @@ _ENGINE_SIGS = {
     "fdtd_probe_num_samples": [c_void_p, c_int, POINTER(c_uint32)],
     "fdtd_read_probe": [c_void_p, c_int, POINTER(c_double), POINTER(c_double), c_uint32],
     "fdtd_read_probe_fd": [c_void_p, c_int, POINTER(c_double), POINTER(c_double)],
+    "fdtd_set_edge_lengths": [c_void_p, POINTER(c_double), POINTER(c_double)],
+    "fdtd_add_dft_box": [c_void_p, c_int, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_double), c_uint32, c_uint32,
+                         POINTER(c_int)],
+    "fdtd_dft_num_samples": [c_void_p, c_int, POINTER(c_uint64)],
+    "fdtd_read_dft": [c_void_p, c_int, POINTER(c_double)],
     "fdtd_iterate": [c_void_p, c_uint32],
     "fdtd_synchronize": [c_void_p],
     "fdtd_get_num_timesteps": [c_void_p, POINTER(c_uint32)],
@@ -95,6 +100,7 @@ _OPERATOR_SIGS = {
     "wop_build": [c_void_p, c_int, c_uint32],
     "wop_get_num_lines": [c_void_p, POINTER(c_uint32)],
     "wop_get_timestep": [c_void_p, POINTER(c_double)],
+    "wop_get_edge_lengths": [c_void_p, c_int, POINTER(c_double)],
     "wop_get_excite_info": [c_void_p, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_uint32)],
     "wop_get_signal": [c_void_p, POINTER(c_double), POINTER(c_double)],
     "wop_get_coeffs": [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p],

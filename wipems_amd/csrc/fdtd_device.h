@@ -943,6 +943,111 @@ __global__ void k_step_end(const S* __restrict__ V, const S* __restrict__ I, uns
 }
 
 // ---------------------------------------------------------------------------
+// Frequency-domain field accumulation over a box of mesh nodes
+// (ProcessFieldsFD::Process, Common/processfields_fd.cpp:72-107) with node
+// interpolation (Engine_Interface_FDTD::GetRawInterpolatedField and
+// GetRawInterpolatedDualField, NODE_INTERPOLATE branches,
+// FDTD/engine_interface_fdtd.cpp:72-97 and 134-158).  E = V / primary edge
+// length, H = I / dual edge length (:257-266, :115-124).  Values and
+// accumulators are FP64; the accumulator layout is [n_freq][3][nodes] of
+// (re, im), nodes ordered z fastest.
+// ---------------------------------------------------------------------------
+#define MAX_DFT_FREQ 64
+
+struct DftAxes {
+    const double* len;     // [Nx+Ny+Nz] edge lengths in metres: primary for E, dual for H
+    int off[3];
+    int N[3];
+};
+
+struct DftDev {
+    int field;             // 0: E from V, 1: H from I
+    int s[3], n[3];        // first node and node count per axis
+    long long nodes;
+    int n_freq;
+    const double* freqs;
+    double2* acc;          // [n_freq][3][nodes]
+};
+
+template <typename S>
+__device__ __forceinline__ double dft_raw(const S* __restrict__ F, const Dims& d, const DftAxes& ax, int n, int x, int y, int z,
+                                          typename StoreTraits<S>::A scale) {
+    typedef StoreTraits<S> T;
+    const int p = n == 0 ? x : (n == 1 ? y : z);
+    const double delta = ax.len[ax.off[n] + p];
+    if (delta == 0.0) return 0.0;
+    const double v = (double)T::load(F + n * d.sc + x * d.sx + y * d.sy + z, scale);
+    return v / delta;
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256)
+k_dft(const S* __restrict__ F, Dims d, Scales<typename StoreTraits<S>::A> sc, DftAxes ax, DftDev b, double t, double g) {
+    __shared__ double2 ph[MAX_DFT_FREQ];
+    for (int k = threadIdx.x; k < b.n_freq; k += blockDim.x) {
+        double sn, cs;
+        sincos(-2.0 * M_PI * b.freqs[k] * t, &sn, &cs);
+        // exp(-2 pi i f T) * 2 * (dT * FD_Interval), in the reference's order
+        ph[k] = make_double2((cs * 2.0) * g, (sn * 2.0) * g);
+    }
+    __syncthreads();
+    const long long node = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= b.nodes) return;
+    const int k = (int)(node % b.n[2]);
+    const long long r = node / b.n[2];
+    const int j = (int)(r % b.n[1]);
+    const int i = (int)(r / b.n[1]);
+    const int pos[3] = {b.s[0] + i, b.s[1] + j, b.s[2] + k};
+    double out[3];
+    if (b.field == 0) {
+        for (int n = 0; n < 3; ++n) {
+            const typename StoreTraits<S>::A scale = sc.v[n];
+            int q[3] = {pos[0], pos[1], pos[2]};
+            if (ax.N[n] < 2) { out[n] = 0.0; continue; }
+            if (pos[n] == ax.N[n] - 1) {           // upper bound: the lower value only
+                --q[n];
+                out[n] = dft_raw<S>(F, d, ax, n, q[0], q[1], q[2], scale);
+                continue;
+            }
+            const double delta = ax.len[ax.off[n] + pos[n]];
+            out[n] = dft_raw<S>(F, d, ax, n, q[0], q[1], q[2], scale);
+            if (delta == 0.0) { out[n] = 0.0; continue; }
+            if (pos[n] == 0) continue;             // lower bound: the upper value only
+            --q[n];
+            const double deltaDown = ax.len[ax.off[n] + q[n]];
+            const double rel = delta / (delta + deltaDown);
+            out[n] = out[n] * (1.0 - rel) + dft_raw<S>(F, d, ax, n, q[0], q[1], q[2], scale) * rel;
+        }
+    } else {
+        const bool edge = pos[0] == ax.N[0] - 1 || pos[1] == ax.N[1] - 1 || pos[2] == ax.N[2] - 1;
+        for (int n = 0; n < 3; ++n) {
+            const int nP = (n + 1) % 3, nPP = (n + 2) % 3;
+            const typename StoreTraits<S>::A scale = sc.c[n];
+            if (edge || pos[nP] == 0 || pos[nPP] == 0) { out[n] = 0.0; continue; }
+            int q[3] = {pos[0], pos[1], pos[2]};
+            double o = dft_raw<S>(F, d, ax, n, q[0], q[1], q[2], scale);
+            --q[nP];
+            o += dft_raw<S>(F, d, ax, n, q[0], q[1], q[2], scale);
+            --q[nPP];
+            o += dft_raw<S>(F, d, ax, n, q[0], q[1], q[2], scale);
+            ++q[nP];
+            o += dft_raw<S>(F, d, ax, n, q[0], q[1], q[2], scale);
+            out[n] = o / 4.0;
+        }
+    }
+    for (int f = 0; f < b.n_freq; ++f) {
+        const double2 p = ph[f];
+        for (int n = 0; n < 3; ++n) {
+            double2* a = b.acc + ((long long)f * 3 + n) * b.nodes + node;
+            double2 v = *a;
+            v.x += out[n] * p.x;
+            v.y += out[n] * p.y;
+            *a = v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // energy: eps0*sum(V^2) + mu0*sum(I^2) over x<Nx-1, y<Ny-1, all z
 // (Engine_Interface_SSE_FDTD::CalcFastEnergy, engine_interface_sse_fdtd.cpp:37-69),
 // accumulated in FP64, deterministic two-pass reduction.

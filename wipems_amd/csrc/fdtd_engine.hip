@@ -26,6 +26,7 @@
 // SSE engines set FTZ|DAZ, tools/denormal.h:19-31), so an FP32 handle
 // reproduces the reference FP32 engine bit for bit.
 #include "fdtd_internal.h"
+#include <memory>
 
 thread_local std::string g_last_error;
 
@@ -460,6 +461,48 @@ int energy_impl(fdtd_engine* h, double* out) {
     HIPCHK(hipMemcpyAsync(r, h->e_out.p, sizeof(r), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     *out = kEPS0 * r[0] + kMUE0 * r[1];
+    return FDTD_OK;
+}
+
+template <typename S>
+int launch_dft(fdtd_engine* h, DftHost* b, uint32_t ts) {
+    typedef typename StoreTraits<S>::A A;
+    const int tot = (int)(h->N[0] + h->N[1] + h->N[2]);
+    DftAxes ax;
+    ax.len = h->edge_len.as<double>() + (b->field ? tot : 0);
+    ax.off[0] = 0; ax.off[1] = (int)h->N[0]; ax.off[2] = (int)(h->N[0] + h->N[1]);
+    for (int n = 0; n < 3; ++n) ax.N[n] = (int)h->N[n];
+    DftDev dd;
+    dd.field = b->field;
+    for (int n = 0; n < 3; ++n) { dd.s[n] = (int)b->s[n]; dd.n[n] = (int)b->n[n]; }
+    dd.nodes = b->nodes;
+    dd.n_freq = (int)b->freqs.size();
+    dd.freqs = b->dfreq.as<double>();
+    dd.acc = b->acc.as<double2>();
+    // Engine_Interface_FDTD::GetTime (engine_interface_fdtd.h:52); H uses the dual time
+    const double t = ((double)ts + (b->field ? 0.5 : 0.0)) * h->dT;
+    const double g = h->dT * (double)b->interval;
+    const long long blocks = (b->nodes + 255) / 256;
+    if (blocks > 0x7fffffffLL) return fail(FDTD_ERR_RANGE, "DFT box too large");
+    const S* F = b->field ? h->I.as<S>() : h->V.as<S>();
+    hipLaunchKernelGGL((k_dft<S>), dim3((unsigned)blocks), dim3(256), 0, h->stream, F, h->d, make_scales<A>(h), ax, dd, t, g);
+    ++b->samples;
+    return FDTD_OK;
+}
+
+int launch_dfts(fdtd_engine* h, uint32_t ts) {
+    for (auto b : h->dft) {
+        if (ts % b->interval) continue;
+        int rc;
+        switch (h->prec) {
+        case FDTD_PREC_FP32: rc = launch_dft<float>(h, b, ts); break;
+        case FDTD_PREC_FP64: rc = launch_dft<double>(h, b, ts); break;
+        case FDTD_PREC_FP16: rc = launch_dft<__half>(h, b, ts); break;
+        default: rc = launch_dft<__hip_bfloat16>(h, b, ts); break;
+        }
+        if (rc) return rc;
+    }
+    HIPCHK(hipGetLastError());
     return FDTD_OK;
 }
 
@@ -971,6 +1014,69 @@ int fdtd_read_probe_fd(fdtd_engine* h, int id, double* re, double* im) {
     return FDTD_OK;
 }
 
+int fdtd_set_edge_lengths(fdtd_engine* h, const double* primary, const double* dual) {
+    if (!h || !primary || !dual) return fail(FDTD_ERR_ARG, "null argument");
+    const size_t tot = (size_t)h->N[0] + h->N[1] + h->N[2];
+    std::vector<double> buf(2 * tot);
+    for (size_t k = 0; k < tot; ++k) {
+        if (!(primary[k] >= 0.0) || !(dual[k] >= 0.0)) return fail(FDTD_ERR_ARG, "edge lengths must be finite and >= 0");
+        buf[k] = primary[k];
+        buf[tot + k] = dual[k];
+    }
+    int rc;
+    if (!h->edge_len.p && (rc = h->edge_len.alloc(buf.size() * sizeof(double)))) return rc;
+    HIPCHK(hipMemcpyAsync(h->edge_len.p, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->has_edges = true;
+    return FDTD_OK;
+}
+
+int fdtd_add_dft_box(fdtd_engine* h, int field, const uint32_t start[3], const uint32_t stop[3],
+                     const double* freqs, uint32_t n_freq, uint32_t fd_interval, int* id) {
+    if (!h || !start || !stop || !freqs || !id) return fail(FDTD_ERR_ARG, "null argument");
+    if (field != FDTD_VOLT && field != FDTD_CURR) return fail(FDTD_ERR_ARG, "field must be FDTD_VOLT (E) or FDTD_CURR (H)");
+    if (!h->has_edges) return fail(FDTD_ERR_STATE, "fdtd_set_edge_lengths must be called before fdtd_add_dft_box");
+    if (n_freq < 1 || n_freq > MAX_DFT_FREQ) return fail(FDTD_ERR_RANGE, "n_freq must be 1..%d", MAX_DFT_FREQ);
+    if (fd_interval < 1) return fail(FDTD_ERR_ARG, "fd_interval must be >= 1");
+    auto b = std::unique_ptr<DftHost>(new DftHost());
+    b->field = field == FDTD_CURR ? 1 : 0;
+    b->nodes = 1;
+    for (int n = 0; n < 3; ++n) {
+        if (start[n] > stop[n] || stop[n] >= h->N[n]) return fail(FDTD_ERR_RANGE, "DFT box out of the mesh on axis %d", n);
+        b->s[n] = start[n];
+        b->n[n] = stop[n] - start[n] + 1;
+        b->nodes *= b->n[n];
+    }
+    b->interval = fd_interval;
+    b->freqs.assign(freqs, freqs + n_freq);
+    int rc;
+    const size_t bytes = (size_t)n_freq * 3 * b->nodes * 2 * sizeof(double);
+    if ((rc = b->acc.alloc(bytes))) return rc;
+    if ((rc = b->dfreq.alloc(n_freq * sizeof(double)))) return rc;
+    HIPCHK(hipMemsetAsync(b->acc.p, 0, bytes, h->stream));
+    HIPCHK(hipMemcpyAsync(b->dfreq.p, freqs, n_freq * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    *id = (int)h->dft.size();
+    h->dft.push_back(b.release());
+    return FDTD_OK;
+}
+
+int fdtd_dft_num_samples(fdtd_engine* h, int id, uint64_t* n) {
+    if (!h || !n) return fail(FDTD_ERR_ARG, "null argument");
+    if (id < 0 || id >= (int)h->dft.size()) return fail(FDTD_ERR_RANGE, "bad DFT box id");
+    *n = (uint64_t)h->dft[id]->samples;
+    return FDTD_OK;
+}
+
+int fdtd_read_dft(fdtd_engine* h, int id, double* out) {
+    if (!h || !out) return fail(FDTD_ERR_ARG, "null argument");
+    if (id < 0 || id >= (int)h->dft.size()) return fail(FDTD_ERR_RANGE, "bad DFT box id");
+    DftHost* b = h->dft[id];
+    HIPCHK(hipMemcpyAsync(out, b->acc.p, b->acc.bytes, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return FDTD_OK;
+}
+
 int fdtd_set_tuning(fdtd_engine* h, int variant, int tile_y, int chunk_x) {
     if (!h) return fail(FDTD_ERR_ARG, "null handle");
     if (variant < 0 || variant > 3) return fail(FDTD_ERR_ARG, "unknown kernel variant %d", variant);
@@ -1006,6 +1112,8 @@ int fdtd_get_kernel_times(fdtd_engine* h, double* volt_ms, double* curr_ms, uint
     return FDTD_OK;
 }
 
+static int advance(fdtd_engine* h, uint32_t nsteps);
+
 int fdtd_iterate(fdtd_engine* h, uint32_t nsteps) {
     if (!h) return fail(FDTD_ERR_ARG, "null handle");
     int rc = ready_check(h);
@@ -1021,9 +1129,27 @@ int fdtd_iterate(fdtd_engine* h, uint32_t nsteps) {
         default: rc = launch_step_end<__hip_bfloat16>(h, 0); break;
         }
         if (rc) return rc;
+        if ((rc = launch_dfts(h, 0))) return rc;
         h->sampled0 = true;
     }
     if (!nsteps) return FDTD_OK;
+    if (h->dft.empty()) return advance(h, nsteps);
+    // run to each DFT sample step in turn, then accumulate the boxes due there
+    uint32_t left = nsteps;
+    while (left) {
+        uint64_t next = UINT64_MAX;
+        for (auto b : h->dft) next = std::min(next, ((uint64_t)h->numTS / b->interval + 1) * b->interval);
+        const uint32_t run = (uint32_t)std::min<uint64_t>(left, next - h->numTS);
+        if ((rc = advance(h, run))) return rc;
+        left -= run;
+        if ((uint64_t)h->numTS == next && (rc = launch_dfts(h, h->numTS))) return rc;
+    }
+    return FDTD_OK;
+}
+
+// nsteps timesteps with no field-box sampling in between
+static int advance(fdtd_engine* h, uint32_t nsteps) {
+    int rc;
     if (h->timing) {
         // events for every launch of the two main kernels, harvested after the loop
         const size_t need = (size_t)4 * nsteps;

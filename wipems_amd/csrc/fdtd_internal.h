@@ -56,6 +56,16 @@ struct ProbeHost {
     long long td_slots = 0;
 };
 
+struct DftHost {
+    int field = 0;                 // 0: E, 1: H
+    uint32_t s[3] = {0, 0, 0}, n[3] = {0, 0, 0};
+    uint32_t interval = 0;
+    long long nodes = 0;
+    long long samples = 0;
+    std::vector<double> freqs;
+    DevBuf acc, dfreq;
+};
+
 struct AdeHost {
     uint32_t count = 0;
     uint32_t flags = 0;
@@ -67,6 +77,7 @@ struct AdeHost {
 using wipems::DevBuf;
 using wipems::ProbeHost;
 using wipems::AdeHost;
+using wipems::DftHost;
 
 struct fdtd_engine {
     int prec = FDTD_PREC_FP32;
@@ -116,6 +127,11 @@ struct fdtd_engine {
     bool probes_dirty = true;
     bool sampled0 = false;
 
+    // frequency-domain field boxes
+    std::vector<DftHost*> dft;
+    DevBuf edge_len;              // [2][Nx+Ny+Nz]: primary, then dual edge lengths (metres)
+    bool has_edges = false;
+
     // energy
     DevBuf e_part, e_out;
     int e_blocks = 0;
@@ -148,6 +164,7 @@ struct fdtd_engine {
         for (auto e : ev) (void)hipEventDestroy(e);
         for (auto p : probes) delete p;
         for (auto a : ade) delete a;
+        for (auto b : dft) delete b;
         if (stream) (void)hipStreamDestroy(stream);
     }
 };

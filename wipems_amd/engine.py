@@ -71,6 +71,7 @@ class Engine:
         self.dT = float(dT)
         self.precision = prec
         self.probe_ids = []
+        self.dft_boxes = []
         self._keep = []
 
     def close(self):
@@ -152,6 +153,34 @@ class Engine:
                                            int(p.td_interval), int(p.fd_interval), _dp(fr), len(p.freqs), byref(i)))
         self.probe_ids.append(i.value)
         return i.value
+
+    # ---- frequency-domain field boxes (ProcessFieldsFD) ----
+    def set_edge_lengths(self, primary, dual):
+        primary = np.ascontiguousarray(primary, dtype=np.float64)
+        dual = np.ascontiguousarray(dual, dtype=np.float64)
+        self._chk(self._lib.fdtd_set_edge_lengths(self._h, _dp(primary), _dp(dual)))
+
+    def add_dft_box(self, b):
+        fr = np.ascontiguousarray(b.freqs, dtype=np.float64)
+        st = (c_uint32 * 3)(*b.start)
+        sp = (c_uint32 * 3)(*b.stop)
+        i = c_int()
+        self._chk(self._lib.fdtd_add_dft_box(self._h, int(b.field), st, sp, _dp(fr), len(fr), int(b.fd_interval), byref(i)))
+        self.dft_boxes.append(b)
+        return i.value
+
+    def dft_num_samples(self, bid):
+        n = c_uint64()
+        self._chk(self._lib.fdtd_dft_num_samples(self._h, int(bid), byref(n)))
+        return n.value
+
+    def read_dft(self, bid):
+        """Complex accumulators [n_freq][3][nx][ny][nz] of box bid."""
+        b = self.dft_boxes[bid]
+        shape = [len(b.freqs), 3] + [b.stop[n] - b.start[n] + 1 for n in range(3)]
+        out = np.empty(shape + [2], dtype=np.float64)
+        self._chk(self._lib.fdtd_read_dft(self._h, int(bid), _dp(out)))
+        return out[..., 0] + 1j * out[..., 1]
 
     # ---- stepping (Engine::IterateTS) ----
     def IterateTS(self, n):
@@ -297,6 +326,10 @@ def engine_from_operator(data: OperatorData, precision="fp32", compressed=None, 
             eng.set_tfsf(which, t["pos"], t["comp"], t["delay"], t["delta"], t["amp"], data.tfsf["max_delay"])
     for p in data.probes:
         eng.add_probe(p)
+    if data.dft_boxes:
+        eng.set_edge_lengths(data.edge_primary, data.edge_dual)
+        for b in data.dft_boxes:
+            eng.add_dft_box(b)
     return eng
 
 

@@ -1194,6 +1194,19 @@ int wop_get_num_lines(wop_operator* op, uint32_t n[3]) {
     return 0;
 }
 
+int wop_get_edge_lengths(wop_operator* op, int dual, double* out) {
+    if (!op || !out) return fail(-1, "bad argument");
+    if (op->lines[0].empty() || op->lines[1].empty() || op->lines[2].empty()) return fail(-1, "mesh not set");
+    size_t k = 0;
+    for (int n = 0; n < 3; ++n)
+        for (uint32_t i = 0; i < op->N[n]; ++i) {
+            uint32_t pos[3] = {0, 0, 0};
+            pos[n] = i;
+            out[k++] = op->EdgeLength(n, pos, dual != 0);
+        }
+    return 0;
+}
+
 int wop_get_timestep(wop_operator* op, double* dT) {
     if (!op || !dT || !op->built) return fail(-1, "operator not built");
     *dT = op->dT;

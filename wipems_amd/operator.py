@@ -10,6 +10,7 @@ orders (Operator_Ext_LorentzMaterial).
 from __future__ import annotations
 
 import ctypes
+import math
 from ctypes import POINTER, byref, c_double, c_int, c_uint32, c_void_p
 from dataclasses import dataclass, field
 from typing import List, Optional
@@ -79,6 +80,36 @@ class ProbeDef:
 
 
 @dataclass
+class DftBoxDef:
+    """Frequency-domain field dump over a box of mesh nodes (ProcessFieldsFD,
+    NODE_INTERPOLATE).  field: 0 = E, 1 = H."""
+    field: int
+    start: tuple
+    stop: tuple
+    freqs: tuple
+    fd_interval: int
+    name: str = ""
+
+
+def fd_interval_for(freqs, dT, nyquist):
+    """Processing::AddFrequency (Common/processing.cpp:150-170): the FD
+    interval starts at the excitation's Nyquist step count and shrinks to
+    CalcNyquistNum(f, dT) = floor(1/f/2/dT) of the highest frequency."""
+    interval = 0
+    for f in freqs:
+        nts = int(math.floor(1.0 / f / 2.0 / dT)) if f != 0 else 2 ** 32 - 1
+        if nts == 0:
+            continue
+        if interval == 0:
+            interval = nyquist
+        if interval > nts:
+            interval = nts
+    if interval == 0:
+        raise ValueError("no frequency below the time-step limit 1/(2 dT)")
+    return interval
+
+
+@dataclass
 class OperatorData:
     num_lines: tuple
     dT: float
@@ -97,6 +128,9 @@ class OperatorData:
     index: Optional[np.ndarray] = None      # compressed form, filled by compress()
     table: Optional[np.ndarray] = None
     source: object = None                   # the Operator that built this data (native compression)
+    edge_primary: Optional[np.ndarray] = None   # [Nx+Ny+Nz] primary edge lengths (m)
+    edge_dual: Optional[np.ndarray] = None      # [Nx+Ny+Nz] dual edge lengths (m)
+    dft_boxes: List[DftBoxDef] = field(default_factory=list)
 
     @property
     def cells(self):
@@ -281,8 +315,15 @@ class Operator:
             self._chk(lib.wop_get_ade(h, o, _up(pos), 1 if prec == 1 else 0, *[_vp(a) for a in lists]))
             ades.append(AdeOrder(pos, lists, fl.value))
         self._built_prec = prec
+        tot = int(n[0]) + int(n[1]) + int(n[2])
+        edges = []
+        for dual in (0, 1):
+            ebuf = np.empty(tot, dtype=np.float64)
+            self._chk(lib.wop_get_edge_lengths(h, dual, _dp(ebuf)))
+            edges.append(ebuf)
+        self._edges = edges
         return OperatorData(tuple(int(x) for x in n), dT.value, prec, *arrs, exc, slabs, ades, nyq.value, mstep.value,
-                            source=self)
+                            source=self, edge_primary=edges[0], edge_dual=edges[1])
 
     def compress(self, max_entries=0):
         """Native per-cell compression of the last build (wop_compress); None if
@@ -304,6 +345,31 @@ class Operator:
         shape = (3,) + tuple(n)
         out = [np.empty(shape) for _ in range(4)]
         self._chk(self._lib.wop_get_ec(self._h, *[_dp(a) for a in out]))
+        return out
+
+    # ---- field-box helpers ----
+    def nf2ff_box(self, data, start, stop, freqs, directions=(True,) * 6, name="nf2ff"):
+        """The six faces of an nf2ff recording box (python/openEMS/nf2ff.py:61-77):
+        one E and one H frequency-domain dump per enabled face, node
+        interpolated (dump_mode 1), coordinates snapped to the main mesh.
+        Appends the boxes to data.dft_boxes and returns them."""
+        s = [self.snap(n, min(start[n], stop[n]), False)[0] for n in range(3)]
+        e = [self.snap(n, max(start[n], stop[n]), False)[0] for n in range(3)]
+        interval = fd_interval_for(freqs, data.dT, data.nyquist)
+        out = []
+        for ny in range(3):
+            for side in (0, 1):
+                if not directions[2 * ny + side]:
+                    continue
+                a, b = list(s), list(e)
+                if side == 0:
+                    b[ny] = a[ny]
+                else:
+                    a[ny] = b[ny]
+                for fld, tag in ((0, "E"), (1, "H")):
+                    out.append(DftBoxDef(fld, tuple(a), tuple(b), tuple(float(f) for f in freqs), interval,
+                                         "%s_%s_%s%s" % (name, tag, "xyz"[ny], "-+"[side])))
+        data.dft_boxes.extend(out)
         return out
 
     # ---- probe helpers (ProcessIntegral::DefineStartStopCoord snapping) ----
