@@ -75,6 +75,11 @@ int  wop_add_lumped_element(wop_operator* op, int ny, double R, double C, int ca
 /* excitation property: type 0 soft E, 1 hard E, 2 soft H, 3 hard H; value vector; delay (s) */
 int  wop_add_excitation(wop_operator* op, int type, const double value[3], double delay, int* id);
 int  wop_add_box(wop_operator* op, int prop, const double start[3], const double stop[3], int priority);
+/* plane-wave (TFSF) excitation on the faces of a box (CSPropExcitation type 10
+ * with one box primitive): propagation direction, E vector, and the frequency
+ * for the numerical phase velocity (<= 0: c0).  One per operator. */
+int  wop_set_plane_wave(wop_operator* op, const double start[3], const double stop[3], const double prop_dir[3],
+                        const double e_vec[3], double freq);
 int  wop_add_sphere(wop_operator* op, int prop, const double center[3], double radius, int priority);
 
 /* excitation time signal */
@@ -105,6 +110,12 @@ int  wop_compress(wop_operator* op, uint32_t* n_entries, uint32_t* index, double
 int  wop_get_excitation_counts(wop_operator* op, uint32_t* volt_count, uint32_t* curr_count);
 int  wop_get_excitation(wop_operator* op, uint32_t* volt_pos, uint32_t* volt_dir, double* volt_amp, uint32_t* volt_delay,
                         uint32_t* curr_pos, uint32_t* curr_dir, double* curr_amp, uint32_t* curr_delay);
+
+/* TFSF injection lists (Operator_Ext_TFSF::BuildExtension, operator_ext_tfsf.cpp:82-410),
+ * flattened in the order Engine_Ext_TFSF applies them: which = 0 voltage list
+ * (after the voltage update), 1 current list.  pos is [3][count]. */
+int  wop_get_tfsf_info(wop_operator* op, uint32_t* volt_count, uint32_t* curr_count, uint32_t* max_delay);
+int  wop_get_tfsf(wop_operator* op, int which, uint32_t* pos, uint32_t* comp, uint32_t* delay, double* delta, double* amp);
 
 int  wop_get_num_upml(wop_operator* op, uint32_t* n);
 int  wop_get_upml_range(wop_operator* op, uint32_t k, uint32_t start[3], uint32_t num_lines[3]);

@@ -153,3 +153,17 @@ def test_vector_kernel_fp64(gpu):
     eng.iterate(100)
     orc.iterate(100)
     compare(eng, orc)
+
+
+@pytest.mark.parametrize("prec,precision,kw", [
+    (0, "fp32", dict(n_cells=30, margin=4)),
+    (0, "fp32", dict(n_cells=28, margin=3, prop_dir=(1.0, 1.0, 1.0), e_vec=(1.0, -1.0, 0.0), graded=True, phase_freq=5e9)),
+    (1, "fp64", dict(n_cells=26, margin=3, prop_dir=(0.0, -1.0, 0.0), e_vec=(1.0, 0.0, 0.0))),
+])
+def test_plane_wave_tfsf_bitwise(gpu, prec, precision, kw):
+    """TFSF injection (repeated edge locations at the box edges applied in
+    list order) against the oracle, bit for bit."""
+    sc = scenarios.plane_wave_box(steps=260, prec=prec, pml=6, **kw)
+    print("case: plane wave %s %s" % (precision, kw), flush=True)
+    eng, orc = run_pair(sc, 260, precision=precision, compressed=None, chunks=(5, 31))
+    compare(eng, orc)

@@ -96,6 +96,10 @@ _OPERATOR_SIGS = {
     "wop_add_excitation": [c_void_p, c_int, POINTER(c_double), c_double, POINTER(c_int)],
     "wop_add_box": [c_void_p, c_int, POINTER(c_double), POINTER(c_double), c_int],
     "wop_add_sphere": [c_void_p, c_int, POINTER(c_double), c_double, c_int],
+    "wop_set_plane_wave": [c_void_p, POINTER(c_double), POINTER(c_double), POINTER(c_double), POINTER(c_double), c_double],
+    "wop_get_tfsf_info": [c_void_p, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_uint32)],
+    "wop_get_tfsf": [c_void_p, c_int, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_uint32), POINTER(c_double),
+                     POINTER(c_double)],
     "wop_set_excite_signal": [c_void_p, c_int, c_double, c_double],
     "wop_build": [c_void_p, c_int, c_uint32],
     "wop_get_num_lines": [c_void_p, POINTER(c_uint32)],

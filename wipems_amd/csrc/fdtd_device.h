@@ -830,9 +830,14 @@ __global__ void k_ade_apply(S* F, AdeDev a, long long sc_field, Scales<typename 
     }
 }
 
+#define MAX_TFSF_ROUNDS 4
 struct TfsfDev {
     int count;
     int length, period, max_delay;
+    // entries are grouped into rounds with distinct field locations; rounds
+    // run one after another so repeated locations add in list order
+    int rounds;
+    int rstart[MAX_TFSF_ROUNDS + 1];
     const long long* off;   // element offset incl. component
     const int* comp;
     const int* delay;
@@ -845,11 +850,12 @@ struct TfsfDev {
 // F += (1-delta)*amp*s[L[d]] + delta*amp*s[L[d+1]] evaluated in double and
 // rounded once to the storage type.
 template <typename S>
-__global__ void k_tfsf(S* F, const unsigned* __restrict__ step, TfsfDev t, Scales<typename StoreTraits<S>::A> sc, int is_curr) {
+__global__ void k_tfsf(S* F, const unsigned* __restrict__ step, TfsfDev t, Scales<typename StoreTraits<S>::A> sc, int is_curr,
+                       int first, int end) {
     typedef StoreTraits<S> T;
     typedef typename T::A A;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= t.count) return;
+    const int i = first + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= end) return;
     const unsigned numTS = *step;
     const A* sig = (const A*)t.sig;
     const A* amp = (const A*)t.amp;

@@ -27,6 +27,7 @@
 // reproduces the reference FP32 engine bit for bit.
 #include "fdtd_internal.h"
 #include <memory>
+#include <unordered_map>
 
 thread_local std::string g_last_error;
 
@@ -201,6 +202,16 @@ int finalize_probes(fdtd_engine* h, uint32_t upto_ts) {
     h->probes_dirty = false;
     invalidate_graph(h);   // kernel arguments changed
     return FDTD_OK;
+}
+
+// one launch per round of distinct locations (TfsfDev::rstart)
+template <typename S>
+void launch_tfsf(fdtd_engine* h, S* F, const unsigned* stp, const TfsfDev& t, Scales<typename StoreTraits<S>::A> sc, int is_curr) {
+    for (int r = 0; r < t.rounds; ++r) {
+        const int first = t.rstart[r], end = t.rstart[r + 1];
+        if (end <= first) continue;
+        hipLaunchKernelGGL((k_tfsf<S>), dim3((end - first + 255) / 256), dim3(256), 0, h->stream, F, stp, t, sc, is_curr, first, end);
+    }
 }
 
 template <typename S>
@@ -393,7 +404,7 @@ int launch_step_kernels(fdtd_engine* h, bool timed) {
     else if (pml) launch_volt_cm<S, true>(h, grd, blk, cfe, pp); else launch_volt_cm<S, false>(h, grd, blk, cfe, pp);
     if (timed) HIPCHK(hipEventRecord(h->ev[evi + 1], h->stream));
     if (h->tfsf_v.count)
-        hipLaunchKernelGGL((k_tfsf<S>), dim3((h->tfsf_v.count + 255) / 256), dim3(256), 0, h->stream, h->V.as<S>(), stp, h->tfsf_v, sc, 0);
+        launch_tfsf<S>(h, h->V.as<S>(), stp, h->tfsf_v, sc, 0);
     for (auto a : h->ade)
         if ((a->flags & 1) && a->count)
             hipLaunchKernelGGL((k_ade_apply<S>), dim3((a->count + 255) / 256), dim3(256), 0, h->stream, h->V.as<S>(), ade_dev(a), h->d.sc, sc, 0);
@@ -411,7 +422,7 @@ int launch_step_kernels(fdtd_engine* h, bool timed) {
     else if (pml) launch_curr_cm<S, true>(h, grdc, blk, cfh, pp); else launch_curr_cm<S, false>(h, grdc, blk, cfh, pp);
     if (timed) HIPCHK(hipEventRecord(h->ev[evi + 3], h->stream));
     if (h->tfsf_c.count)
-        hipLaunchKernelGGL((k_tfsf<S>), dim3((h->tfsf_c.count + 255) / 256), dim3(256), 0, h->stream, h->I.as<S>(), stp, h->tfsf_c, sc, 1);
+        launch_tfsf<S>(h, h->I.as<S>(), stp, h->tfsf_c, sc, 1);
     for (auto a : h->ade)
         if ((a->flags & 2) && a->count)
             hipLaunchKernelGGL((k_ade_apply<S>), dim3((a->count + 255) / 256), dim3(256), 0, h->stream, h->I.as<S>(), ade_dev(a), h->d.sc, sc, 1);
@@ -873,23 +884,48 @@ int fdtd_set_tfsf(fdtd_engine* h, int which, uint32_t count, const uint32_t* pos
     std::vector<char> hb(bytes, 0);
     size_t o_off = 0, o_comp = (size_t)count * 8, o_del = o_comp + (size_t)count * 4;
     size_t o_dd = (o_del + (size_t)count * 4 + 7) / 8 * 8, o_amp = o_dd + (size_t)count * a;
+    // round of each entry = how often its location occurred before it; a
+    // stable sort by round keeps list order among repeats of one location
+    std::vector<long long> offs(count);
+    std::vector<int> round(count);
+    std::unordered_map<long long, int> seen;
+    int rounds = 0;
     for (uint32_t i = 0; i < count; ++i) {
         for (int n = 0; n < 3; ++n)
             if (pos[(size_t)n * count + i] >= h->N[n]) return fail(FDTD_ERR_RANGE, "TFSF position out of range");
         if (comp[i] > 2) return fail(FDTD_ERR_RANGE, "TFSF component");
-        if (delay[i] + 1 > max_delay + 1) return fail(FDTD_ERR_RANGE, "TFSF delay beyond max_delay");
-        long long off = (long long)comp[i] * h->d.sc + cell_off(h, pos[i], pos[count + i], pos[2 * (size_t)count + i]);
+        if (delay[i] > max_delay) return fail(FDTD_ERR_RANGE, "TFSF delay beyond max_delay");
+        offs[i] = (long long)comp[i] * h->d.sc + cell_off(h, pos[i], pos[count + i], pos[2 * (size_t)count + i]);
+        round[i] = seen[offs[i]]++;
+        rounds = std::max(rounds, round[i] + 1);
+    }
+    if (rounds > MAX_TFSF_ROUNDS) return fail(FDTD_ERR_RANGE, "a TFSF location repeats more than %d times", MAX_TFSF_ROUNDS);
+    std::vector<uint32_t> order(count);
+    for (uint32_t i = 0; i < count; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return round[x] < round[y]; });
+    for (uint32_t k = 0; k < count; ++k) {
+        const uint32_t i = order[k];
+        long long off = offs[i];
         int c = (int)comp[i], dl = (int)delay[i];
-        memcpy(hb.data() + o_off + i * 8, &off, 8);
-        memcpy(hb.data() + o_comp + i * 4, &c, 4);
-        memcpy(hb.data() + o_del + i * 4, &dl, 4);
-        if (a == 8) { memcpy(hb.data() + o_dd + i * 8, &delta[i], 8); memcpy(hb.data() + o_amp + i * 8, &amp[i], 8); }
-        else { float f = (float)delta[i], g = (float)amp[i]; memcpy(hb.data() + o_dd + i * 4, &f, 4); memcpy(hb.data() + o_amp + i * 4, &g, 4); }
+        memcpy(hb.data() + o_off + k * 8, &off, 8);
+        memcpy(hb.data() + o_comp + k * 4, &c, 4);
+        memcpy(hb.data() + o_del + k * 4, &dl, 4);
+        if (a == 8) { memcpy(hb.data() + o_dd + k * 8, &delta[i], 8); memcpy(hb.data() + o_amp + k * 8, &amp[i], 8); }
+        else { float f = (float)delta[i], g = (float)amp[i]; memcpy(hb.data() + o_dd + k * 4, &f, 4); memcpy(hb.data() + o_amp + k * 4, &g, 4); }
     }
     HIPCHK(hipMemcpyAsync(buf.p, hb.data(), bytes, hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     char* b = (char*)buf.p;
     t.count = (int)count;
+    t.rounds = rounds;
+    {
+        int k = 0;
+        for (int r = 0; r < rounds; ++r) {
+            t.rstart[r] = k;
+            while (k < (int)count && round[order[k]] == r) ++k;
+        }
+        t.rstart[rounds] = (int)count;
+    }
     t.length = h->exc_v.length;
     t.period = h->exc_v.period;
     t.max_delay = (int)max_delay;

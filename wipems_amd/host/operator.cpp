@@ -121,6 +121,13 @@ struct wop_operator {
     struct Exc { uint32_t pos[3]; uint32_t dir; double amp; uint32_t delay; };
     std::vector<Exc> exc_v, exc_i;
 
+    // plane-wave (TFSF) source: a box and the wave's direction, E vector and
+    // frequency for the numerical phase velocity (0: vacuum speed of light)
+    struct PlaneWave { bool set = false; double a[3], b[3], k[3], E[3], f = 0; } pw;
+    struct TfsfEntry { uint32_t pos[3]; uint32_t comp; uint32_t delay; double delta, amp; };
+    std::vector<TfsfEntry> tfsf_v, tfsf_i;
+    uint32_t tfsf_max_delay = 0;
+
     struct Upml {
         uint32_t start[3], n[3];
         std::vector<double> a[6];          // vv, vvfo, vvfn, ii, iifo, iifn [3][n]
@@ -839,6 +846,233 @@ struct Builder {
         }
     }
 
+    // Operator::CalcNumericPhaseVelocity (FDTD/operator.cpp:2087-2151)
+    double NumericPhaseVelocity(const uint32_t* start, const uint32_t* stop, const double* dir, float freq) const {
+        const double dT = op->dT;
+        const double c0 = C0;   // background eps_r = mue_r = 1
+        double disc[3];
+        for (int n = 0; n < 3; ++n)
+            disc[n] = std::fabs(op->DiscLine(n, start[n], false) - op->DiscLine(n, stop[n], false)) * op->gridDelta /
+                      std::fabs((double)(stop[n] - start[n]));
+        for (int n = 0; n < 3; ++n) {
+            const int nP = (n + 1) % 3, nPP = (n + 2) % 3;
+            if (std::fabs(dir[n]) == 1 && dir[nP] == 0 && dir[nPP] == 0) {
+                // the reference takes the x spacing for every Cartesian direction
+                double kx = asin(disc[0] / c0 / dT * sin(2 * PI_ * freq * dT / 2)) * 2 / disc[0];
+                return 2 * PI_ * freq / kx;
+            }
+        }
+        const double k0 = 2 * PI_ * freq / c0;
+        double k = k0;
+        const double RHS = pow(sin(2 * PI_ * freq * dT / 2) / c0 / dT, 2);
+        double old_phv = 0, phv = c0;
+        int it = 0;
+        while (std::fabs(old_phv - phv) > 1e-6) {
+            ++it;
+            old_phv = phv;
+            double fk = 0, fdk = 0;
+            for (int n = 0; n < 3; ++n) {
+                fk += pow(sin(dir[n] * k * disc[n] / 2) / disc[n], 2);
+                fdk += dir[n] * sin(dir[n] * k * disc[n] / 2) * cos(dir[n] * k * disc[n] / 2) / disc[n];
+            }
+            fk -= RHS;
+            k -= fk / fdk;
+            if (k < k0) k = k0;
+            phv = 2 * PI_ * freq / k;
+            if (it > 99) break;
+        }
+        return phv;
+    }
+
+    // Operator_Ext_TFSF::BuildExtension (operator_ext_tfsf.cpp:82-410).  The
+    // delay, fraction and amplitude tables are built in the reference's
+    // statement order (F rounding at its storage points) and flattened in the
+    // order Engine_Ext_TFSF applies them (engine_ext_tfsf.cpp:36-215): axis n,
+    // lower then upper plane, nP-major then nPP, component nP then nPP.
+    void Tfsf() {
+        op->tfsf_v.clear();
+        op->tfsf_i.clear();
+        op->tfsf_max_delay = 0;
+        const auto& pw = op->pw;
+        if (!pw.set) return;
+        const double dT = op->dT;
+        const uint32_t* N = op->N;
+        double k[3] = {pw.k[0], pw.k[1], pw.k[2]};
+        const double norm = sqrt(k[0] * k[0] + k[1] * k[1] + k[2] * k[2]);
+        if (norm == 0) return;
+        k[0] /= norm; k[1] /= norm; k[2] /= norm;
+        // SnapBox2Mesh (FDTD/operator.cpp:289-314), snap method 0, main mesh
+        uint32_t S[3], T[3];
+        int dims = 0;
+        for (int n = 0; n < 3; ++n) {
+            const double lo = fmin(pw.a[n], pw.b[n]), hi = fmax(pw.a[n], pw.b[n]);
+            const double mn = op->DiscLine(n, 0, false), mx = op->DiscLine(n, N[n] - 1, false);
+            if ((lo < mn && hi < mn) || (lo > mx && hi > mx)) return;
+        }
+        for (int n = 0; n < 3; ++n) {
+            bool in;
+            S[n] = op->SnapToMeshLine(n, fmin(pw.a[n], pw.b[n]), in, false);
+            T[n] = op->SnapToMeshLine(n, fmax(pw.a[n], pw.b[n]), in, false);
+            if (T[n] > S[n]) ++dims;
+        }
+        if (dims != 3) return;
+        double phv = C0;
+        if (pw.f > 0) phv = NumericPhaseVelocity(S, T, k, (float)pw.f);
+        if (phv < 0 || phv > C0 || std::isnan(phv)) phv = C0;
+
+        double origin[3];
+        uint32_t nl[3];
+        bool active[3][2];
+        double E[3] = {pw.E[0], pw.E[1], pw.E[2]}, H[3];
+        for (int n = 0; n < 3; ++n) {
+            nl[n] = T[n] - S[n] + 1;
+            active[n][0] = S[n] != 0;
+            active[n][1] = T[n] != N[n] - 1;
+            const uint32_t ui = k[n] >= 0 ? S[n] - 1 : T[n] + 1;   // wraps at the domain edge, as in the reference
+            origin[n] = op->DiscLine(n, ui, false);
+        }
+        const double dotEk = E[0] * k[0] + E[1] * k[1] + E[2] * k[2];
+        const double angle = acos(dotEk / (E[0] * E[0] + E[1] * E[1] + E[2] * E[2])) / PI_ * 180;
+        if (angle == 0) return;
+        if (angle != 90)
+            for (int n = 0; n < 3; ++n) E[n] -= k[n] * dotEk;
+        for (int n = 0; n < 3; ++n) {
+            const int nP = (n + 1) % 3, nPP = (n + 2) % 3;
+            H[n] = k[nP] * E[nPP] - k[nPP] * E[nP];
+            H[n] /= Z0 * sqrt(1.0 / 1.0);
+        }
+        const double unit = op->gridDelta;
+        uint32_t maxDelay = 0;
+        auto delay_of = [&](const uint32_t* pos, int comp, bool dual, double extra) {
+            double c[3];
+            op->YeeCoords(comp, pos, c, dual);
+            const double dist = std::fabs((c[0] - origin[0]) * k[0]) + std::fabs((c[1] - origin[1]) * k[1]) +
+                                std::fabs((c[2] - origin[2]) * k[2]);
+            const double d = dist * unit / phv / dT + extra;
+            maxDelay = std::max((uint32_t)d, maxDelay);
+            return d;
+        };
+        struct Tab { std::vector<uint32_t> delay; std::vector<F> delta, amp; };
+        for (int n = 0; n < 3; ++n) {
+            const int nP = (n + 1) % 3, nPP = (n + 2) % 3;
+            if (!active[n][0] && !active[n][1]) continue;
+            const size_t numP = (size_t)nl[nP] * nl[nPP];
+            Tab Vt[2][2], Ct[2][2];
+            for (int l = 0; l < 2; ++l)
+                for (int c = 0; c < 2; ++c)
+                    if (active[n][l]) {
+                        Vt[l][c].delay.assign(numP, 0); Vt[l][c].delta.assign(numP, 0); Vt[l][c].amp.assign(numP, 0);
+                        Ct[l][c].delay.assign(numP, 0); Ct[l][c].delta.assign(numP, 0); Ct[l][c].amp.assign(numP, 0);
+                    }
+            auto put = [](Tab& t, size_t u, double d) {
+                t.delay[u] = (uint32_t)floor(d);
+                t.delta[u] = (F)(d - floor(d));
+            };
+            uint32_t pos[3];
+            pos[n] = 0;
+            pos[nP] = S[nP];
+            size_t u = 0;
+            for (uint32_t i = 0; i < nl[nP]; ++i) {
+                pos[nPP] = S[nPP];
+                for (uint32_t j = 0; j < nl[nPP]; ++j) {
+                    // current injection tables
+                    pos[n] = S[n];
+                    if (active[n][0]) {
+                        put(Ct[0][1], u, delay_of(pos, nP, false, 0.0));
+                        Ct[0][1].amp[u] = (F)(E[nP] * op->EdgeLength(nP, pos));
+                        put(Ct[0][0], u, delay_of(pos, nPP, false, 0.0));
+                        Ct[0][0].amp[u] = (F)(E[nPP] * op->EdgeLength(nPP, pos));
+                        --pos[n];
+                        const size_t q = op->idx(pos[0], pos[1], pos[2]);
+                        Ct[0][0].amp[u] = Ct[0][0].amp[u] * IV(nP, q);
+                        Ct[0][1].amp[u] = Ct[0][1].amp[u] * IV(nPP, q);
+                    }
+                    if (active[n][1]) {
+                        pos[n] = T[n];
+                        put(Ct[1][1], u, delay_of(pos, nP, false, 0.0));
+                        Ct[1][1].amp[u] = (F)(E[nP] * op->EdgeLength(nP, pos));
+                        put(Ct[1][0], u, delay_of(pos, nPP, false, 0.0));
+                        Ct[1][0].amp[u] = (F)(E[nPP] * op->EdgeLength(nPP, pos));
+                        const size_t q = op->idx(pos[0], pos[1], pos[2]);
+                        Ct[1][0].amp[u] = Ct[1][0].amp[u] * IV(nP, q);
+                        Ct[1][1].amp[u] = Ct[1][1].amp[u] * IV(nPP, q);
+                    }
+                    if (active[n][0]) Ct[0][0].amp[u] = Ct[0][0].amp[u] * (F)-1;
+                    if (active[n][1]) Ct[1][1].amp[u] = Ct[1][1].amp[u] * (F)-1;
+                    if (pos[nP] == T[nP]) {
+                        if (active[n][0]) Ct[0][1].amp[u] = 0;
+                        if (active[n][1]) Ct[1][1].amp[u] = 0;
+                    }
+                    if (pos[nPP] == T[nPP]) {
+                        if (active[n][0]) Ct[0][0].amp[u] = 0;
+                        if (active[n][1]) Ct[1][0].amp[u] = 0;
+                    }
+                    // voltage injection tables
+                    pos[n] = S[n] - 1;
+                    if (active[n][0]) {
+                        put(Vt[0][1], u, delay_of(pos, nP, true, 1.0));
+                        Vt[0][1].amp[u] = (F)(H[nP] * op->EdgeLength(nP, pos, true));
+                        put(Vt[0][0], u, delay_of(pos, nPP, true, 1.0));
+                        Vt[0][0].amp[u] = (F)(H[nPP] * op->EdgeLength(nPP, pos, true));
+                        ++pos[n];
+                        const size_t q = op->idx(pos[0], pos[1], pos[2]);
+                        Vt[0][0].amp[u] = Vt[0][0].amp[u] * VI(nP, q);
+                        Vt[0][1].amp[u] = Vt[0][1].amp[u] * VI(nPP, q);
+                    }
+                    pos[n] = T[n];
+                    if (active[n][1]) {
+                        put(Vt[1][1], u, delay_of(pos, nP, true, 1.0));
+                        Vt[1][1].amp[u] = (F)(H[nP] * op->EdgeLength(nP, pos, true));
+                        put(Vt[1][0], u, delay_of(pos, nPP, true, 1.0));
+                        Vt[1][0].amp[u] = (F)(H[nPP] * op->EdgeLength(nPP, pos, true));
+                        const size_t q = op->idx(pos[0], pos[1], pos[2]);
+                        Vt[1][0].amp[u] = Vt[1][0].amp[u] * VI(nP, q);
+                        Vt[1][1].amp[u] = Vt[1][1].amp[u] * VI(nPP, q);
+                    }
+                    if (active[n][1]) Vt[1][0].amp[u] = Vt[1][0].amp[u] * (F)-1;
+                    if (active[n][0]) Vt[0][1].amp[u] = Vt[0][1].amp[u] * (F)-1;
+                    if (pos[nP] == T[nP]) {
+                        if (active[n][0]) Vt[0][0].amp[u] = 0;
+                        if (active[n][1]) Vt[1][0].amp[u] = 0;
+                    }
+                    if (pos[nPP] == T[nPP]) {
+                        if (active[n][0]) Vt[0][1].amp[u] = 0;
+                        if (active[n][1]) Vt[1][1].amp[u] = 0;
+                    }
+                    ++pos[nPP];
+                    ++u;
+                }
+                ++pos[nP];
+            }
+            // flatten in the engine's application order
+            for (int l = 0; l < 2; ++l) {
+                if (!active[n][l]) continue;
+                for (int which = 0; which < 2; ++which) {
+                    Tab (&tb)[2][2] = which == 0 ? Vt : Ct;
+                    auto& out = which == 0 ? op->tfsf_v : op->tfsf_i;
+                    uint32_t p[3];
+                    size_t uu = 0;
+                    for (uint32_t i = 0; i < nl[nP]; ++i)
+                        for (uint32_t j = 0; j < nl[nPP]; ++j, ++uu) {
+                            p[nP] = S[nP] + i;
+                            p[nPP] = S[nPP] + j;
+                            p[n] = l == 0 ? (which == 0 ? S[n] : S[n] - 1) : T[n];
+                            for (int c = 0; c < 2; ++c) {
+                                wop_operator::TfsfEntry e;
+                                for (int a = 0; a < 3; ++a) e.pos[a] = p[a];
+                                e.comp = c == 0 ? nP : nPP;
+                                e.delay = tb[l][c].delay[uu];
+                                e.delta = (double)tb[l][c].delta[uu];
+                                e.amp = (double)tb[l][c].amp[uu];
+                                out.push_back(e);
+                            }
+                        }
+                }
+            }
+        }
+        op->tfsf_max_delay = maxDelay + 1;
+    }
+
     // Operator_Ext_LorentzMaterial::BuildExtension (operator_ext_lorentzmaterial.cpp:117-438),
     // order 1, electric Drude/Lorentz poles
     void Lorentz() {
@@ -1005,6 +1239,7 @@ int wop_operator::build(uint32_t nr_ts) {
     b.ApplyMagneticBC();
     // extensions in creation order: excitation, (TFSF), UPML, Lorentz (openems.cpp:1041-1092)
     b.Excitation();
+    b.Tfsf();
     b.Upml();
     b.Lorentz();
     b.Signal(nr_ts);
@@ -1137,6 +1372,41 @@ int wop_add_excitation(wop_operator* op, int type, const double value[3], double
     p.kind = P_EXCITATION; p.exc_type = type; p.delay = delay;
     for (int n = 0; n < 3; ++n) p.exc_val[n] = value[n];
     return add_prop(op, p, id);
+}
+
+int wop_set_plane_wave(wop_operator* op, const double start[3], const double stop[3], const double prop_dir[3],
+                       const double e_vec[3], double freq) {
+    if (!op || !start || !stop || !prop_dir || !e_vec) return fail(-1, "bad argument");
+    for (int n = 0; n < 3; ++n) {
+        op->pw.a[n] = start[n]; op->pw.b[n] = stop[n];
+        op->pw.k[n] = prop_dir[n]; op->pw.E[n] = e_vec[n];
+    }
+    op->pw.f = freq;
+    op->pw.set = true;
+    return 0;
+}
+
+int wop_get_tfsf_info(wop_operator* op, uint32_t* volt_count, uint32_t* curr_count, uint32_t* max_delay) {
+    if (!op || !volt_count || !curr_count || !max_delay || !op->built) return fail(-1, "operator not built");
+    *volt_count = (uint32_t)op->tfsf_v.size();
+    *curr_count = (uint32_t)op->tfsf_i.size();
+    *max_delay = op->tfsf_max_delay;
+    return 0;
+}
+
+int wop_get_tfsf(wop_operator* op, int which, uint32_t* pos, uint32_t* comp, uint32_t* delay, double* delta, double* amp) {
+    if (!op || !op->built || (which != 0 && which != 1)) return fail(-1, "bad argument");
+    const auto& L = which == 0 ? op->tfsf_v : op->tfsf_i;
+    const size_t n = L.size();
+    for (size_t i = 0; i < n; ++i) {
+        const auto& e = L[i];
+        if (pos) for (int a = 0; a < 3; ++a) pos[a * n + i] = e.pos[a];
+        if (comp) comp[i] = e.comp;
+        if (delay) delay[i] = e.delay;
+        if (delta) delta[i] = e.delta;
+        if (amp) amp[i] = e.amp;
+    }
+    return 0;
 }
 
 static int add_prim(wop_operator* op, Primitive pr) {

@@ -244,6 +244,15 @@ class Operator:
         self._chk(self._lib.wop_add_excitation(self._h, int(exc_type), v, float(delay), byref(i)))
         return i.value
 
+    def set_plane_wave(self, start, stop, prop_dir, e_vec, freq=0.0):
+        """Plane-wave (TFSF) excitation on the faces of a box (openEMS
+        AddPlaneWaveExcite + one box; Operator_Ext_TFSF)."""
+        a = (c_double * 3)(*start)
+        b = (c_double * 3)(*stop)
+        k = (c_double * 3)(*prop_dir)
+        e = (c_double * 3)(*e_vec)
+        self._chk(self._lib.wop_set_plane_wave(self._h, a, b, k, e, float(freq)))
+
     def add_box(self, prop, start, stop, priority=0):
         a = (c_double * 3)(*start)
         b = (c_double * 3)(*stop)
@@ -314,6 +323,17 @@ class Operator:
             lists = [np.empty((3, cnt.value), dtype=dt) for _ in range(6)]
             self._chk(lib.wop_get_ade(h, o, _up(pos), 1 if prec == 1 else 0, *[_vp(a) for a in lists]))
             ades.append(AdeOrder(pos, lists, fl.value))
+        nv, nc, md = c_uint32(), c_uint32(), c_uint32()
+        self._chk(lib.wop_get_tfsf_info(h, byref(nv), byref(nc), byref(md)))
+        tfsf = None
+        if nv.value or nc.value:
+            tfsf = {"max_delay": md.value}
+            for which, cnt in ((0, nv.value), (1, nc.value)):
+                t = {"pos": np.empty((3, cnt), np.uint32), "comp": np.empty(cnt, np.uint32),
+                     "delay": np.empty(cnt, np.uint32), "delta": np.empty(cnt), "amp": np.empty(cnt)}
+                self._chk(lib.wop_get_tfsf(h, which, _up(t["pos"]), _up(t["comp"]), _up(t["delay"]),
+                                           _dp(t["delta"]), _dp(t["amp"])))
+                tfsf[which] = t
         self._built_prec = prec
         tot = int(n[0]) + int(n[1]) + int(n[2])
         edges = []
@@ -323,7 +343,7 @@ class Operator:
             edges.append(ebuf)
         self._edges = edges
         return OperatorData(tuple(int(x) for x in n), dT.value, prec, *arrs, exc, slabs, ades, nyq.value, mstep.value,
-                            source=self, edge_primary=edges[0], edge_dual=edges[1])
+                            tfsf=tfsf, source=self, edge_primary=edges[0], edge_dual=edges[1])
 
     def compress(self, max_entries=0):
         """Native per-cell compression of the last build (wop_compress); None if

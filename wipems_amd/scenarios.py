@@ -59,6 +59,43 @@ def cavity_point_source(n_cells=64, steps=2000, prec=0, delta_mm=1.0, f0=0.0, fc
     return Scenario("cavity_%d" % n_cells, data, steps, op, meta={"interval": interval})
 
 
+def plane_wave_box(n_cells=40, steps=400, prec=0, pml=8, margin=6, delta_mm=1.0, prop_dir=(1.0, 0.0, 0.0),
+                   e_vec=(0.0, 0.0, 1.0), f0=5e9, fc=5e9, phase_freq=0.0, graded=False, seed=SEED_BASE + 3):
+    """Vacuum cube with UPML on all faces and a total-field/scattered-field
+    plane wave on a box `margin` cells inside the PML (openEMS
+    AddPlaneWaveExcite, Operator_Ext_TFSF); Gaussian pulse f0/fc.  Probes:
+    the E edge of the polarization at the box centre and at a point in the
+    scattered-field region between box and PML."""
+    if graded:
+        rng = np.random.default_rng(seed)
+        lines = [graded_lines(rng, n_cells, 0.7, 1.3) for _ in range(3)]
+    else:
+        lines = [np.arange(n_cells + 1, dtype=np.float64) * delta_mm for _ in range(3)]
+    op = Operator()
+    op.set_mesh(lines, 1e-3)
+    op.set_boundary([BC_PML if pml > 0 else BC_PEC] * 6, [pml] * 6)
+    op.set_gauss_excite(f0, fc)
+    lo = [lines[n][pml + margin] for n in range(3)]
+    hi = [lines[n][-1 - pml - margin] for n in range(3)]
+    op.set_plane_wave(lo, hi, prop_dir, e_vec, phase_freq)
+    data = op.build(prec, nr_ts=steps)
+    ez = int(np.argmax(np.abs(e_vec)))
+    c = n_cells // 2
+    pc = [lines[n][c] for n in range(3)]
+    pc2 = list(pc)
+    pc2[ez] = lines[ez][c + 1]
+    out = [lines[n][pml + margin // 2] for n in range(3)]
+    out2 = list(out)
+    out2[ez] = lines[ez][pml + margin // 2 + 1]
+    probes = [
+        op.voltage_probe(tuple(pc), tuple(pc2), 1.0, td_interval=1, name="u_total"),
+        op.voltage_probe(tuple(out), tuple(out2), 1.0, td_interval=1, name="u_scat"),
+    ]
+    _finish(op, data, probes)
+    return Scenario("plane_wave_%d" % n_cells, data, steps, op,
+                    meta={"box": (lo, hi), "lines": lines, "centre": pc, "pol_axis": ez})
+
+
 def graded_lines(rng, n_cells, lo=0.5, hi=2.0):
     gaps = rng.uniform(lo, hi, n_cells)
     return np.concatenate([[0.0], np.cumsum(gaps)])
