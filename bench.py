@@ -39,6 +39,10 @@ WORKLOADS = {
                     "512^3 lines uniform 1 mm vacuum, PEC, soft Ez Gaussian at centre, 1000 steps"),
     "cavity_64": ("cavity_point_source", {"n_cells": 64, "steps": 2000},
                   "64^3 cells (65^3 lines) 1 mm vacuum, PEC, soft Ez Gaussian, 2000 steps"),
+    "dispersive_c3": ("dispersive_scenario", {"steps": 20000, "material_scale": 1e-3, "ts_factor": 0.9},
+                      "385x385x193 lines at 0.25 mm, UPML-8, Drude slab + 16 Lorentz spheres (material frequencies x1e-3, "
+                      "relaxation times x1e3, timestep factor 0.9: the stated THz/PHz values diverge in the reference ADE "
+                      "scheme), TFSF plane wave +z / Ex (30/30 GHz Gaussian), 20000 steps"),
     "batched_64": ("batched_port_scenarios", {"count": 64, "steps": 5000},
                    "64 independent domains, per-axis cells U{48..160} at 1 mm, UPML-8, 8 eps boxes, 50 ohm port (3/3 GHz), "
                    "5000 steps, one launch per update phase for all 64 (graph-captured), vs 64 sequential single-domain runs"),
@@ -89,6 +93,18 @@ def cpu_baseline_leg(data, interval, cells, threads, steps):
     val = cells * steps / dt / 1e6
     return {"value": round(val, 2), "unit": "MCells/s", "cores": int(b.nthreads), "kind": "port",
             "sample": "%d steps of the same workload (restated Engine_SSE_Compressed + Engine_Multithread, %d threads, %.1f s)" % (steps, b.nthreads, dt)}
+
+
+def scalar_baseline_leg(data, cells, steps):
+    from oracle.oracle import oracle_from_operator
+    o = oracle_from_operator(data, ftz=True)
+    o.iterate(1)
+    t0 = time.perf_counter()
+    o.iterate(steps)
+    dt = time.perf_counter() - t0
+    return {"value": round(cells * steps / dt / 1e6, 2), "unit": "MCells/s", "cores": 1, "kind": "port",
+            "sample": "%d steps of the same workload on the scalar restatement of the basic engine with every hook "
+                      "(one thread, %.1f s)" % (steps, dt)}
 
 
 def run_batched(args, desc):
@@ -270,6 +286,11 @@ def main():
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         cpu = cpu_baseline_leg(data, sc.meta.get("interval", 0), cells, threads, cpu_steps)
         log("cpu baseline: %.1f MCells/s on %d threads" % (cpu["value"], cpu["cores"]))
+    elif cpu_steps > 0 and args.precision == "fp32":
+        # the multithreaded restatement has no ADE/TFSF hooks: time the scalar
+        # oracle (basic engine loop order, one thread) on the same workload
+        cpu = scalar_baseline_leg(data, cells, args.cpu_steps if args.cpu_steps else max(3, int(5e8 / cells)))
+        log("cpu baseline (scalar): %.1f MCells/s" % cpu["value"])
 
     copy_gbs = copy_bandwidth(1 << 30, 10)
     if roofline is not None:

@@ -167,3 +167,29 @@ def test_plane_wave_tfsf_bitwise(gpu, prec, precision, kw):
     print("case: plane wave %s %s" % (precision, kw), flush=True)
     eng, orc = run_pair(sc, 260, precision=precision, compressed=None, chunks=(5, 31))
     compare(eng, orc)
+
+
+@pytest.mark.parametrize("prec,precision", [(0, "fp32"), (1, "fp64")])
+def test_dispersive_ade_tfsf_bitwise(gpu, prec, precision):
+    """Drude slab + Lorentz spheres (ADE pre/apply hooks) lit by a TFSF plane
+    wave, UPML on all faces, against the oracle bit for bit."""
+    sc = scenarios.dispersive_scenario(num_lines=(41, 37, 45), steps=300, prec=prec, n_spheres=5,
+                                       material_scale=1e-3, ts_factor=0.9)
+    assert sc.data.ade and sc.data.ade[0].pos.shape[1] > 0
+    print("case: dispersive 41x37x45 %s" % precision, flush=True)
+    eng, orc = run_pair(sc, 300, precision=precision, compressed=None, chunks=(1, 16, 33))
+    compare(eng, orc)
+
+
+def test_lorentz_filled_cavity_bitwise(gpu):
+    """Undamped Lorentz pole filling a PEC cavity (the ADE dispersion pin)
+    on the GPU, 3000 steps, against the oracle."""
+    from tests.test_ade import filled_cavity
+    d = filled_cavity(2e9, 2e9, steps=3000)
+    sc = scenarios.Scenario("lorentz_cavity", d, 3000, None)
+    eng = engine_from_operator(d, precision="fp32", compressed=None)
+    _kinds[id(eng)] = [p.kind for p in d.probes]
+    orc = oracle_from_operator(d, ftz=True)
+    eng.iterate(3000)
+    orc.iterate(3000)
+    compare(eng, orc)

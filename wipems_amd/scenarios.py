@@ -96,6 +96,65 @@ def plane_wave_box(n_cells=40, steps=400, prec=0, pml=8, margin=6, delta_mm=1.0,
                     meta={"box": (lo, hi), "lines": lines, "centre": pc, "pol_axis": ez})
 
 
+def dispersive_scenario(num_lines=(385, 385, 193), delta_mm=0.25, steps=20000, prec=0, seed=SEED_BASE + 3, pml=8,
+                        n_spheres=16, f0=30e9, fc=30e9, drude=True, lorentz=True, material_scale=1.0, ts_factor=1.0):
+    """Drude slab and Lorentz spheres lit by a TFSF plane wave: uniform
+    0.25 mm mesh of num_lines lines (defaults: 385 x 385 x 193), UPML-8.
+    Slab: eps_inf = 1, f_p = 2e15/(2 pi) Hz, tau = 1e-13 s, across the
+    interior in x and y, z in [0.55, 0.65] of the domain.  Spheres:
+    eps_inf ~ U[1,3], d_eps ~ U[1,5], f_0 ~ U[50,300] THz, tau = 1e-13 s,
+    as EpsPlasmaFrequency = f_0 sqrt(d_eps/eps_inf), EpsLorPoleFrequency =
+    f_0.  Plane wave along +z, E along x, Gaussian f0/fc.  Probes: the x edge
+    at the centre of the plane below the slab, and one above it.
+    material_scale multiplies every material frequency and divides every
+    relaxation time (1.0: the values above); ts_factor is the timestep factor
+    (Operator::SetTimestepFactor)."""
+    rng = np.random.default_rng(seed)
+    lines = [np.arange(n, dtype=np.float64) * delta_mm for n in num_lines]
+    op = Operator()
+    op.set_mesh(lines, 1e-3)
+    op.set_boundary([BC_PML] * 6, [pml] * 6)
+    op.set_timestep_factor(ts_factor)
+    op.set_gauss_excite(f0, fc)
+    ext = [lines[n][-1] for n in range(3)]
+    if drude:
+        m = op.add_lorentz_material(eps_r=1.0, plasma_freq=material_scale * 2e15 / (2 * math.pi),
+                                    relax_time=1e-13 / material_scale)
+        lo = (lines[0][pml + 2], lines[1][pml + 2], 0.55 * ext[2])
+        hi = (lines[0][-pml - 3], lines[1][-pml - 3], 0.65 * ext[2])
+        op.add_box(m, lo, hi, priority=1)
+    spheres = []
+    if lorentz:
+        for k in range(n_spheres):
+            eps_inf = rng.uniform(1, 3)
+            d_eps = rng.uniform(1, 5)
+            f_0 = rng.uniform(50e12, 300e12)
+            r = rng.uniform(0.03, 0.08) * min(ext)
+            c = [rng.uniform(lines[n][pml] + r, lines[n][-1 - pml] - r) for n in range(3)]
+            fs = material_scale * f_0
+            m = op.add_lorentz_material(eps_r=eps_inf, plasma_freq=fs * math.sqrt(d_eps / eps_inf),
+                                        relax_time=1e-13 / material_scale, lor_pole_freq=fs)
+            op.add_sphere(m, tuple(c), r, priority=2)
+            spheres.append({"centre": c, "radius": r, "eps_inf": eps_inf, "d_eps": d_eps, "f0": f_0})
+    margin = 4
+    blo = [lines[n][pml + margin] for n in range(3)]
+    bhi = [lines[n][-1 - pml - margin] for n in range(3)]
+    op.set_plane_wave(blo, bhi, (0.0, 0.0, 1.0), (1.0, 0.0, 0.0))
+    data = op.build(prec, nr_ts=steps)
+    interval = max(1, data.nyquist // 4)
+    cx, cy = num_lines[0] // 2, num_lines[1] // 2
+    zlo = int(0.4 * (num_lines[2] - 1))
+    zhi = int(0.8 * (num_lines[2] - 1))
+    probes = []
+    for z, nm in ((zlo, "u_below"), (zhi, "u_above")):
+        a = (lines[0][cx], lines[1][cy], lines[2][z])
+        b = (lines[0][cx + 1], lines[1][cy], lines[2][z])
+        probes.append(op.voltage_probe(a, b, 1.0, td_interval=interval, name=nm))
+    _finish(op, data, probes)
+    return Scenario("dispersive_%dx%dx%d" % tuple(num_lines), data, steps, op,
+                    meta={"interval": interval, "spheres": spheres, "seed": seed})
+
+
 def graded_lines(rng, n_cells, lo=0.5, hi=2.0):
     gaps = rng.uniform(lo, hi, n_cells)
     return np.concatenate([[0.0], np.cumsum(gaps)])
