@@ -205,6 +205,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=None, help="steps of the CPU-baseline sample (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=None)
     ap.add_argument("--no-timing-pass", action="store_true")
+    ap.add_argument("--variant", type=int, default=None, help="update-kernel shape (fdtd_set_tuning), default: the engine's")
     ap.add_argument("--no-fp64-check", action="store_true", help="skip the FP64-engine error check")
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="counter-collection mode: run this many steps of the workload, then a 1 GiB device copy "
@@ -238,6 +239,8 @@ def main():
         scales = pilot_scales(data, compressed=compressed)
         log("storage scales (pilot FP32 run, %.1f s): volt %s curr %s" % (time.time() - t0, scales[0], scales[1]))
     eng = engine_from_operator(data, precision=args.precision, compressed=compressed, scales=scales)
+    if args.variant is not None:
+        eng.set_tuning(args.variant, 4, 1)
     log("device %s, coefficients %s, device bytes %.2f GB" % (device_name(), "compressed" if eng.compressed else "full", eng.device_bytes() / 1e9))
 
     if args.profile_steps:

@@ -207,6 +207,15 @@ int  fdtd_get_num_timesteps(fdtd_engine* h, uint32_t* numTS);
  * consecutive z cells per lane; tile_y = rows per 64-lane tile (1, 2 or 4).
  * Results do not depend on the shape, only speed does. */
 int  fdtd_set_tuning(fdtd_engine* h, int variant, int tile_y, int chunk_x);
+/* UPML coefficients: the vector kernels can read per-cell uint16 row
+ * indices into a table of distinct coefficient tuples (possible when each
+ * side has at most 65536 distinct tuples) instead of the full slab arrays.
+ * mode 1 = tables, 0 = full arrays, -1 = tables only together with full
+ * main coefficients (default: with compressed main coefficients the extra
+ * gathers cost more than the bytes they save).  fdtd_pml_compressed
+ * reports whether the tables are in use. */
+int  fdtd_set_pml_compression(fdtd_engine* h, int mode);
+int  fdtd_pml_compressed(fdtd_engine* h, int* active);
 /* Use a captured HIP graph for the step loop (default on). */
 int  fdtd_set_use_graph(fdtd_engine* h, int enable);
 /* Kernel timing: while enabled, fdtd_iterate runs eagerly and records HIP

@@ -193,3 +193,29 @@ def test_lorentz_filled_cavity_bitwise(gpu):
     eng.iterate(3000)
     orc.iterate(3000)
     compare(eng, orc)
+
+
+@pytest.mark.parametrize("precision,prec", [("fp32", 0), ("fp64", 1), ("fp16", 0)])
+def test_pml_coefficient_rows_match_full_arrays(gpu, precision, prec):
+    """UPML coefficients read through the per-cell row index give the same
+    bits as the full slab arrays (and as the oracle for FP32/FP64)."""
+    sc = scenarios.graded_port_scenario(n_cells=38, steps=150, prec=prec, n_boxes=5, pml=7)
+    d = sc.data
+    print("case: pml rows %s" % precision, flush=True)
+    fields = []
+    for use in (True, False):
+        eng = engine_from_operator(d, precision=precision, compressed=False)
+        assert eng.pml_compressed()          # the default with full main coefficients
+        eng.set_pml_compression(1 if use else 0)
+        assert eng.pml_compressed() == use
+        eng.iterate(3)
+        eng.iterate(147)
+        fields.append((eng.get_field(0), eng.get_field(1)))
+        if precision != "fp16" and use:
+            _kinds[id(eng)] = [p.kind for p in d.probes]
+            orc = oracle_from_operator(d, ftz=True)
+            orc.iterate(150)
+            compare(eng, orc)
+        eng.close()
+    for k in range(2):
+        np.testing.assert_array_equal(fields[0][k], fields[1][k])

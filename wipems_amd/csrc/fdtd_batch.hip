@@ -215,6 +215,10 @@ int build_tables(fdtd_batch* b) {
             D.pp.vv = c; D.pp.vvfo = c + t; D.pp.vvfn = c + 2 * t;
             D.pp.ii = c + 3 * t; D.pp.iifo = c + 4 * t; D.pp.iifn = c + 5 * t;
             D.pp.vflux = h->pml_flux.as<A>(); D.pp.cflux = h->pml_flux.as<A>() + t;
+            if (pml_rows_in_use(h)) {
+                D.pp.vidx = h->pml_vidx.as<unsigned short>(); D.pp.vtab = h->pml_vtab.as<A>();
+                D.pp.cidx = h->pml_cidx.as<unsigned short>(); D.pp.ctab = h->pml_ctab.as<A>();
+            }
         }
         const long long vplane = (long long)h->N[1] * h->d.sy;
         const long long cplane = (long long)(h->N[1] - 1) * h->d.sy;

@@ -76,6 +76,7 @@ struct PmlSlab {
     int nzp;               // padded z extent of the slab arrays
     long long off;         // element offset of this slab's component 0
     long long sc;          // component stride inside the slab (nx*ny*nzp)
+    long long ioff;        // cell offset of this slab in the coefficient index arrays
 };
 struct PmlSet {
     int count;
@@ -87,6 +88,10 @@ struct PmlPtrs {
     const A* vv; const A* vvfo; const A* vvfn;   // voltage side
     const A* ii; const A* iifo; const A* iifn;   // current side
     A* vflux; A* cflux;
+    // compressed form used by the vector kernels when present: per slab cell
+    // a uint16 row index, rows of 12 {k0,k1,k2,_, fo0,fo1,fo2,_, fn0,fn1,fn2,_}
+    const unsigned short* vidx; const A* vtab;
+    const unsigned short* cidx; const A* ctab;
 };
 
 // coefficient access modes
@@ -316,6 +321,26 @@ __device__ __forceinline__ void vstore(S* p, const typename StoreTraits<S>::A* i
     }
 }
 
+// UPML coefficients of one vector of slab cells (component n): from the
+// compressed rows when present, else from the full slab arrays
+template <typename A, int VZ>
+__device__ __forceinline__ void pml_coeffs(const A* full0, const A* full1, const A* full2, long long q,
+                                           const A* tab, const unsigned short* rows, int n, A k0[VZ], A kfo[VZ], A kfn[VZ]) {
+    if (tab) {
+#pragma unroll
+        for (int k = 0; k < VZ; ++k) {
+            const A* r = tab + (size_t)rows[k] * 12;
+            k0[k] = r[n]; kfo[k] = r[4 + n]; kfn[k] = r[8 + n];
+        }
+    } else {
+        const SVec<A, VZ> a = *reinterpret_cast<const SVec<A, VZ>*>(full0 + q);
+        const SVec<A, VZ> b = *reinterpret_cast<const SVec<A, VZ>*>(full1 + q);
+        const SVec<A, VZ> c = *reinterpret_cast<const SVec<A, VZ>*>(full2 + q);
+#pragma unroll
+        for (int k = 0; k < VZ; ++k) { k0[k] = a.v[k]; kfo[k] = b.v[k]; kfn[k] = c.v[k]; }
+    }
+}
+
 template <typename A, int CM, int VZ>
 __device__ __forceinline__ void vload_coeffs(const CoeffPtrs<A>& cf, long long p, long long sc, A a[3][VZ], A b[3][VZ]) {
     if (CM == CF_FULL) {
@@ -387,6 +412,9 @@ __device__ __forceinline__ void volt_vec_body(S* __restrict__ V, const S* __rest
     long long pb = 0;
     unsigned pm = 0;
     const int slab = PML ? find_slab_vec(ps, x, y, z0, VZ, pb, pm) : -1;
+    SVec<unsigned short, VZ> prow{};
+    if (PML && slab >= 0 && pp.vtab)
+        prow = *reinterpret_cast<const SVec<unsigned short, VZ>*>(pp.vidx + (pb - ps.slab[slab].off + ps.slab[slab].ioff));
 #pragma unroll
     for (int n = 0; n < 3; ++n) {
         A v[VZ];
@@ -395,9 +423,7 @@ __device__ __forceinline__ void volt_vec_body(S* __restrict__ V, const S* __rest
             const long long q = pb + n * ps.slab[slab].sc;
             A F[VZ], kv[VZ], kfo[VZ], kfn[VZ];
             vload<A, VZ>(pp.vflux + q, F, A(1));
-            vload<A, VZ>(pp.vv + q, kv, A(1));
-            vload<A, VZ>(pp.vvfo + q, kfo, A(1));
-            vload<A, VZ>(pp.vvfn + q, kfn, A(1));
+            pml_coeffs<A, VZ>(pp.vv, pp.vvfo, pp.vvfn, q, pp.vtab, prow.v, n, kv, kfo, kfn);
 #pragma unroll
             for (int k = 0; k < VZ; ++k) {
                 if (pm & (1u << k)) {
@@ -473,6 +499,9 @@ __device__ __forceinline__ void curr_vec_body(S* __restrict__ I, const S* __rest
     const int slab = PML ? find_slab_vec(ps, x, y, z0, VZ, pb, pm) : -1;
     // cells at z >= nz-1 are not updated: keep them out of the slab mask too
     if (valid < VZ) pm &= (1u << valid) - 1u;
+    SVec<unsigned short, VZ> prow{};
+    if (PML && slab >= 0 && pp.ctab)
+        prow = *reinterpret_cast<const SVec<unsigned short, VZ>*>(pp.cidx + (pb - ps.slab[slab].off + ps.slab[slab].ioff));
 #pragma unroll
     for (int n = 0; n < 3; ++n) {
         A c[VZ];
@@ -481,9 +510,7 @@ __device__ __forceinline__ void curr_vec_body(S* __restrict__ I, const S* __rest
             const long long q = pb + n * ps.slab[slab].sc;
             A F[VZ], ki[VZ], kfo[VZ], kfn[VZ];
             vload<A, VZ>(pp.cflux + q, F, A(1));
-            vload<A, VZ>(pp.ii + q, ki, A(1));
-            vload<A, VZ>(pp.iifo + q, kfo, A(1));
-            vload<A, VZ>(pp.iifn + q, kfn, A(1));
+            pml_coeffs<A, VZ>(pp.ii, pp.iifo, pp.iifn, q, pp.ctab, prow.v, n, ki, kfo, kfn);
 #pragma unroll
             for (int k = 0; k < VZ; ++k) {
                 if (pm & (1u << k)) {

@@ -136,9 +136,69 @@ int finalize_pml(fdtd_engine* h) {
         }
     }
     HIPCHK(hipStreamSynchronize(h->stream));
+    int rc2 = compress_pml(h);
+    if (rc2) return rc2;
     h->pml_host_coef.clear();
     h->pml_dirty = false;
     h->pml_final = true;
+    return FDTD_OK;
+}
+
+// Deduplicate the per-cell UPML coefficient 9-tuples (k, fo, fn for three
+// components) of each side bitwise into at most 65536 rows.  They depend on
+// the grading depth and the cell material only, so a slab holds few distinct
+// tuples.  Leaves h->pml_cmp false (full arrays only) when there are more.
+int compress_pml(fdtd_engine* h) {
+    h->pml_cmp = false;
+    const size_t a = h->acc;
+    const size_t key = 9 * a;
+    std::vector<unsigned short> idx[2];
+    std::vector<char> tab[2];
+    for (int side = 0; side < 2; ++side) {
+        idx[side].assign((size_t)h->pml_cells, 0);
+        std::unordered_map<uint64_t, std::vector<uint32_t>> seen;
+        std::vector<char> rows;   // key bytes per row
+        char buf[72];
+        for (int s = 0; s < h->pml.count; ++s) {
+            const PmlSlab& b = h->pml.slab[s];
+            const char* all = h->pml_host_coef[s].data();
+            const size_t cnt = (size_t)3 * b.sc;
+            for (long long c = 0; c < b.sc; ++c) {
+                for (int k = 0; k < 3; ++k)
+                    for (int n = 0; n < 3; ++n)
+                        memcpy(buf + (k * 3 + n) * a, all + (((size_t)(3 * side + k)) * cnt + (size_t)n * b.sc + c) * a, a);
+                uint64_t hsh = 1469598103934665603ull;
+                for (size_t i = 0; i < key; ++i) hsh = (hsh ^ (unsigned char)buf[i]) * 1099511628211ull;
+                auto& bucket = seen[hsh];
+                uint32_t found = UINT32_MAX;
+                for (uint32_t r : bucket)
+                    if (!memcmp(rows.data() + (size_t)r * key, buf, key)) { found = r; break; }
+                if (found == UINT32_MAX) {
+                    found = (uint32_t)(rows.size() / key);
+                    if (found >= 65536) return FDTD_OK;   // too many: keep the full arrays
+                    rows.insert(rows.end(), buf, buf + key);
+                    bucket.push_back(found);
+                }
+                idx[side][(size_t)b.ioff + c] = (unsigned short)found;
+            }
+        }
+        // rows of 12: {k0,k1,k2,0, fo0,fo1,fo2,0, fn0,fn1,fn2,0}
+        const size_t nrows = rows.size() / key;
+        tab[side].assign(nrows * 12 * a, 0);
+        for (size_t r = 0; r < nrows; ++r)
+            for (int k = 0; k < 3; ++k)
+                memcpy(tab[side].data() + (r * 12 + 4 * k) * a, rows.data() + r * key + k * 3 * a, 3 * a);
+    }
+    int rc;
+    if ((rc = h->pml_vidx.alloc(idx[0].size() * 2)) || (rc = h->pml_cidx.alloc(idx[1].size() * 2)) ||
+        (rc = h->pml_vtab.alloc(tab[0].size())) || (rc = h->pml_ctab.alloc(tab[1].size())))
+        return rc;
+    HIPCHK(hipMemcpyAsync(h->pml_vidx.p, idx[0].data(), idx[0].size() * 2, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->pml_cidx.p, idx[1].data(), idx[1].size() * 2, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->pml_vtab.p, tab[0].data(), tab[0].size(), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->pml_ctab.p, tab[1].data(), tab[1].size(), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->pml_cmp = true;
     return FDTD_OK;
 }
 
@@ -373,6 +433,10 @@ int launch_step_kernels(fdtd_engine* h, bool timed) {
         pp.vv = c; pp.vvfo = c + t; pp.vvfn = c + 2 * t;
         pp.ii = c + 3 * t; pp.iifo = c + 4 * t; pp.iifn = c + 5 * t;
         pp.vflux = h->pml_flux.as<A>(); pp.cflux = h->pml_flux.as<A>() + t;
+        if (pml_rows_in_use(h)) {
+            pp.vidx = h->pml_vidx.as<unsigned short>(); pp.vtab = h->pml_vtab.as<A>();
+            pp.cidx = h->pml_cidx.as<unsigned short>(); pp.ctab = h->pml_ctab.as<A>();
+        }
     }
     const bool pml = h->pml.count > 0;
     const unsigned* stp = h->step.as<unsigned>();
@@ -804,6 +868,7 @@ int fdtd_add_upml(fdtd_engine* h, const uint32_t start[3], const uint32_t num_li
     b.nzp = (b.s[2] + b.n[2] - b.za + 15) / 16 * 16;
     b.sc = (long long)num_lines[0] * num_lines[1] * b.nzp;
     b.off = h->pml_total;
+    b.ioff = h->pml_cells;
     const size_t cnt = (size_t)3 * b.sc;
     const size_t dense = (size_t)3 * num_lines[0] * num_lines[1] * num_lines[2];
     std::vector<char> all((size_t)6 * cnt * h->acc, 0);
@@ -821,6 +886,7 @@ int fdtd_add_upml(fdtd_engine* h, const uint32_t start[3], const uint32_t num_li
     // offsets are in elements per array; each array covers all slabs, comp-major per slab
     h->pml.slab[h->pml.count++] = b;
     h->pml_total += (long long)cnt;
+    h->pml_cells += b.sc;
     h->pml_host_coef.push_back(std::move(all));
     h->pml_dirty = true;
     invalidate_graph(h);
@@ -1122,6 +1188,22 @@ int fdtd_set_tuning(fdtd_engine* h, int variant, int tile_y, int chunk_x) {
     h->tile_y = tile_y;
     h->chunk_x = chunk_x;
     invalidate_graph(h);
+    return FDTD_OK;
+}
+
+int fdtd_set_pml_compression(fdtd_engine* h, int mode) {
+    if (!h) return fail(FDTD_ERR_ARG, "null handle");
+    if (mode < -1 || mode > 1) return fail(FDTD_ERR_ARG, "mode must be -1 (auto), 0 or 1");
+    h->pml_cmp_mode = mode;
+    invalidate_graph(h);
+    return FDTD_OK;
+}
+
+int fdtd_pml_compressed(fdtd_engine* h, int* active) {
+    if (!h || !active) return fail(FDTD_ERR_ARG, "null argument");
+    int rc = finalize_pml(h);
+    if (rc) return rc;
+    *active = pml_rows_in_use(h) ? 1 : 0;
     return FDTD_OK;
 }
 

@@ -106,8 +106,13 @@ struct fdtd_engine {
     DevBuf pml_flux;              // 2 arrays over all slabs
     std::vector<std::vector<char>> pml_host_coef;   // staged per slab until finalized
     long long pml_total = 0;
+    long long pml_cells = 0;      // cells of all slabs (padded layout)
     bool pml_dirty = false;
     bool pml_final = false;
+    // compressed UPML coefficients: per cell uint16 rows (volt, curr sides)
+    DevBuf pml_vidx, pml_cidx, pml_vtab, pml_ctab;
+    bool pml_cmp = false;         // tables built
+    int pml_cmp_mode = -1;        // -1 auto (with full main coefficients), 0 off, 1 on
 
     // excitation
     bool has_exc = false;
@@ -172,6 +177,10 @@ struct fdtd_engine {
 namespace wipems {
 void invalidate_graph(fdtd_engine* h);
 int finalize_pml(fdtd_engine* h);
+int compress_pml(fdtd_engine* h);
+inline bool pml_rows_in_use(const fdtd_engine* h) {
+    return h->pml_cmp && (h->pml_cmp_mode == 1 || (h->pml_cmp_mode < 0 && h->cf_mode == CF_FULL));
+}
 int finalize_probes(fdtd_engine* h, uint32_t upto_ts);
 int ready_check(fdtd_engine* h);
 

@@ -199,6 +199,15 @@ class Engine:
     def set_use_graph(self, on):
         self._chk(self._lib.fdtd_set_use_graph(self._h, int(bool(on))))
 
+    def set_pml_compression(self, mode=-1):
+        """1: UPML coefficient rows, 0: full slab arrays, -1: rows only with full main coefficients."""
+        self._chk(self._lib.fdtd_set_pml_compression(self._h, int(mode)))
+
+    def pml_compressed(self):
+        a = c_int()
+        self._chk(self._lib.fdtd_pml_compressed(self._h, byref(a)))
+        return bool(a.value)
+
     def set_tuning(self, variant=3, tile_y=4, chunk_x=1):
         self._chk(self._lib.fdtd_set_tuning(self._h, int(variant), int(tile_y), int(chunk_x)))
 
