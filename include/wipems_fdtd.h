@@ -204,8 +204,12 @@ int  fdtd_get_num_timesteps(fdtd_engine* h, uint32_t* numTS);
 /* Kernel shape (default 3): variant 0 = one thread per cell in hardware block order,
  * 1 = x-marching columns (chunk_x planes per block), 2 = one thread per cell
  * in XCD-banded order (chunk_x = y tiles per band), 3 = 16-byte vectors of
- * consecutive z cells per lane; tile_y = rows per 64-lane tile (1, 2 or 4).
- * Results do not depend on the shape, only speed does. */
+ * consecutive z cells per lane, 4 = one fused launch per timestep (voltage
+ * and current updates of a tile in an x-march over chunk_x planes per
+ * block, ping-pong field buffers) where the domain has no UPML, Lorentz or
+ * TFSF extension and at most 64 soft excitation edges per field, else 3;
+ * tile_y = rows per 64-lane tile (1, 2 or 4) for variants 0-2.  Results do
+ * not depend on the shape, only speed does. */
 int  fdtd_set_tuning(fdtd_engine* h, int variant, int tile_y, int chunk_x);
 /* UPML coefficients: the vector kernels can read per-cell uint16 row
  * indices into a table of distinct coefficient tuples (possible when each

@@ -219,3 +219,71 @@ def test_pml_coefficient_rows_match_full_arrays(gpu, precision, prec):
         eng.close()
     for k in range(2):
         np.testing.assert_array_equal(fields[0][k], fields[1][k])
+
+
+def _h_source_cavity(n=(30, 26, 40), steps=200, prec=0):
+    """PEC box with a soft H (current) excitation edge and a soft E edge."""
+    from wipems_amd.operator import BC_PEC, Operator
+    lines = [np.arange(k + 1, dtype=np.float64) for k in n]
+    op = Operator()
+    op.set_mesh(lines, 1e-3)
+    op.set_boundary([BC_PEC] * 6)
+    op.set_gauss_excite(0.0, 20e9)
+    c = [lines[k][n[k] // 2] for k in range(3)]
+    e = op.add_excitation(2, (0.0, 1.0, 0.0))
+    op.add_box(e, (c[0], c[1], c[2]), (c[0], lines[1][n[1] // 2 + 1], c[2]), priority=5)
+    e2 = op.add_excitation(0, (1.0, 0.0, 0.0))
+    op.add_box(e2, (lines[0][3], lines[1][4], lines[2][5]), (lines[0][4], lines[1][4], lines[2][5]), priority=5)
+    data = op.build(prec, nr_ts=steps)
+    data.probes = [op.voltage_probe((c[0], c[1], c[2]), (c[0], c[1], lines[2][n[2] // 2 + 1]), td_interval=1, name="u")]
+    return scenarios.Scenario("hsrc", data, steps, op)
+
+
+@pytest.mark.parametrize("precision,prec", [("fp32", 0), ("fp64", 1), ("fp16", 0), ("bf16", 0)])
+@pytest.mark.parametrize("case", ["cavity_odd", "cavity_wide", "port_pec", "hsource"])
+@pytest.mark.parametrize("seg", [32, 5])
+def test_fused_step_matches_two_launches(gpu, precision, prec, case, seg):
+    """The fused one-launch step (x-march tiles, ping-pong buffers) gives the
+    bits of the two-launch vector kernels, and of the oracle for FP32/FP64;
+    uneven chunks cross the 16-step graph and both buffer parities."""
+    if case == "cavity_odd":
+        sc = scenarios.cavity_point_source(n_cells=36, steps=120, prec=prec)
+    elif case == "cavity_wide":
+        from wipems_amd.operator import BC_PEC, Operator
+        sc = scenarios.cavity_point_source(n_cells=20, steps=120, prec=prec)
+        lines = [np.arange(k + 1, dtype=np.float64) for k in (23, 19, 300)]
+        op = Operator()
+        op.set_mesh(lines, 1e-3)
+        op.set_boundary([BC_PEC] * 6)
+        op.set_gauss_excite(0.0, 10e9)
+        c = [lines[k][len(lines[k]) // 2] for k in range(3)]
+        e = op.add_excitation(0, (0.0, 0.0, 1.0))
+        op.add_box(e, tuple(c), (c[0], c[1], lines[2][len(lines[2]) // 2 + 1]), priority=5)
+        d = op.build(prec, nr_ts=120)
+        d.probes = [op.voltage_probe(tuple(c), (c[0], c[1], lines[2][len(lines[2]) // 2 + 1]), td_interval=3, name="u")]
+        sc = scenarios.Scenario("wide", d, 120, op)
+    elif case == "port_pec":
+        sc = scenarios.graded_port_scenario(n_cells=34, steps=120, prec=prec, n_boxes=5, pml=0)
+    else:
+        sc = _h_source_cavity(prec=prec)
+    d = sc.data
+    assert not d.upml
+    print("case: fused %s %s seg %d" % (precision, case, seg), flush=True)
+    chunks = (1, 16, 3, 16, 33, 51)
+    out = []
+    for variant in (4, 3):
+        eng = engine_from_operator(d, precision=precision, compressed=None)
+        eng.set_tuning(variant, 4, seg if variant == 4 else 1)
+        for c in chunks:
+            eng.iterate(c)
+        out.append((eng.get_field(0), eng.get_field(1), [eng.read_probe(p)[1] for p in eng.probe_ids]))
+        if variant == 4 and precision in ("fp32", "fp64"):
+            _kinds[id(eng)] = [p.kind for p in d.probes]
+            orc = oracle_from_operator(d, ftz=True)
+            orc.iterate(sum(chunks))
+            compare(eng, orc)
+        eng.close()
+    for k in range(2):
+        np.testing.assert_array_equal(out[0][k], out[1][k])
+    for a, b in zip(out[0][2], out[1][2]):
+        np.testing.assert_array_equal(a, b)

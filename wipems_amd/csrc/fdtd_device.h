@@ -317,7 +317,9 @@ __device__ __forceinline__ void vstore(S* p, const typename StoreTraits<S>::A* i
     if (valid >= VZ) {
         *reinterpret_cast<SVec<S, VZ>*>(p) = t;
     } else {
-        for (int k = 0; k < valid; ++k) p[k] = t.v[k];
+#pragma unroll
+        for (int k = 0; k < VZ; ++k)
+            if (k < valid) p[k] = t.v[k];
     }
 }
 

@@ -26,6 +26,7 @@
 // SSE engines set FTZ|DAZ, tools/denormal.h:19-31), so an FP32 handle
 // reproduces the reference FP32 engine bit for bit.
 #include "fdtd_internal.h"
+#include "fdtd_fused.h"
 #include <memory>
 #include <unordered_map>
 
@@ -67,6 +68,64 @@ void invalidate_graph(fdtd_engine* h) {
     if (h->gexec1) { (void)hipGraphExecDestroy(h->gexec1); h->gexec1 = nullptr; }
     if (h->graph1) { (void)hipGraphDestroy(h->graph1); h->graph1 = nullptr; }
     h->graph_steps = 0;
+    h->graph_base = nullptr;
+}
+
+// the fused step handles domains without UPML, Lorentz or TFSF extensions
+// and with few soft excitation edges (kept in LDS)
+bool fused_ok(const fdtd_engine* h) {
+    if (h->variant != 4) return false;
+    if (h->pml.count || !h->ade.empty() || h->tfsf_v.count || h->tfsf_c.count) return false;
+    if (h->has_exc && (h->exc_v.count > FUSED_MAX_EXC || h->exc_c.count > FUSED_MAX_EXC)) return false;
+    return h->N[0] >= 2 && h->N[1] >= 2 && h->N[2] >= 2;
+}
+
+int ensure_fused_buffers(fdtd_engine* h) {
+    if (h->V2.p) return FDTD_OK;
+    int rc;
+    if ((rc = h->V2.alloc(h->V.bytes)) || (rc = h->I2.alloc(h->I.bytes))) return rc;
+    HIPCHK(hipMemsetAsync(h->V2.p, 0, h->V2.bytes, h->stream));
+    HIPCHK(hipMemsetAsync(h->I2.p, 0, h->I2.bytes, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return FDTD_OK;
+}
+
+template <typename S, int CM, int TY>
+void launch_fused(fdtd_engine* h) {
+    typedef typename StoreTraits<S>::A A;
+    constexpr int VZ = VecWidth<S>::value;
+    constexpr int TZ = 64 * VZ;
+    FusedGrid g;
+    g.tiles_y = ((int)h->N[1] + TY - 1) / TY;
+    g.tiles_z = ((int)h->N[2] + TZ - 1) / TZ;
+    g.seg = std::max(1, h->fused_seg);
+    g.segs_x = ((int)h->N[0] + g.seg - 1) / g.seg;
+    g.nblocks = g.tiles_y * g.tiles_z * g.segs_x;
+    g.per_xcd = (g.nblocks + 7) / 8;
+    CoeffPtrs<A> cfe{}, cfh{};
+    if (h->cf_mode == CF_FULL) {
+        cfe.a = h->cf_vv.as<A>(); cfe.b = h->cf_vi.as<A>();
+        cfh.a = h->cf_ii.as<A>(); cfh.b = h->cf_iv.as<A>();
+    } else {
+        cfe.a = h->cf_tabE.as<A>(); cfe.idx = h->cf_idx.p;
+        cfh.a = h->cf_tabH.as<A>(); cfh.idx = h->cf_idx.p;
+    }
+    ExcDev ev{}, ec{};
+    if (h->has_exc) { ev = h->exc_v; ec = h->exc_c; }
+    hipLaunchKernelGGL((k_step_fused<S, CM, VZ, TY>), dim3((unsigned)(g.per_xcd * 8)), dim3(64 * (TY + 2)), 0, h->stream,
+                       h->V.as<S>(), h->I.as<S>(), h->V2.as<S>(), h->I2.as<S>(), cfe, cfh, h->d, make_scales<A>(h), g, ev, ec,
+                       h->step.as<unsigned>());
+    std::swap(h->V.p, h->V2.p);
+    std::swap(h->I.p, h->I2.p);
+}
+
+template <typename S, int TY>
+void launch_fused_cm(fdtd_engine* h) {
+    switch (h->cf_mode) {
+    case CF_FULL: launch_fused<S, CF_FULL, TY>(h); break;
+    case CF_IDX16: launch_fused<S, CF_IDX16, TY>(h); break;
+    default: launch_fused<S, CF_IDX32, TY>(h); break;
+    }
 }
 
 
@@ -413,6 +472,19 @@ void launch_curr_band(fdtd_engine* h, CoeffPtrs<typename StoreTraits<S>::A> cf, 
 template <typename S>
 int launch_step_kernels(fdtd_engine* h, bool timed) {
     typedef typename StoreTraits<S>::A A;
+    if (fused_ok(h)) {
+        const size_t evi = timed ? h->ev_used : 0;
+        if (timed) HIPCHK(hipEventRecord(h->ev[evi], h->stream));
+        if constexpr (sizeof(S) == 2) launch_fused_cm<S, 2>(h);
+        else launch_fused_cm<S, 4>(h);
+        if (timed) {
+            HIPCHK(hipEventRecord(h->ev[evi + 1], h->stream));
+            HIPCHK(hipEventRecord(h->ev[evi + 2], h->stream));
+            HIPCHK(hipEventRecord(h->ev[evi + 3], h->stream));
+            h->ev_used += 4;
+        }
+        return launch_step_end<S>(h, 1);
+    }
     Scales<A> sc = make_scales<A>(h);
     const dim3 blk(64, 4, 1);
     const dim3 grd((h->N[2] + 63) / 64, (h->N[1] + 3) / 4, h->N[0]);
@@ -462,7 +534,7 @@ int launch_step_kernels(fdtd_engine* h, bool timed) {
     const size_t evi = timed ? h->ev_used : 0;
     if (timed) h->ev_used += 4;
     if (timed) HIPCHK(hipEventRecord(h->ev[evi], h->stream));
-    if (h->variant == 3) { if (pml) launch_volt_vec<S, true>(h, cfe, pp); else launch_volt_vec<S, false>(h, cfe, pp); }
+    if (h->variant >= 3) { if (pml) launch_volt_vec<S, true>(h, cfe, pp); else launch_volt_vec<S, false>(h, cfe, pp); }
     else if (h->variant == 2) { if (pml) launch_volt_band<S, true>(h, cfe, pp); else launch_volt_band<S, false>(h, cfe, pp); }
     else if (h->variant == 1) { if (pml) launch_volt_march<S, true>(h, cfe, pp); else launch_volt_march<S, false>(h, cfe, pp); }
     else if (pml) launch_volt_cm<S, true>(h, grd, blk, cfe, pp); else launch_volt_cm<S, false>(h, grd, blk, cfe, pp);
@@ -480,7 +552,7 @@ int launch_step_kernels(fdtd_engine* h, bool timed) {
         if ((a->flags & 2) && a->count)
             hipLaunchKernelGGL((k_ade_pre<S>), dim3((a->count + 255) / 256), dim3(256), 0, h->stream, h->I.as<S>(), ade_dev(a), h->d.sc, sc, 1);
     if (timed) HIPCHK(hipEventRecord(h->ev[evi + 2], h->stream));
-    if (h->variant == 3) { if (pml) launch_curr_vec<S, true>(h, cfh, pp); else launch_curr_vec<S, false>(h, cfh, pp); }
+    if (h->variant >= 3) { if (pml) launch_curr_vec<S, true>(h, cfh, pp); else launch_curr_vec<S, false>(h, cfh, pp); }
     else if (h->variant == 2) { if (pml) launch_curr_band<S, true>(h, cfh, pp); else launch_curr_band<S, false>(h, cfh, pp); }
     else if (h->variant == 1) { if (pml) launch_curr_march<S, true>(h, cfh, pp); else launch_curr_march<S, false>(h, cfh, pp); }
     else if (pml) launch_curr_cm<S, true>(h, grdc, blk, cfh, pp); else launch_curr_cm<S, false>(h, grdc, blk, cfh, pp);
@@ -1181,12 +1253,13 @@ int fdtd_read_dft(fdtd_engine* h, int id, double* out) {
 
 int fdtd_set_tuning(fdtd_engine* h, int variant, int tile_y, int chunk_x) {
     if (!h) return fail(FDTD_ERR_ARG, "null handle");
-    if (variant < 0 || variant > 3) return fail(FDTD_ERR_ARG, "unknown kernel variant %d", variant);
+    if (variant < 0 || variant > 4) return fail(FDTD_ERR_ARG, "unknown kernel variant %d", variant);
     if (tile_y != 1 && tile_y != 2 && tile_y != 4) return fail(FDTD_ERR_ARG, "tile_y must be 1, 2 or 4");
     if (chunk_x < 1) return fail(FDTD_ERR_ARG, "chunk_x must be >= 1");
     h->variant = variant;
     h->tile_y = tile_y;
     h->chunk_x = chunk_x;
+    if (variant == 4) h->fused_seg = chunk_x;
     invalidate_graph(h);
     return FDTD_OK;
 }
@@ -1268,6 +1341,8 @@ int fdtd_iterate(fdtd_engine* h, uint32_t nsteps) {
 // nsteps timesteps with no field-box sampling in between
 static int advance(fdtd_engine* h, uint32_t nsteps) {
     int rc;
+    const bool fused = fused_ok(h);
+    if (fused && (rc = ensure_fused_buffers(h))) return rc;
     if (h->timing) {
         // events for every launch of the two main kernels, harvested after the loop
         const size_t need = (size_t)4 * nsteps;
@@ -1300,6 +1375,26 @@ static int advance(fdtd_engine* h, uint32_t nsteps) {
     }
     const int G = 16;
     uint32_t left = nsteps;
+    if (fused) {
+        // the fused step swaps buffers: graphs hold an even number of steps
+        // and are replayed only from the buffer parity they were captured at
+        if (left >= (uint32_t)G) {
+            if (h->gexec && h->V.p != h->graph_base) {
+                if ((rc = launch_step(h, false))) return rc;
+                --left;
+            }
+            if (!h->gexec) {
+                h->graph_base = h->V.p;
+                if ((rc = capture_graph(h, G, &h->graph, &h->gexec))) return rc;
+            }
+            while (left >= (uint32_t)G) { HIPCHK(hipGraphLaunch(h->gexec, h->stream)); left -= G; }
+        }
+        for (; left; --left)
+            if ((rc = launch_step(h, false))) return rc;
+        HIPCHK(hipGetLastError());
+        h->numTS += nsteps;
+        return FDTD_OK;
+    }
     if (left >= (uint32_t)G) {
         if (!h->gexec) { if ((rc = capture_graph(h, G, &h->graph, &h->gexec))) return rc; }
         while (left >= (uint32_t)G) { HIPCHK(hipGraphLaunch(h->gexec, h->stream)); left -= G; }

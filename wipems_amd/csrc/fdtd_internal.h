@@ -89,7 +89,8 @@ struct fdtd_engine {
     size_t acc = 4;               // bytes per arithmetic value (coefficients)
     hipStream_t stream = nullptr;
 
-    DevBuf V, I;                  // fields
+    DevBuf V, I;                  // fields (the current ones)
+    DevBuf V2, I2;                // second buffers of the fused step (written by it, then swapped in)
     DevBuf step;                  // unsigned numTS on device
     uint32_t numTS = 0;
 
@@ -148,8 +149,11 @@ struct fdtd_engine {
     int graph_steps = 0;
     hipGraphExec_t gexec1 = nullptr;
     hipGraph_t graph1 = nullptr;
+    void* graph_base = nullptr;   // fused step: current V buffer when the graphs were captured
+    int fused_seg = 32;           // fused step: x planes per block
 
-    // kernel selection: 0 = one thread per cell, 1 = x-march, 2 = banded, 3 = vectors (default)
+    // kernel selection: 0 = one thread per cell, 1 = x-march, 2 = banded, 3 = vectors (default),
+    // 4 = fused step where the extensions allow it, else 3
     int variant = 3;
     int tile_y = 4;
     int chunk_x = 64;
@@ -178,6 +182,8 @@ namespace wipems {
 void invalidate_graph(fdtd_engine* h);
 int finalize_pml(fdtd_engine* h);
 int compress_pml(fdtd_engine* h);
+bool fused_ok(const fdtd_engine* h);
+int ensure_fused_buffers(fdtd_engine* h);
 inline bool pml_rows_in_use(const fdtd_engine* h) {
     return h->pml_cmp && (h->pml_cmp_mode == 1 || (h->pml_cmp_mode < 0 && h->cf_mode == CF_FULL));
 }
