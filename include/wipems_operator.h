@@ -106,6 +106,15 @@ int  wop_get_ec(wop_operator* op, double* C, double* G, double* L, double* R);
 /* compression of the coefficients: returns the number of unique 12-tuples and
  * (if index/table are non-null) the per-cell index and the table [n][12] */
 int  wop_compress(wop_operator* op, uint32_t* n_entries, uint32_t* index, double* table, uint32_t max_entries);
+/* Float build for large domains that never holds full per-cell arrays
+ * (Operator::CalcECOperator + Operator_SSE_Compressed::CompressOperator,
+ * FDTD/operator.cpp:973-1084, FDTD/operator_sse_compressed.cpp:120-176):
+ * x planes are built one at a time and deduplicated into index[Nx*Ny*Nz]
+ * (caller-owned) as they are finished.  Same index and table as wop_build
+ * followed by wop_compress.  No TFSF plane wave or Lorentz materials.
+ * wop_get_streamed_table copies the table [n_entries][12]. */
+int  wop_build_streamed(wop_operator* op, uint32_t nr_ts, uint32_t* index, uint32_t max_entries, uint32_t* n_entries);
+int  wop_get_streamed_table(wop_operator* op, double* table);
 
 int  wop_get_excitation_counts(wop_operator* op, uint32_t* volt_count, uint32_t* curr_count);
 int  wop_get_excitation(wop_operator* op, uint32_t* volt_pos, uint32_t* volt_dir, double* volt_amp, uint32_t* volt_delay,

@@ -1,0 +1,75 @@
+"""The streamed operator build (planes built and compressed one at a time,
+no full per-cell arrays) equals the full build followed by compression:
+same timestep, per-cell index, table, excitation lists and UPML arrays."""
+import numpy as np
+import pytest
+
+from wipems_amd import scenarios
+from wipems_amd.operator import BC_PEC, BC_PMC, BC_PML, Operator
+
+
+def compare(op, full, steps):
+    idx, tab = full.compress(0)
+    st = op.build_streamed(nr_ts=steps, max_entries=0)
+    assert st.dT == full.dT
+    assert st.nyquist == full.nyquist
+    np.testing.assert_array_equal(st.index, np.asarray(idx).reshape(st.index.shape))
+    np.testing.assert_array_equal(st.table, tab)
+    ef, es = full.excitation, st.excitation
+    if ef is None:
+        assert es is None
+    else:
+        for name in ("volt_pos", "volt_dir", "volt_amp", "volt_delay", "curr_pos", "curr_dir", "curr_amp", "curr_delay",
+                     "signal_volt", "signal_curr"):
+            np.testing.assert_array_equal(getattr(es, name), getattr(ef, name), err_msg=name)
+    assert len(st.upml) == len(full.upml)
+    for a, b in zip(st.upml, full.upml):
+        np.testing.assert_array_equal(a.start, b.start)
+        np.testing.assert_array_equal(a.num_lines, b.num_lines)
+        for x, y in zip(a.arrays, b.arrays):
+            np.testing.assert_array_equal(x, y)
+    np.testing.assert_array_equal(st.edge_primary, full.edge_primary)
+    assert st.vv is None and st.cells == full.cells
+    return st
+
+
+def test_graded_port_upml_lumped_boxes():
+    sc = scenarios.graded_port_scenario(n_cells=30, steps=50, prec=0, n_boxes=6, pml=6)
+    compare(sc.op, sc.data, 50)
+
+
+def test_uniform_cavity_and_batched_domain():
+    sc = scenarios.cavity_point_source(n_cells=17, steps=40, prec=0)
+    compare(sc.op, sc.data, 40)
+    sc = scenarios.batched_port_scenarios(count=1, lo=20, hi=26, steps=40)[0]
+    compare(sc.op, sc.data, 40)
+
+
+def test_metal_spheres_pmc_and_sine():
+    op = Operator()
+    rng = np.random.default_rng(3)
+    lines = [np.concatenate([[0.0], np.cumsum(rng.uniform(0.6, 1.4, n))]) for n in (19, 23, 17)]
+    op.set_mesh(lines, 1e-3)
+    op.set_boundary([BC_PEC, BC_PMC, BC_PML, BC_PEC, BC_PMC, BC_PML], [0, 0, 4, 0, 0, 4])
+    op.set_excite_signal(1, 3e9)          # sine
+    m = op.add_metal()
+    op.add_box(m, (lines[0][5], lines[1][6], lines[2][4]), (lines[0][9], lines[1][12], lines[2][8]), priority=5)
+    d = op.add_material(eps_r=4.0, kappa=0.05)
+    op.add_sphere(d, (lines[0][12], lines[1][10], lines[2][9]), 3.5, priority=2)
+    e = op.add_excitation(0, (0.0, 1.0, 0.5))
+    op.add_box(e, (lines[0][14], lines[1][3], lines[2][10]), (lines[0][14], lines[1][5], lines[2][12]), priority=9)
+    h = op.add_excitation(3, (1.0, 0.0, 0.0))
+    op.add_box(h, (lines[0][2], lines[1][2], lines[2][2]), (lines[0][3], lines[1][2], lines[2][2]), priority=9)
+    le = op.add_lumped_element(2, R=75.0, caps=True)
+    op.add_box(le, (lines[0][16], lines[1][16], lines[2][6]), (lines[0][16], lines[1][16], lines[2][9]), priority=8)
+    full = op.build(0, nr_ts=60)
+    compare(op, full, 60)
+
+
+def test_rejects_what_it_cannot_stream_and_table_limit():
+    sc = scenarios.plane_wave_box(n_cells=20, steps=10, pml=4, margin=2)
+    with pytest.raises(Exception):
+        sc.op.build_streamed(nr_ts=10)
+    sc = scenarios.graded_port_scenario(n_cells=20, steps=10, prec=0, n_boxes=3, pml=4)
+    with pytest.raises(Exception):
+        sc.op.build_streamed(nr_ts=10, max_entries=10)

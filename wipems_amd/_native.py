@@ -112,6 +112,8 @@ _OPERATOR_SIGS = {
     "wop_get_coeffs": [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p],
     "wop_get_ec": [c_void_p, POINTER(c_double), POINTER(c_double), POINTER(c_double), POINTER(c_double)],
     "wop_compress": [c_void_p, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_double), c_uint32],
+    "wop_build_streamed": [c_void_p, c_uint32, POINTER(c_uint32), c_uint32, POINTER(c_uint32)],
+    "wop_get_streamed_table": [c_void_p, POINTER(c_double)],
     "wop_get_excitation_counts": [c_void_p, POINTER(c_uint32), POINTER(c_uint32)],
     "wop_get_excitation": [c_void_p, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_double), POINTER(c_uint32),
                            POINTER(c_uint32), POINTER(c_uint32), POINTER(c_double), POINTER(c_uint32)],

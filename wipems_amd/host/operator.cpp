@@ -107,6 +107,10 @@ struct wop_operator {
     int exc_sig_type = WOP_EXC_GAUSS;
     double f0 = 0, fc = 0;
 
+    uint32_t ring = 0;              // plane slots of the streamed build (0: full arrays)
+    bool streamed = false;          // last build kept no full coefficient arrays
+    std::vector<double> stream_table;   // its unique coefficient rows [n][12]
+
     // ---- results ----
     bool built = false;
     int prec = 0;
@@ -142,7 +146,9 @@ struct wop_operator {
     std::vector<Ade> ade;
 
     // ---- mesh helpers (FDTD/operator.cpp:136-242) ----
-    size_t idx(uint32_t x, uint32_t y, uint32_t z) const { return ((size_t)x * N[1] + y) * N[2] + z; }
+    // storage index of a cell; with ring > 0 (streamed build) x planes share
+    // `ring` plane slots
+    size_t idx(uint32_t x, uint32_t y, uint32_t z) const { return ((size_t)(ring ? x % ring : x) * N[1] + y) * N[2] + z; }
     size_t nd() const { return (size_t)N[0] * N[1] * N[2]; }
 
     double DiscLine(int n, uint32_t pos, bool dual) const {
@@ -419,10 +425,17 @@ struct Builder {
         for (int k = 0; k < 4; ++k)
             for (int n = 0; n < 3; ++n) EC[k][n].assign(nd, (F)0);
         op->parallel_x(op->N[0], [&](uint32_t x) {
+            for (uint32_t y = 0; y < op->N[1]; ++y) ECRow(x, y);
+        });
+    }
+
+    // EC of the cells of row (x, y)
+    void ECRow(uint32_t x, uint32_t y) {
             uint32_t pos[3];
             pos[0] = x;
+            pos[1] = y;
             double EffMat[4];
-            for (pos[1] = 0; pos[1] < op->N[1]; ++pos[1]) {
+            {
                 std::vector<int> cand = op->RowCandidates((int)pos[0], (int)pos[1], P_MATERIAL);
                 for (pos[2] = 0; pos[2] < op->N[2]; ++pos[2]) {
                     const size_t i = op->idx(pos[0], pos[1], pos[2]);
@@ -441,13 +454,61 @@ struct Builder {
                     }
                 }
             }
-        });
+    }
+
+    void ECPlane(uint32_t x) {
+        op->parallel_x(op->N[1], [&](uint32_t y) { ECRow(x, y); });
     }
 
     // CalcTimestep_Var3 (FDTD/operator.cpp:1897-1971), AdrOp with
     // SetReflection2Cell: a shift of -1 at index 0 or +1 at the last index
     // stays on the same cell (tools/AdrOp.cpp:183-220)
     double CalcTimestep() {
+        const uint32_t* N = op->N;
+        double dT = 1e200;
+        for (int n = 0; n < 3; ++n) {
+            // per-z-plane minima computed in parallel, combined in the reference's loop order
+            std::vector<double> zmin(N[2], 1e200);
+            op->parallel_x(N[2], [&](uint32_t z) {
+                uint32_t pos[3];
+                pos[2] = z;
+                double best = 1e200;
+                for (pos[1] = 0; pos[1] < N[1]; ++pos[1]) {
+                    for (pos[0] = 0; pos[0] < N[0]; ++pos[0]) {
+                        const double newT = Var3Cell(n, pos);
+                        if ((newT < best) && (newT > 0.0)) best = newT;
+                    }
+                }
+                zmin[z] = best;
+            });
+            for (uint32_t z = 0; z < N[2]; ++z)
+                if (zmin[z] < dT && zmin[z] > 0.0) dT = zmin[z];
+        }
+        return dT;
+    }
+
+    // smallest positive Var3 timestep of the cells of plane x (EC of x-1..x+1 present)
+    double TimestepPlane(uint32_t x) {
+        const uint32_t* N = op->N;
+        std::vector<double> ymin(N[1], 1e200);
+        op->parallel_x(N[1], [&](uint32_t y) {
+            uint32_t pos[3] = {x, y, 0};
+            double best = 1e200;
+            for (int n = 0; n < 3; ++n)
+                for (pos[2] = 0; pos[2] < N[2]; ++pos[2]) {
+                    const double newT = Var3Cell(n, pos);
+                    if ((newT < best) && (newT > 0.0)) best = newT;
+                }
+            ymin[y] = best;
+        });
+        double dT = 1e200;
+        for (uint32_t y = 0; y < N[1]; ++y)
+            if (ymin[y] < dT && ymin[y] > 0.0) dT = ymin[y];
+        return dT;
+    }
+
+    // Var3 timestep estimate of direction n at one cell
+    double Var3Cell(int n, const uint32_t* pos) {
         const uint32_t* N = op->N;
         auto sh = [&](const uint32_t* p, int dir, int step) -> size_t {
             int q[3] = {(int)p[0], (int)p[1], (int)p[2]};
@@ -467,78 +528,69 @@ struct Builder {
             }
             return op->idx(q[0], q[1], q[2]);
         };
-        double dT = 1e200;
-        for (int n = 0; n < 3; ++n) {
-            const int nP = (n + 1) % 3, nPP = (n + 2) % 3;
-            const std::vector<F>& Cn = EC[0][n];
-            const std::vector<F>& CnP = EC[0][nP];
-            const std::vector<F>& CnPP = EC[0][nPP];
-            const std::vector<F>& LnP = EC[2][nP];
-            const std::vector<F>& LnPP = EC[2][nPP];
-            // per-z-plane minima computed in parallel, combined in the reference's loop order
-            std::vector<double> zmin(N[2], 1e200);
-            op->parallel_x(N[2], [&](uint32_t z) {
-                uint32_t pos[3];
-                pos[2] = z;
-                double best = 1e200;
-                for (pos[1] = 0; pos[1] < N[1]; ++pos[1]) {
-                    for (pos[0] = 0; pos[0] < N[0]; ++pos[0]) {
-                        // EC_* are FDTD_FLOAT: each 1/(L*C) term and the pair sums are
-                        // evaluated in F, then accumulated in double (operator.cpp:1920-1951)
-                        const F one = (F)1;
-                        size_t ipos = op->idx(pos[0], pos[1], pos[2]);
-                        double wqp;
-                        wqp = (double)(one / (LnPP[ipos] * Cn[sh(pos, nP, 1)]) + one / (LnPP[ipos] * Cn[ipos]));
-                        wqp += (double)(one / (LnP[ipos] * Cn[sh(pos, nPP, 1)]) + one / (LnP[ipos] * Cn[ipos]));
-                        // Shift(nP,-1): GetShiftedPos(nP,1) from there is the original cell
-                        size_t ip1 = sh(pos, nP, -1);
-                        wqp += (double)(one / (LnPP[ip1] * Cn[sh2(pos, nP, -1, nP, 1)]) + one / (LnPP[ip1] * Cn[ip1]));
-                        // Shift(nPP,-1) keeps the nP shift of the previous Shift call
-                        size_t ip2 = sh2(pos, nP, -1, nPP, -1);
-                        wqp += (double)(one / (LnP[ip2] * Cn[sh2(pos, nP, -1, nPP, 0)]) + one / (LnP[ip2] * Cn[ip2]));
+        const int nP = (n + 1) % 3, nPP = (n + 2) % 3;
+        const std::vector<F>& Cn = EC[0][n];
+        const std::vector<F>& CnP = EC[0][nP];
+        const std::vector<F>& CnPP = EC[0][nPP];
+        const std::vector<F>& LnP = EC[2][nP];
+        const std::vector<F>& LnPP = EC[2][nPP];
+        // EC_* are FDTD_FLOAT: each 1/(L*C) term and the pair sums are
+        // evaluated in F, then accumulated in double (operator.cpp:1920-1951)
+        const F one = (F)1;
+        size_t ipos = op->idx(pos[0], pos[1], pos[2]);
+        double wqp;
+        wqp = (double)(one / (LnPP[ipos] * Cn[sh(pos, nP, 1)]) + one / (LnPP[ipos] * Cn[ipos]));
+        wqp += (double)(one / (LnP[ipos] * Cn[sh(pos, nPP, 1)]) + one / (LnP[ipos] * Cn[ipos]));
+        // Shift(nP,-1): GetShiftedPos(nP,1) from there is the original cell
+        size_t ip1 = sh(pos, nP, -1);
+        wqp += (double)(one / (LnPP[ip1] * Cn[sh2(pos, nP, -1, nP, 1)]) + one / (LnPP[ip1] * Cn[ip1]));
+        // Shift(nPP,-1) keeps the nP shift of the previous Shift call
+        size_t ip2 = sh2(pos, nP, -1, nPP, -1);
+        wqp += (double)(one / (LnP[ip2] * Cn[sh2(pos, nP, -1, nPP, 0)]) + one / (LnP[ip2] * Cn[ip2]));
 
-                        double wt_4[4];
-                        wt_4[0] = (double)(one / (LnPP[ipos] * CnP[ipos]));
-                        wt_4[1] = (double)(one / (LnPP[sh(pos, nP, -1)] * CnP[ipos]));
-                        wt_4[2] = (double)(one / (LnP[ipos] * CnPP[ipos]));
-                        wt_4[3] = (double)(one / (LnP[sh(pos, nPP, -1)] * CnPP[ipos]));
-                        double wt1 = wt_4[0] + wt_4[1] + wt_4[2] + wt_4[3] - 2 * std::min(std::min(wt_4[0], wt_4[1]), std::min(wt_4[2], wt_4[3]));
+        double wt_4[4];
+        wt_4[0] = (double)(one / (LnPP[ipos] * CnP[ipos]));
+        wt_4[1] = (double)(one / (LnPP[sh(pos, nP, -1)] * CnP[ipos]));
+        wt_4[2] = (double)(one / (LnP[ipos] * CnPP[ipos]));
+        wt_4[3] = (double)(one / (LnP[sh(pos, nPP, -1)] * CnPP[ipos]));
+        double wt1 = wt_4[0] + wt_4[1] + wt_4[2] + wt_4[3] - 2 * std::min(std::min(wt_4[0], wt_4[1]), std::min(wt_4[2], wt_4[3]));
 
-                        size_t in1 = sh(pos, n, 1);
-                        wt_4[0] = (double)(one / (LnPP[ipos] * CnP[in1]));
-                        wt_4[1] = (double)(one / (LnPP[sh(pos, nP, -1)] * CnP[in1]));
-                        wt_4[2] = (double)(one / (LnP[ipos] * CnPP[in1]));
-                        wt_4[3] = (double)(one / (LnP[sh(pos, nPP, -1)] * CnPP[in1]));
-                        double wt2 = wt_4[0] + wt_4[1] + wt_4[2] + wt_4[3] - 2 * std::min(std::min(wt_4[0], wt_4[1]), std::min(wt_4[2], wt_4[3]));
+        size_t in1 = sh(pos, n, 1);
+        wt_4[0] = (double)(one / (LnPP[ipos] * CnP[in1]));
+        wt_4[1] = (double)(one / (LnPP[sh(pos, nP, -1)] * CnP[in1]));
+        wt_4[2] = (double)(one / (LnP[ipos] * CnPP[in1]));
+        wt_4[3] = (double)(one / (LnP[sh(pos, nPP, -1)] * CnPP[in1]));
+        double wt2 = wt_4[0] + wt_4[1] + wt_4[2] + wt_4[3] - 2 * std::min(std::min(wt_4[0], wt_4[1]), std::min(wt_4[2], wt_4[3]));
 
-                        double w_total = wqp + wt1 + wt2;
-                        double newT = 2 / sqrt(w_total);
-                        if ((newT < best) && (newT > 0.0)) best = newT;
-                    }
-                }
-                zmin[z] = best;
-            });
-            for (uint32_t z = 0; z < N[2]; ++z)
-                if (zmin[z] < dT && zmin[z] > 0.0) dT = zmin[z];
-        }
-        return dT;
+        double w_total = wqp + wt1 + wt2;
+        return 2 / sqrt(w_total);
     }
 
-    void ApplyElectricBC() {
+    // cells of plane only_x (all cells when only_x < 0): loop bounds per axis
+    void AxisRange(int only_x, uint32_t lo[3], uint32_t hi[3]) const {
+        for (int d = 0; d < 3; ++d) { lo[d] = 0; hi[d] = op->N[d]; }
+        if (only_x >= 0) { lo[0] = (uint32_t)only_x; hi[0] = (uint32_t)only_x + 1; }
+    }
+
+    void ApplyElectricBC(int only_x = -1) {
         bool dirs[6];
         for (int k = 0; k < 6; ++k) dirs[k] = op->BC[k] != -1;
         const uint32_t* N = op->N;
+        uint32_t lo[3], hi[3];
+        AxisRange(only_x, lo, hi);
         uint32_t pos[3];
         for (int n = 0; n < 3; ++n) {
             int nP = (n + 1) % 3, nPP = (n + 2) % 3;
-            for (pos[nP] = 0; pos[nP] < N[nP]; ++pos[nP])
-                for (pos[nPP] = 0; pos[nPP] < N[nPP]; ++pos[nPP]) {
-                    if (dirs[2 * n]) {
+            const bool in_lo = n != 0 || only_x < 0 || only_x == 0;
+            const bool in_hi = n != 0 || only_x < 0 || only_x == (int)N[0] - 1;
+            for (pos[nP] = lo[nP]; pos[nP] < hi[nP]; ++pos[nP])
+                for (pos[nPP] = lo[nPP]; pos[nPP] < hi[nPP]; ++pos[nPP]) {
+                    if (dirs[2 * n] && in_lo) {
                         pos[n] = 0;
                         size_t i = op->idx(pos[0], pos[1], pos[2]);
                         VV(nP, i) = 0; VI(nP, i) = 0; VV(nPP, i) = 0; VI(nPP, i) = 0;
                     }
-                    if (dirs[2 * n + 1]) {
+                    if (dirs[2 * n + 1] && in_hi) {
                         pos[n] = N[n] - 1;
                         size_t i = op->idx(pos[0], pos[1], pos[2]);
                         VV(n, i) = 0; VI(n, i) = 0;
@@ -548,28 +600,33 @@ struct Builder {
         }
     }
 
-    void ApplyMagneticBC() {
+    void ApplyMagneticBC(int only_x = -1) {
         bool dirs[6];
         for (int k = 0; k < 6; ++k) dirs[k] = op->BC[k] == 1;
         const uint32_t* N = op->N;
+        uint32_t lo[3], hi[3];
+        AxisRange(only_x, lo, hi);
         uint32_t pos[3];
         for (int n = 0; n < 3; ++n) {
             int nP = (n + 1) % 3, nPP = (n + 2) % 3;
-            for (pos[nP] = 0; pos[nP] < N[nP]; ++pos[nP])
-                for (pos[nPP] = 0; pos[nPP] < N[nPP]; ++pos[nPP]) {
-                    if (dirs[2 * n]) {
+            const bool at = n != 0 || only_x < 0;
+            for (pos[nP] = lo[nP]; pos[nP] < hi[nP]; ++pos[nP])
+                for (pos[nPP] = lo[nPP]; pos[nPP] < hi[nPP]; ++pos[nPP]) {
+                    if (dirs[2 * n] && (at || only_x == 0)) {
                         pos[n] = 0;
                         size_t i = op->idx(pos[0], pos[1], pos[2]);
                         II(n, i) = 0; IV(n, i) = 0; II(nP, i) = 0; IV(nP, i) = 0; II(nPP, i) = 0; IV(nPP, i) = 0;
                     }
-                    if (dirs[2 * n + 1]) {
+                    if (dirs[2 * n + 1] && (at || only_x == (int)N[0] - 2)) {
                         pos[n] = N[n] - 2;
                         size_t i = op->idx(pos[0], pos[1], pos[2]);
                         II(nP, i) = 0; IV(nP, i) = 0; II(nPP, i) = 0; IV(nPP, i) = 0;
                     }
-                    pos[n] = N[n] - 1;
-                    size_t i = op->idx(pos[0], pos[1], pos[2]);
-                    II(n, i) = 0; IV(n, i) = 0; II(nP, i) = 0; IV(nP, i) = 0; II(nPP, i) = 0; IV(nPP, i) = 0;
+                    if (at || only_x == (int)N[0] - 1) {
+                        pos[n] = N[n] - 1;
+                        size_t i = op->idx(pos[0], pos[1], pos[2]);
+                        II(n, i) = 0; IV(n, i) = 0; II(nP, i) = 0; IV(nP, i) = 0; II(nPP, i) = 0; IV(nPP, i) = 0;
+                    }
                 }
         }
     }
@@ -577,30 +634,35 @@ struct Builder {
     // CalcPEC_Range (FDTD/operator.cpp:1986-2020): edges whose Yee point lies in metal
     void CalcPEC() {
         op->parallel_x(op->N[0], [&](uint32_t x) {
-            uint32_t pos[3];
-            pos[0] = x;
-            double coord[3];
-            for (pos[1] = 0; pos[1] < op->N[1]; ++pos[1]) {
-                std::vector<int> cand = op->RowCandidates((int)pos[0], (int)pos[1], P_MATERIAL | P_METAL);
-                bool any_metal = false;
-                for (int c : cand) if (op->props[op->prims[c].prop].kind & P_METAL) any_metal = true;
-                if (!any_metal) continue;
-                for (pos[2] = 0; pos[2] < op->N[2]; ++pos[2])
-                    for (int n = 0; n < 3; ++n) {
-                        op->YeeCoords(n, pos, coord, false);
-                        int p = op->PropertyAt(coord, P_MATERIAL | P_METAL, &cand);
-                        if (p >= 0 && (op->props[p].kind & P_METAL)) {
-                            size_t i = op->idx(pos[0], pos[1], pos[2]);
-                            VV(n, i) = 0;
-                            VI(n, i) = 0;
-                        }
-                    }
-            }
+            for (uint32_t y = 0; y < op->N[1]; ++y) PECRow(x, y);
         });
     }
 
+    void PECPlane(uint32_t x) {
+        op->parallel_x(op->N[1], [&](uint32_t y) { PECRow(x, y); });
+    }
+
+    void PECRow(uint32_t x, uint32_t y) {
+        uint32_t pos[3] = {x, y, 0};
+        double coord[3];
+        std::vector<int> cand = op->RowCandidates((int)pos[0], (int)pos[1], P_MATERIAL | P_METAL);
+        bool any_metal = false;
+        for (int c : cand) if (op->props[op->prims[c].prop].kind & P_METAL) any_metal = true;
+        if (!any_metal) return;
+        for (pos[2] = 0; pos[2] < op->N[2]; ++pos[2])
+            for (int n = 0; n < 3; ++n) {
+                op->YeeCoords(n, pos, coord, false);
+                int p = op->PropertyAt(coord, P_MATERIAL | P_METAL, &cand);
+                if (p >= 0 && (op->props[p].kind & P_METAL)) {
+                    size_t i = op->idx(pos[0], pos[1], pos[2]);
+                    VV(n, i) = 0;
+                    VI(n, i) = 0;
+                }
+            }
+    }
+
     // Calc_LumpedElements (FDTD/operator.cpp:1535-1712), parallel RC boxes only
-    int LumpedElements() {
+    int LumpedElements(int only_x = -1) {
         for (const Primitive& pr : op->prims) {
             const Property& P = op->props[pr.prop];
             if (!(P.kind & P_LUMPED) || pr.shape != 0) continue;
@@ -638,6 +700,7 @@ struct Builder {
             for (pos[ny] = s[ny]; pos[ny] < e[ny]; ++pos[ny])
                 for (pos[nyP] = s[nyP]; pos[nyP] <= e[nyP]; ++pos[nyP])
                     for (pos[nyPP] = s[nyPP]; pos[nyPP] <= e[nyPP]; ++pos[nyPP]) {
+                        if (only_x >= 0 && pos[0] != (uint32_t)only_x) continue;
                         size_t i = op->idx(pos[0], pos[1], pos[2]);
                         if (C > 0) EC[0][ny][i] = (F)(epsilon * op->EdgeArea(ny, pos) / op->EdgeLength(ny, pos));
                         if (R > 0) EC[1][ny][i] = (F)(kappa * op->EdgeArea(ny, pos) / op->EdgeLength(ny, pos));
@@ -649,6 +712,7 @@ struct Builder {
                     for (pos[nyPP] = s[nyPP]; pos[nyPP] <= e[nyPP]; ++pos[nyPP]) {
                         for (int side = 0; side < 2; ++side) {
                             pos[ny] = side ? e[ny] : s[ny];
+                            if (only_x >= 0 && pos[0] != (uint32_t)only_x) continue;
                             size_t i = op->idx(pos[0], pos[1], pos[2]);
                             if (pos[nyP] < e[nyP]) { VV(nyP, i) = 0; VI(nyP, i) = 0; }
                             if (pos[nyPP] < e[nyPP]) { VV(nyPP, i) = 0; VI(nyPP, i) = 0; }
@@ -660,19 +724,25 @@ struct Builder {
     }
 
     // Operator_Ext_Excitation::BuildExtension (operator_ext_excitation.cpp:101-222), box primitives
-    void Excitation() {
-        op->exc_v.clear();
-        op->exc_i.clear();
+    // only_x >= 0: the entries of plane only_x, appended (the caller clears
+    // the lists and restores the z, y, x order afterwards)
+    void Excitation(int only_x = -1) {
+        if (only_x < 0) {
+            op->exc_v.clear();
+            op->exc_i.clear();
+        }
         const uint32_t* N = op->N;
         const double dT = op->dT;
         bool any = false;
         for (auto& p : op->prims) if (op->props[p.prop].kind & P_EXCITATION) any = true;
         if (!any) return;
+        uint32_t lo[3], hi[3];
+        AxisRange(only_x, lo, hi);
         uint32_t pos[3];
         double c[3];
         for (pos[2] = 0; pos[2] < N[2]; ++pos[2])
             for (pos[1] = 0; pos[1] < N[1]; ++pos[1])
-                for (pos[0] = 0; pos[0] < N[0]; ++pos[0]) {
+                for (pos[0] = lo[0]; pos[0] < hi[0]; ++pos[0]) {
                     for (int n = 0; n < 3; ++n) {
                         if (!op->YeeCoords(n, pos, c, false)) continue;
                         int p = op->PropertyAt(c, P_EXCITATION);
@@ -761,8 +831,11 @@ struct Builder {
 
     // Operator_Ext_UPML::Create_UPML (slab ranges, operator_ext_upml.cpp:77-209) and
     // BuildExtension (:365-466)
-    void Upml() {
+    // UPML slab ranges (Operator_Ext_UPML::Create_UPML) and the slabs' arrays
+    std::vector<std::vector<F>> upml_ext;   // [slab*6 + k][3*cnt]
+    void UpmlSetup() {
         op->upml.clear();
+        upml_ext.clear();
         int* BC = op->BC;
         uint32_t* S = op->pml_size;
         const uint32_t* N = op->N;
@@ -784,65 +857,96 @@ struct Builder {
         if (BC[4] == 3) { R r; start[2] = 0; stop[2] = S[4]; memcpy(r.s, start, 12); memcpy(r.e, stop, 12); ranges.push_back(r); }
         if (BC[5] == 3) { R r; start[2] = N[2] - 1 - S[5]; stop[2] = N[2] - 1; memcpy(r.s, start, 12); memcpy(r.e, stop, 12); ranges.push_back(r); }
 
-        const double dT = op->dT;
         for (const R& r : ranges) {
             wop_operator::Upml u;
             size_t cnt = 1;
             for (int n = 0; n < 3; ++n) { u.start[n] = r.s[n]; u.n[n] = r.e[n] - r.s[n] + 1; cnt *= u.n[n]; }
-            for (int k = 0; k < 6; ++k) u.a[k].assign(3 * cnt, 0.0);
-            std::vector<F> ext[6];
-            for (int k = 0; k < 6; ++k) ext[k].assign(3 * cnt, (F)0);
-            op->parallel_x(u.n[0], [&](uint32_t lx) {
-                uint32_t loc[3], pos[3];
-                loc[0] = lx;
-                pos[0] = lx + u.start[0];
-                double eff[4], kv[3], ki[3];
-                for (loc[1] = 0; loc[1] < u.n[1]; ++loc[1]) {
-                    pos[1] = loc[1] + u.start[1];
-                    std::vector<int> cand = op->RowCandidates((int)pos[0], (int)pos[1], P_MATERIAL);
-                    for (loc[2] = 0; loc[2] < u.n[2]; ++loc[2]) {
-                        pos[2] = loc[2] + u.start[2];
-                        const size_t i = op->idx(pos[0], pos[1], pos[2]);
-                        for (int n = 0; n < 3; ++n) {
-                            const size_t li = (size_t)n * cnt + ((size_t)loc[0] * u.n[1] + loc[1]) * u.n[2] + loc[2];
-                            op->AverageMatQuarterCell(n, pos, eff, cand);
-                            GradingKappa(n, pos, Z0, kv, ki);
-                            const int nP = (n + 1) % 3, nPP = (n + 2) % 3;
-                            if (((kv[0] + kv[1] + kv[2]) != 0) && (eff[1] < 1e3)) {
-                                if ((VV(n, i) + VI(n, i)) != 0) {
-                                    VV(n, i) = (F)((2 * EPS0 - kv[nP] * dT) / (2 * EPS0 + kv[nP] * dT));
-                                    VI(n, i) = (F)((2 * EPS0 * dT) / (2 * EPS0 + kv[nP] * dT) * op->EdgeLength(n, pos) / op->EdgeArea(n, pos));
-                                    ext[0][li] = (F)((2 * EPS0 - kv[nPP] * dT) / (2 * EPS0 + kv[nPP] * dT));
-                                    ext[2][li] = (F)((2 * EPS0 + kv[n] * dT) / (2 * EPS0 + kv[nPP] * dT) / eff[0]);
-                                    ext[1][li] = (F)((2 * EPS0 - kv[n] * dT) / (2 * EPS0 + kv[nPP] * dT) / eff[0]);
-                                }
-                            } else {
-                                ext[0][li] = VV(n, i);
-                                VV(n, i) = 0;
-                                ext[1][li] = 0;
-                                ext[2][li] = 1;
-                            }
-                            if ((ki[0] + ki[1] + ki[2]) != 0) {
-                                if ((II(n, i) + IV(n, i)) != 0) {
-                                    II(n, i) = (F)((2 * EPS0 - ki[nP] * dT) / (2 * EPS0 + ki[nP] * dT));
-                                    IV(n, i) = (F)((2 * EPS0 * dT) / (2 * EPS0 + ki[nP] * dT) * op->EdgeLength(n, pos, true) / op->EdgeArea(n, pos, true));
-                                    ext[3][li] = (F)((2 * EPS0 - ki[nPP] * dT) / (2 * EPS0 + ki[nPP] * dT));
-                                    ext[5][li] = (F)((2 * EPS0 + ki[n] * dT) / (2 * EPS0 + ki[nPP] * dT) / eff[2]);
-                                    ext[4][li] = (F)((2 * EPS0 - ki[n] * dT) / (2 * EPS0 + ki[nPP] * dT) / eff[2]);
-                                }
-                            } else {
-                                ext[3][li] = II(n, i);
-                                II(n, i) = 0;
-                                ext[4][li] = 0;
-                                ext[5][li] = 1;
-                            }
-                        }
-                    }
-                }
-            });
-            for (int k = 0; k < 6; ++k)
-                for (size_t i = 0; i < 3 * cnt; ++i) u.a[k][i] = (double)ext[k][i];
             op->upml.push_back(std::move(u));
+            for (int k = 0; k < 6; ++k) upml_ext.emplace_back(3 * cnt, (F)0);
+        }
+    }
+
+    // cells of slab si at local plane lx, local row ly
+    void UpmlRow(size_t si, uint32_t lx, uint32_t ly) {
+        const double dT = op->dT;
+        const wop_operator::Upml& u = op->upml[si];
+        const size_t cnt = (size_t)u.n[0] * u.n[1] * u.n[2];
+        std::vector<F>* ext = &upml_ext[si * 6];
+        uint32_t loc[3], pos[3];
+        loc[0] = lx;
+        loc[1] = ly;
+        pos[0] = lx + u.start[0];
+        pos[1] = ly + u.start[1];
+        double eff[4], kv[3], ki[3];
+        std::vector<int> cand = op->RowCandidates((int)pos[0], (int)pos[1], P_MATERIAL);
+        for (loc[2] = 0; loc[2] < u.n[2]; ++loc[2]) {
+            pos[2] = loc[2] + u.start[2];
+            const size_t i = op->idx(pos[0], pos[1], pos[2]);
+            for (int n = 0; n < 3; ++n) {
+                const size_t li = (size_t)n * cnt + ((size_t)loc[0] * u.n[1] + loc[1]) * u.n[2] + loc[2];
+                op->AverageMatQuarterCell(n, pos, eff, cand);
+                GradingKappa(n, pos, Z0, kv, ki);
+                const int nP = (n + 1) % 3, nPP = (n + 2) % 3;
+                if (((kv[0] + kv[1] + kv[2]) != 0) && (eff[1] < 1e3)) {
+                    if ((VV(n, i) + VI(n, i)) != 0) {
+                        VV(n, i) = (F)((2 * EPS0 - kv[nP] * dT) / (2 * EPS0 + kv[nP] * dT));
+                        VI(n, i) = (F)((2 * EPS0 * dT) / (2 * EPS0 + kv[nP] * dT) * op->EdgeLength(n, pos) / op->EdgeArea(n, pos));
+                        ext[0][li] = (F)((2 * EPS0 - kv[nPP] * dT) / (2 * EPS0 + kv[nPP] * dT));
+                        ext[2][li] = (F)((2 * EPS0 + kv[n] * dT) / (2 * EPS0 + kv[nPP] * dT) / eff[0]);
+                        ext[1][li] = (F)((2 * EPS0 - kv[n] * dT) / (2 * EPS0 + kv[nPP] * dT) / eff[0]);
+                    }
+                } else {
+                    ext[0][li] = VV(n, i);
+                    VV(n, i) = 0;
+                    ext[1][li] = 0;
+                    ext[2][li] = 1;
+                }
+                if ((ki[0] + ki[1] + ki[2]) != 0) {
+                    if ((II(n, i) + IV(n, i)) != 0) {
+                        II(n, i) = (F)((2 * EPS0 - ki[nP] * dT) / (2 * EPS0 + ki[nP] * dT));
+                        IV(n, i) = (F)((2 * EPS0 * dT) / (2 * EPS0 + ki[nP] * dT) * op->EdgeLength(n, pos, true) / op->EdgeArea(n, pos, true));
+                        ext[3][li] = (F)((2 * EPS0 - ki[nPP] * dT) / (2 * EPS0 + ki[nPP] * dT));
+                        ext[5][li] = (F)((2 * EPS0 + ki[n] * dT) / (2 * EPS0 + ki[nPP] * dT) / eff[2]);
+                        ext[4][li] = (F)((2 * EPS0 - ki[n] * dT) / (2 * EPS0 + ki[nPP] * dT) / eff[2]);
+                    }
+                } else {
+                    ext[3][li] = II(n, i);
+                    II(n, i) = 0;
+                    ext[4][li] = 0;
+                    ext[5][li] = 1;
+                }
+            }
+        }
+    }
+
+    void UpmlFinish() {
+        for (size_t si = 0; si < op->upml.size(); ++si) {
+            wop_operator::Upml& u = op->upml[si];
+            for (int k = 0; k < 6; ++k) {
+                const std::vector<F>& e = upml_ext[si * 6 + k];
+                u.a[k].assign(e.size(), 0.0);
+                for (size_t i = 0; i < e.size(); ++i) u.a[k][i] = (double)e[i];
+            }
+        }
+        upml_ext.clear();
+    }
+
+    void Upml() {
+        UpmlSetup();
+        for (size_t si = 0; si < op->upml.size(); ++si) {
+            const wop_operator::Upml& u = op->upml[si];
+            op->parallel_x(u.n[0], [&](uint32_t lx) {
+                for (uint32_t ly = 0; ly < u.n[1]; ++ly) UpmlRow(si, lx, ly);
+            });
+        }
+        UpmlFinish();
+    }
+
+    void UpmlPlane(uint32_t x) {
+        for (size_t si = 0; si < op->upml.size(); ++si) {
+            const wop_operator::Upml& u = op->upml[si];
+            if (x < u.start[0] || x >= u.start[0] + u.n[0]) continue;
+            op->parallel_x(u.n[1], [&](uint32_t ly) { UpmlRow(si, x - u.start[0], ly); });
         }
     }
 
@@ -1215,6 +1319,9 @@ template <typename F>
 int wop_operator::build(uint32_t nr_ts) {
     sf.clear();
     sd.clear();
+    ring = 0;
+    streamed = false;
+    stream_table.clear();
     Builder<F> b(this);
     b.CalcEC();
     // CalcECOperator timestep handling (FDTD/operator.cpp:981-1007)
@@ -1509,6 +1616,7 @@ static void put(const std::vector<double>& src, int dtype, void* dst) {
 
 int wop_get_coeffs(wop_operator* op, int dtype, void* vv, void* vi, void* ii, void* iv) {
     if (!op || !op->built) return fail(-1, "operator not built");
+    if (op->streamed) return fail(-1, "the streamed build keeps no full coefficient arrays");
     void* dst[4] = {vv, vi, ii, iv};
     for (int k = 0; k < 4; ++k) {
         if (op->prec == 0) put3(op->sf.co[k], dtype, dst[k]);
@@ -1519,6 +1627,7 @@ int wop_get_coeffs(wop_operator* op, int dtype, void* vv, void* vi, void* ii, vo
 
 int wop_get_ec(wop_operator* op, double* C, double* G, double* L, double* R) {
     if (!op || !op->built) return fail(-1, "operator not built");
+    if (op->streamed) return fail(-1, "the streamed build keeps no EC arrays");
     void* dst[4] = {C, G, L, R};
     for (int k = 0; k < 4; ++k) {
         if (op->prec == 0) put3(op->sf.EC[k], 1, dst[k]);
@@ -1535,12 +1644,140 @@ struct TupleHash {
         return (size_t)h;
     }
 };
+
+// Float build that never holds full per-cell arrays: x planes pass through a
+// ring of three plane slots.  Pass 1 computes EC plane by plane and the
+// Var3 timestep from planes x-1..x+1; pass 2 recomputes EC, the coefficients,
+// boundary conditions, PEC, lumped elements, excitation and UPML of each
+// plane in the full build's order (every step touches only its own cell) and
+// deduplicates the plane's 12-tuples into index[] in cell order -- the same
+// index and table the full build followed by wop_compress gives.
+int stream_build(wop_operator* op, uint32_t nr_ts, uint32_t* index, uint32_t max_entries, uint32_t* n_entries) {
+    if (op->pw.set) return fail(-1, "the streamed build has no TFSF plane wave");
+    for (auto& p : op->props)
+        if (p.lorentz) return fail(-1, "the streamed build has no Lorentz/Drude materials");
+    const uint32_t* N = op->N;
+    op->sf.clear();
+    op->sd.clear();
+    op->built = false;
+    op->prec = 0;
+    op->ring = 3;
+    op->streamed = true;
+    op->stream_table.clear();
+    const size_t plane = (size_t)N[1] * N[2];
+    Builder<float> b(op);
+    for (int k = 0; k < 4; ++k)
+        for (int d = 0; d < 3; ++d) { b.EC[k][d].assign(3 * plane, 0.0f); b.co[k][d].assign(3 * plane, 0.0f); }
+    // pass 1: timestep
+    double dT = 1e200;
+    b.ECPlane(0);
+    if (N[0] > 1) b.ECPlane(1);
+    for (uint32_t x = 0; x < N[0]; ++x) {
+        if (x >= 1 && x + 1 < N[0]) b.ECPlane(x + 1);
+        const double t = b.TimestepPlane(x);
+        if (t < dT && t > 0.0) dT = t;
+    }
+    if (op->forced_dT > 0) op->dT = op->forced_dT;
+    else op->dT = dT;
+    op->dT *= op->ts_factor;
+    if (op->exc_sig_type == WOP_EXC_SINE && op->f0 > 0) {
+        double period = 1 / op->f0;
+        unsigned TS = (unsigned)ceil(period / op->dT);
+        op->dT = period / TS;
+    }
+    // pass 2: coefficients plane by plane, compressed as they are finished
+    b.UpmlSetup();
+    op->exc_v.clear();
+    op->exc_i.clear();
+    op->tfsf_v.clear();
+    op->tfsf_i.clear();
+    op->tfsf_max_delay = 0;
+    op->ade.clear();
+    std::unordered_map<std::array<uint64_t, 12>, uint32_t, TupleHash> map;
+    std::vector<std::array<uint64_t, 12>> uniq;
+    for (uint32_t x = 0; x < N[0]; ++x) {
+        b.ECPlane(x);
+        op->parallel_x(N[1], [&](uint32_t y) {
+            for (int d = 0; d < 3; ++d)
+                for (uint32_t z = 0; z < N[2]; ++z) b.ECOperatorPos(d, op->idx(x, y, z));
+        });
+        b.ApplyElectricBC((int)x);
+        b.PECPlane(x);
+        b.LumpedElements((int)x);
+        b.ApplyMagneticBC((int)x);
+        b.Excitation((int)x);
+        b.UpmlPlane(x);
+        for (uint32_t y = 0; y < N[1]; ++y)
+            for (uint32_t z = 0; z < N[2]; ++z) {
+                const size_t i = op->idx(x, y, z);
+                std::array<uint64_t, 12> t;
+                for (int k = 0; k < 4; ++k)
+                    for (int d = 0; d < 3; ++d) {
+                        double v = (double)b.co[k][d][i];
+                        uint64_t bits;
+                        memcpy(&bits, &v, 8);
+                        t[k * 3 + d] = bits;
+                    }
+                auto it = map.find(t);
+                uint32_t id;
+                if (it == map.end()) {
+                    id = (uint32_t)uniq.size();
+                    map.emplace(t, id);
+                    uniq.push_back(t);
+                    if (max_entries && uniq.size() > max_entries) {
+                        op->ring = 0;
+                        *n_entries = (uint32_t)uniq.size();
+                        return fail(-2, "more than %u unique coefficient tuples", max_entries);
+                    }
+                } else id = it->second;
+                index[((size_t)x * N[1] + y) * N[2] + z] = id;
+            }
+    }
+    b.UpmlFinish();
+    // excitation lists in the full build's z, y, x order
+    auto zyx = [](const wop_operator::Exc& a, const wop_operator::Exc& c) {
+        if (a.pos[2] != c.pos[2]) return a.pos[2] < c.pos[2];
+        if (a.pos[1] != c.pos[1]) return a.pos[1] < c.pos[1];
+        return a.pos[0] < c.pos[0];
+    };
+    std::stable_sort(op->exc_v.begin(), op->exc_v.end(), zyx);
+    std::stable_sort(op->exc_i.begin(), op->exc_i.end(), zyx);
+    b.Signal(nr_ts);
+    b.EC[0][0].clear();
+    for (int k = 0; k < 4; ++k)
+        for (int d = 0; d < 3; ++d) { std::vector<float>().swap(b.EC[k][d]); std::vector<float>().swap(b.co[k][d]); }
+    op->stream_table.resize(uniq.size() * 12);
+    for (size_t r = 0; r < uniq.size(); ++r)
+        for (int k = 0; k < 12; ++k) memcpy(&op->stream_table[r * 12 + k], &uniq[r][k], 8);
+    *n_entries = (uint32_t)uniq.size();
+    op->ring = 0;
+    op->built = true;
+    return 0;
+}
 }  // namespace
 
 }  // extern "C"
 
+extern "C" int wop_build_streamed(wop_operator* op, uint32_t nr_ts, uint32_t* index, uint32_t max_entries, uint32_t* n_entries) {
+    if (!op || !index || !n_entries) return fail(-1, "bad argument");
+    if (op->lines[0].empty() || op->lines[1].empty() || op->lines[2].empty()) return fail(-1, "mesh not set");
+    try {
+        return stream_build(op, nr_ts, index, max_entries, n_entries);
+    } catch (const std::bad_alloc&) {
+        op->ring = 0;
+        return fail(-3, "out of host memory");
+    }
+}
+
+extern "C" int wop_get_streamed_table(wop_operator* op, double* table) {
+    if (!op || !table || !op->built || !op->streamed) return fail(-1, "no streamed build");
+    memcpy(table, op->stream_table.data(), op->stream_table.size() * sizeof(double));
+    return 0;
+}
+
 extern "C" int wop_compress(wop_operator* op, uint32_t* n_entries, uint32_t* index, double* table, uint32_t max_entries) {
     if (!op || !op->built || !n_entries) return fail(-1, "operator not built");
+    if (op->streamed) return fail(-1, "the streamed build is compressed already (wop_get_streamed_table)");
     const size_t n = op->nd();
     std::unordered_map<std::array<uint64_t, 12>, uint32_t, TupleHash> map;
     std::vector<std::array<uint64_t, 12>> uniq;

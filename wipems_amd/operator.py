@@ -282,11 +282,34 @@ class Operator:
         n = (c_uint32 * 3)()
         self._chk(lib.wop_get_num_lines(h, n))
         shape = (3, n[0], n[1], n[2])
-        dT = c_double()
-        self._chk(lib.wop_get_timestep(h, byref(dT)))
         dt = np.float64 if prec == 1 else np.float32
         arrs = [np.empty(shape, dtype=dt) for _ in range(4)]
         self._chk(lib.wop_get_coeffs(h, 1 if prec == 1 else 0, *[_vp(a) for a in arrs]))
+        return self._collect(prec, arrs)
+
+    def build_streamed(self, nr_ts=1_000_000_000, max_entries=65536) -> OperatorData:
+        """Float build that never holds full per-cell arrays (wop_build_streamed):
+        returns OperatorData with the compressed index/table and no vv..iv."""
+        lib, h = self._lib, self._h
+        n = (c_uint32 * 3)()
+        self._chk(lib.wop_get_num_lines(h, n))
+        index = np.empty((n[0], n[1], n[2]), dtype=np.uint32)
+        nu = c_uint32()
+        self._chk(lib.wop_build_streamed(h, int(nr_ts), _up(index), int(max_entries), byref(nu)))
+        table = np.empty((nu.value, 12), dtype=np.float64)
+        self._chk(lib.wop_get_streamed_table(h, _dp(table)))
+        data = self._collect(0, [None] * 4)
+        data.index, data.table = index, table
+        data.source = None
+        return data
+
+    def _collect(self, prec, arrs) -> OperatorData:
+        lib, h = self._lib, self._h
+        n = (c_uint32 * 3)()
+        self._chk(lib.wop_get_num_lines(h, n))
+        dt = np.float64 if prec == 1 else np.float32
+        dT = c_double()
+        self._chk(lib.wop_get_timestep(h, byref(dT)))
         length, nyq, mstep = c_uint32(), c_uint32(), c_uint32()
         self._chk(lib.wop_get_excite_info(h, byref(length), byref(nyq), byref(mstep)))
         sv = np.empty(length.value, dtype=np.float64)
