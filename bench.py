@@ -43,6 +43,12 @@ WORKLOADS = {
                       "385x385x193 lines at 0.25 mm, UPML-8, Drude slab + 16 Lorentz spheres (material frequencies x1e-3, "
                       "relaxation times x1e3, timestep factor 0.9: the stated THz/PHz values diverge in the reference ADE "
                       "scheme), TFSF plane wave +z / Ex (30/30 GHz Gaussian), 20000 steps"),
+    "large_dipole": ("large_dipole_scenario", {"num_lines": (1537, 1537, 769), "n_boxes": 200, "steps": 3000, "verbose": True},
+                     "1537x1537x769 lines (1.82 G cells) at 1 mm, UPML-8, 200 random dielectric boxes (eps_r ~ U[1,12]), "
+                     "z dipole of 20 edges (Gaussian f0=fc=2 GHz), FP64 DFT of E and H at 16 frequencies on the six faces "
+                     "of an nf2ff box 10 cells inside the UPML, streamed compressed operator build"),
+    "large_dipole_385": ("large_dipole_scenario", {"num_lines": (385, 385, 193), "n_boxes": 200, "steps": 3000},
+                         "the large dipole workload on 385x385x193 lines (28.6 M cells)"),
     "batched_64": ("batched_port_scenarios", {"count": 64, "steps": 5000},
                    "64 independent domains, per-axis cells U{48..160} at 1 mm, UPML-8, 8 eps boxes, 50 ohm port (3/3 GHz), "
                    "5000 steps, one launch per update phase for all 64 (graph-captured), vs 64 sequential single-domain runs"),
@@ -236,7 +242,10 @@ def main():
     scales = None
     if args.precision in ("fp16", "bf16"):
         t0 = time.time()
-        scales = pilot_scales(data, compressed=compressed)
+        # streamed (large) workloads: the pilot covers the excitation only,
+        # where the fields near the source peak
+        pilot = max(64, data.excitation.length) if data.vv is None else None
+        scales = pilot_scales(data, steps=pilot, compressed=compressed, samples=2 if data.vv is None else 8)
         log("storage scales (pilot FP32 run, %.1f s): volt %s curr %s" % (time.time() - t0, scales[0], scales[1]))
     eng = engine_from_operator(data, precision=args.precision, compressed=compressed, scales=scales)
     if args.variant is not None:
@@ -285,7 +294,9 @@ def main():
     cpu_steps = args.cpu_steps
     if cpu_steps is None:
         cpu_steps = max(10, int(1.5e9 / cells))   # about 10-30 s of host work at ~100 MCells/s
-    if cpu_steps > 0 and args.precision == "fp32" and not data.ade and not data.tfsf:
+    if data.vv is None:
+        log("cpu baseline skipped: the streamed build keeps no full coefficient arrays for the CPU engines")
+    elif cpu_steps > 0 and args.precision == "fp32" and not data.ade and not data.tfsf:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         cpu = cpu_baseline_leg(data, sc.meta.get("interval", 0), cells, threads, cpu_steps)
         log("cpu baseline: %.1f MCells/s on %d threads" % (cpu["value"], cpu["cores"]))
@@ -301,7 +312,10 @@ def main():
     log("plain copy kernel: %.1f GB/s" % copy_gbs)
 
     err = None
-    if not args.no_fp64_check and args.precision != "fp64":
+    if data.vv is None:
+        if args.precision in ("fp16", "bf16") and not args.no_fp64_check:
+            err = fp32_check(data, eng, sc)
+    elif not args.no_fp64_check and args.precision != "fp64":
         err = fp64_check(builder, kw, eng, sc, steps, args)
 
     out = {
@@ -314,6 +328,36 @@ def main():
         "roofline": roofline, "cpu_baseline": cpu, "error_vs_fp64": err,
     }
     print(json.dumps(out), flush=True)
+
+
+def fp32_check(data, eng, sc):
+    """Streamed (float-only) workloads have no FP64 operator: compare a reduced-
+    precision run with the FP32 engine on the same operator, probe series and
+    the DFT accumulators of every field box."""
+    import numpy as np
+    from wipems_amd.engine import engine_from_operator
+    t0 = time.time()
+    e32 = engine_from_operator(data, precision="fp32")
+    total = eng.GetNumberOfTimesteps()
+    t1 = time.perf_counter()
+    e32.iterate(total)
+    e32.synchronize()
+    wall = time.perf_counter() - t1
+    out = {"reference": "fp32 engine", "steps": total,
+           "fp32_ms_per_step": round(1e3 * wall / total, 4), "fp32_MCells_s": round(data.cells * total / wall / 1e6, 1)}
+    for pid, p in enumerate(data.probes):
+        _, v = eng.read_probe(pid)
+        _, v32 = e32.read_probe(pid)
+        out["probe_%s_rel_l2" % (p.name or pid)] = float(np.linalg.norm(v - v32) / max(np.linalg.norm(v32), 1e-300))
+    worst = 0.0
+    for k, b in enumerate(data.dft_boxes):
+        a, r = eng.read_dft(k), e32.read_dft(k)
+        worst = max(worst, float(np.linalg.norm(a - r) / max(np.linalg.norm(r), 1e-300)))
+    if data.dft_boxes:
+        out["dft_max_rel_l2"] = worst
+    e32.close()
+    log("fp32 check (%.1f s): %s" % (time.time() - t0, out))
+    return out
 
 
 def fp64_check(builder, kw, eng, sc, steps, args):

@@ -62,6 +62,9 @@ int  wop_set_timestep_factor(wop_operator* op, double factor);
 /* optional forced timestep (openems.cpp TimeStep): 0 = calculate */
 int  wop_set_timestep(wop_operator* op, double dT);
 int  wop_set_threads(wop_operator* op, int nthreads);
+/* 1: the streamed build prints a progress line to stderr every 64 x planes
+ * (openEMS' verbose level, openems.h:120 SetVerboseLevel) */
+int  wop_set_verbose(wop_operator* op, int level);
 
 /* properties (CSXCAD stand-ins); each returns its id */
 int  wop_add_material(wop_operator* op, double eps_r, double kappa, double mue_r, double sigma, int* id);

@@ -73,3 +73,33 @@ def test_rejects_what_it_cannot_stream_and_table_limit():
     sc = scenarios.graded_port_scenario(n_cells=20, steps=10, prec=0, n_boxes=3, pml=4)
     with pytest.raises(Exception):
         sc.op.build_streamed(nr_ts=10, max_entries=10)
+
+
+def test_many_boxes_dipole_and_plane_rejection():
+    """200 dielectric boxes, a dipole crossing few x planes (the excitation
+    scan skips the others) and an nf2ff box: the large dipole workload at a
+    small size."""
+    sc = scenarios.large_dipole_scenario(num_lines=(41, 37, 33), n_boxes=200, steps=50, pml=4, box_margin=3,
+                                         dipole_cells=6, threads=4)
+    full = sc.op.build(0, nr_ts=50)
+    st = compare(sc.op, full, 50)
+    assert st.excitation.volt_pos.shape[1] == 6
+
+
+def test_long_graded_rows_grow_the_row_index():
+    """Rows of 150 distinct tuples (graded z, UPML along z) pass the 16-tuple
+    linear search and several growths of the row's hash index."""
+    rng = np.random.default_rng(11)
+    lines = [np.concatenate([[0.0], np.cumsum(rng.uniform(0.5, 2.0, n))]) for n in (8, 6, 150)]
+    op = Operator()
+    op.set_mesh(lines, 1e-3)
+    op.set_boundary([BC_PEC, BC_PEC, BC_PMC, BC_PMC, BC_PML, BC_PML], [0, 0, 0, 0, 6, 6])
+    op.set_gauss_excite(2e9, 2e9)
+    d = op.add_material(eps_r=3.0, kappa=0.01)
+    op.add_box(d, (lines[0][2], lines[1][1], lines[2][40]), (lines[0][6], lines[1][4], lines[2][90]), priority=1)
+    e = op.add_excitation(0, (0.0, 0.0, 1.0))
+    op.add_box(e, (lines[0][4], lines[1][3], lines[2][70]), (lines[0][4], lines[1][3], lines[2][72]), priority=5)
+    full = op.build(0, nr_ts=30)
+    st = compare(op, full, 30)
+    row = st.index[3, 2]
+    assert len(np.unique(row)) > 100

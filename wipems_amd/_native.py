@@ -91,6 +91,7 @@ _OPERATOR_SIGS = {
     "wop_set_timestep_factor": [c_void_p, c_double],
     "wop_set_timestep": [c_void_p, c_double],
     "wop_set_threads": [c_void_p, c_int],
+    "wop_set_verbose": [c_void_p, c_int],
     "wop_add_material": [c_void_p, c_double, c_double, c_double, c_double, POINTER(c_int)],
     "wop_add_metal": [c_void_p, POINTER(c_int)],
     "wop_add_lorentz_material": [c_void_p] + [c_double] * 7 + [POINTER(c_int)],

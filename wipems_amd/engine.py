@@ -273,19 +273,20 @@ class Engine:
         return b.value
 
 
-def pilot_scales(data: OperatorData, steps=None, headroom=1024.0, compressed=None):
+def pilot_scales(data: OperatorData, steps=None, headroom=1024.0, compressed=None, samples=8):
     """Per-component power-of-two storage scales for FP16/BF16 handles.
 
     Runs the FP32 engine for `steps` (default: twice the excitation length)
-    and records max |V_n| and |I_n|; the scale of a component is the power of
-    two nearest to max/headroom, so stored values peak near `headroom`
-    (FP16 max 65504) and scaling itself is exact."""
+    and records max |V_n| and |I_n| at `samples` evenly spaced steps; the
+    scale of a component is the power of two nearest to max/headroom, so
+    stored values peak near `headroom` (FP16 max 65504) and scaling itself
+    is exact."""
     if steps is None:
         steps = max(64, 2 * (data.excitation.length if data.excitation is not None else 64))
     eng = engine_from_operator(data, precision="fp32", compressed=compressed)
     peak_v = np.zeros(3)
     peak_i = np.zeros(3)
-    chunk = max(1, steps // 8)
+    chunk = max(1, steps // samples)
     done = 0
     while done < steps:
         eng.iterate(chunk)

@@ -108,11 +108,13 @@ struct wop_operator {
     double f0 = 0, fc = 0;
 
     uint32_t ring = 0;              // plane slots of the streamed build (0: full arrays)
+    std::vector<uint32_t> ring_slot;    // x % ring
     bool streamed = false;          // last build kept no full coefficient arrays
     std::vector<double> stream_table;   // its unique coefficient rows [n][12]
 
     // ---- results ----
     bool built = false;
+    int verbose = 0;
     int prec = 0;
     double dT = 0;
     Store<float> sf;                       // float build results
@@ -148,7 +150,7 @@ struct wop_operator {
     // ---- mesh helpers (FDTD/operator.cpp:136-242) ----
     // storage index of a cell; with ring > 0 (streamed build) x planes share
     // `ring` plane slots
-    size_t idx(uint32_t x, uint32_t y, uint32_t z) const { return ((size_t)(ring ? x % ring : x) * N[1] + y) * N[2] + z; }
+    size_t idx(uint32_t x, uint32_t y, uint32_t z) const { return ((size_t)(ring ? ring_slot[x] : x) * N[1] + y) * N[2] + z; }
     size_t nd() const { return (size_t)N[0] * N[1] * N[2]; }
 
     double DiscLine(int n, uint32_t pos, bool dual) const {
@@ -171,7 +173,20 @@ struct wop_operator {
         if (pos >= (int)N[n] - 1) return lines[n][N[n] - 2] - lines[n][N[n] - 1];
         return lines[n][pos + 1] - lines[n][pos];
     }
-    double EdgeLength(int n, const uint32_t* pos, bool dual = false) const { return DiscDelta(n, pos[n], dual) * gridDelta; }
+    // edge_len[dual][n][i] = DiscDelta(n, i, dual) * gridDelta, filled at build start
+    std::vector<double> edge_len[2][3];
+    void FillEdgeTables() {
+        for (int dl = 0; dl < 2; ++dl)
+            for (int n = 0; n < 3; ++n) {
+                edge_len[dl][n].resize(N[n]);
+                for (uint32_t i = 0; i < N[n]; ++i) edge_len[dl][n][i] = DiscDelta(n, i, dl != 0) * gridDelta;
+            }
+    }
+    double EdgeLength(int n, const uint32_t* pos, bool dual = false) const {
+        const std::vector<double>& t = edge_len[dual ? 1 : 0][n];
+        if (pos[n] < t.size()) return t[pos[n]];
+        return DiscDelta(n, pos[n], dual) * gridDelta;
+    }
     double NodeWidth(int n, const uint32_t* pos, bool dual) const { return EdgeLength(n, pos, !dual); }
     double NodeWidthI(int n, const int* pos, bool dual) const {
         if (pos[0] < 0 || pos[1] < 0 || pos[2] < 0) return 0.0;
@@ -273,6 +288,25 @@ struct wop_operator {
         }
     }
 
+    // two GetMaterial values of one sample point with a single lookup
+    void Material2(const double* c, const std::vector<int>& cand, int ta, int tb, double& a, double& b) const {
+        const int p = PropertyAt(c, P_MATERIAL, &cand);
+        auto val = [&](int type) -> double {
+            if (p >= 0) {
+                const Property& q = props[p];
+                switch (type) {
+                case 0: return q.eps;
+                case 1: return q.kappa;
+                case 2: return q.mue;
+                default: return q.sigma;
+                }
+            }
+            return (type == 0 || type == 2) ? 1.0 : 0.0;
+        };
+        a = val(ta);
+        b = val(tb);
+    }
+
     // AverageMatQuarterCell (FDTD/operator.cpp:1406-1518)
     void AverageMatQuarterCell(int ny, const uint32_t* pos, double* EffMat, const std::vector<int>& cand) const {
         const int n = ny, nP = (n + 1) % 3, nPP = (n + 2) % 3;
@@ -291,8 +325,12 @@ struct wop_operator {
         shiftCoord[nP] = coord[nP] + deltaP * 0.25;
         shiftCoord[nPP] = coord[nPP] + deltaPP * 0.25;
         A_n = NodeAreaI(ny, loc_pos, true);
-        EffMat[0] = GetMaterial(shiftCoord, 0, cand) * A_n;
-        EffMat[1] = GetMaterial(shiftCoord, 1, cand) * A_n;
+        {
+            double e, k;
+            Material2(shiftCoord, cand, 0, 1, e, k);
+            EffMat[0] = e * A_n;
+            EffMat[1] = k * A_n;
+        }
         area += A_n;
 
         shiftCoord[n] = coord[n] + delta * 0.5;
@@ -300,8 +338,12 @@ struct wop_operator {
         shiftCoord[nPP] = coord[nPP] + deltaPP * 0.25;
         --loc_pos[nP];
         A_n = NodeAreaI(ny, loc_pos, true);
-        EffMat[0] += GetMaterial(shiftCoord, 0, cand) * A_n;
-        EffMat[1] += GetMaterial(shiftCoord, 1, cand) * A_n;
+        {
+            double e, k;
+            Material2(shiftCoord, cand, 0, 1, e, k);
+            EffMat[0] += e * A_n;
+            EffMat[1] += k * A_n;
+        }
         area += A_n;
 
         shiftCoord[n] = coord[n] + delta * 0.5;
@@ -310,8 +352,12 @@ struct wop_operator {
         ++loc_pos[nP];
         --loc_pos[nPP];
         A_n = NodeAreaI(ny, loc_pos, true);
-        EffMat[0] += GetMaterial(shiftCoord, 0, cand) * A_n;
-        EffMat[1] += GetMaterial(shiftCoord, 1, cand) * A_n;
+        {
+            double e, k;
+            Material2(shiftCoord, cand, 0, 1, e, k);
+            EffMat[0] += e * A_n;
+            EffMat[1] += k * A_n;
+        }
         area += A_n;
 
         shiftCoord[n] = coord[n] + delta * 0.5;
@@ -319,8 +365,12 @@ struct wop_operator {
         shiftCoord[nPP] = coord[nPP] - deltaPP_M * 0.25;
         --loc_pos[nP];
         A_n = NodeAreaI(ny, loc_pos, true);
-        EffMat[0] += GetMaterial(shiftCoord, 0, cand) * A_n;
-        EffMat[1] += GetMaterial(shiftCoord, 1, cand) * A_n;
+        {
+            double e, k;
+            Material2(shiftCoord, cand, 0, 1, e, k);
+            EffMat[0] += e * A_n;
+            EffMat[1] += k * A_n;
+        }
         area += A_n;
 
         EffMat[0] *= EPS0 / area;
@@ -333,8 +383,9 @@ struct wop_operator {
         shiftCoord[nPP] = coord[nPP] + deltaPP * 0.5;
         --loc_pos[n];
         double delta_ny = NodeWidthI(n, loc_pos, true);
-        EffMat[2] = delta_ny / GetMaterial(shiftCoord, 2, cand);
-        double sigma = GetMaterial(shiftCoord, 3, cand);
+        double mue, sigma;
+        Material2(shiftCoord, cand, 2, 3, mue, sigma);
+        EffMat[2] = delta_ny / mue;
         if (sigma) EffMat[3] = delta_ny / sigma;
         else EffMat[3] = 0;
         length = delta_ny;
@@ -344,8 +395,8 @@ struct wop_operator {
         shiftCoord[nPP] = coord[nPP] + deltaPP * 0.5;
         ++loc_pos[n];
         delta_ny = NodeWidthI(n, loc_pos, true);
-        EffMat[2] += delta_ny / GetMaterial(shiftCoord, 2, cand);
-        sigma = GetMaterial(shiftCoord, 3, cand);
+        Material2(shiftCoord, cand, 2, 3, mue, sigma);
+        EffMat[2] += delta_ny / mue;
         if (sigma) EffMat[3] += delta_ny / sigma;
         else EffMat[3] = 0;
         length += delta_ny;
@@ -365,6 +416,10 @@ struct wop_operator {
     template <typename Fn>
     void parallel_x(uint32_t nx, Fn fn) const {
         const int T = threads();
+        if (T <= 1 || nx <= 1) {
+            for (uint32_t x = 0; x < nx; ++x) fn(x);
+            return;
+        }
         std::atomic<uint32_t> next(0);
         std::vector<std::thread> th;
         for (int t = 0; t < T; ++t)
@@ -738,6 +793,21 @@ struct Builder {
         if (!any) return;
         uint32_t lo[3], hi[3];
         AxisRange(only_x, lo, hi);
+        // only excitation primitives can match (scanned in priority order);
+        // for one plane, only those whose bounding box comes within a cell
+        // of the plane's sample points
+        const double inf = std::numeric_limits<double>::infinity();
+        double blo[3] = {-inf, -inf, -inf}, bhi[3] = {inf, inf, inf};
+        if (only_x >= 0) {
+            const auto& L = op->lines[0];
+            blo[0] = L[std::max(0, only_x - 1)];
+            bhi[0] = L[std::min((int)N[0] - 1, only_x + 1)];
+            const double d = bhi[0] - blo[0];
+            blo[0] -= d;
+            bhi[0] += d;
+        }
+        const std::vector<int> cand = op->Candidates(blo, bhi, P_EXCITATION);
+        if (cand.empty()) return;
         uint32_t pos[3];
         double c[3];
         for (pos[2] = 0; pos[2] < N[2]; ++pos[2])
@@ -745,7 +815,7 @@ struct Builder {
                 for (pos[0] = lo[0]; pos[0] < hi[0]; ++pos[0]) {
                     for (int n = 0; n < 3; ++n) {
                         if (!op->YeeCoords(n, pos, c, false)) continue;
-                        int p = op->PropertyAt(c, P_EXCITATION);
+                        int p = op->PropertyAt(c, P_EXCITATION, &cand);
                         if (p < 0) continue;
                         const Property& P = op->props[p];
                         if (P.exc_val[n] != 0 && (P.exc_type == 0 || P.exc_type == 1)) {
@@ -767,7 +837,7 @@ struct Builder {
                     for (int n = 0; n < 3; ++n) {
                         if (pos[0] >= N[0] - 1 || pos[1] >= N[1] - 1 || pos[2] >= N[2] - 1) continue;
                         if (!op->YeeCoords(n, pos, c, true)) continue;
-                        int p = op->PropertyAt(c, P_EXCITATION);
+                        int p = op->PropertyAt(c, P_EXCITATION, &cand);
                         if (p < 0) continue;
                         const Property& P = op->props[p];
                         if (P.exc_val[n] != 0 && (P.exc_type == 2 || P.exc_type == 3)) {
@@ -793,6 +863,34 @@ struct Builder {
     // -log(1e-6)*log(2.5)/(2*dl*Z*(pow(2.5,W/dl)-1)) * pow(2.5, D/dl)
     static double Grading(double D, double dl, double W, double Z, double) {
         return -log(1e-6) * log(2.5) / (2 * dl * Z * (pow(2.5, W / dl) - 1)) * pow(2.5, D / dl);
+    }
+
+    // GradingKappa's per-axis factors depend only on pos[n] and whether n is
+    // the edge direction: tabulated once per build as [n][n == ny][pos[n]]
+    std::vector<double> gk_v[3][2], gk_i[3][2];
+    void GradingTables(double Zm) {
+        for (int n = 0; n < 3; ++n) {
+            const int other = (n + 1) % 3;
+            for (int isny = 0; isny < 2; ++isny) {
+                gk_v[n][isny].assign(op->N[n], 0.0);
+                gk_i[n][isny].assign(op->N[n], 0.0);
+                for (uint32_t q = 0; q < op->N[n]; ++q) {
+                    uint32_t pos[3] = {0, 0, 0};
+                    pos[n] = q;
+                    double kv[3], ki[3];
+                    GradingKappa(isny ? n : other, pos, Zm, kv, ki);
+                    gk_v[n][isny][q] = kv[n];
+                    gk_i[n][isny][q] = ki[n];
+                }
+            }
+        }
+    }
+    void GradingKappaTab(int ny, const uint32_t* pos, double* kv, double* ki) const {
+        for (int n = 0; n < 3; ++n) {
+            const int isny = n == ny;
+            kv[n] = gk_v[n][isny][pos[n]];
+            ki[n] = gk_i[n][isny][pos[n]];
+        }
     }
 
     // Operator_Ext_UPML::CalcGradingKappa (operator_ext_upml.cpp:289-363)
@@ -857,6 +955,7 @@ struct Builder {
         if (BC[4] == 3) { R r; start[2] = 0; stop[2] = S[4]; memcpy(r.s, start, 12); memcpy(r.e, stop, 12); ranges.push_back(r); }
         if (BC[5] == 3) { R r; start[2] = N[2] - 1 - S[5]; stop[2] = N[2] - 1; memcpy(r.s, start, 12); memcpy(r.e, stop, 12); ranges.push_back(r); }
 
+        GradingTables(Z0);
         for (const R& r : ranges) {
             wop_operator::Upml u;
             size_t cnt = 1;
@@ -885,7 +984,7 @@ struct Builder {
             for (int n = 0; n < 3; ++n) {
                 const size_t li = (size_t)n * cnt + ((size_t)loc[0] * u.n[1] + loc[1]) * u.n[2] + loc[2];
                 op->AverageMatQuarterCell(n, pos, eff, cand);
-                GradingKappa(n, pos, Z0, kv, ki);
+                GradingKappaTab(n, pos, kv, ki);
                 const int nP = (n + 1) % 3, nPP = (n + 2) % 3;
                 if (((kv[0] + kv[1] + kv[2]) != 0) && (eff[1] < 1e3)) {
                     if ((VV(n, i) + VI(n, i)) != 0) {
@@ -943,11 +1042,20 @@ struct Builder {
     }
 
     void UpmlPlane(uint32_t x) {
+        // the slabs cover disjoint cells: one parallel sweep over the rows of
+        // the plane handles every slab that the row crosses
+        std::vector<size_t> here;
         for (size_t si = 0; si < op->upml.size(); ++si) {
             const wop_operator::Upml& u = op->upml[si];
-            if (x < u.start[0] || x >= u.start[0] + u.n[0]) continue;
-            op->parallel_x(u.n[1], [&](uint32_t ly) { UpmlRow(si, x - u.start[0], ly); });
+            if (x >= u.start[0] && x < u.start[0] + u.n[0]) here.push_back(si);
         }
+        if (here.empty()) return;
+        op->parallel_x(op->N[1], [&](uint32_t y) {
+            for (size_t si : here) {
+                const wop_operator::Upml& u = op->upml[si];
+                if (y >= u.start[1] && y < u.start[1] + u.n[1]) UpmlRow(si, x - u.start[0], y - u.start[1]);
+            }
+        });
     }
 
     // Operator::CalcNumericPhaseVelocity (FDTD/operator.cpp:2087-2151)
@@ -1319,6 +1427,7 @@ template <typename F>
 int wop_operator::build(uint32_t nr_ts) {
     sf.clear();
     sd.clear();
+    FillEdgeTables();
     ring = 0;
     streamed = false;
     stream_table.clear();
@@ -1399,12 +1508,15 @@ int wop_set_mesh(wop_operator* op, int axis, const double* lines, uint32_t n) {
     op->lines[axis].assign(lines, lines + n);
     op->N[axis] = n;
     op->built = false;
+    for (int dl = 0; dl < 2; ++dl) op->edge_len[dl][axis].clear();
     return 0;
 }
 
 int wop_set_grid_delta(wop_operator* op, double unit) {
     if (!op || !(unit > 0)) return fail(-1, "bad argument");
     op->gridDelta = unit;
+    for (int dl = 0; dl < 2; ++dl)
+        for (int n = 0; n < 3; ++n) op->edge_len[dl][n].clear();
     return 0;
 }
 
@@ -1434,6 +1546,12 @@ int wop_set_timestep(wop_operator* op, double dT) {
 int wop_set_threads(wop_operator* op, int n) {
     if (!op) return fail(-1, "bad argument");
     op->nthreads = n;
+    return 0;
+}
+
+int wop_set_verbose(wop_operator* op, int level) {
+    if (!op) return fail(-1, "bad argument");
+    op->verbose = level;
     return 0;
 }
 
@@ -1636,22 +1754,136 @@ int wop_get_ec(wop_operator* op, double* C, double* G, double* L, double* R) {
     return 0;
 }
 
+}  // extern "C"
+
 namespace {
 struct TupleHash {
-    size_t operator()(const std::array<uint64_t, 12>& t) const {
+    template <typename T, size_t K>
+    size_t operator()(const std::array<T, K>& t) const {
         uint64_t h = 1469598103934665603ull;
-        for (uint64_t v : t) { h ^= v; h *= 1099511628211ull; h ^= h >> 29; }
+        for (T v : t) { h ^= (uint64_t)v; h *= 1099511628211ull; h ^= h >> 29; }
         return (size_t)h;
+    }
+};
+
+// Deduplicates the tuples of one x plane into per-cell ids of a table that
+// grows over the planes, numbering tuples in cell order of first occurrence.
+// Rows are deduplicated in parallel (local ids in row order of first
+// occurrence; runs along z and short rows need no hashing), then merged row
+// by row into the table, then the row ids are mapped to table ids.
+template <typename Tuple>
+struct PlaneDedup {
+    struct Row {
+        std::vector<Tuple> uniq;
+        // open-addressing index into uniq (slot = id or UINT32_MAX), used
+        // once a row holds more than 16 tuples
+        std::vector<uint32_t> slots;
+        std::vector<uint32_t> global;
+        uint32_t find(const Tuple& t, size_t h) const {
+            const size_t mask = slots.size() - 1;
+            for (size_t k = h & mask;; k = (k + 1) & mask) {
+                if (slots[k] == UINT32_MAX) return UINT32_MAX;
+                if (uniq[slots[k]] == t) return slots[k];
+            }
+        }
+        void insert(uint32_t id, size_t h) {
+            const size_t mask = slots.size() - 1;
+            size_t k = h & mask;
+            while (slots[k] != UINT32_MAX) k = (k + 1) & mask;
+            slots[k] = id;
+        }
+        void reindex() {
+            size_t n = 64;                  // a power of two, at least 4x the tuples
+            while (n < 4 * uniq.size()) n *= 2;
+            slots.assign(n, UINT32_MAX);
+            TupleHash H;
+            for (uint32_t q = 0; q < uniq.size(); ++q) insert(q, H(uniq[q]));
+        }
+    };
+    std::vector<Row> rows;
+    std::unordered_map<Tuple, uint32_t, TupleHash> map;
+    std::vector<Tuple> uniq;
+
+    // get(y, z, tuple&) fills the tuple of cell (y, z); out has Ny*Nz ids.
+    // Returns false once the table holds more than max_entries (0 = no limit).
+    template <typename Get>
+    bool plane(const wop_operator* op, uint32_t* out, size_t max_entries, Get get) {
+        const uint32_t Ny = op->N[1], Nz = op->N[2];
+        rows.resize(Ny);
+        op->parallel_x(Ny, [&](uint32_t y) {
+            Row& r = rows[y];
+            r.uniq.clear();
+            uint32_t* o = out + (size_t)y * Nz;
+            uint32_t prev = UINT32_MAX;
+            Tuple t;
+            TupleHash H;
+            for (uint32_t z = 0; z < Nz; ++z) {
+                get(y, z, t);
+                uint32_t id = UINT32_MAX;
+                size_t h = 0;
+                if (prev != UINT32_MAX && t == r.uniq[prev]) id = prev;
+                else if (r.uniq.size() <= 16) {
+                    for (uint32_t q = 0; q < r.uniq.size(); ++q)
+                        if (r.uniq[q] == t) { id = q; break; }
+                } else {
+                    h = H(t);
+                    id = r.find(t, h);
+                }
+                if (id == UINT32_MAX) {
+                    id = (uint32_t)r.uniq.size();
+                    r.uniq.push_back(t);
+                    if (r.uniq.size() == 17) r.reindex();
+                    else if (r.uniq.size() > 17) {
+                        if (2 * r.uniq.size() > r.slots.size()) r.reindex();
+                        else r.insert(id, h);
+                    }
+                }
+                o[z] = id;
+                prev = id;
+            }
+        });
+        // tuples the table held before this plane: looked up in parallel
+        // (the table is not modified meanwhile); the rest, in row order
+        op->parallel_x(Ny, [&](uint32_t y) {
+            Row& r = rows[y];
+            r.global.assign(r.uniq.size(), UINT32_MAX);
+            for (size_t q = 0; q < r.uniq.size(); ++q) {
+                auto it = map.find(r.uniq[q]);
+                if (it != map.end()) r.global[q] = it->second;
+            }
+        });
+        for (uint32_t y = 0; y < Ny; ++y) {
+            Row& r = rows[y];
+            for (size_t q = 0; q < r.uniq.size(); ++q) {
+                if (r.global[q] != UINT32_MAX) continue;
+                auto it = map.find(r.uniq[q]);
+                if (it == map.end()) {
+                    const uint32_t id = (uint32_t)uniq.size();
+                    map.emplace(r.uniq[q], id);
+                    uniq.push_back(r.uniq[q]);
+                    r.global[q] = id;
+                } else r.global[q] = it->second;
+            }
+            if (max_entries && uniq.size() > max_entries) return false;
+        }
+        op->parallel_x(Ny, [&](uint32_t y) {
+            const Row& r = rows[y];
+            uint32_t* o = out + (size_t)y * Nz;
+            for (uint32_t z = 0; z < Nz; ++z) o[z] = r.global[o[z]];
+        });
+        return true;
     }
 };
 
 // Float build that never holds full per-cell arrays: x planes pass through a
 // ring of three plane slots.  Pass 1 computes EC plane by plane and the
-// Var3 timestep from planes x-1..x+1; pass 2 recomputes EC, the coefficients,
-// boundary conditions, PEC, lumped elements, excitation and UPML of each
-// plane in the full build's order (every step touches only its own cell) and
-// deduplicates the plane's 12-tuples into index[] in cell order -- the same
-// index and table the full build followed by wop_compress gives.
+// Var3 timestep from planes x-1..x+1, and keeps each EC plane as per-cell ids
+// into a table of distinct EC 12-tuples; pass 2 restores EC from those ids,
+// computes the coefficients, boundary conditions, PEC, lumped elements,
+// excitation and UPML of each plane in the full build's order (every step
+// touches only its own cell) and deduplicates the plane's coefficient
+// 12-tuples into index[] in cell order -- the same index and table the full
+// build followed by wop_compress gives.
 int stream_build(wop_operator* op, uint32_t nr_ts, uint32_t* index, uint32_t max_entries, uint32_t* n_entries) {
     if (op->pw.set) return fail(-1, "the streamed build has no TFSF plane wave");
     for (auto& p : op->props)
@@ -1659,8 +1891,11 @@ int stream_build(wop_operator* op, uint32_t nr_ts, uint32_t* index, uint32_t max
     const uint32_t* N = op->N;
     op->sf.clear();
     op->sd.clear();
+    op->FillEdgeTables();
     op->built = false;
     op->prec = 0;
+    op->ring_slot.resize(N[0]);
+    for (uint32_t x = 0; x < N[0]; ++x) op->ring_slot[x] = x % 3;
     op->ring = 3;
     op->streamed = true;
     op->stream_table.clear();
@@ -1668,15 +1903,30 @@ int stream_build(wop_operator* op, uint32_t nr_ts, uint32_t* index, uint32_t max
     Builder<float> b(op);
     for (int k = 0; k < 4; ++k)
         for (int d = 0; d < 3; ++d) { b.EC[k][d].assign(3 * plane, 0.0f); b.co[k][d].assign(3 * plane, 0.0f); }
+    // EC planes kept as ids (4 bytes per cell) into their distinct tuples
+    std::vector<uint32_t> ec_ids((size_t)N[0] * plane);
+    PlaneDedup<std::array<uint32_t, 12>> ec_tab;
+    auto keep_ec = [&](uint32_t x) {
+        ec_tab.plane(op, ec_ids.data() + (size_t)x * plane, 0, [&](uint32_t y, uint32_t z, std::array<uint32_t, 12>& t) {
+            const size_t i = op->idx(x, y, z);
+            for (int k = 0; k < 4; ++k)
+                for (int d = 0; d < 3; ++d) memcpy(&t[k * 3 + d], &b.EC[k][d][i], 4);
+        });
+    };
     // pass 1: timestep
     double dT = 1e200;
     b.ECPlane(0);
-    if (N[0] > 1) b.ECPlane(1);
+    keep_ec(0);
+    if (N[0] > 1) { b.ECPlane(1); keep_ec(1); }
     for (uint32_t x = 0; x < N[0]; ++x) {
-        if (x >= 1 && x + 1 < N[0]) b.ECPlane(x + 1);
+        if (x >= 1 && x + 1 < N[0]) { b.ECPlane(x + 1); keep_ec(x + 1); }
         const double t = b.TimestepPlane(x);
         if (t < dT && t > 0.0) dT = t;
+        if (op->verbose && ((x + 1) % 128 == 0 || x + 1 == N[0]))
+            fprintf(stderr, "streamed build: timestep pass, plane %u of %u, %zu distinct EC tuples\n", x + 1, N[0], ec_tab.uniq.size());
     }
+    ec_tab.map.clear();
+    ec_tab.rows.clear();
     if (op->forced_dT > 0) op->dT = op->forced_dT;
     else op->dT = dT;
     op->dT *= op->ts_factor;
@@ -1693,10 +1943,18 @@ int stream_build(wop_operator* op, uint32_t nr_ts, uint32_t* index, uint32_t max
     op->tfsf_i.clear();
     op->tfsf_max_delay = 0;
     op->ade.clear();
-    std::unordered_map<std::array<uint64_t, 12>, uint32_t, TupleHash> map;
-    std::vector<std::array<uint64_t, 12>> uniq;
+    PlaneDedup<std::array<uint64_t, 12>> co_tab;
     for (uint32_t x = 0; x < N[0]; ++x) {
-        b.ECPlane(x);
+        // EC of plane x as pass 1 computed it
+        op->parallel_x(N[1], [&](uint32_t y) {
+            const uint32_t* id = ec_ids.data() + (size_t)x * plane + (size_t)y * N[2];
+            for (uint32_t z = 0; z < N[2]; ++z) {
+                const size_t i = op->idx(x, y, z);
+                const std::array<uint32_t, 12>& t = ec_tab.uniq[id[z]];
+                for (int k = 0; k < 4; ++k)
+                    for (int d = 0; d < 3; ++d) memcpy(&b.EC[k][d][i], &t[k * 3 + d], 4);
+            }
+        });
         op->parallel_x(N[1], [&](uint32_t y) {
             for (int d = 0; d < 3; ++d)
                 for (uint32_t z = 0; z < N[2]; ++z) b.ECOperatorPos(d, op->idx(x, y, z));
@@ -1707,32 +1965,25 @@ int stream_build(wop_operator* op, uint32_t nr_ts, uint32_t* index, uint32_t max
         b.ApplyMagneticBC((int)x);
         b.Excitation((int)x);
         b.UpmlPlane(x);
-        for (uint32_t y = 0; y < N[1]; ++y)
-            for (uint32_t z = 0; z < N[2]; ++z) {
-                const size_t i = op->idx(x, y, z);
-                std::array<uint64_t, 12> t;
-                for (int k = 0; k < 4; ++k)
-                    for (int d = 0; d < 3; ++d) {
-                        double v = (double)b.co[k][d][i];
-                        uint64_t bits;
-                        memcpy(&bits, &v, 8);
-                        t[k * 3 + d] = bits;
-                    }
-                auto it = map.find(t);
-                uint32_t id;
-                if (it == map.end()) {
-                    id = (uint32_t)uniq.size();
-                    map.emplace(t, id);
-                    uniq.push_back(t);
-                    if (max_entries && uniq.size() > max_entries) {
-                        op->ring = 0;
-                        *n_entries = (uint32_t)uniq.size();
-                        return fail(-2, "more than %u unique coefficient tuples", max_entries);
-                    }
-                } else id = it->second;
-                index[((size_t)x * N[1] + y) * N[2] + z] = id;
-            }
+        const bool ok = co_tab.plane(op, index + (size_t)x * plane, max_entries,
+                                     [&](uint32_t y, uint32_t z, std::array<uint64_t, 12>& t) {
+            const size_t i = op->idx(x, y, z);
+            for (int k = 0; k < 4; ++k)
+                for (int d = 0; d < 3; ++d) {
+                    const double v = (double)b.co[k][d][i];
+                    memcpy(&t[k * 3 + d], &v, 8);
+                }
+        });
+        if (!ok) {
+            op->ring = 0;
+            *n_entries = (uint32_t)co_tab.uniq.size();
+            return fail(-2, "more than %u unique coefficient tuples", max_entries);
+        }
+        if (op->verbose && ((x + 1) % 128 == 0 || x + 1 == N[0]))
+            fprintf(stderr, "streamed build: coefficient pass, plane %u of %u, %zu unique tuples\n", x + 1, N[0], co_tab.uniq.size());
     }
+    std::vector<uint32_t>().swap(ec_ids);
+    std::vector<std::array<uint64_t, 12>>& uniq = co_tab.uniq;
     b.UpmlFinish();
     // excitation lists in the full build's z, y, x order
     auto zyx = [](const wop_operator::Exc& a, const wop_operator::Exc& c) {
@@ -1755,8 +2006,6 @@ int stream_build(wop_operator* op, uint32_t nr_ts, uint32_t* index, uint32_t max
     return 0;
 }
 }  // namespace
-
-}  // extern "C"
 
 extern "C" int wop_build_streamed(wop_operator* op, uint32_t nr_ts, uint32_t* index, uint32_t max_entries, uint32_t* n_entries) {
     if (!op || !index || !n_entries) return fail(-1, "bad argument");

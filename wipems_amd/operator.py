@@ -218,6 +218,9 @@ class Operator:
     def set_threads(self, n):
         self._chk(self._lib.wop_set_threads(self._h, int(n)))
 
+    def set_verbose(self, level):
+        self._chk(self._lib.wop_set_verbose(self._h, int(level)))
+
     def add_material(self, eps_r=1.0, kappa=0.0, mue_r=1.0, sigma=0.0):
         i = c_int()
         self._chk(self._lib.wop_add_material(self._h, eps_r, kappa, mue_r, sigma, byref(i)))

@@ -155,6 +155,51 @@ def dispersive_scenario(num_lines=(385, 385, 193), delta_mm=0.25, steps=20000, p
                     meta={"interval": interval, "spheres": spheres, "seed": seed})
 
 
+def large_dipole_scenario(num_lines=(1537, 1537, 769), steps=20000, seed=SEED_BASE + 5, n_boxes=200, pml=8,
+                          f0=2e9, fc=2e9, n_freq=16, box_margin=10, dipole_cells=20, threads=16, max_entries=65536,
+                          verbose=False):
+    """Large uniform 1 mm domain (defaults: 1537 x 1537 x 769 lines, 1.82 G
+    cells) with UPML-8, `n_boxes` random lossless dielectric boxes (eps_r ~
+    U[1,12]), a z-directed soft dipole of `dipole_cells` edges at the centre
+    (Gaussian f0/fc) and an nf2ff recording box `box_margin` cells inside the
+    UPML with FP64 DFT at n_freq frequencies over (0, f0+fc].  Built with the
+    streamed operator build (compressed coefficients only, float)."""
+    rng = np.random.default_rng(seed)
+    lines = [np.arange(n, dtype=np.float64) for n in num_lines]
+    op = Operator()
+    op.set_mesh(lines, 1e-3)
+    op.set_boundary([BC_PML] * 6, [pml] * 6)
+    op.set_threads(threads)
+    op.set_verbose(1 if verbose else 0)
+    op.set_gauss_excite(f0, fc)
+    ext = [lines[n][-1] for n in range(3)]
+    boxes = []
+    for k in range(n_boxes):
+        size = [rng.uniform(0.02, 0.12) * ext[n] for n in range(3)]
+        a = [rng.uniform(lines[n][pml + 2], lines[n][-pml - 3] - size[n]) for n in range(3)]
+        eps = rng.uniform(1.0, 12.0)
+        m = op.add_material(eps_r=eps)
+        op.add_box(m, tuple(a), tuple(a[n] + size[n] for n in range(3)), priority=1)
+        boxes.append({"start": a, "size": size, "eps_r": eps})
+    c = [lines[n][len(lines[n]) // 2] for n in range(3)]
+    z0 = lines[2][len(lines[2]) // 2 - dipole_cells // 2]
+    z1 = lines[2][len(lines[2]) // 2 - dipole_cells // 2 + dipole_cells]
+    e = op.add_excitation(0, (0.0, 0.0, 1.0))
+    op.add_box(e, (c[0], c[1], z0), (c[0], c[1], z1), priority=10)
+    data = op.build_streamed(nr_ts=steps, max_entries=max_entries)
+    interval = max(1, data.nyquist // 4)
+    feed = (c[0], c[1], lines[2][len(lines[2]) // 2])
+    feed2 = (c[0], c[1], lines[2][len(lines[2]) // 2 + 1])
+    probes = [op.voltage_probe(feed, feed2, 1.0, td_interval=interval, name="u_feed")]
+    _finish(op, data, probes)
+    lo = [lines[n][pml + box_margin] for n in range(3)]
+    hi = [lines[n][-1 - pml - box_margin] for n in range(3)]
+    freqs = np.linspace((f0 + fc) / n_freq, f0 + fc, n_freq)
+    op.nf2ff_box(data, lo, hi, freqs)
+    return Scenario("large_dipole_%dx%dx%d" % tuple(num_lines), data, steps, op,
+                    meta={"interval": interval, "boxes": boxes, "seed": seed, "f0": f0, "fc": fc, "freqs": freqs})
+
+
 def graded_lines(rng, n_cells, lo=0.5, hi=2.0):
     gaps = rng.uniform(lo, hi, n_cells)
     return np.concatenate([[0.0], np.cumsum(gaps)])
