@@ -340,8 +340,13 @@ def fp32_check(data, eng, sc):
     e32 = engine_from_operator(data, precision="fp32")
     total = eng.GetNumberOfTimesteps()
     t1 = time.perf_counter()
-    e32.iterate(total)
-    e32.synchronize()
+    done = 0
+    while done < total:                 # in chunks, with a progress line each
+        n = min(1000, total - done)
+        e32.iterate(n)
+        e32.synchronize()
+        done += n
+        log("fp32 reference: %d of %d steps" % (done, total))
     wall = time.perf_counter() - t1
     out = {"reference": "fp32 engine", "steps": total,
            "fp32_ms_per_step": round(1e3 * wall / total, 4), "fp32_MCells_s": round(data.cells * total / wall / 1e6, 1)}

@@ -287,3 +287,64 @@ def test_fused_step_matches_two_launches(gpu, precision, prec, case, seg):
         np.testing.assert_array_equal(out[0][k], out[1][k])
     for a, b in zip(out[0][2], out[1][2]):
         np.testing.assert_array_equal(a, b)
+
+
+def _short_z_upml(prec, nz_lines=20, pml=8):
+    """Graded mesh whose z extent is so short that one vector of cells holds
+    cells of both z slabs (lines 0-8 and 11-19 with 8-cell UPML) plus a
+    dielectric box and a soft source: the UPML shell launch must apply both
+    slabs to that vector."""
+    from wipems_amd.operator import BC_PML, Operator
+    rng = np.random.default_rng(21)
+    lines = [np.concatenate([[0.0], np.cumsum(rng.uniform(0.6, 1.5, n - 1))]) for n in (30, 28, nz_lines)]
+    op = Operator()
+    op.set_mesh(lines, 1e-3)
+    op.set_boundary([BC_PML] * 6, [pml] * 6)
+    op.set_gauss_excite(4e9, 4e9)
+    m = op.add_material(eps_r=3.5, kappa=0.02)
+    op.add_box(m, (lines[0][10], lines[1][9], lines[2][2]), (lines[0][18], lines[1][17], lines[2][15]), priority=1)
+    e = op.add_excitation(0, (0.0, 0.0, 1.0))
+    op.add_box(e, (lines[0][14], lines[1][13], lines[2][9]), (lines[0][14], lines[1][13], lines[2][11]), priority=5)
+    d = op.build(prec, nr_ts=150)
+    d.probes = [op.voltage_probe((lines[0][14], lines[1][13], lines[2][9]), (lines[0][14], lines[1][13], lines[2][11]),
+                                 td_interval=1, name="u")]
+    return d
+
+
+@pytest.mark.parametrize("precision,prec", [("fp32", 0), ("fp64", 1)])
+@pytest.mark.parametrize("compressed", [False, True])
+def test_upml_shell_launch_short_z_bitwise(gpu, precision, prec, compressed):
+    d = _short_z_upml(prec)
+    assert len(d.upml) == 6
+    print("case: short-z UPML shell, %s, compressed %s" % (precision, compressed), flush=True)
+    eng = engine_from_operator(d, precision=precision, compressed=compressed)
+    _kinds[id(eng)] = [p.kind for p in d.probes]
+    orc = oracle_from_operator(d, ftz=True)
+    eng.iterate(150)
+    orc.iterate(150)
+    compare(eng, orc)
+
+
+@pytest.mark.parametrize("precision,prec", [("fp32", 0), ("fp64", 1), ("fp16", 0), ("bf16", 0)])
+@pytest.mark.parametrize("case", ["short_z", "graded_port"])
+def test_upml_split_launches_equal_one_launch(gpu, precision, prec, case):
+    """Interior + shell launches give the same bits as one launch with the
+    UPML branch, for every storage type (FP16/BF16 have no bitwise oracle,
+    so this is their check of the shell launch)."""
+    if case == "short_z":
+        d = _short_z_upml(prec)
+    else:
+        d = scenarios.graded_port_scenario(n_cells=34, steps=150, prec=prec, n_boxes=5, pml=7).data
+    print("case: UPML split vs one launch, %s, %s" % (case, precision), flush=True)
+    out = []
+    for mode in (0, 1):
+        eng = engine_from_operator(d, precision=precision)
+        eng.set_pml_split(mode)
+        assert eng.pml_split() == bool(mode)
+        eng.iterate(150)
+        eng.synchronize()
+        out.append([eng.get_field(w) for w in (0, 1)] + [eng.read_probe(eng.probe_ids[0])[1]])
+        eng.close()
+    for a, b in zip(out[0], out[1]):
+        np.testing.assert_array_equal(a, b)
+    assert np.abs(out[0][0]).max() > 0

@@ -61,6 +61,8 @@ _ENGINE_SIGS = {
     "fdtd_set_use_graph": [c_void_p, c_int],
     "fdtd_set_pml_compression": [c_void_p, c_int],
     "fdtd_pml_compressed": [c_void_p, POINTER(c_int)],
+    "fdtd_set_pml_split": [c_void_p, c_int],
+    "fdtd_pml_split": [c_void_p, POINTER(c_int)],
     "fdtd_set_tuning": [c_void_p, c_int, c_int, c_int],
     "fdtd_set_kernel_timing": [c_void_p, c_int],
     "fdtd_get_kernel_times": [c_void_p, POINTER(c_double), POINTER(c_double), POINTER(c_uint32), POINTER(c_uint32)],

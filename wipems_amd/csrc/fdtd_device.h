@@ -116,22 +116,16 @@ __device__ __forceinline__ int find_slab(const PmlSet& ps, int x, int y, int z, 
     return -1;
 }
 
-// slab holding any of the cells z0..z0+vz-1 at (x,y); base = slab-array
-// offset of z0 (aligned like z0), mask bit k = cell z0+k lies in the slab
-__device__ __forceinline__ int find_slab_vec(const PmlSet& ps, int x, int y, int z0, int vz, long long& base, unsigned& mask) {
-    for (int s = 0; s < ps.count; ++s) {
-        const PmlSlab& b = ps.slab[s];
-        const int lx = x - b.s[0], ly = y - b.s[1];
-        if ((unsigned)lx >= (unsigned)b.n[0] || (unsigned)ly >= (unsigned)b.n[1]) continue;
-        unsigned m = 0;
-        for (int k = 0; k < vz; ++k)
-            if ((unsigned)(z0 + k - b.s[2]) < (unsigned)b.n[2]) m |= 1u << k;
-        if (!m) continue;
-        base = b.off + ((long long)lx * b.n[1] + ly) * b.nzp + (z0 - b.za);
-        mask = m;
-        return s;
-    }
-    return -1;
+// cells z0..z0+vz-1 at (x,y) that lie in slab b: mask bit k = cell z0+k;
+// base = slab-array offset of z0 (aligned like z0).  Returns the mask.
+__device__ __forceinline__ unsigned slab_lanes(const PmlSlab& b, int x, int y, int z0, int vz, long long& base) {
+    const int lx = x - b.s[0], ly = y - b.s[1];
+    if ((unsigned)lx >= (unsigned)b.n[0] || (unsigned)ly >= (unsigned)b.n[1]) return 0u;
+    unsigned m = 0;
+    for (int k = 0; k < vz; ++k)
+        if ((unsigned)(z0 + k - b.s[2]) < (unsigned)b.n[2]) m |= 1u << k;
+    base = b.off + ((long long)lx * b.n[1] + ly) * b.nzp + (z0 - b.za);
+    return m;
 }
 
 template <typename A, int CM>
@@ -411,54 +405,119 @@ __device__ __forceinline__ void volt_vec_body(S* __restrict__ V, const S* __rest
     A vv[3][VZ], vi[3][VZ];
     vload_coeffs<A, CM, VZ>(cf, p, d.sc, vv, vi);
     const int valid = min(VZ, d.nz - z0);
-    long long pb = 0;
-    unsigned pm = 0;
-    const int slab = PML ? find_slab_vec(ps, x, y, z0, VZ, pb, pm) : -1;
-    SVec<unsigned short, VZ> prow{};
-    if (PML && slab >= 0 && pp.vtab)
-        prow = *reinterpret_cast<const SVec<unsigned short, VZ>*>(pp.vidx + (pb - ps.slab[slab].off + ps.slab[slab].ioff));
 #pragma unroll
     for (int n = 0; n < 3; ++n) {
-        A v[VZ];
+        A v[VZ], out[VZ];
         vload<S, VZ>(V + n * d.sc + p, v, sc.v[n]);
-        if (PML && slab >= 0) {
-            const long long q = pb + n * ps.slab[slab].sc;
-            A F[VZ], kv[VZ], kfo[VZ], kfn[VZ];
-            vload<A, VZ>(pp.vflux + q, F, A(1));
-            pml_coeffs<A, VZ>(pp.vv, pp.vvfo, pp.vvfn, q, pp.vtab, prow.v, n, kv, kfo, kfn);
 #pragma unroll
-            for (int k = 0; k < VZ; ++k) {
-                if (pm & (1u << k)) {
-                    A h = kv[k] * v[k] - kfo[k] * F[k];
-                    A t = F[k];
-                    t = t * vv[n][k];
-                    t = t + vi[n][k] * curl[n][k];
-                    F[k] = t;
-                    v[k] = h + kfn[k] * t;
-                } else {
-                    v[k] = v[k] * vv[n][k];
-                    v[k] = v[k] + vi[n][k] * curl[n][k];
+        for (int k = 0; k < VZ; ++k) {
+            out[k] = v[k] * vv[n][k];
+            out[k] = out[k] + vi[n][k] * curl[n][k];
+        }
+        if (PML) {
+            // the UPML form for the cells of every slab the vector touches
+            for (int s = 0; s < ps.count; ++s) {
+                const PmlSlab& b = ps.slab[s];
+                long long pb = 0;
+                const unsigned pm = slab_lanes(b, x, y, z0, VZ, pb);
+                if (!pm) continue;
+                SVec<unsigned short, VZ> prow{};
+                if (pp.vtab) prow = *reinterpret_cast<const SVec<unsigned short, VZ>*>(pp.vidx + (pb - b.off + b.ioff));
+                const long long q = pb + n * b.sc;
+                A F[VZ], kv[VZ], kfo[VZ], kfn[VZ];
+                vload<A, VZ>(pp.vflux + q, F, A(1));
+                pml_coeffs<A, VZ>(pp.vv, pp.vvfo, pp.vvfn, q, pp.vtab, prow.v, n, kv, kfo, kfn);
+#pragma unroll
+                for (int k = 0; k < VZ; ++k) {
+                    if (pm & (1u << k)) {
+                        A h = kv[k] * v[k] - kfo[k] * F[k];
+                        A t = F[k];
+                        t = t * vv[n][k];
+                        t = t + vi[n][k] * curl[n][k];
+                        F[k] = t;
+                        out[k] = h + kfn[k] * t;
+                    }
                 }
-            }
-            vstore<A, VZ>(pp.vflux + q, F, A(1), VZ);
-        } else {
-#pragma unroll
-            for (int k = 0; k < VZ; ++k) {
-                v[k] = v[k] * vv[n][k];
-                v[k] = v[k] + vi[n][k] * curl[n][k];
+                vstore<A, VZ>(pp.vflux + q, F, A(1), VZ);
             }
         }
-        vstore<S, VZ>(V + n * d.sc + p, v, sc.iv[n], valid);
+        vstore<S, VZ>(V + n * d.sc + p, out, sc.iv[n], valid);
     }
 }
+
+// Vectors one launch of k_volt_vec / k_curr_vec covers: planes x0 + blockIdx.y,
+// rows y0..y1-1 (flat (y, z) index from y0 over the padded rows), and only
+// vectors whose cells z0..z0+VZ-1 lie in [zlo, zhi).  With UPML slabs forming
+// the shell around an interior box, the interior vectors run here without
+// PML code and k_volt_shell / k_curr_shell take every other vector.
+struct VecRange {
+    int x0, y0, y1, zlo, zhi;
+};
 
 template <typename S, int CM, bool PML, int VZ>
 __global__ void __launch_bounds__(256)
 k_volt_vec(S* __restrict__ V, const S* __restrict__ I, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
-           Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp) {
-    // flat (y, z) index over the padded rows of one x plane
-    const long long q0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * VZ;
-    volt_vec_body<S, CM, PML, VZ>(V, I, cf, d, sc, ps, pp, blockIdx.y, q0);
+           Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp, VecRange r) {
+    const long long q0 = (long long)r.y0 * d.sy + ((long long)blockIdx.x * blockDim.x + threadIdx.x) * VZ;
+    const int y = (int)(q0 / d.sy);
+    const int z0 = (int)(q0 - (long long)y * d.sy);
+    if (y >= r.y1 || z0 < r.zlo || z0 + VZ > r.zhi) return;
+    volt_vec_body<S, CM, PML, VZ>(V, I, cf, d, sc, ps, pp, r.x0 + blockIdx.y, q0);
+}
+
+// The vectors outside the interior box of a UPML shell, numbered
+// (x < x0 planes | x1 <= x < xe planes | interior planes: rows y < y0 or
+// y >= y1 | interior rows: vectors before vlo or from vhi on), each with
+// vectors along z fastest.
+struct ShellGrid {
+    int x0, x1, xe;            // interior planes [x0, x1); planes end at xe
+    int y0, y1, ny;            // interior rows [y0, y1) of ny
+    int vlo, vhi, vn;          // interior vectors [vlo, vhi) of vn per row
+    long long nA0, nA1, nB, nC;
+};
+
+__host__ __device__ inline bool shell_vector(const ShellGrid& g, long long t, int vz, long long sy, int& x, long long& q0) {
+    int y, v;
+    const long long row = (long long)g.ny * g.vn;
+    if (t < g.nA0) {
+        x = (int)(t / row); const long long r = t - (long long)x * row;
+        y = (int)(r / g.vn); v = (int)(r - (long long)y * g.vn);
+    } else if ((t -= g.nA0) < g.nA1) {
+        x = (int)(t / row); const long long r = t - (long long)x * row;
+        x += g.x1;
+        y = (int)(r / g.vn); v = (int)(r - (long long)y * g.vn);
+    } else if ((t -= g.nA1) < g.nB) {
+        const int nyb = g.y0 + g.ny - g.y1;
+        const long long per = (long long)nyb * g.vn;
+        x = (int)(t / per); const long long r = t - (long long)x * per;
+        x += g.x0;
+        const int yy = (int)(r / g.vn);
+        v = (int)(r - (long long)yy * g.vn);
+        y = yy < g.y0 ? yy : g.y1 + (yy - g.y0);
+    } else if ((t -= g.nB) < g.nC) {
+        const int nvz = g.vlo + g.vn - g.vhi;
+        const long long per = (long long)(g.y1 - g.y0) * nvz;
+        x = (int)(t / per); const long long r = t - (long long)x * per;
+        x += g.x0;
+        const int yy = (int)(r / nvz);
+        const int j = (int)(r - (long long)yy * nvz);
+        y = g.y0 + yy;
+        v = j < g.vlo ? j : g.vhi + (j - g.vlo);
+    } else {
+        return false;
+    }
+    q0 = (long long)y * sy + (long long)v * vz;
+    return true;
+}
+
+template <typename S, int CM, int VZ>
+__global__ void __launch_bounds__(256)
+k_volt_shell(S* __restrict__ V, const S* __restrict__ I, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
+             Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp, ShellGrid g) {
+    int x;
+    long long q0;
+    if (!shell_vector(g, (long long)blockIdx.x * blockDim.x + threadIdx.x, VZ, d.sy, x, q0)) return;
+    volt_vec_body<S, CM, true, VZ>(V, I, cf, d, sc, ps, pp, x, q0);
 }
 
 template <typename S, int CM, bool PML, int VZ>
@@ -496,55 +555,66 @@ __device__ __forceinline__ void curr_vec_body(S* __restrict__ I, const S* __rest
     A ii[3][VZ], iv[3][VZ];
     vload_coeffs<A, CM, VZ>(cf, p, d.sc, ii, iv);
     const int valid = min(VZ, d.nz - 1 - z0);
-    long long pb = 0;
-    unsigned pm = 0;
-    const int slab = PML ? find_slab_vec(ps, x, y, z0, VZ, pb, pm) : -1;
-    // cells at z >= nz-1 are not updated: keep them out of the slab mask too
-    if (valid < VZ) pm &= (1u << valid) - 1u;
-    SVec<unsigned short, VZ> prow{};
-    if (PML && slab >= 0 && pp.ctab)
-        prow = *reinterpret_cast<const SVec<unsigned short, VZ>*>(pp.cidx + (pb - ps.slab[slab].off + ps.slab[slab].ioff));
+    // cells at z >= nz-1 are not updated: keep them out of the slab masks too
+    const unsigned upd = valid < VZ ? (1u << valid) - 1u : ~0u;
 #pragma unroll
     for (int n = 0; n < 3; ++n) {
-        A c[VZ];
+        A c[VZ], out[VZ];
         vload<S, VZ>(I + n * d.sc + p, c, sc.c[n]);
-        if (PML && slab >= 0) {
-            const long long q = pb + n * ps.slab[slab].sc;
-            A F[VZ], ki[VZ], kfo[VZ], kfn[VZ];
-            vload<A, VZ>(pp.cflux + q, F, A(1));
-            pml_coeffs<A, VZ>(pp.ii, pp.iifo, pp.iifn, q, pp.ctab, prow.v, n, ki, kfo, kfn);
 #pragma unroll
-            for (int k = 0; k < VZ; ++k) {
-                if (pm & (1u << k)) {
-                    A h = ki[k] * c[k] - kfo[k] * F[k];
-                    A t = F[k];
-                    t = t * ii[n][k];
-                    t = t + iv[n][k] * curl[n][k];
-                    F[k] = t;
-                    c[k] = h + kfn[k] * t;
-                } else {
-                    c[k] = c[k] * ii[n][k];
-                    c[k] = c[k] + iv[n][k] * curl[n][k];
+        for (int k = 0; k < VZ; ++k) {
+            out[k] = c[k] * ii[n][k];
+            out[k] = out[k] + iv[n][k] * curl[n][k];
+        }
+        if (PML) {
+            for (int s = 0; s < ps.count; ++s) {
+                const PmlSlab& b = ps.slab[s];
+                long long pb = 0;
+                const unsigned pm = slab_lanes(b, x, y, z0, VZ, pb) & upd;
+                if (!pm) continue;
+                SVec<unsigned short, VZ> prow{};
+                if (pp.ctab) prow = *reinterpret_cast<const SVec<unsigned short, VZ>*>(pp.cidx + (pb - b.off + b.ioff));
+                const long long q = pb + n * b.sc;
+                A F[VZ], ki[VZ], kfo[VZ], kfn[VZ];
+                vload<A, VZ>(pp.cflux + q, F, A(1));
+                pml_coeffs<A, VZ>(pp.ii, pp.iifo, pp.iifn, q, pp.ctab, prow.v, n, ki, kfo, kfn);
+#pragma unroll
+                for (int k = 0; k < VZ; ++k) {
+                    if (pm & (1u << k)) {
+                        A h = ki[k] * c[k] - kfo[k] * F[k];
+                        A t = F[k];
+                        t = t * ii[n][k];
+                        t = t + iv[n][k] * curl[n][k];
+                        F[k] = t;
+                        out[k] = h + kfn[k] * t;
+                    }
                 }
-            }
-            vstore<A, VZ>(pp.cflux + q, F, A(1), VZ);
-        } else {
-#pragma unroll
-            for (int k = 0; k < VZ; ++k) {
-                c[k] = c[k] * ii[n][k];
-                c[k] = c[k] + iv[n][k] * curl[n][k];
+                vstore<A, VZ>(pp.cflux + q, F, A(1), VZ);
             }
         }
-        vstore<S, VZ>(I + n * d.sc + p, c, sc.ic[n], valid);
+        vstore<S, VZ>(I + n * d.sc + p, out, sc.ic[n], valid);
     }
 }
 
 template <typename S, int CM, bool PML, int VZ>
 __global__ void __launch_bounds__(256)
 k_curr_vec(S* __restrict__ I, const S* __restrict__ V, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
-           Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp) {
-    const long long q0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * VZ;
-    curr_vec_body<S, CM, PML, VZ>(I, V, cf, d, sc, ps, pp, blockIdx.y, q0);
+           Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp, VecRange r) {
+    const long long q0 = (long long)r.y0 * d.sy + ((long long)blockIdx.x * blockDim.x + threadIdx.x) * VZ;
+    const int y = (int)(q0 / d.sy);
+    const int z0 = (int)(q0 - (long long)y * d.sy);
+    if (y >= r.y1 || z0 < r.zlo || z0 + VZ > r.zhi) return;
+    curr_vec_body<S, CM, PML, VZ>(I, V, cf, d, sc, ps, pp, r.x0 + blockIdx.y, q0);
+}
+
+template <typename S, int CM, int VZ>
+__global__ void __launch_bounds__(256)
+k_curr_shell(S* __restrict__ I, const S* __restrict__ V, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
+             Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp, ShellGrid g) {
+    int x;
+    long long q0;
+    if (!shell_vector(g, (long long)blockIdx.x * blockDim.x + threadIdx.x, VZ, d.sy, x, q0)) return;
+    curr_vec_body<S, CM, true, VZ>(I, V, cf, d, sc, ps, pp, x, q0);
 }
 
 // ---------------------------------------------------------------------------

@@ -197,10 +197,15 @@ int finalize_pml(fdtd_engine* h) {
     HIPCHK(hipStreamSynchronize(h->stream));
     int rc2 = compress_pml(h);
     if (rc2) return rc2;
+    find_shell(h);
     h->pml_host_coef.clear();
     h->pml_dirty = false;
     h->pml_final = true;
     return FDTD_OK;
+}
+
+void find_shell(fdtd_engine* h) {
+    h->shell_ok = find_interior(h->N, h->pml.slab, h->pml.count, h->ibox);
 }
 
 // Deduplicate the per-cell UPML coefficient 9-tuples (k, fo, fn for three
@@ -401,33 +406,59 @@ void launch_curr_march(fdtd_engine* h, CoeffPtrs<typename StoreTraits<S>::A> cf,
     }
 }
 
-template <typename S, bool PML>
-void launch_volt_vec(fdtd_engine* h, CoeffPtrs<typename StoreTraits<S>::A> cf, PmlPtrs<typename StoreTraits<S>::A> pp) {
-    typedef typename StoreTraits<S>::A A;
+VecLaunch vec_launch(const fdtd_engine* h, int vz, bool curr) {
+    return plan_vec_launch(h->N, h->d.sy, h->pml.count, pml_split_in_use(h), h->ibox, vz, curr);
+}
+
+template <typename S, bool PML, int CM>
+void launch_volt_main(fdtd_engine* h, const VecLaunch& L, CoeffPtrs<typename StoreTraits<S>::A> cf, PmlPtrs<typename StoreTraits<S>::A> pp) {
     constexpr int VZ = VecWidth<S>::value;
-    Scales<A> sc = make_scales<A>(h);
-    const long long plane = (long long)h->N[1] * h->d.sy;
     const dim3 b(256, 1, 1);
-    const dim3 g((unsigned)((plane / VZ + 255) / 256), h->N[0], 1);
-    switch (h->cf_mode) {
-    case CF_FULL: hipLaunchKernelGGL((k_volt_vec<S, CF_FULL, PML, VZ>), g, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, sc, h->pml, pp); break;
-    case CF_IDX16: hipLaunchKernelGGL((k_volt_vec<S, CF_IDX16, PML, VZ>), g, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, sc, h->pml, pp); break;
-    default: hipLaunchKernelGGL((k_volt_vec<S, CF_IDX32, PML, VZ>), g, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, sc, h->pml, pp); break;
+    if (L.nx) {
+        const dim3 g((unsigned)((L.nq / VZ + 255) / 256), L.nx, 1);
+        hipLaunchKernelGGL((k_volt_vec<S, CM, PML, VZ>), g, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.r);
+    }
+    if (L.shell) {
+        const long long n = L.g.nA0 + L.g.nA1 + L.g.nB + L.g.nC;
+        if (n > 0)
+            hipLaunchKernelGGL((k_volt_shell<S, CM, VZ>), dim3((unsigned)((n + 255) / 256)), b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d,
+                               make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.g);
     }
 }
 
-template <typename S, bool PML>
-void launch_curr_vec(fdtd_engine* h, CoeffPtrs<typename StoreTraits<S>::A> cf, PmlPtrs<typename StoreTraits<S>::A> pp) {
-    typedef typename StoreTraits<S>::A A;
+template <typename S, bool PML, int CM>
+void launch_curr_main(fdtd_engine* h, const VecLaunch& L, CoeffPtrs<typename StoreTraits<S>::A> cf, PmlPtrs<typename StoreTraits<S>::A> pp) {
     constexpr int VZ = VecWidth<S>::value;
-    Scales<A> sc = make_scales<A>(h);
-    const long long plane = (long long)(h->N[1] - 1) * h->d.sy;
     const dim3 b(256, 1, 1);
-    const dim3 g((unsigned)((plane / VZ + 255) / 256), h->N[0] > 1 ? h->N[0] - 1 : 1, 1);
+    if (L.nx) {
+        const dim3 g((unsigned)((L.nq / VZ + 255) / 256), L.nx, 1);
+        hipLaunchKernelGGL((k_curr_vec<S, CM, PML, VZ>), g, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.r);
+    }
+    if (L.shell) {
+        const long long n = L.g.nA0 + L.g.nA1 + L.g.nB + L.g.nC;
+        if (n > 0)
+            hipLaunchKernelGGL((k_curr_shell<S, CM, VZ>), dim3((unsigned)((n + 255) / 256)), b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d,
+                               make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.g);
+    }
+}
+
+template <typename S>
+void launch_volt_vec(fdtd_engine* h, CoeffPtrs<typename StoreTraits<S>::A> cf, PmlPtrs<typename StoreTraits<S>::A> pp) {
+    const VecLaunch L = vec_launch(h, VecWidth<S>::value, false);
     switch (h->cf_mode) {
-    case CF_FULL: hipLaunchKernelGGL((k_curr_vec<S, CF_FULL, PML, VZ>), g, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, sc, h->pml, pp); break;
-    case CF_IDX16: hipLaunchKernelGGL((k_curr_vec<S, CF_IDX16, PML, VZ>), g, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, sc, h->pml, pp); break;
-    default: hipLaunchKernelGGL((k_curr_vec<S, CF_IDX32, PML, VZ>), g, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, sc, h->pml, pp); break;
+    case CF_FULL: if (L.pml_main) launch_volt_main<S, true, CF_FULL>(h, L, cf, pp); else launch_volt_main<S, false, CF_FULL>(h, L, cf, pp); break;
+    case CF_IDX16: if (L.pml_main) launch_volt_main<S, true, CF_IDX16>(h, L, cf, pp); else launch_volt_main<S, false, CF_IDX16>(h, L, cf, pp); break;
+    default: if (L.pml_main) launch_volt_main<S, true, CF_IDX32>(h, L, cf, pp); else launch_volt_main<S, false, CF_IDX32>(h, L, cf, pp); break;
+    }
+}
+
+template <typename S>
+void launch_curr_vec(fdtd_engine* h, CoeffPtrs<typename StoreTraits<S>::A> cf, PmlPtrs<typename StoreTraits<S>::A> pp) {
+    const VecLaunch L = vec_launch(h, VecWidth<S>::value, true);
+    switch (h->cf_mode) {
+    case CF_FULL: if (L.pml_main) launch_curr_main<S, true, CF_FULL>(h, L, cf, pp); else launch_curr_main<S, false, CF_FULL>(h, L, cf, pp); break;
+    case CF_IDX16: if (L.pml_main) launch_curr_main<S, true, CF_IDX16>(h, L, cf, pp); else launch_curr_main<S, false, CF_IDX16>(h, L, cf, pp); break;
+    default: if (L.pml_main) launch_curr_main<S, true, CF_IDX32>(h, L, cf, pp); else launch_curr_main<S, false, CF_IDX32>(h, L, cf, pp); break;
     }
 }
 
@@ -534,7 +565,7 @@ int launch_step_kernels(fdtd_engine* h, bool timed) {
     const size_t evi = timed ? h->ev_used : 0;
     if (timed) h->ev_used += 4;
     if (timed) HIPCHK(hipEventRecord(h->ev[evi], h->stream));
-    if (h->variant >= 3) { if (pml) launch_volt_vec<S, true>(h, cfe, pp); else launch_volt_vec<S, false>(h, cfe, pp); }
+    if (h->variant >= 3) launch_volt_vec<S>(h, cfe, pp);
     else if (h->variant == 2) { if (pml) launch_volt_band<S, true>(h, cfe, pp); else launch_volt_band<S, false>(h, cfe, pp); }
     else if (h->variant == 1) { if (pml) launch_volt_march<S, true>(h, cfe, pp); else launch_volt_march<S, false>(h, cfe, pp); }
     else if (pml) launch_volt_cm<S, true>(h, grd, blk, cfe, pp); else launch_volt_cm<S, false>(h, grd, blk, cfe, pp);
@@ -552,7 +583,7 @@ int launch_step_kernels(fdtd_engine* h, bool timed) {
         if ((a->flags & 2) && a->count)
             hipLaunchKernelGGL((k_ade_pre<S>), dim3((a->count + 255) / 256), dim3(256), 0, h->stream, h->I.as<S>(), ade_dev(a), h->d.sc, sc, 1);
     if (timed) HIPCHK(hipEventRecord(h->ev[evi + 2], h->stream));
-    if (h->variant >= 3) { if (pml) launch_curr_vec<S, true>(h, cfh, pp); else launch_curr_vec<S, false>(h, cfh, pp); }
+    if (h->variant >= 3) launch_curr_vec<S>(h, cfh, pp);
     else if (h->variant == 2) { if (pml) launch_curr_band<S, true>(h, cfh, pp); else launch_curr_band<S, false>(h, cfh, pp); }
     else if (h->variant == 1) { if (pml) launch_curr_march<S, true>(h, cfh, pp); else launch_curr_march<S, false>(h, cfh, pp); }
     else if (pml) launch_curr_cm<S, true>(h, grdc, blk, cfh, pp); else launch_curr_cm<S, false>(h, grdc, blk, cfh, pp);
@@ -1277,6 +1308,22 @@ int fdtd_pml_compressed(fdtd_engine* h, int* active) {
     int rc = finalize_pml(h);
     if (rc) return rc;
     *active = pml_rows_in_use(h) ? 1 : 0;
+    return FDTD_OK;
+}
+
+int fdtd_set_pml_split(fdtd_engine* h, int mode) {
+    if (!h) return fail(FDTD_ERR_ARG, "null handle");
+    if (mode < -1 || mode > 1) return fail(FDTD_ERR_ARG, "mode must be -1 (auto), 0 or 1");
+    h->pml_split_mode = mode;
+    invalidate_graph(h);
+    return FDTD_OK;
+}
+
+int fdtd_pml_split(fdtd_engine* h, int* active) {
+    if (!h || !active) return fail(FDTD_ERR_ARG, "null argument");
+    int rc = finalize_pml(h);
+    if (rc) return rc;
+    *active = pml_split_in_use(h) ? 1 : 0;
     return FDTD_OK;
 }
 

@@ -2,6 +2,7 @@
 // single-domain (fdtd_engine.hip) and batched (fdtd_batch.hip) C ABI code.
 #pragma once
 #include "fdtd_device.h"
+#include "fdtd_shell.h"
 
 // ---------------------------------------------------------------------------
 // error handling
@@ -114,6 +115,12 @@ struct fdtd_engine {
     DevBuf pml_vidx, pml_cidx, pml_vtab, pml_ctab;
     bool pml_cmp = false;         // tables built
     int pml_cmp_mode = -1;        // -1 auto (with full main coefficients), 0 off, 1 on
+    // slabs forming a shell around the interior box [ibox[n][0], ibox[n][1]):
+    // the vector kernels then run the interior without PML code and the
+    // shell in a launch of its own
+    bool shell_ok = false;
+    int ibox[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+    int pml_split_mode = -1;      // -1 auto (FP16/BF16 storage), 0 off, 1 on
 
     // excitation
     bool has_exc = false;
@@ -181,9 +188,16 @@ struct fdtd_engine {
 namespace wipems {
 void invalidate_graph(fdtd_engine* h);
 int finalize_pml(fdtd_engine* h);
+void find_shell(fdtd_engine* h);
 int compress_pml(fdtd_engine* h);
 bool fused_ok(const fdtd_engine* h);
 int ensure_fused_buffers(fdtd_engine* h);
+// split interior / shell launches of the vector kernels (see fdtd_set_pml_split)
+inline bool pml_split_in_use(const fdtd_engine* h) {
+    if (!h->shell_ok || h->pml_split_mode == 0) return false;
+    return h->pml_split_mode == 1 || h->elem == 2;
+}
+
 inline bool pml_rows_in_use(const fdtd_engine* h) {
     return h->pml_cmp && (h->pml_cmp_mode == 1 || (h->pml_cmp_mode < 0 && h->cf_mode == CF_FULL));
 }

@@ -208,6 +208,15 @@ class Engine:
         self._chk(self._lib.fdtd_pml_compressed(self._h, byref(a)))
         return bool(a.value)
 
+    def set_pml_split(self, mode=-1):
+        """Interior/shell launches for UPML: 1 split, 0 one launch, -1 auto (FP16/BF16 only)."""
+        self._chk(self._lib.fdtd_set_pml_split(self._h, int(mode)))
+
+    def pml_split(self):
+        a = c_int()
+        self._chk(self._lib.fdtd_pml_split(self._h, byref(a)))
+        return bool(a.value)
+
     def set_tuning(self, variant=3, tile_y=4, chunk_x=1):
         self._chk(self._lib.fdtd_set_tuning(self._h, int(variant), int(tile_y), int(chunk_x)))
 
