@@ -348,3 +348,23 @@ def test_upml_split_launches_equal_one_launch(gpu, precision, prec, case):
     for a, b in zip(out[0], out[1]):
         np.testing.assert_array_equal(a, b)
     assert np.abs(out[0][0]).max() > 0
+
+
+def test_three_slabs_in_one_vector_are_rejected(gpu):
+    """The vector kernels apply at most two UPML slabs to one vector of z
+    cells; a slab set that puts three into one vector is refused when
+    stepping is prepared, before any kernel runs."""
+    from wipems_amd._native import FdtdError
+    from wipems_amd.engine import Engine
+    from wipems_amd.operator import UpmlSlab
+    N = (6, 6, 12)
+    eng = Engine(N, 1e-12, "fp32")
+    z = np.zeros((3, 6, 6, 12), np.float32)
+    eng.set_coeffs(z, z, z, z)
+    for z0 in (0, 1, 2):           # three 1-cell slabs, z 0..2: one 4-cell vector
+        n = (6, 6, 1)
+        arrs = [np.ones((3,) + n, np.float32)] * 6
+        eng.add_upml(UpmlSlab(start=(0, 0, z0), num_lines=n, arrays=arrs))
+    with pytest.raises(FdtdError, match="share a vector"):
+        eng.iterate(1)
+    eng.close()

@@ -337,6 +337,51 @@ __device__ __forceinline__ void pml_coeffs(const A* full0, const A* full1, const
     }
 }
 
+// UPML form of one component for the lanes pm of a vector in one slab
+// (Engine_Ext_UPML pre/post around the main update, fused): per lane
+// h = k*V - fo*F; F = F*a + b*curl; V = h + fn*F.  q = slab-array offset.
+template <typename A, int VZ>
+__device__ __forceinline__ void upml_lanes(A* flux, const A* k0, const A* kfo, const A* kfn, const A* tab,
+                                           const unsigned short* rows, long long q, int n, unsigned pm, const A* v,
+                                           const A* a, const A* b, const A* curl, A* out) {
+    A F[VZ], kv[VZ], fo[VZ], fn[VZ];
+    vload<A, VZ>(flux + q, F, A(1));
+    pml_coeffs<A, VZ>(k0, kfo, kfn, q, tab, rows, n, kv, fo, fn);
+#pragma unroll
+    for (int k = 0; k < VZ; ++k) {
+        if (pm & (1u << k)) {
+            A h = kv[k] * v[k] - fo[k] * F[k];
+            A t = F[k];
+            t = t * a[k];
+            t = t + b[k] * curl[k];
+            F[k] = t;
+            out[k] = h + fn[k] * t;
+        }
+    }
+    vstore<A, VZ>(flux + q, F, A(1), VZ);
+}
+
+// the (at most two, see fdtd_engine's slab check) slabs holding cells of the
+// vector z0..z0+VZ-1 at (x, y), with their lane masks and slab-array offsets
+struct SlabHits {
+    int s[2];
+    unsigned m[2];
+    long long b[2];
+};
+template <int VZ>
+__device__ __forceinline__ SlabHits slab_hits(const PmlSet& ps, int x, int y, int z0, unsigned lanes) {
+    SlabHits h{{-1, -1}, {0u, 0u}, {0, 0}};
+    int c = 0;
+    for (int s = 0; s < ps.count && c < 2; ++s) {
+        long long pb = 0;
+        const unsigned m = slab_lanes(ps.slab[s], x, y, z0, VZ, pb) & lanes;
+        if (!m) continue;
+        h.s[c] = s; h.m[c] = m; h.b[c] = pb;
+        ++c;
+    }
+    return h;
+}
+
 template <typename A, int CM, int VZ>
 __device__ __forceinline__ void vload_coeffs(const CoeffPtrs<A>& cf, long long p, long long sc, A a[3][VZ], A b[3][VZ]) {
     if (CM == CF_FULL) {
@@ -405,6 +450,17 @@ __device__ __forceinline__ void volt_vec_body(S* __restrict__ V, const S* __rest
     A vv[3][VZ], vi[3][VZ];
     vload_coeffs<A, CM, VZ>(cf, p, d.sc, vv, vi);
     const int valid = min(VZ, d.nz - z0);
+    SlabHits hit{{-1, -1}, {0u, 0u}, {0, 0}};
+    SVec<unsigned short, VZ> prow[2] = {};
+    if (PML) {
+        hit = slab_hits<VZ>(ps, x, y, z0, ~0u);
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+            if (hit.s[j] >= 0 && pp.vtab) {
+                const PmlSlab& b = ps.slab[hit.s[j]];
+                prow[j] = *reinterpret_cast<const SVec<unsigned short, VZ>*>(pp.vidx + (hit.b[j] - b.off + b.ioff));
+            }
+    }
 #pragma unroll
     for (int n = 0; n < 3; ++n) {
         A v[VZ], out[VZ];
@@ -415,30 +471,13 @@ __device__ __forceinline__ void volt_vec_body(S* __restrict__ V, const S* __rest
             out[k] = out[k] + vi[n][k] * curl[n][k];
         }
         if (PML) {
-            // the UPML form for the cells of every slab the vector touches
-            for (int s = 0; s < ps.count; ++s) {
-                const PmlSlab& b = ps.slab[s];
-                long long pb = 0;
-                const unsigned pm = slab_lanes(b, x, y, z0, VZ, pb);
-                if (!pm) continue;
-                SVec<unsigned short, VZ> prow{};
-                if (pp.vtab) prow = *reinterpret_cast<const SVec<unsigned short, VZ>*>(pp.vidx + (pb - b.off + b.ioff));
-                const long long q = pb + n * b.sc;
-                A F[VZ], kv[VZ], kfo[VZ], kfn[VZ];
-                vload<A, VZ>(pp.vflux + q, F, A(1));
-                pml_coeffs<A, VZ>(pp.vv, pp.vvfo, pp.vvfn, q, pp.vtab, prow.v, n, kv, kfo, kfn);
+            // the UPML form for the cells of the slabs the vector touches
 #pragma unroll
-                for (int k = 0; k < VZ; ++k) {
-                    if (pm & (1u << k)) {
-                        A h = kv[k] * v[k] - kfo[k] * F[k];
-                        A t = F[k];
-                        t = t * vv[n][k];
-                        t = t + vi[n][k] * curl[n][k];
-                        F[k] = t;
-                        out[k] = h + kfn[k] * t;
-                    }
-                }
-                vstore<A, VZ>(pp.vflux + q, F, A(1), VZ);
+            for (int j = 0; j < 2; ++j) {
+                if (hit.s[j] < 0) continue;
+                const PmlSlab& b = ps.slab[hit.s[j]];
+                upml_lanes<A, VZ>(pp.vflux, pp.vv, pp.vvfo, pp.vvfn, pp.vtab, prow[j].v, hit.b[j] + n * b.sc, n, hit.m[j], v,
+                                  vv[n], vi[n], curl[n], out);
             }
         }
         vstore<S, VZ>(V + n * d.sc + p, out, sc.iv[n], valid);
@@ -557,6 +596,17 @@ __device__ __forceinline__ void curr_vec_body(S* __restrict__ I, const S* __rest
     const int valid = min(VZ, d.nz - 1 - z0);
     // cells at z >= nz-1 are not updated: keep them out of the slab masks too
     const unsigned upd = valid < VZ ? (1u << valid) - 1u : ~0u;
+    SlabHits hit{{-1, -1}, {0u, 0u}, {0, 0}};
+    SVec<unsigned short, VZ> prow[2] = {};
+    if (PML) {
+        hit = slab_hits<VZ>(ps, x, y, z0, upd);
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+            if (hit.s[j] >= 0 && pp.ctab) {
+                const PmlSlab& b = ps.slab[hit.s[j]];
+                prow[j] = *reinterpret_cast<const SVec<unsigned short, VZ>*>(pp.cidx + (hit.b[j] - b.off + b.ioff));
+            }
+    }
 #pragma unroll
     for (int n = 0; n < 3; ++n) {
         A c[VZ], out[VZ];
@@ -567,29 +617,12 @@ __device__ __forceinline__ void curr_vec_body(S* __restrict__ I, const S* __rest
             out[k] = out[k] + iv[n][k] * curl[n][k];
         }
         if (PML) {
-            for (int s = 0; s < ps.count; ++s) {
-                const PmlSlab& b = ps.slab[s];
-                long long pb = 0;
-                const unsigned pm = slab_lanes(b, x, y, z0, VZ, pb) & upd;
-                if (!pm) continue;
-                SVec<unsigned short, VZ> prow{};
-                if (pp.ctab) prow = *reinterpret_cast<const SVec<unsigned short, VZ>*>(pp.cidx + (pb - b.off + b.ioff));
-                const long long q = pb + n * b.sc;
-                A F[VZ], ki[VZ], kfo[VZ], kfn[VZ];
-                vload<A, VZ>(pp.cflux + q, F, A(1));
-                pml_coeffs<A, VZ>(pp.ii, pp.iifo, pp.iifn, q, pp.ctab, prow.v, n, ki, kfo, kfn);
 #pragma unroll
-                for (int k = 0; k < VZ; ++k) {
-                    if (pm & (1u << k)) {
-                        A h = ki[k] * c[k] - kfo[k] * F[k];
-                        A t = F[k];
-                        t = t * ii[n][k];
-                        t = t + iv[n][k] * curl[n][k];
-                        F[k] = t;
-                        out[k] = h + kfn[k] * t;
-                    }
-                }
-                vstore<A, VZ>(pp.cflux + q, F, A(1), VZ);
+            for (int j = 0; j < 2; ++j) {
+                if (hit.s[j] < 0) continue;
+                const PmlSlab& b = ps.slab[hit.s[j]];
+                upml_lanes<A, VZ>(pp.cflux, pp.ii, pp.iifo, pp.iifn, pp.ctab, prow[j].v, hit.b[j] + n * b.sc, n, hit.m[j], c,
+                                  ii[n], iv[n], curl[n], out);
             }
         }
         vstore<S, VZ>(I + n * d.sc + p, out, sc.ic[n], valid);

@@ -179,6 +179,21 @@ int capture_graph(fdtd_engine* h, int steps, hipGraph_t* g, hipGraphExec_t* ge);
 
 int finalize_pml(fdtd_engine* h) {
     if (!h->pml_dirty) return FDTD_OK;
+    // the vector kernels apply at most two slabs to one vector of z cells
+    const int vz = (int)(16 / h->elem);
+    for (int i = 0; i < h->pml.count; ++i)
+        for (int j = i + 1; j < h->pml.count; ++j)
+            for (int k = j + 1; k < h->pml.count; ++k) {
+                const PmlSlab* t[3] = {&h->pml.slab[i], &h->pml.slab[j], &h->pml.slab[k]};
+                int lo[3] = {0, 0, 0}, hi[3] = {1 << 30, 1 << 30, 1 << 30};
+                for (const PmlSlab* b : t) {
+                    for (int n = 0; n < 2; ++n) { lo[n] = std::max(lo[n], b->s[n]); hi[n] = std::min(hi[n], b->s[n] + b->n[n]); }
+                    lo[2] = std::max(lo[2], b->s[2] / vz);
+                    hi[2] = std::min(hi[2], (b->s[2] + b->n[2] - 1) / vz + 1);
+                }
+                if (lo[0] < hi[0] && lo[1] < hi[1] && lo[2] < hi[2])
+                    return fail(FDTD_ERR_ARG, "UPML slabs %d, %d and %d share a vector of %d z cells", i, j, k, vz);
+            }
     const size_t a = h->acc;
     int rc = h->pml_coef.alloc((size_t)6 * h->pml_total * a);
     if (rc) return rc;
