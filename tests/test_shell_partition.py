@@ -72,3 +72,9 @@ def test_non_shell_slabs_use_one_launch(harness):
                           ((16, 16, 16), [((0, 0, 0), (4, 16, 16)), ((0, 0, 0), (1, 1, 1))][:1])])
     assert " shell 0 " in lines[0]
     assert " shell 1 " in lines[-1]
+
+
+def test_no_upml_single_launch(harness):
+    # no slabs: one launch over everything, XCD-contiguous block order
+    lines = run(harness, [((37, 29, 45), []), ((9, 300, 17), [])])
+    assert len(lines) == 12 and all(" shell 0 " in l for l in lines)
