@@ -494,14 +494,32 @@ struct VecRange {
 };
 
 template <typename S, int CM, bool PML, int VZ>
-__global__ void __launch_bounds__(256)
-k_volt_vec(S* __restrict__ V, const S* __restrict__ I, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
-           Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp, VecRange r) {
+__device__ __forceinline__ void volt_vec_range(S* __restrict__ V, const S* __restrict__ I, const CoeffPtrs<typename StoreTraits<S>::A>& cf,
+                                               const Dims& d, const Scales<typename StoreTraits<S>::A>& sc, const PmlSet& ps,
+                                               const PmlPtrs<typename StoreTraits<S>::A>& pp, const VecRange& r) {
     const long long q0 = (long long)r.y0 * d.sy + ((long long)blockIdx.x * blockDim.x + threadIdx.x) * VZ;
     const int y = (int)(q0 / d.sy);
     const int z0 = (int)(q0 - (long long)y * d.sy);
     if (y >= r.y1 || z0 < r.zlo || z0 + VZ > r.zhi) return;
     volt_vec_body<S, CM, PML, VZ>(V, I, cf, d, sc, ps, pp, r.x0 + blockIdx.y, q0);
+}
+
+// Without UPML code the voltage update fits 5 waves per SIMD for every
+// storage type (the bound keeps FP16/BF16 at 96 VGPRs without spills); the
+// current update would spill under the same bound and the UPML forms need
+// more registers, so they get none.
+template <typename S, int CM, int VZ>
+__global__ void __launch_bounds__(256, 5)
+k_volt_vec(S* __restrict__ V, const S* __restrict__ I, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
+           Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp, VecRange r) {
+    volt_vec_range<S, CM, false, VZ>(V, I, cf, d, sc, ps, pp, r);
+}
+
+template <typename S, int CM, int VZ>
+__global__ void __launch_bounds__(256)
+k_volt_vec_upml(S* __restrict__ V, const S* __restrict__ I, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
+                Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp, VecRange r) {
+    volt_vec_range<S, CM, true, VZ>(V, I, cf, d, sc, ps, pp, r);
 }
 
 // The vectors outside the interior box of a UPML shell, numbered
@@ -630,14 +648,28 @@ __device__ __forceinline__ void curr_vec_body(S* __restrict__ I, const S* __rest
 }
 
 template <typename S, int CM, bool PML, int VZ>
-__global__ void __launch_bounds__(256)
-k_curr_vec(S* __restrict__ I, const S* __restrict__ V, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
-           Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp, VecRange r) {
+__device__ __forceinline__ void curr_vec_range(S* __restrict__ I, const S* __restrict__ V, const CoeffPtrs<typename StoreTraits<S>::A>& cf,
+                                               const Dims& d, const Scales<typename StoreTraits<S>::A>& sc, const PmlSet& ps,
+                                               const PmlPtrs<typename StoreTraits<S>::A>& pp, const VecRange& r) {
     const long long q0 = (long long)r.y0 * d.sy + ((long long)blockIdx.x * blockDim.x + threadIdx.x) * VZ;
     const int y = (int)(q0 / d.sy);
     const int z0 = (int)(q0 - (long long)y * d.sy);
     if (y >= r.y1 || z0 < r.zlo || z0 + VZ > r.zhi) return;
     curr_vec_body<S, CM, PML, VZ>(I, V, cf, d, sc, ps, pp, r.x0 + blockIdx.y, q0);
+}
+
+template <typename S, int CM, int VZ>
+__global__ void __launch_bounds__(256)
+k_curr_vec(S* __restrict__ I, const S* __restrict__ V, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
+           Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp, VecRange r) {
+    curr_vec_range<S, CM, false, VZ>(I, V, cf, d, sc, ps, pp, r);
+}
+
+template <typename S, int CM, int VZ>
+__global__ void __launch_bounds__(256)
+k_curr_vec_upml(S* __restrict__ I, const S* __restrict__ V, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
+                Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp, VecRange r) {
+    curr_vec_range<S, CM, true, VZ>(I, V, cf, d, sc, ps, pp, r);
 }
 
 template <typename S, int CM, int VZ>
