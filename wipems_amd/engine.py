@@ -209,7 +209,7 @@ class Engine:
         return bool(a.value)
 
     def set_pml_split(self, mode=-1):
-        """Interior/shell launches for UPML: 1 split, 0 one launch, -1 auto (FP16/BF16 only)."""
+        """Interior/shell launches for UPML: 1 split, 0 one launch, -1 auto (FP16/BF16 with an interior of >= 88% of the cells)."""
         self._chk(self._lib.fdtd_set_pml_split(self._h, int(mode)))
 
     def pml_split(self):
