@@ -155,6 +155,15 @@ int  fdtd_add_ade(fdtd_engine* h, uint32_t count, const uint32_t* pos, int dtype
                   const void* v_int, const void* v_ext, const void* v_lor,
                   const void* i_int, const void* i_ext, const void* i_lor, uint32_t flags);
 
+/* Mur ABC face (Engine_Ext_Mur_ABC, engine_ext_mur_abc.cpp:83-168;
+ * coefficients from Operator_Ext_Mur_ABC::BuildExtension,
+ * operator_ext_mur_abc.cpp:117-180): face = 2*axis + top (x-lo 0 .. z-hi 5),
+ * k_nyP/k_nyPP the first-order coefficients of the two tangential
+ * components over the boundary plane, [N(axis+1)][N(axis+2)], start_ts the
+ * timestep from which the face acts (wop_get_mur).  Hooks run in the
+ * reference's extension order: pre before the ADE pre hook, post after the
+ * TFSF injection, apply after the ADE apply and before the excitation. */
+int  fdtd_add_mur(fdtd_engine* h, int face, int dtype, const void* k_nyP, const void* k_nyPP, uint32_t start_ts);
 /* Total-field/scattered-field injection list.  which = FDTD_VOLT (added after
  * the voltage update using the current signal) or FDTD_CURR (added after the
  * current update using the voltage signal).  Each entry adds

@@ -59,6 +59,10 @@ int  wop_set_mesh(wop_operator* op, int axis, const double* lines, uint32_t n);
 int  wop_set_grid_delta(wop_operator* op, double unit);
 int  wop_set_boundary(wop_operator* op, const int bc[6], const uint32_t pml_size[6]);
 int  wop_set_timestep_factor(wop_operator* op, double factor);
+/* Mur ABC (boundary code WOP_BC_MUR) phase velocity of one face
+ * (face = 2*axis + top), 0 = the material's own (openems.cpp:749-754,
+ * Operator_Ext_Mur_ABC::SetPhaseVelocity) */
+int  wop_set_mur_phase_velocity(wop_operator* op, int face, double v_phase);
 /* optional forced timestep (openems.cpp TimeStep): 0 = calculate */
 int  wop_set_timestep(wop_operator* op, double dT);
 int  wop_set_threads(wop_operator* op, int nthreads);
@@ -129,6 +133,13 @@ int  wop_get_excitation(wop_operator* op, uint32_t* volt_pos, uint32_t* volt_dir
 int  wop_get_tfsf_info(wop_operator* op, uint32_t* volt_count, uint32_t* curr_count, uint32_t* max_delay);
 int  wop_get_tfsf(wop_operator* op, int which, uint32_t* pos, uint32_t* comp, uint32_t* delay, double* delta, double* amp);
 
+/* Mur ABC faces after a build (Operator_Ext_Mur_ABC, operator_ext_mur_abc.cpp:117-180):
+ * per face the coefficients of its two tangential components over the
+ * boundary plane, [N(axis+1)][N(axis+2)], and the timestep from which the
+ * boundary acts (Engine_Ext_Mur_ABC's delay for excitations on the plane,
+ * engine_ext_mur_abc.cpp:42-58), for fdtd_add_mur */
+int  wop_get_num_mur(wop_operator* op, uint32_t* n);
+int  wop_get_mur(wop_operator* op, uint32_t k, int* face, int dtype, void* k_nyP, void* k_nyPP, uint32_t* start_ts);
 int  wop_get_num_upml(wop_operator* op, uint32_t* n);
 int  wop_get_upml_range(wop_operator* op, uint32_t k, uint32_t start[3], uint32_t num_lines[3]);
 int  wop_get_upml(wop_operator* op, uint32_t k, int dtype, void* vv, void* vvfo, void* vvfn, void* ii, void* iifo, void* iifn);

@@ -23,8 +23,10 @@
  *   ade_pre/apply             single-cell ODE of the Drude circuit in closed form
  *   voltage/current probes    hand-summed edge lists on fixed fields
  *   energy                    exact sum on small grids
- *   tfsf                      parity unpinned beyond the lookup rules (no plane-wave
- *                             builder is exercised yet)
+ *   tfsf                      scattered-field leakage of plane-wave boxes and the
+ *                             closed-form incident wave (tests/test_tfsf.py)
+ *   mur                       plane-wave reflection of a parallel-plate line against
+ *                             a PEC end and a 3x longer line (tests/test_mur.py)
  */
 #include <math.h>
 #include <stdint.h>
@@ -112,6 +114,7 @@ void orc_destroy(orc* o) {
     for (uint32_t s = 0; s < 6; ++s) {
         for (int k = 0; k < 6; ++k) { free(o->f32.ucoef[s][k]); free(o->f64.ucoef[s][k]); }
         free(o->f32.vflux[s]); free(o->f32.cflux[s]); free(o->f64.vflux[s]); free(o->f64.cflux[s]);
+        free(o->f32.mur_k[s]); free(o->f32.mur_buf[s]); free(o->f64.mur_k[s]); free(o->f64.mur_buf[s]);
     }
     free(o);
 }
@@ -184,6 +187,33 @@ int orc_add_ade(orc* o, uint32_t count, const uint32_t* pos, int dtype, const vo
     }
     if (o->dbl) ADDA(o->f64, double) else ADDA(o->f32, float)
 #undef ADDA
+    return 0;
+}
+
+int orc_add_mur(orc* o, int face, int dtype, const void* k_nyP, const void* k_nyPP, uint32_t start_ts) {
+    if (face < 0 || face > 5) return -1;
+    const int ny = face / 2;
+    const size_t plane = (size_t)o->N[(ny + 1) % 3] * o->N[(ny + 2) % 3];
+#define ADDM(F, T)                                                                     \
+    {                                                                                  \
+        uint32_t s = F.n_mur;                                                          \
+        if (s >= 6) return -1;                                                         \
+        while (s > 0 && F.mur_face[s - 1] > face) {                                    \
+            F.mur_face[s] = F.mur_face[s - 1]; F.mur_start[s] = F.mur_start[s - 1];     \
+            F.mur_k[s] = F.mur_k[s - 1]; F.mur_buf[s] = F.mur_buf[s - 1];               \
+            --s;                                                                       \
+        }                                                                              \
+        T* k = (T*)malloc(2 * plane * sizeof(T));                                      \
+        T* a = (T*)copy_as(o, dtype, k_nyP, plane);                                    \
+        T* b = (T*)copy_as(o, dtype, k_nyPP, plane);                                   \
+        memcpy(k, a, plane * sizeof(T)); memcpy(k + plane, b, plane * sizeof(T));      \
+        free(a); free(b);                                                              \
+        F.mur_face[s] = face; F.mur_start[s] = start_ts; F.mur_k[s] = k;               \
+        F.mur_buf[s] = (T*)zeros(o, 2 * plane);                                        \
+        ++F.n_mur;                                                                     \
+    }
+    if (o->dbl) ADDM(o->f64, double) else ADDM(o->f32, float)
+#undef ADDM
     return 0;
 }
 

@@ -38,6 +38,12 @@ typedef struct {
     uint32_t tf_n[2], tf_maxdelay[2];
     uint32_t *tf_pos[2], *tf_comp[2], *tf_del[2];
     REAL *tf_delta[2], *tf_amp[2];
+    /* Mur ABC faces, ascending face (2*axis + top) */
+    uint32_t n_mur;
+    int mur_face[6];
+    uint32_t mur_start[6];
+    REAL* mur_k[6];                   /* [2][na][nb]: nyP, nyPP */
+    REAL* mur_buf[6];
 } FN(fields_t);
 
 #define IDX(x, y, z) (((size_t)(x) * N1 + (y)) * N2 + (z))
@@ -233,16 +239,55 @@ static void FN(tfsf)(FN(fields_t) * f, const uint32_t* Nl, int curr, uint32_t nu
     }
 }
 
+/* Engine_Ext_Mur_ABC (engine_ext_mur_abc.cpp:83-168): phase 0 DoPreVoltageUpdates,
+ * 1 DoPostVoltageUpdates, 2 Apply2Voltages; each face acts from its start
+ * step on (IsActive, engine_ext_mur_abc.h:55).  Faces run in ascending order
+ * for the pre hook and in descending order for post and apply: the
+ * engine's extension list holds equal-priority extensions in reverse
+ * creation order (FDTD/engine.cpp:84-88) and openems.cpp:386-397 creates
+ * them x-lo .. z-hi. */
+static void FN(mur)(FN(fields_t) * f, const uint32_t* Nl, uint32_t numTS, int phase) {
+    const size_t N1 = Nl[1], N2 = Nl[2], nd = (size_t)Nl[0] * N1 * N2;
+    for (uint32_t q = 0; q < f->n_mur; ++q) {
+        const uint32_t s = phase == 0 ? q : f->n_mur - 1 - q;
+        if (numTS < f->mur_start[s]) continue;
+        const int face = f->mur_face[s], ny = face / 2, nyP = (ny + 1) % 3, nyPP = (ny + 2) % 3;
+        const uint32_t line = face % 2 ? Nl[ny] - 1 : 0, shift = face % 2 ? Nl[ny] - 2 : 1;
+        const size_t na = Nl[nyP], nb = Nl[nyPP];
+        uint32_t pos[3], pss[3];
+        pos[ny] = line;
+        pss[ny] = shift;
+        for (uint32_t a = 0; a < na; ++a) {
+            pos[nyP] = pss[nyP] = a;
+            for (uint32_t b = 0; b < nb; ++b) {
+                pos[nyPP] = pss[nyPP] = b;
+                const size_t p = IDX(pos[0], pos[1], pos[2]), ps = IDX(pss[0], pss[1], pss[2]);
+                for (int t = 0; t < 2; ++t) {
+                    REAL* V = f->V + (size_t)(t ? nyPP : nyP) * nd;
+                    const size_t j = (size_t)t * na * nb + (size_t)a * nb + b;
+                    const REAL k = f->mur_k[s][j];
+                    if (phase == 0) f->mur_buf[s][j] = V[ps] - k * V[p];
+                    else if (phase == 1) f->mur_buf[s][j] += k * V[ps];
+                    else V[p] = f->mur_buf[s][j];
+                }
+            }
+        }
+    }
+}
+
 /* One timestep in the order of Engine::IterateTS (FDTD/engine.cpp:255-274) with
- * the extensions sorted by priority: pre hooks Excitation, Lorentz, TFSF, UPML;
- * post and apply hooks UPML, TFSF, Lorentz, Excitation. */
+ * the extensions sorted by priority: pre hooks Excitation, Mur, Lorentz, TFSF,
+ * UPML; post and apply hooks UPML, TFSF, Lorentz, Mur, Excitation. */
 static void FN(step)(FN(fields_t) * f, const uint32_t* Nl, uint32_t numTS) {
+    FN(mur)(f, Nl, numTS, 0);
     FN(ade_pre)(f, Nl, 0);
     FN(upml_pre)(f, Nl, 0);
     FN(update_voltages)(f, Nl);
     FN(upml_post)(f, Nl, 0);
     FN(tfsf)(f, Nl, 0, numTS);
+    FN(mur)(f, Nl, numTS, 1);
     FN(ade_apply)(f, Nl, 0);
+    FN(mur)(f, Nl, numTS, 2);
     FN(excite)(f, Nl, 0, numTS);
 
     FN(ade_pre)(f, Nl, 1);

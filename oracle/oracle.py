@@ -33,6 +33,7 @@ def lib():
                                          c_uint32, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_double), POINTER(c_uint32)]
         L.orc_add_upml.argtypes = [c_void_p, POINTER(c_uint32), POINTER(c_uint32), c_int] + [c_void_p] * 6
         L.orc_add_ade.argtypes = [c_void_p, c_uint32, POINTER(c_uint32), c_int] + [c_void_p] * 6 + [c_uint32]
+        L.orc_add_mur.argtypes = [c_void_p, c_int, c_int, c_void_p, c_void_p, c_uint32]
         L.orc_set_tfsf.argtypes = [c_void_p, c_int, c_uint32, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_uint32),
                                    POINTER(c_double), POINTER(c_double), c_uint32]
         L.orc_add_probe.argtypes = [c_void_p, c_int, POINTER(c_uint32), POINTER(c_uint32), c_int, c_uint32, c_double,
@@ -114,6 +115,12 @@ class Oracle:
         code = 1 if lists[0].dtype == np.float64 else 0
         self._L.orc_add_ade(self._h, pos.shape[1], _up(pos), code, *[_vp(x) for x in lists], int(a.flags))
 
+    def add_mur(self, m):
+        kp = np.ascontiguousarray(m.k_nyP)
+        kpp = np.ascontiguousarray(m.k_nyPP, dtype=kp.dtype)
+        code = 1 if kp.dtype == np.float64 else 0
+        self._L.orc_add_mur(self._h, int(m.face), code, _vp(kp), _vp(kpp), int(m.start_ts))
+
     def set_tfsf(self, which, pos, comp, delay, delta, amp, max_delay):
         pos = np.ascontiguousarray(pos, dtype=np.uint32)
         comp = np.ascontiguousarray(comp, dtype=np.uint32)
@@ -176,6 +183,8 @@ def oracle_from_operator(data, use_double=None, ftz=True):
         o.add_upml(s)
     for a in data.ade:
         o.add_ade(a)
+    for m in getattr(data, "mur", []):
+        o.add_mur(m)
     if data.tfsf:
         for which in (0, 1):
             t = data.tfsf[which]

@@ -389,3 +389,54 @@ def test_upml_split_auto_policy(gpu):
         eng = engine_from_operator(d, precision=precision)
         assert eng.pml_split() == want, (precision, want)
         eng.close()
+
+
+def _mur_case(prec, bc, n=(26, 22, 24), steps=400):
+    from wipems_amd.operator import Operator
+    rng = np.random.default_rng(31)
+    lines = [np.concatenate([[0.0], np.cumsum(rng.uniform(0.6, 1.5, k - 1))]) for k in n]
+    op = Operator()
+    op.set_mesh(lines, 1e-3)
+    op.set_boundary(bc, [5] * 6)
+    op.set_gauss_excite(5e9, 10e9)
+    m = op.add_material(eps_r=3.0, kappa=0.01)
+    op.add_box(m, (lines[0][0], lines[1][4], lines[2][5]), (lines[0][6], lines[1][12], lines[2][14]), priority=1)
+    e = op.add_excitation(0, (0.0, 0.0, 1.0))
+    c = [lines[k][n[k] // 2] for k in range(3)]
+    op.add_box(e, (c[0], c[1], lines[2][10]), (c[0], c[1], lines[2][13]), priority=5)
+    # an excitation on the x-lo plane: that face starts after the pulse
+    e2 = op.add_excitation(0, (0.0, 1.0, 0.0), delay=2e-11)
+    op.add_box(e2, (lines[0][0], lines[1][8], lines[2][9]), (lines[0][0], lines[1][10], lines[2][9]), priority=5)
+    d = op.build(prec, nr_ts=steps)
+    d.probes = [op.voltage_probe((c[0], c[1], lines[2][10]), (c[0], c[1], lines[2][13]), td_interval=1, name="u")]
+    return d
+
+
+@pytest.mark.parametrize("precision,prec", [("fp32", 0), ("fp64", 1)])
+@pytest.mark.parametrize("bc", [[2] * 6, [2, 2, 3, 3, 2, 0]])
+def test_mur_abc_bitwise(gpu, precision, prec, bc):
+    d = _mur_case(prec, bc)
+    assert len(d.mur) == bc.count(2)
+    assert d.mur[0].face == 0 and d.mur[0].start_ts > 0
+    print("case: Mur ABC, bc %s, %s" % (bc, precision), flush=True)
+    eng = engine_from_operator(d, precision=precision)
+    _kinds[id(eng)] = [p.kind for p in d.probes]
+    orc = oracle_from_operator(d, ftz=True)
+    assert d.mur[0].start_ts < 380
+    for chunk in (1, 37, 362):          # the x-lo face starts after the pulse on its plane
+        eng.iterate(chunk)
+        orc.iterate(chunk)
+    compare(eng, orc)
+    assert np.abs(eng.get_field(0)[2][0]).max() > 0     # V_z on the PEC-coefficient x-lo plane: set by Mur
+
+
+def test_mur_abc_fp16_close_to_fp32(gpu):
+    from wipems_amd.engine import pilot_scales
+    d = _mur_case(0, [2] * 6)
+    out = []
+    for precision in ("fp32", "fp16"):
+        eng = engine_from_operator(d, precision=precision, scales=pilot_scales(d) if precision == "fp16" else None)
+        eng.iterate(400)
+        out.append(eng.read_probe(eng.probe_ids[0])[1])
+        eng.close()
+    assert rel_l2(out[1], out[0]) < 1e-2

@@ -28,6 +28,11 @@ def compare(op, full, steps):
         np.testing.assert_array_equal(a.num_lines, b.num_lines)
         for x, y in zip(a.arrays, b.arrays):
             np.testing.assert_array_equal(x, y)
+    assert len(st.mur) == len(full.mur)
+    for a, b in zip(st.mur, full.mur):
+        assert (a.face, a.start_ts) == (b.face, b.start_ts)
+        np.testing.assert_array_equal(a.k_nyP, b.k_nyP)
+        np.testing.assert_array_equal(a.k_nyPP, b.k_nyPP)
     np.testing.assert_array_equal(st.edge_primary, full.edge_primary)
     assert st.vv is None and st.cells == full.cells
     return st
@@ -103,3 +108,20 @@ def test_long_graded_rows_grow_the_row_index():
     st = compare(op, full, 30)
     row = st.index[3, 2]
     assert len(np.unique(row)) > 100
+
+
+def test_mur_faces_with_upml_and_boundary_excitation():
+    from wipems_amd.operator import BC_MUR
+    rng = np.random.default_rng(5)
+    lines = [np.concatenate([[0.0], np.cumsum(rng.uniform(0.6, 1.4, n))]) for n in (21, 17, 19)]
+    op = Operator()
+    op.set_mesh(lines, 1e-3)
+    op.set_boundary([BC_MUR, BC_MUR, BC_PML, BC_PML, BC_MUR, BC_PEC], [0, 0, 4, 4, 0, 0])
+    op.set_gauss_excite(3e9, 3e9)
+    d = op.add_material(eps_r=5.0)
+    op.add_box(d, (lines[0][0], lines[1][3], lines[2][2]), (lines[0][6], lines[1][9], lines[2][8]), priority=2)
+    e = op.add_excitation(0, (0.0, 1.0, 0.0), delay=1e-11)
+    op.add_box(e, (lines[0][0], lines[1][5], lines[2][6]), (lines[0][0], lines[1][8], lines[2][6]), priority=9)
+    full = op.build(0, nr_ts=40)
+    assert [m.face for m in full.mur] == [0, 1, 4] and full.mur[0].start_ts > 0
+    compare(op, full, 40)

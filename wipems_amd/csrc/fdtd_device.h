@@ -933,6 +933,58 @@ __global__ void k_excite(S* F, const unsigned* __restrict__ step, ExcDev e, Scal
     }
 }
 
+// ---------------------------------------------------------------------------
+// Mur ABC, first order (Engine_Ext_Mur_ABC, engine_ext_mur_abc.cpp:83-168).
+// For the two tangential components t of the boundary plane `line` of axis
+// ny, with the neighbouring plane `shift` one line inside:
+//   pre   (before the voltage update)  buf = V_t(shift) - k_t * V_t(line)
+//   post  (after the update)           buf = buf + k_t * V_t(shift)
+//   apply                              V_t(line) = buf
+// from timestep start_ts on (before it, nothing: excitations on the plane).
+// ---------------------------------------------------------------------------
+struct MurDev {
+    int ny, nyP, nyPP;
+    int line, shift;
+    int na, nb;                // lines along nyP and nyPP
+    unsigned start_ts;
+    long long sny;             // element stride along ny
+    const void* k;             // [2][na][nb] coefficients (nyP, nyPP)
+    void* buf;                 // [2][na][nb]
+};
+
+template <typename S>
+__global__ void k_mur(S* V, const unsigned* __restrict__ step, MurDev m, Dims d, Scales<typename StoreTraits<S>::A> sc, int phase) {
+    typedef StoreTraits<S> T;
+    typedef typename T::A A;
+    const long long plane = (long long)m.na * m.nb;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= plane || *step < m.start_ts) return;
+    const int a = (int)(i / m.nb), b = (int)(i - (long long)a * m.nb);
+    int pos[3];
+    pos[m.ny] = m.line;
+    pos[m.nyP] = a;
+    pos[m.nyPP] = b;
+    const long long p = pos[0] * d.sx + pos[1] * d.sy + pos[2];
+    const long long ps = p + (long long)(m.shift - m.line) * m.sny;
+    const A* k = (const A*)m.k;
+    A* buf = (A*)m.buf;
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int c = t == 0 ? m.nyP : m.nyPP;
+        const long long j = t * plane + i;
+        S* F = V + c * d.sc;
+        if (phase == 0) {
+            const A kv = k[j] * T::load(F + p, sc.v[c]);
+            buf[j] = T::load(F + ps, sc.v[c]) - kv;
+        } else if (phase == 1) {
+            const A kv = k[j] * T::load(F + ps, sc.v[c]);
+            buf[j] = buf[j] + kv;
+        } else {
+            T::store(F + p, buf[j], sc.iv[c]);
+        }
+    }
+}
+
 struct AdeDev {
     int count;
     const long long* off;          // [count] element offset of position (component 0)

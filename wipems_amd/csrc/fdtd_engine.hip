@@ -75,7 +75,7 @@ void invalidate_graph(fdtd_engine* h) {
 // and with few soft excitation edges (kept in LDS)
 bool fused_ok(const fdtd_engine* h) {
     if (h->variant != 4) return false;
-    if (h->pml.count || !h->ade.empty() || h->tfsf_v.count || h->tfsf_c.count) return false;
+    if (h->pml.count || !h->ade.empty() || !h->mur.empty() || h->tfsf_v.count || h->tfsf_c.count) return false;
     if (h->has_exc && (h->exc_v.count > FUSED_MAX_EXC || h->exc_c.count > FUSED_MAX_EXC)) return false;
     return h->N[0] >= 2 && h->N[1] >= 2 && h->N[2] >= 2;
 }
@@ -580,6 +580,19 @@ int launch_step_kernels(fdtd_engine* h, bool timed) {
         ad.volt_lor_on = (a->flags & 4) != 0; ad.curr_lor_on = (a->flags & 8) != 0;
         return ad;
     };
+    // Mur ABC phases: pre in ascending face order, post and apply in
+    // descending order (the reference's extension order: equal priorities,
+    // faces created x-lo .. z-hi, list reversed), so on shared edges the
+    // lower face's value is the one that stays
+    auto mur_phase = [&](int phase) {
+        const int n = (int)h->mur.size();
+        for (int q = 0; q < n; ++q) {
+            const MurHost* m = h->mur[phase == 0 ? q : n - 1 - q];
+            const long long plane = (long long)m->d.na * m->d.nb;
+            hipLaunchKernelGGL((k_mur<S>), dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, h->stream, h->V.as<S>(), stp, m->d, h->d, sc, phase);
+        }
+    };
+    mur_phase(0);
     for (auto a : h->ade)
         if ((a->flags & 1) && a->count)
             hipLaunchKernelGGL((k_ade_pre<S>), dim3((a->count + 255) / 256), dim3(256), 0, h->stream, h->V.as<S>(), ade_dev(a), h->d.sc, sc, 0);
@@ -593,9 +606,11 @@ int launch_step_kernels(fdtd_engine* h, bool timed) {
     if (timed) HIPCHK(hipEventRecord(h->ev[evi + 1], h->stream));
     if (h->tfsf_v.count)
         launch_tfsf<S>(h, h->V.as<S>(), stp, h->tfsf_v, sc, 0);
+    mur_phase(1);
     for (auto a : h->ade)
         if ((a->flags & 1) && a->count)
             hipLaunchKernelGGL((k_ade_apply<S>), dim3((a->count + 255) / 256), dim3(256), 0, h->stream, h->V.as<S>(), ade_dev(a), h->d.sc, sc, 0);
+    mur_phase(2);
     if (h->has_exc && h->exc_v.count)
         hipLaunchKernelGGL((k_excite<S>), dim3(std::min(1024, (h->exc_v.count + 255) / 256)), dim3(256), 0, h->stream, h->V.as<S>(), stp, h->exc_v, sc, 0);
 
@@ -1052,6 +1067,46 @@ int fdtd_add_ade(fdtd_engine* h, uint32_t count, const uint32_t* pos, int dtype,
         return fail(FDTD_ERR_HIP, "ADE upload failed");
     }
     h->ade.push_back(a);
+    invalidate_graph(h);
+    return FDTD_OK;
+}
+
+int fdtd_add_mur(fdtd_engine* h, int face, int dtype, const void* k_nyP, const void* k_nyPP, uint32_t start_ts) {
+    if (!h || !k_nyP || !k_nyPP) return fail(FDTD_ERR_ARG, "null argument");
+    if (h->numTS) return fail(FDTD_ERR_STATE, "Mur faces must be added before the first timestep");
+    if (face < 0 || face > 5) return fail(FDTD_ERR_RANGE, "face must be 0..5 (2*axis + top)");
+    for (auto m : h->mur)
+        if (m->d.ny * 2 + (m->d.line > 0) == face) return fail(FDTD_ERR_ARG, "Mur face %d added twice", face);
+    const int ny = face / 2;
+    if (h->N[ny] < 2) return fail(FDTD_ERR_RANGE, "Mur face on an axis with fewer than two lines");
+    MurHost* m = new MurHost();
+    MurDev& d = m->d;
+    d.ny = ny; d.nyP = (ny + 1) % 3; d.nyPP = (ny + 2) % 3;
+    d.line = (face % 2) ? (int)h->N[ny] - 1 : 0;
+    d.shift = (face % 2) ? (int)h->N[ny] - 2 : 1;
+    d.na = (int)h->N[d.nyP];
+    d.nb = (int)h->N[d.nyPP];
+    d.start_ts = start_ts;
+    d.sny = ny == 0 ? h->d.sx : (ny == 1 ? h->d.sy : 1);
+    const size_t plane = (size_t)d.na * d.nb, ab = h->acc;
+    int rc;
+    std::vector<char> all(2 * plane * ab), tmp;
+    const void* src[2] = {k_nyP, k_nyPP};
+    for (int t = 0; t < 2; ++t) {
+        if ((rc = convert_host(h, dtype, src[t], plane, tmp))) { delete m; return rc; }
+        memcpy(all.data() + t * plane * ab, tmp.data(), plane * ab);
+    }
+    if ((rc = m->k.alloc(all.size())) || (rc = m->buf.alloc(all.size()))) { delete m; return rc; }
+    if (hipMemcpyAsync(m->k.p, all.data(), all.size(), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+        hipMemsetAsync(m->buf.p, 0, m->buf.bytes, h->stream) != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess) {
+        delete m;
+        return fail(FDTD_ERR_HIP, "Mur upload failed");
+    }
+    d.k = m->k.p;
+    d.buf = m->buf.p;
+    auto it = h->mur.begin();
+    while (it != h->mur.end() && (*it)->d.ny * 2 + ((*it)->d.line > 0) < face) ++it;
+    h->mur.insert(it, m);
     invalidate_graph(h);
     return FDTD_OK;
 }

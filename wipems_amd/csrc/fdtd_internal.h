@@ -73,11 +73,18 @@ struct AdeHost {
     DevBuf off, coef, state;   // coef: 6 lists [3][count]; state: 4 lists [3][count]
 };
 
+// one Mur ABC face: coefficients and the saved boundary values, [2][na][nb]
+struct MurHost {
+    MurDev d{};
+    DevBuf k, buf;
+};
+
 } // namespace wipems
 
 using wipems::DevBuf;
 using wipems::ProbeHost;
 using wipems::AdeHost;
+using wipems::MurHost;
 using wipems::DftHost;
 
 struct fdtd_engine {
@@ -130,6 +137,9 @@ struct fdtd_engine {
     // ADE orders
     std::vector<AdeHost*> ade;
 
+    // Mur ABC faces, kept in ascending face order
+    std::vector<MurHost*> mur;
+
     // TFSF
     TfsfDev tfsf_v{}, tfsf_c{};
     DevBuf tfsf_buf_v, tfsf_buf_c;
@@ -180,6 +190,7 @@ struct fdtd_engine {
         for (auto e : ev) (void)hipEventDestroy(e);
         for (auto p : probes) delete p;
         for (auto a : ade) delete a;
+        for (auto m : mur) delete m;
         for (auto b : dft) delete b;
         if (stream) (void)hipStreamDestroy(stream);
     }

@@ -136,6 +136,11 @@ class Engine:
         code = _dtype_code(lists[0])
         self._chk(self._lib.fdtd_add_ade(self._h, pos.shape[1], _up(pos), code, *[_vp(a) for a in lists], int(order.flags)))
 
+    def add_mur(self, m):
+        kp = np.ascontiguousarray(m.k_nyP)
+        kpp = np.ascontiguousarray(m.k_nyPP, dtype=kp.dtype)
+        self._chk(self._lib.fdtd_add_mur(self._h, int(m.face), _dtype_code(kp), _vp(kp), _vp(kpp), int(m.start_ts)))
+
     def set_tfsf(self, which, pos, comp, delay, delta, amp, max_delay):
         pos = np.ascontiguousarray(pos, dtype=np.uint32)
         comp = np.ascontiguousarray(comp, dtype=np.uint32)
@@ -339,6 +344,8 @@ def engine_from_operator(data: OperatorData, precision="fp32", compressed=None, 
         eng.add_upml(s)
     for a in data.ade:
         eng.add_ade(a)
+    for m in data.mur:
+        eng.add_mur(m)
     if data.tfsf:
         for which in (0, 1):
             t = data.tfsf[which]

@@ -140,6 +140,18 @@ struct wop_operator {
     };
     std::vector<Upml> upml;
 
+    // Mur ABC faces (face = 2*ny + top): first-order coefficients per
+    // (nyP, nyPP) position of the boundary plane, and the step from which
+    // the boundary is active
+    double mur_vph[6] = {0, 0, 0, 0, 0, 0};
+    struct Mur {
+        int face = 0;
+        uint32_t na = 0, nb = 0;
+        std::vector<double> kP, kPP;       // [na][nb]
+        uint32_t start_ts = 0;
+    };
+    std::vector<Mur> mur;
+
     struct Ade {
         std::vector<uint32_t> pos[3];
         std::vector<double> lists[6];      // v_int, v_ext, v_lor, i_int, i_ext, i_lor [3][count]
@@ -1362,6 +1374,62 @@ struct Builder {
     }
 
     // Excitation::CalcGaussianPulsExcitation etc. (FDTD/excitation.cpp:151-277)
+    // Operator_Ext_Mur_ABC::SetDirection/BuildExtension (operator_ext_mur_abc.cpp:88-180)
+    // and the excitation start rule of Engine_Ext_Mur_ABC (engine_ext_mur_abc.cpp:42-58).
+    // Needs the excitation lists and the signal length (call after Signal).
+    void Mur() {
+        op->mur.clear();
+        const uint32_t* N = op->N;
+        const double dT = op->dT;
+        for (int face = 0; face < 6; ++face) {
+            if (op->BC[face] != WOP_BC_MUR) continue;
+            const int ny = face / 2, nyP = (ny + 1) % 3, nyPP = (ny + 2) % 3;
+            const bool top = face % 2;
+            const uint32_t line = top ? N[ny] - 1 : 0;
+            wop_operator::Mur m;
+            m.face = face;
+            m.na = N[nyP];
+            m.nb = N[nyPP];
+            m.kP.assign((size_t)m.na * m.nb, 0.0);
+            m.kPP.assign((size_t)m.na * m.nb, 0.0);
+            uint32_t pos[3] = {0, 0, 0};
+            pos[ny] = line;
+            const double delta = std::fabs(op->EdgeLength(ny, pos));
+            double coord[3];
+            for (int n = 0; n < 3; ++n) coord[n] = op->DiscLine(n, pos[n], false);
+            if (line == 0) coord[ny] = op->DiscLine(ny, pos[ny], false) + delta / 2 / op->gridDelta;
+            else coord[ny] = op->DiscLine(ny, pos[ny], false) - delta / 2 / op->gridDelta;
+            const double vph = op->mur_vph[face];
+            for (uint32_t a = 0; a < m.na; ++a) {
+                coord[nyP] = op->DiscLine(nyP, a, false);
+                for (uint32_t b = 0; b < m.nb; ++b) {
+                    coord[nyPP] = op->DiscLine(nyPP, b, false);
+                    const int p = op->PropertyAt(coord, P_MATERIAL);
+                    const size_t i = (size_t)a * m.nb + b;
+                    double c0t;
+                    if (p >= 0) {
+                        const Property& P = op->props[p];
+                        c0t = vph > 0.0 ? vph * dT : C0 * dT / sqrt(P.eps * P.mue);
+                        m.kP[i] = (double)(F)((c0t - delta) / (c0t + delta));
+                        c0t = vph > 0.0 ? vph * dT : C0 * dT / sqrt(P.eps * P.mue);
+                        m.kPP[i] = (double)(F)((c0t - delta) / (c0t + delta));
+                    } else {
+                        // background material eps_r = mue_r = 1
+                        c0t = vph > 0.0 ? vph * dT : C0 / sqrt(1.0 * 1.0) * dT;
+                        m.kP[i] = (double)(F)((c0t - delta) / (c0t + delta));
+                        m.kPP[i] = m.kP[i];
+                    }
+                }
+            }
+            int max_delay = -1;
+            for (const auto& e : op->exc_v)
+                if (((int)e.dir == nyP || (int)e.dir == nyPP) && e.pos[ny] == line && (int)e.delay > max_delay)
+                    max_delay = (int)e.delay;
+            m.start_ts = max_delay >= 0 ? (uint32_t)max_delay + op->exc_length + 10 : 0;
+            op->mur.push_back(std::move(m));
+        }
+    }
+
     void Signal(uint32_t nr_ts) {
         const double dT = op->dT;
         std::vector<F> sv, si;
@@ -1459,6 +1527,7 @@ int wop_operator::build(uint32_t nr_ts) {
     b.Upml();
     b.Lorentz();
     b.Signal(nr_ts);
+    b.Mur();
     built = true;
     return 0;
 }
@@ -1524,9 +1593,37 @@ int wop_set_boundary(wop_operator* op, const int bc[6], const uint32_t pml_size[
     if (!op || !bc) return fail(-1, "bad argument");
     for (int k = 0; k < 6; ++k) {
         if (bc[k] < -1 || bc[k] > 3) return fail(-1, "bad boundary code %d", bc[k]);
-        if (bc[k] == WOP_BC_MUR) return fail(-1, "Mur ABC is not supported by this engine");
         op->BC[k] = bc[k];
         op->pml_size[k] = (bc[k] == WOP_BC_PML && pml_size) ? pml_size[k] : 0;
+    }
+    return 0;
+}
+
+int wop_set_mur_phase_velocity(wop_operator* op, int face, double v) {
+    if (!op || face < 0 || face > 5 || v < 0) return fail(-1, "bad argument");
+    op->mur_vph[face] = v;
+    return 0;
+}
+
+int wop_get_num_mur(wop_operator* op, uint32_t* n) {
+    if (!op || !n || !op->built) return fail(-1, "operator not built");
+    *n = (uint32_t)op->mur.size();
+    return 0;
+}
+
+int wop_get_mur(wop_operator* op, uint32_t k, int* face, int dtype, void* k_nyP, void* k_nyPP, uint32_t* start_ts) {
+    if (!op || !op->built || k >= op->mur.size() || !face || !start_ts) return fail(-1, "bad argument");
+    const wop_operator::Mur& m = op->mur[k];
+    *face = m.face;
+    *start_ts = m.start_ts;
+    const std::vector<double>* src[2] = {&m.kP, &m.kPP};
+    void* dst[2] = {k_nyP, k_nyPP};
+    for (int j = 0; j < 2; ++j) {
+        if (!dst[j]) continue;
+        for (size_t i = 0; i < src[j]->size(); ++i) {
+            if (dtype == 1) ((double*)dst[j])[i] = (*src[j])[i];
+            else ((float*)dst[j])[i] = (float)(*src[j])[i];
+        }
     }
     return 0;
 }
@@ -1994,6 +2091,7 @@ int stream_build(wop_operator* op, uint32_t nr_ts, uint32_t* index, uint32_t max
     std::stable_sort(op->exc_v.begin(), op->exc_v.end(), zyx);
     std::stable_sort(op->exc_i.begin(), op->exc_i.end(), zyx);
     b.Signal(nr_ts);
+    b.Mur();
     b.EC[0][0].clear();
     for (int k = 0; k < 4; ++k)
         for (int d = 0; d < 3; ++d) { std::vector<float>().swap(b.EC[k][d]); std::vector<float>().swap(b.co[k][d]); }

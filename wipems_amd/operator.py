@@ -43,6 +43,17 @@ class UpmlSlab:
 
 
 @dataclass
+class MurFace:
+    """One Mur ABC face (Operator_Ext_Mur_ABC): face = 2*axis + top, first-order
+    coefficients of the two tangential components over the boundary plane
+    [N(axis+1)][N(axis+2)], and the timestep from which it acts."""
+    face: int
+    k_nyP: np.ndarray
+    k_nyPP: np.ndarray
+    start_ts: int
+
+
+@dataclass
 class AdeOrder:
     pos: np.ndarray                # uint32 [3][count]
     lists: List[np.ndarray]        # v_int, v_ext, v_lor, i_int, i_ext, i_lor, each [3][count]
@@ -131,6 +142,7 @@ class OperatorData:
     edge_primary: Optional[np.ndarray] = None   # [Nx+Ny+Nz] primary edge lengths (m)
     edge_dual: Optional[np.ndarray] = None      # [Nx+Ny+Nz] dual edge lengths (m)
     dft_boxes: List[DftBoxDef] = field(default_factory=list)
+    mur: List[MurFace] = field(default_factory=list)
 
     @property
     def cells(self):
@@ -360,6 +372,7 @@ class Operator:
                 self._chk(lib.wop_get_tfsf(h, which, _up(t["pos"]), _up(t["comp"]), _up(t["delay"]),
                                            _dp(t["delta"]), _dp(t["amp"])))
                 tfsf[which] = t
+        mur = self._mur_faces(n, prec)
         self._built_prec = prec
         tot = int(n[0]) + int(n[1]) + int(n[2])
         edges = []
@@ -369,7 +382,28 @@ class Operator:
             edges.append(ebuf)
         self._edges = edges
         return OperatorData(tuple(int(x) for x in n), dT.value, prec, *arrs, exc, slabs, ades, nyq.value, mstep.value,
-                            tfsf=tfsf, source=self, edge_primary=edges[0], edge_dual=edges[1])
+                            tfsf=tfsf, source=self, edge_primary=edges[0], edge_dual=edges[1], mur=mur)
+
+    def _mur_faces(self, n, prec):
+        lib, h = self._lib, self._h
+        cnt = c_uint32()
+        self._chk(lib.wop_get_num_mur(h, byref(cnt)))
+        faces = []
+        for k in range(cnt.value):
+            face, start = c_int(), c_uint32()
+            self._chk(lib.wop_get_mur(h, k, byref(face), 1, None, None, byref(start)))
+            ny = face.value // 2
+            shape = (int(n[(ny + 1) % 3]), int(n[(ny + 2) % 3]))
+            kp = np.empty(shape, np.float64)
+            kpp = np.empty(shape, np.float64)
+            self._chk(lib.wop_get_mur(h, k, byref(face), 1, _vp(kp), _vp(kpp), byref(start)))
+            dt = np.float64 if prec == 1 else np.float32
+            faces.append(MurFace(face.value, kp.astype(dt), kpp.astype(dt), start.value))
+        return faces
+
+    def set_mur_phase_velocity(self, face, v_phase):
+        """Phase velocity of one Mur face (face = 2*axis + top); 0 = the material's own."""
+        self._chk(self._lib.wop_set_mur_phase_velocity(self._h, int(face), float(v_phase)))
 
     def compress(self, max_entries=0):
         """Native per-cell compression of the last build (wop_compress); None if
