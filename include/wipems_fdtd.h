@@ -164,6 +164,13 @@ int  fdtd_add_ade(fdtd_engine* h, uint32_t count, const uint32_t* pos, int dtype
  * reference's extension order: pre before the ADE pre hook, post after the
  * TFSF injection, apply after the ADE apply and before the excitation. */
 int  fdtd_add_mur(fdtd_engine* h, int face, int dtype, const void* k_nyP, const void* k_nyPP, uint32_t start_ts);
+/* Lumped RLC cells (Engine_Ext_LumpedRLC, engine_ext_lumpedRLC.cpp:81-149;
+ * lists of Operator_Ext_LumpedRLC, operator_ext_lumpedRLC.cpp:110-452):
+ * pos [3][count], dir [count], coef [9][count] = ilv, i2v, vv2, vj1, vj2,
+ * vvd, ib0, b1, b2 (wop_get_rlc).  Replaces any earlier list; count 0
+ * clears it.  Each edge may appear once.  Pre hook after the Lorentz pre
+ * hook, apply before the Lorentz apply. */
+int  fdtd_set_rlc(fdtd_engine* h, uint32_t count, const uint32_t* pos, const uint32_t* dir, int dtype, const void* coef);
 /* Total-field/scattered-field injection list.  which = FDTD_VOLT (added after
  * the voltage update using the current signal) or FDTD_CURR (added after the
  * current update using the voltage signal).  Each entry adds

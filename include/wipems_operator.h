@@ -80,6 +80,10 @@ int  wop_add_lorentz_material(wop_operator* op, double eps_r, double kappa, doub
 /* lumped parallel RC (R or C <= 0 means not given), direction ny, metal caps */
 int  wop_add_lumped_element(wop_operator* op, int ny, double R, double C, int caps, int* id);
 /* excitation property: type 0 soft E, 1 hard E, 2 soft H, 3 hard H; value vector; delay (s) */
+/* lumped RLC element (CSPropLumpedElement with an inductance or of series
+ * type; built by Operator_Ext_LumpedRLC, operator_ext_lumpedRLC.cpp:110-452):
+ * type 0 parallel, 1 series; R, L, C < 0 = unset */
+int  wop_add_lumped_rlc(wop_operator* op, int ny, int type, double R, double L, double C, int caps, int* id);
 int  wop_add_excitation(wop_operator* op, int type, const double value[3], double delay, int* id);
 int  wop_add_box(wop_operator* op, int prop, const double start[3], const double stop[3], int priority);
 /* plane-wave (TFSF) excitation on the faces of a box (CSPropExcitation type 10
@@ -140,6 +144,11 @@ int  wop_get_tfsf(wop_operator* op, int which, uint32_t* pos, uint32_t* comp, ui
  * engine_ext_mur_abc.cpp:42-58), for fdtd_add_mur */
 int  wop_get_num_mur(wop_operator* op, uint32_t* n);
 int  wop_get_mur(wop_operator* op, uint32_t k, int* face, int dtype, void* k_nyP, void* k_nyPP, uint32_t* start_ts);
+/* lumped RLC cells after a build: pos [3][count], dir [count], coef [9][count]
+ * = ilv, i2v, vv2, vj1, vj2, vvd, ib0, b1, b2 (Operator_Ext_LumpedRLC's lists),
+ * for fdtd_set_rlc */
+int  wop_get_rlc_count(wop_operator* op, uint32_t* count);
+int  wop_get_rlc(wop_operator* op, uint32_t* pos, uint32_t* dir, int dtype, void* coef);
 int  wop_get_num_upml(wop_operator* op, uint32_t* n);
 int  wop_get_upml_range(wop_operator* op, uint32_t k, uint32_t start[3], uint32_t num_lines[3]);
 int  wop_get_upml(wop_operator* op, uint32_t k, int dtype, void* vv, void* vvfo, void* vvfn, void* ii, void* iifo, void* iifn);

@@ -217,6 +217,27 @@ int orc_add_mur(orc* o, int face, int dtype, const void* k_nyP, const void* k_ny
     return 0;
 }
 
+int orc_set_rlc(orc* o, uint32_t count, const uint32_t* pos, const uint32_t* dir, int dtype, const void* coef) {
+#define SETR(F, T)                                                                     \
+    {                                                                                  \
+        F.n_rlc = count;                                                               \
+        F.rlc_pos = copy_u32(pos, 3 * (size_t)count);                                  \
+        F.rlc_dir = copy_u32(dir, count);                                              \
+        T* all = (T*)copy_as(o, dtype, coef, 9 * (size_t)count);                       \
+        for (int k = 0; k < 9; ++k) {                                                  \
+            F.rlc_c[k] = (T*)malloc(count * sizeof(T));                                \
+            memcpy(F.rlc_c[k], all + (size_t)k * count, count * sizeof(T));            \
+        }                                                                              \
+        free(all);                                                                     \
+        F.rlc_il = (T*)zeros(o, count);                                                \
+        for (int k = 0; k < 3; ++k) { F.rlc_vd[k] = (T*)zeros(o, count); F.rlc_j[k] = (T*)zeros(o, count); } \
+    }
+    if (!count) return 0;
+    if (o->dbl) SETR(o->f64, double) else SETR(o->f32, float)
+#undef SETR
+    return 0;
+}
+
 int orc_set_tfsf(orc* o, int which, uint32_t count, const uint32_t* pos, const uint32_t* comp, const uint32_t* delay,
                  const double* delta, const double* amp, uint32_t max_delay) {
 #define SETT(F, T)                                                         \

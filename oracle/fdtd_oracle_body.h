@@ -44,6 +44,13 @@ typedef struct {
     uint32_t mur_start[6];
     REAL* mur_k[6];                   /* [2][na][nb]: nyP, nyPP */
     REAL* mur_buf[6];
+    /* lumped RLC cells */
+    uint32_t n_rlc;
+    uint32_t *rlc_pos, *rlc_dir;      /* [3][n], [n] */
+    REAL* rlc_c[9];                   /* ilv, i2v, vv2, vj1, vj2, vvd, ib0, b1, b2 */
+    REAL* rlc_il;
+    REAL* rlc_vd[3];                  /* rotated once per step, as the reference rotates its pointers */
+    REAL* rlc_j[3];
 } FN(fields_t);
 
 #define IDX(x, y, z) (((size_t)(x) * N1 + (y)) * N2 + (z))
@@ -275,17 +282,49 @@ static void FN(mur)(FN(fields_t) * f, const uint32_t* Nl, uint32_t numTS, int ph
     }
 }
 
+/* Engine_Ext_LumpedRLC::DoPreVoltageUpdates / Apply2VoltagesImpl
+ * (engine_ext_lumpedRLC.cpp:81-149) */
+static void FN(rlc_pre)(FN(fields_t) * f) {
+    if (!f->n_rlc) return;
+    REAL* t = f->rlc_vd[2];
+    f->rlc_vd[2] = f->rlc_vd[1];
+    f->rlc_vd[1] = f->rlc_vd[0];
+    f->rlc_vd[0] = t;
+    for (uint32_t i = 0; i < f->n_rlc; ++i) f->rlc_il[i] += (f->rlc_c[1][i]) * (f->rlc_c[0][i]) * f->rlc_vd[1][i];
+}
+
+static void FN(rlc_apply)(FN(fields_t) * f, const uint32_t* Nl) {
+    if (!f->n_rlc) return;
+    const size_t N1 = Nl[1], N2 = Nl[2], nd = (size_t)Nl[0] * N1 * N2, n = f->n_rlc;
+    REAL* t = f->rlc_j[2];
+    f->rlc_j[2] = f->rlc_j[1];
+    f->rlc_j[1] = f->rlc_j[0];
+    f->rlc_j[0] = t;
+    REAL **vd = f->rlc_vd, **j = f->rlc_j, **c = f->rlc_c;
+    const uint32_t* p = f->rlc_pos;
+    for (size_t i = 0; i < n; ++i)
+        vd[0][i] = f->V[f->rlc_dir[i] * nd + IDX(p[i], p[n + i], p[2 * n + i])];
+    for (size_t i = 0; i < n; ++i) {
+        vd[0][i] = (c[5][i]) * (vd[0][i] - f->rlc_il[i] + (c[2][i]) * vd[2][i] + (c[3][i]) * j[1][i] + (c[4][i]) * j[2][i]);
+        j[0][i] = (c[6][i]) * (vd[0][i] - vd[2][i]) - ((c[7][i]) * (c[6][i])) * j[1][i] - ((c[8][i]) * (c[6][i])) * j[2][i];
+    }
+    for (size_t i = 0; i < n; ++i)
+        f->V[f->rlc_dir[i] * nd + IDX(p[i], p[n + i], p[2 * n + i])] = vd[0][i];
+}
+
 /* One timestep in the order of Engine::IterateTS (FDTD/engine.cpp:255-274) with
- * the extensions sorted by priority: pre hooks Excitation, Mur, Lorentz, TFSF,
- * UPML; post and apply hooks UPML, TFSF, Lorentz, Mur, Excitation. */
+ * the extensions sorted by priority: pre hooks Excitation, Mur, Lorentz, RLC,
+ * TFSF, UPML; post and apply hooks UPML, TFSF, RLC, Lorentz, Mur, Excitation. */
 static void FN(step)(FN(fields_t) * f, const uint32_t* Nl, uint32_t numTS) {
     FN(mur)(f, Nl, numTS, 0);
     FN(ade_pre)(f, Nl, 0);
+    FN(rlc_pre)(f);
     FN(upml_pre)(f, Nl, 0);
     FN(update_voltages)(f, Nl);
     FN(upml_post)(f, Nl, 0);
     FN(tfsf)(f, Nl, 0, numTS);
     FN(mur)(f, Nl, numTS, 1);
+    FN(rlc_apply)(f, Nl);
     FN(ade_apply)(f, Nl, 0);
     FN(mur)(f, Nl, numTS, 2);
     FN(excite)(f, Nl, 0, numTS);

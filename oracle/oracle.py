@@ -34,6 +34,7 @@ def lib():
         L.orc_add_upml.argtypes = [c_void_p, POINTER(c_uint32), POINTER(c_uint32), c_int] + [c_void_p] * 6
         L.orc_add_ade.argtypes = [c_void_p, c_uint32, POINTER(c_uint32), c_int] + [c_void_p] * 6 + [c_uint32]
         L.orc_add_mur.argtypes = [c_void_p, c_int, c_int, c_void_p, c_void_p, c_uint32]
+        L.orc_set_rlc.argtypes = [c_void_p, c_uint32, POINTER(c_uint32), POINTER(c_uint32), c_int, c_void_p]
         L.orc_set_tfsf.argtypes = [c_void_p, c_int, c_uint32, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_uint32),
                                    POINTER(c_double), POINTER(c_double), c_uint32]
         L.orc_add_probe.argtypes = [c_void_p, c_int, POINTER(c_uint32), POINTER(c_uint32), c_int, c_uint32, c_double,
@@ -115,6 +116,13 @@ class Oracle:
         code = 1 if lists[0].dtype == np.float64 else 0
         self._L.orc_add_ade(self._h, pos.shape[1], _up(pos), code, *[_vp(x) for x in lists], int(a.flags))
 
+    def set_rlc(self, r):
+        pos = np.ascontiguousarray(r.pos, dtype=np.uint32)
+        d = np.ascontiguousarray(r.dir, dtype=np.uint32)
+        coef = np.ascontiguousarray(r.coef)
+        code = 1 if coef.dtype == np.float64 else 0
+        self._L.orc_set_rlc(self._h, len(d), _up(pos), _up(d), code, _vp(coef))
+
     def add_mur(self, m):
         kp = np.ascontiguousarray(m.k_nyP)
         kpp = np.ascontiguousarray(m.k_nyPP, dtype=kp.dtype)
@@ -185,6 +193,8 @@ def oracle_from_operator(data, use_double=None, ftz=True):
         o.add_ade(a)
     for m in getattr(data, "mur", []):
         o.add_mur(m)
+    if getattr(data, "rlc", None) is not None:
+        o.set_rlc(data.rlc)
     if data.tfsf:
         for which in (0, 1):
             t = data.tfsf[which]

@@ -440,3 +440,41 @@ def test_mur_abc_fp16_close_to_fp32(gpu):
         out.append(eng.read_probe(eng.probe_ids[0])[1])
         eng.close()
     assert rel_l2(out[1], out[0]) < 1e-2
+
+
+def _rlc_case(prec, bc=(0,) * 6, steps=300):
+    from wipems_amd.operator import Operator
+    rng = np.random.default_rng(41)
+    n = (24, 22, 26)
+    lines = [np.concatenate([[0.0], np.cumsum(rng.uniform(0.6, 1.5, k - 1))]) for k in n]
+    op = Operator()
+    op.set_mesh(lines, 1e-3)
+    op.set_boundary(list(bc), [5] * 6)
+    op.set_gauss_excite(4e9, 4e9)
+    c = [lines[k][n[k] // 2] for k in range(3)]
+    e = op.add_excitation(0, (0.0, 0.0, 1.0))
+    op.add_box(e, (c[0], c[1], lines[2][9]), (c[0], c[1], lines[2][11]), priority=5)
+    par = op.add_lumped_rlc(2, 0, R=0.0, L=2e-9, C=0.5e-12, caps=False)
+    op.add_box(par, (lines[0][8], lines[1][8], lines[2][12]), (lines[0][8], lines[1][8], lines[2][14]), priority=6)
+    ser = op.add_lumped_rlc(0, 1, R=25.0, L=1e-9, C=1e-12, caps=True)
+    op.add_box(ser, (lines[0][14], lines[1][13], lines[2][10]), (lines[0][16], lines[1][14], lines[2][11]), priority=6)
+    d = op.build(prec, nr_ts=steps)
+    d.probes = [op.voltage_probe((lines[0][8], lines[1][8], lines[2][12]), (lines[0][8], lines[1][8], lines[2][14]),
+                                 td_interval=1, name="u_par")]
+    return d
+
+
+@pytest.mark.parametrize("precision,prec", [("fp32", 0), ("fp64", 1)])
+@pytest.mark.parametrize("bc", [(0,) * 6, (2, 2, 3, 3, 2, 1)])
+def test_lumped_rlc_bitwise(gpu, precision, prec, bc):
+    d = _rlc_case(prec, bc)
+    assert d.rlc is not None and len(d.rlc.dir) == 2 + 2 * 2 * 2
+    print("case: lumped RLC, bc %s, %s" % (bc, precision), flush=True)
+    eng = engine_from_operator(d, precision=precision)
+    _kinds[id(eng)] = [p.kind for p in d.probes]
+    orc = oracle_from_operator(d, ftz=True)
+    for chunk in (1, 2, 297):
+        eng.iterate(chunk)
+        orc.iterate(chunk)
+    compare(eng, orc)
+    assert np.abs(eng.read_probe(eng.probe_ids[0])[1]).max() > 0

@@ -78,6 +78,11 @@ def test_rejects_what_it_cannot_stream_and_table_limit():
     sc = scenarios.graded_port_scenario(n_cells=20, steps=10, prec=0, n_boxes=3, pml=4)
     with pytest.raises(Exception):
         sc.op.build_streamed(nr_ts=10, max_entries=10)
+    r = sc.op.add_lumped_rlc(2, 1, R=10.0, L=1e-9, C=1e-12)
+    sc.op.add_box(r, (sc.op.lines[0][5], sc.op.lines[1][5], sc.op.lines[2][5]),
+                  (sc.op.lines[0][5], sc.op.lines[1][5], sc.op.lines[2][6]), priority=9)
+    with pytest.raises(Exception, match="lumped RLC"):
+        sc.op.build_streamed(nr_ts=10)
 
 
 def test_many_boxes_dipole_and_plane_rejection():

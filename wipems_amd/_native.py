@@ -63,6 +63,7 @@ _ENGINE_SIGS = {
     "fdtd_pml_compressed": [c_void_p, POINTER(c_int)],
     "fdtd_set_pml_split": [c_void_p, c_int],
     "fdtd_add_mur": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_uint32],
+    "fdtd_set_rlc": [c_void_p, c_uint32, POINTER(c_uint32), POINTER(c_uint32), c_int, c_void_p],
     "fdtd_pml_split": [c_void_p, POINTER(c_int)],
     "fdtd_set_tuning": [c_void_p, c_int, c_int, c_int],
     "fdtd_set_kernel_timing": [c_void_p, c_int],
@@ -123,6 +124,9 @@ _OPERATOR_SIGS = {
                            POINTER(c_uint32), POINTER(c_uint32), POINTER(c_double), POINTER(c_uint32)],
     "wop_get_num_upml": [c_void_p, POINTER(c_uint32)],
     "wop_get_num_mur": [c_void_p, POINTER(c_uint32)],
+    "wop_add_lumped_rlc": [c_void_p, c_int, c_int, c_double, c_double, c_double, c_int, POINTER(c_int)],
+    "wop_get_rlc_count": [c_void_p, POINTER(c_uint32)],
+    "wop_get_rlc": [c_void_p, POINTER(c_uint32), POINTER(c_uint32), c_int, c_void_p],
     "wop_get_mur": [c_void_p, c_uint32, POINTER(c_int), c_int, c_void_p, c_void_p, POINTER(c_uint32)],
     "wop_set_mur_phase_velocity": [c_void_p, c_int, c_double],
     "wop_get_upml_range": [c_void_p, c_uint32, POINTER(c_uint32), POINTER(c_uint32)],

@@ -985,6 +985,70 @@ __global__ void k_mur(S* V, const unsigned* __restrict__ step, MurDev m, Dims d,
     }
 }
 
+// ---------------------------------------------------------------------------
+// Lumped RLC (Engine_Ext_LumpedRLC, engine_ext_lumpedRLC.cpp:81-149).  Per
+// cell: the parallel inductor current Il, the last three node voltages Vd and
+// the last three series currents J.  The reference rotates the three-slot
+// buffers once per step (Vd in the pre hook, J in the apply hook); here the
+// slot of logical entry k at timestep n is (k - (n + 1)) mod 3.
+//   pre:   Il += (i2v * ilv) * Vd[1]
+//   apply: Vd[0] = V; Vd[0] = vvd * ((((Vd[0] - Il) + vv2*Vd[2]) + vj1*J[1]) + vj2*J[2]);
+//          J[0] = (ib0*(Vd[0] - Vd[2]) - (b1*ib0)*J[1]) - (b2*ib0)*J[2];  V = Vd[0]
+// ---------------------------------------------------------------------------
+struct RlcDev {
+    int count;
+    const long long* off;      // [count] element offset incl. the component
+    const int* comp;           // [count]
+    const void* c;             // [9][count]: ilv, i2v, vv2, vj1, vj2, vvd, ib0, b1, b2
+    void* il;                  // [count]
+    void* vd;                  // [3][count]
+    void* j;                   // [3][count]
+};
+
+__device__ __forceinline__ int rlc_slot(int k, unsigned n) {
+    return (int)((unsigned)(k + 3) - (n + 1u) % 3u) % 3;
+}
+
+template <typename S>
+__global__ void k_rlc(S* V, const unsigned* __restrict__ step, RlcDev r, Scales<typename StoreTraits<S>::A> sc, int apply) {
+    typedef StoreTraits<S> T;
+    typedef typename T::A A;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= r.count) return;
+    const unsigned n = *step;
+    const A* c = (const A*)r.c;
+    A* il = (A*)r.il;
+    A* vd = (A*)r.vd;
+    A* jj = (A*)r.j;
+    const long long cn = r.count;
+    const A* ilv = c; const A* i2v = c + cn; const A* vv2 = c + 2 * cn; const A* vj1 = c + 3 * cn;
+    const A* vj2 = c + 4 * cn; const A* vvd = c + 5 * cn; const A* ib0 = c + 6 * cn; const A* b1 = c + 7 * cn; const A* b2 = c + 8 * cn;
+    A* v0 = vd + rlc_slot(0, n) * cn;
+    A* v1 = vd + rlc_slot(1, n) * cn;
+    A* v2 = vd + rlc_slot(2, n) * cn;
+    if (!apply) {
+        il[i] = il[i] + (i2v[i] * ilv[i]) * v1[i];
+        return;
+    }
+    A* j0 = jj + rlc_slot(0, n) * cn;
+    A* j1 = jj + rlc_slot(1, n) * cn;
+    A* j2 = jj + rlc_slot(2, n) * cn;
+    const int comp = r.comp[i];
+    S* F = V + r.off[i];
+    A x = T::load(F, sc.v[comp]);
+    A t = x - il[i];
+    t = t + vv2[i] * v2[i];
+    t = t + vj1[i] * j1[i];
+    t = t + vj2[i] * j2[i];
+    x = vvd[i] * t;
+    v0[i] = x;
+    A u = ib0[i] * (x - v2[i]);
+    u = u - (b1[i] * ib0[i]) * j1[i];
+    u = u - (b2[i] * ib0[i]) * j2[i];
+    j0[i] = u;
+    T::store(F, x, sc.iv[comp]);
+}
+
 struct AdeDev {
     int count;
     const long long* off;          // [count] element offset of position (component 0)

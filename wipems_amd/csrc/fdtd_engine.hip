@@ -75,7 +75,7 @@ void invalidate_graph(fdtd_engine* h) {
 // and with few soft excitation edges (kept in LDS)
 bool fused_ok(const fdtd_engine* h) {
     if (h->variant != 4) return false;
-    if (h->pml.count || !h->ade.empty() || !h->mur.empty() || h->tfsf_v.count || h->tfsf_c.count) return false;
+    if (h->pml.count || !h->ade.empty() || !h->mur.empty() || h->rlc || h->tfsf_v.count || h->tfsf_c.count) return false;
     if (h->has_exc && (h->exc_v.count > FUSED_MAX_EXC || h->exc_c.count > FUSED_MAX_EXC)) return false;
     return h->N[0] >= 2 && h->N[1] >= 2 && h->N[2] >= 2;
 }
@@ -596,6 +596,11 @@ int launch_step_kernels(fdtd_engine* h, bool timed) {
     for (auto a : h->ade)
         if ((a->flags & 1) && a->count)
             hipLaunchKernelGGL((k_ade_pre<S>), dim3((a->count + 255) / 256), dim3(256), 0, h->stream, h->V.as<S>(), ade_dev(a), h->d.sc, sc, 0);
+    // lumped RLC: pre after the Lorentz pre hook, apply before the Lorentz
+    // apply (the extension is created after Lorentz, so it sits before it in
+    // the engine's list, FDTD/engine.cpp:84-88)
+    if (h->rlc)
+        hipLaunchKernelGGL((k_rlc<S>), dim3((h->rlc->d.count + 255) / 256), dim3(256), 0, h->stream, h->V.as<S>(), stp, h->rlc->d, sc, 0);
     const size_t evi = timed ? h->ev_used : 0;
     if (timed) h->ev_used += 4;
     if (timed) HIPCHK(hipEventRecord(h->ev[evi], h->stream));
@@ -607,6 +612,8 @@ int launch_step_kernels(fdtd_engine* h, bool timed) {
     if (h->tfsf_v.count)
         launch_tfsf<S>(h, h->V.as<S>(), stp, h->tfsf_v, sc, 0);
     mur_phase(1);
+    if (h->rlc)
+        hipLaunchKernelGGL((k_rlc<S>), dim3((h->rlc->d.count + 255) / 256), dim3(256), 0, h->stream, h->V.as<S>(), stp, h->rlc->d, sc, 1);
     for (auto a : h->ade)
         if ((a->flags & 1) && a->count)
             hipLaunchKernelGGL((k_ade_apply<S>), dim3((a->count + 255) / 256), dim3(256), 0, h->stream, h->V.as<S>(), ade_dev(a), h->d.sc, sc, 0);
@@ -1068,6 +1075,52 @@ int fdtd_add_ade(fdtd_engine* h, uint32_t count, const uint32_t* pos, int dtype,
     }
     h->ade.push_back(a);
     invalidate_graph(h);
+    return FDTD_OK;
+}
+
+int fdtd_set_rlc(fdtd_engine* h, uint32_t count, const uint32_t* pos, const uint32_t* dir, int dtype, const void* coef) {
+    if (!h) return fail(FDTD_ERR_ARG, "null handle");
+    if (h->numTS) return fail(FDTD_ERR_STATE, "lumped RLC cells must be set before the first timestep");
+    delete h->rlc;
+    h->rlc = nullptr;
+    invalidate_graph(h);
+    if (!count) return FDTD_OK;
+    if (!pos || !dir || !coef) return fail(FDTD_ERR_ARG, "null argument");
+    std::vector<long long> off(count);
+    std::vector<int> comp(count);
+    for (uint32_t i = 0; i < count; ++i) {
+        for (int n = 0; n < 3; ++n)
+            if (pos[(size_t)n * count + i] >= h->N[n]) return fail(FDTD_ERR_RANGE, "RLC position out of range");
+        if (dir[i] > 2) return fail(FDTD_ERR_RANGE, "RLC direction out of range");
+        comp[i] = (int)dir[i];
+        off[i] = (long long)dir[i] * h->d.sc + cell_off(h, pos[i], pos[count + i], pos[2 * (size_t)count + i]);
+    }
+    std::vector<long long> sorted(off);
+    std::sort(sorted.begin(), sorted.end());
+    if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+        return fail(FDTD_ERR_ARG, "two lumped RLC cells on one edge");
+    RlcHost* r = new RlcHost();
+    const size_t ab = h->acc;
+    std::vector<char> c;
+    int rc;
+    if ((rc = convert_host(h, dtype, coef, (size_t)9 * count, c))) { delete r; return rc; }
+    if ((rc = r->off.alloc(count * sizeof(long long))) || (rc = r->comp.alloc(count * sizeof(int))) ||
+        (rc = r->coef.alloc(c.size())) || (rc = r->state.alloc((size_t)7 * count * ab))) { delete r; return rc; }
+    if (hipMemcpyAsync(r->off.p, off.data(), count * sizeof(long long), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+        hipMemcpyAsync(r->comp.p, comp.data(), count * sizeof(int), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+        hipMemcpyAsync(r->coef.p, c.data(), c.size(), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+        hipMemsetAsync(r->state.p, 0, r->state.bytes, h->stream) != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess) {
+        delete r;
+        return fail(FDTD_ERR_HIP, "RLC upload failed");
+    }
+    r->d.count = (int)count;
+    r->d.off = r->off.as<long long>();
+    r->d.comp = r->comp.as<int>();
+    r->d.c = r->coef.p;
+    r->d.il = r->state.p;
+    r->d.vd = (char*)r->state.p + (size_t)count * ab;
+    r->d.j = (char*)r->state.p + (size_t)4 * count * ab;
+    h->rlc = r;
     return FDTD_OK;
 }
 

@@ -73,6 +73,12 @@ struct AdeHost {
     DevBuf off, coef, state;   // coef: 6 lists [3][count]; state: 4 lists [3][count]
 };
 
+// lumped RLC cells: offsets, components, coefficients [9][count], state
+struct RlcHost {
+    RlcDev d{};
+    DevBuf off, comp, coef, state;   // state: il [count], vd [3][count], j [3][count]
+};
+
 // one Mur ABC face: coefficients and the saved boundary values, [2][na][nb]
 struct MurHost {
     MurDev d{};
@@ -85,6 +91,7 @@ using wipems::DevBuf;
 using wipems::ProbeHost;
 using wipems::AdeHost;
 using wipems::MurHost;
+using wipems::RlcHost;
 using wipems::DftHost;
 
 struct fdtd_engine {
@@ -140,6 +147,9 @@ struct fdtd_engine {
     // Mur ABC faces, kept in ascending face order
     std::vector<MurHost*> mur;
 
+    // lumped RLC cells (one list)
+    RlcHost* rlc = nullptr;
+
     // TFSF
     TfsfDev tfsf_v{}, tfsf_c{};
     DevBuf tfsf_buf_v, tfsf_buf_c;
@@ -191,6 +201,7 @@ struct fdtd_engine {
         for (auto p : probes) delete p;
         for (auto a : ade) delete a;
         for (auto m : mur) delete m;
+        delete rlc;
         for (auto b : dft) delete b;
         if (stream) (void)hipStreamDestroy(stream);
     }

@@ -136,6 +136,12 @@ class Engine:
         code = _dtype_code(lists[0])
         self._chk(self._lib.fdtd_add_ade(self._h, pos.shape[1], _up(pos), code, *[_vp(a) for a in lists], int(order.flags)))
 
+    def set_rlc(self, r):
+        pos = np.ascontiguousarray(r.pos, dtype=np.uint32)
+        d = np.ascontiguousarray(r.dir, dtype=np.uint32)
+        coef = np.ascontiguousarray(r.coef)
+        self._chk(self._lib.fdtd_set_rlc(self._h, len(d), _up(pos), _up(d), _dtype_code(coef), _vp(coef)))
+
     def add_mur(self, m):
         kp = np.ascontiguousarray(m.k_nyP)
         kpp = np.ascontiguousarray(m.k_nyPP, dtype=kp.dtype)
@@ -346,6 +352,8 @@ def engine_from_operator(data: OperatorData, precision="fp32", compressed=None, 
         eng.add_ade(a)
     for m in data.mur:
         eng.add_mur(m)
+    if data.rlc is not None:
+        eng.set_rlc(data.rlc)
     if data.tfsf:
         for which in (0, 1):
             t = data.tfsf[which]

@@ -53,6 +53,8 @@ struct Property {
     double eps_plasma_f = 0, eps_relax_t = 0, eps_lor_f = 0;
     int ny = 0;
     double R = -1, C = -1;
+    double L = -1;            // inductance (lumped RLC); < 0: unset
+    int le_type = 0;          // 0 parallel, 1 series (CSPropLumpedElement::LEtype)
     bool caps = false;
     int exc_type = 0;
     double exc_val[3] = {0, 0, 0};
@@ -151,6 +153,14 @@ struct wop_operator {
         uint32_t start_ts = 0;
     };
     std::vector<Mur> mur;
+
+    // lumped RLC cells (Operator_Ext_LumpedRLC): position, direction and the
+    // coefficient lists ilv, i2v, vv2, vj1, vj2, vvd, ib0, b1, b2
+    struct Rlc {
+        std::vector<uint32_t> pos[3];
+        std::vector<uint32_t> dir;
+        std::vector<double> c[9];
+    } rlc;
 
     struct Ade {
         std::vector<uint32_t> pos[3];
@@ -733,6 +743,8 @@ struct Builder {
         for (const Primitive& pr : op->prims) {
             const Property& P = op->props[pr.prop];
             if (!(P.kind & P_LUMPED) || pr.shape != 0) continue;
+            // only parallel RC here (Operator::IsLEparRC, FDTD/operator.cpp:1714-1724)
+            if (!(P.le_type == 0 && !(P.L > 0.0))) continue;
             double C = P.C;
             if (C <= 0) C = NAN;
             double R = P.R;
@@ -1374,6 +1386,105 @@ struct Builder {
     }
 
     // Excitation::CalcGaussianPulsExcitation etc. (FDTD/excitation.cpp:151-277)
+    // Operator_Ext_LumpedRLC::BuildExtension (operator_ext_lumpedRLC.cpp:110-452):
+    // series RLC and parallel RLC with an inductance.  Each element's cells get
+    // the per-cell C (parallel: C or the cell's own capacitance; series: the
+    // cell's capacitance raised to keep its impedance 20x above the element's
+    // at fMax) and G, their main coefficients are recomputed, and the ADE
+    // coefficient lists are appended in the reference's loop order.  Runs
+    // after UPML and Lorentz (the extension is created last, openems.cpp:1096).
+    void LumpedRLC() {
+        auto& out = op->rlc;
+        for (int k = 0; k < 3; ++k) out.pos[k].clear();
+        out.dir.clear();
+        for (auto& v : out.c) v.clear();
+        const double dT = op->dT;
+        const double fMax = op->f0 + op->fc;           // GetCenterFreq() + GetCutOffFreq()
+        const double ZF = 20.0;                        // LUMPED_RLC_Z_FACT (operator_ext_lumpedRLC.h:29)
+        for (const Primitive& pr : op->prims) {
+            const Property& P = op->props[pr.prop];
+            if (!(P.kind & P_LUMPED) || pr.shape != 0) continue;
+            // IsLElumpedRLC (operator_ext_lumpedRLC.cpp:500-512)
+            const bool parallel = P.le_type == 0;
+            if (!((parallel && P.L > 0.0) || P.le_type == 1)) continue;
+            double C = P.C, R = P.R, L = P.L;
+            if (C < 0) C = NAN;
+            if (R < 0) R = NAN;
+            if (L < 0) L = NAN;
+            const int dir = P.ny;
+            if (dir < 0 || dir > 2) continue;
+            const int d1 = (dir + 1) % 3, d2 = (dir + 2) % 3;
+            uint32_t st[3], sp[3];
+            bool in;
+            for (int n = 0; n < 3; ++n) {
+                st[n] = op->SnapToMeshLine(n, pr.a[n], in, false);
+                sp[n] = op->SnapToMeshLine(n, pr.b[n], in, false);
+            }
+            if (st[dir] == sp[dir]) continue;
+            const unsigned n0 = sp[dir] - st[dir], n1 = sp[d1] - st[d1] + 1, n2 = sp[d2] - st[d2] + 1;
+            const unsigned npar = n1 * n2;
+            const double dL = L * npar / n0, dR = R * npar / n0;
+            double dG = (1 / R) * n0 / npar, dC = C * n0 / npar;
+            const double ib0 = 2.0 * dT * dC / (4.0 * dL * dC + 2.0 * dT * dR * dC + dT * dT),
+                         b1 = (dT * dT - 4.0 * dL * dC) / (dT * dC),
+                         b2 = (4.0 * dL * dC - 2.0 * dT * dR * dC + dT * dT) / (2.0 * dT * dC);
+            if (parallel && R == 0.0) dG = 0.0;
+            uint32_t pos[3];
+            for (pos[dir] = st[dir]; pos[dir] < sp[dir]; ++pos[dir])
+                for (pos[d1] = st[d1]; pos[d1] <= sp[d1]; ++pos[d1])
+                    for (pos[d2] = st[d2]; pos[d2] <= sp[d2]; ++pos[d2]) {
+                        const size_t i = op->idx(pos[0], pos[1], pos[2]);
+                        double c[9];
+                        if (parallel) {
+                            if (dC > 0) EC[0][dir][i] = (F)dC;
+                            else dC = (double)EC[0][dir][i];      // the first cell's own capacitance, kept for the rest
+                            c[1] = (dT / dC) / (1.0 + dT * dG / (2.0 * dC));
+                            if (R >= 0) EC[1][dir][i] = (F)dG;
+                            c[0] = L > 0 ? dT / dL : 0.0;
+                            if (dC == 0) {
+                                double Cd = (double)EC[0][dir][i];
+                                const double Zmin = std::max(dR, 2 * PI_ * fMax * dL);
+                                const double Zcd_min = 1.0 / (2.0 * PI_ * fMax * Cd);
+                                if (Zcd_min < ZF * Zmin) {
+                                    Cd = 1.0 / (2 * PI_ * fMax * Zmin * ZF);
+                                    EC[0][dir][i] = (F)Cd;
+                                }
+                            }
+                            c[2] = 0.0; c[3] = 0.0; c[4] = 0.0; c[5] = 1.0; c[6] = 0.0; c[7] = 0.0; c[8] = 0.0;
+                        } else {
+                            EC[1][dir][i] = (F)0.0;
+                            F Cd = EC[0][dir][i];
+                            const double Zmin = sqrt(pow(dR, 2) + pow(2 * PI_ * fMax * dL - 1.0 / (dC * 2 * PI_ * fMax), 2));
+                            const double Zcd_min = 1.0 / (2.0 * PI_ * fMax * Cd);
+                            if (Zcd_min < ZF * Zmin) {
+                                Cd = (F)(1.0 / (2 * PI_ * fMax * Zmin * ZF));
+                                EC[0][dir][i] = Cd;
+                            }
+                            c[0] = 0.0; c[1] = 0.0;
+                            c[2] = 0.5 * dT * ib0 / Cd;
+                            c[3] = 0.5 * dT * (b1 * ib0 - 1.0) / Cd;
+                            c[4] = 0.5 * dT * b2 * ib0 / Cd;
+                            c[5] = 1.0 / (1.0 + 0.5 * dT * ib0 / Cd);
+                            c[6] = ib0; c[7] = b1; c[8] = b2;
+                        }
+                        ECOperatorPos(dir, i);
+                        for (int k = 0; k < 9; ++k) out.c[k].push_back((double)(F)c[k]);
+                        out.dir.push_back((uint32_t)dir);
+                        for (int k = 0; k < 3; ++k) out.pos[k].push_back(pos[k]);
+                    }
+            if (P.caps) {
+                for (pos[d1] = st[d1]; pos[d1] <= sp[d1]; ++pos[d1])
+                    for (pos[d2] = st[d2]; pos[d2] <= sp[d2]; ++pos[d2])
+                        for (int side = 0; side < 2; ++side) {
+                            pos[dir] = side ? sp[dir] : st[dir];
+                            const size_t i = op->idx(pos[0], pos[1], pos[2]);
+                            if (pos[d1] < sp[d1]) { VV(d1, i) = 0; VI(d1, i) = 0; }
+                            if (pos[d2] < sp[d2]) { VV(d2, i) = 0; VI(d2, i) = 0; }
+                        }
+            }
+        }
+    }
+
     // Operator_Ext_Mur_ABC::SetDirection/BuildExtension (operator_ext_mur_abc.cpp:88-180)
     // and the excitation start rule of Engine_Ext_Mur_ABC (engine_ext_mur_abc.cpp:42-58).
     // Needs the excitation lists and the signal length (call after Signal).
@@ -1526,6 +1637,7 @@ int wop_operator::build(uint32_t nr_ts) {
     b.Tfsf();
     b.Upml();
     b.Lorentz();
+    b.LumpedRLC();
     b.Signal(nr_ts);
     b.Mur();
     built = true;
@@ -1686,6 +1798,32 @@ int wop_add_lumped_element(wop_operator* op, int ny, double R, double C, int cap
     Property p;
     p.kind = P_LUMPED; p.ny = ny; p.R = R; p.C = C; p.caps = caps != 0;
     return add_prop(op, p, id);
+}
+
+int wop_add_lumped_rlc(wop_operator* op, int ny, int type, double R, double L, double C, int caps, int* id) {
+    if (!op || ny < 0 || ny > 2 || type < 0 || type > 1) return fail(-1, "bad argument");
+    Property p;
+    p.kind = P_LUMPED; p.ny = ny; p.le_type = type; p.R = R; p.L = L; p.C = C; p.caps = caps != 0;
+    return add_prop(op, p, id);
+}
+
+int wop_get_rlc_count(wop_operator* op, uint32_t* count) {
+    if (!op || !count || !op->built) return fail(-1, "operator not built");
+    *count = (uint32_t)op->rlc.dir.size();
+    return 0;
+}
+
+int wop_get_rlc(wop_operator* op, uint32_t* pos, uint32_t* dir, int dtype, void* coef) {
+    if (!op || !op->built || !pos || !dir || !coef) return fail(-1, "bad argument");
+    const size_t n = op->rlc.dir.size();
+    for (int k = 0; k < 3; ++k) std::copy(op->rlc.pos[k].begin(), op->rlc.pos[k].end(), pos + k * n);
+    std::copy(op->rlc.dir.begin(), op->rlc.dir.end(), dir);
+    for (int k = 0; k < 9; ++k)
+        for (size_t i = 0; i < n; ++i) {
+            if (dtype == 1) ((double*)coef)[k * n + i] = op->rlc.c[k][i];
+            else ((float*)coef)[k * n + i] = (float)op->rlc.c[k][i];
+        }
+    return 0;
 }
 
 int wop_add_excitation(wop_operator* op, int type, const double value[3], double delay, int* id) {
@@ -1983,8 +2121,12 @@ struct PlaneDedup {
 // build followed by wop_compress gives.
 int stream_build(wop_operator* op, uint32_t nr_ts, uint32_t* index, uint32_t max_entries, uint32_t* n_entries) {
     if (op->pw.set) return fail(-1, "the streamed build has no TFSF plane wave");
-    for (auto& p : op->props)
+    for (auto& p : op->props) {
         if (p.lorentz) return fail(-1, "the streamed build has no Lorentz/Drude materials");
+        if ((p.kind & P_LUMPED) && ((p.le_type == 0 && p.L > 0.0) || p.le_type == 1))
+            return fail(-1, "the streamed build has no lumped RLC elements");
+    }
+    op->rlc = wop_operator::Rlc();
     const uint32_t* N = op->N;
     op->sf.clear();
     op->sd.clear();

@@ -54,6 +54,15 @@ class MurFace:
 
 
 @dataclass
+class RlcCells:
+    """Lumped RLC cells (Operator_Ext_LumpedRLC): pos uint32 [3][count], dir
+    [count], coef [9][count] = ilv, i2v, vv2, vj1, vj2, vvd, ib0, b1, b2."""
+    pos: np.ndarray
+    dir: np.ndarray
+    coef: np.ndarray
+
+
+@dataclass
 class AdeOrder:
     pos: np.ndarray                # uint32 [3][count]
     lists: List[np.ndarray]        # v_int, v_ext, v_lor, i_int, i_ext, i_lor, each [3][count]
@@ -143,6 +152,7 @@ class OperatorData:
     edge_dual: Optional[np.ndarray] = None      # [Nx+Ny+Nz] dual edge lengths (m)
     dft_boxes: List[DftBoxDef] = field(default_factory=list)
     mur: List[MurFace] = field(default_factory=list)
+    rlc: Optional[RlcCells] = None
 
     @property
     def cells(self):
@@ -251,6 +261,13 @@ class Operator:
     def add_lumped_element(self, ny, R=-1.0, C=-1.0, caps=True):
         i = c_int()
         self._chk(self._lib.wop_add_lumped_element(self._h, int(ny), float(R), float(C), int(bool(caps)), byref(i)))
+        return i.value
+
+    def add_lumped_rlc(self, ny, rlc_type, R=-1.0, L=-1.0, C=-1.0, caps=True):
+        """Lumped RLC element: rlc_type 0 parallel (with L > 0), 1 series; values < 0 are unset."""
+        i = c_int()
+        self._chk(self._lib.wop_add_lumped_rlc(self._h, int(ny), int(rlc_type), float(R), float(L), float(C),
+                                               int(bool(caps)), byref(i)))
         return i.value
 
     def add_excitation(self, exc_type, value, delay=0.0):
@@ -373,6 +390,7 @@ class Operator:
                                            _dp(t["delta"]), _dp(t["amp"])))
                 tfsf[which] = t
         mur = self._mur_faces(n, prec)
+        rlc = self._rlc_cells(prec)
         self._built_prec = prec
         tot = int(n[0]) + int(n[1]) + int(n[2])
         edges = []
@@ -382,7 +400,19 @@ class Operator:
             edges.append(ebuf)
         self._edges = edges
         return OperatorData(tuple(int(x) for x in n), dT.value, prec, *arrs, exc, slabs, ades, nyq.value, mstep.value,
-                            tfsf=tfsf, source=self, edge_primary=edges[0], edge_dual=edges[1], mur=mur)
+                            tfsf=tfsf, source=self, edge_primary=edges[0], edge_dual=edges[1], mur=mur, rlc=rlc)
+
+    def _rlc_cells(self, prec):
+        cnt = c_uint32()
+        self._chk(self._lib.wop_get_rlc_count(self._h, byref(cnt)))
+        n = cnt.value
+        if not n:
+            return None
+        pos = np.empty((3, n), np.uint32)
+        d = np.empty(n, np.uint32)
+        coef = np.empty((9, n), np.float64)
+        self._chk(self._lib.wop_get_rlc(self._h, _up(pos), _up(d), 1, _vp(coef)))
+        return RlcCells(pos, d, coef.astype(np.float64 if prec == 1 else np.float32))
 
     def _mur_faces(self, n, prec):
         lib, h = self._lib, self._h
