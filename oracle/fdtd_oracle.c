@@ -103,18 +103,11 @@ orc* orc_create(const uint32_t N[3], double dT, int use_double, int ftz) {
 
 void orc_destroy(orc* o) {
     if (!o) return;
-    void** ptrs[] = {(void**)&o->f32.V, (void**)&o->f32.I, (void**)&o->f32.vv, (void**)&o->f32.vi, (void**)&o->f32.ii, (void**)&o->f32.iv,
-                     (void**)&o->f64.V, (void**)&o->f64.I, (void**)&o->f64.vv, (void**)&o->f64.vi, (void**)&o->f64.ii, (void**)&o->f64.iv};
-    for (size_t k = 0; k < sizeof(ptrs) / sizeof(ptrs[0]); ++k) free(*ptrs[k]);
+    fields_free_f(&o->f32);
+    fields_free_d(&o->f64);
     for (int p = 0; p < o->nprobes; ++p) {
         free(o->probes[p].freqs); free(o->probes[p].off); free(o->probes[p].sign);
         free(o->probes[p].td); free(o->probes[p].fd);
-    }
-    /* extension arrays are small test data; release them too */
-    for (uint32_t s = 0; s < 6; ++s) {
-        for (int k = 0; k < 6; ++k) { free(o->f32.ucoef[s][k]); free(o->f64.ucoef[s][k]); }
-        free(o->f32.vflux[s]); free(o->f32.cflux[s]); free(o->f64.vflux[s]); free(o->f64.cflux[s]);
-        free(o->f32.mur_k[s]); free(o->f32.mur_buf[s]); free(o->f64.mur_k[s]); free(o->f64.mur_buf[s]);
     }
     free(o);
 }

@@ -53,6 +53,31 @@ typedef struct {
     REAL* rlc_j[3];
 } FN(fields_t);
 
+/* Releases every array the fields own (free(NULL) is a no-op). */
+static void FN(fields_free)(FN(fields_t) * f) {
+    free(f->vv); free(f->vi); free(f->ii); free(f->iv); free(f->V); free(f->I);
+    for (int s = 0; s < 6; ++s) {
+        for (int k = 0; k < 6; ++k) free(f->ucoef[s][k]);
+        free(f->vflux[s]); free(f->cflux[s]);
+        free(f->mur_k[s]); free(f->mur_buf[s]);
+    }
+    free(f->sig_v); free(f->sig_i);
+    free(f->v_pos); free(f->v_dir); free(f->v_del); free(f->c_pos); free(f->c_dir); free(f->c_del);
+    free(f->v_amp); free(f->c_amp);
+    for (int a = 0; a < 8; ++a) {
+        free(f->ade_pos[a]);
+        for (int k = 0; k < 6; ++k) free(f->ade_k[a][k]);
+        for (int k = 0; k < 4; ++k) free(f->ade_s[a][k]);
+    }
+    for (int w = 0; w < 2; ++w) {
+        free(f->tf_pos[w]); free(f->tf_comp[w]); free(f->tf_del[w]); free(f->tf_delta[w]); free(f->tf_amp[w]);
+    }
+    free(f->rlc_pos); free(f->rlc_dir);
+    for (int k = 0; k < 9; ++k) free(f->rlc_c[k]);
+    free(f->rlc_il);
+    for (int k = 0; k < 3; ++k) { free(f->rlc_vd[k]); free(f->rlc_j[k]); }
+}
+
 #define IDX(x, y, z) (((size_t)(x) * N1 + (y)) * N2 + (z))
 
 /* Engine::UpdateVoltages (FDTD/engine.cpp:108-161).  The z-1 neighbour at
