@@ -171,6 +171,20 @@ int  fdtd_add_mur(fdtd_engine* h, int face, int dtype, const void* k_nyP, const 
  * clears it.  Each edge may appear once.  Pre hook after the Lorentz pre
  * hook, apply before the Lorentz apply. */
 int  fdtd_set_rlc(fdtd_engine* h, uint32_t count, const uint32_t* pos, const uint32_t* dir, int dtype, const void* coef);
+/* Steady-state detection for periodic excitations: Operator_Ext_SteadyState
+ * (FDTD/extensions/operator_ext_steadystate.cpp:21-25,44-50,64-76, period in
+ * seconds rounded to whole timesteps, one voltage probe per pos [3][count] /
+ * dir [count] entry, at most 256) and Engine_Ext_SteadyState::Apply2Voltages
+ * (engine_ext_steadystate.cpp:51-117).  Runs first among the voltage apply
+ * hooks; every step records the probes, every period from the second on it
+ * compares the last two periods and the domain energy (FP64).  period <= 0
+ * removes it.  Set before the first timestep. */
+int  fdtd_set_steady_state(fdtd_engine* h, double period, uint32_t count, const uint32_t* pos, const uint32_t* dir);
+/* the period in timesteps, 0 when no detection is set */
+int  fdtd_get_steady_state_period(fdtd_engine* h, uint32_t* period_ts);
+/* Engine_Ext_SteadyState::GetLastDiff (engine_ext_steadystate.h:38): 1 until
+ * the first evaluation; synchronises the engine's stream */
+int  fdtd_steady_state_last_diff(fdtd_engine* h, double* diff);
 /* Total-field/scattered-field injection list.  which = FDTD_VOLT (added after
  * the voltage update using the current signal) or FDTD_CURR (added after the
  * current update using the voltage signal).  Each entry adds

@@ -110,6 +110,10 @@ int  wop_get_timestep(wop_operator* op, double* dT);
  * step of maximum excitation amplitude (GetMaxExcitationTimestep) */
 int  wop_get_excite_info(wop_operator* op, uint32_t* length, uint32_t* nyquist, uint32_t* max_step);
 int  wop_get_signal(wop_operator* op, double* volt, double* curr);
+/* Excitation::GetSignalPeriod (FDTD/excitation.h:74; 1/f0 for a sine, else 0)
+ * and the timestep count int(period/dT) at which the excitation and TFSF
+ * hooks wrap the signal (engine_ext_excitation.cpp:43-45) */
+int  wop_get_signal_period(wop_operator* op, double* period, uint32_t* period_ts);
 /* coefficients as dense [3][Nx][Ny][Nz] arrays in dtype (0 f32, 1 f64) */
 int  wop_get_coeffs(wop_operator* op, int dtype, void* vv, void* vi, void* ii, void* iv);
 /* EC values after Calc_EC (before the operator build), [3][N] each, double */

@@ -27,6 +27,11 @@
  *                             closed-form incident wave (tests/test_tfsf.py)
  *   mur                       plane-wave reflection of a parallel-plate line against
  *                             a PEC end and a 3x longer line (tests/test_mur.py)
+ *   rlc_pre/apply             discrete parallel-LC frequency, series-loop frequency
+ *                             and decay (tests/test_rlc.py)
+ *   steady_state              closed-form windows and energy ratios of a linear
+ *                             ramp, an exactly periodic field, a sine-driven line
+ *                             that settles (tests/test_steady_state.py)
  */
 #include <math.h>
 #include <stdint.h>
@@ -231,6 +236,24 @@ int orc_set_rlc(orc* o, uint32_t count, const uint32_t* pos, const uint32_t* dir
     return 0;
 }
 
+int orc_set_steady_state(orc* o, uint32_t period_ts, uint32_t count, const uint32_t* pos, const uint32_t* dir) {
+    if (!period_ts) return -1;
+#define SETS(F)                                                                  \
+    {                                                                            \
+        free(F.ss_pos); free(F.ss_dir); free(F.ss_rec);                          \
+        F.ss_n = count; F.ss_p = period_ts;                                      \
+        F.ss_pos = copy_u32(pos, 3 * (size_t)count);                             \
+        F.ss_dir = copy_u32(dir, count);                                         \
+        F.ss_rec = (double*)calloc((count ? count : 1) * 2 * (size_t)period_ts, sizeof(double)); \
+        F.ss_last_diff = 1; F.ss_last_energy = 0;                                \
+    }
+    if (o->dbl) SETS(o->f64) else SETS(o->f32)
+#undef SETS
+    return 0;
+}
+
+double orc_steady_state_last_diff(orc* o) { return o->dbl ? o->f64.ss_last_diff : o->f32.ss_last_diff; }
+
 int orc_set_tfsf(orc* o, int which, uint32_t count, const uint32_t* pos, const uint32_t* comp, const uint32_t* delay,
                  const double* delta, const double* amp, uint32_t max_delay) {
 #define SETT(F, T)                                                         \
@@ -397,17 +420,5 @@ int orc_read_probe_fd(orc* o, int id, double* re, double* im) {
 
 /* eps0*sum(V^2) + mu0*sum(I^2) over x<Nx-1, y<Ny-1, all z, in double */
 double orc_energy(orc* o) {
-    const size_t N0 = o->N[0], N1 = o->N[1], N2 = o->N[2], nd = N0 * N1 * N2;
-    double e = 0, h = 0;
-    for (size_t x = 0; x + 1 < N0; ++x)
-        for (size_t y = 0; y + 1 < N1; ++y)
-            for (size_t z = 0; z < N2; ++z)
-                for (int c = 0; c < 3; ++c) {
-                    const size_t q = c * nd + (x * N1 + y) * N2 + z;
-                    const double v = o->dbl ? o->f64.V[q] : (double)o->f32.V[q];
-                    const double i = o->dbl ? o->f64.I[q] : (double)o->f32.I[q];
-                    e += v * v;
-                    h += i * i;
-                }
-    return 8.85418781762e-12 * e + 1.256637062e-6 * h;
+    return o->dbl ? fast_energy_d(&o->f64, o->N) : fast_energy_f(&o->f32, o->N);
 }

@@ -51,6 +51,11 @@ typedef struct {
     REAL* rlc_il;
     REAL* rlc_vd[3];                  /* rotated once per step, as the reference rotates its pointers */
     REAL* rlc_j[3];
+    /* steady-state detection */
+    uint32_t ss_n, ss_p;
+    uint32_t *ss_pos, *ss_dir;        /* [3][n], [n] */
+    double* ss_rec;                   /* [n][2p] */
+    double ss_last_diff, ss_last_energy;
 } FN(fields_t);
 
 /* Releases every array the fields own (free(NULL) is a no-op). */
@@ -76,6 +81,7 @@ static void FN(fields_free)(FN(fields_t) * f) {
     for (int k = 0; k < 9; ++k) free(f->rlc_c[k]);
     free(f->rlc_il);
     for (int k = 0; k < 3; ++k) { free(f->rlc_vd[k]); free(f->rlc_j[k]); }
+    free(f->ss_pos); free(f->ss_dir); free(f->ss_rec);
 }
 
 #define IDX(x, y, z) (((size_t)(x) * N1 + (y)) * N2 + (z))
@@ -337,9 +343,76 @@ static void FN(rlc_apply)(FN(fields_t) * f, const uint32_t* Nl) {
         f->V[f->rlc_dir[i] * nd + IDX(p[i], p[n + i], p[2 * n + i])] = vd[0][i];
 }
 
+/* Engine_Interface_SSE_FDTD::CalcFastEnergy (FDTD/engine_interface_sse_fdtd.cpp:37-69):
+ * eps0*sum(V^2) + mu0*sum(I^2) over x < Nx-1, y < Ny-1 and every z, here
+ * accumulated in double in x, y, z, component order (the reference sums in
+ * four float lanes; the engine under test accumulates in FP64 too). */
+static double FN(fast_energy)(const FN(fields_t) * f, const uint32_t* Nl) {
+    const size_t N0 = Nl[0], N1 = Nl[1], N2 = Nl[2], nd = N0 * N1 * N2;
+    double e = 0, h = 0;
+    for (size_t x = 0; x + 1 < N0; ++x)
+        for (size_t y = 0; y + 1 < N1; ++y)
+            for (size_t z = 0; z < N2; ++z)
+                for (int c = 0; c < 3; ++c) {
+                    const size_t q = c * nd + IDX(x, y, z);
+                    const double v = (double)f->V[q], i = (double)f->I[q];
+                    e += v * v;
+                    h += i * i;
+                }
+    return 8.85418781762e-12 * e + 1.256637062e-6 * h;
+}
+
+/* Engine_Ext_SteadyState::Apply2Voltages (engine_ext_steadystate.cpp:51-117):
+ * record every probe at slot numTS % 2p; at numTS % p == 0, numTS >= 2p,
+ * compare the energy with the last evaluation and, per probe, the window at
+ * new_pos with the one at old_pos (new_pos = 0 whenever numTS % 2p <= p,
+ * which both possible values satisfy). */
+static void FN(steady_state)(FN(fields_t) * f, const uint32_t* Nl, uint32_t numTS) {
+    if (!f->ss_p) return;
+    const size_t N1 = Nl[1], N2 = Nl[2], nd = (size_t)Nl[0] * N1 * N2;
+    const uint32_t p = f->ss_p, n = f->ss_n, rel = numTS % (2 * p);
+    const uint32_t* q = f->ss_pos;
+    for (uint32_t i = 0; i < n; ++i)
+        f->ss_rec[(size_t)i * 2 * p + rel] = (double)f->V[f->ss_dir[i] * nd + IDX(q[i], q[n + i], q[2 * n + i])];
+    if (numTS % p != 0 || numTS < 2 * p) return;
+    int no_valid = 1;
+    double diff = 0;
+    const double energy = FN(fast_energy)(f, Nl);
+    if (f->ss_last_energy > 0) {
+        diff = fabs(energy - f->ss_last_energy) / f->ss_last_energy;
+        no_valid = 0;
+    }
+    f->ss_last_energy = energy;
+    uint32_t old_pos = 0, new_pos = p;
+    if (rel <= p) { new_pos = 0; old_pos = p; }
+    double* cp = (double*)malloc((n ? n : 1) * sizeof(double));
+    double* dp = (double*)malloc((n ? n : 1) * sizeof(double));
+    double max_pow = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const double* b = f->ss_rec + (size_t)i * 2 * p;
+        cp[i] = 0;
+        dp[i] = 0;
+        for (uint32_t t = 0; t < p; ++t) {
+            cp[i] += b[t + new_pos] * b[t + new_pos];
+            dp[i] += (b[t + old_pos] - b[t + new_pos]) * (b[t + old_pos] - b[t + new_pos]);
+        }
+        if (max_pow < cp[i]) max_pow = cp[i];
+    }
+    for (uint32_t i = 0; i < n; ++i)
+        if (cp[i] > max_pow * 1e-2) {
+            if (diff < dp[i] / cp[i]) diff = dp[i] / cp[i];
+            no_valid = 0;
+        }
+    if (no_valid || diff > 1) diff = 1;
+    f->ss_last_diff = diff;
+    free(cp);
+    free(dp);
+}
+
 /* One timestep in the order of Engine::IterateTS (FDTD/engine.cpp:255-274) with
  * the extensions sorted by priority: pre hooks Excitation, Mur, Lorentz, RLC,
- * TFSF, UPML; post and apply hooks UPML, TFSF, RLC, Lorentz, Mur, Excitation. */
+ * TFSF, UPML; post and apply hooks steady state, UPML, TFSF, RLC, Lorentz,
+ * Mur, Excitation (the steady-state extension has only an apply hook). */
 static void FN(step)(FN(fields_t) * f, const uint32_t* Nl, uint32_t numTS) {
     FN(mur)(f, Nl, numTS, 0);
     FN(ade_pre)(f, Nl, 0);
@@ -349,6 +422,7 @@ static void FN(step)(FN(fields_t) * f, const uint32_t* Nl, uint32_t numTS) {
     FN(upml_post)(f, Nl, 0);
     FN(tfsf)(f, Nl, 0, numTS);
     FN(mur)(f, Nl, numTS, 1);
+    FN(steady_state)(f, Nl, numTS);
     FN(rlc_apply)(f, Nl);
     FN(ade_apply)(f, Nl, 0);
     FN(mur)(f, Nl, numTS, 2);

@@ -7,6 +7,7 @@ through it.
 from __future__ import annotations
 
 import ctypes
+import math
 import os
 from ctypes import POINTER, c_double, c_int, c_uint32, c_void_p
 
@@ -48,6 +49,9 @@ def lib():
         L.orc_read_probe_fd.argtypes = [c_void_p, c_int, POINTER(c_double), POINTER(c_double)]
         L.orc_energy.argtypes = [c_void_p]
         L.orc_energy.restype = c_double
+        L.orc_set_steady_state.argtypes = [c_void_p, c_uint32, c_uint32, POINTER(c_uint32), POINTER(c_uint32)]
+        L.orc_steady_state_last_diff.argtypes = [c_void_p]
+        L.orc_steady_state_last_diff.restype = c_double
         _lib = L
     return _lib
 
@@ -72,6 +76,7 @@ class Oracle:
         n = (c_uint32 * 3)(*num_lines)
         self._h = self._L.orc_create(n, float(dT), int(use_double), int(ftz))
         self.num_lines = tuple(num_lines)
+        self.dT = float(dT)
         self._nprobe = 0
 
     def __del__(self):
@@ -122,6 +127,22 @@ class Oracle:
         coef = np.ascontiguousarray(r.coef)
         code = 1 if coef.dtype == np.float64 else 0
         self._L.orc_set_rlc(self._h, len(d), _up(pos), _up(d), code, _vp(coef))
+
+    def set_steady_state(self, ss):
+        """Operator_Ext_SteadyState::BuildExtension rounds the period to whole
+        timesteps with C round() (half away from zero)."""
+        x = ss.period / self.dT
+        p = math.floor(x)
+        if x - p >= 0.5:
+            p += 1
+        pos = np.ascontiguousarray(ss.pos, dtype=np.uint32).reshape(3, -1)
+        d = np.ascontiguousarray(ss.dir, dtype=np.uint32)
+        if self._L.orc_set_steady_state(self._h, int(p), len(d), _up(pos), _up(d)):
+            raise ValueError("steady-state period below one timestep")
+        return int(p)
+
+    def GetLastDiff(self):
+        return self._L.orc_steady_state_last_diff(self._h)
 
     def add_mur(self, m):
         kp = np.ascontiguousarray(m.k_nyP)
@@ -195,6 +216,8 @@ def oracle_from_operator(data, use_double=None, ftz=True):
         o.add_mur(m)
     if getattr(data, "rlc", None) is not None:
         o.set_rlc(data.rlc)
+    if getattr(data, "steady_state", None) is not None:
+        o.set_steady_state(data.steady_state)
     if data.tfsf:
         for which in (0, 1):
             t = data.tfsf[which]

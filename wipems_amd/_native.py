@@ -64,6 +64,9 @@ _ENGINE_SIGS = {
     "fdtd_set_pml_split": [c_void_p, c_int],
     "fdtd_add_mur": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_uint32],
     "fdtd_set_rlc": [c_void_p, c_uint32, POINTER(c_uint32), POINTER(c_uint32), c_int, c_void_p],
+    "fdtd_set_steady_state": [c_void_p, c_double, c_uint32, POINTER(c_uint32), POINTER(c_uint32)],
+    "fdtd_get_steady_state_period": [c_void_p, POINTER(c_uint32)],
+    "fdtd_steady_state_last_diff": [c_void_p, POINTER(c_double)],
     "fdtd_pml_split": [c_void_p, POINTER(c_int)],
     "fdtd_set_tuning": [c_void_p, c_int, c_int, c_int],
     "fdtd_set_kernel_timing": [c_void_p, c_int],
@@ -113,6 +116,7 @@ _OPERATOR_SIGS = {
     "wop_get_timestep": [c_void_p, POINTER(c_double)],
     "wop_get_edge_lengths": [c_void_p, c_int, POINTER(c_double)],
     "wop_get_excite_info": [c_void_p, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_uint32)],
+    "wop_get_signal_period": [c_void_p, POINTER(c_double), POINTER(c_uint32)],
     "wop_get_signal": [c_void_p, POINTER(c_double), POINTER(c_double)],
     "wop_get_coeffs": [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p],
     "wop_get_ec": [c_void_p, POINTER(c_double), POINTER(c_double), POINTER(c_double), POINTER(c_double)],

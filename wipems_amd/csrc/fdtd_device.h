@@ -1049,6 +1049,92 @@ __global__ void k_rlc(S* V, const unsigned* __restrict__ step, RlcDev r, Scales<
     T::store(F, x, sc.iv[comp]);
 }
 
+// ---------------------------------------------------------------------------
+// Steady-state detection, Engine_Ext_SteadyState::Apply2Voltages
+// (FDTD/extensions/engine_ext_steadystate.cpp:51-117).  The hook runs first
+// among the voltage apply hooks (priority +2e6): after the post-update hooks,
+// before lumped RLC, Lorentz, Mur and the excitation.
+//
+// Every step records the voltage at each probe edge into a ring of two
+// periods (slot = numTS mod 2p, FP64).  At numTS = k*p, k >= 2, the energy of
+// the whole domain (CalcFastEnergy) and, per probe, the power of the "new"
+// window and its squared difference to the "old" one give
+//   diff = max(|E - E_last| / E_last, max over probes with power above 1% of
+//              the largest: diff_pow / curr_pow), 1 when nothing is valid or
+//              the value exceeds 1.
+// The windows follow the reference's choice: rel = numTS mod 2p is 0 or p at
+// these steps, both satisfy rel <= p, so new = slots [0, p), old = [p, 2p).
+// ---------------------------------------------------------------------------
+#define SS_MAX_PROBES 256
+struct SsDev {
+    int n;                     // probes, <= SS_MAX_PROBES
+    unsigned p;                // period in timesteps
+    const long long* off;      // [n] element offset incl. the component
+    const int* comp;           // [n]
+    double* rec;               // [n][2p]
+    double* state;             // [0] last diff, [1] last energy
+};
+
+template <typename S>
+__global__ void k_ss_record(const S* __restrict__ V, const unsigned* __restrict__ step, SsDev d, Scales<typename StoreTraits<S>::A> sc) {
+    typedef StoreTraits<S> T;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.n) return;
+    const unsigned two_p = 2u * d.p;
+    const unsigned rel = *step % two_p;
+    d.rec[(size_t)i * two_p + rel] = (double)T::load(V + d.off[i], sc.v[d.comp[i]]);
+}
+
+// one block of 256: reduces the energy partials of k_energy_partial, then the
+// per-probe window sums (each a serial loop in the reference's order, so the
+// probe terms are exact for identical records), then the decision in thread 0
+static __global__ void __launch_bounds__(256) k_ss_eval(const double* __restrict__ part, int nblk, SsDev d, double eps0, double mue0) {
+    __shared__ double se[256], sh[256];
+    __shared__ double cpow[SS_MAX_PROBES], dpow[SS_MAX_PROBES];
+    double e = 0.0, h = 0.0;
+    for (int i = threadIdx.x; i < nblk; i += blockDim.x) { e += part[2 * i]; h += part[2 * i + 1]; }
+    se[threadIdx.x] = e;
+    sh[threadIdx.x] = h;
+    const unsigned p = d.p;
+    const unsigned new_pos = 0, old_pos = p;
+    for (int i = threadIdx.x; i < d.n; i += blockDim.x) {
+        const double* b = d.rec + (size_t)i * 2u * p;
+        double c = 0.0, q = 0.0;
+        for (unsigned t = 0; t < p; ++t) {
+            const double bn = b[t + new_pos], bo = b[t + old_pos];
+            c += bn * bn;
+            q += (bo - bn) * (bo - bn);
+        }
+        cpow[i] = c;
+        dpow[i] = q;
+    }
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) { se[threadIdx.x] += se[threadIdx.x + s]; sh[threadIdx.x] += sh[threadIdx.x + s]; }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    const double energy = eps0 * se[0] + mue0 * sh[0];
+    const double last = d.state[1];
+    bool no_valid = true;
+    double diff = 0.0;
+    if (last > 0) {
+        diff = fabs(energy - last) / last;
+        no_valid = false;
+    }
+    double max_pow = 0.0;
+    for (int i = 0; i < d.n; ++i) max_pow = max_pow < cpow[i] ? cpow[i] : max_pow;
+    for (int i = 0; i < d.n; ++i)
+        if (cpow[i] > max_pow * 1e-2) {
+            const double r = dpow[i] / cpow[i];
+            diff = diff < r ? r : diff;
+            no_valid = false;
+        }
+    if (no_valid || diff > 1) diff = 1;
+    d.state[0] = diff;
+    d.state[1] = energy;
+}
+
 struct AdeDev {
     int count;
     const long long* off;          // [count] element offset of position (component 0)

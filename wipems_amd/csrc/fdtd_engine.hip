@@ -75,7 +75,7 @@ void invalidate_graph(fdtd_engine* h) {
 // and with few soft excitation edges (kept in LDS)
 bool fused_ok(const fdtd_engine* h) {
     if (h->variant != 4) return false;
-    if (h->pml.count || !h->ade.empty() || !h->mur.empty() || h->rlc || h->tfsf_v.count || h->tfsf_c.count) return false;
+    if (h->pml.count || !h->ade.empty() || !h->mur.empty() || h->rlc || h->ss || h->tfsf_v.count || h->tfsf_c.count) return false;
     if (h->has_exc && (h->exc_v.count > FUSED_MAX_EXC || h->exc_c.count > FUSED_MAX_EXC)) return false;
     return h->N[0] >= 2 && h->N[1] >= 2 && h->N[2] >= 2;
 }
@@ -612,6 +612,15 @@ int launch_step_kernels(fdtd_engine* h, bool timed) {
     if (h->tfsf_v.count)
         launch_tfsf<S>(h, h->V.as<S>(), stp, h->tfsf_v, sc, 0);
     mur_phase(1);
+    // steady-state detection: first of the apply hooks (priority +2e6)
+    if (h->ss) {
+        if (h->ss->d.n)
+            hipLaunchKernelGGL((k_ss_record<S>), dim3((h->ss->d.n + 255) / 256), dim3(256), 0, h->stream, h->V.as<S>(), stp, h->ss->d, sc);
+        if (h->ss_eval_now) {
+            hipLaunchKernelGGL((k_energy_partial<S>), dim3(h->e_blocks), dim3(256), 0, h->stream, h->V.as<S>(), h->I.as<S>(), h->d, sc, h->e_part.as<double>());
+            hipLaunchKernelGGL(k_ss_eval, dim3(1), dim3(256), 0, h->stream, h->e_part.as<double>(), h->e_blocks, h->ss->d, kEPS0, kMUE0);
+        }
+    }
     if (h->rlc)
         hipLaunchKernelGGL((k_rlc<S>), dim3((h->rlc->d.count + 255) / 256), dim3(256), 0, h->stream, h->V.as<S>(), stp, h->rlc->d, sc, 1);
     for (auto a : h->ade)
@@ -667,15 +676,21 @@ int ready_check(fdtd_engine* h) {
     return FDTD_OK;
 }
 
-template <typename S>
-int energy_impl(fdtd_engine* h, double* out) {
-    typedef typename StoreTraits<S>::A A;
+int ensure_energy_buffers(fdtd_engine* h) {
     int rc;
     if (!h->e_part.p) {
         h->e_blocks = 2048;
         if ((rc = h->e_part.alloc((size_t)h->e_blocks * 2 * sizeof(double)))) return rc;
         if ((rc = h->e_out.alloc(2 * sizeof(double)))) return rc;
     }
+    return FDTD_OK;
+}
+
+template <typename S>
+int energy_impl(fdtd_engine* h, double* out) {
+    typedef typename StoreTraits<S>::A A;
+    int rc;
+    if ((rc = ensure_energy_buffers(h))) return rc;
     hipLaunchKernelGGL((k_energy_partial<S>), dim3(h->e_blocks), dim3(256), 0, h->stream, h->V.as<S>(), h->I.as<S>(), h->d, make_scales<A>(h), h->e_part.as<double>());
     hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(256), 0, h->stream, h->e_part.as<double>(), h->e_blocks, h->e_out.as<double>());
     double r[2];
@@ -1124,6 +1139,69 @@ int fdtd_set_rlc(fdtd_engine* h, uint32_t count, const uint32_t* pos, const uint
     return FDTD_OK;
 }
 
+int fdtd_set_steady_state(fdtd_engine* h, double period, uint32_t count, const uint32_t* pos, const uint32_t* dir) {
+    if (!h) return fail(FDTD_ERR_ARG, "null handle");
+    if (h->numTS) return fail(FDTD_ERR_STATE, "steady-state detection must be set before the first timestep");
+    delete h->ss;
+    h->ss = nullptr;
+    invalidate_graph(h);
+    if (!(period > 0)) return FDTD_OK;
+    // Operator_Ext_SteadyState::BuildExtension: the period in whole timesteps
+    const double pts = round(period / h->dT);
+    if (!(pts >= 1) || pts > 1e8) return fail(FDTD_ERR_ARG, "steady-state period must be 1..1e8 timesteps");
+    const uint32_t p = (uint32_t)pts;
+    if (count > SS_MAX_PROBES) return fail(FDTD_ERR_RANGE, "at most 256 steady-state probes");
+    if (count && (!pos || !dir)) return fail(FDTD_ERR_ARG, "null argument");
+    std::vector<long long> off(count ? count : 1);
+    std::vector<int> comp(count ? count : 1);
+    for (uint32_t i = 0; i < count; ++i) {
+        for (int n = 0; n < 3; ++n)
+            if (pos[(size_t)n * count + i] >= h->N[n]) return fail(FDTD_ERR_RANGE, "steady-state probe position out of range");
+        if (dir[i] > 2) return fail(FDTD_ERR_RANGE, "steady-state probe direction out of range");
+        comp[i] = (int)dir[i];
+        off[i] = (long long)dir[i] * h->d.sc + cell_off(h, pos[i], pos[count + i], pos[2 * (size_t)count + i]);
+    }
+    int rc;
+    if ((rc = ensure_energy_buffers(h))) return rc;
+    SsHost* q = new SsHost();
+    const size_t nb = count ? count : 1;
+    if ((rc = q->off.alloc(nb * sizeof(long long))) || (rc = q->comp.alloc(nb * sizeof(int))) ||
+        (rc = q->rec.alloc(nb * 2 * (size_t)p * sizeof(double))) || (rc = q->state.alloc(2 * sizeof(double)))) { delete q; return rc; }
+    const double st[2] = {1.0, 0.0};   // last diff starts at 1, last energy at 0
+    if (hipMemcpyAsync(q->off.p, off.data(), nb * sizeof(long long), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+        hipMemcpyAsync(q->comp.p, comp.data(), nb * sizeof(int), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+        hipMemsetAsync(q->rec.p, 0, q->rec.bytes, h->stream) != hipSuccess ||
+        hipMemcpyAsync(q->state.p, st, sizeof(st), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess) {
+        delete q;
+        return fail(FDTD_ERR_HIP, "steady-state upload failed");
+    }
+    q->d.n = (int)count;
+    q->d.p = p;
+    q->d.off = q->off.as<long long>();
+    q->d.comp = q->comp.as<int>();
+    q->d.rec = q->rec.as<double>();
+    q->d.state = q->state.as<double>();
+    h->ss = q;
+    return FDTD_OK;
+}
+
+int fdtd_get_steady_state_period(fdtd_engine* h, uint32_t* period_ts) {
+    if (!h || !period_ts) return fail(FDTD_ERR_ARG, "null argument");
+    *period_ts = h->ss ? h->ss->d.p : 0;
+    return FDTD_OK;
+}
+
+int fdtd_steady_state_last_diff(fdtd_engine* h, double* diff) {
+    if (!h || !diff) return fail(FDTD_ERR_ARG, "null argument");
+    if (!h->ss) return fail(FDTD_ERR_STATE, "no steady-state detection set");
+    double st[2];
+    HIPCHK(hipMemcpyAsync(st, h->ss->state.p, sizeof(st), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    *diff = st[0];
+    return FDTD_OK;
+}
+
 int fdtd_add_mur(fdtd_engine* h, int face, int dtype, const void* k_nyP, const void* k_nyPP, uint32_t start_ts) {
     if (!h || !k_nyP || !k_nyPP) return fail(FDTD_ERR_ARG, "null argument");
     if (h->numTS) return fail(FDTD_ERR_STATE, "Mur faces must be added before the first timestep");
@@ -1500,15 +1578,33 @@ int fdtd_iterate(fdtd_engine* h, uint32_t nsteps) {
         h->sampled0 = true;
     }
     if (!nsteps) return FDTD_OK;
-    if (h->dft.empty()) return advance(h, nsteps);
-    // run to each DFT sample step in turn, then accumulate the boxes due there
+    if (h->dft.empty() && !h->ss) return advance(h, nsteps);
+    // run to each DFT sample step and each steady-state evaluation step in
+    // turn; field boxes accumulate after their step, the steady-state check
+    // runs inside its step (one step launched on its own)
     uint32_t left = nsteps;
     while (left) {
         uint64_t next = UINT64_MAX;
         for (auto b : h->dft) next = std::min(next, ((uint64_t)h->numTS / b->interval + 1) * b->interval);
-        const uint32_t run = (uint32_t)std::min<uint64_t>(left, next - h->numTS);
-        if ((rc = advance(h, run))) return rc;
-        left -= run;
+        uint64_t run = std::min<uint64_t>(left, next - h->numTS);
+        if (h->ss) {
+            const uint64_t p = h->ss->d.p;
+            const uint64_t ev = std::max<uint64_t>(2 * p, ((uint64_t)h->numTS + p - 1) / p * p);
+            if (ev == h->numTS) {
+                h->ss_eval_now = true;
+                rc = launch_step(h, false);
+                h->ss_eval_now = false;
+                if (rc) return rc;
+                HIPCHK(hipGetLastError());
+                ++h->numTS;
+                --left;
+                if ((uint64_t)h->numTS == next && (rc = launch_dfts(h, h->numTS))) return rc;
+                continue;
+            }
+            run = std::min<uint64_t>(run, ev - h->numTS);
+        }
+        if ((rc = advance(h, (uint32_t)run))) return rc;
+        left -= (uint32_t)run;
         if ((uint64_t)h->numTS == next && (rc = launch_dfts(h, h->numTS))) return rc;
     }
     return FDTD_OK;

@@ -79,6 +79,12 @@ struct RlcHost {
     DevBuf off, comp, coef, state;   // state: il [count], vd [3][count], j [3][count]
 };
 
+// steady-state detection: probe list, record ring and the last result
+struct SsHost {
+    SsDev d{};
+    DevBuf off, comp, rec, state;
+};
+
 // one Mur ABC face: coefficients and the saved boundary values, [2][na][nb]
 struct MurHost {
     MurDev d{};
@@ -92,6 +98,7 @@ using wipems::ProbeHost;
 using wipems::AdeHost;
 using wipems::MurHost;
 using wipems::RlcHost;
+using wipems::SsHost;
 using wipems::DftHost;
 
 struct fdtd_engine {
@@ -150,6 +157,10 @@ struct fdtd_engine {
     // lumped RLC cells (one list)
     RlcHost* rlc = nullptr;
 
+    // steady-state detection; ss_eval_now makes the next step evaluate it
+    SsHost* ss = nullptr;
+    bool ss_eval_now = false;
+
     // TFSF
     TfsfDev tfsf_v{}, tfsf_c{};
     DevBuf tfsf_buf_v, tfsf_buf_c;
@@ -202,6 +213,7 @@ struct fdtd_engine {
         for (auto a : ade) delete a;
         for (auto m : mur) delete m;
         delete rlc;
+        delete ss;
         for (auto b : dft) delete b;
         if (stream) (void)hipStreamDestroy(stream);
     }

@@ -142,6 +142,27 @@ class Engine:
         coef = np.ascontiguousarray(r.coef)
         self._chk(self._lib.fdtd_set_rlc(self._h, len(d), _up(pos), _up(d), _dtype_code(coef), _vp(coef)))
 
+    def set_steady_state(self, ss):
+        """Steady-state detection (Operator_Ext_SteadyState + Engine_Ext_SteadyState);
+        ss = SteadyState(period, pos, dir), or None to remove it."""
+        if ss is None:
+            self._chk(self._lib.fdtd_set_steady_state(self._h, 0.0, 0, None, None))
+            return
+        pos = np.ascontiguousarray(ss.pos, dtype=np.uint32).reshape(3, -1)
+        d = np.ascontiguousarray(ss.dir, dtype=np.uint32)
+        self._chk(self._lib.fdtd_set_steady_state(self._h, float(ss.period), len(d), _up(pos), _up(d)))
+
+    def steady_state_period(self):
+        v = c_uint32()
+        self._chk(self._lib.fdtd_get_steady_state_period(self._h, byref(v)))
+        return v.value
+
+    def GetLastDiff(self):
+        """Engine_Ext_SteadyState::GetLastDiff: relative change over the last period."""
+        v = c_double()
+        self._chk(self._lib.fdtd_steady_state_last_diff(self._h, byref(v)))
+        return v.value
+
     def add_mur(self, m):
         kp = np.ascontiguousarray(m.k_nyP)
         kpp = np.ascontiguousarray(m.k_nyPP, dtype=kp.dtype)
@@ -354,6 +375,8 @@ def engine_from_operator(data: OperatorData, precision="fp32", compressed=None, 
         eng.add_mur(m)
     if data.rlc is not None:
         eng.set_rlc(data.rlc)
+    if data.steady_state is not None:
+        eng.set_steady_state(data.steady_state)
     if data.tfsf:
         for which in (0, 1):
             t = data.tfsf[which]

@@ -1951,6 +1951,17 @@ int wop_get_excite_info(wop_operator* op, uint32_t* length, uint32_t* nyquist, u
     return 0;
 }
 
+int wop_get_signal_period(wop_operator* op, double* period, uint32_t* period_ts) {
+    if (!op || !op->built) return fail(-1, "operator not built");
+    // Excitation::SetupSinusoidal sets the period 1/f0, every other signal 0
+    // (FDTD/excitation.cpp:55-93); the excitation hooks wrap the signal index
+    // at int(period/dT) (engine_ext_excitation.cpp:43-45, engine_ext_tfsf.cpp:41)
+    const double T = (op->exc_sig_type == WOP_EXC_SINE && op->f0 > 0) ? 1 / op->f0 : 0.0;
+    if (period) *period = T;
+    if (period_ts) *period_ts = T > 0 ? (uint32_t)(int)(T / op->dT) : 0;
+    return 0;
+}
+
 int wop_get_signal(wop_operator* op, double* v, double* i) {
     if (!op || !op->built || !v || !i) return fail(-1, "operator not built");
     memcpy(v, op->sig_v.data(), op->sig_v.size() * 8);

@@ -63,6 +63,34 @@ class RlcCells:
 
 
 @dataclass
+class SteadyState:
+    """Steady-state detection of a periodic excitation (Operator_Ext_SteadyState,
+    FDTD/extensions/operator_ext_steadystate.cpp:21-25,64-76): the signal period
+    in seconds and one voltage probe per column of pos [3][n] / dir [n]."""
+    period: float
+    pos: np.ndarray                # uint32 [3][n]
+    dir: np.ndarray                # uint32 [n]
+
+
+def default_steady_state_probes(num_lines):
+    """The probe set openems.cpp:1062-1089 gives the detection: the three
+    components at the centre node, then for each axis n the three components
+    at the centre with pos[n] scaled by 1/4 and then by 3/4.  Both factors are
+    integer divisions there (1/4 == 3/4 == 0), so each of those nodes has
+    pos[n] = 0 and appears twice: 21 probes in all."""
+    c = [int(n) // 2 for n in num_lines]
+    cols = [(tuple(c), d) for d in range(3)]
+    for n in range(3):
+        q = list(c)
+        q[n] *= 1 // 4
+        cols += [(tuple(q), d) for d in range(3)]
+        q[n] *= 3 // 4
+        cols += [(tuple(q), d) for d in range(3)]
+    pos = np.array([p for p, _ in cols], dtype=np.uint32).T.copy()
+    return pos, np.array([d for _, d in cols], dtype=np.uint32)
+
+
+@dataclass
 class AdeOrder:
     pos: np.ndarray                # uint32 [3][count]
     lists: List[np.ndarray]        # v_int, v_ext, v_lor, i_int, i_ext, i_lor, each [3][count]
@@ -153,6 +181,8 @@ class OperatorData:
     dft_boxes: List[DftBoxDef] = field(default_factory=list)
     mur: List[MurFace] = field(default_factory=list)
     rlc: Optional[RlcCells] = None
+    signal_period: float = 0.0             # Excitation::GetSignalPeriod (s), 0 when not periodic
+    steady_state: Optional[SteadyState] = None
 
     @property
     def cells(self):
@@ -356,7 +386,9 @@ class Operator:
         cpos = np.zeros((3, max(nc, 1)), np.uint32); cdir = np.zeros(max(nc, 1), np.uint32)
         camp = np.zeros(max(nc, 1)); cdel = np.zeros(max(nc, 1), np.uint32)
         self._chk(lib.wop_get_excitation(h, _up(vpos), _up(vdir), _dp(vamp), _up(vdel), _up(cpos), _up(cdir), _dp(camp), _up(cdel)))
-        exc = Excitation(length.value, 0, sv, si, np.ascontiguousarray(vpos[:, :nv]), vdir[:nv].copy(), vamp[:nv].copy(), vdel[:nv].copy(),
+        per, per_ts = c_double(), c_uint32()
+        self._chk(lib.wop_get_signal_period(h, byref(per), byref(per_ts)))
+        exc = Excitation(length.value, per_ts.value, sv, si, np.ascontiguousarray(vpos[:, :nv]), vdir[:nv].copy(), vamp[:nv].copy(), vdel[:nv].copy(),
                          np.ascontiguousarray(cpos[:, :nc]), cdir[:nc].copy(), camp[:nc].copy(), cdel[:nc].copy())
         nu = c_uint32()
         self._chk(lib.wop_get_num_upml(h, byref(nu)))
@@ -399,8 +431,15 @@ class Operator:
             self._chk(lib.wop_get_edge_lengths(h, dual, _dp(ebuf)))
             edges.append(ebuf)
         self._edges = edges
-        return OperatorData(tuple(int(x) for x in n), dT.value, prec, *arrs, exc, slabs, ades, nyq.value, mstep.value,
-                            tfsf=tfsf, source=self, edge_primary=edges[0], edge_dual=edges[1], mur=mur, rlc=rlc)
+        nl = tuple(int(x) for x in n)
+        # openems.cpp:1060-1091: a periodic excitation gets a steady-state detection
+        ss = None
+        if per.value > 0:
+            spos, sdir = default_steady_state_probes(nl)
+            ss = SteadyState(per.value, spos, sdir)
+        return OperatorData(nl, dT.value, prec, *arrs, exc, slabs, ades, nyq.value, mstep.value,
+                            tfsf=tfsf, source=self, edge_primary=edges[0], edge_dual=edges[1], mur=mur, rlc=rlc,
+                            signal_period=per.value, steady_state=ss)
 
     def _rlc_cells(self, prec):
         cnt = c_uint32()
