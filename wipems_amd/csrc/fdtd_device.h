@@ -296,9 +296,44 @@ struct alignas(sizeof(S) * VZ) SVec {
     S v[VZ];
 };
 
+// Streaming (nontemporal) access for data one launch touches once: the
+// updated field's own components, the coefficient arrays and index, UPML flux
+// and slab coefficients.  They bypass the caches' normal retention, so the
+// other field's planes, which the x-1 and y-1 neighbours read again, stay in
+// L2.  Loads and stores of 4, 8, 16 or 32 bytes.
+template <int B> struct RawBits;
+template <> struct RawBits<4> { typedef unsigned int T; };
+template <> struct RawBits<8> { typedef unsigned int T __attribute__((ext_vector_type(2))); };
+template <> struct RawBits<16> { typedef unsigned int T __attribute__((ext_vector_type(4))); };
+template <> struct RawBits<32> { typedef unsigned int T __attribute__((ext_vector_type(8))); };
+
+template <typename V>
+__device__ __forceinline__ V ld_stream(const V* p) {
+    typedef typename RawBits<sizeof(V)>::T R;
+    const R r = __builtin_nontemporal_load(reinterpret_cast<const R*>(p));
+    V v;
+    __builtin_memcpy(&v, &r, sizeof(V));
+    return v;
+}
+
+template <typename V>
+__device__ __forceinline__ void st_stream(V* p, const V& v) {
+    typedef typename RawBits<sizeof(V)>::T R;
+    R r;
+    __builtin_memcpy(&r, &v, sizeof(V));
+    __builtin_nontemporal_store(r, reinterpret_cast<R*>(p));
+}
+
 template <typename S, int VZ>
 __device__ __forceinline__ void vload(const S* p, typename StoreTraits<S>::A* out, typename StoreTraits<S>::A s) {
     const SVec<S, VZ> t = *reinterpret_cast<const SVec<S, VZ>*>(p);
+#pragma unroll
+    for (int k = 0; k < VZ; ++k) out[k] = StoreTraits<S>::load(&t.v[k], s);
+}
+
+template <typename S, int VZ>
+__device__ __forceinline__ void vload_stream(const S* p, typename StoreTraits<S>::A* out, typename StoreTraits<S>::A s) {
+    const SVec<S, VZ> t = ld_stream(reinterpret_cast<const SVec<S, VZ>*>(p));
 #pragma unroll
     for (int k = 0; k < VZ; ++k) out[k] = StoreTraits<S>::load(&t.v[k], s);
 }
@@ -309,7 +344,7 @@ __device__ __forceinline__ void vstore(S* p, const typename StoreTraits<S>::A* i
 #pragma unroll
     for (int k = 0; k < VZ; ++k) StoreTraits<S>::store(&t.v[k], in[k], inv_s);
     if (valid >= VZ) {
-        *reinterpret_cast<SVec<S, VZ>*>(p) = t;
+        st_stream(reinterpret_cast<SVec<S, VZ>*>(p), t);
     } else {
 #pragma unroll
         for (int k = 0; k < VZ; ++k)
@@ -329,9 +364,9 @@ __device__ __forceinline__ void pml_coeffs(const A* full0, const A* full1, const
             k0[k] = r[n]; kfo[k] = r[4 + n]; kfn[k] = r[8 + n];
         }
     } else {
-        const SVec<A, VZ> a = *reinterpret_cast<const SVec<A, VZ>*>(full0 + q);
-        const SVec<A, VZ> b = *reinterpret_cast<const SVec<A, VZ>*>(full1 + q);
-        const SVec<A, VZ> c = *reinterpret_cast<const SVec<A, VZ>*>(full2 + q);
+        const SVec<A, VZ> a = ld_stream(reinterpret_cast<const SVec<A, VZ>*>(full0 + q));
+        const SVec<A, VZ> b = ld_stream(reinterpret_cast<const SVec<A, VZ>*>(full1 + q));
+        const SVec<A, VZ> c = ld_stream(reinterpret_cast<const SVec<A, VZ>*>(full2 + q));
 #pragma unroll
         for (int k = 0; k < VZ; ++k) { k0[k] = a.v[k]; kfo[k] = b.v[k]; kfn[k] = c.v[k]; }
     }
@@ -345,7 +380,7 @@ __device__ __forceinline__ void upml_lanes(A* flux, const A* k0, const A* kfo, c
                                            const unsigned short* rows, long long q, int n, unsigned pm, const A* v,
                                            const A* a, const A* b, const A* curl, A* out) {
     A F[VZ], kv[VZ], fo[VZ], fn[VZ];
-    vload<A, VZ>(flux + q, F, A(1));
+    vload_stream<A, VZ>(flux + q, F, A(1));
     pml_coeffs<A, VZ>(k0, kfo, kfn, q, tab, rows, n, kv, fo, fn);
 #pragma unroll
     for (int k = 0; k < VZ; ++k) {
@@ -387,19 +422,19 @@ __device__ __forceinline__ void vload_coeffs(const CoeffPtrs<A>& cf, long long p
     if (CM == CF_FULL) {
 #pragma unroll
         for (int n = 0; n < 3; ++n) {
-            const SVec<A, VZ> ta = *reinterpret_cast<const SVec<A, VZ>*>(cf.a + n * sc + p);
-            const SVec<A, VZ> tb = *reinterpret_cast<const SVec<A, VZ>*>(cf.b + n * sc + p);
+            const SVec<A, VZ> ta = ld_stream(reinterpret_cast<const SVec<A, VZ>*>(cf.a + n * sc + p));
+            const SVec<A, VZ> tb = ld_stream(reinterpret_cast<const SVec<A, VZ>*>(cf.b + n * sc + p));
 #pragma unroll
             for (int k = 0; k < VZ; ++k) { a[n][k] = ta.v[k]; b[n][k] = tb.v[k]; }
         }
     } else {
         unsigned c[VZ];
         if (CM == CF_IDX16) {
-            const SVec<unsigned short, VZ> ti = *reinterpret_cast<const SVec<unsigned short, VZ>*>((const unsigned short*)cf.idx + p);
+            const SVec<unsigned short, VZ> ti = ld_stream(reinterpret_cast<const SVec<unsigned short, VZ>*>((const unsigned short*)cf.idx + p));
 #pragma unroll
             for (int k = 0; k < VZ; ++k) c[k] = ti.v[k];
         } else {
-            const SVec<unsigned, VZ> ti = *reinterpret_cast<const SVec<unsigned, VZ>*>((const unsigned*)cf.idx + p);
+            const SVec<unsigned, VZ> ti = ld_stream(reinterpret_cast<const SVec<unsigned, VZ>*>((const unsigned*)cf.idx + p));
 #pragma unroll
             for (int k = 0; k < VZ; ++k) c[k] = ti.v[k];
         }
@@ -464,7 +499,7 @@ __device__ __forceinline__ void volt_vec_body(S* __restrict__ V, const S* __rest
 #pragma unroll
     for (int n = 0; n < 3; ++n) {
         A v[VZ], out[VZ];
-        vload<S, VZ>(V + n * d.sc + p, v, sc.v[n]);
+        vload_stream<S, VZ>(V + n * d.sc + p, v, sc.v[n]);
 #pragma unroll
         for (int k = 0; k < VZ; ++k) {
             out[k] = v[k] * vv[n][k];
@@ -628,7 +663,7 @@ __device__ __forceinline__ void curr_vec_body(S* __restrict__ I, const S* __rest
 #pragma unroll
     for (int n = 0; n < 3; ++n) {
         A c[VZ], out[VZ];
-        vload<S, VZ>(I + n * d.sc + p, c, sc.c[n]);
+        vload_stream<S, VZ>(I + n * d.sc + p, c, sc.c[n]);
 #pragma unroll
         for (int k = 0; k < VZ; ++k) {
             out[k] = c[k] * ii[n][k];
