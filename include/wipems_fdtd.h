@@ -226,6 +226,21 @@ int  fdtd_add_dft_box(fdtd_engine* h, int field, const uint32_t start[3], const 
 int  fdtd_dft_num_samples(fdtd_engine* h, int id, uint64_t* n);
 /* Copies the accumulators, [n_freq][3][nx][ny][nz][2] doubles (re, im), z fastest. */
 int  fdtd_read_dft(fdtd_engine* h, int id, double* out);
+/* Time-domain field dump of a node box (ProcessFieldsTD::Process,
+ * Common/processfields_td.cpp:54-91, over ProcessFields::CalcField,
+ * Common/processfields.cpp:280-325, NODE_INTERPOLATE): at every timestep
+ * divisible by `interval` (step 0 included) the interpolated E (field =
+ * FDTD_VOLT) or H (FDTD_CURR) of nodes start..stop is rounded to float and
+ * handed to `sink` as [3][nx][ny][nz] (z fastest) with the timestep and its
+ * time (H at the dual time (ts + 1/2) dT).  The copy to the host overlaps the
+ * following steps: a sample reaches the sink when its host slot is needed
+ * again or before fdtd_iterate returns, always in timestep order, on the
+ * thread that called fdtd_iterate; `field` is valid during the call only
+ * and the sink must not call into the engine.  Needs fdtd_set_edge_lengths. */
+typedef void (*fdtd_td_sink)(void* user, int id, uint32_t ts, double time, const float* field, uint64_t count);
+int  fdtd_add_td_box(fdtd_engine* h, int field, const uint32_t start[3], const uint32_t stop[3], uint32_t interval,
+                     fdtd_td_sink sink, void* user, int* id);
+int  fdtd_td_num_samples(fdtd_engine* h, int id, uint64_t* n);
 
 /* ---- time stepping ---- */
 int  fdtd_iterate(fdtd_engine* h, uint32_t nsteps);

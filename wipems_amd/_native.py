@@ -64,6 +64,8 @@ _ENGINE_SIGS = {
     "fdtd_set_pml_split": [c_void_p, c_int],
     "fdtd_add_mur": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_uint32],
     "fdtd_set_rlc": [c_void_p, c_uint32, POINTER(c_uint32), POINTER(c_uint32), c_int, c_void_p],
+    "fdtd_add_td_box": [c_void_p, c_int, POINTER(c_uint32), POINTER(c_uint32), c_uint32, c_void_p, c_void_p, POINTER(c_int)],
+    "fdtd_td_num_samples": [c_void_p, c_int, POINTER(c_uint64)],
     "fdtd_set_steady_state": [c_void_p, c_double, c_uint32, POINTER(c_uint32), POINTER(c_uint32)],
     "fdtd_get_steady_state_period": [c_void_p, POINTER(c_uint32)],
     "fdtd_steady_state_last_diff": [c_void_p, POINTER(c_double)],
@@ -141,6 +143,15 @@ _OPERATOR_SIGS = {
     "wop_snap_to_mesh_line": [c_void_p, c_int, c_double, c_int, POINTER(c_uint32), POINTER(c_int)],
 }
 
+_DUMP_SIGS = {
+    "wdump_last_error": [c_char_p, c_size_t],
+    "wdump_create": [c_char_p, POINTER(c_double), c_uint32, POINTER(c_double), c_uint32, POINTER(c_double), c_uint32,
+                     c_double, POINTER(c_void_p)],
+    "wdump_write_td": [c_void_p, c_uint32, c_double, c_void_p],
+    "wdump_write_fd": [c_void_p, c_uint32, POINTER(c_double), POINTER(c_double)],
+    "wdump_close": [c_void_p],
+}
+
 _libs = {}
 
 
@@ -170,6 +181,11 @@ def engine_lib():
 def operator_lib():
     """The host operator library (raises FdtdError if it is not built)."""
     return _load("libwipems_operator.so", _OPERATOR_SIGS)
+
+
+def dump_lib():
+    """The HDF5 field-dump writer (raises FdtdError if it is not built)."""
+    return _load("libwipems_dump.so", _DUMP_SIGS)
 
 
 def engine_path():

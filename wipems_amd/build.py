@@ -61,6 +61,22 @@ def build_operator(verbose=False):
     return out
 
 
+HDF5_ROOT = "/opt/conda"
+
+
+def build_dump(verbose=False):
+    """HDF5 field-dump writer (host only), linked against the image's HDF5."""
+    out = os.path.join(PKG, "libwipems_dump.so")
+    tmp = out + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-I" + INC, "-I" + os.path.join(HDF5_ROOT, "include"),
+          "-o", tmp, os.path.join(PKG, "host", "h5dump.cpp"), "-L" + os.path.join(HDF5_ROOT, "lib"), "-lhdf5",
+          "-Wl,-rpath," + os.path.join(HDF5_ROOT, "lib")])
+    os.replace(tmp, out)
+    if verbose:
+        print("built", out)
+    return out
+
+
 def build_oracle(verbose=False):
     d = os.path.join(ROOT, "oracle")
     _run(["make", "-s", "-j4", "all"], cwd=d)
@@ -70,6 +86,7 @@ def build_oracle(verbose=False):
 
 def build_all(with_oracle=True, verbose=False):
     build_operator(verbose)
+    build_dump(verbose)
     build_engine(verbose)
     if with_oracle:
         build_oracle(verbose)

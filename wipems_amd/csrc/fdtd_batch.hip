@@ -338,7 +338,8 @@ int fdtd_batch_create(fdtd_engine* const* engines, int count, fdtd_batch** out) 
         if (h->prec != prec) return fail(FDTD_ERR_ARG, "engines of a batch must share the storage precision");
         if (!h->ade.empty() || h->tfsf_v.count || h->tfsf_c.count)
             return fail(FDTD_ERR_ARG, "engine %d has dispersive or TFSF extensions: not supported in a batch", i);
-        if (!h->dft.empty()) return fail(FDTD_ERR_ARG, "engine %d has frequency-domain field boxes: not supported in a batch", i);
+        if (!h->dft.empty() || !h->td.empty())
+            return fail(FDTD_ERR_ARG, "engine %d has field dump boxes: not supported in a batch", i);
         if (!h->mur.empty() || h->rlc || h->ss)
             return fail(FDTD_ERR_ARG, "engine %d has Mur ABC faces, lumped RLC cells or steady-state detection: not supported in a batch", i);
         if (h->exc_v.sequential || h->exc_c.sequential) return fail(FDTD_ERR_ARG, "engine %d has repeated excitation edges", i);

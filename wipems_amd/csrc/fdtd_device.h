@@ -1387,26 +1387,11 @@ __device__ __forceinline__ double dft_raw(const S* __restrict__ F, const Dims& d
     return v / delta;
 }
 
+// the interpolated E (field 0, from V) or H (field 1, from I) at node pos
 template <typename S>
-__global__ void __launch_bounds__(256)
-k_dft(const S* __restrict__ F, Dims d, Scales<typename StoreTraits<S>::A> sc, DftAxes ax, DftDev b, double t, double g) {
-    __shared__ double2 ph[MAX_DFT_FREQ];
-    for (int k = threadIdx.x; k < b.n_freq; k += blockDim.x) {
-        double sn, cs;
-        sincos(-2.0 * M_PI * b.freqs[k] * t, &sn, &cs);
-        // exp(-2 pi i f T) * 2 * (dT * FD_Interval), in the reference's order
-        ph[k] = make_double2((cs * 2.0) * g, (sn * 2.0) * g);
-    }
-    __syncthreads();
-    const long long node = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (node >= b.nodes) return;
-    const int k = (int)(node % b.n[2]);
-    const long long r = node / b.n[2];
-    const int j = (int)(r % b.n[1]);
-    const int i = (int)(r / b.n[1]);
-    const int pos[3] = {b.s[0] + i, b.s[1] + j, b.s[2] + k};
-    double out[3];
-    if (b.field == 0) {
+__device__ __forceinline__ void node_field(const S* __restrict__ F, const Dims& d, const Scales<typename StoreTraits<S>::A>& sc,
+                                           const DftAxes& ax, int field, const int pos[3], double out[3]) {
+    if (field == 0) {
         for (int n = 0; n < 3; ++n) {
             const typename StoreTraits<S>::A scale = sc.v[n];
             int q[3] = {pos[0], pos[1], pos[2]};
@@ -1442,6 +1427,28 @@ k_dft(const S* __restrict__ F, Dims d, Scales<typename StoreTraits<S>::A> sc, Df
             out[n] = o / 4.0;
         }
     }
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256)
+k_dft(const S* __restrict__ F, Dims d, Scales<typename StoreTraits<S>::A> sc, DftAxes ax, DftDev b, double t, double g) {
+    __shared__ double2 ph[MAX_DFT_FREQ];
+    for (int k = threadIdx.x; k < b.n_freq; k += blockDim.x) {
+        double sn, cs;
+        sincos(-2.0 * M_PI * b.freqs[k] * t, &sn, &cs);
+        // exp(-2 pi i f T) * 2 * (dT * FD_Interval), in the reference's order
+        ph[k] = make_double2((cs * 2.0) * g, (sn * 2.0) * g);
+    }
+    __syncthreads();
+    const long long node = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= b.nodes) return;
+    const int k = (int)(node % b.n[2]);
+    const long long r = node / b.n[2];
+    const int j = (int)(r % b.n[1]);
+    const int i = (int)(r / b.n[1]);
+    const int pos[3] = {b.s[0] + i, b.s[1] + j, b.s[2] + k};
+    double out[3];
+    node_field<S>(F, d, sc, ax, b.field, pos, out);
     for (int f = 0; f < b.n_freq; ++f) {
         const double2 p = ph[f];
         for (int n = 0; n < 3; ++n) {
@@ -1452,6 +1459,24 @@ k_dft(const S* __restrict__ F, Dims d, Scales<typename StoreTraits<S>::A> sc, Df
             *a = v;
         }
     }
+}
+
+// Time-domain field dump of a node box (ProcessFieldsTD::Process via
+// ProcessFields::CalcField, Common/processfields.cpp:280-325): the
+// interpolated field rounded to float, [3][nodes] with nodes z fastest.
+template <typename S>
+__global__ void __launch_bounds__(256)
+k_snapshot(const S* __restrict__ F, Dims d, Scales<typename StoreTraits<S>::A> sc, DftAxes ax, DftDev b, float* __restrict__ out) {
+    const long long node = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= b.nodes) return;
+    const int k = (int)(node % b.n[2]);
+    const long long r = node / b.n[2];
+    const int j = (int)(r % b.n[1]);
+    const int i = (int)(r / b.n[1]);
+    const int pos[3] = {b.s[0] + i, b.s[1] + j, b.s[2] + k};
+    double v[3];
+    node_field<S>(F, d, sc, ax, b.field, pos, v);
+    for (int n = 0; n < 3; ++n) out[n * b.nodes + node] = (float)v[n];
 }
 
 // ---------------------------------------------------------------------------

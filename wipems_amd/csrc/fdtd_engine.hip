@@ -723,6 +723,67 @@ int launch_dft(fdtd_engine* h, DftHost* b, uint32_t ts) {
     return FDTD_OK;
 }
 
+// hands the sample in host slot `slot` of box b to its sink
+int drain_td_slot(fdtd_engine* h, TdHost* b, int slot) {
+    if (!b->pending[slot]) return FDTD_OK;
+    HIPCHK(hipEventSynchronize(b->ev[slot]));
+    b->pending[slot] = false;
+    const double t = ((double)b->pts[slot] + (b->field ? 0.5 : 0.0)) * h->dT;
+    b->sink(b->user, b->id, b->pts[slot], t, b->host[slot], (uint64_t)3 * b->nodes);
+    return FDTD_OK;
+}
+
+// every pending sample, oldest first per box
+int drain_tds(fdtd_engine* h) {
+    int rc;
+    for (auto b : h->td)
+        for (int q = 0; q < 2; ++q)
+            if ((rc = drain_td_slot(h, b, (int)((b->count + q) & 1)))) return rc;
+    return FDTD_OK;
+}
+
+template <typename S>
+int launch_td(fdtd_engine* h, TdHost* b, uint32_t ts) {
+    typedef typename StoreTraits<S>::A A;
+    const int slot = (int)(b->count & 1);
+    int rc;
+    if ((rc = drain_td_slot(h, b, slot))) return rc;
+    const int tot = (int)(h->N[0] + h->N[1] + h->N[2]);
+    DftAxes ax;
+    ax.len = h->edge_len.as<double>() + (b->field ? tot : 0);
+    ax.off[0] = 0; ax.off[1] = (int)h->N[0]; ax.off[2] = (int)(h->N[0] + h->N[1]);
+    for (int n = 0; n < 3; ++n) ax.N[n] = (int)h->N[n];
+    DftDev dd{};
+    dd.field = b->field;
+    for (int n = 0; n < 3; ++n) { dd.s[n] = (int)b->s[n]; dd.n[n] = (int)b->n[n]; }
+    dd.nodes = b->nodes;
+    const S* F = b->field ? h->I.as<S>() : h->V.as<S>();
+    hipLaunchKernelGGL((k_snapshot<S>), dim3((unsigned)((b->nodes + 255) / 256)), dim3(256), 0, h->stream, F, h->d, make_scales<A>(h), ax, dd,
+                       b->buf.as<float>());
+    HIPCHK(hipMemcpyAsync(b->host[slot], b->buf.p, (size_t)3 * b->nodes * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipEventRecord(b->ev[slot], h->stream));
+    b->pending[slot] = true;
+    b->pts[slot] = ts;
+    ++b->count;
+    return FDTD_OK;
+}
+
+int launch_tds(fdtd_engine* h, uint32_t ts) {
+    for (auto b : h->td) {
+        if (ts % b->interval) continue;
+        int rc;
+        switch (h->prec) {
+        case FDTD_PREC_FP32: rc = launch_td<float>(h, b, ts); break;
+        case FDTD_PREC_FP64: rc = launch_td<double>(h, b, ts); break;
+        case FDTD_PREC_FP16: rc = launch_td<__half>(h, b, ts); break;
+        default: rc = launch_td<__hip_bfloat16>(h, b, ts); break;
+        }
+        if (rc) return rc;
+    }
+    HIPCHK(hipGetLastError());
+    return FDTD_OK;
+}
+
 int launch_dfts(fdtd_engine* h, uint32_t ts) {
     for (auto b : h->dft) {
         if (ts % b->interval) continue;
@@ -1470,6 +1531,45 @@ int fdtd_add_dft_box(fdtd_engine* h, int field, const uint32_t start[3], const u
     return FDTD_OK;
 }
 
+int fdtd_add_td_box(fdtd_engine* h, int field, const uint32_t start[3], const uint32_t stop[3], uint32_t interval,
+                    fdtd_td_sink sink, void* user, int* id) {
+    if (!h || !start || !stop || !sink || !id) return fail(FDTD_ERR_ARG, "null argument");
+    if (field != FDTD_VOLT && field != FDTD_CURR) return fail(FDTD_ERR_ARG, "field must be FDTD_VOLT (E) or FDTD_CURR (H)");
+    if (!h->has_edges) return fail(FDTD_ERR_STATE, "fdtd_set_edge_lengths must be called before fdtd_add_td_box");
+    if (interval < 1) return fail(FDTD_ERR_ARG, "interval must be >= 1");
+    auto b = std::unique_ptr<TdHost>(new TdHost());
+    b->field = field == FDTD_CURR ? 1 : 0;
+    b->nodes = 1;
+    for (int n = 0; n < 3; ++n) {
+        if (start[n] > stop[n] || stop[n] >= h->N[n]) return fail(FDTD_ERR_RANGE, "dump box out of the mesh on axis %d", n);
+        b->s[n] = start[n];
+        b->n[n] = stop[n] - start[n] + 1;
+        b->nodes *= b->n[n];
+    }
+    if ((b->nodes + 255) / 256 > 0x7fffffffLL) return fail(FDTD_ERR_RANGE, "dump box too large");
+    b->interval = interval;
+    b->sink = sink;
+    b->user = user;
+    b->id = (int)h->td.size();
+    const size_t bytes = (size_t)3 * b->nodes * sizeof(float);
+    int rc;
+    if ((rc = b->buf.alloc(bytes))) return rc;
+    for (int k = 0; k < 2; ++k) {
+        HIPCHK(hipHostMalloc((void**)&b->host[k], bytes, hipHostMallocDefault));
+        HIPCHK(hipEventCreateWithFlags(&b->ev[k], hipEventDisableTiming));
+    }
+    *id = b->id;
+    h->td.push_back(b.release());
+    return FDTD_OK;
+}
+
+int fdtd_td_num_samples(fdtd_engine* h, int id, uint64_t* n) {
+    if (!h || !n) return fail(FDTD_ERR_ARG, "null argument");
+    if (id < 0 || id >= (int)h->td.size()) return fail(FDTD_ERR_RANGE, "bad dump box id");
+    *n = h->td[id]->count;
+    return FDTD_OK;
+}
+
 int fdtd_dft_num_samples(fdtd_engine* h, int id, uint64_t* n) {
     if (!h || !n) return fail(FDTD_ERR_ARG, "null argument");
     if (id < 0 || id >= (int)h->dft.size()) return fail(FDTD_ERR_RANGE, "bad DFT box id");
@@ -1556,8 +1656,17 @@ int fdtd_get_kernel_times(fdtd_engine* h, double* volt_ms, double* curr_ms, uint
 
 static int advance(fdtd_engine* h, uint32_t nsteps);
 
+static int iterate_steps(fdtd_engine* h, uint32_t nsteps);
+
 int fdtd_iterate(fdtd_engine* h, uint32_t nsteps) {
     if (!h) return fail(FDTD_ERR_ARG, "null handle");
+    const int rc = iterate_steps(h, nsteps);
+    // time-domain samples still in flight reach their sinks before returning
+    const int rc2 = drain_tds(h);
+    return rc ? rc : rc2;
+}
+
+static int iterate_steps(fdtd_engine* h, uint32_t nsteps) {
     int rc = ready_check(h);
     if (rc) return rc;
     if ((rc = finalize_pml(h))) return rc;
@@ -1572,10 +1681,11 @@ int fdtd_iterate(fdtd_engine* h, uint32_t nsteps) {
         }
         if (rc) return rc;
         if ((rc = launch_dfts(h, 0))) return rc;
+        if ((rc = launch_tds(h, 0))) return rc;
         h->sampled0 = true;
     }
     if (!nsteps) return FDTD_OK;
-    if (h->dft.empty() && !h->ss) return advance(h, nsteps);
+    if (h->dft.empty() && h->td.empty() && !h->ss) return advance(h, nsteps);
     // run to each DFT sample step and each steady-state evaluation step in
     // turn; field boxes accumulate after their step, the steady-state check
     // runs inside its step (one step launched on its own)
@@ -1583,6 +1693,7 @@ int fdtd_iterate(fdtd_engine* h, uint32_t nsteps) {
     while (left) {
         uint64_t next = UINT64_MAX;
         for (auto b : h->dft) next = std::min(next, ((uint64_t)h->numTS / b->interval + 1) * b->interval);
+        for (auto b : h->td) next = std::min(next, ((uint64_t)h->numTS / b->interval + 1) * b->interval);
         uint64_t run = std::min<uint64_t>(left, next - h->numTS);
         if (h->ss) {
             const uint64_t p = h->ss->d.p;
@@ -1595,14 +1706,14 @@ int fdtd_iterate(fdtd_engine* h, uint32_t nsteps) {
                 HIPCHK(hipGetLastError());
                 ++h->numTS;
                 --left;
-                if ((uint64_t)h->numTS == next && (rc = launch_dfts(h, h->numTS))) return rc;
+                if ((uint64_t)h->numTS == next && ((rc = launch_dfts(h, h->numTS)) || (rc = launch_tds(h, h->numTS)))) return rc;
                 continue;
             }
             run = std::min<uint64_t>(run, ev - h->numTS);
         }
         if ((rc = advance(h, (uint32_t)run))) return rc;
         left -= (uint32_t)run;
-        if ((uint64_t)h->numTS == next && (rc = launch_dfts(h, h->numTS))) return rc;
+        if ((uint64_t)h->numTS == next && ((rc = launch_dfts(h, h->numTS)) || (rc = launch_tds(h, h->numTS)))) return rc;
     }
     return FDTD_OK;
 }

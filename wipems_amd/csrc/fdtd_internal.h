@@ -67,6 +67,32 @@ struct DftHost {
     DevBuf acc, dfreq;
 };
 
+// time-domain dump box: the snapshot kernel writes one device buffer, which
+// is copied to one of two pinned host slots; a slot is handed to the sink
+// when it is needed again or when fdtd_iterate returns
+struct TdHost {
+    int field = 0;                 // 0: E, 1: H
+    uint32_t s[3] = {0, 0, 0}, n[3] = {0, 0, 0};
+    uint32_t interval = 0;
+    long long nodes = 0;
+    fdtd_td_sink sink = nullptr;
+    void* user = nullptr;
+    int id = 0;
+    DevBuf buf;                    // float [3][nodes]
+    float* host[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    bool pending[2] = {false, false};
+    uint32_t pts[2] = {0, 0};
+    uint64_t count = 0;
+    ~TdHost() {
+        for (int k = 0; k < 2; ++k) {
+            if (ev[k]) (void)hipEventSynchronize(ev[k]);
+            if (host[k]) (void)hipHostFree(host[k]);
+            if (ev[k]) (void)hipEventDestroy(ev[k]);
+        }
+    }
+};
+
 struct AdeHost {
     uint32_t count = 0;
     uint32_t flags = 0;
@@ -100,6 +126,7 @@ using wipems::MurHost;
 using wipems::RlcHost;
 using wipems::SsHost;
 using wipems::DftHost;
+using wipems::TdHost;
 
 struct fdtd_engine {
     int prec = FDTD_PREC_FP32;
@@ -173,6 +200,8 @@ struct fdtd_engine {
 
     // frequency-domain field boxes
     std::vector<DftHost*> dft;
+    // time-domain dump boxes
+    std::vector<TdHost*> td;
     DevBuf edge_len;              // [2][Nx+Ny+Nz]: primary, then dual edge lengths (metres)
     bool has_edges = false;
 
@@ -215,6 +244,7 @@ struct fdtd_engine {
         delete rlc;
         delete ss;
         for (auto b : dft) delete b;
+        for (auto b : td) delete b;
         if (stream) (void)hipStreamDestroy(stream);
     }
 };

@@ -73,11 +73,16 @@ class Engine:
         self.probe_ids = []
         self.dft_boxes = []
         self._keep = []
+        self._td_callbacks = []
+        self._td_errors = []
+        self._td_dumps = []
 
     def close(self):
         if getattr(self, "_h", None):
             self._lib.fdtd_destroy(self._h)
             self._h = None
+        for d in getattr(self, "_td_dumps", []):
+            d.close()
 
     def __del__(self):
         try:
@@ -201,6 +206,38 @@ class Engine:
         self.dft_boxes.append(b)
         return i.value
 
+    def add_td_box(self, field, start, stop, interval, fn):
+        """Time-domain dump box (fdtd_add_td_box): fn(ts, time, ptr, count) gets
+        every sample, ptr a ctypes float pointer to [3][nx][ny][nz] valid
+        during the call.  Errors raised by fn surface from the next IterateTS."""
+        from .dump import TD_SINK
+
+        def sink(user, bid, ts, t, ptr, n):
+            try:
+                fn(ts, t, ptr, n)
+            except Exception as e:          # ctypes would swallow it
+                self._td_errors.append(e)
+        cb = TD_SINK(sink)
+        self._td_callbacks.append(cb)
+        st = (c_uint32 * 3)(*start)
+        sp = (c_uint32 * 3)(*stop)
+        i = c_int()
+        self._chk(self._lib.fdtd_add_td_box(self._h, int(field), st, sp, int(interval), ctypes.cast(cb, c_void_p), None,
+                                            byref(i)))
+        return i.value
+
+    def add_td_dump(self, path, field, start, stop, interval, lines, unit):
+        """A TD field dump written to the HDF5 file `path` (wipems_amd.dump.TdDump)."""
+        from .dump import TdDump
+        d = TdDump(self, path, field, start, stop, interval, lines, unit)
+        self._td_dumps.append(d)
+        return d
+
+    def td_num_samples(self, bid):
+        n = c_uint64()
+        self._chk(self._lib.fdtd_td_num_samples(self._h, int(bid), byref(n)))
+        return n.value
+
     def dft_num_samples(self, bid):
         n = c_uint64()
         self._chk(self._lib.fdtd_dft_num_samples(self._h, int(bid), byref(n)))
@@ -217,6 +254,9 @@ class Engine:
     # ---- stepping (Engine::IterateTS) ----
     def IterateTS(self, n):
         self._chk(self._lib.fdtd_iterate(self._h, int(n)))
+        if self._td_errors:
+            e, self._td_errors = self._td_errors[0], []
+            raise e
 
     iterate = IterateTS
 
