@@ -557,6 +557,17 @@ k_volt_vec_upml(S* __restrict__ V, const S* __restrict__ I, CoeffPtrs<typename S
     volt_vec_range<S, CM, true, VZ>(V, I, cf, d, sc, ps, pp, r);
 }
 
+// The same with a 5-wave register bound, for FP32/FP64 storage with full
+// coefficient arrays: there it costs at most 4 spilled VGPRs and gains 2-3%
+// (graded 257^3 port, FP32); the index forms spill 12 and lose as much, and
+// 2-byte storage would spill 83, so they keep the unbounded kernel.
+template <typename S, int CM, int VZ>
+__global__ void __launch_bounds__(256, 5)
+k_volt_vec_upml_b5(S* __restrict__ V, const S* __restrict__ I, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
+                   Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp, VecRange r) {
+    volt_vec_range<S, CM, true, VZ>(V, I, cf, d, sc, ps, pp, r);
+}
+
 // The vectors outside the interior box of a UPML shell, numbered
 // (x < x0 planes | x1 <= x < xe planes | interior planes: rows y < y0 or
 // y >= y1 | interior rows: vectors before vlo or from vhi on), each with
@@ -704,6 +715,13 @@ template <typename S, int CM, int VZ>
 __global__ void __launch_bounds__(256)
 k_curr_vec_upml(S* __restrict__ I, const S* __restrict__ V, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
                 Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp, VecRange r) {
+    curr_vec_range<S, CM, true, VZ>(I, V, cf, d, sc, ps, pp, r);
+}
+
+template <typename S, int CM, int VZ>
+__global__ void __launch_bounds__(256, 5)
+k_curr_vec_upml_b5(S* __restrict__ I, const S* __restrict__ V, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
+                   Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp, VecRange r) {
     curr_vec_range<S, CM, true, VZ>(I, V, cf, d, sc, ps, pp, r);
 }
 

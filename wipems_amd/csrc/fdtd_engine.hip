@@ -428,7 +428,9 @@ void launch_volt_main(fdtd_engine* h, const VecLaunch& L, CoeffPtrs<typename Sto
     const dim3 b(256, 1, 1);
     if (L.nx) {
         const dim3 g((unsigned)((L.nq / VZ + 255) / 256), L.nx, 1);
-        if constexpr (PML)
+        if constexpr (PML && CM == CF_FULL && sizeof(S) >= 4)
+            hipLaunchKernelGGL((k_volt_vec_upml_b5<S, CM, VZ>), g, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.r);
+        else if constexpr (PML)
             hipLaunchKernelGGL((k_volt_vec_upml<S, CM, VZ>), g, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.r);
         else
             hipLaunchKernelGGL((k_volt_vec<S, CM, VZ>), g, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.r);
@@ -447,7 +449,9 @@ void launch_curr_main(fdtd_engine* h, const VecLaunch& L, CoeffPtrs<typename Sto
     const dim3 b(256, 1, 1);
     if (L.nx) {
         const dim3 g((unsigned)((L.nq / VZ + 255) / 256), L.nx, 1);
-        if constexpr (PML)
+        if constexpr (PML && CM == CF_FULL && sizeof(S) >= 4)
+            hipLaunchKernelGGL((k_curr_vec_upml_b5<S, CM, VZ>), g, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.r);
+        else if constexpr (PML)
             hipLaunchKernelGGL((k_curr_vec_upml<S, CM, VZ>), g, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.r);
         else
             hipLaunchKernelGGL((k_curr_vec<S, CM, VZ>), g, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.r);
