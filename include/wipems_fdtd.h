@@ -269,9 +269,10 @@ int  fdtd_pml_compressed(fdtd_engine* h, int* active);
  * interior box (as the operator's UPML does), the interior can run in a
  * launch without UPML code and the shell in a launch of its own, instead of
  * one launch whose wavefronts branch into the UPML form.  mode 1 = split,
- * 0 = one launch, -1 = split for FP16/BF16 storage when the interior holds
- * at least 88% of the cells (default: otherwise the shell's scattered
- * vectors cost more than the branching they remove).  Results are identical
+ * 0 = one launch, -1 = split when the interior holds at least 88% of the
+ * cells for FP16/BF16 storage or 95% for FP32, one launch for FP64 (default:
+ * otherwise the shell's scattered vectors cost more than the branching they
+ * remove).  Results are identical
  * either way.  fdtd_pml_split reports whether the split launches are in
  * use. */
 int  fdtd_set_pml_split(fdtd_engine* h, int mode);

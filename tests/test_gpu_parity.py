@@ -371,21 +371,22 @@ def test_three_slabs_in_one_vector_are_rejected(gpu):
 
 
 def test_upml_split_auto_policy(gpu):
-    """Default: split launches only for 2-byte storage on a thin shell
-    (interior >= 88% of the cells)."""
+    """Default: split launches for 2-byte storage when the interior holds
+    >= 88% of the cells, for FP32 when it holds >= 95%, never for FP64."""
     from wipems_amd.operator import BC_PML, Operator
 
-    def domain(pml):
-        lines = np.arange(129, dtype=np.float64)
+    def domain(pml, n=129, prec=0):
+        lines = np.arange(n, dtype=np.float64)
         op = Operator()
         op.set_mesh([lines, lines, lines], 1e-3)
         op.set_boundary([BC_PML] * 6, [pml] * 6)
         op.set_gauss_excite(2e9, 2e9)
-        return op.build(0, nr_ts=4)
+        return op.build(prec, nr_ts=4)
 
     thin, thick = domain(1), domain(8)          # interior 91% / 76% of the cells
+    thinner, thinner64 = domain(1, 257), domain(1, 257, 1)     # interior 95.4%
     for d, precision, want in [(thin, "fp16", True), (thin, "bf16", True), (thin, "fp32", False),
-                               (thick, "fp16", False)]:
+                               (thick, "fp16", False), (thinner, "fp32", True), (thinner64, "fp64", False)]:
         eng = engine_from_operator(d, precision=precision)
         assert eng.pml_split() == want, (precision, want)
         eng.close()
