@@ -12,12 +12,13 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared(header):
     text = open(os.path.join(ROOT, "include", header)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b((?:fdtd|wop|fdtdb|wdump)_[a-z0-9_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b((?:fdtd|wop|fdtdb|wdump|nf2ff)_[a-z0-9_]+)\s*\(", text)))
 
 
 @pytest.mark.parametrize("header,lib", [("wipems_fdtd.h", "wipems_amd/libwipems_fdtd.so"),
                                         ("wipems_operator.h", "wipems_amd/libwipems_operator.so"),
-                                        ("wipems_dump.h", "wipems_amd/libwipems_dump.so")])
+                                        ("wipems_dump.h", "wipems_amd/libwipems_dump.so"),
+                                        ("wipems_nf2ff.h", "wipems_amd/libwipems_fdtd.so")])
 def test_exports(header, lib):
     path = os.path.join(ROOT, lib)
     assert os.path.exists(path), "%s not built" % lib

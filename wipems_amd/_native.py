@@ -64,6 +64,12 @@ _ENGINE_SIGS = {
     "fdtd_set_pml_split": [c_void_p, c_int],
     "fdtd_add_mur": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_uint32],
     "fdtd_set_rlc": [c_void_p, c_uint32, POINTER(c_uint32), POINTER(c_uint32), c_int, c_void_p],
+    "nf2ff_create": [c_double, POINTER(c_double), c_uint32, POINTER(c_double), c_uint32, POINTER(c_double), c_double,
+                     c_double, c_double, POINTER(c_void_p)],
+    "nf2ff_add_plane": [c_void_p, POINTER(c_double), c_uint32, POINTER(c_double), c_uint32, POINTER(c_double), c_uint32,
+                        POINTER(c_double), POINTER(c_double)],
+    "nf2ff_get": [c_void_p] + [POINTER(c_double)] * 7,
+    "nf2ff_destroy": [c_void_p],
     "fdtd_add_td_box": [c_void_p, c_int, POINTER(c_uint32), POINTER(c_uint32), c_uint32, c_void_p, c_void_p, POINTER(c_int)],
     "fdtd_td_num_samples": [c_void_p, c_int, POINTER(c_uint64)],
     "fdtd_set_steady_state": [c_void_p, c_double, c_uint32, POINTER(c_uint32), POINTER(c_uint32)],

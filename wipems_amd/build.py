@@ -27,7 +27,7 @@ HIP_FLAGS = [
     "-I" + INC,
 ]
 
-HIP_SOURCES = ["csrc/fdtd_engine.hip", "csrc/fdtd_batch.hip"]
+HIP_SOURCES = ["csrc/fdtd_engine.hip", "csrc/fdtd_batch.hip", "csrc/nf2ff.hip"]
 
 
 def _run(cmd, cwd=None):
