@@ -1,7 +1,6 @@
 /* wipems_nf2ff.h -- near-field to far-field transform on the GPU.
  *
- * Replaces nf2ff_calc (nf2ff/nf2ff_calc.cpp) for Cartesian meshes without
- * mirror planes:
+ * Replaces nf2ff_calc (nf2ff/nf2ff_calc.cpp) for Cartesian meshes:
  *
  *   nf2ff_create     nf2ff_calc::nf2ff_calc (:164-214): frequency, theta and
  *                    phi lists (rad), centre (m), far-field radius (m, the
@@ -35,6 +34,12 @@ typedef struct nf2ff_calc nf2ff_calc;
 
 int nf2ff_create(double freq, const double* theta, uint32_t n_theta, const double* phi, uint32_t n_phi,
                  const double center[3], double radius, double eps_r, double mue_r, nf2ff_calc** out);
+
+/* nf2ff_calc::SetMirror (:231-245): a symmetry plane at `pos` (m) normal to
+ * `axis`, type 0 off, 1 PEC, 2 PMC.  Later nf2ff_add_plane calls add each
+ * face's images as AddPlane does (:281-328): one plane, two planes (three
+ * images) or all three (seven). */
+int nf2ff_set_mirror(nf2ff_calc* c, int type, int axis, double pos);
 
 /* lines_*: node coordinates in metres; E, H: [3][nx][ny][nz] complex values as
  * (re, im) double pairs (the field-box accumulators, fdtd_read_dft). */

@@ -57,9 +57,58 @@ class NF2FF:
         self.rad_power = 0.0
         self.max_dir = 0.0
 
+    MIRROR_OFF, MIRROR_PEC, MIRROR_PMC = 0, 1, 2
+
+    def set_mirror(self, kind, axis, pos):
+        """nf2ff_calc::SetMirror (:231-245): a PEC or PMC symmetry plane."""
+        if not hasattr(self, "mirror"):
+            self.mirror = [0, 0, 0]
+            self.mirror_pos = [0.0, 0.0, 0.0]
+        self.mirror[axis] = kind
+        self.mirror_pos[axis] = float(pos)
+
+    def _mirror_plane(self, n, lines, E, H):
+        """AddMirrorPlane (:247-279): mirror the lines of axis n about the
+        plane, scale the fields in place, add the image face."""
+        nP, nPP = (n + 1) % 3, (n + 2) % 3
+        lines[n] = 2.0 * self.mirror_pos[n] - lines[n]
+        ef, hf = np.ones(3), np.ones(3)
+        if self.mirror[n] == self.MIRROR_PEC:
+            hf[n] = -1.0
+            ef[nP] = ef[nPP] = -1.0
+        else:
+            ef[n] = -1.0
+            hf[nP] = hf[nPP] = -1.0
+        for d in range(3):
+            E[d] *= ef[d]
+            H[d] *= hf[d]
+        self._single(lines, E, H)
+
     def add_plane(self, lines, E, H):
-        """lines: three arrays of node coordinates (m), one of length 1;
-        E, H: complex [3][n0][n1][n2]."""
+        """AddPlane (:281-328): the face and, with mirror planes set, its
+        images in the reference's order."""
+        lines = [np.array(l, dtype=np.float64) for l in lines]
+        E = np.array(E, dtype=complex)
+        H = np.array(H, dtype=complex)
+        self._single(lines, E, H)
+        m = getattr(self, "mirror", [0, 0, 0])
+        for n in range(3):
+            nP, nPP = (n + 1) % 3, (n + 2) % 3
+            if m[n] and not m[nP] and not m[nPP]:
+                self._mirror_plane(n, lines, E, H)
+                break
+            if not m[n] and m[nP] and m[nPP]:
+                self._mirror_plane(nP, lines, E, H)
+                self._mirror_plane(nPP, lines, E, H)
+                self._mirror_plane(nP, lines, E, H)
+                break
+        if m[0] and m[1] and m[2]:
+            for n in (0, 1, 0, 2, 0, 1, 0):
+                self._mirror_plane(n, lines, E, H)
+
+    def _single(self, lines, E, H):
+        """AddSinglePlane: lines: three arrays of node coordinates (m), one of
+        length 1; E, H: complex [3][n0][n1][n2]."""
         n = [len(l) for l in lines]
         ny = [a for a in range(3) if n[a] == 1 and n[(a + 1) % 3] > 2 and n[(a + 2) % 3] > 2]
         if not ny:
@@ -116,11 +165,11 @@ def box_faces(lo, hi, counts):
     return faces
 
 
-def hertzian_dipole_fields(points, freq, Il=1.0):
-    """Exact E, H of a z-directed infinitesimal dipole of moment I*l at the
-    origin (free space), at points [3][n] (m): complex [3][n] each."""
+def hertzian_dipole_fields(points, freq, Il=1.0, at=(0.0, 0.0, 0.0)):
+    """Exact E, H of a z-directed infinitesimal dipole of moment I*l at `at`
+    (free space), at points [3][n] (m): complex [3][n] each."""
     k = 2 * math.pi * freq / C0
-    x, y, z = points
+    x, y, z = (np.asarray(points, dtype=np.float64) - np.asarray(at, dtype=np.float64)[:, None])
     r = np.sqrt(x * x + y * y + z * z)
     ct = z / r
     st = np.sqrt(np.maximum(0.0, 1 - ct * ct))

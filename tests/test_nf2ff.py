@@ -151,3 +151,104 @@ def test_gpu_nf2ff_from_fdtd_run(gpu):
     assert r["rad_power"] == pytest.approx(orc.rad_power, rel=1e-10)
     assert r["max_dir"] == pytest.approx(1.5, abs=0.02)
     assert r["rad_power"] > 0
+
+
+# ---------------------------------------------------------------------------
+# mirror planes (image theory)
+# ---------------------------------------------------------------------------
+L, P0 = 0.08, (0.02, -0.015, 0.03)
+MIRROR_CASES = {
+    "pec_z": {2: 1},
+    "pmc_z": {2: 2},
+    "pec_x_pmc_y": {0: 1, 1: 2},
+    "pec_xyz": {0: 1, 1: 1, 2: 1},
+}
+
+
+def _images(mirrors):
+    """The z-dipole at P0 and its images: across a plane normal to z a PEC
+    keeps its sign and a PMC flips it; across a plane normal to x or y it is
+    the other way round."""
+    out = [(np.array(P0), 1.0)]
+    for axis, kind in mirrors.items():
+        new = []
+        for p, s in out:
+            q = p.copy()
+            q[axis] = -q[axis]
+            flip = (kind == 2) if axis == 2 else (kind == 1)
+            new.append((q, -s if flip else s))
+        out += new
+    return out
+
+
+def _fields(lines, images):
+    g = np.meshgrid(*lines, indexing="ij")
+    pts = np.stack([a.reshape(-1) for a in g])
+    shape = (3,) + tuple(len(l) for l in lines)
+    E = np.zeros((3, pts.shape[1]), complex)
+    H = np.zeros_like(E)
+    for p, s in images:
+        e, h = hertzian_dipole_fields(pts, F, at=p)
+        E += s * e
+        H += s * h
+    return E.reshape(shape), H.reshape(shape)
+
+
+def _mirror_pair(mirrors, cnt=13):
+    """(half-box faces with mirrors set, full-box faces without): the half
+    box spans [0, L] on mirrored axes and leaves out the faces on the planes."""
+    half_axes = [np.linspace(0, L, cnt) if a in mirrors else np.linspace(-L, L, 2 * cnt - 1) for a in range(3)]
+    full_axes = [np.linspace(-L, L, 2 * cnt - 1) for _ in range(3)]
+    imgs = _images(mirrors)
+
+    def faces(axes, skip_planes):
+        out = []
+        for ny in range(3):
+            for v in (axes[ny][0], axes[ny][-1]):
+                if skip_planes and ny in mirrors and v == 0.0:
+                    continue
+                l = list(axes)
+                l[ny] = np.array([v])
+                out.append((l,) + _fields(l, imgs))
+        return out
+    return faces(half_axes, True), faces(full_axes, False)
+
+
+def _run(nf, faces, mirrors=None):
+    for a, k in (mirrors or {}).items():
+        nf.set_mirror(k, a, 0.0)
+    for l, E, H in faces:
+        nf.add_plane(l, E, H)
+    return nf
+
+
+@pytest.mark.parametrize("case", sorted(MIRROR_CASES))
+def test_oracle_mirror_planes_equal_full_box(case):
+    mirrors = MIRROR_CASES[case]
+    half, full = _mirror_pair(mirrors)
+    theta = np.linspace(0, math.pi, 9)
+    phi = np.linspace(0, 2 * math.pi, 6, endpoint=False)
+    m = _run(OracleNF2FF(F, theta, phi), half, mirrors)
+    f = _run(OracleNF2FF(F, theta, phi), full)
+    s = np.abs(f.E_theta).max() + np.abs(f.E_phi).max()
+    np.testing.assert_allclose(m.E_theta, f.E_theta, rtol=0, atol=1e-9 * s)
+    np.testing.assert_allclose(m.E_phi, f.E_phi, rtol=0, atol=1e-9 * s)
+    assert m.rad_power == pytest.approx(f.rad_power, rel=1e-9)
+    # without the mirrors the half box is a different (open) surface
+    h = _run(OracleNF2FF(F, theta, phi), half)
+    assert np.abs(h.E_theta - f.E_theta).max() > 1e-2 * s
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", sorted(MIRROR_CASES))
+def test_gpu_nf2ff_mirror_planes(gpu, case):
+    from wipems_amd.nf2ff import NF2FF
+    print("case: nf2ff mirrors %s" % case, flush=True)
+    mirrors = MIRROR_CASES[case]
+    half, _ = _mirror_pair(mirrors)
+    theta = np.linspace(0, math.pi, 9)
+    phi = np.linspace(0, 2 * math.pi, 6, endpoint=False)
+    dev = _run(NF2FF(F, theta, phi), half, mirrors)
+    orc = _run(OracleNF2FF(F, theta, phi), half, mirrors)
+    _compare(dev, orc)
+    dev.close()

@@ -70,6 +70,7 @@ _ENGINE_SIGS = {
                         POINTER(c_double), POINTER(c_double)],
     "nf2ff_get": [c_void_p] + [POINTER(c_double)] * 7,
     "nf2ff_destroy": [c_void_p],
+    "nf2ff_set_mirror": [c_void_p, c_int, c_int, c_double],
     "fdtd_add_td_box": [c_void_p, c_int, POINTER(c_uint32), POINTER(c_uint32), c_uint32, c_void_p, c_void_p, POINTER(c_int)],
     "fdtd_td_num_samples": [c_void_p, c_int, POINTER(c_uint64)],
     "fdtd_set_steady_state": [c_void_p, c_double, c_uint32, POINTER(c_uint32), POINTER(c_uint32)],

@@ -88,6 +88,8 @@ struct nf2ff_calc {
     std::vector<std::complex<double>> Et, Ep, Ht, Hp;
     std::vector<double> Prad;
     double rad_power = 0, max_dir = 0;
+    int mirror[3] = {0, 0, 0};      // 0 off, 1 PEC, 2 PMC
+    double mirror_pos[3] = {0, 0, 0};
     hipStream_t stream = nullptr;
     DevBuf dir, out;
     ~nf2ff_calc() {
@@ -130,11 +132,12 @@ int nf2ff_create(double freq, const double* theta, uint32_t n_theta, const doubl
     return FDTD_OK;
 }
 
-int nf2ff_add_plane(nf2ff_calc* c, const double* lines_x, uint32_t nx, const double* lines_y, uint32_t ny,
-                    const double* lines_z, uint32_t nz, const double* E, const double* H) {
-    if (!c || !lines_x || !lines_y || !lines_z || !E || !H) return fail(FDTD_ERR_ARG, "null argument");
-    const uint32_t n[3] = {nx, ny, nz};
-    const double* lines[3] = {lines_x, lines_y, lines_z};
+}  // extern "C"
+
+// nf2ff_calc::AddSinglePlane for one face (lines in metres, fields
+// [3][nx][ny][nz] complex)
+static int add_single(nf2ff_calc* c, const double* const lines[3], const uint32_t n[3], const double* E, const double* H) {
+    const uint32_t nx = n[0], ny = n[1], nz = n[2];
     // nf2ff_calc::GetNormalDir (:216-229): the last axis with one line whose
     // two others have more than two
     int nrm = -1;
@@ -219,6 +222,65 @@ int nf2ff_add_plane(nf2ff_calc* c, const double* lines_x, uint32_t nx, const dou
         }
     }
     c->max_dir = 4 * M_PI * pmax / c->rad_power;
+    return FDTD_OK;
+}
+
+// AddMirrorPlane (:247-279): mirror axis a's lines about the plane, scale the
+// fields (PEC: normal H and tangential E flip; PMC: normal E and tangential
+// H flip) in place, add the image face
+static int add_mirror(nf2ff_calc* c, int a, std::vector<double> (&lines)[3], const uint32_t n[3], std::vector<double>& E,
+                      std::vector<double>& H) {
+    for (auto& v : lines[a]) v = 2.0 * c->mirror_pos[a] - v;
+    double ef[3] = {1, 1, 1}, hf[3] = {1, 1, 1};
+    const int aP = (a + 1) % 3, aPP = (a + 2) % 3;
+    if (c->mirror[a] == 1) { hf[a] = -1; ef[aP] = ef[aPP] = -1; }
+    else { ef[a] = -1; hf[aP] = hf[aPP] = -1; }
+    const size_t nodes = (size_t)n[0] * n[1] * n[2];
+    for (int d = 0; d < 3; ++d)
+        for (size_t i = 0; i < 2 * nodes; ++i) {
+            E[2 * d * nodes + i] *= ef[d];
+            H[2 * d * nodes + i] *= hf[d];
+        }
+    const double* lp[3] = {lines[0].data(), lines[1].data(), lines[2].data()};
+    return add_single(c, lp, n, E.data(), H.data());
+}
+
+extern "C" {
+
+int nf2ff_set_mirror(nf2ff_calc* c, int type, int axis, double pos) {
+    if (!c) return fail(FDTD_ERR_ARG, "null handle");
+    if (axis < 0 || axis > 2) return fail(FDTD_ERR_RANGE, "mirror axis must be 0..2");
+    if (type < 0 || type > 2) return fail(FDTD_ERR_ARG, "mirror type must be 0 (off), 1 (PEC) or 2 (PMC)");
+    c->mirror[axis] = type;
+    c->mirror_pos[axis] = pos;
+    return FDTD_OK;
+}
+
+// AddPlane (:281-328): the face, then its images in the reference's order
+int nf2ff_add_plane(nf2ff_calc* c, const double* lines_x, uint32_t nx, const double* lines_y, uint32_t ny,
+                    const double* lines_z, uint32_t nz, const double* E, const double* H) {
+    if (!c || !lines_x || !lines_y || !lines_z || !E || !H) return fail(FDTD_ERR_ARG, "null argument");
+    const uint32_t n[3] = {nx, ny, nz};
+    const double* lp[3] = {lines_x, lines_y, lines_z};
+    int rc;
+    if ((rc = add_single(c, lp, n, E, H))) return rc;
+    const int* m = c->mirror;
+    if (!m[0] && !m[1] && !m[2]) return FDTD_OK;
+    const size_t vals = 6 * (size_t)nx * ny * nz;
+    std::vector<double> lines[3] = {std::vector<double>(lines_x, lines_x + nx), std::vector<double>(lines_y, lines_y + ny),
+                                    std::vector<double>(lines_z, lines_z + nz)};
+    std::vector<double> e(E, E + vals), h(H, H + vals);
+    for (int a = 0; a < 3; ++a) {
+        const int aP = (a + 1) % 3, aPP = (a + 2) % 3;
+        if (m[a] && !m[aP] && !m[aPP]) return add_mirror(c, a, lines, n, e, h);
+        if (!m[a] && m[aP] && m[aPP]) {
+            if ((rc = add_mirror(c, aP, lines, n, e, h)) || (rc = add_mirror(c, aPP, lines, n, e, h))) return rc;
+            return add_mirror(c, aP, lines, n, e, h);
+        }
+    }
+    if (m[0] && m[1] && m[2])
+        for (int a : {0, 1, 0, 2, 0, 1, 0})
+            if ((rc = add_mirror(c, a, lines, n, e, h))) return rc;
     return FDTD_OK;
 }
 

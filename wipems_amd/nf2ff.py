@@ -1,7 +1,6 @@
 """Near-field to far-field transform on the GPU (include/wipems_nf2ff.h).
 
-``NF2FF`` mirrors nf2ff_calc (nf2ff/nf2ff_calc.cpp) for Cartesian meshes
-without mirror planes; ``far_field`` feeds it the six faces of an nf2ff box
+``NF2FF`` mirrors nf2ff_calc (nf2ff/nf2ff_calc.cpp) for Cartesian meshes; ``far_field`` feeds it the six faces of an nf2ff box
 recorded by an Engine (Operator.nf2ff_box), the data the reference's nf2ff
 reads back from the boxes' HDF5 dumps (python/openEMS/nf2ff.py:79-139).
 """
@@ -31,6 +30,12 @@ class NF2FF:
 
     def _chk(self, rc):
         return check(self._lib, rc, "fdtd_last_error")
+
+    MIRROR_OFF, MIRROR_PEC, MIRROR_PMC = 0, 1, 2
+
+    def set_mirror(self, kind, axis, pos):
+        """nf2ff_calc::SetMirror: a PEC (1) or PMC (2) symmetry plane at pos (m)."""
+        self._chk(self._lib.nf2ff_set_mirror(self._h, int(kind), int(axis), float(pos)))
 
     def add_plane(self, lines, E, H):
         """lines: node coordinates (m) per axis, one axis with a single line;
