@@ -187,3 +187,49 @@ def test_gpu_td_sink_error_surfaces(gpu):
     eng.add_td_box(0, (0, 0, 0), (2, 2, 2), 3, bad)
     with pytest.raises(ValueError, match="sink failed"):
         eng.iterate(4)
+
+
+@pytest.mark.gpu
+def test_gpu_iterate_splits_combined(gpu):
+    """Steady-state evaluations (a step launched on its own every period),
+    frequency-domain samples and time-domain dump samples on one run, with
+    chunks that straddle all three: fields bitwise, the steady-state value,
+    FD accumulators and TD samples as the oracle gives them."""
+    from oracle.dft import DftBox, interp_e, run_with_boxes
+    from oracle.oracle import oracle_from_operator
+    from tests.test_steady_state import line_op
+    from wipems_amd.engine import engine_from_operator
+    from wipems_amd.operator import DftBoxDef
+    print("case: steady state + FD + TD boxes on one run", flush=True)
+    op = line_op(5e9, nx=60)
+    d = op.build(0)
+    d.dft_boxes = [DftBoxDef(0, (10, 0, 0), (40, 4, 4), (5e9,), 3, "e")]
+    eng = engine_from_operator(d, precision="fp32")
+    period = eng.steady_state_period()
+    td = {}
+
+    def keep(ts, t, ptr, n):
+        td[ts] = np.ctypeslib.as_array(ptr, shape=(int(n),)).reshape((3, 21, 5, 5)).copy()
+    eng.add_td_box(0, (20, 0, 0), (40, 4, 4), 7, keep)
+    o = oracle_from_operator(d)
+    o.set_steady_state(d.steady_state)
+    boxes = [DftBox(0, (10, 0, 0), (40, 4, 4), (5e9,), 3, d.edge_primary, d.edge_dual, d.num_lines)]
+    N = d.num_lines
+    lens = [d.edge_primary[sum(N[:n]):sum(N[:n + 1])] for n in range(3)]
+    sampled0 = False
+    ts = 0
+    for c in (5, period - 3, 1, 2 * period + 7, 16, 33, period):
+        eng.iterate(c)
+        for _ in range(c):
+            run_with_boxes(o, boxes, 1, d.dT, sampled0)
+            sampled0 = True
+            ts += 1
+            if ts % 7 == 0:
+                ref = interp_e(o.get_field(0), lens, (20, 0, 0), (40, 4, 4)).astype(np.float32)
+                np.testing.assert_array_equal(td[ts], ref)
+        assert eng.GetLastDiff() == pytest.approx(o.GetLastDiff(), rel=1e-9, abs=2e-12)
+    for which in (0, 1):
+        np.testing.assert_array_equal(eng.get_field(which), o.get_field(which))
+    acc = eng.read_dft(0)
+    np.testing.assert_allclose(acc, boxes[0].acc, rtol=0, atol=1e-11 * np.abs(boxes[0].acc).max())
+    assert sorted(td) == list(range(0, ts + 1, 7))
