@@ -268,13 +268,16 @@ int  fdtd_pml_compressed(fdtd_engine* h, int* active);
 /* UPML shell launch (vector kernels): when the slabs form a shell around an
  * interior box (as the operator's UPML does), the interior can run in a
  * launch without UPML code and the shell in a launch of its own, instead of
- * one launch whose wavefronts branch into the UPML form.  mode 1 = split,
- * 0 = one launch, -1 = split when the interior holds at least 88% of the
- * cells for FP16/BF16 storage or 95% for FP32, one launch for FP64 (default:
- * otherwise the shell's scattered vectors cost more than the branching they
- * remove).  Results are identical
- * either way.  fdtd_pml_split reports whether the split launches are in
- * use. */
+ * one launch whose wavefronts branch into the UPML form.  mode 0 = one
+ * launch; 1 = split, the shell launch holding the x-slab planes, the y-slab
+ * rows and the z-slab ends of the interior rows; 2 = split in rows form: the
+ * interior launch sweeps the interior rows in full and applies the z slabs
+ * at their two ends itself (only vectors reaching out of the interior z
+ * range search the slabs), the shell launch holds the x-slab planes and
+ * y-slab rows only, all contiguous runs of memory; -1 = the default (split
+ * when the interior holds at least 88% of the cells for FP16/BF16 storage
+ * or 95% for FP32, one launch for FP64).  Results are identical in every
+ * form.  fdtd_pml_split reports whether a split form is in use. */
 int  fdtd_set_pml_split(fdtd_engine* h, int mode);
 int  fdtd_pml_split(fdtd_engine* h, int* active);
 /* Use a captured HIP graph for the step loop (default on). */
@@ -289,6 +292,11 @@ int  fdtd_get_kernel_times(fdtd_engine* h, double* volt_ms, double* curr_ms,
 
 /* ---- field access ---- */
 int  fdtd_calc_energy(fdtd_engine* h, double* energy);
+/* Per-component max |value| of the volt (FDTD_VOLT) or curr field in
+ * physical units, reduced on the device.  No counterpart in the reference:
+ * the storage-scale pilot of FP16/BF16 handles reads it instead of copying
+ * whole fields to the host. */
+int  fdtd_field_absmax(fdtd_engine* h, int which, double out[3]);
 int  fdtd_get_field(fdtd_engine* h, int which, int dtype, void* out);
 int  fdtd_set_field(fdtd_engine* h, int which, int dtype, const void* in);
 int  fdtd_get_value(fdtd_engine* h, int which, uint32_t n, uint32_t x, uint32_t y, uint32_t z, double* v);

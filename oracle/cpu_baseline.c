@@ -89,6 +89,19 @@ static void assign_jobs(uint32_t jobs, uint32_t nthreads, uint32_t* jpt) {
         if (jobs > 0) { ++jpt[n]; --jobs; }
 }
 
+typedef struct { cpub* e; uint32_t x0, x1; } zjob_t;
+
+static void* zero_worker(void* arg) {
+    zjob_t* j = (zjob_t*)arg;
+    cpub* e = j->e;
+    const size_t a = (size_t)j->x0 * e->N[1] * e->Nv, b = (size_t)j->x1 * e->N[1] * e->Nv;
+    for (int n = 0; n < 3; ++n) {
+        memset(e->V + n * e->nrow + a, 0, (b - a) * 16);
+        memset(e->I + n * e->nrow + a, 0, (b - a) * 16);
+    }
+    return NULL;
+}
+
 cpub* cpub_create(const uint32_t N[3], int nthreads) {
     cpub* e = (cpub*)calloc(1, sizeof(cpub));
     memcpy(e->N, N, 12);
@@ -96,8 +109,6 @@ cpub* cpub_create(const uint32_t N[3], int nthreads) {
     e->nrow = (size_t)N[0] * N[1] * e->Nv;
     e->V = (v4sf*)aligned_alloc(64, (3 * e->nrow * 16 + 63) / 64 * 64);
     e->I = (v4sf*)aligned_alloc(64, (3 * e->nrow * 16 + 63) / 64 * 64);
-    memset(e->V, 0, 3 * e->nrow * 16);
-    memset(e->I, 0, 3 * e->nrow * 16);
     if (nthreads < 1) nthreads = 1;
     if (nthreads > 256) nthreads = 256;
     /* Operator_Multithread::CalcStartStopLines (FDTD/operator_multithread.cpp:89-106) */
@@ -111,6 +122,14 @@ cpub* cpub_create(const uint32_t N[3], int nthreads) {
     for (uint32_t n = 1; n < nt; ++n) { e->tstart[n] = e->tstop[n - 1] + 1; e->tstop[n] = e->tstart[n] + jpt[n] - 1; }
     for (uint32_t n = 0; n < nt; ++n) e->tstop_h[n] = (n == nt - 1) ? e->tstop[n] - 1 : e->tstop[n];
     pthread_barrier_init(&e->bar, NULL, nt);
+    /* zero the fields from the threads that update them (first touch) */
+    zjob_t zj[256];
+    pthread_t th[256];
+    for (uint32_t t = 0; t < nt; ++t) {
+        zj[t].e = e; zj[t].x0 = e->tstart[t]; zj[t].x1 = e->tstop[t] + 1;
+        pthread_create(&th[t], NULL, zero_worker, &zj[t]);
+    }
+    for (uint32_t t = 0; t < nt; ++t) pthread_join(th[t], NULL);
     return e;
 }
 
@@ -184,6 +203,139 @@ int cpub_set_coeffs(cpub* e, const float* vv, const float* vi, const float* ii, 
             for (size_t u = 0; u < nu; ++u) memcpy(&e->tab[k][n][u], &uniq[u].v[(k * 3 + n) * 4], 16);
         }
     free(uniq);
+    return (int)nu;
+}
+
+/* The same 4-lane compression from a per-cell compressed operator (index
+ * [Nx][Ny][Nz] into rows of 12 values {vv0..2, vi0..2, ii0..2, iv0..2}, as
+ * the streamed build gives it).  Per-cell rows are bytewise distinct, so two
+ * 4-lane tuples are equal exactly when their four cell indices are; padding
+ * lanes (z >= Nz) hold zeros and get index n_entries.  The 4-tuples are
+ * deduplicated per thread over x slabs and merged; the numbering follows
+ * the merge, which changes no arithmetic. */
+typedef struct { uint32_t c[4]; } lanes4_t;
+
+static uint64_t hash_quad(const lanes4_t* q) {
+    uint64_t h = 1469598103934665603ull;
+    for (int i = 0; i < 4; ++i) { h ^= q->c[i]; h *= 1099511628211ull; }
+    return h ^ (h >> 29);
+}
+
+typedef struct {
+    lanes4_t* keys; uint32_t* vals; size_t cap, n;   /* open addressing, vals = local id */
+    lanes4_t* list;  size_t lcap;                    /* local id -> quad */
+} qset_t;
+
+static void qset_init(qset_t* q, size_t cap) {
+    q->cap = cap; q->n = 0;
+    q->keys = (lanes4_t*)malloc(cap * sizeof(lanes4_t));
+    q->vals = (uint32_t*)malloc(cap * sizeof(uint32_t));
+    for (size_t i = 0; i < cap; ++i) q->vals[i] = UINT32_MAX;
+    q->lcap = 1024;
+    q->list = (lanes4_t*)malloc(q->lcap * sizeof(lanes4_t));
+}
+
+static void qset_free(qset_t* q) { free(q->keys); free(q->vals); free(q->list); }
+
+static uint32_t qset_get(qset_t* q, const lanes4_t* k) {
+    if (2 * (q->n + 1) > q->cap) {   /* grow */
+        qset_t g;
+        qset_init(&g, 2 * q->cap);
+        free(g.list);
+        g.list = q->list; g.lcap = q->lcap; g.n = q->n;
+        for (size_t i = 0; i < q->cap; ++i)
+            if (q->vals[i] != UINT32_MAX) {
+                size_t s = hash_quad(&q->keys[i]) & (g.cap - 1);
+                while (g.vals[s] != UINT32_MAX) s = (s + 1) & (g.cap - 1);
+                g.keys[s] = q->keys[i]; g.vals[s] = q->vals[i];
+            }
+        free(q->keys); free(q->vals);
+        *q = g;
+    }
+    size_t s = hash_quad(k) & (q->cap - 1);
+    while (q->vals[s] != UINT32_MAX) {
+        if (memcmp(&q->keys[s], k, sizeof(lanes4_t)) == 0) return q->vals[s];
+        s = (s + 1) & (q->cap - 1);
+    }
+    if (q->n == q->lcap) { q->lcap *= 2; q->list = (lanes4_t*)realloc(q->list, q->lcap * sizeof(lanes4_t)); }
+    q->keys[s] = *k; q->vals[s] = (uint32_t)q->n; q->list[q->n] = *k;
+    return (uint32_t)q->n++;
+}
+
+typedef struct { cpub* e; const uint32_t* cell; uint32_t pad_id, x0, x1; qset_t set; } qjob_t;
+
+static void* quad_worker(void* arg) {
+    qjob_t* j = (qjob_t*)arg;
+    cpub* e = j->e;
+    qset_init(&j->set, 4096);
+    lanes4_t q;
+    for (uint32_t x = j->x0; x < j->x1; ++x)
+        for (uint32_t y = 0; y < e->N[1]; ++y) {
+            const uint32_t* c = j->cell + ((size_t)x * e->N[1] + y) * e->N[2];
+            for (uint32_t zv = 0; zv < e->Nv; ++zv) {
+                for (int l = 0; l < 4; ++l) {
+                    const uint32_t z = zv + l * e->Nv;
+                    q.c[l] = z < e->N[2] ? c[z] : j->pad_id;
+                }
+                e->idx[((size_t)x * e->N[1] + y) * e->Nv + zv] = qset_get(&j->set, &q);
+            }
+        }
+    return NULL;
+}
+
+typedef struct { cpub* e; uint32_t x0, x1; const uint32_t* remap; } rjob_t;
+
+static void* remap_worker(void* arg) {
+    rjob_t* j = (rjob_t*)arg;
+    cpub* e = j->e;
+    for (size_t p = (size_t)j->x0 * e->N[1] * e->Nv; p < (size_t)j->x1 * e->N[1] * e->Nv; ++p) e->idx[p] = j->remap[e->idx[p]];
+    return NULL;
+}
+
+int cpub_set_coeffs_compressed(cpub* e, const uint32_t* cell, const double* table, uint32_t n_entries) {
+    const int nt = e->nthreads;
+    uint32_t jpt[256];
+    assign_jobs(e->N[0], (uint32_t)nt, jpt);
+    e->idx = (uint32_t*)malloc(e->nrow * sizeof(uint32_t));
+    qjob_t* jobs = (qjob_t*)calloc((size_t)nt, sizeof(qjob_t));
+    pthread_t th[256];
+    uint32_t x = 0;
+    for (int t = 0; t < nt; ++t) {
+        jobs[t].e = e; jobs[t].cell = cell; jobs[t].pad_id = n_entries; jobs[t].x0 = x; jobs[t].x1 = x + jpt[t]; x += jpt[t];
+        pthread_create(&th[t], NULL, quad_worker, &jobs[t]);
+    }
+    for (int t = 0; t < nt; ++t) pthread_join(th[t], NULL);
+    /* merge: global ids in thread order, then local -> global per thread */
+    qset_t g;
+    qset_init(&g, 4096);
+    uint32_t* remaps[256];
+    for (int t = 0; t < nt; ++t) {
+        remaps[t] = (uint32_t*)malloc((jobs[t].set.n ? jobs[t].set.n : 1) * sizeof(uint32_t));
+        for (size_t i = 0; i < jobs[t].set.n; ++i) remaps[t][i] = qset_get(&g, &jobs[t].set.list[i]);
+    }
+    rjob_t rj[256];
+    for (int t = 0; t < nt; ++t) {
+        rj[t].e = e; rj[t].x0 = jobs[t].x0; rj[t].x1 = jobs[t].x1; rj[t].remap = remaps[t];
+        pthread_create(&th[t], NULL, remap_worker, &rj[t]);
+    }
+    for (int t = 0; t < nt; ++t) pthread_join(th[t], NULL);
+    const size_t nu = g.n;
+    e->U = (uint32_t)nu;
+    static const int col[4] = {0, 3, 9, 6};   /* tab order vv, vi, iv, ii -> table columns */
+    for (int k = 0; k < 4; ++k)
+        for (int n = 0; n < 3; ++n) {
+            e->tab[k][n] = (v4sf*)aligned_alloc(16, (nu ? nu : 1) * 16);
+            for (size_t u = 0; u < nu; ++u) {
+                float* o = (float*)&e->tab[k][n][u];
+                for (int l = 0; l < 4; ++l) {
+                    const uint32_t c = g.list[u].c[l];
+                    o[l] = c < n_entries ? (float)table[(size_t)c * 12 + col[k] + n] : 0.0f;
+                }
+            }
+        }
+    for (int t = 0; t < nt; ++t) { qset_free(&jobs[t].set); free(remaps[t]); }
+    qset_free(&g);
+    free(jobs);
     return (int)nu;
 }
 

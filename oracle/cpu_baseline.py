@@ -26,6 +26,7 @@ def lib():
         L.cpub_create.argtypes = [POINTER(c_uint32), c_int]
         L.cpub_destroy.argtypes = [c_void_p]
         L.cpub_set_coeffs.argtypes = [c_void_p] + [POINTER(ctypes.c_float)] * 4
+        L.cpub_set_coeffs_compressed.argtypes = [c_void_p, POINTER(c_uint32), POINTER(c_double), c_uint32]
         L.cpub_set_excitation.argtypes = [c_void_p, c_uint32, c_uint32, POINTER(c_double), c_uint32, POINTER(c_uint32),
                                           POINTER(c_uint32), POINTER(c_double), POINTER(c_uint32)]
         L.cpub_add_upml.argtypes = [c_void_p, POINTER(c_uint32), POINTER(c_uint32)] + [POINTER(ctypes.c_float)] * 6
@@ -101,8 +102,14 @@ class CpuBaseline:
         n = (c_uint32 * 3)(*data.num_lines)
         self._h = self._L.cpub_create(n, int(nthreads))
         self.num_lines = tuple(data.num_lines)
-        arrs = [np.ascontiguousarray(a, dtype=np.float32) for a in (data.vv, data.vi, data.ii, data.iv)]
-        self.unique = self._L.cpub_set_coeffs(self._h, *[_fp(a) for a in arrs])
+        if data.vv is None:
+            # streamed build: per-cell index + table, compressed to 4-lane tuples here
+            idx = np.ascontiguousarray(data.index, dtype=np.uint32)
+            tab = np.ascontiguousarray(data.table, dtype=np.float64)
+            self.unique = self._L.cpub_set_coeffs_compressed(self._h, _up(idx), _dp(tab), int(tab.shape[0]))
+        else:
+            arrs = [np.ascontiguousarray(a, dtype=np.float32) for a in (data.vv, data.vi, data.ii, data.iv)]
+            self.unique = self._L.cpub_set_coeffs(self._h, *[_fp(a) for a in arrs])
         e = data.excitation
         if e is not None and e.length:
             sv = np.ascontiguousarray(e.signal_volt, dtype=np.float64)

@@ -97,13 +97,14 @@ def test_graded_upml_port_fp32_compressed(gpu):
     compare(eng, orc)
 
 
-@pytest.mark.parametrize("variant,tile_y,chunk", [(0, 4, 64), (1, 1, 7), (1, 2, 16), (1, 4, 64), (1, 4, 1000), (2, 4, 1), (2, 2, 3), (2, 4, 8), (2, 1, 64), (3, 1, 1), (3, 2, 1), (3, 4, 1)])
-def test_kernel_shapes_bitwise(gpu, variant, tile_y, chunk):
-    """Every kernel shape gives the same bits (graded mesh, UPML, port)."""
+@pytest.mark.parametrize("split", [0, 1, 2])
+def test_kernel_shapes_bitwise(gpu, split):
+    """One UPML launch, the interior/shell split and its rows form give the
+    oracle's bits (graded mesh, UPML, port)."""
     sc = scenarios.graded_port_scenario(n_cells=36, steps=150, prec=0, n_boxes=5, pml=8)
     d = sc.data
     eng = engine_from_operator(d, precision="fp32", compressed=False)
-    eng.set_tuning(variant, tile_y, chunk)
+    eng.set_pml_split(split)
     _kinds[id(eng)] = [p.kind for p in d.probes]
     orc = oracle_from_operator(d, ftz=True)
     eng.iterate(150)
@@ -127,16 +128,14 @@ def test_cavity64_matches_golden_fixture(gpu):
     assert eng.energy() == pytest.approx(float(g["energy"]), rel=1e-12)
 
 
-@pytest.mark.parametrize("variant", [0, 3])
 @pytest.mark.parametrize("n_cells", [21, 30, 33])
-def test_vector_kernel_odd_sizes(gpu, variant, n_cells):
+def test_vector_kernel_odd_sizes(gpu, n_cells):
     """Vector kernels with Nz not a multiple of the vector width, compressed
     and full coefficients, against the oracle."""
     sc = scenarios.graded_port_scenario(n_cells=n_cells, steps=120, prec=0, n_boxes=4, pml=5)
     d = sc.data
     for compressed in (False, True):
         eng = engine_from_operator(d, precision="fp32", compressed=compressed)
-        eng.set_tuning(variant, 4, 1)
         _kinds[id(eng)] = [p.kind for p in d.probes]
         orc = oracle_from_operator(d, ftz=True)
         eng.iterate(120)
@@ -147,7 +146,6 @@ def test_vector_kernel_odd_sizes(gpu, variant, n_cells):
 def test_vector_kernel_fp64(gpu):
     sc = scenarios.graded_port_scenario(n_cells=27, steps=100, prec=1, n_boxes=4, pml=5)
     eng = engine_from_operator(sc.data, precision="fp64", compressed=False)
-    eng.set_tuning(3, 2, 1)
     _kinds[id(eng)] = [p.kind for p in sc.data.probes]
     orc = oracle_from_operator(sc.data)
     eng.iterate(100)
@@ -241,11 +239,10 @@ def _h_source_cavity(n=(30, 26, 40), steps=200, prec=0):
 
 @pytest.mark.parametrize("precision,prec", [("fp32", 0), ("fp64", 1), ("fp16", 0), ("bf16", 0)])
 @pytest.mark.parametrize("case", ["cavity_odd", "cavity_wide", "port_pec", "hsource"])
-@pytest.mark.parametrize("seg", [32, 5])
-def test_fused_step_matches_two_launches(gpu, precision, prec, case, seg):
-    """The fused one-launch step (x-march tiles, ping-pong buffers) gives the
-    bits of the two-launch vector kernels, and of the oracle for FP32/FP64;
-    uneven chunks cross the 16-step graph and both buffer parities."""
+def test_graph_chunks_match_eager(gpu, precision, prec, case):
+    """Uneven iterate() chunks (crossing the 16-step graph, the 1-step graph
+    and the eager path) give the bits of one eager step at a time, and of
+    the oracle for FP32/FP64."""
     if case == "cavity_odd":
         sc = scenarios.cavity_point_source(n_cells=36, steps=120, prec=prec)
     elif case == "cavity_wide":
@@ -268,16 +265,16 @@ def test_fused_step_matches_two_launches(gpu, precision, prec, case, seg):
         sc = _h_source_cavity(prec=prec)
     d = sc.data
     assert not d.upml
-    print("case: fused %s %s seg %d" % (precision, case, seg), flush=True)
+    print("case: graph chunks %s %s" % (precision, case), flush=True)
     chunks = (1, 16, 3, 16, 33, 51)
     out = []
-    for variant in (4, 3):
+    for graph in (True, False):
         eng = engine_from_operator(d, precision=precision, compressed=None)
-        eng.set_tuning(variant, 4, seg if variant == 4 else 1)
+        eng.set_use_graph(graph)
         for c in chunks:
             eng.iterate(c)
         out.append((eng.get_field(0), eng.get_field(1), [eng.read_probe(p)[1] for p in eng.probe_ids]))
-        if variant == 4 and precision in ("fp32", "fp64"):
+        if graph and precision in ("fp32", "fp64"):
             _kinds[id(eng)] = [p.kind for p in d.probes]
             orc = oracle_from_operator(d, ftz=True)
             orc.iterate(sum(chunks))
@@ -313,11 +310,13 @@ def _short_z_upml(prec, nz_lines=20, pml=8):
 
 @pytest.mark.parametrize("precision,prec", [("fp32", 0), ("fp64", 1)])
 @pytest.mark.parametrize("compressed", [False, True])
-def test_upml_shell_launch_short_z_bitwise(gpu, precision, prec, compressed):
+@pytest.mark.parametrize("split", [1, 2])
+def test_upml_shell_launch_short_z_bitwise(gpu, precision, prec, compressed, split):
     d = _short_z_upml(prec)
     assert len(d.upml) == 6
-    print("case: short-z UPML shell, %s, compressed %s" % (precision, compressed), flush=True)
+    print("case: short-z UPML shell, %s, compressed %s, split %d" % (precision, compressed, split), flush=True)
     eng = engine_from_operator(d, precision=precision, compressed=compressed)
+    eng.set_pml_split(split)
     _kinds[id(eng)] = [p.kind for p in d.probes]
     orc = oracle_from_operator(d, ftz=True)
     eng.iterate(150)
@@ -328,8 +327,9 @@ def test_upml_shell_launch_short_z_bitwise(gpu, precision, prec, compressed):
 @pytest.mark.parametrize("precision,prec", [("fp32", 0), ("fp64", 1), ("fp16", 0), ("bf16", 0)])
 @pytest.mark.parametrize("case", ["short_z", "graded_port"])
 def test_upml_split_launches_equal_one_launch(gpu, precision, prec, case):
-    """Interior + shell launches give the same bits as one launch with the
-    UPML branch, for every storage type (FP16/BF16 have no bitwise oracle,
+    """Interior + shell launches (z slabs in the shell launch, or in the
+    interior rows) give the same bits as one launch with the UPML branch,
+    for every storage type (FP16/BF16 have no bitwise oracle,
     so this is their check of the shell launch)."""
     if case == "short_z":
         d = _short_z_upml(prec)
@@ -337,7 +337,7 @@ def test_upml_split_launches_equal_one_launch(gpu, precision, prec, case):
         d = scenarios.graded_port_scenario(n_cells=34, steps=150, prec=prec, n_boxes=5, pml=7).data
     print("case: UPML split vs one launch, %s, %s" % (case, precision), flush=True)
     out = []
-    for mode in (0, 1):
+    for mode in (0, 1, 2):
         eng = engine_from_operator(d, precision=precision)
         eng.set_pml_split(mode)
         assert eng.pml_split() == bool(mode)
@@ -345,8 +345,9 @@ def test_upml_split_launches_equal_one_launch(gpu, precision, prec, case):
         eng.synchronize()
         out.append([eng.get_field(w) for w in (0, 1)] + [eng.read_probe(eng.probe_ids[0])[1]])
         eng.close()
-    for a, b in zip(out[0], out[1]):
-        np.testing.assert_array_equal(a, b)
+    for other in out[1:]:
+        for a, b in zip(out[0], other):
+            np.testing.assert_array_equal(a, b)
     assert np.abs(out[0][0]).max() > 0
 
 

@@ -330,3 +330,23 @@ def test_upml_reflection_below_minus_40dB():
     # the same small box with PEC walls reflects strongly: the check has teeth
     _, vp = run(n, 0)
     assert np.linalg.norm(vp - vb) / np.linalg.norm(vb) > 0.1
+
+
+def test_oracle_equals_cpu_baseline_streamed_dipole():
+    """The CPU baseline fed the streamed build's per-cell index and table (its
+    own 4-lane compression of them) steps the dipole workload bit for bit
+    like the oracle on the expanded coefficient arrays."""
+    from oracle.cpu_baseline import CpuBaseline
+    sc = scenarios.large_dipole_scenario(num_lines=(29, 27, 23), n_boxes=40, steps=120, pml=4, box_margin=2,
+                                         dipole_cells=6, threads=2)
+    d = sc.data
+    assert d.vv is None and d.index is not None
+    full = d.expanded()
+    o = oracle_from_operator(full, ftz=True)
+    o.iterate(120)
+    b = CpuBaseline(d, 3)
+    b.run(120, sc.meta["interval"])
+    for w in (0, 1):
+        np.testing.assert_array_equal(o.get_field(w), b.get_field(w))
+    np.testing.assert_array_equal(o.read_probe(0)[1], b.samples(0))
+    assert np.abs(o.get_field(0)).max() > 0

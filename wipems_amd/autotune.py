@@ -9,25 +9,7 @@ from __future__ import annotations
 
 from .engine import Batch, engine_from_operator
 
-SINGLE_CANDIDATES = [(3, 1, 1), (0, 4, 1), (0, 2, 1), (2, 4, 4), (2, 4, 16)]
 BATCH_CANDIDATES = [64, 128, 256]
-
-
-def tune_engine(data, precision="fp32", compressed=None, scales=None, steps=20, candidates=SINGLE_CANDIDATES):
-    """Best (variant, tile_y, chunk_x) for one domain and the timings [ms/step]."""
-    eng = engine_from_operator(data, precision=precision, compressed=compressed, scales=scales)
-    results = {}
-    for cand in candidates:
-        eng.set_tuning(*cand)
-        eng.iterate(2)
-        eng.set_kernel_timing(True)
-        eng.iterate(steps)
-        tv, tc, nv, nc = eng.kernel_times()
-        eng.set_kernel_timing(False)
-        results[cand] = (tv + tc) / max(nv, 1)
-    eng.close()
-    best = min(results, key=results.get)
-    return best, results
 
 
 def tune_batch(datas, precision="fp32", compressed=None, scales=None, steps=10, candidates=BATCH_CANDIDATES):

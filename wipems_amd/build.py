@@ -39,11 +39,21 @@ def _run(cmd, cwd=None):
 
 
 def build_engine(verbose=False):
+    """Each HIP source is compiled to its own object in parallel (no
+    relocatable device code: every translation unit holds its own kernels),
+    then linked into one shared library."""
+    from concurrent.futures import ThreadPoolExecutor
+    import tempfile
     hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
     srcs = [os.path.join(PKG, s) for s in HIP_SOURCES if os.path.exists(os.path.join(PKG, s))]
     out = os.path.join(PKG, "libwipems_fdtd.so")
     tmp = out + ".tmp"
-    _run([hipcc] + HIP_FLAGS + ["-o", tmp] + srcs)
+    cflags = [f for f in HIP_FLAGS if f != "-shared"]
+    with tempfile.TemporaryDirectory() as td:
+        objs = [os.path.join(td, os.path.basename(src) + ".o") for src in srcs]
+        with ThreadPoolExecutor(max_workers=len(srcs)) as ex:
+            list(ex.map(lambda so: _run([hipcc] + cflags + ["-c", "-o", so[1], so[0]]), zip(srcs, objs)))
+        _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, out)
     if verbose:
         print("built", out)

@@ -402,9 +402,17 @@ int fdtd_batch_get_kernel_times(fdtd_batch* b, double* volt_ms, double* curr_ms,
 int fdtd_batch_iterate(fdtd_batch* b, uint32_t nsteps) {
     if (!b) return fail(FDTD_ERR_ARG, "null batch");
     int rc;
+    for (auto h : b->eng)
+        if (h->numTS != b->numTS) return fail(FDTD_ERR_STATE, "an engine of the batch was stepped on its own");
+    // Ordering rule: the probe buffers and the domain table are read by
+    // kernels already queued on b->stream, while finalize_probes copies and
+    // frees old probe buffers on each member's own stream and batch_tables
+    // replaces the table.  Drain b->stream before either can happen.
+    bool grow = b->dirty;
+    for (auto h : b->eng) grow |= probes_need_realloc(h, b->numTS + nsteps);
+    if (grow) HIPCHK(hipStreamSynchronize(b->stream));
     // probe buffers sized for the new horizon (may move them: rebuild tables)
     for (auto h : b->eng) {
-        if (h->numTS != b->numTS) return fail(FDTD_ERR_STATE, "an engine of the batch was stepped on its own");
         const bool before = h->probes_dirty;
         std::vector<double*> old;
         for (auto p : h->probes) old.push_back(p->td.as<double>());

@@ -104,18 +104,6 @@ struct CoeffPtrs {
     const void* idx;       // per-cell index (compressed modes)
 };
 
-__device__ __forceinline__ int find_slab(const PmlSet& ps, int x, int y, int z, long long& local) {
-    for (int s = 0; s < ps.count; ++s) {
-        const PmlSlab& b = ps.slab[s];
-        int lx = x - b.s[0], ly = y - b.s[1], lz = z - b.s[2];
-        if ((unsigned)lx < (unsigned)b.n[0] && (unsigned)ly < (unsigned)b.n[1] && (unsigned)lz < (unsigned)b.n[2]) {
-            local = b.off + ((long long)lx * b.n[1] + ly) * b.nzp + (z - b.za);
-            return s;
-        }
-    }
-    return -1;
-}
-
 // cells z0..z0+vz-1 at (x,y) that lie in slab b: mask bit k = cell z0+k;
 // base = slab-array offset of z0 (aligned like z0).  Returns the mask.
 __device__ __forceinline__ unsigned slab_lanes(const PmlSlab& b, int x, int y, int z0, int vz, long long& base) {
@@ -128,161 +116,15 @@ __device__ __forceinline__ unsigned slab_lanes(const PmlSlab& b, int x, int y, i
     return m;
 }
 
-template <typename A, int CM>
-__device__ __forceinline__ void load_coeffs(const CoeffPtrs<A>& cf, long long p, long long sc, A a[3], A b[3]) {
-    if (CM == CF_FULL) {
-#pragma unroll
-        for (int n = 0; n < 3; ++n) { a[n] = cf.a[n * sc + p]; b[n] = cf.b[n * sc + p]; }
-    } else {
-        unsigned c = (CM == CF_IDX16) ? (unsigned)((const unsigned short*)cf.idx)[p]
-                                      : ((const unsigned*)cf.idx)[p];
-        const A* row = cf.a + (size_t)c * 8;
-#pragma unroll
-        for (int n = 0; n < 3; ++n) { a[n] = row[n]; b[n] = row[4 + n]; }
-    }
-}
-
 // ---------------------------------------------------------------------------
-// main voltage update (Engine::UpdateVoltages, FDTD/engine.cpp:108-161, with the
-// z-boundary of Engine_SSE_Compressed, FDTD/engine_sse_compressed.cpp:95-153:
-// the z-1 neighbour at z=0 is a zero-filled lane) with UPML Pre/Post fused
-// (FDTD/extensions/engine_ext_upml.cpp:57-135):
+// Main updates (Engine_SSE_Compressed::UpdateVoltages/UpdateCurrents,
+// FDTD/engine_sse_compressed.cpp:48-263, in the operation order of
+// FDTD/engine.cpp:108-210; the z-1 neighbour at z=0 is a zero-filled lane,
+// :95-153) with UPML Pre/Post fused per cell (engine_ext_upml.cpp:57-234):
 //   pre : h = pv*V - fo*F ; V = F ; F = h
 //   main: V = V*vv + vi*curl
 //   post: h = F ; F = V ; V = h + fn*F
-// ---------------------------------------------------------------------------
-template <typename S, int CM, bool PML>
-__device__ __forceinline__ void volt_cell(S* __restrict__ V, const S* __restrict__ I, const CoeffPtrs<typename StoreTraits<S>::A>& cf,
-                                          const Dims& d, const Scales<typename StoreTraits<S>::A>& sc, const PmlSet& ps,
-                                          const PmlPtrs<typename StoreTraits<S>::A>& pp, int x, int y, int z) {
-    typedef StoreTraits<S> T;
-    typedef typename T::A A;
-    const long long p = x * d.sx + y * d.sy + z;
-    const long long c1 = d.sc, c2 = 2 * d.sc;
-    const long long ox = x > 0 ? d.sx : 0;   // x-1 neighbour, self at x=0 (shift[0])
-    const long long oy = y > 0 ? d.sy : 0;   // y-1 neighbour, self at y=0 (shift[1])
-
-    const A i0 = T::load(I + p, sc.c[0]);
-    const A i1 = T::load(I + c1 + p, sc.c[1]);
-    const A i2 = T::load(I + c2 + p, sc.c[2]);
-    const A i2_ym = T::load(I + c2 + p - oy, sc.c[2]);
-    const A i0_ym = T::load(I + p - oy, sc.c[0]);
-    const A i2_xm = T::load(I + c2 + p - ox, sc.c[2]);
-    const A i1_xm = T::load(I + c1 + p - ox, sc.c[1]);
-    const A i1_zm = z > 0 ? T::load(I + c1 + p - 1, sc.c[1]) : A(0);
-    const A i0_zm = z > 0 ? T::load(I + p - 1, sc.c[0]) : A(0);
-
-    A curl[3];
-    curl[0] = ((i2 - i2_ym) - i1) + i1_zm;
-    curl[1] = ((i0 - i0_zm) - i2) + i2_xm;
-    curl[2] = ((i1 - i1_xm) - i0) + i0_ym;
-
-    A vv[3], vi[3];
-    load_coeffs<A, CM>(cf, p, d.sc, vv, vi);
-
-    long long lp = 0;
-    int slab = PML ? find_slab(ps, x, y, z, lp) : -1;
-#pragma unroll
-    for (int n = 0; n < 3; ++n) {
-        S* vp = V + n * d.sc + p;
-        A v = T::load(vp, sc.v[n]);
-        if (PML && slab >= 0) {
-            const PmlSlab& b = ps.slab[slab];
-            const long long q = lp + n * b.sc;
-            A F = pp.vflux[q];
-            A h = pp.vv[q] * v - pp.vvfo[q] * F;
-            A t = F;
-            t = t * vv[n];
-            t = t + vi[n] * curl[n];
-            pp.vflux[q] = t;
-            v = h + pp.vvfn[q] * t;
-        } else {
-            v = v * vv[n];
-            v = v + vi[n] * curl[n];
-        }
-        T::store(vp, v, sc.iv[n]);
-    }
-}
-
-template <typename S, int CM, bool PML>
-__global__ void __launch_bounds__(256)
-k_volt(S* __restrict__ V, const S* __restrict__ I, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
-       Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp) {
-    const int z = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    const int x = blockIdx.z;
-    if (z >= d.nz || y >= d.ny) return;
-    volt_cell<S, CM, PML>(V, I, cf, d, sc, ps, pp, x, y, z);
-}
-
-// ---------------------------------------------------------------------------
-// main current update (Engine::UpdateCurrents, FDTD/engine.cpp:163-210) over
-// x<Nx-1, y<Ny-1, z<Nz-1 with UPML fused (engine_ext_upml.cpp:141-234).
-// ---------------------------------------------------------------------------
-template <typename S, int CM, bool PML>
-__device__ __forceinline__ void curr_cell(S* __restrict__ I, const S* __restrict__ V, const CoeffPtrs<typename StoreTraits<S>::A>& cf,
-                                          const Dims& d, const Scales<typename StoreTraits<S>::A>& sc, const PmlSet& ps,
-                                          const PmlPtrs<typename StoreTraits<S>::A>& pp, int x, int y, int z) {
-    typedef StoreTraits<S> T;
-    typedef typename T::A A;
-    const long long p = x * d.sx + y * d.sy + z;
-    const long long c1 = d.sc, c2 = 2 * d.sc;
-
-    const A v0 = T::load(V + p, sc.v[0]);
-    const A v1 = T::load(V + c1 + p, sc.v[1]);
-    const A v2 = T::load(V + c2 + p, sc.v[2]);
-    const A v2_yp = T::load(V + c2 + p + d.sy, sc.v[2]);
-    const A v0_yp = T::load(V + p + d.sy, sc.v[0]);
-    const A v2_xp = T::load(V + c2 + p + d.sx, sc.v[2]);
-    const A v1_xp = T::load(V + c1 + p + d.sx, sc.v[1]);
-    const A v1_zp = T::load(V + c1 + p + 1, sc.v[1]);
-    const A v0_zp = T::load(V + p + 1, sc.v[0]);
-
-    A curl[3];
-    curl[0] = ((v2 - v2_yp) - v1) + v1_zp;
-    curl[1] = ((v0 - v0_zp) - v2) + v2_xp;
-    curl[2] = ((v1 - v1_xp) - v0) + v0_yp;
-
-    A ii[3], iv[3];
-    load_coeffs<A, CM>(cf, p, d.sc, ii, iv);
-
-    long long lp = 0;
-    int slab = PML ? find_slab(ps, x, y, z, lp) : -1;
-#pragma unroll
-    for (int n = 0; n < 3; ++n) {
-        S* ip = I + n * d.sc + p;
-        A c = T::load(ip, sc.c[n]);
-        if (PML && slab >= 0) {
-            const PmlSlab& b = ps.slab[slab];
-            const long long q = lp + n * b.sc;
-            A F = pp.cflux[q];
-            A h = pp.ii[q] * c - pp.iifo[q] * F;
-            A t = F;
-            t = t * ii[n];
-            t = t + iv[n] * curl[n];
-            pp.cflux[q] = t;
-            c = h + pp.iifn[q] * t;
-        } else {
-            c = c * ii[n];
-            c = c + iv[n] * curl[n];
-        }
-        T::store(ip, c, sc.ic[n]);
-    }
-}
-
-template <typename S, int CM, bool PML>
-__global__ void __launch_bounds__(256)
-k_curr(S* __restrict__ I, const S* __restrict__ V, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
-       Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp) {
-    const int z = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    const int x = blockIdx.z;
-    if (z >= d.nz - 1 || y >= d.ny - 1 || x >= d.nx - 1) return;
-    curr_cell<S, CM, PML>(I, V, cf, d, sc, ps, pp, x, y, z);
-}
-
-// ---------------------------------------------------------------------------
-// vectorised updates: each lane updates VZ consecutive z cells with one wide
+// Vectorised: each lane updates VZ consecutive z cells with one wide
 // load/store per field component (16 bytes for FP32, FP16/BF16 and FP64), so
 // the number of memory instructions per cell drops VZ-fold; the z-1 (E) and
 // z+1 (H) neighbours come from the neighbouring element in the same vector,
@@ -450,10 +292,11 @@ __device__ __forceinline__ void vload_coeffs(const CoeffPtrs<A>& cf, long long p
 }
 
 // one vector of VZ cells at plane x, flat (y,z) offset q0 of the padded plane
-template <typename S, int CM, bool PML, int VZ>
+template <typename S, int CM, bool PML, int VZ, bool ZG = false>
 __device__ __forceinline__ void volt_vec_body(S* __restrict__ V, const S* __restrict__ I, const CoeffPtrs<typename StoreTraits<S>::A>& cf,
                                               const Dims& d, const Scales<typename StoreTraits<S>::A>& sc, const PmlSet& ps,
-                                              const PmlPtrs<typename StoreTraits<S>::A>& pp, int x, long long q0) {
+                                              const PmlPtrs<typename StoreTraits<S>::A>& pp, int x, long long q0,
+                                              int zin0 = 0, int zin1 = 0) {
     typedef StoreTraits<S> T;
     typedef typename T::A A;
     const int y = (int)(q0 / d.sy);
@@ -487,7 +330,8 @@ __device__ __forceinline__ void volt_vec_body(S* __restrict__ V, const S* __rest
     const int valid = min(VZ, d.nz - z0);
     SlabHits hit{{-1, -1}, {0u, 0u}, {0, 0}};
     SVec<unsigned short, VZ> prow[2] = {};
-    if (PML) {
+    // ZG: the slab search only for vectors reaching out of [zin0, zin1)
+    if (PML && (!ZG || z0 < zin0 || z0 + VZ > zin1)) {
         hit = slab_hits<VZ>(ps, x, y, z0, ~0u);
 #pragma unroll
         for (int j = 0; j < 2; ++j)
@@ -526,9 +370,10 @@ __device__ __forceinline__ void volt_vec_body(S* __restrict__ V, const S* __rest
 // PML code and k_volt_shell / k_curr_shell take every other vector.
 struct VecRange {
     int x0, y0, y1, zlo, zhi;
+    int zin0, zin1;        // rows form: vectors outside [zin0, zin1) take the UPML path
 };
 
-template <typename S, int CM, bool PML, int VZ>
+template <typename S, int CM, bool PML, int VZ, bool ZG = false>
 __device__ __forceinline__ void volt_vec_range(S* __restrict__ V, const S* __restrict__ I, const CoeffPtrs<typename StoreTraits<S>::A>& cf,
                                                const Dims& d, const Scales<typename StoreTraits<S>::A>& sc, const PmlSet& ps,
                                                const PmlPtrs<typename StoreTraits<S>::A>& pp, const VecRange& r) {
@@ -536,7 +381,7 @@ __device__ __forceinline__ void volt_vec_range(S* __restrict__ V, const S* __res
     const int y = (int)(q0 / d.sy);
     const int z0 = (int)(q0 - (long long)y * d.sy);
     if (y >= r.y1 || z0 < r.zlo || z0 + VZ > r.zhi) return;
-    volt_vec_body<S, CM, PML, VZ>(V, I, cf, d, sc, ps, pp, r.x0 + blockIdx.y, q0);
+    volt_vec_body<S, CM, PML, VZ, ZG>(V, I, cf, d, sc, ps, pp, r.x0 + blockIdx.y, q0, r.zin0, r.zin1);
 }
 
 // Without UPML code the voltage update fits 5 waves per SIMD for every
@@ -566,6 +411,26 @@ __global__ void __launch_bounds__(256, 5)
 k_volt_vec_upml_b5(S* __restrict__ V, const S* __restrict__ I, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
                    Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp, VecRange r) {
     volt_vec_range<S, CM, true, VZ>(V, I, cf, d, sc, ps, pp, r);
+}
+
+// Rows form of the split: the interior rows in full, UPML applied to the
+// vectors that reach the z slabs at the two ends of each row (the slab
+// search runs only for those), so the z-slab cells are read in the same
+// sweep as the row instead of in a scattered shell pass.  The bound keeps
+// 5 waves per SIMD for 4- and 8-byte storage (at most 8 VGPRs spilled);
+// 2-byte storage would spill 75 under it and runs the unbounded kernel.
+template <typename S, int CM, int VZ>
+__global__ void __launch_bounds__(256, 5)
+k_volt_rows(S* __restrict__ V, const S* __restrict__ I, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
+            Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp, VecRange r) {
+    volt_vec_range<S, CM, true, VZ, true>(V, I, cf, d, sc, ps, pp, r);
+}
+
+template <typename S, int CM, int VZ>
+__global__ void __launch_bounds__(256)
+k_volt_rows_u(S* __restrict__ V, const S* __restrict__ I, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
+              Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp, VecRange r) {
+    volt_vec_range<S, CM, true, VZ, true>(V, I, cf, d, sc, ps, pp, r);
 }
 
 // The vectors outside the interior box of a UPML shell, numbered
@@ -623,10 +488,11 @@ k_volt_shell(S* __restrict__ V, const S* __restrict__ I, CoeffPtrs<typename Stor
     volt_vec_body<S, CM, true, VZ>(V, I, cf, d, sc, ps, pp, x, q0);
 }
 
-template <typename S, int CM, bool PML, int VZ>
+template <typename S, int CM, bool PML, int VZ, bool ZG = false>
 __device__ __forceinline__ void curr_vec_body(S* __restrict__ I, const S* __restrict__ V, const CoeffPtrs<typename StoreTraits<S>::A>& cf,
                                               const Dims& d, const Scales<typename StoreTraits<S>::A>& sc, const PmlSet& ps,
-                                              const PmlPtrs<typename StoreTraits<S>::A>& pp, int x, long long q0) {
+                                              const PmlPtrs<typename StoreTraits<S>::A>& pp, int x, long long q0,
+                                              int zin0 = 0, int zin1 = 0) {
     typedef StoreTraits<S> T;
     typedef typename T::A A;
     const int y = (int)(q0 / d.sy);
@@ -662,7 +528,7 @@ __device__ __forceinline__ void curr_vec_body(S* __restrict__ I, const S* __rest
     const unsigned upd = valid < VZ ? (1u << valid) - 1u : ~0u;
     SlabHits hit{{-1, -1}, {0u, 0u}, {0, 0}};
     SVec<unsigned short, VZ> prow[2] = {};
-    if (PML) {
+    if (PML && (!ZG || z0 < zin0 || z0 + VZ > zin1)) {
         hit = slab_hits<VZ>(ps, x, y, z0, upd);
 #pragma unroll
         for (int j = 0; j < 2; ++j)
@@ -696,7 +562,7 @@ __device__ __forceinline__ void curr_vec_body(S* __restrict__ I, const S* __rest
 // REV: blocks walk planes and tiles in descending order, so the x+1 and
 // y+1 neighbours a block reads were loaded by the blocks just before it
 // (the mirror of the voltage update's x-1 / y-1 neighbours)
-template <typename S, int CM, bool PML, int VZ, bool REV = false>
+template <typename S, int CM, bool PML, int VZ, bool REV = false, bool ZG = false>
 __device__ __forceinline__ void curr_vec_range(S* __restrict__ I, const S* __restrict__ V, const CoeffPtrs<typename StoreTraits<S>::A>& cf,
                                                const Dims& d, const Scales<typename StoreTraits<S>::A>& sc, const PmlSet& ps,
                                                const PmlPtrs<typename StoreTraits<S>::A>& pp, const VecRange& r) {
@@ -706,7 +572,7 @@ __device__ __forceinline__ void curr_vec_range(S* __restrict__ I, const S* __res
     const int y = (int)(q0 / d.sy);
     const int z0 = (int)(q0 - (long long)y * d.sy);
     if (y >= r.y1 || z0 < r.zlo || z0 + VZ > r.zhi) return;
-    curr_vec_body<S, CM, PML, VZ>(I, V, cf, d, sc, ps, pp, r.x0 + px, q0);
+    curr_vec_body<S, CM, PML, VZ, ZG>(I, V, cf, d, sc, ps, pp, r.x0 + px, q0, r.zin0, r.zin1);
 }
 
 // FP32 storage walks the planes in descending order (512^3: 0.946 -> 0.907 ms
@@ -733,6 +599,20 @@ k_curr_vec_upml_b5(S* __restrict__ I, const S* __restrict__ V, CoeffPtrs<typenam
 }
 
 template <typename S, int CM, int VZ>
+__global__ void __launch_bounds__(256, 5)
+k_curr_rows(S* __restrict__ I, const S* __restrict__ V, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
+            Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp, VecRange r) {
+    curr_vec_range<S, CM, true, VZ, false, true>(I, V, cf, d, sc, ps, pp, r);
+}
+
+template <typename S, int CM, int VZ>
+__global__ void __launch_bounds__(256)
+k_curr_rows_u(S* __restrict__ I, const S* __restrict__ V, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
+              Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp, VecRange r) {
+    curr_vec_range<S, CM, true, VZ, false, true>(I, V, cf, d, sc, ps, pp, r);
+}
+
+template <typename S, int CM, int VZ>
 __global__ void __launch_bounds__(256)
 k_curr_shell(S* __restrict__ I, const S* __restrict__ V, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
              Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp, ShellGrid g) {
@@ -740,215 +620,6 @@ k_curr_shell(S* __restrict__ I, const S* __restrict__ V, CoeffPtrs<typename Stor
     long long q0;
     if (!shell_vector(g, (long long)blockIdx.x * blockDim.x + threadIdx.x, VZ, d.sy, x, q0)) return;
     curr_vec_body<S, CM, true, VZ>(I, V, cf, d, sc, ps, pp, x, q0);
-}
-
-// ---------------------------------------------------------------------------
-// banded launch order: one thread per cell like k_volt/k_curr, but block ids
-// are remapped so that each XCD (hardware places block b on XCD b % 8) walks
-// a contiguous run of logical tiles ordered (y band, x, y tile in band, z
-// tile).  A band plane (band_tiles x tile_y rows x Nz) is small, so the x-1
-// plane and the y-1 rows a tile needs were just fetched into the same XCD's
-// L2 by the tiles before it.
-// ---------------------------------------------------------------------------
-struct BandGrid {
-    int tiles_z, tiles_y, band_tiles, nbands, nx;
-    int nblocks, per_xcd;
-};
-
-__device__ __forceinline__ bool band_tile(const BandGrid& g, int& tz, int& ty, int& x) {
-    const int b = blockIdx.x;
-    const int t = (b & 7) * g.per_xcd + (b >> 3);
-    if (t >= g.nblocks) return false;
-    tz = t % g.tiles_z;
-    int r = t / g.tiles_z;
-    const int tyb = r % g.band_tiles;
-    r /= g.band_tiles;
-    x = r % g.nx;
-    const int band = r / g.nx;
-    ty = band * g.band_tiles + tyb;
-    return ty < g.tiles_y;
-}
-
-template <typename S, int CM, bool PML>
-__global__ void __launch_bounds__(256)
-k_volt_band(S* __restrict__ V, const S* __restrict__ I, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
-            Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp, BandGrid g) {
-    int tz, ty, x;
-    if (!band_tile(g, tz, ty, x)) return;
-    const int z = tz * 64 + threadIdx.x;
-    const int y = ty * blockDim.y + threadIdx.y;
-    if (z >= d.nz || y >= d.ny) return;
-    volt_cell<S, CM, PML>(V, I, cf, d, sc, ps, pp, x, y, z);
-}
-
-template <typename S, int CM, bool PML>
-__global__ void __launch_bounds__(256)
-k_curr_band(S* __restrict__ I, const S* __restrict__ V, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
-            Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp, BandGrid g) {
-    int tz, ty, x;
-    if (!band_tile(g, tz, ty, x)) return;
-    const int z = tz * 64 + threadIdx.x;
-    const int y = ty * blockDim.y + threadIdx.y;
-    if (z >= d.nz - 1 || y >= d.ny - 1) return;
-    curr_cell<S, CM, PML>(I, V, cf, d, sc, ps, pp, x, y, z);
-}
-
-// ---------------------------------------------------------------------------
-// x-marching versions of the two updates (2.5-D blocking).  A block is a
-// 64(z) x TY(y) tile of columns; each thread walks its (y,z) column through an
-// x chunk and keeps the x-neighbour values of the previous plane in
-// registers, so every field element is fetched from HBM once per kernel; the
-// y and z neighbours are re-read by adjacent lanes/waves of the same block
-// and hit L1/L2.  The linear block id is remapped so that consecutive tiles
-// (which share their y halo rows) run on the same XCD and share its L2.
-// Arithmetic is identical to k_volt / k_curr.
-// ---------------------------------------------------------------------------
-struct MarchGrid {
-    int tiles_z, tiles_y, chunks_x;    // logical tile grid
-    int chunk;                         // x planes per chunk
-    int nblocks;                       // tiles_z * tiles_y * chunks_x
-    int per_xcd;                       // ceil(nblocks / 8)
-};
-
-__device__ __forceinline__ bool march_tile(const MarchGrid& g, int& tz, int& ty, int& cx) {
-    // hardware dispatch places block b on XCD b % 8; give each XCD a
-    // contiguous run of logical tiles (z fastest, then y, then x chunk)
-    const int b = blockIdx.x;
-    const int t = (b & 7) * g.per_xcd + (b >> 3);
-    if (t >= g.nblocks) return false;
-    tz = t % g.tiles_z;
-    const int r = t / g.tiles_z;
-    ty = r % g.tiles_y;
-    cx = r / g.tiles_y;
-    return true;
-}
-
-template <typename S, int CM, bool PML>
-__global__ void __launch_bounds__(256)
-k_volt_march(S* __restrict__ V, const S* __restrict__ I, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
-             Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp, MarchGrid g) {
-    typedef StoreTraits<S> T;
-    typedef typename T::A A;
-    int tz, ty, cx;
-    if (!march_tile(g, tz, ty, cx)) return;
-    const int z = tz * 64 + threadIdx.x;
-    const int y = ty * blockDim.y + threadIdx.y;
-    if (z >= d.nz || y >= d.ny) return;
-    const int x0 = cx * g.chunk;
-    const int x1 = min(d.nx, x0 + g.chunk);
-    const long long c1 = d.sc, c2 = 2 * d.sc;
-    const long long oy = y > 0 ? d.sy : 0;
-    long long p = x0 * d.sx + y * d.sy + z;
-    // x-1 neighbours of the first plane (the cell itself at x = 0)
-    A i1_prev, i2_prev;
-    if (x0 > 0) {
-        i1_prev = T::load(I + c1 + p - d.sx, sc.c[1]);
-        i2_prev = T::load(I + c2 + p - d.sx, sc.c[2]);
-    } else {
-        i1_prev = T::load(I + c1 + p, sc.c[1]);
-        i2_prev = T::load(I + c2 + p, sc.c[2]);
-    }
-    for (int x = x0; x < x1; ++x, p += d.sx) {
-        const A i0 = T::load(I + p, sc.c[0]);
-        const A i1 = T::load(I + c1 + p, sc.c[1]);
-        const A i2 = T::load(I + c2 + p, sc.c[2]);
-        const A i2_ym = T::load(I + c2 + p - oy, sc.c[2]);
-        const A i0_ym = T::load(I + p - oy, sc.c[0]);
-        const A i1_zm = z > 0 ? T::load(I + c1 + p - 1, sc.c[1]) : A(0);
-        const A i0_zm = z > 0 ? T::load(I + p - 1, sc.c[0]) : A(0);
-        A curl[3];
-        curl[0] = ((i2 - i2_ym) - i1) + i1_zm;
-        curl[1] = ((i0 - i0_zm) - i2) + i2_prev;
-        curl[2] = ((i1 - i1_prev) - i0) + i0_ym;
-        i1_prev = i1;
-        i2_prev = i2;
-
-        A vv[3], vi[3];
-        load_coeffs<A, CM>(cf, p, d.sc, vv, vi);
-        long long lp = 0;
-        int slab = PML ? find_slab(ps, x, y, z, lp) : -1;
-#pragma unroll
-        for (int n = 0; n < 3; ++n) {
-            S* vp = V + n * d.sc + p;
-            A v = T::load(vp, sc.v[n]);
-            if (PML && slab >= 0) {
-                const PmlSlab& b = ps.slab[slab];
-                const long long q = lp + n * b.sc;
-                A F = pp.vflux[q];
-                A h = pp.vv[q] * v - pp.vvfo[q] * F;
-                A t = F;
-                t = t * vv[n];
-                t = t + vi[n] * curl[n];
-                pp.vflux[q] = t;
-                v = h + pp.vvfn[q] * t;
-            } else {
-                v = v * vv[n];
-                v = v + vi[n] * curl[n];
-            }
-            T::store(vp, v, sc.iv[n]);
-        }
-    }
-}
-
-template <typename S, int CM, bool PML>
-__global__ void __launch_bounds__(256)
-k_curr_march(S* __restrict__ I, const S* __restrict__ V, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
-             Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp, MarchGrid g) {
-    typedef StoreTraits<S> T;
-    typedef typename T::A A;
-    int tz, ty, cx;
-    if (!march_tile(g, tz, ty, cx)) return;
-    const int z = tz * 64 + threadIdx.x;
-    const int y = ty * blockDim.y + threadIdx.y;
-    if (z >= d.nz - 1 || y >= d.ny - 1) return;
-    const int x0 = cx * g.chunk;
-    const int x1 = min(d.nx - 1, x0 + g.chunk);
-    if (x0 >= x1) return;
-    const long long c1 = d.sc, c2 = 2 * d.sc;
-    long long p = x0 * d.sx + y * d.sy + z;
-    // V1, V2 of the current plane come from the previous iteration's x+1 loads
-    A v1 = T::load(V + c1 + p, sc.v[1]);
-    A v2 = T::load(V + c2 + p, sc.v[2]);
-    for (int x = x0; x < x1; ++x, p += d.sx) {
-        const A v0 = T::load(V + p, sc.v[0]);
-        const A v2_xp = T::load(V + c2 + p + d.sx, sc.v[2]);
-        const A v1_xp = T::load(V + c1 + p + d.sx, sc.v[1]);
-        const A v2_yp = T::load(V + c2 + p + d.sy, sc.v[2]);
-        const A v0_yp = T::load(V + p + d.sy, sc.v[0]);
-        const A v1_zp = T::load(V + c1 + p + 1, sc.v[1]);
-        const A v0_zp = T::load(V + p + 1, sc.v[0]);
-        A curl[3];
-        curl[0] = ((v2 - v2_yp) - v1) + v1_zp;
-        curl[1] = ((v0 - v0_zp) - v2) + v2_xp;
-        curl[2] = ((v1 - v1_xp) - v0) + v0_yp;
-        v1 = v1_xp;
-        v2 = v2_xp;
-
-        A ii[3], iv[3];
-        load_coeffs<A, CM>(cf, p, d.sc, ii, iv);
-        long long lp = 0;
-        int slab = PML ? find_slab(ps, x, y, z, lp) : -1;
-#pragma unroll
-        for (int n = 0; n < 3; ++n) {
-            S* ip = I + n * d.sc + p;
-            A c = T::load(ip, sc.c[n]);
-            if (PML && slab >= 0) {
-                const PmlSlab& b = ps.slab[slab];
-                const long long q = lp + n * b.sc;
-                A F = pp.cflux[q];
-                A h = pp.ii[q] * c - pp.iifo[q] * F;
-                A t = F;
-                t = t * ii[n];
-                t = t + iv[n] * curl[n];
-                pp.cflux[q] = t;
-                c = h + pp.iifn[q] * t;
-            } else {
-                c = c * ii[n];
-                c = c + iv[n] * curl[n];
-            }
-            T::store(ip, c, sc.ic[n]);
-        }
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1502,6 +1173,37 @@ k_snapshot(const S* __restrict__ F, Dims d, Scales<typename StoreTraits<S>::A> s
     double v[3];
     node_field<S>(F, d, sc, ax, b.field, pos, v);
     for (int n = 0; n < 3; ++n) out[n * b.nodes + node] = (float)v[n];
+}
+
+// ---------------------------------------------------------------------------
+// per-component max |value| of one field (physical units), over the padded
+// component arrays (padding holds zeros): grid (blocks, 3), one partial per
+// block and component.  Feeds the storage-scale pilot of FP16/BF16 handles
+// without moving the fields to the host.
+// ---------------------------------------------------------------------------
+template <typename S>
+__global__ void __launch_bounds__(256)
+k_absmax(const S* __restrict__ F, long long sc, Scales<typename StoreTraits<S>::A> s, int is_curr, double* __restrict__ part) {
+    typedef typename StoreTraits<S>::A A;
+    constexpr int VZ = VecWidth<S>::value;
+    __shared__ A sm[256];
+    const int n = blockIdx.y;
+    const S* p = F + n * sc;
+    const A scale = is_curr ? s.c[n] : s.v[n];
+    A m = A(0);
+    for (long long i = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * VZ; i < sc; i += (long long)gridDim.x * blockDim.x * VZ) {
+        A v[VZ];
+        vload<S, VZ>(p + i, v, scale);
+#pragma unroll
+        for (int k = 0; k < VZ; ++k) m = fmax(m, fabs(v[k]));
+    }
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(size_t)n * gridDim.x + blockIdx.x] = (double)sm[0];
 }
 
 // ---------------------------------------------------------------------------

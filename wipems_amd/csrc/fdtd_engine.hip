@@ -26,8 +26,8 @@
 // SSE engines set FTZ|DAZ, tools/denormal.h:19-31), so an FP32 handle
 // reproduces the reference FP32 engine bit for bit.
 #include "fdtd_internal.h"
-#include "fdtd_fused.h"
 #include <memory>
+#include <mutex>
 #include <unordered_map>
 
 thread_local std::string g_last_error;
@@ -65,66 +65,7 @@ void invalidate_graph(fdtd_engine* h) {
     if (h->gexec1) { (void)hipGraphExecDestroy(h->gexec1); h->gexec1 = nullptr; }
     if (h->graph1) { (void)hipGraphDestroy(h->graph1); h->graph1 = nullptr; }
     h->graph_steps = 0;
-    h->graph_base = nullptr;
 }
-
-// the fused step handles domains without UPML, Lorentz or TFSF extensions
-// and with few soft excitation edges (kept in LDS)
-bool fused_ok(const fdtd_engine* h) {
-    if (h->variant != 4) return false;
-    if (h->pml.count || !h->ade.empty() || !h->mur.empty() || h->rlc || h->ss || h->tfsf_v.count || h->tfsf_c.count) return false;
-    if (h->has_exc && (h->exc_v.count > FUSED_MAX_EXC || h->exc_c.count > FUSED_MAX_EXC)) return false;
-    return h->N[0] >= 2 && h->N[1] >= 2 && h->N[2] >= 2;
-}
-
-int ensure_fused_buffers(fdtd_engine* h) {
-    if (h->V2.p) return FDTD_OK;
-    int rc;
-    if ((rc = h->V2.alloc(h->V.bytes)) || (rc = h->I2.alloc(h->I.bytes))) return rc;
-    HIPCHK(hipMemsetAsync(h->V2.p, 0, h->V2.bytes, h->stream));
-    HIPCHK(hipMemsetAsync(h->I2.p, 0, h->I2.bytes, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
-    return FDTD_OK;
-}
-
-template <typename S, int CM, int TY>
-void launch_fused(fdtd_engine* h) {
-    typedef typename StoreTraits<S>::A A;
-    constexpr int VZ = VecWidth<S>::value;
-    constexpr int TZ = 64 * VZ;
-    FusedGrid g;
-    g.tiles_y = ((int)h->N[1] + TY - 1) / TY;
-    g.tiles_z = ((int)h->N[2] + TZ - 1) / TZ;
-    g.seg = std::max(1, h->fused_seg);
-    g.segs_x = ((int)h->N[0] + g.seg - 1) / g.seg;
-    g.nblocks = g.tiles_y * g.tiles_z * g.segs_x;
-    g.per_xcd = (g.nblocks + 7) / 8;
-    CoeffPtrs<A> cfe{}, cfh{};
-    if (h->cf_mode == CF_FULL) {
-        cfe.a = h->cf_vv.as<A>(); cfe.b = h->cf_vi.as<A>();
-        cfh.a = h->cf_ii.as<A>(); cfh.b = h->cf_iv.as<A>();
-    } else {
-        cfe.a = h->cf_tabE.as<A>(); cfe.idx = h->cf_idx.p;
-        cfh.a = h->cf_tabH.as<A>(); cfh.idx = h->cf_idx.p;
-    }
-    ExcDev ev{}, ec{};
-    if (h->has_exc) { ev = h->exc_v; ec = h->exc_c; }
-    hipLaunchKernelGGL((k_step_fused<S, CM, VZ, TY>), dim3((unsigned)(g.per_xcd * 8)), dim3(64 * (TY + 2)), 0, h->stream,
-                       h->V.as<S>(), h->I.as<S>(), h->V2.as<S>(), h->I2.as<S>(), cfe, cfh, h->d, make_scales<A>(h), g, ev, ec,
-                       h->step.as<unsigned>());
-    std::swap(h->V.p, h->V2.p);
-    std::swap(h->I.p, h->I2.p);
-}
-
-template <typename S, int TY>
-void launch_fused_cm(fdtd_engine* h) {
-    switch (h->cf_mode) {
-    case CF_FULL: launch_fused<S, CF_FULL, TY>(h); break;
-    case CF_IDX16: launch_fused<S, CF_IDX16, TY>(h); break;
-    default: launch_fused<S, CF_IDX32, TY>(h); break;
-    }
-}
-
 
 long long cell_off(const fdtd_engine* h, uint32_t x, uint32_t y, uint32_t z) {
     return (long long)x * h->d.sx + (long long)y * h->d.sy + z;
@@ -147,23 +88,98 @@ int convert_host(const fdtd_engine* h, int dtype, const void* src, size_t n, std
     return FDTD_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Large host <-> device copies go through two library-owned pinned chunks
+// (hipHostMalloc), double-buffered: the runtime never has to pin (lock) the
+// caller's pageable memory for them, and a multi-GB copy holds no
+// multi-GB temporary.  One staging area per process, used under a lock.
+// ---------------------------------------------------------------------------
+struct PinnedStage {
+    std::mutex mu;
+    char* buf[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    static constexpr size_t chunk = (size_t)32 << 20;
+    int ready() {
+        for (int k = 0; k < 2; ++k) {
+            if (!buf[k]) HIPCHK(hipHostMalloc((void**)&buf[k], chunk, hipHostMallocDefault));
+            if (!ev[k]) HIPCHK(hipEventCreateWithFlags(&ev[k], hipEventDisableTiming));
+        }
+        return FDTD_OK;
+    }
+};
+static PinnedStage g_stage;
+static const size_t kDirectCopy = (size_t)1 << 20;   // smaller copies go straight through
+
+int stage_d2h(hipStream_t st, void* dst, const void* src, size_t bytes) {
+    if (bytes <= kDirectCopy) {
+        HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        return FDTD_OK;
+    }
+    std::lock_guard<std::mutex> lk(g_stage.mu);
+    int rc = g_stage.ready();
+    if (rc) return rc;
+    const size_t C = PinnedStage::chunk;
+    const size_t nchunks = (bytes + C - 1) / C;
+    auto issue = [&](size_t k) -> int {
+        const size_t off = k * C, n = std::min(C, bytes - off);
+        HIPCHK(hipMemcpyAsync(g_stage.buf[k & 1], (const char*)src + off, n, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipEventRecord(g_stage.ev[k & 1], st));
+        return FDTD_OK;
+    };
+    if ((rc = issue(0))) return rc;
+    for (size_t k = 0; k < nchunks; ++k) {
+        if (k + 1 < nchunks && (rc = issue(k + 1))) return rc;
+        HIPCHK(hipEventSynchronize(g_stage.ev[k & 1]));
+        const size_t off = k * C, n = std::min(C, bytes - off);
+        memcpy((char*)dst + off, g_stage.buf[k & 1], n);
+    }
+    return FDTD_OK;
+}
+
+int stage_h2d(hipStream_t st, void* dst, const void* src, size_t bytes) {
+    if (bytes <= kDirectCopy) {
+        HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+        return FDTD_OK;
+    }
+    std::lock_guard<std::mutex> lk(g_stage.mu);
+    int rc = g_stage.ready();
+    if (rc) return rc;
+    const size_t C = PinnedStage::chunk;
+    bool used[2] = {false, false};
+    for (size_t off = 0, k = 0; off < bytes; off += C, ++k) {
+        const size_t n = std::min(C, bytes - off);
+        if (used[k & 1]) HIPCHK(hipEventSynchronize(g_stage.ev[k & 1]));   // its previous copy has left the chunk
+        memcpy(g_stage.buf[k & 1], (const char*)src + off, n);
+        HIPCHK(hipMemcpyAsync((char*)dst + off, g_stage.buf[k & 1], n, hipMemcpyHostToDevice, st));
+        HIPCHK(hipEventRecord(g_stage.ev[k & 1], st));
+        used[k & 1] = true;
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    return FDTD_OK;
+}
+
 // dense [3][Nx][Ny][Nz] host array -> padded device array in arithmetic type
 int upload_dense3(fdtd_engine* h, int dtype, const void* src, DevBuf& buf) {
     const size_t nd = (size_t)h->N[0] * h->N[1] * h->N[2];
     int rc = buf.alloc((size_t)3 * h->d.sc * h->acc);
     if (rc) return rc;
     HIPCHK(hipMemsetAsync(buf.p, 0, buf.bytes, h->stream));
-    std::vector<char> row;
     const size_t in_elem = dtype == FDTD_DTYPE_F64 ? 8 : 4;
-    // copy plane by plane with the strided 2D copy
+    // per component: convert on the host, stage the dense rows into a device
+    // buffer, then a device-to-device strided copy into the padded layout
     std::vector<char> tmp;
+    DevBuf dense;
+    if ((rc = dense.alloc(nd * h->acc))) return rc;
     for (int n = 0; n < 3; ++n) {
         const char* s = (const char*)src + (size_t)n * nd * in_elem;
         rc = convert_host(h, dtype, s, nd, tmp);
         if (rc) return rc;
+        if ((rc = stage_h2d(h->stream, dense.p, tmp.data(), nd * h->acc))) return rc;
         HIPCHK(hipMemcpy2DAsync((char*)buf.p + (size_t)n * h->d.sc * h->acc, (size_t)h->d.sy * h->acc,
-                                tmp.data(), (size_t)h->N[2] * h->acc, (size_t)h->N[2] * h->acc,
-                                (size_t)h->N[0] * h->N[1], hipMemcpyHostToDevice, h->stream));
+                                dense.p, (size_t)h->N[2] * h->acc, (size_t)h->N[2] * h->acc,
+                                (size_t)h->N[0] * h->N[1], hipMemcpyDeviceToDevice, h->stream));
         HIPCHK(hipStreamSynchronize(h->stream));
     }
     return FDTD_OK;
@@ -278,17 +294,23 @@ int compress_pml(fdtd_engine* h) {
     return FDTD_OK;
 }
 
+bool probes_need_realloc(const fdtd_engine* h, uint32_t upto_ts) {
+    if (h->probes_dirty) return true;
+    for (auto p : h->probes)
+        if (p->td_interval > 0 && (long long)(upto_ts / p->td_interval) + 1 > p->td_slots) return true;
+    return false;
+}
+
+// Rebuilds the device probe table and grows the TD sample buffers to hold
+// samples up to upto_ts.  Old buffers are copied and freed on h->stream: a
+// caller that launched step-end kernels reading them on another stream (a
+// batch) must wait for that stream first.
 int finalize_probes(fdtd_engine* h, uint32_t upto_ts) {
-    // (re)build the device probe table; td buffers sized for samples up to upto_ts
     std::vector<ProbeDev> pd;
     std::vector<long long> off;
     std::vector<int> comp;
     std::vector<signed char> sign;
-    bool realloc_needed = h->probes_dirty;
-    for (auto p : h->probes) {
-        if (p->td_interval > 0 && (long long)(upto_ts / p->td_interval) + 1 > p->td_slots) realloc_needed = true;
-    }
-    if (!realloc_needed) return FDTD_OK;
+    if (!probes_need_realloc(h, upto_ts)) return FDTD_OK;
     for (auto p : h->probes) {
         ProbeDev q{};
         q.kind = p->kind;
@@ -359,67 +381,8 @@ int launch_step_end(fdtd_engine* h, int advance) {
     return FDTD_OK;
 }
 
-template <typename S, bool PML>
-void launch_volt_cm(fdtd_engine* h, dim3 g, dim3 b, CoeffPtrs<typename StoreTraits<S>::A> cf, PmlPtrs<typename StoreTraits<S>::A> pp) {
-    typedef typename StoreTraits<S>::A A;
-    Scales<A> sc = make_scales<A>(h);
-    switch (h->cf_mode) {
-    case CF_FULL: hipLaunchKernelGGL((k_volt<S, CF_FULL, PML>), g, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, sc, h->pml, pp); break;
-    case CF_IDX16: hipLaunchKernelGGL((k_volt<S, CF_IDX16, PML>), g, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, sc, h->pml, pp); break;
-    default: hipLaunchKernelGGL((k_volt<S, CF_IDX32, PML>), g, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, sc, h->pml, pp); break;
-    }
-}
-
-template <typename S, bool PML>
-void launch_curr_cm(fdtd_engine* h, dim3 g, dim3 b, CoeffPtrs<typename StoreTraits<S>::A> cf, PmlPtrs<typename StoreTraits<S>::A> pp) {
-    typedef typename StoreTraits<S>::A A;
-    Scales<A> sc = make_scales<A>(h);
-    switch (h->cf_mode) {
-    case CF_FULL: hipLaunchKernelGGL((k_curr<S, CF_FULL, PML>), g, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, sc, h->pml, pp); break;
-    case CF_IDX16: hipLaunchKernelGGL((k_curr<S, CF_IDX16, PML>), g, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, sc, h->pml, pp); break;
-    default: hipLaunchKernelGGL((k_curr<S, CF_IDX32, PML>), g, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, sc, h->pml, pp); break;
-    }
-}
-
-MarchGrid make_march(const fdtd_engine* h, int ny_eff, int nx_eff) {
-    MarchGrid g;
-    g.tiles_z = (h->N[2] + 63) / 64;
-    g.tiles_y = (ny_eff + h->tile_y - 1) / h->tile_y;
-    g.chunk = std::max(1, std::min(h->chunk_x, nx_eff));
-    g.chunks_x = (nx_eff + g.chunk - 1) / g.chunk;
-    g.nblocks = g.tiles_z * g.tiles_y * g.chunks_x;
-    g.per_xcd = (g.nblocks + 7) / 8;
-    return g;
-}
-
-template <typename S, bool PML>
-void launch_volt_march(fdtd_engine* h, CoeffPtrs<typename StoreTraits<S>::A> cf, PmlPtrs<typename StoreTraits<S>::A> pp) {
-    typedef typename StoreTraits<S>::A A;
-    Scales<A> sc = make_scales<A>(h);
-    MarchGrid g = make_march(h, h->N[1], h->N[0]);
-    const dim3 b(64, h->tile_y, 1), gr(g.per_xcd * 8);
-    switch (h->cf_mode) {
-    case CF_FULL: hipLaunchKernelGGL((k_volt_march<S, CF_FULL, PML>), gr, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, sc, h->pml, pp, g); break;
-    case CF_IDX16: hipLaunchKernelGGL((k_volt_march<S, CF_IDX16, PML>), gr, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, sc, h->pml, pp, g); break;
-    default: hipLaunchKernelGGL((k_volt_march<S, CF_IDX32, PML>), gr, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, sc, h->pml, pp, g); break;
-    }
-}
-
-template <typename S, bool PML>
-void launch_curr_march(fdtd_engine* h, CoeffPtrs<typename StoreTraits<S>::A> cf, PmlPtrs<typename StoreTraits<S>::A> pp) {
-    typedef typename StoreTraits<S>::A A;
-    Scales<A> sc = make_scales<A>(h);
-    MarchGrid g = make_march(h, h->N[1] - 1, h->N[0] - 1);
-    const dim3 b(64, h->tile_y, 1), gr(g.per_xcd * 8);
-    switch (h->cf_mode) {
-    case CF_FULL: hipLaunchKernelGGL((k_curr_march<S, CF_FULL, PML>), gr, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, sc, h->pml, pp, g); break;
-    case CF_IDX16: hipLaunchKernelGGL((k_curr_march<S, CF_IDX16, PML>), gr, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, sc, h->pml, pp, g); break;
-    default: hipLaunchKernelGGL((k_curr_march<S, CF_IDX32, PML>), gr, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, sc, h->pml, pp, g); break;
-    }
-}
-
 VecLaunch vec_launch(const fdtd_engine* h, int vz, bool curr) {
-    return plan_vec_launch(h->N, h->d.sy, h->pml.count, pml_split_in_use(h), h->ibox, vz, curr);
+    return plan_vec_launch(h->N, h->d.sy, h->pml.count, pml_split_in_use(h), h->ibox, vz, curr, pml_rows_form(h));
 }
 
 template <typename S, bool PML, int CM>
@@ -432,6 +395,12 @@ void launch_volt_main(fdtd_engine* h, const VecLaunch& L, CoeffPtrs<typename Sto
             hipLaunchKernelGGL((k_volt_vec_upml_b5<S, CM, VZ>), g, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.r);
         else if constexpr (PML)
             hipLaunchKernelGGL((k_volt_vec_upml<S, CM, VZ>), g, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.r);
+        else if (L.rows) {
+            if constexpr (sizeof(S) >= 4)
+                hipLaunchKernelGGL((k_volt_rows<S, CM, VZ>), g, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.r);
+            else
+                hipLaunchKernelGGL((k_volt_rows_u<S, CM, VZ>), g, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.r);
+        }
         else
             hipLaunchKernelGGL((k_volt_vec<S, CM, VZ>), g, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.r);
     }
@@ -453,6 +422,12 @@ void launch_curr_main(fdtd_engine* h, const VecLaunch& L, CoeffPtrs<typename Sto
             hipLaunchKernelGGL((k_curr_vec_upml_b5<S, CM, VZ>), g, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.r);
         else if constexpr (PML)
             hipLaunchKernelGGL((k_curr_vec_upml<S, CM, VZ>), g, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.r);
+        else if (L.rows) {
+            if constexpr (sizeof(S) >= 4)
+                hipLaunchKernelGGL((k_curr_rows<S, CM, VZ>), g, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.r);
+            else
+                hipLaunchKernelGGL((k_curr_rows_u<S, CM, VZ>), g, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.r);
+        }
         else
             hipLaunchKernelGGL((k_curr_vec<S, CM, VZ>), g, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.r);
     }
@@ -484,64 +459,10 @@ void launch_curr_vec(fdtd_engine* h, CoeffPtrs<typename StoreTraits<S>::A> cf, P
     }
 }
 
-BandGrid make_band(const fdtd_engine* h, int ny_eff, int nx_eff) {
-    BandGrid g;
-    g.tiles_z = (h->N[2] + 63) / 64;
-    g.tiles_y = (ny_eff + h->tile_y - 1) / h->tile_y;
-    g.band_tiles = std::max(1, std::min(h->chunk_x, g.tiles_y));
-    g.nbands = (g.tiles_y + g.band_tiles - 1) / g.band_tiles;
-    g.nx = nx_eff;
-    g.nblocks = g.tiles_z * g.band_tiles * g.nx * g.nbands;
-    g.per_xcd = (g.nblocks + 7) / 8;
-    return g;
-}
-
-template <typename S, bool PML>
-void launch_volt_band(fdtd_engine* h, CoeffPtrs<typename StoreTraits<S>::A> cf, PmlPtrs<typename StoreTraits<S>::A> pp) {
-    typedef typename StoreTraits<S>::A A;
-    Scales<A> sc = make_scales<A>(h);
-    BandGrid g = make_band(h, h->N[1], h->N[0]);
-    const dim3 b(64, h->tile_y, 1), gr(g.per_xcd * 8);
-    switch (h->cf_mode) {
-    case CF_FULL: hipLaunchKernelGGL((k_volt_band<S, CF_FULL, PML>), gr, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, sc, h->pml, pp, g); break;
-    case CF_IDX16: hipLaunchKernelGGL((k_volt_band<S, CF_IDX16, PML>), gr, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, sc, h->pml, pp, g); break;
-    default: hipLaunchKernelGGL((k_volt_band<S, CF_IDX32, PML>), gr, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, sc, h->pml, pp, g); break;
-    }
-}
-
-template <typename S, bool PML>
-void launch_curr_band(fdtd_engine* h, CoeffPtrs<typename StoreTraits<S>::A> cf, PmlPtrs<typename StoreTraits<S>::A> pp) {
-    typedef typename StoreTraits<S>::A A;
-    Scales<A> sc = make_scales<A>(h);
-    BandGrid g = make_band(h, h->N[1] - 1, h->N[0] - 1);
-    const dim3 b(64, h->tile_y, 1), gr(g.per_xcd * 8);
-    switch (h->cf_mode) {
-    case CF_FULL: hipLaunchKernelGGL((k_curr_band<S, CF_FULL, PML>), gr, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, sc, h->pml, pp, g); break;
-    case CF_IDX16: hipLaunchKernelGGL((k_curr_band<S, CF_IDX16, PML>), gr, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, sc, h->pml, pp, g); break;
-    default: hipLaunchKernelGGL((k_curr_band<S, CF_IDX32, PML>), gr, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, sc, h->pml, pp, g); break;
-    }
-}
-
 template <typename S>
 int launch_step_kernels(fdtd_engine* h, bool timed) {
     typedef typename StoreTraits<S>::A A;
-    if (fused_ok(h)) {
-        const size_t evi = timed ? h->ev_used : 0;
-        if (timed) HIPCHK(hipEventRecord(h->ev[evi], h->stream));
-        if constexpr (sizeof(S) == 2) launch_fused_cm<S, 2>(h);
-        else launch_fused_cm<S, 4>(h);
-        if (timed) {
-            HIPCHK(hipEventRecord(h->ev[evi + 1], h->stream));
-            HIPCHK(hipEventRecord(h->ev[evi + 2], h->stream));
-            HIPCHK(hipEventRecord(h->ev[evi + 3], h->stream));
-            h->ev_used += 4;
-        }
-        return launch_step_end<S>(h, 1);
-    }
     Scales<A> sc = make_scales<A>(h);
-    const dim3 blk(64, 4, 1);
-    const dim3 grd((h->N[2] + 63) / 64, (h->N[1] + 3) / 4, h->N[0]);
-    const dim3 grdc((h->N[2] + 63) / 64, (h->N[1] + 3) / 4, h->N[0] > 1 ? h->N[0] - 1 : 1);
 
     CoeffPtrs<A> cfe{}, cfh{};
     if (h->cf_mode == CF_FULL) {
@@ -605,10 +526,7 @@ int launch_step_kernels(fdtd_engine* h, bool timed) {
     const size_t evi = timed ? h->ev_used : 0;
     if (timed) h->ev_used += 4;
     if (timed) HIPCHK(hipEventRecord(h->ev[evi], h->stream));
-    if (h->variant >= 3) launch_volt_vec<S>(h, cfe, pp);
-    else if (h->variant == 2) { if (pml) launch_volt_band<S, true>(h, cfe, pp); else launch_volt_band<S, false>(h, cfe, pp); }
-    else if (h->variant == 1) { if (pml) launch_volt_march<S, true>(h, cfe, pp); else launch_volt_march<S, false>(h, cfe, pp); }
-    else if (pml) launch_volt_cm<S, true>(h, grd, blk, cfe, pp); else launch_volt_cm<S, false>(h, grd, blk, cfe, pp);
+    launch_volt_vec<S>(h, cfe, pp);
     if (timed) HIPCHK(hipEventRecord(h->ev[evi + 1], h->stream));
     if (h->tfsf_v.count)
         launch_tfsf<S>(h, h->V.as<S>(), stp, h->tfsf_v, sc, 0);
@@ -636,10 +554,7 @@ int launch_step_kernels(fdtd_engine* h, bool timed) {
         if ((a->flags & 2) && a->count)
             hipLaunchKernelGGL((k_ade_pre<S>), dim3((a->count + 255) / 256), dim3(256), 0, h->stream, h->I.as<S>(), ade_dev(a), h->d.sc, sc, 1);
     if (timed) HIPCHK(hipEventRecord(h->ev[evi + 2], h->stream));
-    if (h->variant >= 3) launch_curr_vec<S>(h, cfh, pp);
-    else if (h->variant == 2) { if (pml) launch_curr_band<S, true>(h, cfh, pp); else launch_curr_band<S, false>(h, cfh, pp); }
-    else if (h->variant == 1) { if (pml) launch_curr_march<S, true>(h, cfh, pp); else launch_curr_march<S, false>(h, cfh, pp); }
-    else if (pml) launch_curr_cm<S, true>(h, grdc, blk, cfh, pp); else launch_curr_cm<S, false>(h, grdc, blk, cfh, pp);
+    launch_curr_vec<S>(h, cfh, pp);
     if (timed) HIPCHK(hipEventRecord(h->ev[evi + 3], h->stream));
     if (h->tfsf_c.count)
         launch_tfsf<S>(h, h->I.as<S>(), stp, h->tfsf_c, sc, 1);
@@ -677,12 +592,35 @@ int ready_check(fdtd_engine* h) {
     return FDTD_OK;
 }
 
+#define ENERGY_BLOCKS 2048
+
 int ensure_energy_buffers(fdtd_engine* h) {
     int rc;
     if (!h->e_part.p) {
-        h->e_blocks = 2048;
+        h->e_blocks = ENERGY_BLOCKS;
         if ((rc = h->e_part.alloc((size_t)h->e_blocks * 2 * sizeof(double)))) return rc;
         if ((rc = h->e_out.alloc(2 * sizeof(double)))) return rc;
+    }
+    return FDTD_OK;
+}
+
+template <typename S>
+int absmax_impl(fdtd_engine* h, int which, double out[3]) {
+    typedef typename StoreTraits<S>::A A;
+    const unsigned blocks = 1024;
+    DevBuf part;
+    int rc = part.alloc((size_t)3 * blocks * sizeof(double));
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_absmax<S>), dim3(blocks, 3), dim3(256), 0, h->stream, (which == FDTD_VOLT ? h->V : h->I).template as<S>(),
+                       h->d.sc, make_scales<A>(h), which, part.as<double>());
+    HIPCHK(hipGetLastError());
+    std::vector<double> r((size_t)3 * blocks);
+    HIPCHK(hipMemcpyAsync(r.data(), part.p, r.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (int n = 0; n < 3; ++n) {
+        double m = 0.0;
+        for (unsigned b = 0; b < blocks; ++b) m = std::max(m, r[(size_t)n * blocks + b]);
+        out[n] = m;
     }
     return FDTD_OK;
 }
@@ -812,9 +750,8 @@ int get_field_impl(fdtd_engine* h, int which, H* out) {
     int rc = tmp.alloc(n * sizeof(H));
     if (rc) return rc;
     hipLaunchKernelGGL((k_pack<S, H>), dim3(2048), dim3(256), 0, h->stream, (which == FDTD_VOLT ? h->V : h->I).template as<S>(), tmp.as<H>(), h->d, make_scales<A>(h), which);
-    HIPCHK(hipMemcpyAsync(out, tmp.p, n * sizeof(H), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
-    return FDTD_OK;
+    HIPCHK(hipGetLastError());
+    return stage_d2h(h->stream, out, tmp.p, n * sizeof(H));
 }
 
 template <typename S, typename H>
@@ -824,7 +761,7 @@ int set_field_impl(fdtd_engine* h, int which, const H* in) {
     DevBuf tmp;
     int rc = tmp.alloc(n * sizeof(H));
     if (rc) return rc;
-    HIPCHK(hipMemcpyAsync(tmp.p, in, n * sizeof(H), hipMemcpyHostToDevice, h->stream));
+    if ((rc = stage_h2d(h->stream, tmp.p, in, n * sizeof(H)))) return rc;
     hipLaunchKernelGGL((k_unpack<S, H>), dim3(2048), dim3(256), 0, h->stream, (which == FDTD_VOLT ? h->V : h->I).template as<S>(), tmp.as<H>(), h->d, make_scales<A>(h), which);
     HIPCHK(hipStreamSynchronize(h->stream));
     return FDTD_OK;
@@ -878,6 +815,8 @@ int fdtd_last_error(char* buf, size_t len) {
 
 int fdtd_device_name(char* buf, size_t len) {
     if (!buf || !len) return fail(FDTD_ERR_ARG, "null buffer");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(FDTD_ERR_NODEVICE, "no HIP device available");
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));
     hipDeviceProp_t p;
@@ -1585,20 +1524,14 @@ int fdtd_read_dft(fdtd_engine* h, int id, double* out) {
     if (!h || !out) return fail(FDTD_ERR_ARG, "null argument");
     if (id < 0 || id >= (int)h->dft.size()) return fail(FDTD_ERR_RANGE, "bad DFT box id");
     DftHost* b = h->dft[id];
-    HIPCHK(hipMemcpyAsync(out, b->acc.p, b->acc.bytes, hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
-    return FDTD_OK;
+    return stage_d2h(h->stream, out, b->acc.p, b->acc.bytes);
 }
 
 int fdtd_set_tuning(fdtd_engine* h, int variant, int tile_y, int chunk_x) {
     if (!h) return fail(FDTD_ERR_ARG, "null handle");
-    if (variant < 0 || variant > 4) return fail(FDTD_ERR_ARG, "unknown kernel variant %d", variant);
+    if (variant != 3) return fail(FDTD_ERR_ARG, "unknown kernel variant %d (3: vector kernels)", variant);
     if (tile_y != 1 && tile_y != 2 && tile_y != 4) return fail(FDTD_ERR_ARG, "tile_y must be 1, 2 or 4");
     if (chunk_x < 1) return fail(FDTD_ERR_ARG, "chunk_x must be >= 1");
-    h->variant = variant;
-    h->tile_y = tile_y;
-    h->chunk_x = chunk_x;
-    if (variant == 4) h->fused_seg = chunk_x;
     invalidate_graph(h);
     return FDTD_OK;
 }
@@ -1621,7 +1554,7 @@ int fdtd_pml_compressed(fdtd_engine* h, int* active) {
 
 int fdtd_set_pml_split(fdtd_engine* h, int mode) {
     if (!h) return fail(FDTD_ERR_ARG, "null handle");
-    if (mode < -1 || mode > 1) return fail(FDTD_ERR_ARG, "mode must be -1 (auto), 0 or 1");
+    if (mode < -1 || mode > 2) return fail(FDTD_ERR_ARG, "mode must be -1 (auto), 0, 1 or 2");
     h->pml_split_mode = mode;
     invalidate_graph(h);
     return FDTD_OK;
@@ -1725,8 +1658,6 @@ static int iterate_steps(fdtd_engine* h, uint32_t nsteps) {
 // nsteps timesteps with no field-box sampling in between
 static int advance(fdtd_engine* h, uint32_t nsteps) {
     int rc;
-    const bool fused = fused_ok(h);
-    if (fused && (rc = ensure_fused_buffers(h))) return rc;
     if (h->timing) {
         // events for every launch of the two main kernels, harvested after the loop
         const size_t need = (size_t)4 * nsteps;
@@ -1759,26 +1690,6 @@ static int advance(fdtd_engine* h, uint32_t nsteps) {
     }
     const int G = 16;
     uint32_t left = nsteps;
-    if (fused) {
-        // the fused step swaps buffers: graphs hold an even number of steps
-        // and are replayed only from the buffer parity they were captured at
-        if (left >= (uint32_t)G) {
-            if (h->gexec && h->V.p != h->graph_base) {
-                if ((rc = launch_step(h, false))) return rc;
-                --left;
-            }
-            if (!h->gexec) {
-                h->graph_base = h->V.p;
-                if ((rc = capture_graph(h, G, &h->graph, &h->gexec))) return rc;
-            }
-            while (left >= (uint32_t)G) { HIPCHK(hipGraphLaunch(h->gexec, h->stream)); left -= G; }
-        }
-        for (; left; --left)
-            if ((rc = launch_step(h, false))) return rc;
-        HIPCHK(hipGetLastError());
-        h->numTS += nsteps;
-        return FDTD_OK;
-    }
     if (left >= (uint32_t)G) {
         if (!h->gexec) { if ((rc = capture_graph(h, G, &h->graph, &h->gexec))) return rc; }
         while (left >= (uint32_t)G) { HIPCHK(hipGraphLaunch(h->gexec, h->stream)); left -= G; }
@@ -1811,6 +1722,17 @@ int fdtd_calc_energy(fdtd_engine* h, double* energy) {
     case FDTD_PREC_FP64: return energy_impl<double>(h, energy);
     case FDTD_PREC_FP16: return energy_impl<__half>(h, energy);
     default: return energy_impl<__hip_bfloat16>(h, energy);
+    }
+}
+
+int fdtd_field_absmax(fdtd_engine* h, int which, double out[3]) {
+    if (!h || !out) return fail(FDTD_ERR_ARG, "null argument");
+    if (which != FDTD_VOLT && which != FDTD_CURR) return fail(FDTD_ERR_ARG, "bad field selector");
+    switch (h->prec) {
+    case FDTD_PREC_FP32: return absmax_impl<float>(h, which, out);
+    case FDTD_PREC_FP64: return absmax_impl<double>(h, which, out);
+    case FDTD_PREC_FP16: return absmax_impl<__half>(h, which, out);
+    default: return absmax_impl<__hip_bfloat16>(h, which, out);
     }
 }
 

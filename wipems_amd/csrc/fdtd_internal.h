@@ -139,7 +139,6 @@ struct fdtd_engine {
     hipStream_t stream = nullptr;
 
     DevBuf V, I;                  // fields (the current ones)
-    DevBuf V2, I2;                // second buffers of the fused step (written by it, then swapped in)
     DevBuf step;                  // unsigned numTS on device
     uint32_t numTS = 0;
 
@@ -168,7 +167,7 @@ struct fdtd_engine {
     // shell in a launch of its own
     bool shell_ok = false;
     int ibox[3][2] = {{0, 0}, {0, 0}, {0, 0}};
-    int pml_split_mode = -1;      // -1 auto (FP16/BF16 storage), 0 off, 1 on
+    int pml_split_mode = -1;      // -1 auto, 0 off, 1 split with a z-slab shell part, 2 rows form
 
     // excitation
     bool has_exc = false;
@@ -216,14 +215,6 @@ struct fdtd_engine {
     int graph_steps = 0;
     hipGraphExec_t gexec1 = nullptr;
     hipGraph_t graph1 = nullptr;
-    void* graph_base = nullptr;   // fused step: current V buffer when the graphs were captured
-    int fused_seg = 32;           // fused step: x planes per block
-
-    // kernel selection: 0 = one thread per cell, 1 = x-march, 2 = banded, 3 = vectors (default),
-    // 4 = fused step where the extensions allow it, else 3
-    int variant = 3;
-    int tile_y = 4;
-    int chunk_x = 64;
 
     // timing
     bool timing = false;
@@ -254,8 +245,6 @@ void invalidate_graph(fdtd_engine* h);
 int finalize_pml(fdtd_engine* h);
 void find_shell(fdtd_engine* h);
 int compress_pml(fdtd_engine* h);
-bool fused_ok(const fdtd_engine* h);
-int ensure_fused_buffers(fdtd_engine* h);
 // split interior / shell launches of the vector kernels (see fdtd_set_pml_split):
 // by default where the interior holds at least 88% of the cells for 2-byte
 // storage and 95% for FP32 (the shell launch's scattered z-slab vectors cost
@@ -264,16 +253,22 @@ int ensure_fused_buffers(fdtd_engine* h);
 // 769x769x385 with 92% no gain); FP64 keeps one launch
 inline bool pml_split_in_use(const fdtd_engine* h) {
     if (!h->shell_ok || h->pml_split_mode == 0) return false;
-    if (h->pml_split_mode == 1) return true;
+    if (h->pml_split_mode >= 1) return true;
     if (h->elem != 2 && h->elem != 4) return false;
     long long inner = 1;
     for (int n = 0; n < 3; ++n) inner *= h->ibox[n][1] - h->ibox[n][0];
     return (double)inner >= (h->elem == 2 ? 0.88 : 0.95) * (double)h->N[0] * h->N[1] * h->N[2];
 }
 
+// rows form of the split (k_volt_rows / k_curr_rows with x/y-only shell)
+inline bool pml_rows_form(const fdtd_engine* h) {
+    return pml_split_in_use(h) && h->pml_split_mode == 2;
+}
+
 inline bool pml_rows_in_use(const fdtd_engine* h) {
     return h->pml_cmp && (h->pml_cmp_mode == 1 || (h->pml_cmp_mode < 0 && h->cf_mode == CF_FULL));
 }
+bool probes_need_realloc(const fdtd_engine* h, uint32_t upto_ts);
 int finalize_probes(fdtd_engine* h, uint32_t upto_ts);
 int ready_check(fdtd_engine* h);
 

@@ -309,6 +309,12 @@ class Engine:
 
     CalcFastEnergy = energy
 
+    def field_absmax(self, which):
+        """Per-component max |value| of the volt (0) or curr (1) field, on the device."""
+        out = (c_double * 3)()
+        self._chk(self._lib.fdtd_field_absmax(self._h, int(which), out))
+        return np.array(list(out))
+
     def get_field(self, which, dtype=np.float64):
         out = np.empty((3,) + self.num_lines, dtype=dtype)
         self._chk(self._lib.fdtd_get_field(self._h, int(which), 1 if dtype == np.float64 else 0, _vp(out)))
@@ -372,10 +378,8 @@ def pilot_scales(data: OperatorData, steps=None, headroom=1024.0, compressed=Non
     while done < steps:
         eng.iterate(chunk)
         done += chunk
-        V = eng.get_field(VOLT, np.float32)
-        I = eng.get_field(CURR, np.float32)
-        peak_v = np.maximum(peak_v, np.abs(V).reshape(3, -1).max(axis=1))
-        peak_i = np.maximum(peak_i, np.abs(I).reshape(3, -1).max(axis=1))
+        peak_v = np.maximum(peak_v, eng.field_absmax(VOLT))
+        peak_i = np.maximum(peak_i, eng.field_absmax(CURR))
     eng.close()
     top_v = max(peak_v.max(), 1e-30)
     top_i = max(peak_i.max(), 1e-30)

@@ -193,6 +193,28 @@ class OperatorData:
         """Per-cell coefficient dedup (Operator_SSE_Compressed::CompressOperator)."""
         return compress_coeffs(self, max_entries)
 
+    def expanded(self):
+        """A copy with the full [3][Nx][Ny][Nz] coefficient arrays rebuilt from
+        the compressed index and table (the streamed build keeps only those),
+        in the build's float type."""
+        import copy
+        if self.vv is not None:
+            return self
+        if self.index is None:
+            raise ValueError("no coefficients to expand")
+        dt = np.float64 if self.prec == 1 else np.float32
+        tab = self.table.astype(dt)
+        idx = self.index.reshape(-1)
+        out = copy.copy(self)
+        arrs = []
+        for k in range(4):
+            a = np.empty((3,) + tuple(self.num_lines), dtype=dt)
+            for n in range(3):
+                a[n] = tab[idx, 3 * k + n].reshape(self.num_lines)
+            arrs.append(a)
+        out.vv, out.vi, out.ii, out.iv = arrs
+        return out
+
 
 def compress_coeffs(data: OperatorData, max_entries=0):
     """Deduplicate the per-cell 12-tuples {vv,vi,ii,iv}x3 bitwise, first-seen order.

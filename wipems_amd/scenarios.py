@@ -157,13 +157,17 @@ def dispersive_scenario(num_lines=(385, 385, 193), delta_mm=0.25, steps=20000, p
 
 def large_dipole_scenario(num_lines=(1537, 1537, 769), steps=20000, seed=SEED_BASE + 5, n_boxes=200, pml=8,
                           f0=2e9, fc=2e9, n_freq=16, box_margin=10, dipole_cells=20, threads=16, max_entries=65536,
-                          verbose=False):
+                          verbose=False, prec=0, streamed=True):
     """Large uniform 1 mm domain (defaults: 1537 x 1537 x 769 lines, 1.82 G
     cells) with UPML-8, `n_boxes` random lossless dielectric boxes (eps_r ~
     U[1,12]), a z-directed soft dipole of `dipole_cells` edges at the centre
     (Gaussian f0/fc) and an nf2ff recording box `box_margin` cells inside the
     UPML with FP64 DFT at n_freq frequencies over (0, f0+fc].  Built with the
-    streamed operator build (compressed coefficients only, float)."""
+    streamed operator build (compressed coefficients only, float); with
+    streamed=False the full build in precision `prec` (0 float, 1 double),
+    whose float result equals the streamed one.  Box sizes and positions are
+    fractions of the domain, so smaller num_lines give the same geometry
+    scaled down."""
     rng = np.random.default_rng(seed)
     lines = [np.arange(n, dtype=np.float64) for n in num_lines]
     op = Operator()
@@ -186,7 +190,7 @@ def large_dipole_scenario(num_lines=(1537, 1537, 769), steps=20000, seed=SEED_BA
     z1 = lines[2][len(lines[2]) // 2 - dipole_cells // 2 + dipole_cells]
     e = op.add_excitation(0, (0.0, 0.0, 1.0))
     op.add_box(e, (c[0], c[1], z0), (c[0], c[1], z1), priority=10)
-    data = op.build_streamed(nr_ts=steps, max_entries=max_entries)
+    data = op.build_streamed(nr_ts=steps, max_entries=max_entries) if streamed else op.build(prec, nr_ts=steps)
     interval = max(1, data.nyquist // 4)
     feed = (c[0], c[1], lines[2][len(lines[2]) // 2])
     feed2 = (c[0], c[1], lines[2][len(lines[2]) // 2 + 1])
