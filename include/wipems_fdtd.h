@@ -292,6 +292,17 @@ int  fdtd_get_kernel_times(fdtd_engine* h, double* volt_ms, double* curr_ms,
 
 /* ---- field access ---- */
 int  fdtd_calc_energy(fdtd_engine* h, double* energy);
+/* fdtd_calc_energy without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc), which fdtd_calc_energy does on its first call per handle.  The
+ * partial sums live in a library-owned static device array of 2 * 2048 + 2
+ * doubles (32 KiB) per GPU, held until the process ends; the reduction uses
+ * 2048 blocks for every domain size, so no domain exceeds it.  The result is
+ * bitwise equal to fdtd_calc_energy's.  Every call on one GPU shares that
+ * array: calls that overlap on one GPU (two handles on two threads or
+ * streams) race and must be serialised by the caller. */
+int  fdtd_calc_energy_noalloc(fdtd_engine* h, double* energy);
 /* Per-component max |value| of the volt (FDTD_VOLT) or curr field in
  * physical units, reduced on the device.  No counterpart in the reference:
  * the storage-scale pilot of FP16/BF16 handles reads it instead of copying

@@ -81,6 +81,7 @@ _ENGINE_SIGS = {
     "fdtd_set_kernel_timing": [c_void_p, c_int],
     "fdtd_get_kernel_times": [c_void_p, POINTER(c_double), POINTER(c_double), POINTER(c_uint32), POINTER(c_uint32)],
     "fdtd_calc_energy": [c_void_p, POINTER(c_double)],
+    "fdtd_calc_energy_noalloc": [c_void_p, POINTER(c_double)],
     "fdtd_field_absmax": [c_void_p, c_int, POINTER(c_double)],
     "fdtd_get_field": [c_void_p, c_int, c_int, c_void_p],
     "fdtd_set_field": [c_void_p, c_int, c_int, c_void_p],

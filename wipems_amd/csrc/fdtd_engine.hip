@@ -592,7 +592,40 @@ int ready_check(fdtd_engine* h) {
     return FDTD_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Energy reduction without allocation (fdtd_calc_energy_noalloc): the
+// per-block partial sums and the two totals live in one file-scope device
+// array per GPU instead of buffers the handle allocates.  The reduction runs
+// a fixed number of blocks whatever the domain size, so this array bounds
+// the block count only.  Host code never passes `scratch` itself to a kernel
+// or a copy: it uses the address hipGetSymbolAddress gives for the device
+// that is current, fetched once per device when the first handle on it is
+// created and kept in g_scratch_ptr.
+// ---------------------------------------------------------------------------
 #define ENERGY_BLOCKS 2048
+#define SCRATCH_DOUBLES (2 * ENERGY_BLOCKS + 2)
+#define MAX_SCRATCH_DEVICES 64
+static __device__ __align__(256) double scratch[SCRATCH_DOUBLES];
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[MAX_SCRATCH_DEVICES] = {};
+
+// called with device `dev` current
+static int init_scratch(int dev) {
+    if (dev < 0 || dev >= MAX_SCRATCH_DEVICES) return fail(FDTD_ERR_RANGE, "device %d beyond the %d the scratch table holds", dev, MAX_SCRATCH_DEVICES);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[dev]) return FDTD_OK;
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(FDTD_ERR_HIP, "hipGetSymbolAddress(scratch) gave a null address on device %d", dev);
+    g_scratch_ptr[dev] = (double*)p;
+    return FDTD_OK;
+}
+
+static double* scratch_of(int dev) {
+    if (dev < 0 || dev >= MAX_SCRATCH_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_ptr[dev];
+}
 
 int ensure_energy_buffers(fdtd_engine* h) {
     int rc;
@@ -622,6 +655,24 @@ int absmax_impl(fdtd_engine* h, int which, double out[3]) {
         for (unsigned b = 0; b < blocks; ++b) m = std::max(m, r[(size_t)n * blocks + b]);
         out[n] = m;
     }
+    return FDTD_OK;
+}
+
+// the same reduction as energy_impl with its partial sums and totals in the
+// device's scratch array; allocates nothing
+template <typename S>
+int energy_noalloc_impl(fdtd_engine* h, double* part, double* out) {
+    typedef typename StoreTraits<S>::A A;
+    const int blocks = ENERGY_BLOCKS;
+    if ((size_t)2 * blocks + 2 > (size_t)SCRATCH_DOUBLES)
+        return fail(FDTD_ERR_RANGE, "energy reduction of %d blocks needs %d doubles of scratch, %d available", blocks, 2 * blocks + 2, SCRATCH_DOUBLES);
+    hipLaunchKernelGGL((k_energy_partial<S>), dim3(blocks), dim3(256), 0, h->stream, h->V.as<S>(), h->I.as<S>(), h->d, make_scales<A>(h), part);
+    hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(256), 0, h->stream, part, blocks, part + 2 * blocks);
+    HIPCHK(hipGetLastError());
+    double r[2];
+    HIPCHK(hipMemcpyAsync(r, part + 2 * blocks, sizeof(r), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    *out = kEPS0 * r[0] + kMUE0 * r[1];
     return FDTD_OK;
 }
 
@@ -849,6 +900,7 @@ int fdtd_create(const fdtd_grid_desc* desc, fdtd_engine** out) {
     hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { delete h; return fail(FDTD_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e)); }
     int rc;
+    if ((rc = init_scratch(h->dev))) { delete h; return rc; }
     const size_t fbytes = (size_t)3 * h->d.sc * h->elem;
     if ((rc = h->V.alloc(fbytes)) || (rc = h->I.alloc(fbytes)) || (rc = h->step.alloc(sizeof(unsigned)))) { delete h; return rc; }
     if (hipMemsetAsync(h->V.p, 0, fbytes, h->stream) != hipSuccess || hipMemsetAsync(h->I.p, 0, fbytes, h->stream) != hipSuccess ||
@@ -1733,6 +1785,18 @@ int fdtd_field_absmax(fdtd_engine* h, int which, double out[3]) {
     case FDTD_PREC_FP64: return absmax_impl<double>(h, which, out);
     case FDTD_PREC_FP16: return absmax_impl<__half>(h, which, out);
     default: return absmax_impl<__hip_bfloat16>(h, which, out);
+    }
+}
+
+int fdtd_calc_energy_noalloc(fdtd_engine* h, double* energy) {
+    if (!h || !energy) return fail(FDTD_ERR_ARG, "null argument");
+    double* part = scratch_of(h->dev);
+    if (!part) return fail(FDTD_ERR_STATE, "no scratch address for device %d", h->dev);
+    switch (h->prec) {
+    case FDTD_PREC_FP32: return energy_noalloc_impl<float>(h, part, energy);
+    case FDTD_PREC_FP64: return energy_noalloc_impl<double>(h, part, energy);
+    case FDTD_PREC_FP16: return energy_noalloc_impl<__half>(h, part, energy);
+    default: return energy_noalloc_impl<__hip_bfloat16>(h, part, energy);
     }
 }
 

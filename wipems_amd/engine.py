@@ -309,6 +309,13 @@ class Engine:
 
     CalcFastEnergy = energy
 
+    def energy_noalloc(self):
+        """The same energy through fdtd_calc_energy_noalloc (no device
+        allocation; partial sums in the library's static scratch array)."""
+        e = c_double()
+        self._chk(self._lib.fdtd_calc_energy_noalloc(self._h, byref(e)))
+        return e.value
+
     def field_absmax(self, which):
         """Per-component max |value| of the volt (0) or curr (1) field, on the device."""
         out = (c_double * 3)()
