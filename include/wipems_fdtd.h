@@ -259,25 +259,22 @@ int  fdtd_set_tuning(fdtd_engine* h, int variant, int tile_y, int chunk_x);
 /* UPML coefficients: the vector kernels can read per-cell uint16 row
  * indices into a table of distinct coefficient tuples (possible when each
  * side has at most 65536 distinct tuples) instead of the full slab arrays.
- * mode 1 = tables, 0 = full arrays, -1 = tables only together with full
- * main coefficients (default: with compressed main coefficients the extra
- * gathers cost more than the bytes they save).  fdtd_pml_compressed
- * reports whether the tables are in use. */
+ * mode 1 = tables, 0 = full arrays, -1 = tables together with full main
+ * coefficients or with the interior/shell split (default: a single launch
+ * with compressed main coefficients loses more to the extra gathers than it
+ * saves in bytes).  fdtd_pml_compressed reports whether the tables are in
+ * use. */
 int  fdtd_set_pml_compression(fdtd_engine* h, int mode);
 int  fdtd_pml_compressed(fdtd_engine* h, int* active);
 /* UPML shell launch (vector kernels): when the slabs form a shell around an
  * interior box (as the operator's UPML does), the interior can run in a
  * launch without UPML code and the shell in a launch of its own, instead of
- * one launch whose wavefronts branch into the UPML form.  mode 0 = one
- * launch; 1 = split, the shell launch holding the x-slab planes, the y-slab
- * rows and the z-slab ends of the interior rows; 2 = split in rows form: the
- * interior launch sweeps the interior rows in full and applies the z slabs
- * at their two ends itself (only vectors reaching out of the interior z
- * range search the slabs), the shell launch holds the x-slab planes and
- * y-slab rows only, all contiguous runs of memory; -1 = the default (split
- * when the interior holds at least 88% of the cells for FP16/BF16 storage
- * or 95% for FP32, one launch for FP64).  Results are identical in every
- * form.  fdtd_pml_split reports whether a split form is in use. */
+ * one launch whose wavefronts branch into the UPML form.  mode 1 = split,
+ * 0 = one launch, -1 = split when the interior holds at least 88% of the
+ * cells for FP16/BF16 storage or 95% for FP32, one launch for FP64 (default:
+ * otherwise the shell's scattered vectors cost more than the branching they
+ * remove).  Results are identical either way.  fdtd_pml_split reports
+ * whether the split launches are in use. */
 int  fdtd_set_pml_split(fdtd_engine* h, int mode);
 int  fdtd_pml_split(fdtd_engine* h, int* active);
 /* Use a captured HIP graph for the step loop (default on). */

@@ -1,9 +1,8 @@
 // Host-only check of the split vector launches (fdtd_shell.h): for UPML slab
 // sets read from stdin, every vector the update bodies process is covered
 // exactly once by the interior launch plus the shell launch, the interior
-// launch touches no slab cell (rows form: only vectors whose z range leaves
-// the gate [zin0, zin1), which take the UPML path, touch slab cells), and
-// the shell enumeration stays inside the field arrays.  Input: cases of "N0 N1 N2 count" followed by count lines
+// launch touches no slab cell, and the shell enumeration stays inside the
+// field arrays.  Input: cases of "N0 N1 N2 count" followed by count lines
 // "s0 s1 s2 n0 n1 n2".  Prints one line per case and vector width; exit 1
 // on the first violation.
 #include <cstdio>
@@ -11,11 +10,11 @@
 #include <vector>
 #include "fdtd_shell.h"
 
-static int check(const uint32_t N[3], const std::vector<PmlSlab>& slabs, int vz, bool curr, bool rows) {
+static int check(const uint32_t N[3], const std::vector<PmlSlab>& slabs, int vz, bool curr) {
     int ibox[3][2] = {{0, 0}, {0, 0}, {0, 0}};
     const bool ok = find_interior(N, slabs.data(), (int)slabs.size(), ibox);
     const long long sy = ((long long)N[2] + 15) / 16 * 16;
-    const VecLaunch L = plan_vec_launch(N, sy, (int)slabs.size(), ok, ibox, vz, curr, rows);
+    const VecLaunch L = plan_vec_launch(N, sy, (int)slabs.size(), ok, ibox, vz, curr);
     const int nx = (int)N[0], ny = (int)N[1], nz = (int)N[2];
     const int vrow = (int)(sy / vz);
     std::vector<unsigned char> cnt((size_t)nx * ny * vrow, 0);
@@ -40,8 +39,7 @@ static int check(const uint32_t N[3], const std::vector<PmlSlab>& slabs, int vz,
             if (y >= L.r.y1 || z0 < L.r.zlo || z0 + vz > L.r.zhi) continue;
             const int x = L.r.x0 + (int)bx;
             if (!body(x, y, z0)) continue;
-            const bool gated = L.rows && (z0 < L.r.zin0 || z0 + vz > L.r.zin1);
-            if (L.shell && !gated && in_slab(x, y, z0)) { printf("interior launch reaches a slab cell at %d %d %d\n", x, y, z0); return 1; }
+            if (L.shell && in_slab(x, y, z0)) { printf("interior launch reaches a slab cell at %d %d %d\n", x, y, z0); return 1; }
             ++cnt[((size_t)x * ny + y) * vrow + z0 / vz];
             ++main_vecs;
         }
@@ -75,8 +73,8 @@ static int check(const uint32_t N[3], const std::vector<PmlSlab>& slabs, int vz,
                     return 1;
                 }
             }
-    printf("N %u %u %u slabs %zu shell %d vz %d %s%s: interior %lld shell %lld\n", N[0], N[1], N[2], slabs.size(), (int)ok, vz,
-           curr ? "curr" : "volt", rows ? " rows" : "", main_vecs, shell_vecs);
+    printf("N %u %u %u slabs %zu shell %d vz %d %s: interior %lld shell %lld\n", N[0], N[1], N[2], slabs.size(), (int)ok, vz,
+           curr ? "curr" : "volt", main_vecs, shell_vecs);
     return 0;
 }
 
@@ -92,8 +90,7 @@ int main() {
         }
         for (int vz : {2, 4, 8})
             for (bool curr : {false, true})
-                for (bool rows : {false, true})
-                    if (check(N, slabs, vz, curr, rows)) return 1;
+                if (check(N, slabs, vz, curr)) return 1;
     }
     return 0;
 }

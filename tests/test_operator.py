@@ -157,3 +157,99 @@ def test_gaussian_signal_constants():
     np.testing.assert_array_equal(d.excitation.signal_volt, ref.astype(np.float32).astype(np.float64))
     assert d.max_excite_step == int(np.argmax(np.abs(ref.astype(np.float32))))
     assert d.excitation.signal_volt[0] == np.float32(math.exp(-9))
+
+
+def _exact_integral(lo, hi, breaks, fn):
+    """Exact integral of a piecewise-constant fn over [lo, hi]: fn is
+    evaluated at the midpoint of every piece between the breakpoints."""
+    pts = sorted({lo, hi} | {b for b in breaks if lo < b < hi})
+    return sum((b - a) * fn(0.5 * (a + b)) for a, b in zip(pts[:-1], pts[1:]))
+
+
+def test_quarter_cell_averaging_brute_force_graded():
+    """EC of every interior edge on small random graded meshes with random
+    overlapping boxes (corners on mesh lines, priorities), against the
+    physical definition, integrated exactly piece by piece over the
+    material geometry (Operator::Calc_ECPos / AverageMatQuarterCell,
+    FDTD/operator.cpp:1176-1224,1406-1518):
+      C = eps0 * (integral of eps_r over the dual face) / edge length,
+      G = (integral of kappa over the dual face) / edge length,
+      L = mu0 * primary face area / (integral of dl/mu_r along the dual edge),
+      R = primary face area / (integral of dl/sigma along the dual edge)."""
+    rng = np.random.default_rng(20261204)
+    for trial in range(5):
+        ncell = [int(v) for v in rng.integers(2, 4, 3)]
+        lines = [np.concatenate([[0.0], np.cumsum(rng.uniform(0.5, 2.0, k))]) for k in ncell]
+        # an extra line outside the boxes' reach keeps every edge of the
+        # box region interior
+        lines = [np.concatenate([[-1.3], l, [l[-1] + 1.7]]) for l in lines]
+        op = Operator()
+        op.set_mesh(lines, 1e-3)
+        op.set_boundary([BC_PEC] * 6)
+        op.set_gauss_excite(0, 5e9)
+        boxes = []
+        # a background box with sigma > 0 everywhere (a zero sigma in one half
+        # of a dual edge hits the reference's sigma quirk, not checked here)
+        bg = dict(eps=1.0, kappa=0.0, mue=1.0, sigma=0.5)
+        m = op.add_material(eps_r=bg["eps"], kappa=bg["kappa"], mue_r=bg["mue"], sigma=bg["sigma"])
+        op.add_box(m, tuple(l[0] - 1 for l in lines), tuple(l[-1] + 1 for l in lines), priority=0)
+        for k in range(3):
+            a = [int(rng.integers(0, len(l) - 1)) for l in lines]
+            b = [int(rng.integers(ai + 1, len(l))) for ai, l in zip(a, lines)]
+            mat = dict(eps=rng.uniform(1, 10), kappa=rng.uniform(0, 1), mue=rng.uniform(1, 4), sigma=rng.uniform(0.2, 3))
+            m = op.add_material(eps_r=mat["eps"], kappa=mat["kappa"], mue_r=mat["mue"], sigma=mat["sigma"])
+            lo = tuple(lines[n][a[n]] for n in range(3))
+            hi = tuple(lines[n][b[n]] for n in range(3))
+            op.add_box(m, lo, hi, priority=k + 1)
+            boxes.append((k + 1, lo, hi, mat))
+        op.build(1, 10)
+        C, G, L, R = op.get_ec()
+
+        def mat_at(p):
+            best = (0, bg)
+            for pr, lo, hi, mat in boxes:
+                if all(lo[n] <= p[n] <= hi[n] for n in range(3)) and pr > best[0]:
+                    best = (pr, mat)
+            return best[1]
+
+        breaks = [sorted(set(lines[n]) | {c for _, lo, hi, _ in boxes for c in (lo[n], hi[n])}) for n in range(3)]
+        N = [len(l) for l in lines]
+        unit = 1e-3
+        checked = 0
+        for n in range(3):
+            nP, nPP = (n + 1) % 3, (n + 2) % 3
+            for pos in np.ndindex(*N):
+                # E edge n at pos: primary edge [l_n[p], l_n[p+1]], dual face spanning the half cells around it
+                if 1 <= pos[nP] <= N[nP] - 2 and 1 <= pos[nPP] <= N[nPP] - 2 and pos[n] <= N[n] - 2:
+                    ln, lP, lPP = lines[n], lines[nP], lines[nPP]
+                    d = ln[pos[n] + 1] - ln[pos[n]]
+                    zc = 0.5 * (ln[pos[n]] + ln[pos[n] + 1])
+                    pa, pb = 0.5 * (lP[pos[nP] - 1] + lP[pos[nP]]), 0.5 * (lP[pos[nP]] + lP[pos[nP] + 1])
+                    qa, qb = 0.5 * (lPP[pos[nPP] - 1] + lPP[pos[nPP]]), 0.5 * (lPP[pos[nPP]] + lPP[pos[nPP] + 1])
+
+                    def face(key):
+                        def at(u, v):
+                            p = [0.0] * 3
+                            p[n], p[nP], p[nPP] = zc, u, v
+                            return mat_at(p)[key]
+                        return _exact_integral(pa, pb, breaks[nP], lambda u: _exact_integral(qa, qb, breaks[nPP], lambda v: at(u, v)))
+                    assert C[n][pos] == pytest.approx(EPS0 * face("eps") / d * unit, rel=1e-12), (trial, n, pos)
+                    assert G[n][pos] == pytest.approx(face("kappa") / d * unit, rel=1e-12, abs=1e-300), (trial, n, pos)
+                    checked += 1
+                # H edge n at pos: dual edge between the cell centres around line pos[n], primary face of cell (pos[nP], pos[nPP])
+                if 1 <= pos[n] <= N[n] - 2 and pos[nP] <= N[nP] - 2 and pos[nPP] <= N[nPP] - 2:
+                    ln, lP, lPP = lines[n], lines[nP], lines[nPP]
+                    ua, ub = 0.5 * (ln[pos[n] - 1] + ln[pos[n]]), 0.5 * (ln[pos[n]] + ln[pos[n] + 1])
+                    pc, qc = 0.5 * (lP[pos[nP]] + lP[pos[nP] + 1]), 0.5 * (lPP[pos[nPP]] + lPP[pos[nPP] + 1])
+                    area = (lP[pos[nP] + 1] - lP[pos[nP]]) * (lPP[pos[nPP] + 1] - lPP[pos[nPP]])
+
+                    def line(key):
+                        def at(u):
+                            p = [0.0] * 3
+                            p[n], p[nP], p[nPP] = u, pc, qc
+                            return 1.0 / mat_at(p)[key]
+                        return _exact_integral(ua, ub, breaks[n], at)
+                    assert L[n][pos] == pytest.approx(MUE0 * area / line("mue") * unit, rel=1e-12), (trial, n, pos)
+                    assert R[n][pos] == pytest.approx(area / line("sigma") * unit, rel=1e-12), (trial, n, pos)
+                    checked += 1
+        assert checked > 20

@@ -62,7 +62,7 @@ def test_operator_shells(harness):
         assert slabs
         cases.append((N, slabs))
     lines = run(harness, cases)
-    assert len(lines) == 12 * len(cases)
+    assert len(lines) == 6 * len(cases)
     assert all(" shell 1 " in l for l in lines)
 
 
@@ -77,4 +77,4 @@ def test_non_shell_slabs_use_one_launch(harness):
 def test_no_upml_single_launch(harness):
     # no slabs: one launch over everything, XCD-contiguous block order
     lines = run(harness, [((37, 29, 45), []), ((9, 300, 17), [])])
-    assert len(lines) == 24 and all(" shell 0 " in l for l in lines)
+    assert len(lines) == 12 and all(" shell 0 " in l for l in lines)

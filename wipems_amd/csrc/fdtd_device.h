@@ -291,18 +291,14 @@ __device__ __forceinline__ void vload_coeffs(const CoeffPtrs<A>& cf, long long p
     }
 }
 
-// one vector of VZ cells at plane x, flat (y,z) offset q0 of the padded plane
-template <typename S, int CM, bool PML, int VZ, bool ZG = false>
-__device__ __forceinline__ void volt_vec_body(S* __restrict__ V, const S* __restrict__ I, const CoeffPtrs<typename StoreTraits<S>::A>& cf,
-                                              const Dims& d, const Scales<typename StoreTraits<S>::A>& sc, const PmlSet& ps,
-                                              const PmlPtrs<typename StoreTraits<S>::A>& pp, int x, long long q0,
-                                              int zin0 = 0, int zin1 = 0) {
+// curl of the voltage update for the vector z0..z0+VZ-1 at (x, y), p = its
+// element offset: reads 3 + 4 current components (x-1, y-1 neighbours; the
+// z-1 neighbour of the first cell, zero at z = 0)
+template <typename S, int VZ>
+__device__ __forceinline__ void volt_curl(const S* __restrict__ I, const Dims& d, const Scales<typename StoreTraits<S>::A>& sc,
+                                          int x, int y, int z0, long long p, typename StoreTraits<S>::A curl[3][VZ]) {
     typedef StoreTraits<S> T;
     typedef typename T::A A;
-    const int y = (int)(q0 / d.sy);
-    const int z0 = (int)(q0 - (long long)y * d.sy);
-    if (y >= d.ny || z0 >= d.nz) return;
-    const long long p = x * d.sx + q0;
     const long long c1 = d.sc, c2 = 2 * d.sc;
     const long long ox = x > 0 ? d.sx : 0;
     const long long oy = y > 0 ? d.sy : 0;
@@ -316,7 +312,6 @@ __device__ __forceinline__ void volt_vec_body(S* __restrict__ V, const S* __rest
     vload<S, VZ>(I + c1 + p - ox, i1_xm, sc.c[1]);
     const A i1_zm0 = z0 > 0 ? T::load(I + c1 + p - 1, sc.c[1]) : A(0);
     const A i0_zm0 = z0 > 0 ? T::load(I + p - 1, sc.c[0]) : A(0);
-    A curl[3][VZ];
 #pragma unroll
     for (int k = 0; k < VZ; ++k) {
         const A i1_zm = k == 0 ? i1_zm0 : i1[k - 1];
@@ -325,13 +320,22 @@ __device__ __forceinline__ void volt_vec_body(S* __restrict__ V, const S* __rest
         curl[1][k] = ((i0[k] - i0_zm) - i2[k]) + i2_xm[k];
         curl[2][k] = ((i1[k] - i1_xm[k]) - i0[k]) + i0_ym[k];
     }
+}
+
+// the voltage update of one vector from its curl: V = V*vv + vi*curl, with
+// the UPML form for cells of the (at most two) slabs the vector touches
+template <typename S, int CM, bool PML, int VZ>
+__device__ __forceinline__ void volt_apply(S* __restrict__ V, const CoeffPtrs<typename StoreTraits<S>::A>& cf, const Dims& d,
+                                           const Scales<typename StoreTraits<S>::A>& sc, const PmlSet& ps,
+                                           const PmlPtrs<typename StoreTraits<S>::A>& pp, int x, int y, int z0, long long p,
+                                           const typename StoreTraits<S>::A curl[3][VZ]) {
+    typedef typename StoreTraits<S>::A A;
     A vv[3][VZ], vi[3][VZ];
     vload_coeffs<A, CM, VZ>(cf, p, d.sc, vv, vi);
     const int valid = min(VZ, d.nz - z0);
     SlabHits hit{{-1, -1}, {0u, 0u}, {0, 0}};
     SVec<unsigned short, VZ> prow[2] = {};
-    // ZG: the slab search only for vectors reaching out of [zin0, zin1)
-    if (PML && (!ZG || z0 < zin0 || z0 + VZ > zin1)) {
+    if (PML) {
         hit = slab_hits<VZ>(ps, x, y, z0, ~0u);
 #pragma unroll
         for (int j = 0; j < 2; ++j)
@@ -350,7 +354,6 @@ __device__ __forceinline__ void volt_vec_body(S* __restrict__ V, const S* __rest
             out[k] = out[k] + vi[n][k] * curl[n][k];
         }
         if (PML) {
-            // the UPML form for the cells of the slabs the vector touches
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
                 if (hit.s[j] < 0) continue;
@@ -363,6 +366,21 @@ __device__ __forceinline__ void volt_vec_body(S* __restrict__ V, const S* __rest
     }
 }
 
+// one vector of VZ cells at plane x, flat (y,z) offset q0 of the padded plane
+template <typename S, int CM, bool PML, int VZ>
+__device__ __forceinline__ void volt_vec_body(S* __restrict__ V, const S* __restrict__ I, const CoeffPtrs<typename StoreTraits<S>::A>& cf,
+                                              const Dims& d, const Scales<typename StoreTraits<S>::A>& sc, const PmlSet& ps,
+                                              const PmlPtrs<typename StoreTraits<S>::A>& pp, int x, long long q0) {
+    typedef typename StoreTraits<S>::A A;
+    const int y = (int)(q0 / d.sy);
+    const int z0 = (int)(q0 - (long long)y * d.sy);
+    if (y >= d.ny || z0 >= d.nz) return;
+    const long long p = x * d.sx + q0;
+    A curl[3][VZ];
+    volt_curl<S, VZ>(I, d, sc, x, y, z0, p, curl);
+    volt_apply<S, CM, PML, VZ>(V, cf, d, sc, ps, pp, x, y, z0, p, curl);
+}
+
 // Vectors one launch of k_volt_vec / k_curr_vec covers: planes x0 + blockIdx.y,
 // rows y0..y1-1 (flat (y, z) index from y0 over the padded rows), and only
 // vectors whose cells z0..z0+VZ-1 lie in [zlo, zhi).  With UPML slabs forming
@@ -370,10 +388,9 @@ __device__ __forceinline__ void volt_vec_body(S* __restrict__ V, const S* __rest
 // PML code and k_volt_shell / k_curr_shell take every other vector.
 struct VecRange {
     int x0, y0, y1, zlo, zhi;
-    int zin0, zin1;        // rows form: vectors outside [zin0, zin1) take the UPML path
 };
 
-template <typename S, int CM, bool PML, int VZ, bool ZG = false>
+template <typename S, int CM, bool PML, int VZ>
 __device__ __forceinline__ void volt_vec_range(S* __restrict__ V, const S* __restrict__ I, const CoeffPtrs<typename StoreTraits<S>::A>& cf,
                                                const Dims& d, const Scales<typename StoreTraits<S>::A>& sc, const PmlSet& ps,
                                                const PmlPtrs<typename StoreTraits<S>::A>& pp, const VecRange& r) {
@@ -381,7 +398,7 @@ __device__ __forceinline__ void volt_vec_range(S* __restrict__ V, const S* __res
     const int y = (int)(q0 / d.sy);
     const int z0 = (int)(q0 - (long long)y * d.sy);
     if (y >= r.y1 || z0 < r.zlo || z0 + VZ > r.zhi) return;
-    volt_vec_body<S, CM, PML, VZ, ZG>(V, I, cf, d, sc, ps, pp, r.x0 + blockIdx.y, q0, r.zin0, r.zin1);
+    volt_vec_body<S, CM, PML, VZ>(V, I, cf, d, sc, ps, pp, r.x0 + blockIdx.y, q0);
 }
 
 // Without UPML code the voltage update fits 5 waves per SIMD for every
@@ -411,26 +428,6 @@ __global__ void __launch_bounds__(256, 5)
 k_volt_vec_upml_b5(S* __restrict__ V, const S* __restrict__ I, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
                    Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp, VecRange r) {
     volt_vec_range<S, CM, true, VZ>(V, I, cf, d, sc, ps, pp, r);
-}
-
-// Rows form of the split: the interior rows in full, UPML applied to the
-// vectors that reach the z slabs at the two ends of each row (the slab
-// search runs only for those), so the z-slab cells are read in the same
-// sweep as the row instead of in a scattered shell pass.  The bound keeps
-// 5 waves per SIMD for 4- and 8-byte storage (at most 8 VGPRs spilled);
-// 2-byte storage would spill 75 under it and runs the unbounded kernel.
-template <typename S, int CM, int VZ>
-__global__ void __launch_bounds__(256, 5)
-k_volt_rows(S* __restrict__ V, const S* __restrict__ I, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
-            Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp, VecRange r) {
-    volt_vec_range<S, CM, true, VZ, true>(V, I, cf, d, sc, ps, pp, r);
-}
-
-template <typename S, int CM, int VZ>
-__global__ void __launch_bounds__(256)
-k_volt_rows_u(S* __restrict__ V, const S* __restrict__ I, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
-              Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp, VecRange r) {
-    volt_vec_range<S, CM, true, VZ, true>(V, I, cf, d, sc, ps, pp, r);
 }
 
 // The vectors outside the interior box of a UPML shell, numbered
@@ -488,17 +485,14 @@ k_volt_shell(S* __restrict__ V, const S* __restrict__ I, CoeffPtrs<typename Stor
     volt_vec_body<S, CM, true, VZ>(V, I, cf, d, sc, ps, pp, x, q0);
 }
 
-template <typename S, int CM, bool PML, int VZ, bool ZG = false>
-__device__ __forceinline__ void curr_vec_body(S* __restrict__ I, const S* __restrict__ V, const CoeffPtrs<typename StoreTraits<S>::A>& cf,
-                                              const Dims& d, const Scales<typename StoreTraits<S>::A>& sc, const PmlSet& ps,
-                                              const PmlPtrs<typename StoreTraits<S>::A>& pp, int x, long long q0,
-                                              int zin0 = 0, int zin1 = 0) {
+// curl of the current update for the vector z0..z0+VZ-1 at (x, y): reads 3
+// + 4 voltage components (x+1, y+1 neighbours; the z+1 neighbour of the
+// last cell only where it is a valid line)
+template <typename S, int VZ>
+__device__ __forceinline__ void curr_curl(const S* __restrict__ V, const Dims& d, const Scales<typename StoreTraits<S>::A>& sc,
+                                          int z0, long long p, typename StoreTraits<S>::A curl[3][VZ]) {
     typedef StoreTraits<S> T;
     typedef typename T::A A;
-    const int y = (int)(q0 / d.sy);
-    const int z0 = (int)(q0 - (long long)y * d.sy);
-    if (y >= d.ny - 1 || z0 >= d.nz - 1) return;
-    const long long p = x * d.sx + q0;
     const long long c1 = d.sc, c2 = 2 * d.sc;
     A v0[VZ], v1[VZ], v2[VZ], v2_yp[VZ], v0_yp[VZ], v2_xp[VZ], v1_xp[VZ];
     vload<S, VZ>(V + p, v0, sc.v[0]);
@@ -508,11 +502,9 @@ __device__ __forceinline__ void curr_vec_body(S* __restrict__ I, const S* __rest
     vload<S, VZ>(V + p + d.sy, v0_yp, sc.v[0]);
     vload<S, VZ>(V + c2 + p + d.sx, v2_xp, sc.v[2]);
     vload<S, VZ>(V + c1 + p + d.sx, v1_xp, sc.v[1]);
-    // z+1 neighbour of the last element: only needed when it is a valid line
     const bool tail = z0 + VZ < d.nz;
     const A v1_zpl = tail ? T::load(V + c1 + p + VZ, sc.v[1]) : A(0);
     const A v0_zpl = tail ? T::load(V + p + VZ, sc.v[0]) : A(0);
-    A curl[3][VZ];
 #pragma unroll
     for (int k = 0; k < VZ; ++k) {
         const A v1_zp = k == VZ - 1 ? v1_zpl : v1[k + 1];
@@ -521,6 +513,15 @@ __device__ __forceinline__ void curr_vec_body(S* __restrict__ I, const S* __rest
         curl[1][k] = ((v0[k] - v0_zp) - v2[k]) + v2_xp[k];
         curl[2][k] = ((v1[k] - v1_xp[k]) - v0[k]) + v0_yp[k];
     }
+}
+
+// the current update of one vector from its curl, UPML form included
+template <typename S, int CM, bool PML, int VZ>
+__device__ __forceinline__ void curr_apply(S* __restrict__ I, const CoeffPtrs<typename StoreTraits<S>::A>& cf, const Dims& d,
+                                           const Scales<typename StoreTraits<S>::A>& sc, const PmlSet& ps,
+                                           const PmlPtrs<typename StoreTraits<S>::A>& pp, int x, int y, int z0, long long p,
+                                           const typename StoreTraits<S>::A curl[3][VZ]) {
+    typedef typename StoreTraits<S>::A A;
     A ii[3][VZ], iv[3][VZ];
     vload_coeffs<A, CM, VZ>(cf, p, d.sc, ii, iv);
     const int valid = min(VZ, d.nz - 1 - z0);
@@ -528,7 +529,7 @@ __device__ __forceinline__ void curr_vec_body(S* __restrict__ I, const S* __rest
     const unsigned upd = valid < VZ ? (1u << valid) - 1u : ~0u;
     SlabHits hit{{-1, -1}, {0u, 0u}, {0, 0}};
     SVec<unsigned short, VZ> prow[2] = {};
-    if (PML && (!ZG || z0 < zin0 || z0 + VZ > zin1)) {
+    if (PML) {
         hit = slab_hits<VZ>(ps, x, y, z0, upd);
 #pragma unroll
         for (int j = 0; j < 2; ++j)
@@ -559,10 +560,24 @@ __device__ __forceinline__ void curr_vec_body(S* __restrict__ I, const S* __rest
     }
 }
 
+template <typename S, int CM, bool PML, int VZ>
+__device__ __forceinline__ void curr_vec_body(S* __restrict__ I, const S* __restrict__ V, const CoeffPtrs<typename StoreTraits<S>::A>& cf,
+                                              const Dims& d, const Scales<typename StoreTraits<S>::A>& sc, const PmlSet& ps,
+                                              const PmlPtrs<typename StoreTraits<S>::A>& pp, int x, long long q0) {
+    typedef typename StoreTraits<S>::A A;
+    const int y = (int)(q0 / d.sy);
+    const int z0 = (int)(q0 - (long long)y * d.sy);
+    if (y >= d.ny - 1 || z0 >= d.nz - 1) return;
+    const long long p = x * d.sx + q0;
+    A curl[3][VZ];
+    curr_curl<S, VZ>(V, d, sc, z0, p, curl);
+    curr_apply<S, CM, PML, VZ>(I, cf, d, sc, ps, pp, x, y, z0, p, curl);
+}
+
 // REV: blocks walk planes and tiles in descending order, so the x+1 and
 // y+1 neighbours a block reads were loaded by the blocks just before it
 // (the mirror of the voltage update's x-1 / y-1 neighbours)
-template <typename S, int CM, bool PML, int VZ, bool REV = false, bool ZG = false>
+template <typename S, int CM, bool PML, int VZ, bool REV = false>
 __device__ __forceinline__ void curr_vec_range(S* __restrict__ I, const S* __restrict__ V, const CoeffPtrs<typename StoreTraits<S>::A>& cf,
                                                const Dims& d, const Scales<typename StoreTraits<S>::A>& sc, const PmlSet& ps,
                                                const PmlPtrs<typename StoreTraits<S>::A>& pp, const VecRange& r) {
@@ -572,7 +587,7 @@ __device__ __forceinline__ void curr_vec_range(S* __restrict__ I, const S* __res
     const int y = (int)(q0 / d.sy);
     const int z0 = (int)(q0 - (long long)y * d.sy);
     if (y >= r.y1 || z0 < r.zlo || z0 + VZ > r.zhi) return;
-    curr_vec_body<S, CM, PML, VZ, ZG>(I, V, cf, d, sc, ps, pp, r.x0 + px, q0, r.zin0, r.zin1);
+    curr_vec_body<S, CM, PML, VZ>(I, V, cf, d, sc, ps, pp, r.x0 + px, q0);
 }
 
 // FP32 storage walks the planes in descending order (512^3: 0.946 -> 0.907 ms
@@ -599,20 +614,6 @@ k_curr_vec_upml_b5(S* __restrict__ I, const S* __restrict__ V, CoeffPtrs<typenam
 }
 
 template <typename S, int CM, int VZ>
-__global__ void __launch_bounds__(256, 5)
-k_curr_rows(S* __restrict__ I, const S* __restrict__ V, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
-            Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp, VecRange r) {
-    curr_vec_range<S, CM, true, VZ, false, true>(I, V, cf, d, sc, ps, pp, r);
-}
-
-template <typename S, int CM, int VZ>
-__global__ void __launch_bounds__(256)
-k_curr_rows_u(S* __restrict__ I, const S* __restrict__ V, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
-              Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp, VecRange r) {
-    curr_vec_range<S, CM, true, VZ, false, true>(I, V, cf, d, sc, ps, pp, r);
-}
-
-template <typename S, int CM, int VZ>
 __global__ void __launch_bounds__(256)
 k_curr_shell(S* __restrict__ I, const S* __restrict__ V, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
              Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp, ShellGrid g) {
@@ -620,6 +621,58 @@ k_curr_shell(S* __restrict__ I, const S* __restrict__ V, CoeffPtrs<typename Stor
     long long q0;
     if (!shell_vector(g, (long long)blockIdx.x * blockDim.x + threadIdx.x, VZ, d.sy, x, q0)) return;
     curr_vec_body<S, CM, true, VZ>(I, V, cf, d, sc, ps, pp, x, q0);
+}
+
+// ---------------------------------------------------------------------------
+// UPML on the current cells the main update never touches (the planes
+// x = Nx-1, y = Ny-1 and z = Nz-1): Engine_Ext_UPML's pre and post hooks run
+// over every slab cell (engine_ext_upml.cpp:141-234) while
+// Engine::UpdateCurrents skips these, so for them pre + post alone give
+//   I' = (k*I - fo*F) + fn*F,  F unchanged.
+// They stay zero in a run from zero fields; values set there by SetCurr or
+// fdtd_set_field evolve as in the reference.  One thread per face cell:
+// face X (x = Nx-1, all y, z), face Y (y = Ny-1, x < Nx-1), face Z (z = Nz-1,
+// x < Nx-1, y < Ny-1).
+// ---------------------------------------------------------------------------
+template <typename S>
+__global__ void __launch_bounds__(256)
+k_curr_upml_faces(S* __restrict__ I, Dims d, Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp) {
+    typedef StoreTraits<S> T;
+    typedef typename T::A A;
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long fx = (long long)d.ny * d.nz, fy = (long long)(d.nx - 1) * d.nz, fz = (long long)(d.nx - 1) * (d.ny - 1);
+    int x, y, z;
+    if (t < fx) {
+        x = d.nx - 1; y = (int)(t / d.nz); z = (int)(t - (long long)y * d.nz);
+    } else if ((t -= fx) < fy) {
+        y = d.ny - 1; x = (int)(t / d.nz); z = (int)(t - (long long)x * d.nz);
+    } else if ((t -= fy) < fz) {
+        z = d.nz - 1; x = (int)(t / (d.ny - 1)); y = (int)(t - (long long)x * (d.ny - 1));
+    } else {
+        return;
+    }
+    for (int s = 0; s < ps.count; ++s) {
+        const PmlSlab& b = ps.slab[s];
+        const int lx = x - b.s[0], ly = y - b.s[1], lz = z - b.s[2];
+        if ((unsigned)lx >= (unsigned)b.n[0] || (unsigned)ly >= (unsigned)b.n[1] || (unsigned)lz >= (unsigned)b.n[2]) continue;
+        const long long l = ((long long)lx * b.n[1] + ly) * b.nzp + (z - b.za);   // component-0 slab offset from b.off
+        const long long p = x * d.sx + (long long)y * d.sy + z;
+        for (int n = 0; n < 3; ++n) {
+            const long long q = b.off + n * b.sc + l;
+            A k, fo, fn;
+            if (pp.ctab) {
+                const A* r = pp.ctab + (size_t)pp.cidx[l + b.ioff] * 12;
+                k = r[n]; fo = r[4 + n]; fn = r[8 + n];
+            } else {
+                k = pp.ii[q]; fo = pp.iifo[q]; fn = pp.iifn[q];
+            }
+            const A F = pp.cflux[q];
+            const A v = T::load(I + n * d.sc + p, sc.c[n]);
+            const A h = k * v - fo * F;
+            T::store(I + n * d.sc + p, h + fn * F, sc.ic[n]);
+        }
+        return;
+    }
 }
 
 // ---------------------------------------------------------------------------

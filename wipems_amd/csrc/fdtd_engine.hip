@@ -234,6 +234,11 @@ int finalize_pml(fdtd_engine* h) {
 
 void find_shell(fdtd_engine* h) {
     h->shell_ok = find_interior(h->N, h->pml.slab, h->pml.count, h->ibox);
+    // slabs reaching the planes the current update skips (k_curr_upml_faces)
+    h->pml_upper = false;
+    for (int s = 0; s < h->pml.count; ++s)
+        for (int n = 0; n < 3; ++n)
+            if (h->pml.slab[s].s[n] + h->pml.slab[s].n[n] >= (int)h->N[n]) h->pml_upper = true;
 }
 
 // Deduplicate the per-cell UPML coefficient 9-tuples (k, fo, fn for three
@@ -381,62 +386,56 @@ int launch_step_end(fdtd_engine* h, int advance) {
     return FDTD_OK;
 }
 
+// The skipped current cells on the upper faces (k_curr_upml_faces) stay
+// zero unless something writes them: a current-side hook or a caller's
+// SetCurr / fdtd_set_field on the current field.
+bool faces_live(const fdtd_engine* h) {
+    if (h->curr_written || (h->has_exc && h->exc_c.count) || h->tfsf_c.count) return true;
+    for (auto a : h->ade)
+        if ((a->flags & 2) && a->count) return true;
+    return false;
+}
+
 VecLaunch vec_launch(const fdtd_engine* h, int vz, bool curr) {
-    return plan_vec_launch(h->N, h->d.sy, h->pml.count, pml_split_in_use(h), h->ibox, vz, curr, pml_rows_form(h));
+    return plan_vec_launch(h->N, h->d.sy, h->pml.count, pml_split_in_use(h), h->ibox, vz, curr);
 }
 
 template <typename S, bool PML, int CM>
 void launch_volt_main(fdtd_engine* h, const VecLaunch& L, CoeffPtrs<typename StoreTraits<S>::A> cf, PmlPtrs<typename StoreTraits<S>::A> pp) {
     constexpr int VZ = VecWidth<S>::value;
     const dim3 b(256, 1, 1);
+    const long long nshell = L.shell ? L.g.nA0 + L.g.nA1 + L.g.nB + L.g.nC : 0;
     if (L.nx) {
         const dim3 g((unsigned)((L.nq / VZ + 255) / 256), L.nx, 1);
         if constexpr (PML && CM == CF_FULL && sizeof(S) >= 4)
             hipLaunchKernelGGL((k_volt_vec_upml_b5<S, CM, VZ>), g, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.r);
         else if constexpr (PML)
             hipLaunchKernelGGL((k_volt_vec_upml<S, CM, VZ>), g, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.r);
-        else if (L.rows) {
-            if constexpr (sizeof(S) >= 4)
-                hipLaunchKernelGGL((k_volt_rows<S, CM, VZ>), g, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.r);
-            else
-                hipLaunchKernelGGL((k_volt_rows_u<S, CM, VZ>), g, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.r);
-        }
         else
             hipLaunchKernelGGL((k_volt_vec<S, CM, VZ>), g, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.r);
     }
-    if (L.shell) {
-        const long long n = L.g.nA0 + L.g.nA1 + L.g.nB + L.g.nC;
-        if (n > 0)
-            hipLaunchKernelGGL((k_volt_shell<S, CM, VZ>), dim3((unsigned)((n + 255) / 256)), b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d,
-                               make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.g);
-    }
+    if (nshell > 0)
+        hipLaunchKernelGGL((k_volt_shell<S, CM, VZ>), dim3((unsigned)((nshell + 255) / 256)), b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d,
+                           make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.g);
 }
 
 template <typename S, bool PML, int CM>
 void launch_curr_main(fdtd_engine* h, const VecLaunch& L, CoeffPtrs<typename StoreTraits<S>::A> cf, PmlPtrs<typename StoreTraits<S>::A> pp) {
     constexpr int VZ = VecWidth<S>::value;
     const dim3 b(256, 1, 1);
+    const long long nshell = L.shell ? L.g.nA0 + L.g.nA1 + L.g.nB + L.g.nC : 0;
     if (L.nx) {
         const dim3 g((unsigned)((L.nq / VZ + 255) / 256), L.nx, 1);
         if constexpr (PML && CM == CF_FULL && sizeof(S) >= 4)
             hipLaunchKernelGGL((k_curr_vec_upml_b5<S, CM, VZ>), g, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.r);
         else if constexpr (PML)
             hipLaunchKernelGGL((k_curr_vec_upml<S, CM, VZ>), g, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.r);
-        else if (L.rows) {
-            if constexpr (sizeof(S) >= 4)
-                hipLaunchKernelGGL((k_curr_rows<S, CM, VZ>), g, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.r);
-            else
-                hipLaunchKernelGGL((k_curr_rows_u<S, CM, VZ>), g, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.r);
-        }
         else
             hipLaunchKernelGGL((k_curr_vec<S, CM, VZ>), g, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.r);
     }
-    if (L.shell) {
-        const long long n = L.g.nA0 + L.g.nA1 + L.g.nB + L.g.nC;
-        if (n > 0)
-            hipLaunchKernelGGL((k_curr_shell<S, CM, VZ>), dim3((unsigned)((n + 255) / 256)), b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d,
-                               make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.g);
-    }
+    if (nshell > 0)
+        hipLaunchKernelGGL((k_curr_shell<S, CM, VZ>), dim3((unsigned)((nshell + 255) / 256)), b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d,
+                           make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.g);
 }
 
 template <typename S>
@@ -555,6 +554,10 @@ int launch_step_kernels(fdtd_engine* h, bool timed) {
             hipLaunchKernelGGL((k_ade_pre<S>), dim3((a->count + 255) / 256), dim3(256), 0, h->stream, h->I.as<S>(), ade_dev(a), h->d.sc, sc, 1);
     if (timed) HIPCHK(hipEventRecord(h->ev[evi + 2], h->stream));
     launch_curr_vec<S>(h, cfh, pp);
+    if (pml && h->pml_upper && faces_live(h)) {
+        const long long nf = (long long)h->N[1] * h->N[2] + (long long)(h->N[0] - 1) * h->N[2] + (long long)(h->N[0] - 1) * (h->N[1] - 1);
+        hipLaunchKernelGGL((k_curr_upml_faces<S>), dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, h->stream, h->I.as<S>(), h->d, sc, h->pml, pp);
+    }
     if (timed) HIPCHK(hipEventRecord(h->ev[evi + 3], h->stream));
     if (h->tfsf_c.count)
         launch_tfsf<S>(h, h->I.as<S>(), stp, h->tfsf_c, sc, 1);
@@ -1606,7 +1609,7 @@ int fdtd_pml_compressed(fdtd_engine* h, int* active) {
 
 int fdtd_set_pml_split(fdtd_engine* h, int mode) {
     if (!h) return fail(FDTD_ERR_ARG, "null handle");
-    if (mode < -1 || mode > 2) return fail(FDTD_ERR_ARG, "mode must be -1 (auto), 0, 1 or 2");
+    if (mode < -1 || mode > 1) return fail(FDTD_ERR_ARG, "mode must be -1 (auto), 0 or 1");
     h->pml_split_mode = mode;
     invalidate_graph(h);
     return FDTD_OK;
@@ -1811,6 +1814,7 @@ int fdtd_get_field(fdtd_engine* h, int which, int dtype, void* out) {
 int fdtd_set_field(fdtd_engine* h, int which, int dtype, const void* in) {
     if (!h || !in) return fail(FDTD_ERR_ARG, "null argument");
     if (which != FDTD_VOLT && which != FDTD_CURR) return fail(FDTD_ERR_ARG, "bad field selector");
+    if (which == FDTD_CURR && !h->curr_written) { h->curr_written = true; invalidate_graph(h); }
     if (dtype == FDTD_DTYPE_F32) return field_dispatch_set<float>(h, which, (const float*)in);
     if (dtype == FDTD_DTYPE_F64) return field_dispatch_set<double>(h, which, (const double*)in);
     return fail(FDTD_ERR_ARG, "bad dtype");
@@ -1833,6 +1837,7 @@ int fdtd_set_value(fdtd_engine* h, int which, uint32_t n, uint32_t x, uint32_t y
     if (!h) return fail(FDTD_ERR_ARG, "null handle");
     if (which != FDTD_VOLT && which != FDTD_CURR) return fail(FDTD_ERR_ARG, "bad field selector");
     if (n > 2 || x >= h->N[0] || y >= h->N[1] || z >= h->N[2]) return fail(FDTD_ERR_RANGE, "position out of range");
+    if (which == FDTD_CURR && !h->curr_written) { h->curr_written = true; invalidate_graph(h); }
     const long long off = (long long)n * h->d.sc + cell_off(h, x, y, z);
     char raw[8];
     const double s = which == FDTD_VOLT ? h->vscale[n] : h->cscale[n];

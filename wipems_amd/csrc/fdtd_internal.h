@@ -166,8 +166,10 @@ struct fdtd_engine {
     // the vector kernels then run the interior without PML code and the
     // shell in a launch of its own
     bool shell_ok = false;
+    bool pml_upper = false;       // a slab reaches x = Nx-1, y = Ny-1 or z = Nz-1
+    bool curr_written = false;    // the caller wrote current values (SetCurr, fdtd_set_field)
     int ibox[3][2] = {{0, 0}, {0, 0}, {0, 0}};
-    int pml_split_mode = -1;      // -1 auto, 0 off, 1 split with a z-slab shell part, 2 rows form
+    int pml_split_mode = -1;      // -1 auto (FP16/BF16 storage), 0 off, 1 on
 
     // excitation
     bool has_exc = false;
@@ -253,20 +255,19 @@ int compress_pml(fdtd_engine* h);
 // 769x769x385 with 92% no gain); FP64 keeps one launch
 inline bool pml_split_in_use(const fdtd_engine* h) {
     if (!h->shell_ok || h->pml_split_mode == 0) return false;
-    if (h->pml_split_mode >= 1) return true;
+    if (h->pml_split_mode == 1) return true;
     if (h->elem != 2 && h->elem != 4) return false;
     long long inner = 1;
     for (int n = 0; n < 3; ++n) inner *= h->ibox[n][1] - h->ibox[n][0];
     return (double)inner >= (h->elem == 2 ? 0.88 : 0.95) * (double)h->N[0] * h->N[1] * h->N[2];
 }
 
-// rows form of the split (k_volt_rows / k_curr_rows with x/y-only shell)
-inline bool pml_rows_form(const fdtd_engine* h) {
-    return pml_split_in_use(h) && h->pml_split_mode == 2;
-}
-
+// UPML coefficient rows: by default with full main coefficients, and with
+// the shell split (its scattered vectors gain more from 2-byte rows than the
+// extra gathers cost: 1537x1537x769 dipole 16.16 -> 15.27 ms per voltage
+// update); an index-form single launch keeps the full slab arrays
 inline bool pml_rows_in_use(const fdtd_engine* h) {
-    return h->pml_cmp && (h->pml_cmp_mode == 1 || (h->pml_cmp_mode < 0 && h->cf_mode == CF_FULL));
+    return h->pml_cmp && (h->pml_cmp_mode == 1 || (h->pml_cmp_mode < 0 && (h->cf_mode == CF_FULL || pml_split_in_use(h))));
 }
 bool probes_need_realloc(const fdtd_engine* h, uint32_t upto_ts);
 int finalize_probes(fdtd_engine* h, uint32_t upto_ts);

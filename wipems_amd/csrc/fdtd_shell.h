@@ -49,21 +49,16 @@ inline bool find_interior(const uint32_t N[3], const PmlSlab* slab, int count, i
 // slabs that do not form a shell) one launch covers planes [0, xe) and rows
 // [0, ye); with a shell, the main launch covers the vectors of the interior
 // box only and the shell launch the rest.
-// rows = true: the main launch covers the interior rows in full and applies
-// the z slabs itself (k_*_rows); the shell launch then holds the x-slab
-// planes and the y-slab rows only, which are contiguous runs of memory.
 struct VecLaunch {
     VecRange r;
     unsigned nx;               // planes of the main launch (0: none)
     long long nq;              // flat (y, z) extent of the main launch per plane
     bool pml_main;             // main launch runs the UPML form
     bool shell;
-    bool rows;                 // main launch is the rows form
     ShellGrid g;
 };
 
-inline VecLaunch plan_vec_launch(const uint32_t N[3], long long sy, int pml_count, bool shell_ok, const int ibox[3][2], int vz, bool curr,
-                                 bool rows = false) {
+inline VecLaunch plan_vec_launch(const uint32_t N[3], long long sy, int pml_count, bool shell_ok, const int ibox[3][2], int vz, bool curr) {
     VecLaunch L{};
     const int nx = (int)N[0], ny = (int)N[1], nz = (int)N[2];
     const int xe = curr ? std::max(1, nx - 1) : nx;
@@ -71,21 +66,20 @@ inline VecLaunch plan_vec_launch(const uint32_t N[3], long long sy, int pml_coun
     L.pml_main = pml_count > 0 && !shell_ok;
     L.shell = pml_count > 0 && shell_ok;
     if (!L.shell) {
-        L.r = VecRange{0, 0, ny, 0, 1 << 30, 0, 0};
+        L.r = VecRange{0, 0, ny, 0, 1 << 30};
         L.nx = (unsigned)xe;
         L.nq = (long long)std::max(ye, 0) * sy;
         return L;
     }
-    L.rows = rows;
     const int x0 = std::min(ibox[0][0], xe), x1 = std::min(std::max(ibox[0][1], x0), xe);
     const int y0 = ibox[1][0], y1 = std::max(ibox[1][1], y0);
     const int vn = (nz + vz - 1) / vz;
     const int vlo = std::min((ibox[2][0] + vz - 1) / vz, vn);
     const int vhi = std::max(std::min(ibox[2][1] / vz, vn), vlo);
-    L.r = rows ? VecRange{x0, y0, y1, 0, 1 << 30, ibox[2][0], ibox[2][1]} : VecRange{x0, y0, y1, vlo * vz, vhi * vz, 0, 0};
+    L.r = VecRange{x0, y0, y1, vlo * vz, vhi * vz};
     L.nx = (unsigned)(x1 - x0);
     L.nq = (long long)(y1 - y0) * sy;
-    if (y1 == y0 || (!rows && vhi == vlo)) L.nx = 0;
+    if (y1 == y0 || vhi == vlo) L.nx = 0;
     ShellGrid& g = L.g;
     g.x0 = x0; g.x1 = x1; g.xe = xe;
     g.y0 = y0; g.y1 = y1; g.ny = ny;
@@ -93,6 +87,6 @@ inline VecLaunch plan_vec_launch(const uint32_t N[3], long long sy, int pml_coun
     g.nA0 = (long long)x0 * ny * vn;
     g.nA1 = (long long)(xe - x1) * ny * vn;
     g.nB = (long long)(x1 - x0) * (y0 + ny - y1) * vn;
-    g.nC = rows ? 0 : (long long)(x1 - x0) * (y1 - y0) * (vlo + vn - vhi);
+    g.nC = (long long)(x1 - x0) * (y1 - y0) * (vlo + vn - vhi);
     return L;
 }

@@ -289,16 +289,23 @@ def uniform_bench(n_lines=512, steps=1000, prec=0):
     return cavity_point_source(n_cells=n_lines - 1, steps=steps, prec=prec)
 
 
+def dft_time2freq(t, val, freqs):
+    """2 dt sum_n val(t_n) exp(-2 pi i f t_n) at each frequency, dt = t[1] - t[0]
+    (DFT_time2freq, python/openEMS/utilities.py:21-35)."""
+    t = np.asarray(t)
+    val = np.asarray(val)
+    out = np.empty(len(freqs), dtype=np.complex128)
+    for k, fk in enumerate(freqs):
+        out[k] = np.sum(val * np.exp(-1j * 2 * np.pi * fk * t))
+    return 2 * out * (t[1] - t[0])
+
+
 def s11_from_probes(t_u, u, t_i, i, freqs, Z_ref=50.0):
     """S11 of a lumped port from its U/I time series, as python/openEMS/ports.py
-    Port.CalcPort (:103-141) with DFT_time2freq (python/openEMS/utilities.py:21-35)."""
-    def dft(t, val, f):
-        out = np.empty(len(f), dtype=np.complex128)
-        for k, fk in enumerate(f):
-            out[k] = np.sum(val * np.exp(-1j * 2 * np.pi * fk * t))
-        return 2 * out * (t[1] - t[0])
-    uf = dft(np.asarray(t_u), np.asarray(u), freqs)
-    itf = dft(np.asarray(t_i), np.asarray(i), freqs)
+    Port.CalcPort (:103-141): uf_inc = (U + I Z_ref)/2, uf_ref = U - uf_inc,
+    S11 = uf_ref / uf_inc."""
+    uf = dft_time2freq(t_u, u, freqs)
+    itf = dft_time2freq(t_i, i, freqs)
     uf_inc = 0.5 * (uf + itf * Z_ref)
     uf_ref = uf - uf_inc
     return uf_ref / uf_inc
