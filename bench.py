@@ -240,6 +240,7 @@ def main():
     ap.add_argument("--coeffs", default="auto", choices=["auto", "full", "compressed"])
     ap.add_argument("--split", type=int, default=None, help="UPML launch form (fdtd_set_pml_split): 0 or 1; default: the engine's")
     ap.add_argument("--pml-rows", type=int, default=None, help="UPML coefficient rows (fdtd_set_pml_compression): 0 or 1; default: the engine's")
+    ap.add_argument("--fuse-hooks", type=int, default=None, help="fused sparse hooks (fdtd_set_fuse_hooks): 0 or 1; default: the engine's")
     ap.add_argument("--cpu-steps", type=int, default=None, help="steps of the CPU-baseline sample (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=None)
     ap.add_argument("--no-timing-pass", action="store_true")
@@ -283,6 +284,8 @@ def main():
         eng.set_pml_split(args.split)
     if args.pml_rows is not None:
         eng.set_pml_compression(args.pml_rows)
+    if args.fuse_hooks is not None:
+        eng.set_fuse_hooks(args.fuse_hooks)
     log("device %s, coefficients %s, UPML split %s, device bytes %.2f GB" % (device_name(), "compressed" if eng.compressed else "full",
                                                                           eng.pml_split(), eng.device_bytes() / 1e9))
 

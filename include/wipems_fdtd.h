@@ -270,13 +270,23 @@ int  fdtd_pml_compressed(fdtd_engine* h, int* active);
  * interior box (as the operator's UPML does), the interior can run in a
  * launch without UPML code and the shell in a launch of its own, instead of
  * one launch whose wavefronts branch into the UPML form.  mode 1 = split,
- * 0 = one launch, -1 = split when the interior holds at least 88% of the
- * cells for FP16/BF16 storage or 95% for FP32, one launch for FP64 (default:
- * otherwise the shell's scattered vectors cost more than the branching they
- * remove).  Results are identical either way.  fdtd_pml_split reports
- * whether the split launches are in use. */
+ * 0 = one launch, -1 (default) = split when the interior holds at least
+ * half of the cells, for FP16/BF16 storage and for FP32 with compressed
+ * coefficients; one launch for FP32 with full coefficient arrays and for
+ * FP64 (measured slower split).  Results are identical either way.
+ * fdtd_pml_split reports whether the split launches are in use. */
 int  fdtd_set_pml_split(fdtd_engine* h, int mode);
 int  fdtd_pml_split(fdtd_engine* h, int* active);
+/* Fused sparse hooks: the TFSF injection, the dispersive apply and the soft
+ * excitation of a half step run as one launch that walks each touched field
+ * location's hooks in the reference's order, and that launch also performs
+ * the other field's ADE pre-update of the next half step (which no hook of
+ * the current half step changes).  Results are identical to one launch per
+ * hook.  mode 1 / -1 (default) = fused where no Mur ABC, lumped RLC or
+ * steady-state hook sits between them, 0 = one launch per hook.
+ * fdtd_hooks_fused reports whether the fused form is in use. */
+int  fdtd_set_fuse_hooks(fdtd_engine* h, int mode);
+int  fdtd_hooks_fused(fdtd_engine* h, int* active);
 /* Use a captured HIP graph for the step loop (default on). */
 int  fdtd_set_use_graph(fdtd_engine* h, int enable);
 /* Kernel timing: while enabled, fdtd_iterate runs eagerly and records HIP

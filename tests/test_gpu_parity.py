@@ -371,8 +371,9 @@ def test_three_slabs_in_one_vector_are_rejected(gpu):
 
 
 def test_upml_split_auto_policy(gpu):
-    """Default: split launches for 2-byte storage when the interior holds
-    >= 88% of the cells, for FP32 when it holds >= 95%, never for FP64."""
+    """Default: split launches when the interior holds at least half of the
+    cells, for 2-byte storage and for FP32 with compressed coefficients; one
+    launch for FP32 with full coefficient arrays and for FP64."""
     from wipems_amd.operator import BC_PML, Operator
 
     def domain(pml, n=129, prec=0):
@@ -383,12 +384,14 @@ def test_upml_split_auto_policy(gpu):
         op.set_gauss_excite(2e9, 2e9)
         return op.build(prec, nr_ts=4)
 
-    thin, thick = domain(1), domain(8)          # interior 91% / 76% of the cells
-    thinner, thinner64 = domain(1, 257), domain(1, 257, 1)     # interior 95.4%
-    for d, precision, want in [(thin, "fp16", True), (thin, "bf16", True), (thin, "fp32", False),
-                               (thick, "fp16", False), (thinner, "fp32", True), (thinner64, "fp64", False)]:
-        eng = engine_from_operator(d, precision=precision)
-        assert eng.pml_split() == want, (precision, want)
+    thin, thick, fat = domain(1), domain(8), domain(24)     # interior 91%, 76%, 37% of the cells
+    thin64 = domain(1, 129, 1)
+    cases = [(thin, "fp16", None, True), (thin, "bf16", None, True), (thin, "fp32", True, True), (thick, "fp32", True, True),
+             (thick, "fp16", None, True), (thin, "fp32", False, False), (fat, "fp32", True, False), (fat, "fp16", None, False),
+             (thin64, "fp64", False, False)]
+    for d, precision, compressed, want in cases:
+        eng = engine_from_operator(d, precision=precision, compressed=compressed)
+        assert eng.pml_split() == want, (precision, compressed, want)
         eng.close()
 
 
@@ -479,3 +482,37 @@ def test_lumped_rlc_bitwise(gpu, precision, prec, bc):
         orc.iterate(chunk)
     compare(eng, orc)
     assert np.abs(eng.read_probe(eng.probe_ids[0])[1]).max() > 0
+
+
+@pytest.mark.parametrize("precision,prec", [("fp32", 0), ("fp64", 1), ("fp16", 0), ("bf16", 0)])
+@pytest.mark.parametrize("case", ["dispersive_tfsf", "graded_port"])
+def test_fused_hooks_equal_separate_launches(gpu, precision, prec, case):
+    """The fused hook launch (TFSF rounds + ADE apply + excitation per
+    location, with the next half step's ADE pre-update) gives the bits of one
+    launch per hook, for every storage type, across uneven iterate() chunks
+    (graph runs of 16 and 1 steps, eager steps) and a field write between
+    chunks (the ADE pre-update must see it)."""
+    if case == "dispersive_tfsf":
+        d = scenarios.dispersive_scenario(num_lines=(41, 37, 29), steps=150, prec=prec, n_spheres=4,
+                                          material_scale=1e-3, ts_factor=0.9).data
+    else:
+        d = scenarios.graded_port_scenario(n_cells=30, steps=150, prec=prec, n_boxes=4, pml=6).data
+    print("case: fused hooks %s %s" % (case, precision), flush=True)
+    scales = ([2.0 ** -4] * 3, [2.0 ** -12] * 3) if precision in ("fp16", "bf16") else None
+    out = []
+    for mode in (0, 1):
+        eng = engine_from_operator(d, precision=precision, scales=scales)
+        eng.set_fuse_hooks(mode)
+        assert eng.hooks_fused() == bool(mode)
+        for c in (1, 17, 3):
+            eng.iterate(c)
+        V = eng.get_field(0)
+        V[0, 20, 18, 14] += 1e-3
+        eng.set_field(0, V)
+        for c in (16, 40, 5):
+            eng.iterate(c)
+        out.append([eng.get_field(w) for w in (0, 1)] + [eng.read_probe(p)[1] for p in eng.probe_ids])
+        eng.close()
+    for a, b in zip(out[0], out[1]):
+        np.testing.assert_array_equal(a, b)
+    assert np.abs(out[0][0]).max() > 0

@@ -59,6 +59,8 @@ _ENGINE_SIGS = {
     "fdtd_synchronize": [c_void_p],
     "fdtd_get_num_timesteps": [c_void_p, POINTER(c_uint32)],
     "fdtd_set_use_graph": [c_void_p, c_int],
+    "fdtd_set_fuse_hooks": [c_void_p, c_int],
+    "fdtd_hooks_fused": [c_void_p, POINTER(c_int)],
     "fdtd_set_pml_compression": [c_void_p, c_int],
     "fdtd_pml_compressed": [c_void_p, POINTER(c_int)],
     "fdtd_set_pml_split": [c_void_p, c_int],

@@ -926,41 +926,11 @@ struct AdeDev {
     const void* i_int; const void* i_ext; const void* i_lor;
     void* volt_ade; void* volt_lor; void* curr_ade; void* curr_lor;  // state [3][count]
     int volt_on, curr_on, volt_lor_on, curr_lor_on;
+    // fused hooks: per entry, bit n set when component n shares its location
+    // with a TFSF or excitation entry of the half step (volt, curr); such
+    // components are applied through the location's hook program
+    const unsigned char* shared[2];
 };
-
-// Engine_Ext_LorentzMaterial::DoPre{Voltage,Current}UpdatesImpl, engine_ext_lorentzmaterial.cpp:79-168
-template <typename S>
-__global__ void k_ade_pre(const S* __restrict__ F, AdeDev a, long long sc_field, Scales<typename StoreTraits<S>::A> sc, int is_curr) {
-    typedef StoreTraits<S> T;
-    typedef typename T::A A;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.count) return;
-    const A* kint = (const A*)(is_curr ? a.i_int : a.v_int);
-    const A* kext = (const A*)(is_curr ? a.i_ext : a.v_ext);
-    const A* klor = (const A*)(is_curr ? a.i_lor : a.v_lor);
-    A* J = (A*)(is_curr ? a.curr_ade : a.volt_ade);
-    A* JL = (A*)(is_curr ? a.curr_lor : a.volt_lor);
-    const int lor_on = is_curr ? a.curr_lor_on : a.volt_lor_on;
-#pragma unroll
-    for (int n = 0; n < 3; ++n) {
-        const long long q = (long long)n * a.count + i;
-        const A f = T::load(F + n * sc_field + a.off[i], is_curr ? sc.c[n] : sc.v[n]);
-        if (lor_on) {
-            A jl = JL[q];
-            A j = J[q];
-            jl = jl + klor[q] * j;
-            j = j * kint[q];
-            j = j + kext[q] * (f - jl);
-            JL[q] = jl;
-            J[q] = j;
-        } else {
-            A j = J[q];
-            j = j * kint[q];
-            j = j + kext[q] * f;
-            J[q] = j;
-        }
-    }
-}
 
 // Engine_Ext_Dispersive::Apply2{Voltages,Current}Impl, engine_ext_dispersive.cpp:76-127
 template <typename S>
@@ -1028,6 +998,163 @@ __global__ void k_tfsf(S* F, const unsigned* __restrict__ step, TfsfDev t, Scale
     const A second = delta[i] * amp[i] * sig[lookup(dl + 1)];
     double r = f + (1.0 - (double)delta[i]) * (double)amp[i] * (double)sig[lookup(dl)] + (double)second;
     T::store(fp, (A)r, is_curr ? sc.ic[n] : sc.iv[n]);
+}
+
+// ---------------------------------------------------------------------------
+// Fused sparse hooks of one half step: the TFSF injection, the dispersive
+// apply (V -= J) and the soft excitation of every location they touch, in
+// one launch instead of one per hook (and per TFSF round).  The host groups
+// the entries by field location (HookProg: one thread per location, its ops
+// in the reference's hook order -- TFSF in list order, then the ADE orders,
+// then the excitation in list order), so a location's sum order is the
+// reference's; the value is rounded to the storage type after every op, as
+// the separate hooks store it.  The same launch runs the ADE pre-update of
+// the next half step (Engine_Ext_LorentzMaterial::DoPre*Updates) for the
+// other field, which no hook of this half step changes: pre_next = 1.
+// ---------------------------------------------------------------------------
+#define MAX_ADE_ORDERS 4
+struct AdeSet {
+    int n;
+    AdeDev o[MAX_ADE_ORDERS];
+};
+
+struct HookProg {
+    int ntarget;
+    const long long* off;          // [ntarget] element offset incl. component
+    const int* start;              // [ntarget + 1] op ranges
+    const unsigned* ops;           // type << 30 | index: 0 TFSF entry, 1 ADE (order << 26 | comp*count + entry), 2 excitation entry
+};
+
+template <typename S>
+__device__ __forceinline__ typename StoreTraits<S>::A storage_round(typename StoreTraits<S>::A v, typename StoreTraits<S>::A s,
+                                                                    typename StoreTraits<S>::A is) {
+    if constexpr (sizeof(S) >= 4) {
+        return v;
+    } else {
+        S t;
+        StoreTraits<S>::store(&t, v, is);
+        return StoreTraits<S>::load(&t, s);
+    }
+}
+
+template <typename S>
+__device__ __forceinline__ void ade_pre_entry(const S* __restrict__ F, const AdeDev& a, int i, long long sc_field,
+                                              const Scales<typename StoreTraits<S>::A>& sc, int is_curr) {
+    typedef StoreTraits<S> T;
+    typedef typename T::A A;
+    const A* kint = (const A*)(is_curr ? a.i_int : a.v_int);
+    const A* kext = (const A*)(is_curr ? a.i_ext : a.v_ext);
+    const A* klor = (const A*)(is_curr ? a.i_lor : a.v_lor);
+    A* J = (A*)(is_curr ? a.curr_ade : a.volt_ade);
+    A* JL = (A*)(is_curr ? a.curr_lor : a.volt_lor);
+    const int lor_on = is_curr ? a.curr_lor_on : a.volt_lor_on;
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+        const long long q = (long long)n * a.count + i;
+        const A f = T::load(F + n * sc_field + a.off[i], is_curr ? sc.c[n] : sc.v[n]);
+        if (lor_on) {
+            A jl = JL[q];
+            A j = J[q];
+            jl = jl + klor[q] * j;
+            j = j * kint[q];
+            j = j + kext[q] * (f - jl);
+            JL[q] = jl;
+            J[q] = j;
+        } else {
+            A j = J[q];
+            j = j * kint[q];
+            j = j + kext[q] * f;
+            J[q] = j;
+        }
+    }
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256)
+k_hooks(S* __restrict__ F, const S* __restrict__ G, const unsigned* __restrict__ step, HookProg hp, TfsfDev t, ExcDev e, AdeSet ade,
+        Scales<typename StoreTraits<S>::A> sc, long long sc_field, int is_curr, int pre_next) {
+    typedef StoreTraits<S> T;
+    typedef typename T::A A;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < hp.ntarget) {
+        const unsigned numTS = *step;
+        const long long off = hp.off[i];
+        const int n = (int)(off / sc_field);
+        const A s = is_curr ? sc.c[n] : sc.v[n];
+        const A is = is_curr ? sc.ic[n] : sc.iv[n];
+        A v = T::load(F + off, s);
+        for (int k = hp.start[i]; k < hp.start[i + 1]; ++k) {
+            const unsigned op = hp.ops[k];
+            const unsigned idx = op & 0x3fffffffu;
+            switch (op >> 30) {
+            case 0: {   // TFSF (k_tfsf)
+                const A* sig = (const A*)t.sig;
+                const A* amp = (const A*)t.amp;
+                const A* delta = (const A*)t.delta;
+                auto lookup = [&](unsigned m) -> unsigned {
+                    unsigned L;
+                    if (numTS < m) L = 0;
+                    else if ((numTS - m >= (unsigned)t.length) && (t.period == 0)) L = 0;
+                    else L = numTS - m;
+                    if (t.period > 0) L = L % (unsigned)t.period;
+                    return L;
+                };
+                const unsigned dl = (unsigned)t.delay[idx];
+                const A second = delta[idx] * amp[idx] * sig[lookup(dl + 1)];
+                const double r = (double)v + (1.0 - (double)delta[idx]) * (double)amp[idx] * (double)sig[lookup(dl)] + (double)second;
+                v = (A)r;
+                break;
+            }
+            case 1: {   // dispersive apply of a shared location (k_ade_apply)
+                const AdeDev& a = ade.o[idx >> 26];
+                const A* J = (const A*)(is_curr ? a.curr_ade : a.volt_ade);
+                v = v - J[idx & 0x3ffffffu];
+                break;
+            }
+            default: {  // soft excitation (k_excite)
+                int p = (int)numTS + 1;
+                if (e.period > 0) p = e.period;
+                int kk = (int)numTS - e.delay[idx];
+                kk *= (kk > 0);
+                kk %= p;
+                kk *= (kk < e.length);
+                v = v + ((const A*)e.amp)[idx] * ((const A*)e.sig)[kk];
+                break;
+            }
+            }
+            v = storage_round<S>(v, s, is);
+        }
+        T::store(F + off, v, is);
+    }
+    // dispersive apply of the locations no other hook touches, per entry
+    for (int o = 0; o < ade.n; ++o) {
+        const AdeDev& a = ade.o[o];
+        if (i >= a.count || !(is_curr ? a.curr_on : a.volt_on)) continue;
+        const A* J = (const A*)(is_curr ? a.curr_ade : a.volt_ade);
+        const unsigned m = a.shared[is_curr] ? a.shared[is_curr][i] : 0u;
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+            if (m & (1u << n)) continue;
+            S* fp = F + n * sc_field + a.off[i];
+            A f = T::load(fp, is_curr ? sc.c[n] : sc.v[n]);
+            f = f - J[(long long)n * a.count + i];
+            T::store(fp, f, is_curr ? sc.ic[n] : sc.iv[n]);
+        }
+    }
+    if (pre_next) {
+        for (int o = 0; o < ade.n; ++o) {
+            const AdeDev& a = ade.o[o];
+            if (i < a.count && (is_curr ? a.volt_on : a.curr_on)) ade_pre_entry<S>(G, a, i, sc_field, sc, !is_curr);
+        }
+    }
+}
+
+// Engine_Ext_LorentzMaterial::DoPre{Voltage,Current}UpdatesImpl, engine_ext_lorentzmaterial.cpp:79-168
+template <typename S>
+__global__ void k_ade_pre(const S* __restrict__ F, AdeDev a, long long sc_field, Scales<typename StoreTraits<S>::A> sc, int is_curr) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.count) return;
+    ade_pre_entry<S>(F, a, i, sc_field, sc, is_curr);
 }
 
 // ---------------------------------------------------------------------------

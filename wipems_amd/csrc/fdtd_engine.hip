@@ -29,6 +29,7 @@
 #include <memory>
 #include <mutex>
 #include <unordered_map>
+#include <unordered_set>
 
 thread_local std::string g_last_error;
 
@@ -186,7 +187,7 @@ int upload_dense3(fdtd_engine* h, int dtype, const void* src, DevBuf& buf) {
 }
 
 template <typename S>
-int launch_step_kernels(fdtd_engine* h, bool timed);
+int launch_step_kernels(fdtd_engine* h, bool timed, bool first, bool last);
 
 int capture_graph(fdtd_engine* h, int steps, hipGraph_t* g, hipGraphExec_t* ge);
 
@@ -390,10 +391,113 @@ int launch_step_end(fdtd_engine* h, int advance) {
 // zero unless something writes them: a current-side hook or a caller's
 // SetCurr / fdtd_set_field on the current field.
 bool faces_live(const fdtd_engine* h) {
-    if (h->curr_written || (h->has_exc && h->exc_c.count) || h->tfsf_c.count) return true;
+    return h->curr_written || h->curr_hook_faces;
+}
+
+// whether a current-side hook writes a cell of the planes x = Nx-1,
+// y = Ny-1 or z = Nz-1 (element offsets incl. component)
+bool on_upper_face(const fdtd_engine* h, long long off) {
+    off %= h->d.sc;
+    const long long x = off / h->d.sx, r = off - x * h->d.sx, y = r / h->d.sy, z = r - y * h->d.sy;
+    return x == (long long)h->N[0] - 1 || y == (long long)h->N[1] - 1 || z == (long long)h->N[2] - 1;
+}
+
+void find_curr_hook_faces(fdtd_engine* h) {
+    bool hit = false;
+    for (long long o : h->exc_host_off[1]) hit |= on_upper_face(h, o);
+    for (long long o : h->tfsf_host_off[1]) hit |= on_upper_face(h, o);
     for (auto a : h->ade)
-        if ((a->flags & 2) && a->count) return true;
-    return false;
+        if (a->flags & 2)
+            for (long long o : a->host_off) hit |= on_upper_face(h, o);
+    h->curr_hook_faces = hit;
+}
+
+// Per-location programs of the fused hooks (k_hooks), built when the hook
+// lists change.  Not used when a Mur, lumped-RLC or steady-state hook sits
+// between them in the reference's order, or beyond MAX_ADE_ORDERS orders.
+int build_hooks(fdtd_engine* h) {
+    if (!h->hooks_dirty) return FDTD_OK;
+    h->hooks_dirty = false;
+    find_curr_hook_faces(h);
+    h->hooks_fused = false;
+    h->hook[0] = HookProg{};
+    h->hook[1] = HookProg{};
+    invalidate_graph(h);
+    if (h->fuse_hooks_mode == 0 || !h->mur.empty() || h->rlc || h->ss || h->ade.size() > MAX_ADE_ORDERS) return FDTD_OK;
+    for (auto a : h->ade)
+        if ((size_t)3 * a->count >= (1u << 26)) return FDTD_OK;
+    for (int w = 0; w < 2; ++w) {
+        std::vector<std::pair<long long, unsigned>> items;
+        const std::vector<long long>& tf = h->tfsf_host_off[w];
+        for (size_t k = 0; k < tf.size(); ++k) items.emplace_back(tf[k], (unsigned)k);
+        // locations of TFSF and excitation entries: an ADE component there
+        // joins the location's program, every other one is applied per entry
+        std::unordered_set<long long> others(tf.begin(), tf.end());
+        if (h->has_exc) others.insert(h->exc_host_off[w].begin(), h->exc_host_off[w].end());
+        // a location in two ADE orders is applied in order through its program too
+        if (h->ade.size() > 1) {
+            std::unordered_map<long long, int> seen;
+            for (auto a : h->ade)
+                if ((a->flags & (w == 0 ? 1u : 2u)) && a->count)
+                    for (long long o0 : a->host_off)
+                        for (int n = 0; n < 3; ++n)
+                            if (++seen[o0 + n * h->d.sc] == 2) others.insert(o0 + n * h->d.sc);
+        }
+        for (size_t o = 0; o < h->ade.size(); ++o) {
+            AdeHost* a = h->ade[o];
+            a->shared[w].alloc(0);
+            if (!(a->flags & (w == 0 ? 1u : 2u)) || !a->count) continue;
+            std::vector<unsigned char> mask(a->count, 0);
+            bool any = false;
+            for (uint32_t i = 0; i < a->count; ++i)
+                for (int n = 0; n < 3; ++n) {
+                    const long long off = a->host_off[i] + n * h->d.sc;
+                    if (!others.count(off)) continue;
+                    mask[i] |= (unsigned char)(1u << n);
+                    any = true;
+                    items.emplace_back(off, (1u << 30) | ((unsigned)o << 26) | (unsigned)(n * a->count + i));
+                }
+            if (any) {
+                int rc;
+                if ((rc = a->shared[w].alloc(a->count))) return rc;
+                HIPCHK(hipMemcpyAsync(a->shared[w].p, mask.data(), a->count, hipMemcpyHostToDevice, h->stream));
+            }
+        }
+        if (h->has_exc) {
+            const std::vector<long long>& ex = h->exc_host_off[w];
+            for (size_t i = 0; i < ex.size(); ++i) items.emplace_back(ex[i], (2u << 30) | (unsigned)i);
+        }
+        if (items.size() >= (1u << 30)) return FDTD_OK;
+        std::stable_sort(items.begin(), items.end(), [](const std::pair<long long, unsigned>& x, const std::pair<long long, unsigned>& y) {
+            return x.first < y.first;
+        });
+        std::vector<long long> off;
+        std::vector<int> start;
+        std::vector<unsigned> ops;
+        for (size_t k = 0; k < items.size(); ++k) {
+            if (k == 0 || items[k].first != items[k - 1].first) { off.push_back(items[k].first); start.push_back((int)k); }
+            ops.push_back(items[k].second);
+        }
+        start.push_back((int)items.size());
+        const size_t nt = off.size();
+        const size_t b_off = nt * 8, b_start = (nt + 1) * 4, b_ops = ops.size() * 4;
+        int rc;
+        if ((rc = h->hook_buf[w].alloc(b_off + b_start + b_ops + 16))) return rc;
+        char* base = (char*)h->hook_buf[w].p;
+        if (nt) {
+            HIPCHK(hipMemcpyAsync(base, off.data(), b_off, hipMemcpyHostToDevice, h->stream));
+            HIPCHK(hipMemcpyAsync(base + b_off + b_start, ops.data(), b_ops, hipMemcpyHostToDevice, h->stream));
+        }
+        HIPCHK(hipMemcpyAsync(base + b_off, start.data(), b_start, hipMemcpyHostToDevice, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        HookProg& hp = h->hook[w];
+        hp.ntarget = (int)nt;
+        hp.off = (const long long*)base;
+        hp.start = (const int*)(base + b_off);
+        hp.ops = (const unsigned*)(base + b_off + b_start);
+    }
+    h->hooks_fused = true;
+    return FDTD_OK;
 }
 
 VecLaunch vec_launch(const fdtd_engine* h, int vz, bool curr) {
@@ -458,8 +562,12 @@ void launch_curr_vec(fdtd_engine* h, CoeffPtrs<typename StoreTraits<S>::A> cf, P
     }
 }
 
+// first / last: the step opens / closes a run of steps launched together
+// (a captured graph or one eager loop); with fused hooks the voltage ADE
+// pre-update of a step is done by the previous step's current hooks, except
+// in the first step of a run
 template <typename S>
-int launch_step_kernels(fdtd_engine* h, bool timed) {
+int launch_step_kernels(fdtd_engine* h, bool timed, bool first, bool last) {
     typedef typename StoreTraits<S>::A A;
     Scales<A> sc = make_scales<A>(h);
 
@@ -499,6 +607,8 @@ int launch_step_kernels(fdtd_engine* h, bool timed) {
         ad.volt_ade = st; ad.volt_lor = st + n3; ad.curr_ade = st + 2 * n3; ad.curr_lor = st + 3 * n3;
         ad.volt_on = (a->flags & 1) != 0; ad.curr_on = (a->flags & 2) != 0;
         ad.volt_lor_on = (a->flags & 4) != 0; ad.curr_lor_on = (a->flags & 8) != 0;
+        ad.shared[0] = a->shared[0].as<unsigned char>();
+        ad.shared[1] = a->shared[1].as<unsigned char>();
         return ad;
     };
     // Mur ABC phases: pre in ascending face order, post and apply in
@@ -513,10 +623,32 @@ int launch_step_kernels(fdtd_engine* h, bool timed) {
             hipLaunchKernelGGL((k_mur<S>), dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, h->stream, h->V.as<S>(), stp, m->d, h->d, sc, phase);
         }
     };
+    const bool fh = h->hooks_fused;
+    AdeSet aset{};
+    bool volt_ade = false, curr_ade = false;
+    int ade_max = 0;
+    for (auto a : h->ade) {
+        if (aset.n < MAX_ADE_ORDERS) aset.o[aset.n++] = ade_dev(a);
+        volt_ade |= (a->flags & 1) && a->count;
+        curr_ade |= (a->flags & 2) && a->count;
+        ade_max = std::max(ade_max, (int)a->count);
+    }
+    // one fused hook launch: the post-update hooks of `which` and, with
+    // pre_next, the other field's ADE pre-update
+    auto hooks = [&](int which, bool pre_next) {
+        const HookProg& hp = h->hook[which];
+        const int n = std::max(std::max(hp.ntarget, pre_next ? ade_max : 0), (which == 0 ? volt_ade : curr_ade) ? ade_max : 0);
+        if (!n) return;
+        S* F = which == 0 ? h->V.as<S>() : h->I.as<S>();
+        const S* G = which == 0 ? h->I.as<S>() : h->V.as<S>();
+        hipLaunchKernelGGL((k_hooks<S>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, F, G, stp, hp,
+                           which == 0 ? h->tfsf_v : h->tfsf_c, which == 0 ? h->exc_v : h->exc_c, aset, sc, h->d.sc, which, pre_next ? 1 : 0);
+    };
     mur_phase(0);
-    for (auto a : h->ade)
-        if ((a->flags & 1) && a->count)
-            hipLaunchKernelGGL((k_ade_pre<S>), dim3((a->count + 255) / 256), dim3(256), 0, h->stream, h->V.as<S>(), ade_dev(a), h->d.sc, sc, 0);
+    if (!fh || first)
+        for (auto a : h->ade)
+            if ((a->flags & 1) && a->count)
+                hipLaunchKernelGGL((k_ade_pre<S>), dim3((a->count + 255) / 256), dim3(256), 0, h->stream, h->V.as<S>(), ade_dev(a), h->d.sc, sc, 0);
     // lumped RLC: pre after the Lorentz pre hook, apply before the Lorentz
     // apply (the extension is created after Lorentz, so it sits before it in
     // the engine's list, FDTD/engine.cpp:84-88)
@@ -527,7 +659,8 @@ int launch_step_kernels(fdtd_engine* h, bool timed) {
     if (timed) HIPCHK(hipEventRecord(h->ev[evi], h->stream));
     launch_volt_vec<S>(h, cfe, pp);
     if (timed) HIPCHK(hipEventRecord(h->ev[evi + 1], h->stream));
-    if (h->tfsf_v.count)
+    if (fh) hooks(0, curr_ade);
+    if (!fh && h->tfsf_v.count)
         launch_tfsf<S>(h, h->V.as<S>(), stp, h->tfsf_v, sc, 0);
     mur_phase(1);
     // steady-state detection: first of the apply hooks (priority +2e6)
@@ -541,17 +674,19 @@ int launch_step_kernels(fdtd_engine* h, bool timed) {
     }
     if (h->rlc)
         hipLaunchKernelGGL((k_rlc<S>), dim3((h->rlc->d.count + 255) / 256), dim3(256), 0, h->stream, h->V.as<S>(), stp, h->rlc->d, sc, 1);
-    for (auto a : h->ade)
-        if ((a->flags & 1) && a->count)
-            hipLaunchKernelGGL((k_ade_apply<S>), dim3((a->count + 255) / 256), dim3(256), 0, h->stream, h->V.as<S>(), ade_dev(a), h->d.sc, sc, 0);
+    if (!fh)
+        for (auto a : h->ade)
+            if ((a->flags & 1) && a->count)
+                hipLaunchKernelGGL((k_ade_apply<S>), dim3((a->count + 255) / 256), dim3(256), 0, h->stream, h->V.as<S>(), ade_dev(a), h->d.sc, sc, 0);
     mur_phase(2);
-    if (h->has_exc && h->exc_v.count)
+    if (!fh && h->has_exc && h->exc_v.count)
         hipLaunchKernelGGL((k_excite<S>), dim3(std::min(1024, (h->exc_v.count + 255) / 256)), dim3(256), 0, h->stream, h->V.as<S>(), stp, h->exc_v, sc, 0);
 
     // ---- current half step ----
-    for (auto a : h->ade)
-        if ((a->flags & 2) && a->count)
-            hipLaunchKernelGGL((k_ade_pre<S>), dim3((a->count + 255) / 256), dim3(256), 0, h->stream, h->I.as<S>(), ade_dev(a), h->d.sc, sc, 1);
+    if (!fh)
+        for (auto a : h->ade)
+            if ((a->flags & 2) && a->count)
+                hipLaunchKernelGGL((k_ade_pre<S>), dim3((a->count + 255) / 256), dim3(256), 0, h->stream, h->I.as<S>(), ade_dev(a), h->d.sc, sc, 1);
     if (timed) HIPCHK(hipEventRecord(h->ev[evi + 2], h->stream));
     launch_curr_vec<S>(h, cfh, pp);
     if (pml && h->pml_upper && faces_live(h)) {
@@ -559,6 +694,10 @@ int launch_step_kernels(fdtd_engine* h, bool timed) {
         hipLaunchKernelGGL((k_curr_upml_faces<S>), dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, h->stream, h->I.as<S>(), h->d, sc, h->pml, pp);
     }
     if (timed) HIPCHK(hipEventRecord(h->ev[evi + 3], h->stream));
+    if (fh) {
+        hooks(1, volt_ade && !last);
+        return launch_step_end<S>(h, 1);
+    }
     if (h->tfsf_c.count)
         launch_tfsf<S>(h, h->I.as<S>(), stp, h->tfsf_c, sc, 1);
     for (auto a : h->ade)
@@ -570,19 +709,19 @@ int launch_step_kernels(fdtd_engine* h, bool timed) {
     return launch_step_end<S>(h, 1);
 }
 
-int launch_step(fdtd_engine* h, bool timed) {
+int launch_step(fdtd_engine* h, bool timed, bool first, bool last) {
     switch (h->prec) {
-    case FDTD_PREC_FP32: return launch_step_kernels<float>(h, timed);
-    case FDTD_PREC_FP64: return launch_step_kernels<double>(h, timed);
-    case FDTD_PREC_FP16: return launch_step_kernels<__half>(h, timed);
-    default: return launch_step_kernels<__hip_bfloat16>(h, timed);
+    case FDTD_PREC_FP32: return launch_step_kernels<float>(h, timed, first, last);
+    case FDTD_PREC_FP64: return launch_step_kernels<double>(h, timed, first, last);
+    case FDTD_PREC_FP16: return launch_step_kernels<__half>(h, timed, first, last);
+    default: return launch_step_kernels<__hip_bfloat16>(h, timed, first, last);
     }
 }
 
 int capture_graph(fdtd_engine* h, int steps, hipGraph_t* g, hipGraphExec_t* ge) {
     HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     for (int s = 0; s < steps; ++s) {
-        int rc = launch_step(h, false);
+        int rc = launch_step(h, false, s == 0, s == steps - 1);
         if (rc) { hipGraph_t dummy; (void)hipStreamEndCapture(h->stream, &dummy); return rc; }
     }
     HIPCHK(hipStreamEndCapture(h->stream, g));
@@ -1011,6 +1150,8 @@ int fdtd_set_excitation(fdtd_engine* h, const fdtd_excitation* x) {
     if ((rc = check(nc, x->curr_pos, x->curr_dir, x->curr_amp, x->curr_delay))) return rc;
     // one buffer: sigV, sigI, then per list: off(8) comp(4) delay(4) amp(a)
     const size_t L = x->length;
+    h->exc_host_off[0].clear();
+    h->exc_host_off[1].clear();
     size_t bytes = 2 * L * a + (size_t)(nv + nc) * (8 + 4 + 4 + a) + 64;
     if ((rc = h->exc_buf.alloc(bytes))) return rc;
     std::vector<char> hb(bytes, 0);
@@ -1046,6 +1187,7 @@ int fdtd_set_excitation(fdtd_engine* h, const fdtd_excitation* x) {
             memcpy(hb.data() + o_del + i * 4, &dl, 4);
             keys[i] = off;
         }
+        (&ed == &h->exc_v ? h->exc_host_off[0] : h->exc_host_off[1]).assign(keys.begin(), keys.end());
         size_t o_amp = put_vals(amp, cnt);
         std::sort(keys.begin(), keys.end());
         ed.sequential = std::adjacent_find(keys.begin(), keys.end()) != keys.end();
@@ -1060,6 +1202,7 @@ int fdtd_set_excitation(fdtd_engine* h, const fdtd_excitation* x) {
     HIPCHK(hipMemcpyAsync(h->exc_buf.p, hb.data(), bytes, hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     h->has_exc = true;
+    h->hooks_dirty = true;
     invalidate_graph(h);
     return FDTD_OK;
 }
@@ -1144,7 +1287,9 @@ int fdtd_add_ade(fdtd_engine* h, uint32_t count, const uint32_t* pos, int dtype,
         delete a;
         return fail(FDTD_ERR_HIP, "ADE upload failed");
     }
+    a->host_off = std::move(off);
     h->ade.push_back(a);
+    h->hooks_dirty = true;
     invalidate_graph(h);
     return FDTD_OK;
 }
@@ -1154,6 +1299,7 @@ int fdtd_set_rlc(fdtd_engine* h, uint32_t count, const uint32_t* pos, const uint
     if (h->numTS) return fail(FDTD_ERR_STATE, "lumped RLC cells must be set before the first timestep");
     delete h->rlc;
     h->rlc = nullptr;
+    h->hooks_dirty = true;
     invalidate_graph(h);
     if (!count) return FDTD_OK;
     if (!pos || !dir || !coef) return fail(FDTD_ERR_ARG, "null argument");
@@ -1200,6 +1346,7 @@ int fdtd_set_steady_state(fdtd_engine* h, double period, uint32_t count, const u
     if (h->numTS) return fail(FDTD_ERR_STATE, "steady-state detection must be set before the first timestep");
     delete h->ss;
     h->ss = nullptr;
+    h->hooks_dirty = true;
     invalidate_graph(h);
     if (!(period > 0)) return FDTD_OK;
     // Operator_Ext_SteadyState::BuildExtension: the period in whole timesteps
@@ -1294,6 +1441,7 @@ int fdtd_add_mur(fdtd_engine* h, int face, int dtype, const void* k_nyP, const v
     auto it = h->mur.begin();
     while (it != h->mur.end() && (*it)->d.ny * 2 + ((*it)->d.line > 0) < face) ++it;
     h->mur.insert(it, m);
+    h->hooks_dirty = true;
     invalidate_graph(h);
     return FDTD_OK;
 }
@@ -1307,6 +1455,9 @@ int fdtd_set_tfsf(fdtd_engine* h, int which, uint32_t count, const uint32_t* pos
     TfsfDev& t = which == FDTD_VOLT ? h->tfsf_v : h->tfsf_c;
     DevBuf& buf = which == FDTD_VOLT ? h->tfsf_buf_v : h->tfsf_buf_c;
     t = TfsfDev{};
+    h->tfsf_host_off[which == FDTD_VOLT ? 0 : 1].clear();
+    h->hooks_dirty = true;
+    invalidate_graph(h);
     if (!count) return FDTD_OK;
     if (!pos || !comp || !delay || !delta || !amp) return fail(FDTD_ERR_ARG, "null TFSF list");
     const size_t a = h->acc;
@@ -1335,9 +1486,12 @@ int fdtd_set_tfsf(fdtd_engine* h, int which, uint32_t count, const uint32_t* pos
     std::vector<uint32_t> order(count);
     for (uint32_t i = 0; i < count; ++i) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return round[x] < round[y]; });
+    std::vector<long long>& host_off = h->tfsf_host_off[which == FDTD_VOLT ? 0 : 1];
+    host_off.resize(count);
     for (uint32_t k = 0; k < count; ++k) {
         const uint32_t i = order[k];
         long long off = offs[i];
+        host_off[k] = off;
         int c = (int)comp[i], dl = (int)delay[i];
         memcpy(hb.data() + o_off + k * 8, &off, 8);
         memcpy(hb.data() + o_comp + k * 4, &c, 4);
@@ -1368,6 +1522,7 @@ int fdtd_set_tfsf(fdtd_engine* h, int which, uint32_t count, const uint32_t* pos
     t.amp = b + o_amp;
     // voltage injection uses the current signal and vice versa (engine_ext_tfsf.cpp:54,145)
     t.sig = which == FDTD_VOLT ? h->exc_c.sig : h->exc_v.sig;
+    h->hooks_dirty = true;
     invalidate_graph(h);
     return FDTD_OK;
 }
@@ -1623,6 +1778,23 @@ int fdtd_pml_split(fdtd_engine* h, int* active) {
     return FDTD_OK;
 }
 
+int fdtd_set_fuse_hooks(fdtd_engine* h, int mode) {
+    if (!h) return fail(FDTD_ERR_ARG, "null handle");
+    if (mode < -1 || mode > 1) return fail(FDTD_ERR_ARG, "mode must be -1 (auto), 0 or 1");
+    h->fuse_hooks_mode = mode;
+    h->hooks_dirty = true;
+    invalidate_graph(h);
+    return FDTD_OK;
+}
+
+int fdtd_hooks_fused(fdtd_engine* h, int* active) {
+    if (!h || !active) return fail(FDTD_ERR_ARG, "null argument");
+    int rc = build_hooks(h);
+    if (rc) return rc;
+    *active = h->hooks_fused ? 1 : 0;
+    return FDTD_OK;
+}
+
 int fdtd_set_use_graph(fdtd_engine* h, int enable) {
     if (!h) return fail(FDTD_ERR_ARG, "null handle");
     h->use_graph = enable != 0;
@@ -1662,6 +1834,7 @@ static int iterate_steps(fdtd_engine* h, uint32_t nsteps) {
     int rc = ready_check(h);
     if (rc) return rc;
     if ((rc = finalize_pml(h))) return rc;
+    if ((rc = build_hooks(h))) return rc;
     if ((rc = finalize_probes(h, h->numTS + nsteps))) return rc;
     if (!h->sampled0) {
         // ProcessingArray::Process before the loop (openems.cpp:1277-1278): sample at numTS=0
@@ -1692,7 +1865,7 @@ static int iterate_steps(fdtd_engine* h, uint32_t nsteps) {
             const uint64_t ev = std::max<uint64_t>(2 * p, ((uint64_t)h->numTS + p - 1) / p * p);
             if (ev == h->numTS) {
                 h->ss_eval_now = true;
-                rc = launch_step(h, false);
+                rc = launch_step(h, false, true, true);
                 h->ss_eval_now = false;
                 if (rc) return rc;
                 HIPCHK(hipGetLastError());
@@ -1723,7 +1896,7 @@ static int advance(fdtd_engine* h, uint32_t nsteps) {
         }
         h->ev_used = 0;
         for (uint32_t s = 0; s < nsteps; ++s)
-            if ((rc = launch_step(h, true))) return rc;
+            if ((rc = launch_step(h, true, s == 0, s + 1 == nsteps))) return rc;
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventSynchronize(h->ev[need - 1]));
         for (uint32_t s = 0; s < nsteps; ++s) {
@@ -1738,7 +1911,7 @@ static int advance(fdtd_engine* h, uint32_t nsteps) {
     }
     if (!h->use_graph) {
         for (uint32_t s = 0; s < nsteps; ++s)
-            if ((rc = launch_step(h, false))) return rc;
+            if ((rc = launch_step(h, false, s == 0, s + 1 == nsteps))) return rc;
         HIPCHK(hipGetLastError());
         h->numTS += nsteps;
         return FDTD_OK;

@@ -97,6 +97,8 @@ struct AdeHost {
     uint32_t count = 0;
     uint32_t flags = 0;
     DevBuf off, coef, state;   // coef: 6 lists [3][count]; state: 4 lists [3][count]
+    std::vector<long long> host_off;   // the offsets, kept for the fused hook programs
+    DevBuf shared[2];                  // fused hooks: AdeDev::shared masks (volt, curr)
 };
 
 // lumped RLC cells: offsets, components, coefficients [9][count], state
@@ -168,6 +170,7 @@ struct fdtd_engine {
     bool shell_ok = false;
     bool pml_upper = false;       // a slab reaches x = Nx-1, y = Ny-1 or z = Nz-1
     bool curr_written = false;    // the caller wrote current values (SetCurr, fdtd_set_field)
+    bool curr_hook_faces = false; // a current-side hook writes a cell of those planes
     int ibox[3][2] = {{0, 0}, {0, 0}, {0, 0}};
     int pml_split_mode = -1;      // -1 auto (FP16/BF16 storage), 0 off, 1 on
 
@@ -192,6 +195,15 @@ struct fdtd_engine {
     // TFSF
     TfsfDev tfsf_v{}, tfsf_c{};
     DevBuf tfsf_buf_v, tfsf_buf_c;
+
+    // fused sparse hooks (k_hooks): host copies of the hook offsets in
+    // device order, and the per-location programs of each half step
+    std::vector<long long> exc_host_off[2], tfsf_host_off[2];
+    HookProg hook[2]{};
+    DevBuf hook_buf[2];
+    bool hooks_dirty = true;
+    bool hooks_fused = false;     // programs built and in use
+    int fuse_hooks_mode = -1;     // -1 auto (when no Mur, RLC or steady-state hook interleaves), 0 off, 1 on where possible
 
     // probes
     std::vector<ProbeHost*> probes;
@@ -248,18 +260,20 @@ int finalize_pml(fdtd_engine* h);
 void find_shell(fdtd_engine* h);
 int compress_pml(fdtd_engine* h);
 // split interior / shell launches of the vector kernels (see fdtd_set_pml_split):
-// by default where the interior holds at least 88% of the cells for 2-byte
-// storage and 95% for FP32 (the shell launch's scattered z-slab vectors cost
-// more than the branching they remove when the shell is thicker; measured:
-// FP32 1537x1537x769 with 96% interior 18.4 -> 15.9 ms per voltage update,
-// 769x769x385 with 92% no gain); FP64 keeps one launch
+// by default where the interior holds at least half of the cells, for
+// 2-byte storage and for FP32 with compressed coefficients; FP32 with full
+// coefficient arrays and FP64 keep one launch.  Measured per step on
+// MI355X: 1537x1537x769 dipole 34.1 -> 30.5 ms, 385x385x193 dipole 0.763 ->
+// 0.721 ms, dispersive 385x385x193 0.910 -> 0.891 ms (FP16: 0.780 ->
+// 0.667), graded 257^3 FP16 0.448 -> 0.437 ms; graded 257^3 FP32 with full
+// coefficients 0.475 -> 0.498 ms.
 inline bool pml_split_in_use(const fdtd_engine* h) {
     if (!h->shell_ok || h->pml_split_mode == 0) return false;
     if (h->pml_split_mode == 1) return true;
-    if (h->elem != 2 && h->elem != 4) return false;
+    if (h->elem == 8 || (h->elem == 4 && h->cf_mode == CF_FULL)) return false;
     long long inner = 1;
     for (int n = 0; n < 3; ++n) inner *= h->ibox[n][1] - h->ibox[n][0];
-    return (double)inner >= (h->elem == 2 ? 0.88 : 0.95) * (double)h->N[0] * h->N[1] * h->N[2];
+    return (double)inner >= 0.5 * (double)h->N[0] * h->N[1] * h->N[2];
 }
 
 // UPML coefficient rows: by default with full main coefficients, and with

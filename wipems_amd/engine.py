@@ -271,6 +271,15 @@ class Engine:
     def set_use_graph(self, on):
         self._chk(self._lib.fdtd_set_use_graph(self._h, int(bool(on))))
 
+    def set_fuse_hooks(self, mode=-1):
+        """1/-1: one launch for the sparse hooks of a half step (where allowed), 0: one launch per hook."""
+        self._chk(self._lib.fdtd_set_fuse_hooks(self._h, int(mode)))
+
+    def hooks_fused(self):
+        a = c_int()
+        self._chk(self._lib.fdtd_hooks_fused(self._h, byref(a)))
+        return bool(a.value)
+
     def set_pml_compression(self, mode=-1):
         """1: UPML coefficient rows, 0: full slab arrays, -1: rows only with full main coefficients."""
         self._chk(self._lib.fdtd_set_pml_compression(self._h, int(mode)))
@@ -281,7 +290,7 @@ class Engine:
         return bool(a.value)
 
     def set_pml_split(self, mode=-1):
-        """Interior/shell launches for UPML: 1 split, 0 one launch, -1 auto (FP16/BF16 with an interior of >= 88% of the cells)."""
+        """Interior/shell launches for UPML: 1 split, 0 one launch, -1 auto (2-byte storage, or FP32 with compressed coefficients, interior >= half)."""
         self._chk(self._lib.fdtd_set_pml_split(self._h, int(mode)))
 
     def pml_split(self):
