@@ -71,12 +71,15 @@ def kernel_bytes(eng, data):
     """Algorithmic HBM bytes per cell of one launch of the voltage (or current)
     update kernel: read 3 own + 3 other field components, write 3 own, plus
     the coefficients (12 floats in full mode, a 2- or 4-byte index with the
-    compressed table)."""
+    compressed table, a 2-byte index with the factored table; tables and mesh
+    factors stay in cache)."""
     fb = 8 if eng.precision == 1 else (4 if eng.precision == 0 else 2)
     # coefficients are FP32 for the reduced-precision storage modes
     cb = 8 if eng.precision == 1 else 4
     field = 9 * fb
-    if eng.compressed:
+    if eng.factored:
+        coef = 2
+    elif eng.compressed:
         coef = 2 if (data.table is not None and len(data.table) <= 65536) else 4
     else:
         coef = 6 * cb
@@ -237,7 +240,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="large_dipole", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64", "fp16", "bf16"])
-    ap.add_argument("--coeffs", default="auto", choices=["auto", "full", "compressed"])
+    ap.add_argument("--coeffs", default="auto", choices=["auto", "full", "compressed", "factored"])
     ap.add_argument("--split", type=int, default=None, help="UPML launch form (fdtd_set_pml_split): 0 or 1; default: the engine's")
     ap.add_argument("--pml-rows", type=int, default=None, help="UPML coefficient rows (fdtd_set_pml_compression): 0 or 1; default: the engine's")
     ap.add_argument("--fuse-hooks", type=int, default=None, help="fused sparse hooks (fdtd_set_fuse_hooks): 0 or 1; default: the engine's")
@@ -272,21 +275,22 @@ def main():
     cells = data.cells
     log("workload %s: %d cells, dT=%.4e s, operator build %.1f s" % (args.workload, cells, data.dT, time.time() - t0))
 
-    compressed = {"auto": None, "full": False, "compressed": True}[args.coeffs]
+    compressed = {"auto": None, "full": False, "compressed": True, "factored": None}[args.coeffs]
+    factored = {"factored": True, "full": False, "compressed": False}.get(args.coeffs)
     scales = None
     if args.precision in ("fp16", "bf16"):
         t0 = time.time()
         pilot = max(64, data.excitation.length) if data.vv is None else None
         scales = pilot_scales(data, steps=pilot, compressed=compressed, samples=8)
         log("storage scales (pilot FP32 run, %.1f s): volt %s curr %s" % (time.time() - t0, scales[0], scales[1]))
-    eng = engine_from_operator(data, precision=args.precision, compressed=compressed, scales=scales)
+    eng = engine_from_operator(data, precision=args.precision, compressed=compressed, scales=scales, factored=factored)
     if args.split is not None:
         eng.set_pml_split(args.split)
     if args.pml_rows is not None:
         eng.set_pml_compression(args.pml_rows)
     if args.fuse_hooks is not None:
         eng.set_fuse_hooks(args.fuse_hooks)
-    log("device %s, coefficients %s, UPML split %s, device bytes %.2f GB" % (device_name(), "compressed" if eng.compressed else "full",
+    log("device %s, coefficients %s, UPML split %s, device bytes %.2f GB" % (device_name(), "factored" if eng.factored else ("compressed" if eng.compressed else "full"),
                                                                           eng.pml_split(), eng.device_bytes() / 1e9))
 
     if args.profile_steps:
@@ -342,7 +346,7 @@ def main():
         if args.workload.startswith("large_dipole"):
             err = subbox_check(args, kw, stated)
         else:
-            err = fp64_check(builder, kw, sc, stated, args, compressed, scales)
+            err = fp64_check(builder, kw, sc, stated, args, compressed, scales, factored)
 
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "MCells/s", "n_gpus": 1, "steps": steps,
@@ -350,7 +354,7 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": {"fp32": "f32", "fp64": "f64", "fp16": "f16-store/f32", "bf16": "bf16-store/f32"}[args.precision],
         "data": "synthetic (seeded geometry)",
         "config": {"workload": args.workload, "description": desc, "cells": cells, "stated_steps": stated,
-                   "coefficients": "compressed" if eng.compressed else "full", "precision": args.precision},
+                   "coefficients": "factored" if eng.factored else ("compressed" if eng.compressed else "full"), "precision": args.precision},
         "roofline": roofline, "cpu_baseline": cpu, "error_vs_fp64": err,
     }
     print(json.dumps(out), flush=True)
@@ -429,7 +433,7 @@ def subbox_check(args, kw, stated):
     return out
 
 
-def fp64_check(builder, kw, sc, stated, args, compressed, scales):
+def fp64_check(builder, kw, sc, stated, args, compressed, scales, factored=None):
     """Run the workload for its full stated number of steps on the timed
     storage precision and on an FP64 handle (double operator build, double
     fields and coefficients), and compare the probe series; for a lumped port
@@ -442,7 +446,7 @@ def fp64_check(builder, kw, sc, stated, args, compressed, scales):
     kw64["prec"] = 1
     sc64 = getattr(scenarios, builder)(**kw64)
     e64 = engine_from_operator(sc64.data, precision="fp64", compressed=False)
-    e = engine_from_operator(sc.data, precision=args.precision, compressed=compressed, scales=scales)
+    e = engine_from_operator(sc.data, precision=args.precision, compressed=compressed, scales=scales, factored=factored)
     done = 0
     while done < stated:
         n = min(2000, stated - done)

@@ -135,6 +135,16 @@ int  fdtd_set_coeffs(fdtd_engine* h, int dtype,
  * n_entries rows of 12 values {vv0,vv1,vv2, vi0,vi1,vi2, ii0,ii1,ii2, iv0,iv1,iv2}. */
 int  fdtd_set_coeffs_compressed(fdtd_engine* h, const uint32_t* index,
                                 int dtype, const void* table, uint32_t n_entries);
+/* Factored coefficients (no counterpart in the reference; for FP32, FP16
+ * and BF16 handles, whose graded meshes defeat the plain compression): a
+ * per-cell index [Nx][Ny][Nz] into at most 65536 rows of 12 floats
+ * {vv0..2, m_E0..2, ii0..2, m_H0..2} and per-line mesh factors fv, fc
+ * [3][Nx+Ny+Nz]; the engine uses vi = m_E * ((fv[n][x] * fv[n][Nx+y]) *
+ * fv[n][Nx+Ny+z]) and iv likewise with fc, in float arithmetic in that order
+ * (wop_factor builds all four).  The update kernels read 2 bytes of index per
+ * cell instead of 24 bytes of coefficients. */
+int  fdtd_set_coeffs_factored(fdtd_engine* h, const uint32_t* index, const float* table, uint32_t n_entries,
+                              const float* fv, const float* fc);
 /* Per-component storage scales for FP16/BF16 handles (volt, curr); stored
  * value = physical value / scale.  Ignored for FP32/FP64 handles. */
 int  fdtd_set_storage_scale(fdtd_engine* h, const double volt_scale[3], const double curr_scale[3]);

@@ -121,6 +121,18 @@ int  wop_get_ec(wop_operator* op, double* C, double* G, double* L, double* R);
 /* compression of the coefficients: returns the number of unique 12-tuples and
  * (if index/table are non-null) the per-cell index and the table [n][12] */
 int  wop_compress(wop_operator* op, uint32_t* n_entries, uint32_t* index, double* table, uint32_t max_entries);
+/* Factored form of the coefficients for float engines with 2-byte field
+ * storage (no counterpart in the reference, whose compression,
+ * FDTD/operator_sse_compressed.cpp:120-176, dedups {vv,vi,ii,iv} as they
+ * are): vi = m_E * ((fv[n][x] * fv[n][Nx+y]) * fv[n][Nx+Ny+z]) and
+ * iv = m_H * ((fc[n][x] * fc[n][Nx+y]) * fc[n][Nx+Ny+z]) in float arithmetic,
+ * fv/fc [3][Nx+Ny+Nz] the mesh factors of the EC formulas and m rounded to
+ * `bits` (8..23) significant bits; index[Nx*Ny*Nz] (caller-owned) addresses
+ * table[max_entries][12] floats {vv0..2, m_E0..2, ii0..2, m_H0..2}.
+ * Returns -2 when more than max_entries tuples exist; max_rel_err receives
+ * the largest |rebuilt - built| / |built| over nonzero vi/iv. */
+int  wop_factor(wop_operator* op, int bits, uint32_t max_entries, uint32_t* index, float* table, uint32_t* n_entries,
+                float* fv, float* fc, double* max_rel_err);
 /* Float build for large domains that never holds full per-cell arrays
  * (Operator::CalcECOperator + Operator_SSE_Compressed::CompressOperator,
  * FDTD/operator.cpp:973-1084, FDTD/operator_sse_compressed.cpp:120-176):

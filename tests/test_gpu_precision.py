@@ -13,9 +13,10 @@ from wipems_amd.engine import engine_from_operator, pilot_scales
 pytestmark = pytest.mark.gpu
 
 
-def s11_run(sc, precision, steps, scales=None, freqs=None):
+def s11_run(sc, precision, steps, scales=None, freqs=None, factored=False):
     d = sc.data
-    eng = engine_from_operator(d, precision=precision, compressed=False, scales=scales)
+    eng = engine_from_operator(d, precision=precision, compressed=False, scales=scales, factored=factored)
+    assert eng.factored == factored
     eng.iterate(steps)
     tu, u = eng.read_probe(0)
     ti, i = eng.read_probe(1)
@@ -38,17 +39,19 @@ def test_fp16_storage_rounding_first_steps(gpu):
     np.testing.assert_array_equal(v16, emu)
 
 
-# measured on MI355X (48^3 cells, 3000 steps): FP32 3.1e-8, FP16 7.7e-5, BF16 1.8e-3
-@pytest.mark.parametrize("precision,tol", [("fp32", 1e-6), ("fp16", 1e-3), ("bf16", 1e-2)])
-def test_reduced_precision_s11_vs_fp64(gpu, precision, tol):
+# measured on MI355X (48^3 cells, 3000 steps): FP32 3.1e-8, FP16 7.7e-5, BF16 1.8e-3;
+# "factored": coefficients in the factored form (m rounded to 16 bits)
+@pytest.mark.parametrize("precision,factored,tol", [("fp32", False, 1e-6), ("fp16", False, 1e-3), ("bf16", False, 1e-2),
+                                                    ("fp32", True, 1e-4), ("fp16", True, 1e-3)])
+def test_reduced_precision_s11_vs_fp64(gpu, precision, factored, tol):
     sc64 = scenarios.graded_port_scenario(n_cells=48, steps=3000, prec=1, n_boxes=6, pml=8)
     sc32 = scenarios.graded_port_scenario(n_cells=48, steps=3000, prec=0, n_boxes=6, pml=8)
     freqs = np.linspace(1e9, 10e9, 91)
     s_ref, _, _ = s11_run(sc64, "fp64", 3000, freqs=freqs)
     scales = pilot_scales(sc32.data) if precision in ("fp16", "bf16") else None
-    s, _, _ = s11_run(sc32, precision, 3000, scales=scales, freqs=freqs)
+    s, _, _ = s11_run(sc32, precision, 3000, scales=scales, freqs=freqs, factored=factored)
     err = np.max(np.abs(s - s_ref))
-    print("%s max|S11-S11_fp64| = %.3e" % (precision, err))
+    print("%s%s max|S11-S11_fp64| = %.3e" % (precision, " factored" if factored else "", err))
     assert err <= tol
 
 

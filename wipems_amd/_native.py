@@ -8,7 +8,7 @@ from __future__ import annotations
 
 import ctypes
 import os
-from ctypes import POINTER, c_char_p, c_double, c_int, c_int32, c_size_t, c_uint32, c_uint64, c_void_p
+from ctypes import POINTER, c_char_p, c_double, c_float, c_int, c_int32, c_size_t, c_uint32, c_uint64, c_void_p
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 
@@ -39,6 +39,7 @@ _ENGINE_SIGS = {
     "fdtd_device_name": [c_char_p, c_size_t],
     "fdtd_set_coeffs": [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p],
     "fdtd_set_coeffs_compressed": [c_void_p, POINTER(c_uint32), c_int, c_void_p, c_uint32],
+    "fdtd_set_coeffs_factored": [c_void_p, POINTER(c_uint32), POINTER(c_float), c_uint32, POINTER(c_float), POINTER(c_float)],
     "fdtd_set_storage_scale": [c_void_p, POINTER(c_double), POINTER(c_double)],
     "fdtd_set_excitation": [c_void_p, POINTER(ExcitationDesc)],
     "fdtd_add_upml": [c_void_p, POINTER(c_uint32), POINTER(c_uint32), c_int] + [c_void_p] * 6,
@@ -134,6 +135,8 @@ _OPERATOR_SIGS = {
     "wop_get_coeffs": [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p],
     "wop_get_ec": [c_void_p, POINTER(c_double), POINTER(c_double), POINTER(c_double), POINTER(c_double)],
     "wop_compress": [c_void_p, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_double), c_uint32],
+    "wop_factor": [c_void_p, c_int, c_uint32, POINTER(c_uint32), POINTER(c_float), POINTER(c_uint32), POINTER(c_float),
+                   POINTER(c_float), POINTER(c_double)],
     "wop_build_streamed": [c_void_p, c_uint32, POINTER(c_uint32), c_uint32, POINTER(c_uint32)],
     "wop_get_streamed_table": [c_void_p, POINTER(c_double)],
     "wop_get_excitation_counts": [c_void_p, POINTER(c_uint32), POINTER(c_uint32)],

@@ -343,6 +343,7 @@ int fdtd_batch_create(fdtd_engine* const* engines, int count, fdtd_batch** out) 
         if (!h->mur.empty() || h->rlc || h->ss)
             return fail(FDTD_ERR_ARG, "engine %d has Mur ABC faces, lumped RLC cells or steady-state detection: not supported in a batch", i);
         if (h->exc_v.sequential || h->exc_c.sequential) return fail(FDTD_ERR_ARG, "engine %d has repeated excitation edges", i);
+        if (h->cf_mode == CF_FACT16) return fail(FDTD_ERR_ARG, "engine %d has factored coefficients: not supported in a batch", i);
         if (h->numTS != ts0 || h->sampled0 != engines[0]->sampled0) return fail(FDTD_ERR_STATE, "engines of a batch must be at the same timestep");
         for (int j = 0; j < i; ++j)
             if (engines[j] == h) return fail(FDTD_ERR_ARG, "engine %d appears twice", i);

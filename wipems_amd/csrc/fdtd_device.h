@@ -94,14 +94,20 @@ struct PmlPtrs {
     const unsigned short* cidx; const A* ctab;
 };
 
-// coefficient access modes
-enum { CF_FULL = 0, CF_IDX16 = 1, CF_IDX32 = 2 };
+// coefficient access modes; CF_FACT16 (float arithmetic only): a uint16
+// index into rows {a0,a1,a2,_,m0,m1,m2,_} and b = m * ((gx[x] * gy[y]) * gz[z])
+// from per-line mesh factors
+enum { CF_FULL = 0, CF_IDX16 = 1, CF_IDX32 = 2, CF_FACT16 = 3 };
 
 template <typename A>
 struct CoeffPtrs {
     const A* a;            // full: vv (E) or ii (H) [3][cells]; table: [U][8] rows {a0,a1,a2,_,b0,b1,b2,_}
     const A* b;            // full: vi (E) or iv (H)
     const void* idx;       // per-cell index (compressed modes)
+    // CF_FACT16 mesh factors: component n's x factors at g + n*gs, y factors
+    // at + gy, z factors at + gz (16-aligned, z padded like the fields)
+    const A* g;
+    int gy, gz, gs;
 };
 
 // cells z0..z0+vz-1 at (x,y) that lie in slab b: mask bit k = cell z0+k;
@@ -260,8 +266,27 @@ __device__ __forceinline__ SlabHits slab_hits(const PmlSet& ps, int x, int y, in
 }
 
 template <typename A, int CM, int VZ>
-__device__ __forceinline__ void vload_coeffs(const CoeffPtrs<A>& cf, long long p, long long sc, A a[3][VZ], A b[3][VZ]) {
-    if (CM == CF_FULL) {
+__device__ __forceinline__ void vload_coeffs(const CoeffPtrs<A>& cf, long long p, long long sc, int x, int y, int z0,
+                                             A a[3][VZ], A b[3][VZ]) {
+    if (CM == CF_FACT16) {
+        const SVec<unsigned short, VZ> ti = ld_stream(reinterpret_cast<const SVec<unsigned short, VZ>*>((const unsigned short*)cf.idx + p));
+#pragma unroll
+        for (int k = 0; k < VZ; ++k) {
+            const A* row = cf.a + (size_t)ti.v[k] * 8;
+            const SVec<A, 4> r0 = *reinterpret_cast<const SVec<A, 4>*>(row);
+            const SVec<A, 4> r1 = *reinterpret_cast<const SVec<A, 4>*>(row + 4);
+#pragma unroll
+            for (int n = 0; n < 3; ++n) { a[n][k] = r0.v[n]; b[n][k] = r1.v[n]; }
+        }
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+            const A* g = cf.g + n * cf.gs;
+            const A gxy = g[x] * g[cf.gy + y];
+            const SVec<A, VZ> gz = *reinterpret_cast<const SVec<A, VZ>*>(g + cf.gz + z0);
+#pragma unroll
+            for (int k = 0; k < VZ; ++k) b[n][k] = b[n][k] * (gxy * gz.v[k]);
+        }
+    } else if (CM == CF_FULL) {
 #pragma unroll
         for (int n = 0; n < 3; ++n) {
             const SVec<A, VZ> ta = ld_stream(reinterpret_cast<const SVec<A, VZ>*>(cf.a + n * sc + p));
@@ -331,7 +356,7 @@ __device__ __forceinline__ void volt_apply(S* __restrict__ V, const CoeffPtrs<ty
                                            const typename StoreTraits<S>::A curl[3][VZ]) {
     typedef typename StoreTraits<S>::A A;
     A vv[3][VZ], vi[3][VZ];
-    vload_coeffs<A, CM, VZ>(cf, p, d.sc, vv, vi);
+    vload_coeffs<A, CM, VZ>(cf, p, d.sc, x, y, z0, vv, vi);
     const int valid = min(VZ, d.nz - z0);
     SlabHits hit{{-1, -1}, {0u, 0u}, {0, 0}};
     SVec<unsigned short, VZ> prow[2] = {};
@@ -402,11 +427,14 @@ __device__ __forceinline__ void volt_vec_range(S* __restrict__ V, const S* __res
 }
 
 // Without UPML code the voltage update fits 5 waves per SIMD for every
-// storage type (the bound keeps FP16/BF16 at 96 VGPRs without spills); the
-// current update would spill under the same bound and the UPML forms need
-// more registers, so they get none.
+// storage type (the bound keeps FP16/BF16 at 96 VGPRs without spills),
+// except 2-byte storage with factored coefficients: forced to 96 VGPRs it
+// serialises its loads (graded 257^3 port: 132 us against 70 us for the
+// current update at 127 VGPRs), so it gets 4 waves.  The current update would
+// spill under the same bound and the UPML forms need more registers, so they
+// get none.
 template <typename S, int CM, int VZ>
-__global__ void __launch_bounds__(256, 5)
+__global__ void __launch_bounds__(256, (CM == CF_FACT16 && sizeof(S) == 2) ? 4 : 5)
 k_volt_vec(S* __restrict__ V, const S* __restrict__ I, CoeffPtrs<typename StoreTraits<S>::A> cf, Dims d,
            Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp, VecRange r) {
     volt_vec_range<S, CM, false, VZ>(V, I, cf, d, sc, ps, pp, r);
@@ -441,33 +469,36 @@ struct ShellGrid {
     long long nA0, nA1, nB, nC;
 };
 
-__host__ __device__ inline bool shell_vector(const ShellGrid& g, long long t, int vz, long long sy, int& x, long long& q0) {
-    int y, v;
-    const long long row = (long long)g.ny * g.vn;
-    if (t < g.nA0) {
-        x = (int)(t / row); const long long r = t - (long long)x * row;
-        y = (int)(r / g.vn); v = (int)(r - (long long)y * g.vn);
-    } else if ((t -= g.nA0) < g.nA1) {
-        x = (int)(t / row); const long long r = t - (long long)x * row;
+// Vector t of the shell, in 32-bit arithmetic: the engine splits only shells
+// of fewer than 2^31 vectors.
+__host__ __device__ inline bool shell_vector(const ShellGrid& g, unsigned t, int vz, long long sy, int& x, long long& q0) {
+    unsigned y, v;
+    const unsigned vn = (unsigned)g.vn;
+    const unsigned row = (unsigned)g.ny * vn;
+    if (t < (unsigned)g.nA0) {
+        x = (int)(t / row); const unsigned r = t - (unsigned)x * row;
+        y = r / vn; v = r - y * vn;
+    } else if ((t -= (unsigned)g.nA0) < (unsigned)g.nA1) {
+        x = (int)(t / row); const unsigned r = t - (unsigned)x * row;
         x += g.x1;
-        y = (int)(r / g.vn); v = (int)(r - (long long)y * g.vn);
-    } else if ((t -= g.nA1) < g.nB) {
-        const int nyb = g.y0 + g.ny - g.y1;
-        const long long per = (long long)nyb * g.vn;
-        x = (int)(t / per); const long long r = t - (long long)x * per;
+        y = r / vn; v = r - y * vn;
+    } else if ((t -= (unsigned)g.nA1) < (unsigned)g.nB) {
+        const unsigned nyb = (unsigned)(g.y0 + g.ny - g.y1);
+        const unsigned per = nyb * vn;
+        x = (int)(t / per); const unsigned r = t - (unsigned)x * per;
         x += g.x0;
-        const int yy = (int)(r / g.vn);
-        v = (int)(r - (long long)yy * g.vn);
-        y = yy < g.y0 ? yy : g.y1 + (yy - g.y0);
-    } else if ((t -= g.nB) < g.nC) {
-        const int nvz = g.vlo + g.vn - g.vhi;
-        const long long per = (long long)(g.y1 - g.y0) * nvz;
-        x = (int)(t / per); const long long r = t - (long long)x * per;
+        const unsigned yy = r / vn;
+        v = r - yy * vn;
+        y = yy < (unsigned)g.y0 ? yy : (unsigned)g.y1 + (yy - (unsigned)g.y0);
+    } else if ((t -= (unsigned)g.nB) < (unsigned)g.nC) {
+        const unsigned nvz = (unsigned)(g.vlo + g.vn - g.vhi);
+        const unsigned per = (unsigned)(g.y1 - g.y0) * nvz;
+        x = (int)(t / per); const unsigned r = t - (unsigned)x * per;
         x += g.x0;
-        const int yy = (int)(r / nvz);
-        const int j = (int)(r - (long long)yy * nvz);
-        y = g.y0 + yy;
-        v = j < g.vlo ? j : g.vhi + (j - g.vlo);
+        const unsigned yy = r / nvz;
+        const unsigned j = r - yy * nvz;
+        y = (unsigned)g.y0 + yy;
+        v = j < (unsigned)g.vlo ? j : (unsigned)g.vhi + (j - (unsigned)g.vlo);
     } else {
         return false;
     }
@@ -481,7 +512,7 @@ k_volt_shell(S* __restrict__ V, const S* __restrict__ I, CoeffPtrs<typename Stor
              Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp, ShellGrid g) {
     int x;
     long long q0;
-    if (!shell_vector(g, (long long)blockIdx.x * blockDim.x + threadIdx.x, VZ, d.sy, x, q0)) return;
+    if (!shell_vector(g, blockIdx.x * blockDim.x + threadIdx.x, VZ, d.sy, x, q0)) return;
     volt_vec_body<S, CM, true, VZ>(V, I, cf, d, sc, ps, pp, x, q0);
 }
 
@@ -523,7 +554,7 @@ __device__ __forceinline__ void curr_apply(S* __restrict__ I, const CoeffPtrs<ty
                                            const typename StoreTraits<S>::A curl[3][VZ]) {
     typedef typename StoreTraits<S>::A A;
     A ii[3][VZ], iv[3][VZ];
-    vload_coeffs<A, CM, VZ>(cf, p, d.sc, ii, iv);
+    vload_coeffs<A, CM, VZ>(cf, p, d.sc, x, y, z0, ii, iv);
     const int valid = min(VZ, d.nz - 1 - z0);
     // cells at z >= nz-1 are not updated: keep them out of the slab masks too
     const unsigned upd = valid < VZ ? (1u << valid) - 1u : ~0u;
@@ -619,7 +650,7 @@ k_curr_shell(S* __restrict__ I, const S* __restrict__ V, CoeffPtrs<typename Stor
              Scales<typename StoreTraits<S>::A> sc, PmlSet ps, PmlPtrs<typename StoreTraits<S>::A> pp, ShellGrid g) {
     int x;
     long long q0;
-    if (!shell_vector(g, (long long)blockIdx.x * blockDim.x + threadIdx.x, VZ, d.sy, x, q0)) return;
+    if (!shell_vector(g, blockIdx.x * blockDim.x + threadIdx.x, VZ, d.sy, x, q0)) return;
     curr_vec_body<S, CM, true, VZ>(I, V, cf, d, sc, ps, pp, x, q0);
 }
 

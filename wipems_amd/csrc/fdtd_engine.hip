@@ -501,14 +501,30 @@ int build_hooks(fdtd_engine* h) {
 }
 
 VecLaunch vec_launch(const fdtd_engine* h, int vz, bool curr) {
-    return plan_vec_launch(h->N, h->d.sy, h->pml.count, pml_split_in_use(h), h->ibox, vz, curr);
+    VecLaunch L = plan_vec_launch(h->N, h->d.sy, h->pml.count, pml_split_in_use(h), h->ibox, vz, curr);
+    // shell vectors are numbered in 32 bits, at up to 3x the count in
+    // half-width vectors: larger shells run as one UPML launch
+    if (L.shell && 3 * (L.g.nA0 + L.g.nA1 + L.g.nB + L.g.nC) >= (1LL << 31))
+        L = plan_vec_launch(h->N, h->d.sy, h->pml.count, false, h->ibox, vz, curr);
+    return L;
 }
+
+// Vectors of the UPML shell kernels: half the interior width for 2-byte
+// storage (4 cells, 104 VGPRs instead of 179: 4 waves per SIMD instead of 2;
+// large dipole FP16 19.2 instead of 21.6 ms per step, graded 257^3 port
+// 0.317 instead of 0.326 ms), the full width for FP32 and FP64 (FP32 at 2
+// cells measured 1-2% slower on the large dipole and the dispersive case).
+template <typename S> struct ShellWidth {
+    enum { value = sizeof(S) == 2 ? VecWidth<S>::value / 2 : VecWidth<S>::value };
+};
 
 template <typename S, bool PML, int CM>
 void launch_volt_main(fdtd_engine* h, const VecLaunch& L, CoeffPtrs<typename StoreTraits<S>::A> cf, PmlPtrs<typename StoreTraits<S>::A> pp) {
     constexpr int VZ = VecWidth<S>::value;
     const dim3 b(256, 1, 1);
-    const long long nshell = L.shell ? L.g.nA0 + L.g.nA1 + L.g.nB + L.g.nC : 0;
+    constexpr int VS = ShellWidth<S>::value;
+    const ShellGrid gs = VS == VZ ? L.g : shell_grid_split(L, (int)h->N[2], VZ, VZ / VS);
+    const long long nshell = L.shell ? gs.nA0 + gs.nA1 + gs.nB + gs.nC : 0;
     if (L.nx) {
         const dim3 g((unsigned)((L.nq / VZ + 255) / 256), L.nx, 1);
         if constexpr (PML && CM == CF_FULL && sizeof(S) >= 4)
@@ -519,15 +535,17 @@ void launch_volt_main(fdtd_engine* h, const VecLaunch& L, CoeffPtrs<typename Sto
             hipLaunchKernelGGL((k_volt_vec<S, CM, VZ>), g, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.r);
     }
     if (nshell > 0)
-        hipLaunchKernelGGL((k_volt_shell<S, CM, VZ>), dim3((unsigned)((nshell + 255) / 256)), b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d,
-                           make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.g);
+        hipLaunchKernelGGL((k_volt_shell<S, CM, VS>), dim3((unsigned)((nshell + 255) / 256)), b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d,
+                           make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, gs);
 }
 
 template <typename S, bool PML, int CM>
 void launch_curr_main(fdtd_engine* h, const VecLaunch& L, CoeffPtrs<typename StoreTraits<S>::A> cf, PmlPtrs<typename StoreTraits<S>::A> pp) {
     constexpr int VZ = VecWidth<S>::value;
     const dim3 b(256, 1, 1);
-    const long long nshell = L.shell ? L.g.nA0 + L.g.nA1 + L.g.nB + L.g.nC : 0;
+    constexpr int VS = ShellWidth<S>::value;
+    const ShellGrid gs = VS == VZ ? L.g : shell_grid_split(L, (int)h->N[2], VZ, VZ / VS);
+    const long long nshell = L.shell ? gs.nA0 + gs.nA1 + gs.nB + gs.nC : 0;
     if (L.nx) {
         const dim3 g((unsigned)((L.nq / VZ + 255) / 256), L.nx, 1);
         if constexpr (PML && CM == CF_FULL && sizeof(S) >= 4)
@@ -538,8 +556,8 @@ void launch_curr_main(fdtd_engine* h, const VecLaunch& L, CoeffPtrs<typename Sto
             hipLaunchKernelGGL((k_curr_vec<S, CM, VZ>), g, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.r);
     }
     if (nshell > 0)
-        hipLaunchKernelGGL((k_curr_shell<S, CM, VZ>), dim3((unsigned)((nshell + 255) / 256)), b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d,
-                           make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.g);
+        hipLaunchKernelGGL((k_curr_shell<S, CM, VS>), dim3((unsigned)((nshell + 255) / 256)), b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d,
+                           make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, gs);
 }
 
 template <typename S>
@@ -548,6 +566,11 @@ void launch_volt_vec(fdtd_engine* h, CoeffPtrs<typename StoreTraits<S>::A> cf, P
     switch (h->cf_mode) {
     case CF_FULL: if (L.pml_main) launch_volt_main<S, true, CF_FULL>(h, L, cf, pp); else launch_volt_main<S, false, CF_FULL>(h, L, cf, pp); break;
     case CF_IDX16: if (L.pml_main) launch_volt_main<S, true, CF_IDX16>(h, L, cf, pp); else launch_volt_main<S, false, CF_IDX16>(h, L, cf, pp); break;
+    case CF_FACT16:
+        if constexpr (sizeof(typename StoreTraits<S>::A) == 4) {
+            if (L.pml_main) launch_volt_main<S, true, CF_FACT16>(h, L, cf, pp); else launch_volt_main<S, false, CF_FACT16>(h, L, cf, pp);
+        }
+        break;
     default: if (L.pml_main) launch_volt_main<S, true, CF_IDX32>(h, L, cf, pp); else launch_volt_main<S, false, CF_IDX32>(h, L, cf, pp); break;
     }
 }
@@ -558,6 +581,11 @@ void launch_curr_vec(fdtd_engine* h, CoeffPtrs<typename StoreTraits<S>::A> cf, P
     switch (h->cf_mode) {
     case CF_FULL: if (L.pml_main) launch_curr_main<S, true, CF_FULL>(h, L, cf, pp); else launch_curr_main<S, false, CF_FULL>(h, L, cf, pp); break;
     case CF_IDX16: if (L.pml_main) launch_curr_main<S, true, CF_IDX16>(h, L, cf, pp); else launch_curr_main<S, false, CF_IDX16>(h, L, cf, pp); break;
+    case CF_FACT16:
+        if constexpr (sizeof(typename StoreTraits<S>::A) == 4) {
+            if (L.pml_main) launch_curr_main<S, true, CF_FACT16>(h, L, cf, pp); else launch_curr_main<S, false, CF_FACT16>(h, L, cf, pp);
+        }
+        break;
     default: if (L.pml_main) launch_curr_main<S, true, CF_IDX32>(h, L, cf, pp); else launch_curr_main<S, false, CF_IDX32>(h, L, cf, pp); break;
     }
 }
@@ -578,6 +606,10 @@ int launch_step_kernels(fdtd_engine* h, bool timed, bool first, bool last) {
     } else {
         cfe.a = h->cf_tabE.as<A>(); cfe.idx = h->cf_idx.p;
         cfh.a = h->cf_tabH.as<A>(); cfh.idx = h->cf_idx.p;
+        if (h->cf_mode == CF_FACT16) {
+            cfe.g = h->cf_gE.as<A>(); cfh.g = h->cf_gH.as<A>();
+            cfe.gy = cfh.gy = h->cf_gy; cfe.gz = cfh.gz = h->cf_gz; cfe.gs = cfh.gs = h->cf_gs;
+        }
     }
     PmlPtrs<A> pp{};
     if (h->pml.count) {
@@ -1061,6 +1093,31 @@ int fdtd_destroy(fdtd_engine* h) {
     return FDTD_OK;
 }
 
+// per-cell index (uint16 or uint32) in the padded field layout, and the E
+// and H tables of 8-wide rows
+static int upload_index_table(fdtd_engine* h, const uint32_t* index, bool i16, const std::vector<char>& te,
+                              const std::vector<char>& th) {
+    const size_t ib = i16 ? 2 : 4;
+    int rc;
+    if ((rc = h->cf_idx.alloc((size_t)h->d.sc * ib))) return rc;
+    if ((rc = h->cf_tabE.alloc(te.size())) || (rc = h->cf_tabH.alloc(th.size()))) return rc;
+    HIPCHK(hipMemsetAsync(h->cf_idx.p, 0, h->cf_idx.bytes, h->stream));
+    std::vector<char> rows((size_t)h->N[1] * h->N[2] * ib);
+    for (uint32_t x = 0; x < h->N[0]; ++x) {
+        const uint32_t* src = index + (size_t)x * h->N[1] * h->N[2];
+        if (i16) { uint16_t* o = (uint16_t*)rows.data(); for (size_t i = 0; i < (size_t)h->N[1] * h->N[2]; ++i) o[i] = (uint16_t)src[i]; }
+        else memcpy(rows.data(), src, rows.size());
+        HIPCHK(hipMemcpy2DAsync((char*)h->cf_idx.p + (size_t)x * h->d.sx * ib, (size_t)h->d.sy * ib, rows.data(), (size_t)h->N[2] * ib,
+                                (size_t)h->N[2] * ib, h->N[1], hipMemcpyHostToDevice, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+    }
+    HIPCHK(hipMemcpyAsync(h->cf_tabE.p, te.data(), te.size(), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->cf_tabH.p, th.data(), th.size(), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->cf_vv.alloc(0); h->cf_vi.alloc(0); h->cf_ii.alloc(0); h->cf_iv.alloc(0);
+    return FDTD_OK;
+}
+
 int fdtd_set_coeffs(fdtd_engine* h, int dtype, const void* vv, const void* vi, const void* ii, const void* iv) {
     if (!h || !vv || !vi || !ii || !iv) return fail(FDTD_ERR_ARG, "null argument");
     if (h->numTS) return fail(FDTD_ERR_STATE, "coefficients must be set before the first timestep");
@@ -1069,6 +1126,7 @@ int fdtd_set_coeffs(fdtd_engine* h, int dtype, const void* vv, const void* vi, c
         (rc = upload_dense3(h, dtype, ii, h->cf_ii)) || (rc = upload_dense3(h, dtype, iv, h->cf_iv)))
         return rc;
     h->cf_idx.alloc(0); h->cf_tabE.alloc(0); h->cf_tabH.alloc(0);
+    h->cf_gE.alloc(0); h->cf_gH.alloc(0);
     h->cf_mode = CF_FULL;
     invalidate_graph(h);
     return FDTD_OK;
@@ -1094,26 +1152,58 @@ int fdtd_set_coeffs_compressed(fdtd_engine* h, const uint32_t* index, int dtype,
         memcpy(th.data() + ((size_t)r * 8 + 4) * a, row + 9 * a, 3 * a);  // iv
     }
     const bool i16 = n_entries <= 65536;
-    const size_t ib = i16 ? 2 : 4;
-    if ((rc = h->cf_idx.alloc((size_t)h->d.sc * ib))) return rc;
-    if ((rc = h->cf_tabE.alloc(te.size())) || (rc = h->cf_tabH.alloc(th.size()))) return rc;
-    HIPCHK(hipMemsetAsync(h->cf_idx.p, 0, h->cf_idx.bytes, h->stream));
-    // pack index rows into the padded layout
-    std::vector<char> rows((size_t)h->N[1] * h->N[2] * ib);
-    for (uint32_t x = 0; x < h->N[0]; ++x) {
-        const uint32_t* src = index + (size_t)x * h->N[1] * h->N[2];
-        if (i16) { uint16_t* o = (uint16_t*)rows.data(); for (size_t i = 0; i < (size_t)h->N[1] * h->N[2]; ++i) o[i] = (uint16_t)src[i]; }
-        else memcpy(rows.data(), src, rows.size());
-        HIPCHK(hipMemcpy2DAsync((char*)h->cf_idx.p + (size_t)x * h->d.sx * ib, (size_t)h->d.sy * ib, rows.data(), (size_t)h->N[2] * ib,
-                                (size_t)h->N[2] * ib, h->N[1], hipMemcpyHostToDevice, h->stream));
-        HIPCHK(hipStreamSynchronize(h->stream));
-    }
-    HIPCHK(hipMemcpyAsync(h->cf_tabE.p, te.data(), te.size(), hipMemcpyHostToDevice, h->stream));
-    HIPCHK(hipMemcpyAsync(h->cf_tabH.p, th.data(), th.size(), hipMemcpyHostToDevice, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
-    h->cf_vv.alloc(0); h->cf_vi.alloc(0); h->cf_ii.alloc(0); h->cf_iv.alloc(0);
+    if ((rc = upload_index_table(h, index, i16, te, th))) return rc;
+    h->cf_gE.alloc(0); h->cf_gH.alloc(0);
     h->n_entries = n_entries;
     h->cf_mode = i16 ? CF_IDX16 : CF_IDX32;
+    invalidate_graph(h);
+    return FDTD_OK;
+}
+
+int fdtd_set_coeffs_factored(fdtd_engine* h, const uint32_t* index, const float* table, uint32_t n_entries, const float* fv,
+                             const float* fc) {
+    if (!h || !index || !table || !n_entries || !fv || !fc) return fail(FDTD_ERR_ARG, "null argument");
+    if (h->numTS) return fail(FDTD_ERR_STATE, "coefficients must be set before the first timestep");
+    if (h->acc != 4) return fail(FDTD_ERR_ARG, "factored coefficients need float arithmetic (FP32, FP16 or BF16 handles)");
+    if (n_entries > 65536) return fail(FDTD_ERR_RANGE, "factored tables hold at most 65536 rows (got %u)", n_entries);
+    const size_t nd = (size_t)h->N[0] * h->N[1] * h->N[2];
+    for (size_t i = 0; i < nd; ++i)
+        if (index[i] >= n_entries) return fail(FDTD_ERR_RANGE, "index[%zu]=%u >= n_entries=%u", i, index[i], n_entries);
+    const size_t L = (size_t)h->N[0] + h->N[1] + h->N[2];
+    for (size_t i = 0; i < 3 * L; ++i)
+        if (!std::isfinite(fv[i]) || !std::isfinite(fc[i])) return fail(FDTD_ERR_ARG, "mesh factor %zu is not finite", i);
+    std::vector<char> te((size_t)n_entries * 8 * 4, 0), th((size_t)n_entries * 8 * 4, 0);
+    for (uint32_t r = 0; r < n_entries; ++r) {
+        const float* row = table + (size_t)r * 12;
+        memcpy(te.data() + (size_t)r * 32, row, 12);           // vv
+        memcpy(te.data() + (size_t)r * 32 + 16, row + 3, 12);  // m_E
+        memcpy(th.data() + (size_t)r * 32, row + 6, 12);       // ii
+        memcpy(th.data() + (size_t)r * 32 + 16, row + 9, 12);  // m_H
+    }
+    int rc;
+    if ((rc = upload_index_table(h, index, true, te, th))) return rc;
+    // factors per component: x lines, y lines (both padded to 16), z lines
+    // padded to the field row length; padding holds zeros
+    const int gx = (int)((h->N[0] + 15) / 16 * 16), gy = (int)((h->N[1] + 15) / 16 * 16);
+    const int gs = gx + gy + (int)h->d.sy;
+    std::vector<float> ge((size_t)3 * gs, 0.0f), gh((size_t)3 * gs, 0.0f);
+    for (int n = 0; n < 3; ++n) {
+        const float* sv = fv + n * L;
+        const float* sc = fc + n * L;
+        for (uint32_t i = 0; i < h->N[0]; ++i) { ge[(size_t)n * gs + i] = sv[i]; gh[(size_t)n * gs + i] = sc[i]; }
+        for (uint32_t i = 0; i < h->N[1]; ++i) { ge[(size_t)n * gs + gx + i] = sv[h->N[0] + i]; gh[(size_t)n * gs + gx + i] = sc[h->N[0] + i]; }
+        for (uint32_t i = 0; i < h->N[2]; ++i) {
+            ge[(size_t)n * gs + gx + gy + i] = sv[h->N[0] + h->N[1] + i];
+            gh[(size_t)n * gs + gx + gy + i] = sc[h->N[0] + h->N[1] + i];
+        }
+    }
+    if ((rc = h->cf_gE.alloc(ge.size() * 4)) || (rc = h->cf_gH.alloc(gh.size() * 4))) return rc;
+    HIPCHK(hipMemcpyAsync(h->cf_gE.p, ge.data(), ge.size() * 4, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->cf_gH.p, gh.data(), gh.size() * 4, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->cf_gy = gx; h->cf_gz = gx + gy; h->cf_gs = gs;
+    h->n_entries = n_entries;
+    h->cf_mode = CF_FACT16;
     invalidate_graph(h);
     return FDTD_OK;
 }
@@ -2056,7 +2146,7 @@ int fdtd_copy_bench(uint64_t bytes, int reps, double* gbps) {
 int fdtd_device_bytes(fdtd_engine* h, uint64_t* bytes) {
     if (!h || !bytes) return fail(FDTD_ERR_ARG, "null argument");
     uint64_t b = h->V.bytes + h->I.bytes + h->cf_vv.bytes + h->cf_vi.bytes + h->cf_ii.bytes + h->cf_iv.bytes + h->cf_idx.bytes +
-                 h->cf_tabE.bytes + h->cf_tabH.bytes + h->pml_coef.bytes + h->pml_flux.bytes + h->exc_buf.bytes;
+                 h->cf_tabE.bytes + h->cf_tabH.bytes + h->cf_gE.bytes + h->cf_gH.bytes + h->pml_coef.bytes + h->pml_flux.bytes + h->exc_buf.bytes;
     for (auto a : h->ade) b += a->off.bytes + a->coef.bytes + a->state.bytes;
     *bytes = b;
     return FDTD_OK;

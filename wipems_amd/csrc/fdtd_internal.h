@@ -146,7 +146,9 @@ struct fdtd_engine {
 
     int cf_mode = -1;             // CF_* once set
     DevBuf cf_vv, cf_vi, cf_ii, cf_iv;   // full mode
-    DevBuf cf_idx, cf_tabE, cf_tabH;     // compressed mode
+    DevBuf cf_idx, cf_tabE, cf_tabH;     // compressed and factored modes
+    DevBuf cf_gE, cf_gH;                 // factored mode: per-line mesh factors
+    int cf_gy = 0, cf_gz = 0, cf_gs = 0; // their y / z offsets and component stride
     uint32_t n_entries = 0;
 
     double vscale[3] = {1, 1, 1}, cscale[3] = {1, 1, 1};

@@ -90,3 +90,19 @@ inline VecLaunch plan_vec_launch(const uint32_t N[3], long long sy, int pml_coun
     g.nC = (long long)(x1 - x0) * (y1 - y0) * (vlo + vn - vhi);
     return L;
 }
+
+// The shell of L (planned with vectors of vz cells) in vectors of vz / f
+// cells: the interior range [vlo*vz, vhi*vz) stays the same, so the two
+// launches still partition the vectors' cells.
+inline ShellGrid shell_grid_split(const VecLaunch& L, int nz, int vz, int f) {
+    ShellGrid g = L.g;
+    const int w = vz / f;
+    g.vn = (nz + w - 1) / w;
+    g.vlo = std::min(L.g.vlo * f, g.vn);
+    g.vhi = std::max(std::min(L.g.vhi * f, g.vn), g.vlo);
+    g.nA0 = (long long)g.x0 * g.ny * g.vn;
+    g.nA1 = (long long)(g.xe - g.x1) * g.ny * g.vn;
+    g.nB = (long long)(g.x1 - g.x0) * (g.y0 + g.ny - g.y1) * g.vn;
+    g.nC = (long long)(g.x1 - g.x0) * (g.y1 - g.y0) * (g.vlo + g.vn - g.vhi);
+    return g;
+}

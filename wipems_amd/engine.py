@@ -10,7 +10,7 @@ raises FdtdError).
 from __future__ import annotations
 
 import ctypes
-from ctypes import POINTER, byref, c_double, c_int, c_uint32, c_uint64, c_void_p
+from ctypes import POINTER, byref, c_double, c_float, c_int, c_uint32, c_uint64, c_void_p
 
 import numpy as np
 
@@ -70,6 +70,8 @@ class Engine:
         self.num_lines = tuple(int(x) for x in num_lines)
         self.dT = float(dT)
         self.precision = prec
+        self.compressed = False
+        self.factored = False
         self.probe_ids = []
         self.dft_boxes = []
         self._keep = []
@@ -104,6 +106,15 @@ class Engine:
         idx = np.ascontiguousarray(index, dtype=np.uint32)
         tab = np.ascontiguousarray(table, dtype=np.float64)
         self._chk(self._lib.fdtd_set_coeffs_compressed(self._h, _up(idx), 1, _vp(tab), int(tab.shape[0])))
+
+    def set_coeffs_factored(self, fac):
+        """Factored coefficients (operator.FactoredCoeffs) for float handles."""
+        idx = np.ascontiguousarray(fac.index, dtype=np.uint32)
+        tab = np.ascontiguousarray(fac.table, dtype=np.float32)
+        fv = np.ascontiguousarray(fac.fv, dtype=np.float32)
+        fc = np.ascontiguousarray(fac.fc, dtype=np.float32)
+        fp = lambda a: a.ctypes.data_as(POINTER(c_float))
+        self._chk(self._lib.fdtd_set_coeffs_factored(self._h, _up(idx), fp(tab), int(tab.shape[0]), fp(fv), fp(fc)))
 
     def set_storage_scale(self, volt_scale, curr_scale):
         v = (c_double * 3)(*volt_scale)
@@ -404,25 +415,42 @@ def pilot_scales(data: OperatorData, steps=None, headroom=1024.0, compressed=Non
     return sv, si
 
 
-def engine_from_operator(data: OperatorData, precision="fp32", compressed=None, max_entries=65536, scales=None):
+def engine_from_operator(data: OperatorData, precision="fp32", compressed=None, max_entries=65536, scales=None,
+                         factored=None):
     """Create an Engine and load the operator, extensions and probes into it.
 
     compressed: True -> per-cell index + table (fails if more than max_entries
     unique tuples), False -> full coefficient arrays, None -> compress when the
-    table fits in max_entries."""
+    table fits in max_entries.
+    factored: True -> factored coefficients (float handles; fails if they do
+    not fit 65536 rows), None -> factored for FP16/BF16 handles whose plain
+    table does not fit (when the Operator's full build is at hand), False ->
+    never."""
     eng = Engine(data.num_lines, data.dT, precision)
     use_cmp = compressed
-    if compressed is None or compressed:
+    fac = None
+    if factored:
+        fac = data.factor()
+        if fac is None:
+            raise ValueError("more than 65536 factored coefficient tuples")
+        use_cmp = False
+    elif compressed is None or compressed:
         if data.index is None:
             res = data.compress(max_entries if compressed is None else 0)
         else:
             res = (data.index, data.table)
         use_cmp = res is not None
-    if use_cmp:
+        if (not use_cmp and factored is None and precision in ("fp16", "bf16") and data.source is not None
+                and getattr(data.source, "_built_prec", None) == data.prec and data.vv is not None):
+            fac = data.factor()
+    if fac is not None:
+        eng.set_coeffs_factored(fac)
+    elif use_cmp:
         eng.set_coeffs_compressed(data.index, data.table)
     else:
         eng.set_coeffs(data.vv, data.vi, data.ii, data.iv)
     eng.compressed = bool(use_cmp)
+    eng.factored = fac is not None
     if scales is not None:
         eng.set_storage_scale(*scales)
     if data.excitation is not None and data.excitation.length:

@@ -2310,6 +2310,87 @@ extern "C" int wop_compress(wop_operator* op, uint32_t* n_entries, uint32_t* ind
     return 0;
 }
 
+// m rounded to `bits` significant bits (ties away from zero)
+static float round_mantissa(double m, int bits) {
+    if (m == 0.0) return 0.0f;
+    int e;
+    const double f = std::frexp(m, &e);
+    return (float)std::ldexp(std::round(std::ldexp(f, bits)), e - bits);
+}
+
+// Factored coefficients: the EC formulas (Calc_ECOperatorPos,
+// FDTD/operator.cpp:943-971, with C = eps*A/L and L = mue*A'/L' from
+// Calc_ECPos, :1176-1224) make vi a material term times dT*L/A, L the primary
+// edge along n and A the dual area across it, and iv a material term times
+// dT*L'/A', L' the dual edge and A' the primary area.  Both mesh terms are
+// products of one factor per axis, so with float factors
+//   fv[n][axis][i] = P_n(i) (axis n) or 1/D_axis(i), fc[n][axis][i] = D_n(i) or 1/P_axis(i)
+// (the x factor carrying dT/EPS0 resp. dT/MUE0) the coefficients are
+//   vi = m_E * ((fv[n][0][x] * fv[n][1][y]) * fv[n][2][z]),  iv likewise,
+// in float arithmetic in that order, with m = vi / g rounded to `bits`
+// significant bits.  Cells of one material share m however the mesh is
+// graded, so the tuples {vv, m_E, ii, m_H} collapse to far fewer rows than
+// {vv, vi, ii, iv}.  The UPML rows and lumped elements are ordinary tuples.
+extern "C" int wop_factor(wop_operator* op, int bits, uint32_t max_entries, uint32_t* index, float* table, uint32_t* n_entries,
+                          float* fv, float* fc, double* max_rel_err) {
+    if (!op || !op->built || !n_entries || !index || !table || !fv || !fc || !max_entries) return fail(-1, "bad argument");
+    if (op->streamed) return fail(-1, "the streamed build keeps no per-cell coefficients");
+    if (bits < 8 || bits > 23) return fail(-1, "bits must be 8..23");
+    const uint32_t Nx = op->N[0], Ny = op->N[1], Nz = op->N[2];
+    const size_t L = (size_t)Nx + Ny + Nz;
+    for (int n = 0; n < 3; ++n) {
+        size_t k = 0;
+        for (int a = 0; a < 3; ++a)
+            for (uint32_t i = 0; i < op->N[a]; ++i, ++k) {
+                uint32_t pos[3] = {0, 0, 0};
+                pos[a] = i;
+                const double P = op->EdgeLength(a, pos, false), D = op->EdgeLength(a, pos, true);
+                if (!(P > 0) || !(D > 0)) return fail(-1, "edge length <= 0 on axis %d line %u", a, i);
+                double e = a == n ? P : 1.0 / D;
+                double h = a == n ? D : 1.0 / P;
+                if (a == 0) { e *= op->dT / EPS0; h *= op->dT / MUE0; }
+                fv[n * L + k] = (float)e;
+                fc[n * L + k] = (float)h;
+                if (!std::isfinite(fv[n * L + k]) || fv[n * L + k] == 0.0f || !std::isfinite(fc[n * L + k]) || fc[n * L + k] == 0.0f)
+                    return fail(-1, "mesh factor out of float range on axis %d line %u", a, i);
+            }
+    }
+    std::vector<double> rowerr(Ny, 0.0);
+    PlaneDedup<std::array<uint32_t, 12>> tab;
+    const size_t plane = (size_t)Ny * Nz;
+    for (uint32_t x = 0; x < Nx; ++x) {
+        const bool ok = tab.plane(op, index + (size_t)x * plane, max_entries, [&](uint32_t y, uint32_t z, std::array<uint32_t, 12>& t) {
+            const size_t i = op->idx(x, y, z);
+            const uint32_t at[3] = {x, Nx + y, Nx + Ny + z};
+            for (int n = 0; n < 3; ++n) {
+                float q[4];
+                q[0] = (float)op->coef_at(0, n, i);
+                q[2] = (float)op->coef_at(2, n, i);
+                for (int side = 0; side < 2; ++side) {
+                    const float* f = (side ? fc : fv) + n * L;
+                    const float g = (f[at[0]] * f[at[1]]) * f[at[2]];
+                    const float c = (float)op->coef_at(side ? 3 : 1, n, i);
+                    const float m = round_mantissa((double)c / (double)g, bits);
+                    const float r = m * g;
+                    if (c != 0.0f) rowerr[y] = std::max(rowerr[y], std::fabs((double)r - c) / std::fabs((double)c));
+                    else if (r != 0.0f) rowerr[y] = INFINITY;
+                    q[side ? 3 : 1] = m;
+                }
+                for (int k = 0; k < 4; ++k) memcpy(&t[k * 3 + n], &q[k], 4);
+            }
+        });
+        if (!ok) {
+            *n_entries = (uint32_t)tab.uniq.size();
+            return fail(-2, "more than %u unique factored tuples", max_entries);
+        }
+    }
+    *n_entries = (uint32_t)tab.uniq.size();
+    for (size_t r = 0; r < tab.uniq.size(); ++r)
+        for (int k = 0; k < 12; ++k) memcpy(&table[r * 12 + k], &tab.uniq[r][k], 4);
+    if (max_rel_err) *max_rel_err = *std::max_element(rowerr.begin(), rowerr.end());
+    return 0;
+}
+
 extern "C" {
 
 int wop_get_excitation_counts(wop_operator* op, uint32_t* vc, uint32_t* cc) {

@@ -11,7 +11,7 @@ from __future__ import annotations
 
 import ctypes
 import math
-from ctypes import POINTER, byref, c_double, c_int, c_uint32, c_void_p
+from ctypes import POINTER, byref, c_double, c_float, c_int, c_uint32, c_void_p
 from dataclasses import dataclass, field
 from typing import List, Optional
 
@@ -33,6 +33,24 @@ def _up(a):
 
 def _vp(a):
     return a.ctypes.data_as(c_void_p)
+
+
+def _fp(a):
+    return a.ctypes.data_as(POINTER(c_float))
+
+
+@dataclass
+class FactoredCoeffs:
+    """Coefficients as vi = m_E * ((fv[n][x] * fv[n][Nx+y]) * fv[n][Nx+Ny+z])
+    (float arithmetic, that order; iv likewise with fc and m_H): a per-cell
+    index into rows {vv0..2, m_E0..2, ii0..2, m_H0..2} plus per-line mesh
+    factors.  max_rel_err bounds |rebuilt - built| / |built| over all cells."""
+    index: np.ndarray              # uint32 [Nx][Ny][Nz]
+    table: np.ndarray              # float32 [n][12]
+    fv: np.ndarray                 # float32 [3][Nx+Ny+Nz]
+    fc: np.ndarray                 # float32 [3][Nx+Ny+Nz]
+    bits: int
+    max_rel_err: float
 
 
 @dataclass
@@ -192,6 +210,13 @@ class OperatorData:
     def compress(self, max_entries=0):
         """Per-cell coefficient dedup (Operator_SSE_Compressed::CompressOperator)."""
         return compress_coeffs(self, max_entries)
+
+    def factor(self, bits=16, max_entries=65536):
+        """Factored coefficients (FactoredCoeffs) from the Operator that built
+        this data; None if more than max_entries tuples."""
+        if self.source is None or getattr(self.source, "_built_prec", None) != self.prec or self.vv is None:
+            raise ValueError("factoring needs the full build of the Operator that made this data")
+        return self.source.factor(bits, max_entries)
 
     def expanded(self):
         """A copy with the full [3][Nx][Ny][Nz] coefficient arrays rebuilt from
@@ -510,6 +535,24 @@ class Operator:
         table = np.empty((nu.value, 12), dtype=np.float64)
         self._chk(self._lib.wop_compress(self._h, byref(nu), _up(index), _dp(table), 0))
         return index, table
+
+    def factor(self, bits=16, max_entries=65536):
+        """Factored coefficients of the last build (wop_factor) with m rounded
+        to `bits` significant bits; None if more than max_entries tuples."""
+        n = self.num_lines
+        L = int(n[0]) + int(n[1]) + int(n[2])
+        index = np.empty(n, dtype=np.uint32)
+        table = np.empty((int(max_entries), 12), dtype=np.float32)
+        fv = np.empty((3, L), dtype=np.float32)
+        fc = np.empty((3, L), dtype=np.float32)
+        nu = c_uint32()
+        err = c_double()
+        rc = self._lib.wop_factor(self._h, int(bits), int(max_entries), _up(index), _fp(table), byref(nu), _fp(fv), _fp(fc),
+                                  byref(err))
+        if rc == -2:
+            return None
+        self._chk(rc)
+        return FactoredCoeffs(index, table[:nu.value].copy(), fv, fc, int(bits), err.value)
 
     def get_ec(self):
         n = self.num_lines
