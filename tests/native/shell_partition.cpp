@@ -1,7 +1,7 @@
 // Host-only check of the split vector launches (fdtd_shell.h): for UPML slab
 // sets read from stdin, every cell the update bodies write is covered
 // exactly once by the interior launch plus the shell launch (the shell in
-// vectors of the interior's width or half of it), the interior launch
+// vectors of the interior's width, half or a quarter of it), the interior launch
 // touches no slab cell, and the shell enumeration stays inside the field
 // arrays.  Input: cases of "N0 N1 N2 count" followed by count lines
 // "s0 s1 s2 n0 n1 n2".  Prints one line per case and vector width; exit 1
@@ -92,9 +92,9 @@ int main() {
             slabs[s] = b;
         }
         for (int vz : {2, 4, 8})
-            for (int f : {1, 2})
+            for (int f : {1, 2, 4})
                 for (bool curr : {false, true})
-                    if (check(N, slabs, vz, curr, f)) return 1;
+                    if (f <= vz && check(N, slabs, vz, curr, f)) return 1;
     }
     return 0;
 }

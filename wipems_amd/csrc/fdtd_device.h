@@ -202,14 +202,24 @@ __device__ __forceinline__ void vstore(S* p, const typename StoreTraits<S>::A* i
 
 // UPML coefficients of one vector of slab cells (component n): from the
 // compressed rows when present, else from the full slab arrays
-template <typename A, int VZ>
+template <typename A, int VZ, bool ONE>
 __device__ __forceinline__ void pml_coeffs(const A* full0, const A* full1, const A* full2, long long q,
                                            const A* tab, const unsigned short* rows, int n, A k0[VZ], A kfo[VZ], A kfn[VZ]) {
     if (tab) {
+        bool same = ONE;
 #pragma unroll
-        for (int k = 0; k < VZ; ++k) {
-            const A* r = tab + (size_t)rows[k] * 12;
-            k0[k] = r[n]; kfo[k] = r[4 + n]; kfn[k] = r[8 + n];
+        for (int k = 1; k < VZ; ++k) same &= rows[k] == rows[0];
+        if (same) {          // one row for the whole vector: the common case
+            const A* r = tab + (size_t)rows[0] * 12;
+            const A a = r[n], b = r[4 + n], c = r[8 + n];
+#pragma unroll
+            for (int k = 0; k < VZ; ++k) { k0[k] = a; kfo[k] = b; kfn[k] = c; }
+        } else {
+#pragma unroll
+            for (int k = 0; k < VZ; ++k) {
+                const A* r = tab + (size_t)rows[k] * 12;
+                k0[k] = r[n]; kfo[k] = r[4 + n]; kfn[k] = r[8 + n];
+            }
         }
     } else {
         const SVec<A, VZ> a = ld_stream(reinterpret_cast<const SVec<A, VZ>*>(full0 + q));
@@ -223,13 +233,13 @@ __device__ __forceinline__ void pml_coeffs(const A* full0, const A* full1, const
 // UPML form of one component for the lanes pm of a vector in one slab
 // (Engine_Ext_UPML pre/post around the main update, fused): per lane
 // h = k*V - fo*F; F = F*a + b*curl; V = h + fn*F.  q = slab-array offset.
-template <typename A, int VZ>
+template <typename A, int VZ, bool ONE>
 __device__ __forceinline__ void upml_lanes(A* flux, const A* k0, const A* kfo, const A* kfn, const A* tab,
                                            const unsigned short* rows, long long q, int n, unsigned pm, const A* v,
                                            const A* a, const A* b, const A* curl, A* out) {
     A F[VZ], kv[VZ], fo[VZ], fn[VZ];
     vload_stream<A, VZ>(flux + q, F, A(1));
-    pml_coeffs<A, VZ>(k0, kfo, kfn, q, tab, rows, n, kv, fo, fn);
+    pml_coeffs<A, VZ, ONE>(k0, kfo, kfn, q, tab, rows, n, kv, fo, fn);
 #pragma unroll
     for (int k = 0; k < VZ; ++k) {
         if (pm & (1u << k)) {
@@ -265,19 +275,49 @@ __device__ __forceinline__ SlabHits slab_hits(const PmlSet& ps, int x, int y, in
     return h;
 }
 
-template <typename A, int CM, int VZ>
-__device__ __forceinline__ void vload_coeffs(const CoeffPtrs<A>& cf, long long p, long long sc, int x, int y, int z0,
-                                             A a[3][VZ], A b[3][VZ]) {
-    if (CM == CF_FACT16) {
-        const SVec<unsigned short, VZ> ti = ld_stream(reinterpret_cast<const SVec<unsigned short, VZ>*>((const unsigned short*)cf.idx + p));
+// Table reads of one row per vector where all its cells share it: for FP32
+// and FP64 storage (large dipole FP32 30.6 -> 29.8 ms per step, dispersive
+// 0.894 -> 0.879 ms); 2-byte storage measured 1-2% slower with the second
+// path (registers) and gathers per cell.
+template <typename S> struct OneRow { enum { value = sizeof(S) >= 4 }; };
+
+// rows {a0,a1,a2,_,b0,b1,b2,_} of the cells' table entries c[0..VZ-1]; with
+// ONE, a vector whose cells share one entry (runs of one material along z)
+// loads it once
+template <typename A, int VZ, bool ONE>
+__device__ __forceinline__ void table_rows(const A* tab, const unsigned c[VZ], A a[3][VZ], A b[3][VZ]) {
+    bool same = ONE;
+#pragma unroll
+    for (int k = 1; k < VZ; ++k) same &= c[k] == c[0];
+    if (same) {
+        const A* row = tab + (size_t)c[0] * 8;
+        const SVec<A, 4> r0 = *reinterpret_cast<const SVec<A, 4>*>(row);
+        const SVec<A, 4> r1 = *reinterpret_cast<const SVec<A, 4>*>(row + 4);
+#pragma unroll
+        for (int k = 0; k < VZ; ++k)
+#pragma unroll
+            for (int n = 0; n < 3; ++n) { a[n][k] = r0.v[n]; b[n][k] = r1.v[n]; }
+    } else {
 #pragma unroll
         for (int k = 0; k < VZ; ++k) {
-            const A* row = cf.a + (size_t)ti.v[k] * 8;
+            const A* row = tab + (size_t)c[k] * 8;
             const SVec<A, 4> r0 = *reinterpret_cast<const SVec<A, 4>*>(row);
             const SVec<A, 4> r1 = *reinterpret_cast<const SVec<A, 4>*>(row + 4);
 #pragma unroll
             for (int n = 0; n < 3; ++n) { a[n][k] = r0.v[n]; b[n][k] = r1.v[n]; }
         }
+    }
+}
+
+template <typename A, int CM, int VZ, bool ONE>
+__device__ __forceinline__ void vload_coeffs(const CoeffPtrs<A>& cf, long long p, long long sc, int x, int y, int z0,
+                                             A a[3][VZ], A b[3][VZ]) {
+    if (CM == CF_FACT16) {
+        const SVec<unsigned short, VZ> ti = ld_stream(reinterpret_cast<const SVec<unsigned short, VZ>*>((const unsigned short*)cf.idx + p));
+        unsigned c[VZ];
+#pragma unroll
+        for (int k = 0; k < VZ; ++k) c[k] = ti.v[k];
+        table_rows<A, VZ, ONE>(cf.a, c, a, b);
 #pragma unroll
         for (int n = 0; n < 3; ++n) {
             const A* g = cf.g + n * cf.gs;
@@ -305,14 +345,7 @@ __device__ __forceinline__ void vload_coeffs(const CoeffPtrs<A>& cf, long long p
 #pragma unroll
             for (int k = 0; k < VZ; ++k) c[k] = ti.v[k];
         }
-#pragma unroll
-        for (int k = 0; k < VZ; ++k) {
-            const A* row = cf.a + (size_t)c[k] * 8;
-            const SVec<A, 4> r0 = *reinterpret_cast<const SVec<A, 4>*>(row);
-            const SVec<A, 4> r1 = *reinterpret_cast<const SVec<A, 4>*>(row + 4);
-#pragma unroll
-            for (int n = 0; n < 3; ++n) { a[n][k] = r0.v[n]; b[n][k] = r1.v[n]; }
-        }
+        table_rows<A, VZ, ONE>(cf.a, c, a, b);
     }
 }
 
@@ -356,7 +389,7 @@ __device__ __forceinline__ void volt_apply(S* __restrict__ V, const CoeffPtrs<ty
                                            const typename StoreTraits<S>::A curl[3][VZ]) {
     typedef typename StoreTraits<S>::A A;
     A vv[3][VZ], vi[3][VZ];
-    vload_coeffs<A, CM, VZ>(cf, p, d.sc, x, y, z0, vv, vi);
+    vload_coeffs<A, CM, VZ, OneRow<S>::value>(cf, p, d.sc, x, y, z0, vv, vi);
     const int valid = min(VZ, d.nz - z0);
     SlabHits hit{{-1, -1}, {0u, 0u}, {0, 0}};
     SVec<unsigned short, VZ> prow[2] = {};
@@ -383,7 +416,7 @@ __device__ __forceinline__ void volt_apply(S* __restrict__ V, const CoeffPtrs<ty
             for (int j = 0; j < 2; ++j) {
                 if (hit.s[j] < 0) continue;
                 const PmlSlab& b = ps.slab[hit.s[j]];
-                upml_lanes<A, VZ>(pp.vflux, pp.vv, pp.vvfo, pp.vvfn, pp.vtab, prow[j].v, hit.b[j] + n * b.sc, n, hit.m[j], v,
+                upml_lanes<A, VZ, OneRow<S>::value>(pp.vflux, pp.vv, pp.vvfo, pp.vvfn, pp.vtab, prow[j].v, hit.b[j] + n * b.sc, n, hit.m[j], v,
                                   vv[n], vi[n], curl[n], out);
             }
         }
@@ -554,7 +587,7 @@ __device__ __forceinline__ void curr_apply(S* __restrict__ I, const CoeffPtrs<ty
                                            const typename StoreTraits<S>::A curl[3][VZ]) {
     typedef typename StoreTraits<S>::A A;
     A ii[3][VZ], iv[3][VZ];
-    vload_coeffs<A, CM, VZ>(cf, p, d.sc, x, y, z0, ii, iv);
+    vload_coeffs<A, CM, VZ, OneRow<S>::value>(cf, p, d.sc, x, y, z0, ii, iv);
     const int valid = min(VZ, d.nz - 1 - z0);
     // cells at z >= nz-1 are not updated: keep them out of the slab masks too
     const unsigned upd = valid < VZ ? (1u << valid) - 1u : ~0u;
@@ -583,7 +616,7 @@ __device__ __forceinline__ void curr_apply(S* __restrict__ I, const CoeffPtrs<ty
             for (int j = 0; j < 2; ++j) {
                 if (hit.s[j] < 0) continue;
                 const PmlSlab& b = ps.slab[hit.s[j]];
-                upml_lanes<A, VZ>(pp.cflux, pp.ii, pp.iifo, pp.iifn, pp.ctab, prow[j].v, hit.b[j] + n * b.sc, n, hit.m[j], c,
+                upml_lanes<A, VZ, OneRow<S>::value>(pp.cflux, pp.ii, pp.iifo, pp.iifn, pp.ctab, prow[j].v, hit.b[j] + n * b.sc, n, hit.m[j], c,
                                   ii[n], iv[n], curl[n], out);
             }
         }
