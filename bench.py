@@ -88,8 +88,11 @@ def kernel_bytes(eng, data):
 
 def read_traffic(kernel, workload, precision="fp32"):
     """HBM bytes per launch from the committed rocprofv3 PMC summary of this
-    kernel, workload and storage precision, if there is one."""
-    path = os.path.join(ROOT, "profiles", "%s_%s_pmc.md" % (kernel, workload))
+    kernel, workload and storage precision, if there is one
+    (profiles/<kernel>_<workload>_pmc.md for FP32 storage,
+    profiles/<kernel>_<workload>_<precision>_pmc.md for the others)."""
+    name = "%s_%s_pmc.md" % (kernel, workload) if precision == "fp32" else "%s_%s_%s_pmc.md" % (kernel, workload, precision)
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     fields = {}
@@ -112,20 +115,10 @@ def cpu_baseline_leg(data, interval, cells, threads, steps):
     log("cpu baseline: operator compressed to %d unique 4-lane tuples in %.1f s, %d threads" % (b.unique, time.time() - t0, b.nthreads))
     dt = b.run(steps, interval)
     val = cells * steps / dt / 1e6
+    hooks = " with its Lorentz/Drude ADE and TFSF hooks on thread 0" if (data.ade or data.tfsf) else ""
     return {"value": round(val, 2), "unit": "MCells/s", "cores": int(b.nthreads), "kind": "port",
-            "sample": "%d steps of the same workload (restated Engine_SSE_Compressed + Engine_Multithread, %d threads, %.1f s)" % (steps, b.nthreads, dt)}
-
-
-def scalar_baseline_leg(data, cells, steps):
-    from oracle.oracle import oracle_from_operator
-    o = oracle_from_operator(data, ftz=True)
-    o.iterate(1)
-    t0 = time.perf_counter()
-    o.iterate(steps)
-    dt = time.perf_counter() - t0
-    return {"value": round(cells * steps / dt / 1e6, 2), "unit": "MCells/s", "cores": 1, "kind": "port",
-            "sample": "%d steps of the same workload on the scalar restatement of the basic engine with every hook "
-                      "(one thread, %.1f s)" % (steps, dt)}
+            "sample": "%d steps of the same workload (restated Engine_SSE_Compressed + Engine_Multithread%s, %d threads, %.1f s)" % (
+                steps, hooks, b.nthreads, dt)}
 
 
 def run_batched(args, desc):
@@ -139,11 +132,22 @@ def run_batched(args, desc):
     datas = [sc.data for sc in scs]
     cells = sum(d.cells for d in datas)
     log("workload batched_64: %d domains, %d cells in total, operator builds %.1f s" % (len(datas), cells, time.time() - t0))
-    best, trials = tune_batch(datas, precision=args.precision)
-    log("autotuned batch block size: %d threads (ms/step %s)" % (best, {k: round(v, 4) for k, v in trials.items()}))
+    if args.batch_shape is not None:
+        best = tuple(args.batch_shape)
+        log("batch shape (threads per block, split) set: %s" % (best,))
+    else:
+        best, trials = tune_batch(datas, precision=args.precision)
+        log("autotuned batch shape (threads per block, split): %s (update ms/step %s)" % (best, {"%d/%d" % k: round(v, 4) for k, v in trials.items()}))
     engines = [engine_from_operator(d, precision=args.precision) for d in datas]
     b = Batch(engines)
-    b.set_block(best)
+    b.set_block(best[0])
+    b.set_split(best[1])
+    if args.profile_steps:
+        b.iterate(args.profile_steps)
+        b.synchronize()
+        gb = copy_bandwidth(1 << 30, 3)
+        log("profile mode: %d steps + 4 launches of the 1 GiB copy kernel (%.0f GB/s)" % (args.profile_steps, gb))
+        return
     b.iterate(args.warmup)
     b.synchronize()
     t1 = time.perf_counter()
@@ -208,7 +212,7 @@ def run_batched(args, desc):
         "warmup": args.warmup, "ms_per_step": round(1e3 * wall / steps, 5), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": {"fp32": "f32", "fp64": "f64", "fp16": "f16-store/f32", "bf16": "bf16-store/f32"}[args.precision],
         "data": "synthetic (seeded geometry)",
-        "config": {"workload": "batched_64", "description": desc, "cells": cells, "domains": len(datas), "block": best,
+        "config": {"workload": "batched_64", "description": desc, "cells": cells, "domains": len(datas), "block": best[0], "split": best[1],
                    "sequential_value": round(seq_value, 2), "batched_speedup": round(value / seq_value, 3), "precision": args.precision},
         "roofline": roofline, "cpu_baseline": cpu,
     }
@@ -244,12 +248,17 @@ def main():
     ap.add_argument("--split", type=int, default=None, help="UPML launch form (fdtd_set_pml_split): 0 or 1; default: the engine's")
     ap.add_argument("--pml-rows", type=int, default=None, help="UPML coefficient rows (fdtd_set_pml_compression): 0 or 1; default: the engine's")
     ap.add_argument("--fuse-hooks", type=int, default=None, help="fused sparse hooks (fdtd_set_fuse_hooks): 0 or 1; default: the engine's")
+    ap.add_argument("--shell-align", type=int, default=None, help="z alignment of the interior launch in vectors (fdtd_set_shell_align); default: the engine's")
+    ap.add_argument("--tile-order", type=int, nargs=2, default=None, metavar=("BAND_TILES", "XCD"),
+                    help="tile order of the update launches (fdtd_set_tile_order); default: the engine's")
     ap.add_argument("--cpu-steps", type=int, default=None, help="steps of the CPU-baseline sample (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=None)
     ap.add_argument("--no-timing-pass", action="store_true")
     ap.add_argument("--no-fp64-check", action="store_true", help="skip the FP64-engine error check")
     ap.add_argument("--check-lines", type=int, nargs=3, default=(193, 193, 97),
                     help="mesh lines of the smaller mesh the large dipole's FP64 check runs on")
+    ap.add_argument("--batch-shape", type=int, nargs=2, default=None, metavar=("THREADS", "SPLIT"),
+                    help="batched workload: threads per block and split instead of the autotuner's choice")
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="counter-collection mode: run this many steps of the workload, then a 1 GiB device copy "
                          "of known size (the calibration kernel for FETCH_SIZE/WRITE_SIZE), and exit")
@@ -290,8 +299,13 @@ def main():
         eng.set_pml_compression(args.pml_rows)
     if args.fuse_hooks is not None:
         eng.set_fuse_hooks(args.fuse_hooks)
-    log("device %s, coefficients %s, UPML split %s, device bytes %.2f GB" % (device_name(), "factored" if eng.factored else ("compressed" if eng.compressed else "full"),
-                                                                          eng.pml_split(), eng.device_bytes() / 1e9))
+    if args.tile_order is not None:
+        eng.set_tile_order(*args.tile_order)
+    if args.shell_align is not None:
+        eng.set_shell_align(args.shell_align)
+    log("device %s, coefficients %s, UPML split %s, tile order (band tiles, XCD) %s, device bytes %.2f GB" % (
+        device_name(), "factored" if eng.factored else ("compressed" if eng.compressed else "full"),
+        eng.pml_split(), eng.tile_order(), eng.device_bytes() / 1e9))
 
     if args.profile_steps:
         eng.iterate(args.profile_steps)
@@ -362,17 +376,13 @@ def main():
 
 def cpu_leg(args, data, sc, cells):
     """The CPU baseline on the same workload: the multithreaded restatement
-    on every core of this process's affinity set, or the scalar oracle on one
-    thread where the workload has hooks the restatement lacks."""
+    (with its thread-0 ADE and TFSF hooks where the workload has them) on
+    every core of this process's affinity set."""
     cpu_steps = args.cpu_steps
     if cpu_steps is None:
         cpu_steps = max(4, int(2e10 / cells))   # about 10-30 s of host work at ~1 GCells/s
     if cpu_steps <= 0:
         return None
-    if data.ade or data.tfsf:
-        out = scalar_baseline_leg(data, cells, max(3, int(5e8 / cells)))
-        log("cpu baseline (scalar): %.1f MCells/s" % out["value"])
-        return out
     threads = args.cpu_threads or host_cores()
     # fields of the 4-lane engine (V and I, 24 B per cell), its 4-lane index and the UPML slabs
     need = int(cells * 24 * 1.1 + cells + sum(48 * 1.1 * s.arrays[0].size for s in data.upml))

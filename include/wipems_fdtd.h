@@ -256,16 +256,19 @@ int  fdtd_td_num_samples(fdtd_engine* h, int id, uint64_t* n);
 int  fdtd_iterate(fdtd_engine* h, uint32_t nsteps);
 int  fdtd_synchronize(fdtd_engine* h);
 int  fdtd_get_num_timesteps(fdtd_engine* h, uint32_t* numTS);
-/* Kernel shape (default 3): variant 0 = one thread per cell in hardware block order,
- * 1 = x-marching columns (chunk_x planes per block), 2 = one thread per cell
- * in XCD-banded order (chunk_x = y tiles per band), 3 = 16-byte vectors of
- * consecutive z cells per lane, 4 = one fused launch per timestep (voltage
- * and current updates of a tile in an x-march over chunk_x planes per
- * block, ping-pong field buffers) where the domain has no UPML, Lorentz or
- * TFSF extension and at most 64 soft excitation edges per field, else 3;
- * tile_y = rows per 64-lane tile (1, 2 or 4) for variants 0-2.  Results do
- * not depend on the shape, only speed does. */
-int  fdtd_set_tuning(fdtd_engine* h, int variant, int tile_y, int chunk_x);
+/* Tile order of the main update launches.  A tile is 256 vectors of
+ * consecutive (y, z) cells of one plane.  band_tiles = 0: the launch sweeps
+ * whole planes; n > 0: the tiles of a plane are cut into y bands of n tiles
+ * and the launch sweeps band by band, each over all its planes, so the
+ * neighbour plane a tile reads again is still in L2 where a whole plane
+ * would not fit; -1 (default) = the engine's choice.  xcd = 1: each of the
+ * eight XCDs takes one contiguous sub-band of every band-plane, so the
+ * neighbour rows its blocks read again were loaded by the same XCD; 0 off;
+ * -1 the engine's choice.  Results do not depend on the order, only speed
+ * does.  fdtd_tile_order reports the order in use for the voltage update
+ * (band_tiles 0 = whole planes). */
+int  fdtd_set_tile_order(fdtd_engine* h, int band_tiles, int xcd);
+int  fdtd_tile_order(fdtd_engine* h, int* band_tiles, int* xcd);
 /* UPML coefficients: the vector kernels can read per-cell uint16 row
  * indices into a table of distinct coefficient tuples (possible when each
  * side has at most 65536 distinct tuples) instead of the full slab arrays.
@@ -287,6 +290,12 @@ int  fdtd_pml_compressed(fdtd_engine* h, int* active);
  * fdtd_pml_split reports whether the split launches are in use. */
 int  fdtd_set_pml_split(fdtd_engine* h, int mode);
 int  fdtd_pml_split(fdtd_engine* h, int* active);
+/* With the split: the interior launch's z range starts and ends at
+ * multiples of `vectors` 16-byte vectors (rounded inwards; the shell launch
+ * takes the cells in between), so the two launches share fewer partial
+ * cache lines at the row ends.  1, 2, 4 or 8; -1 (default) = the engine's
+ * choice.  Results are identical. */
+int  fdtd_set_shell_align(fdtd_engine* h, int vectors);
 /* Fused sparse hooks: the TFSF injection, the dispersive apply and the soft
  * excitation of a half step run as one launch that walks each touched field
  * location's hooks in the reference's order, and that launch also performs
@@ -349,6 +358,11 @@ int  fdtd_batch_create(fdtd_engine* const* engines, int count, fdtd_batch** out)
 int  fdtd_batch_destroy(fdtd_batch* b);
 /* threads per block of the batched update kernels: 64, 128 or 256 (tuning) */
 int  fdtd_batch_set_block(fdtd_batch* b, int threads);
+/* UPML launch form of the batch members: 1 / -1 (default) = every member
+ * whose UPML slabs form a shell runs its interior in a launch without UPML
+ * code and its shell in a shell launch shared by all such members; 0 = each
+ * member's whole domain in the UPML form.  Results are identical. */
+int  fdtd_batch_set_split(fdtd_batch* b, int mode);
 int  fdtd_batch_set_use_graph(fdtd_batch* b, int enable);
 int  fdtd_batch_iterate(fdtd_batch* b, uint32_t nsteps);
 int  fdtd_batch_synchronize(fdtd_batch* b);

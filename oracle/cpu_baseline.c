@@ -16,6 +16,10 @@
  *     (FDTD/engine_multithread.cpp:232-400, FDTD/extensions/engine_extension.cpp:50-90)
  *   - Engine_Ext_UPML split over threads along x (engine_ext_upml.cpp:46-55)
  *   - Engine_Ext_Excitation on thread 0 (engine_ext_excitation.cpp:33-94)
+ *   - Engine_Ext_LorentzMaterial (Drude/Lorentz ADE) and Engine_Ext_TFSF on
+ *     thread 0, as extensions without thread support run
+ *     (engine_ext_lorentzmaterial.cpp:79-168, engine_ext_dispersive.cpp:76-127,
+ *     engine_ext_tfsf.cpp:36-215)
  *   - FTZ|DAZ on every worker (tools/denormal.h:19-31)
  * Compiled -O3 -msse2 -ffp-contract=off.
  */
@@ -46,6 +50,22 @@ typedef struct {
     double weight;
 } probe_t;
 
+/* one ADE order: cells as float offsets into V/I (component 0; + c*3 rows) */
+typedef struct {
+    uint32_t count, flags;   /* 1 volt, 2 curr, 4 volt Lorentz, 8 curr Lorentz */
+    size_t* off;             /* [3][count] float offsets incl. component */
+    float* k[6];             /* v_int, v_ext, v_lor, i_int, i_ext, i_lor [3][count] */
+    float* s[4];             /* volt_ADE, volt_Lor, curr_ADE, curr_Lor [3][count] */
+} ade_t;
+
+/* TFSF injection list of one field */
+typedef struct {
+    uint32_t n, maxdelay;
+    size_t* off;             /* float offsets incl. component */
+    uint32_t* del;
+    float *delta, *amp;
+} tfsf_t;
+
 typedef struct cpub {
     uint32_t N[3], Nv;
     size_t nrow;          /* Nx*Ny*Nv */
@@ -55,8 +75,12 @@ typedef struct cpub {
     uint32_t U;
     /* excitation */
     uint32_t exc_len, exc_period, nv;
-    float *sig_v, *amp;
+    float *sig_v, *sig_i, *amp;
     uint32_t *pos, *dir, *del;
+    /* Lorentz/Drude ADE orders, TFSF lists (volt, curr) */
+    int n_ade;
+    ade_t ade[8];
+    tfsf_t tf[2];
     /* upml */
     int n_upml;
     upml_t upml[6];
@@ -137,7 +161,13 @@ void cpub_destroy(cpub* e) {
     if (!e) return;
     free(e->V); free(e->I); free(e->idx);
     for (int k = 0; k < 4; ++k) for (int n = 0; n < 3; ++n) free(e->tab[k][n]);
-    free(e->sig_v); free(e->amp); free(e->pos); free(e->dir); free(e->del);
+    free(e->sig_v); free(e->sig_i); free(e->amp); free(e->pos); free(e->dir); free(e->del);
+    for (int a = 0; a < e->n_ade; ++a) {
+        free(e->ade[a].off);
+        for (int k = 0; k < 6; ++k) free(e->ade[a].k[k]);
+        for (int k = 0; k < 4; ++k) free(e->ade[a].s[k]);
+    }
+    for (int w = 0; w < 2; ++w) { free(e->tf[w].off); free(e->tf[w].del); free(e->tf[w].delta); free(e->tf[w].amp); }
     for (int s = 0; s < e->n_upml; ++s) {
         for (int k = 0; k < 6; ++k) free(e->upml[s].c[k]);
         free(e->upml[s].vflux); free(e->upml[s].cflux);
@@ -353,6 +383,56 @@ int cpub_set_excitation(cpub* e, uint32_t length, uint32_t period, const double*
     return 0;
 }
 
+/* the current signal (the voltage TFSF hook reads it, engine_ext_tfsf.cpp:43-60) */
+int cpub_set_signal_curr(cpub* e, uint32_t length, const double* sig_i) {
+    free(e->sig_i);
+    e->sig_i = (float*)malloc((length ? length : 1) * 4);
+    for (uint32_t i = 0; i < length; ++i) e->sig_i[i] = (float)sig_i[i];
+    return 0;
+}
+
+static size_t float_off(const cpub* e, uint32_t n, uint32_t x, uint32_t y, uint32_t z) {
+    return (row(e, n, x, y) + z % e->Nv) * 4 + z / e->Nv;
+}
+
+/* one ADE order (Operator_Ext_LorentzMaterial tables): pos [3][count],
+ * k lists [3][count] each (NULL = zeros), flags as above */
+int cpub_add_ade(cpub* e, uint32_t count, const uint32_t* pos, const float* k0, const float* k1, const float* k2,
+                 const float* k3, const float* k4, const float* k5, uint32_t flags) {
+    if (e->n_ade >= 8) return -1;
+    ade_t* a = &e->ade[e->n_ade++];
+    const size_t c3 = 3 * (size_t)count;
+    a->count = count; a->flags = flags;
+    a->off = (size_t*)malloc((c3 ? c3 : 1) * sizeof(size_t));
+    for (uint32_t i = 0; i < count; ++i)
+        for (uint32_t c = 0; c < 3; ++c) a->off[c * (size_t)count + i] = float_off(e, c, pos[i], pos[count + i], pos[2 * (size_t)count + i]);
+    const float* src[6] = {k0, k1, k2, k3, k4, k5};
+    for (int k = 0; k < 6; ++k) {
+        a->k[k] = (float*)calloc(c3 ? c3 : 1, 4);
+        if (src[k]) memcpy(a->k[k], src[k], c3 * 4);
+    }
+    for (int k = 0; k < 4; ++k) a->s[k] = (float*)calloc(c3 ? c3 : 1, 4);
+    return 0;
+}
+
+int cpub_set_tfsf(cpub* e, int which, uint32_t count, const uint32_t* pos, const uint32_t* comp, const uint32_t* delay,
+                  const double* delta, const double* amp, uint32_t max_delay) {
+    if (which < 0 || which > 1) return -1;
+    tfsf_t* t = &e->tf[which];
+    t->n = count; t->maxdelay = max_delay;
+    t->off = (size_t*)malloc((count ? count : 1) * sizeof(size_t));
+    t->del = (uint32_t*)malloc((count ? count : 1) * 4);
+    t->delta = (float*)malloc((count ? count : 1) * 4);
+    t->amp = (float*)malloc((count ? count : 1) * 4);
+    for (uint32_t i = 0; i < count; ++i) {
+        t->off[i] = float_off(e, comp[i], pos[i], pos[count + i], pos[2 * (size_t)count + i]);
+        t->del[i] = delay[i];
+        t->delta[i] = (float)delta[i];
+        t->amp[i] = (float)amp[i];
+    }
+    return 0;
+}
+
 int cpub_add_upml(cpub* e, const uint32_t start[3], const uint32_t n[3], const float* vv, const float* vvfo, const float* vvfn,
                   const float* ii, const float* iifo, const float* iifn) {
     if (e->n_upml >= 6) return -1;
@@ -511,35 +591,118 @@ static void excite(cpub* e) {
     }
 }
 
+/* Engine_Ext_LorentzMaterial::DoPre{Voltage,Current}UpdatesImpl (engine_ext_lorentzmaterial.cpp:79-168) */
+static void ade_pre(cpub* e, int curr) {
+    float* F = (float*)(curr ? e->I : e->V);
+    for (int o = 0; o < e->n_ade; ++o) {
+        ade_t* a = &e->ade[o];
+        if (!(a->flags & (curr ? 2u : 1u))) continue;
+        const int lor = (a->flags & (curr ? 8u : 4u)) != 0;
+        const float *kint = a->k[curr ? 3 : 0], *kext = a->k[curr ? 4 : 1], *klor = a->k[curr ? 5 : 2];
+        float* J = a->s[curr ? 2 : 0];
+        float* JL = a->s[curr ? 3 : 1];
+        for (size_t i = 0; i < a->count; ++i)
+            for (int c = 0; c < 3; ++c) {
+                const size_t q = c * (size_t)a->count + i;
+                const float f = F[a->off[q]];
+                if (lor) {
+                    JL[q] += klor[q] * J[q];
+                    J[q] *= kint[q];
+                    J[q] += kext[q] * (f - JL[q]);
+                } else {
+                    J[q] *= kint[q];
+                    J[q] += kext[q] * f;
+                }
+            }
+    }
+}
+
+/* Engine_Ext_Dispersive::Apply2{Voltages,Current}Impl (engine_ext_dispersive.cpp:76-127) */
+static void ade_apply(cpub* e, int curr) {
+    float* F = (float*)(curr ? e->I : e->V);
+    for (int o = 0; o < e->n_ade; ++o) {
+        ade_t* a = &e->ade[o];
+        if (!(a->flags & (curr ? 2u : 1u))) continue;
+        const float* J = a->s[curr ? 2 : 0];
+        for (size_t i = 0; i < a->count; ++i)
+            for (int c = 0; c < 3; ++c) {
+                const size_t q = c * (size_t)a->count + i;
+                F[a->off[q]] = F[a->off[q]] - J[q];
+            }
+    }
+}
+
+/* Engine_Ext_TFSF::DoPost{Voltage,Current}Updates (engine_ext_tfsf.cpp:36-215):
+ * the voltage hook fills the delay lookup for n <= maxDelay, the current
+ * hook for n < maxDelay, and reads the other field's signal */
+static void tfsf(cpub* e, int curr) {
+    const tfsf_t* t = &e->tf[curr];
+    if (!t->n) return;
+    static uint32_t L[65536];
+    const uint32_t fill = curr ? t->maxdelay : t->maxdelay + 1;
+    const uint32_t numTS = e->numTS, len = e->exc_len;
+    const int per = (int)e->exc_period;
+    for (uint32_t n = 0; n < fill && n < 65536; ++n) {
+        if (numTS < n) L[n] = 0;
+        else if ((numTS - n >= len) && (per == 0)) L[n] = 0;
+        else L[n] = numTS - n;
+        if (per > 0) L[n] = L[n] % (uint32_t)per;
+    }
+    const float* sig = curr ? e->sig_v : e->sig_i;
+    float* F = (float*)(curr ? e->I : e->V);
+    for (uint32_t i = 0; i < t->n; ++i) {
+        const uint32_t d = t->del[i];
+        const float dd = t->delta[i], a = t->amp[i];
+        F[t->off[i]] = (float)((double)F[t->off[i]] + (1.0 - dd) * a * sig[L[d]] + dd * a * sig[L[d + 1]]);
+    }
+}
+
 typedef struct { cpub* e; int tid; } targ_t;
 
-/* NS_Engine_Multithread::thread::operator() loop body (FDTD/engine_multithread.cpp:308-400);
- * the extension list sorted by priority is [UPML x n_upml, Excitation] */
+/* NS_Engine_Multithread::thread::operator() loop body (FDTD/engine_multithread.cpp:308-400)
+ * with the extension calls of :232-291, a barrier after each.  The extension
+ * list sorted by priority is [UPML x n_upml, TFSF, Lorentz, Excitation]
+ * (TFSF and Lorentz only when present); pre hooks run it in reverse. */
 static void* worker(void* arg) {
     targ_t* a = (targ_t*)arg;
     cpub* e = a->e;
     const int tid = a->tid;
     _mm_setcsr(_mm_getcsr() | 0x8040);
     const uint32_t start = e->tstart[tid], stop = e->tstop[tid], stop_h = e->tstop_h[tid];
+    const int has_tf = e->tf[0].n || e->tf[1].n, has_ade = e->n_ade > 0;
     for (uint32_t it = 0; it < e->iterTS; ++it) {
-        /* DoPreVoltageUpdates: reverse order -> Excitation (no-op), then UPMLs */
+        /* DoPreVoltageUpdates: Excitation (no-op), Lorentz, TFSF (no-op), UPMLs */
         pthread_barrier_wait(&e->bar);
+        if (has_ade) { if (tid == 0) ade_pre(e, 0); pthread_barrier_wait(&e->bar); }
+        if (has_tf) pthread_barrier_wait(&e->bar);
         for (int s = e->n_upml - 1; s >= 0; --s) { upml_pre(e, &e->upml[s], tid, 0); pthread_barrier_wait(&e->bar); }
         update_voltages(e, start, stop - start + 1);
         pthread_barrier_wait(&e->bar);
+        /* DoPostVoltageUpdates: UPMLs, TFSF, Lorentz (no-op), Excitation (no-op) */
         for (int s = 0; s < e->n_upml; ++s) { upml_post(e, &e->upml[s], tid, 0); pthread_barrier_wait(&e->bar); }
-        pthread_barrier_wait(&e->bar);                 /* Excitation post (no-op) */
-        for (int s = 0; s < e->n_upml; ++s) pthread_barrier_wait(&e->bar);   /* UPML Apply2 (no-op) */
+        if (has_tf) { if (tid == 0) tfsf(e, 0); pthread_barrier_wait(&e->bar); }
+        if (has_ade) pthread_barrier_wait(&e->bar);
+        pthread_barrier_wait(&e->bar);
+        /* Apply2Voltages: UPMLs (no-op), TFSF (no-op), Lorentz, Excitation */
+        for (int s = 0; s < e->n_upml; ++s) pthread_barrier_wait(&e->bar);
+        if (has_tf) pthread_barrier_wait(&e->bar);
+        if (has_ade) { if (tid == 0) ade_apply(e, 0); pthread_barrier_wait(&e->bar); }
         if (tid == 0) excite(e);
         pthread_barrier_wait(&e->bar);
-        /* current half step */
-        pthread_barrier_wait(&e->bar);                 /* Excitation pre (no-op) */
+        /* current half step, same structure */
+        pthread_barrier_wait(&e->bar);
+        if (has_ade) { if (tid == 0) ade_pre(e, 1); pthread_barrier_wait(&e->bar); }
+        if (has_tf) pthread_barrier_wait(&e->bar);
         for (int s = e->n_upml - 1; s >= 0; --s) { upml_pre(e, &e->upml[s], tid, 1); pthread_barrier_wait(&e->bar); }
         if (stop_h + 1 > start) update_currents(e, start, stop_h - start + 1);
         pthread_barrier_wait(&e->bar);
         for (int s = 0; s < e->n_upml; ++s) { upml_post(e, &e->upml[s], tid, 1); pthread_barrier_wait(&e->bar); }
+        if (has_tf) { if (tid == 0) tfsf(e, 1); pthread_barrier_wait(&e->bar); }
+        if (has_ade) pthread_barrier_wait(&e->bar);
         pthread_barrier_wait(&e->bar);
         for (int s = 0; s < e->n_upml; ++s) pthread_barrier_wait(&e->bar);
+        if (has_tf) pthread_barrier_wait(&e->bar);
+        if (has_ade) { if (tid == 0) ade_apply(e, 1); pthread_barrier_wait(&e->bar); }
         pthread_barrier_wait(&e->bar);                 /* Excitation Apply2Current (no current sources) */
         if (tid == 0) ++e->numTS;
     }

@@ -30,6 +30,10 @@ def lib():
         L.cpub_set_excitation.argtypes = [c_void_p, c_uint32, c_uint32, POINTER(c_double), c_uint32, POINTER(c_uint32),
                                           POINTER(c_uint32), POINTER(c_double), POINTER(c_uint32)]
         L.cpub_add_upml.argtypes = [c_void_p, POINTER(c_uint32), POINTER(c_uint32)] + [POINTER(ctypes.c_float)] * 6
+        L.cpub_set_signal_curr.argtypes = [c_void_p, c_uint32, POINTER(c_double)]
+        L.cpub_add_ade.argtypes = [c_void_p, c_uint32, POINTER(c_uint32)] + [POINTER(ctypes.c_float)] * 6 + [c_uint32]
+        L.cpub_set_tfsf.argtypes = [c_void_p, c_int, c_uint32, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_uint32),
+                                    POINTER(c_double), POINTER(c_double), c_uint32]
         L.cpub_add_probe.argtypes = [c_void_p, c_int, c_uint32, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_int), c_double]
         L.cpub_run.restype = c_double
         L.cpub_run.argtypes = [c_void_p, c_uint32, c_uint32]
@@ -111,8 +115,12 @@ class CpuBaseline:
             arrs = [np.ascontiguousarray(a, dtype=np.float32) for a in (data.vv, data.vi, data.ii, data.iv)]
             self.unique = self._L.cpub_set_coeffs(self._h, *[_fp(a) for a in arrs])
         e = data.excitation
+        if e is not None and len(getattr(e, "curr_dir", ())):
+            raise ValueError("the CPU baseline restates voltage sources only")
         if e is not None and e.length:
             sv = np.ascontiguousarray(e.signal_volt, dtype=np.float64)
+            si = np.ascontiguousarray(e.signal_curr, dtype=np.float64)
+            self._L.cpub_set_signal_curr(self._h, e.length, _dp(si))
             pos = np.ascontiguousarray(e.volt_pos, dtype=np.uint32)
             self._L.cpub_set_excitation(self._h, e.length, e.period, _dp(sv), len(e.volt_dir), _up(pos),
                                         _up(np.ascontiguousarray(e.volt_dir, np.uint32)), _dp(np.ascontiguousarray(e.volt_amp, np.float64)),
@@ -122,8 +130,26 @@ class CpuBaseline:
             nl = (c_uint32 * 3)(*[int(x) for x in s.num_lines])
             ua = [np.ascontiguousarray(a, dtype=np.float32) for a in s.arrays]
             self._L.cpub_add_upml(self._h, st, nl, *[_fp(a) for a in ua])
-        if data.ade or data.tfsf:
-            raise ValueError("the CPU baseline restates the main updates, UPML and excitation only")
+        if getattr(data, "mur", None) or getattr(data, "rlc", None) is not None or getattr(data, "steady_state", None) is not None:
+            raise ValueError("the CPU baseline restates the main updates, UPML, ADE, TFSF and voltage excitation only")
+        self._keep = []
+        for a in data.ade:
+            pos = np.ascontiguousarray(a.pos, dtype=np.uint32)
+            lists = [None if x is None else np.ascontiguousarray(x, dtype=np.float32) for x in a.lists]
+            self._keep += [pos] + lists
+            ptrs = [None if x is None else _fp(x) for x in lists]
+            if self._L.cpub_add_ade(self._h, pos.shape[1], _up(pos), *ptrs, int(a.flags)):
+                raise ValueError("too many ADE orders")
+        if data.tfsf:
+            for which in (0, 1):
+                t = data.tfsf[which]
+                pos = np.ascontiguousarray(t["pos"], dtype=np.uint32)
+                comp = np.ascontiguousarray(t["comp"], dtype=np.uint32)
+                delay = np.ascontiguousarray(t["delay"], dtype=np.uint32)
+                delta = np.ascontiguousarray(t["delta"], dtype=np.float64)
+                amp = np.ascontiguousarray(t["amp"], dtype=np.float64)
+                self._L.cpub_set_tfsf(self._h, which, len(comp), _up(pos), _up(comp), _up(delay), _dp(delta), _dp(amp),
+                                      int(data.tfsf["max_delay"]))
         self.probes = data.probes
         for p in data.probes:
             comp, pos, sign = probe_edges(p, data.num_lines)

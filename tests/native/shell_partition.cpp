@@ -1,7 +1,8 @@
 // Host-only check of the split vector launches (fdtd_shell.h): for UPML slab
 // sets read from stdin, every cell the update bodies write is covered
 // exactly once by the interior launch plus the shell launch (the shell in
-// vectors of the interior's width, half or a quarter of it), the interior launch
+// vectors of the interior's width, half or a quarter of it; the interior's z
+// range aligned to 1, 4 or 8 vectors), the interior launch
 // touches no slab cell, and the shell enumeration stays inside the field
 // arrays.  Input: cases of "N0 N1 N2 count" followed by count lines
 // "s0 s1 s2 n0 n1 n2".  Prints one line per case and vector width; exit 1
@@ -11,11 +12,11 @@
 #include <vector>
 #include "fdtd_shell.h"
 
-static int check(const uint32_t N[3], const std::vector<PmlSlab>& slabs, int vz, bool curr, int f) {
+static int check(const uint32_t N[3], const std::vector<PmlSlab>& slabs, int vz, bool curr, int f, int za) {
     int ibox[3][2] = {{0, 0}, {0, 0}, {0, 0}};
     const bool ok = find_interior(N, slabs.data(), (int)slabs.size(), ibox);
     const long long sy = ((long long)N[2] + 15) / 16 * 16;
-    const VecLaunch L = plan_vec_launch(N, sy, (int)slabs.size(), ok, ibox, vz, curr);
+    const VecLaunch L = plan_vec_launch(N, sy, (int)slabs.size(), ok, ibox, vz, curr, za);
     const ShellGrid G = f == 1 ? L.g : shell_grid_split(L, (int)N[2], vz, f);
     const int w = vz / f;
     const int nx = (int)N[0], ny = (int)N[1], nz = (int)N[2];
@@ -76,8 +77,8 @@ static int check(const uint32_t N[3], const std::vector<PmlSlab>& slabs, int vz,
                     return 1;
                 }
             }
-    printf("N %u %u %u slabs %zu shell %d vz %d/%d %s: interior %lld shell %lld\n", N[0], N[1], N[2], slabs.size(), (int)ok, vz, w,
-           curr ? "curr" : "volt", main_vecs, shell_vecs);
+    printf("N %u %u %u slabs %zu shell %d vz %d/%d za %d %s: interior %lld shell %lld\n", N[0], N[1], N[2], slabs.size(), (int)ok, vz, w,
+           za, curr ? "curr" : "volt", main_vecs, shell_vecs);
     return 0;
 }
 
@@ -93,8 +94,9 @@ int main() {
         }
         for (int vz : {2, 4, 8})
             for (int f : {1, 2, 4})
-                for (bool curr : {false, true})
-                    if (f <= vz && check(N, slabs, vz, curr, f)) return 1;
+                for (int za : {1, 4, 8})
+                    for (bool curr : {false, true})
+                        if (f <= vz && check(N, slabs, vz, curr, f, za)) return 1;
     }
     return 0;
 }

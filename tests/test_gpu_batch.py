@@ -14,8 +14,10 @@ def small_batch(prec=0):
     return scenarios.batched_port_scenarios(count=5, lo=22, hi=34, steps=200, prec=prec, pml=6, n_boxes=3)
 
 
-@pytest.mark.parametrize("block", [64, 256])
-def test_batch_members_match_solo_runs(gpu, block):
+@pytest.mark.parametrize("block,split", [(64, 1), (256, 1), (128, 0), (256, 0)])
+def test_batch_members_match_solo_runs(gpu, block, split):
+    """Interior/shell launches per member (split 1) and one UPML-form launch
+    (split 0) give each member the bits of its solo run."""
     scs = small_batch()
     solo = [engine_from_operator(s.data, precision="fp32") for s in scs]
     for e in solo:
@@ -24,6 +26,7 @@ def test_batch_members_match_solo_runs(gpu, block):
     members = [engine_from_operator(s.data, precision="fp32") for s in scs]
     b = Batch(members)
     b.set_block(block)
+    b.set_split(split)
     b.iterate(7)
     b.iterate(193)
     b.synchronize()
@@ -66,3 +69,42 @@ def test_batch_rejects_mixed_precision(gpu):
     from wipems_amd import FdtdError
     with pytest.raises(FdtdError):
         Batch([a, c])
+
+
+@pytest.mark.parametrize("precision", ["fp16", "bf16"])
+def test_batch_reduced_precision_members_match_solo_runs(gpu, precision):
+    """2-byte storage: split members (half-width shell vectors) and unsplit
+    members give the bits of the solo runs."""
+    from wipems_amd.engine import pilot_scales
+    scs = small_batch()[:3]
+    scales = [pilot_scales(s.data, steps=64) for s in scs]
+    solo = [engine_from_operator(s.data, precision=precision, scales=sc) for s, sc in zip(scs, scales)]
+    for e in solo:
+        e.iterate(150)
+        e.synchronize()
+    for split in (1, 0):
+        members = [engine_from_operator(s.data, precision=precision, scales=sc) for s, sc in zip(scs, scales)]
+        b = Batch(members)
+        b.set_split(split)
+        b.iterate(150)
+        b.synchronize()
+        for e, m in zip(solo, members):
+            for w in (0, 1):
+                np.testing.assert_array_equal(m.get_field(w), e.get_field(w))
+        b.close()
+
+
+def test_batch_mixed_coefficient_modes(gpu):
+    """Members with compressed and with full coefficient arrays in one batch
+    (per-domain dispatch) match the oracle."""
+    scs = small_batch()[:3]
+    members = [engine_from_operator(s.data, precision="fp32", compressed=(k != 1)) for k, s in enumerate(scs)]
+    assert members[0].compressed and not members[1].compressed
+    b = Batch(members)
+    b.iterate(120)
+    b.synchronize()
+    for k in (0, 1):
+        o = oracle_from_operator(scs[k].data, ftz=True)
+        o.iterate(120)
+        for w in (0, 1):
+            np.testing.assert_array_equal(members[k].get_field(w), o.get_field(w))

@@ -516,3 +516,90 @@ def test_fused_hooks_equal_separate_launches(gpu, precision, prec, case):
     for a, b in zip(out[0], out[1]):
         np.testing.assert_array_equal(a, b)
     assert np.abs(out[0][0]).max() > 0
+
+
+@pytest.mark.parametrize("precision,compressed,split,band,xcd", [
+    ("fp32", False, 0, 0, 1), ("fp32", True, 1, 0, 1), ("fp32", True, 0, 3, 1), ("fp32", False, 0, 5, 0),
+    ("fp32", True, 1, 2, 0), ("fp64", False, 0, 3, 1),
+])
+def test_tile_orders_bitwise(gpu, precision, compressed, split, band, xcd):
+    """Banded tile orders (y bands of `band` tiles swept plane by plane, the
+    last band partial; with xcd, tile (k % 8) * (bpp / 8) + k / 8 of each
+    band-plane, padding blocks idle) give the oracle's bits, with one UPML
+    launch and with the interior/shell split, for the ascending voltage and
+    the descending current sweep; 64 cells per axis give planes of 6 tiles
+    (FP32; 11 for FP64), no multiple of 8 or of the band."""
+    sc = scenarios.graded_port_scenario(n_cells=64, steps=160, prec=1 if precision == "fp64" else 0, n_boxes=5, pml=6)
+    d = sc.data
+    eng = engine_from_operator(d, precision=precision, compressed=compressed)
+    eng.set_tile_order(band, xcd)
+    eng.set_pml_split(split)
+    got_band, got_xcd = eng.tile_order()
+    assert (got_band, got_xcd) == (band, bool(xcd))
+    _kinds[id(eng)] = [p.kind for p in d.probes]
+    orc = oracle_from_operator(d, ftz=True)
+    eng.iterate(160)
+    orc.iterate(160)
+    compare(eng, orc)
+
+
+@pytest.mark.parametrize("precision", ["fp16", "bf16"])
+def test_tile_orders_reduced_precision(gpu, precision):
+    """2-byte storage: banded and whole-plane tile orders give identical fields
+    (planes of 3 tiles in the one-launch form)."""
+    sc = scenarios.graded_port_scenario(n_cells=64, steps=120, prec=0, n_boxes=5, pml=6)
+    d = sc.data
+    from wipems_amd.engine import pilot_scales
+    scales = pilot_scales(d, steps=64)
+    out = []
+    for band, xcd in ((0, 0), (2, 1), (1, 0)):
+        eng = engine_from_operator(d, precision=precision, scales=scales)
+        eng.set_pml_split(0)
+        eng.set_tile_order(band, xcd)
+        eng.iterate(120)
+        out.append((eng.get_field(0), eng.get_field(1)))
+        eng.close()
+    for k in (1, 2):
+        for which in (0, 1):
+            np.testing.assert_array_equal(out[0][which], out[k][which])
+
+
+@pytest.mark.parametrize("precision,compressed,align", [
+    ("fp32", True, 4), ("fp32", True, 8), ("fp32", False, 8), ("fp64", False, 4),
+])
+def test_shell_align_bitwise(gpu, precision, compressed, align):
+    """The interior/shell split with the interior's z range aligned to 4 or 8
+    vectors (the shell launch taking the non-slab cells between) gives the
+    oracle's bits; 64 cells per axis with 6-cell UPML."""
+    sc = scenarios.graded_port_scenario(n_cells=64, steps=140, prec=1 if precision == "fp64" else 0, n_boxes=5, pml=6)
+    d = sc.data
+    eng = engine_from_operator(d, precision=precision, compressed=compressed)
+    eng.set_pml_split(1)
+    eng.set_shell_align(align)
+    assert eng.pml_split()
+    _kinds[id(eng)] = [p.kind for p in d.probes]
+    orc = oracle_from_operator(d, ftz=True)
+    eng.iterate(140)
+    orc.iterate(140)
+    compare(eng, orc)
+
+
+@pytest.mark.parametrize("precision", ["fp16", "bf16"])
+def test_shell_align_reduced_precision(gpu, precision):
+    """2-byte storage (half-width shell vectors): aligned and unaligned
+    interior ranges give identical fields."""
+    from wipems_amd.engine import pilot_scales
+    sc = scenarios.graded_port_scenario(n_cells=64, steps=120, prec=0, n_boxes=5, pml=6)
+    d = sc.data
+    scales = pilot_scales(d, steps=64)
+    out = []
+    for align in (1, 4, 8):
+        eng = engine_from_operator(d, precision=precision, scales=scales)
+        eng.set_pml_split(1)
+        eng.set_shell_align(align)
+        eng.iterate(120)
+        out.append((eng.get_field(0), eng.get_field(1)))
+        eng.close()
+    for k in (1, 2):
+        for which in (0, 1):
+            np.testing.assert_array_equal(out[0][which], out[k][which])

@@ -350,3 +350,24 @@ def test_oracle_equals_cpu_baseline_streamed_dipole():
         np.testing.assert_array_equal(o.get_field(w), b.get_field(w))
     np.testing.assert_array_equal(o.read_probe(0)[1], b.samples(0))
     assert np.abs(o.get_field(0)).max() > 0
+
+
+def test_oracle_equals_cpu_baseline_dispersive_tfsf():
+    """The CPU baseline with its thread-0 Lorentz/Drude ADE and TFSF hooks
+    (4-lane layout, x-slab threads, the reference's extension order and
+    barriers) steps a Drude slab + Lorentz spheres lit by a plane wave, with
+    UPML, bit for bit like the oracle; z = 29 lines is no multiple of 4."""
+    from oracle.cpu_baseline import CpuBaseline
+    sc = scenarios.dispersive_scenario(num_lines=(33, 31, 29), steps=240, n_spheres=4, pml=4,
+                                       material_scale=1e-3, ts_factor=0.9)
+    d = sc.data
+    assert d.ade and d.tfsf and d.tfsf[0]["pos"].size and d.tfsf[1]["pos"].size
+    o = oracle_from_operator(d, ftz=True)
+    o.iterate(240)
+    b = CpuBaseline(d, 3)
+    b.run(240, sc.meta["interval"])
+    for w in (0, 1):
+        np.testing.assert_array_equal(o.get_field(w), b.get_field(w))
+    for pid in range(len(d.probes)):
+        np.testing.assert_array_equal(o.read_probe(pid)[1], b.samples(pid))
+    assert np.abs(o.get_field(0)).max() > 0

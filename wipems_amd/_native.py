@@ -80,7 +80,9 @@ _ENGINE_SIGS = {
     "fdtd_get_steady_state_period": [c_void_p, POINTER(c_uint32)],
     "fdtd_steady_state_last_diff": [c_void_p, POINTER(c_double)],
     "fdtd_pml_split": [c_void_p, POINTER(c_int)],
-    "fdtd_set_tuning": [c_void_p, c_int, c_int, c_int],
+    "fdtd_set_tile_order": [c_void_p, c_int, c_int],
+    "fdtd_set_shell_align": [c_void_p, c_int],
+    "fdtd_tile_order": [c_void_p, POINTER(c_int), POINTER(c_int)],
     "fdtd_set_kernel_timing": [c_void_p, c_int],
     "fdtd_get_kernel_times": [c_void_p, POINTER(c_double), POINTER(c_double), POINTER(c_uint32), POINTER(c_uint32)],
     "fdtd_calc_energy": [c_void_p, POINTER(c_double)],
@@ -95,6 +97,7 @@ _ENGINE_SIGS = {
     "fdtd_batch_create": [POINTER(c_void_p), c_int, POINTER(c_void_p)],
     "fdtd_batch_destroy": [c_void_p],
     "fdtd_batch_set_block": [c_void_p, c_int],
+    "fdtd_batch_set_split": [c_void_p, c_int],
     "fdtd_batch_set_use_graph": [c_void_p, c_int],
     "fdtd_batch_iterate": [c_void_p, c_uint32],
     "fdtd_batch_synchronize": [c_void_p],

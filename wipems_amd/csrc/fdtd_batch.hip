@@ -2,20 +2,23 @@
 //
 // A batch groups engine handles that were configured through the ordinary
 // single-domain calls (operator, excitation, UPML, probes).  Each timestep is
-// three launches for the whole batch instead of three per domain:
+// a fixed short sequence of launches for the whole batch instead of one
+// sequence per domain:
 //
-//   k_volt_batch   every domain's voltage update (UPML fused), then
-//   k_excite_batch the soft sources of every domain,
-//   k_curr_batch   every domain's current update, then
-//   k_step_end_batch  ++numTS and the probes of every domain (one block each).
+//   k_volt_batch_main   every domain's voltage update over its interior box
+//                       (no UPML code), or over the whole domain in the
+//                       UPML form where the domain is not split,
+//   k_volt_batch_shell  the UPML shells of the split domains,
+//   k_excite_batch      the soft sources of every domain,
+//   k_curr_batch_main / k_curr_batch_shell, the same for the currents,
+//   k_step_end_batch    ++numTS and the probes of every domain (one block each).
 //
-// The update grids are the concatenation of each domain's flat-vector grid
-// (x planes x flat chunks); a block finds its domain by binary search over
-// the domains' first block index, so uneven grid sizes pack without padding
-// and large domains get proportionally more blocks.  The step sequence is
-// captured once in a HIP graph and replayed.  Per-cell arithmetic is the same
-// device code as the single-domain vector kernels, so every domain of a batch
-// gives the same bits as running it alone.
+// Each update grid is the concatenation of the domains' grids; a per-block
+// table gives the domain of every block, so uneven grid sizes pack without
+// padding and large domains get proportionally more blocks.  The step
+// sequence is captured once in a HIP graph and replayed.  Per-cell
+// arithmetic is the same device code as the single-domain vector kernels,
+// so every domain of a batch gives the same bits as running it alone.
 #include "fdtd_internal.h"
 
 using namespace wipems;
@@ -26,15 +29,18 @@ template <typename A>
 struct SimDesc {
     Dims d;
     int cm;                 // CF_* coefficient mode
-    int pml;                // 1 if the domain has UPML slabs
+    int main_pml;           // the main launch runs the UPML form (UPML, not split)
     void* V;
     void* I;
     CoeffPtrs<A> cfe, cfh;
     Scales<A> sc;
     PmlSet ps;
     PmlPtrs<A> pp;
-    long long vstart, cstart;   // first block of this domain in the volt / curr grids
-    long long vbpx, cbpx;       // blocks per x plane
+    VecRange rv, rc;            // main-launch ranges (interior box when split)
+    long long vbpp, cbpp;       // main-launch blocks per plane
+    ShellGrid gv, gc;           // UPML shells (split domains)
+    long long vstart, cstart;   // first block of this domain in the main volt / curr grids
+    long long vsstart, csstart; // ... and in the shell grids
     unsigned* step;
     const ProbeDev* probes;
     int nprobes;
@@ -56,61 +62,90 @@ struct ExcEntry {
     const A* sig;
 };
 
-template <typename A>
-__device__ __forceinline__ int find_sim(const SimDesc<A>* sims, int nsims, long long b, bool curr) {
-    int lo = 0, hi = nsims - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        const long long st = curr ? sims[mid].cstart : sims[mid].vstart;
-        if (st <= b) lo = mid; else hi = mid - 1;
-    }
-    return lo;
-}
-
-template <typename S>
-__global__ void __launch_bounds__(256)
-k_volt_batch(const SimDesc<typename StoreTraits<S>::A>* __restrict__ sims, int nsims) {
-    typedef typename StoreTraits<S>::A A;
-    constexpr int VZ = VecWidth<S>::value;
-    const long long b = blockIdx.x;
-    const SimDesc<A>& D = sims[find_sim(sims, nsims, b, false)];
-    const long long local = b - D.vstart;
-    const int x = (int)(local / D.vbpx);
-    const long long q0 = ((local - (long long)x * D.vbpx) * blockDim.x + threadIdx.x) * VZ;
+// CM >= 0: every domain of the launch has that coefficient mode; CM < 0:
+// per-domain dispatch on D.cm
+template <typename S, int CM, bool PML, int VZ>
+__device__ __forceinline__ void batch_volt_body(const SimDesc<typename StoreTraits<S>::A>& D, int x, long long q0) {
     S* V = (S*)D.V;
     const S* I = (const S*)D.I;
-    if (D.pml) {
-        if (D.cm == CF_FULL) volt_vec_body<S, CF_FULL, true, VZ>(V, I, D.cfe, D.d, D.sc, D.ps, D.pp, x, q0);
-        else if (D.cm == CF_IDX16) volt_vec_body<S, CF_IDX16, true, VZ>(V, I, D.cfe, D.d, D.sc, D.ps, D.pp, x, q0);
-        else volt_vec_body<S, CF_IDX32, true, VZ>(V, I, D.cfe, D.d, D.sc, D.ps, D.pp, x, q0);
-    } else {
-        if (D.cm == CF_FULL) volt_vec_body<S, CF_FULL, false, VZ>(V, I, D.cfe, D.d, D.sc, D.ps, D.pp, x, q0);
-        else if (D.cm == CF_IDX16) volt_vec_body<S, CF_IDX16, false, VZ>(V, I, D.cfe, D.d, D.sc, D.ps, D.pp, x, q0);
-        else volt_vec_body<S, CF_IDX32, false, VZ>(V, I, D.cfe, D.d, D.sc, D.ps, D.pp, x, q0);
-    }
+    if constexpr (CM >= 0) { volt_vec_body<S, CM, PML, VZ>(V, I, D.cfe, D.d, D.sc, D.ps, D.pp, x, q0); return; }
+    if (D.cm == CF_FULL) volt_vec_body<S, CF_FULL, PML, VZ>(V, I, D.cfe, D.d, D.sc, D.ps, D.pp, x, q0);
+    else if (D.cm == CF_IDX16) volt_vec_body<S, CF_IDX16, PML, VZ>(V, I, D.cfe, D.d, D.sc, D.ps, D.pp, x, q0);
+    else volt_vec_body<S, CF_IDX32, PML, VZ>(V, I, D.cfe, D.d, D.sc, D.ps, D.pp, x, q0);
 }
 
-template <typename S>
-__global__ void __launch_bounds__(256)
-k_curr_batch(const SimDesc<typename StoreTraits<S>::A>* __restrict__ sims, int nsims) {
-    typedef typename StoreTraits<S>::A A;
-    constexpr int VZ = VecWidth<S>::value;
-    const long long b = blockIdx.x;
-    const SimDesc<A>& D = sims[find_sim(sims, nsims, b, true)];
-    const long long local = b - D.cstart;
-    const int x = (int)(local / D.cbpx);
-    const long long q0 = ((local - (long long)x * D.cbpx) * blockDim.x + threadIdx.x) * VZ;
+template <typename S, int CM, bool PML, int VZ>
+__device__ __forceinline__ void batch_curr_body(const SimDesc<typename StoreTraits<S>::A>& D, int x, long long q0) {
     S* I = (S*)D.I;
     const S* V = (const S*)D.V;
-    if (D.pml) {
-        if (D.cm == CF_FULL) curr_vec_body<S, CF_FULL, true, VZ>(I, V, D.cfh, D.d, D.sc, D.ps, D.pp, x, q0);
-        else if (D.cm == CF_IDX16) curr_vec_body<S, CF_IDX16, true, VZ>(I, V, D.cfh, D.d, D.sc, D.ps, D.pp, x, q0);
-        else curr_vec_body<S, CF_IDX32, true, VZ>(I, V, D.cfh, D.d, D.sc, D.ps, D.pp, x, q0);
-    } else {
-        if (D.cm == CF_FULL) curr_vec_body<S, CF_FULL, false, VZ>(I, V, D.cfh, D.d, D.sc, D.ps, D.pp, x, q0);
-        else if (D.cm == CF_IDX16) curr_vec_body<S, CF_IDX16, false, VZ>(I, V, D.cfh, D.d, D.sc, D.ps, D.pp, x, q0);
-        else curr_vec_body<S, CF_IDX32, false, VZ>(I, V, D.cfh, D.d, D.sc, D.ps, D.pp, x, q0);
-    }
+    if constexpr (CM >= 0) { curr_vec_body<S, CM, PML, VZ>(I, V, D.cfh, D.d, D.sc, D.ps, D.pp, x, q0); return; }
+    if (D.cm == CF_FULL) curr_vec_body<S, CF_FULL, PML, VZ>(I, V, D.cfh, D.d, D.sc, D.ps, D.pp, x, q0);
+    else if (D.cm == CF_IDX16) curr_vec_body<S, CF_IDX16, PML, VZ>(I, V, D.cfh, D.d, D.sc, D.ps, D.pp, x, q0);
+    else curr_vec_body<S, CF_IDX32, PML, VZ>(I, V, D.cfh, D.d, D.sc, D.ps, D.pp, x, q0);
+}
+
+// vector of block b of a domain's main launch: plane r.x0 + local / bpp,
+// tile local % bpp over the rows [y0, y1), only vectors within [zlo, zhi)
+__device__ __forceinline__ bool batch_main_vector(const VecRange& r, const Dims& d, long long local, long long bpp, int vz,
+                                                  int& x, long long& q0) {
+    const long long px = local / bpp;
+    const long long tile = local - px * bpp;
+    q0 = (long long)r.y0 * d.sy + (tile * blockDim.x + threadIdx.x) * vz;
+    const int y = (int)(q0 / d.sy);
+    const int z0 = (int)(q0 - (long long)y * d.sy);
+    x = r.x0 + (int)px;
+    return y < r.y1 && z0 >= r.zlo && z0 + vz <= r.zhi;
+}
+
+// MAIN_PML: the launch holds only domains whose main launch runs the UPML
+// form (the host launches the domains of each kind in a launch of its own)
+template <typename S, int CM, bool MAIN_PML>
+__global__ void __launch_bounds__(256)
+k_volt_batch_main(const SimDesc<typename StoreTraits<S>::A>* __restrict__ sims, const unsigned* __restrict__ blk) {
+    typedef typename StoreTraits<S>::A A;
+    constexpr int VZ = VecWidth<S>::value;
+    const SimDesc<A>& D = sims[blk[blockIdx.x]];
+    int x;
+    long long q0;
+    if (!batch_main_vector(D.rv, D.d, (long long)blockIdx.x - D.vstart, D.vbpp, VZ, x, q0)) return;
+    batch_volt_body<S, CM, MAIN_PML, VZ>(D, x, q0);
+}
+
+template <typename S, int CM, bool MAIN_PML>
+__global__ void __launch_bounds__(256)
+k_curr_batch_main(const SimDesc<typename StoreTraits<S>::A>* __restrict__ sims, const unsigned* __restrict__ blk) {
+    typedef typename StoreTraits<S>::A A;
+    constexpr int VZ = VecWidth<S>::value;
+    const SimDesc<A>& D = sims[blk[blockIdx.x]];
+    int x;
+    long long q0;
+    if (!batch_main_vector(D.rc, D.d, (long long)blockIdx.x - D.cstart, D.cbpp, VZ, x, q0)) return;
+    batch_curr_body<S, CM, MAIN_PML, VZ>(D, x, q0);
+}
+
+// shells of the split domains, vectors of VS cells (shell_vector numbering)
+template <typename S, int CM, int VS>
+__global__ void __launch_bounds__(256)
+k_volt_batch_shell(const SimDesc<typename StoreTraits<S>::A>* __restrict__ sims, const unsigned* __restrict__ blk) {
+    typedef typename StoreTraits<S>::A A;
+    const SimDesc<A>& D = sims[blk[blockIdx.x]];
+    int x;
+    long long q0;
+    const unsigned t = (unsigned)(((long long)blockIdx.x - D.vsstart) * blockDim.x + threadIdx.x);
+    if (!shell_vector(D.gv, t, VS, D.d.sy, x, q0)) return;
+    batch_volt_body<S, CM, true, VS>(D, x, q0);
+}
+
+template <typename S, int CM, int VS>
+__global__ void __launch_bounds__(256)
+k_curr_batch_shell(const SimDesc<typename StoreTraits<S>::A>* __restrict__ sims, const unsigned* __restrict__ blk) {
+    typedef typename StoreTraits<S>::A A;
+    const SimDesc<A>& D = sims[blk[blockIdx.x]];
+    int x;
+    long long q0;
+    const unsigned t = (unsigned)(((long long)blockIdx.x - D.csstart) * blockDim.x + threadIdx.x);
+    if (!shell_vector(D.gc, t, VS, D.d.sy, x, q0)) return;
+    batch_curr_body<S, CM, true, VS>(D, x, q0);
 }
 
 // Engine_Ext_Excitation::Apply2{Voltages,Current}Impl for every domain
@@ -147,14 +182,24 @@ __global__ void k_step_end_batch(const SimDesc<typename StoreTraits<S>::A>* __re
 
 } // namespace
 
+// One update launch of a phase: its grid and per-block domain table.
+struct BatchLaunch {
+    DevBuf blk;             // uint32 domain index per block
+    unsigned blocks = 0;
+};
+
 struct fdtd_batch {
     std::vector<fdtd_engine*> eng;
     int prec = FDTD_PREC_FP32;
     hipStream_t stream = nullptr;
     DevBuf sims, vexc, cexc;
     int n_vexc = 0, n_cexc = 0;
-    long long vblocks = 0, cblocks = 0;
+    // per phase (0 volt, 1 curr): main launch in the UPML form, main launch
+    // without UPML code, shell launch
+    BatchLaunch lpml[2], lplain[2], lshell[2];
+    int cm = -1;            // coefficient mode shared by every domain, -1 mixed
     int block = 256;
+    int split_mode = -1;    // -1 auto (split every domain whose slabs form a shell), 0 never, 1 same as auto
     bool dirty = true;
     bool use_graph = true;
     hipGraph_t graph = nullptr;
@@ -184,20 +229,34 @@ struct fdtd_batch {
 
 namespace {
 
+// shells of split domains use half-width vectors for 2-byte storage, as the
+// single-domain shell launches do
+inline int batch_shell_width(size_t elem) { return elem == 2 ? 4 : (int)(16 / elem); }
+
+int upload_table(hipStream_t st, BatchLaunch& L, const std::vector<unsigned>& t) {
+    L.blocks = (unsigned)t.size();
+    int rc = L.blk.alloc(std::max<size_t>(1, t.size()) * sizeof(unsigned));
+    if (rc) return rc;
+    if (!t.empty()) HIPCHK(hipMemcpyAsync(L.blk.p, t.data(), t.size() * sizeof(unsigned), hipMemcpyHostToDevice, st));
+    return FDTD_OK;
+}
+
 template <typename A>
 int build_tables(fdtd_batch* b) {
     const int n = (int)b->eng.size();
     std::vector<SimDesc<A>> sd(n);
     std::vector<ExcEntry<A>> ve, ce;
-    long long vb = 0, cb = 0;
+    std::vector<unsigned> tpml[2], tplain[2], tshell[2];
+    b->cm = b->eng[0]->cf_mode;
     for (int i = 0; i < n; ++i) {
         fdtd_engine* h = b->eng[i];
+        if (h->cf_mode != b->cm) b->cm = -1;
         const int vz = (int)(16 / h->elem);
+        const int vs = batch_shell_width(h->elem);
         SimDesc<A>& D = sd[i];
         D = SimDesc<A>{};
         D.d = h->d;
         D.cm = h->cf_mode;
-        D.pml = h->pml.count > 0;
         D.V = h->V.p;
         D.I = h->I.p;
         if (h->cf_mode == CF_FULL) {
@@ -209,25 +268,43 @@ int build_tables(fdtd_batch* b) {
         }
         D.sc = make_scales<A>(h);
         D.ps = h->pml;
+        bool split = h->pml.count > 0 && h->shell_ok && b->split_mode != 0;
+        if (split) {
+            // shell vectors are numbered in 32 bits (at up to twice the count in half-width vectors)
+            const VecLaunch L = plan_vec_launch(h->N, h->d.sy, h->pml.count, true, h->ibox, vz, false);
+            if (2 * (L.g.nA0 + L.g.nA1 + L.g.nB + L.g.nC) >= (1LL << 31)) split = false;
+        }
         if (h->pml.count) {
             A* c = h->pml_coef.as<A>();
             const long long t = h->pml_total;
             D.pp.vv = c; D.pp.vvfo = c + t; D.pp.vvfn = c + 2 * t;
             D.pp.ii = c + 3 * t; D.pp.iifo = c + 4 * t; D.pp.iifn = c + 5 * t;
             D.pp.vflux = h->pml_flux.as<A>(); D.pp.cflux = h->pml_flux.as<A>() + t;
-            if (pml_rows_in_use(h)) {
+            // coefficient rows as the single-domain engine picks them, with
+            // this batch's split in place of the engine's own
+            const bool rows = h->pml_cmp && (h->pml_cmp_mode == 1 || (h->pml_cmp_mode < 0 && (h->cf_mode == CF_FULL || split)));
+            if (rows) {
                 D.pp.vidx = h->pml_vidx.as<unsigned short>(); D.pp.vtab = h->pml_vtab.as<A>();
                 D.pp.cidx = h->pml_cidx.as<unsigned short>(); D.pp.ctab = h->pml_ctab.as<A>();
             }
         }
-        const long long vplane = (long long)h->N[1] * h->d.sy;
-        const long long cplane = (long long)(h->N[1] - 1) * h->d.sy;
-        D.vbpx = (vplane / vz + b->block - 1) / b->block;
-        D.cbpx = (cplane / vz + b->block - 1) / b->block;
-        D.vstart = vb;
-        D.cstart = cb;
-        vb += D.vbpx * h->N[0];
-        cb += D.cbpx * (h->N[0] - 1);
+        D.main_pml = h->pml.count > 0 && !split;
+        for (int w = 0; w < 2; ++w) {
+            const VecLaunch L = plan_vec_launch(h->N, h->d.sy, h->pml.count, split, h->ibox, vz, w == 1, shell_zalign_of(h));
+            const ShellGrid g = vs == vz ? L.g : shell_grid_split(L, (int)h->N[2], vz, vz / vs);
+            const long long bpp = (L.nq / vz + b->block - 1) / b->block;
+            std::vector<unsigned>& tm = D.main_pml ? tpml[w] : tplain[w];
+            (w ? D.cstart : D.vstart) = (long long)tm.size();
+            (w ? D.rc : D.rv) = L.r;
+            (w ? D.cbpp : D.vbpp) = bpp;
+            tm.insert(tm.end(), (size_t)(bpp * L.nx), (unsigned)i);
+            (w ? D.gc : D.gv) = g;
+            (w ? D.csstart : D.vsstart) = (long long)tshell[w].size();
+            if (L.shell) {
+                const long long nsh = g.nA0 + g.nA1 + g.nB + g.nC;
+                tshell[w].insert(tshell[w].end(), (size_t)((nsh + b->block - 1) / b->block), (unsigned)i);
+            }
+        }
         D.step = h->step.as<unsigned>();
         D.probes = h->probe_dev.as<ProbeDev>();
         D.nprobes = (int)h->probes.size();
@@ -256,17 +333,23 @@ int build_tables(fdtd_batch* b) {
             }
         }
     }
+    for (int w = 0; w < 2; ++w)
+        if (tpml[w].size() >= (1ull << 31) || tplain[w].size() >= (1ull << 31) || tshell[w].size() >= (1ull << 31))
+            return fail(FDTD_ERR_ARG, "batch too large: more than 2^31 blocks in one launch");
     int rc;
+    for (int w = 0; w < 2; ++w)
+        if ((rc = upload_table(b->stream, b->lpml[w], tpml[w])) || (rc = upload_table(b->stream, b->lplain[w], tplain[w])) ||
+            (rc = upload_table(b->stream, b->lshell[w], tshell[w])))
+            return rc;
     if ((rc = b->sims.alloc(sd.size() * sizeof(SimDesc<A>)))) return rc;
-    HIPCHK(hipMemcpy(b->sims.p, sd.data(), sd.size() * sizeof(SimDesc<A>), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyAsync(b->sims.p, sd.data(), sd.size() * sizeof(SimDesc<A>), hipMemcpyHostToDevice, b->stream));
     if ((rc = b->vexc.alloc(std::max<size_t>(1, ve.size()) * sizeof(ExcEntry<A>)))) return rc;
     if ((rc = b->cexc.alloc(std::max<size_t>(1, ce.size()) * sizeof(ExcEntry<A>)))) return rc;
-    if (!ve.empty()) HIPCHK(hipMemcpy(b->vexc.p, ve.data(), ve.size() * sizeof(ExcEntry<A>), hipMemcpyHostToDevice));
-    if (!ce.empty()) HIPCHK(hipMemcpy(b->cexc.p, ce.data(), ce.size() * sizeof(ExcEntry<A>), hipMemcpyHostToDevice));
+    if (!ve.empty()) HIPCHK(hipMemcpyAsync(b->vexc.p, ve.data(), ve.size() * sizeof(ExcEntry<A>), hipMemcpyHostToDevice, b->stream));
+    if (!ce.empty()) HIPCHK(hipMemcpyAsync(b->cexc.p, ce.data(), ce.size() * sizeof(ExcEntry<A>), hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
     b->n_vexc = (int)ve.size();
     b->n_cexc = (int)ce.size();
-    b->vblocks = vb;
-    b->cblocks = cb;
     b->dirty = false;
     b->drop_graph();
     return FDTD_OK;
@@ -276,18 +359,49 @@ int batch_tables(fdtd_batch* b) {
     return b->prec == FDTD_PREC_FP64 ? build_tables<double>(b) : build_tables<float>(b);
 }
 
+// the launches of one update phase (w = 0 volt, 1 curr)
+template <typename S, int CM>
+void batch_phase(fdtd_batch* b, int w) {
+    typedef typename StoreTraits<S>::A A;
+    constexpr int VS = sizeof(S) == 2 ? VecWidth<S>::value / 2 : VecWidth<S>::value;
+    const SimDesc<A>* sims = b->sims.as<SimDesc<A>>();
+    const dim3 blk(b->block);
+    if (b->lpml[w].blocks) {
+        if (w == 0) hipLaunchKernelGGL((k_volt_batch_main<S, CM, true>), dim3(b->lpml[w].blocks), blk, 0, b->stream, sims, b->lpml[w].blk.as<unsigned>());
+        else hipLaunchKernelGGL((k_curr_batch_main<S, CM, true>), dim3(b->lpml[w].blocks), blk, 0, b->stream, sims, b->lpml[w].blk.as<unsigned>());
+    }
+    if (b->lplain[w].blocks) {
+        if (w == 0) hipLaunchKernelGGL((k_volt_batch_main<S, CM, false>), dim3(b->lplain[w].blocks), blk, 0, b->stream, sims, b->lplain[w].blk.as<unsigned>());
+        else hipLaunchKernelGGL((k_curr_batch_main<S, CM, false>), dim3(b->lplain[w].blocks), blk, 0, b->stream, sims, b->lplain[w].blk.as<unsigned>());
+    }
+    if (b->lshell[w].blocks) {
+        if (w == 0) hipLaunchKernelGGL((k_volt_batch_shell<S, CM, VS>), dim3(b->lshell[w].blocks), blk, 0, b->stream, sims, b->lshell[w].blk.as<unsigned>());
+        else hipLaunchKernelGGL((k_curr_batch_shell<S, CM, VS>), dim3(b->lshell[w].blocks), blk, 0, b->stream, sims, b->lshell[w].blk.as<unsigned>());
+    }
+}
+
+template <typename S>
+void batch_phase_any(fdtd_batch* b, int w) {
+    switch (b->cm) {
+    case CF_FULL: batch_phase<S, CF_FULL>(b, w); break;
+    case CF_IDX16: batch_phase<S, CF_IDX16>(b, w); break;
+    case CF_IDX32: batch_phase<S, CF_IDX32>(b, w); break;
+    default: batch_phase<S, -1>(b, w); break;
+    }
+}
+
 template <typename S>
 int batch_step(fdtd_batch* b, bool timed, size_t evi) {
     typedef typename StoreTraits<S>::A A;
     const SimDesc<A>* sims = b->sims.as<SimDesc<A>>();
     const int n = (int)b->eng.size();
     if (timed) HIPCHK(hipEventRecord(b->ev[evi], b->stream));
-    hipLaunchKernelGGL((k_volt_batch<S>), dim3((unsigned)b->vblocks), dim3(b->block), 0, b->stream, sims, n);
+    batch_phase_any<S>(b, 0);
     if (timed) HIPCHK(hipEventRecord(b->ev[evi + 1], b->stream));
     if (b->n_vexc)
         hipLaunchKernelGGL((k_excite_batch<S>), dim3((b->n_vexc + 255) / 256), dim3(256), 0, b->stream, sims, b->vexc.as<ExcEntry<A>>(), b->n_vexc, 0);
     if (timed) HIPCHK(hipEventRecord(b->ev[evi + 2], b->stream));
-    hipLaunchKernelGGL((k_curr_batch<S>), dim3((unsigned)b->cblocks), dim3(b->block), 0, b->stream, sims, n);
+    batch_phase_any<S>(b, 1);
     if (timed) HIPCHK(hipEventRecord(b->ev[evi + 3], b->stream));
     if (b->n_cexc)
         hipLaunchKernelGGL((k_excite_batch<S>), dim3((b->n_cexc + 255) / 256), dim3(256), 0, b->stream, sims, b->cexc.as<ExcEntry<A>>(), b->n_cexc, 1);
@@ -374,6 +488,14 @@ int fdtd_batch_set_block(fdtd_batch* b, int threads) {
     if (!b) return fail(FDTD_ERR_ARG, "null batch");
     if (threads != 64 && threads != 128 && threads != 256) return fail(FDTD_ERR_ARG, "block must be 64, 128 or 256 threads");
     b->block = threads;
+    b->dirty = true;
+    return FDTD_OK;
+}
+
+int fdtd_batch_set_split(fdtd_batch* b, int mode) {
+    if (!b) return fail(FDTD_ERR_ARG, "null batch");
+    if (mode < -1 || mode > 1) return fail(FDTD_ERR_ARG, "mode must be -1 (auto), 0 or 1");
+    b->split_mode = mode;
     b->dirty = true;
     return FDTD_OK;
 }

@@ -444,19 +444,46 @@ __device__ __forceinline__ void volt_vec_body(S* __restrict__ V, const S* __rest
 // vectors whose cells z0..z0+VZ-1 lie in [zlo, zhi).  With UPML slabs forming
 // the shell around an interior box, the interior vectors run here without
 // PML code and k_volt_shell / k_curr_shell take every other vector.
+// bpp > 0 selects the banded order (a 1D grid): the rows are cut into bands
+// of tpb tiles, and the launch sweeps band by band, each band over all its
+// planes, so the x-1 (x+1) plane of a band is still in L2 when the next
+// plane reads it again even where a whole plane would not fit.  A band-plane
+// takes bpp blocks; with xb set bpp is a multiple of 8 and block k of a
+// band-plane takes tile (k % 8) * (bpp / 8) + k / 8: workgroups go to the
+// eight XCDs in turn, so each XCD sweeps one contiguous sub-band and the y-1
+// (y+1) rows its blocks read again were loaded by the same XCD.  bpp = 0:
+// a 2D grid, tile = blockIdx.x, plane = blockIdx.y.
 struct VecRange {
     int x0, y0, y1, zlo, zhi;
+    int bpp, tpb, nx, xb;
 };
+
+// tile and plane of block b (linear, ascending order) of a banded launch;
+// false for the padding blocks of a band-plane
+__device__ __forceinline__ bool band_tile(const VecRange& r, unsigned b, long long& tile, int& px) {
+    const unsigned per = (unsigned)r.bpp * (unsigned)r.nx;
+    const unsigned band = b / per;
+    const unsigned rem = b - band * per;
+    px = (int)(rem / (unsigned)r.bpp);
+    const unsigned k = rem - (unsigned)px * (unsigned)r.bpp;
+    const unsigned t = r.xb ? (k & 7u) * ((unsigned)r.bpp >> 3) + (k >> 3) : k;
+    if (t >= (unsigned)r.tpb) return false;
+    tile = (long long)band * r.tpb + t;
+    return true;
+}
 
 template <typename S, int CM, bool PML, int VZ>
 __device__ __forceinline__ void volt_vec_range(S* __restrict__ V, const S* __restrict__ I, const CoeffPtrs<typename StoreTraits<S>::A>& cf,
                                                const Dims& d, const Scales<typename StoreTraits<S>::A>& sc, const PmlSet& ps,
                                                const PmlPtrs<typename StoreTraits<S>::A>& pp, const VecRange& r) {
-    const long long q0 = (long long)r.y0 * d.sy + ((long long)blockIdx.x * blockDim.x + threadIdx.x) * VZ;
+    long long tile = blockIdx.x;
+    int px = (int)blockIdx.y;
+    if (r.bpp && !band_tile(r, blockIdx.x, tile, px)) return;
+    const long long q0 = (long long)r.y0 * d.sy + (tile * blockDim.x + threadIdx.x) * VZ;
     const int y = (int)(q0 / d.sy);
     const int z0 = (int)(q0 - (long long)y * d.sy);
     if (y >= r.y1 || z0 < r.zlo || z0 + VZ > r.zhi) return;
-    volt_vec_body<S, CM, PML, VZ>(V, I, cf, d, sc, ps, pp, r.x0 + blockIdx.y, q0);
+    volt_vec_body<S, CM, PML, VZ>(V, I, cf, d, sc, ps, pp, r.x0 + px, q0);
 }
 
 // Without UPML code the voltage update fits 5 waves per SIMD for every
@@ -645,9 +672,11 @@ template <typename S, int CM, bool PML, int VZ, bool REV = false>
 __device__ __forceinline__ void curr_vec_range(S* __restrict__ I, const S* __restrict__ V, const CoeffPtrs<typename StoreTraits<S>::A>& cf,
                                                const Dims& d, const Scales<typename StoreTraits<S>::A>& sc, const PmlSet& ps,
                                                const PmlPtrs<typename StoreTraits<S>::A>& pp, const VecRange& r) {
-    const long long bx = REV ? (long long)(gridDim.x - 1 - blockIdx.x) : (long long)blockIdx.x;
-    const int px = REV ? (int)(gridDim.y - 1 - blockIdx.y) : (int)blockIdx.y;
-    const long long q0 = (long long)r.y0 * d.sy + (bx * blockDim.x + threadIdx.x) * VZ;
+    const unsigned bx = REV ? gridDim.x - 1 - blockIdx.x : blockIdx.x;
+    long long tile = bx;
+    int px = REV ? (int)(gridDim.y - 1 - blockIdx.y) : (int)blockIdx.y;
+    if (r.bpp && !band_tile(r, bx, tile, px)) return;
+    const long long q0 = (long long)r.y0 * d.sy + (tile * blockDim.x + threadIdx.x) * VZ;
     const int y = (int)(q0 / d.sy);
     const int z0 = (int)(q0 - (long long)y * d.sy);
     if (y >= r.y1 || z0 < r.zlo || z0 + VZ > r.zhi) return;

@@ -501,7 +501,7 @@ int build_hooks(fdtd_engine* h) {
 }
 
 VecLaunch vec_launch(const fdtd_engine* h, int vz, bool curr) {
-    VecLaunch L = plan_vec_launch(h->N, h->d.sy, h->pml.count, pml_split_in_use(h), h->ibox, vz, curr);
+    VecLaunch L = plan_vec_launch(h->N, h->d.sy, h->pml.count, pml_split_in_use(h), h->ibox, vz, curr, shell_zalign_of(h));
     // shell vectors are numbered in 32 bits, at up to 3x the count in
     // half-width vectors: larger shells run as one UPML launch
     if (L.shell && 3 * (L.g.nA0 + L.g.nA1 + L.g.nB + L.g.nC) >= (1LL << 31))
@@ -518,6 +518,22 @@ template <typename S> struct ShellWidth {
     enum { value = sizeof(S) == 2 ? VecWidth<S>::value / 2 : VecWidth<S>::value };
 };
 
+// Grid of a main launch over nx planes of nvec vectors each, with the tile
+// order set in r (a 2D grid of tiles x planes, or a banded 1D grid).
+static dim3 vec_grid(const fdtd_engine* h, long long nvec, unsigned nx, VecRange& r) {
+    const unsigned tiles = (unsigned)((nvec + 255) / 256);
+    r.bpp = 0; r.tpb = 0; r.nx = (int)nx; r.xb = 0;
+    int tpb = tile_band_of(h, tiles);
+    const bool xb = tile_xcd_of(h, tiles);
+    if (!tpb && !xb) return dim3(tiles, nx, 1);
+    if (!tpb) tpb = (int)tiles;
+    const unsigned bpp = xb ? 8u * (((unsigned)tpb + 7u) / 8u) : (unsigned)tpb;
+    const unsigned long long nb = (tiles + (unsigned)tpb - 1) / (unsigned)tpb;
+    if (nb * bpp * nx >= (1ull << 31)) return dim3(tiles, nx, 1);
+    r.tpb = tpb; r.xb = xb ? 1 : 0; r.bpp = (int)bpp;
+    return dim3((unsigned)(nb * bpp * nx), 1, 1);
+}
+
 template <typename S, bool PML, int CM>
 void launch_volt_main(fdtd_engine* h, const VecLaunch& L, CoeffPtrs<typename StoreTraits<S>::A> cf, PmlPtrs<typename StoreTraits<S>::A> pp) {
     constexpr int VZ = VecWidth<S>::value;
@@ -526,13 +542,14 @@ void launch_volt_main(fdtd_engine* h, const VecLaunch& L, CoeffPtrs<typename Sto
     const ShellGrid gs = VS == VZ ? L.g : shell_grid_split(L, (int)h->N[2], VZ, VZ / VS);
     const long long nshell = L.shell ? gs.nA0 + gs.nA1 + gs.nB + gs.nC : 0;
     if (L.nx) {
-        const dim3 g((unsigned)((L.nq / VZ + 255) / 256), L.nx, 1);
+        VecRange r = L.r;
+        const dim3 g = vec_grid(h, L.nq / VZ, L.nx, r);
         if constexpr (PML && CM == CF_FULL && sizeof(S) >= 4)
-            hipLaunchKernelGGL((k_volt_vec_upml_b5<S, CM, VZ>), g, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.r);
+            hipLaunchKernelGGL((k_volt_vec_upml_b5<S, CM, VZ>), g, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, r);
         else if constexpr (PML)
-            hipLaunchKernelGGL((k_volt_vec_upml<S, CM, VZ>), g, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.r);
+            hipLaunchKernelGGL((k_volt_vec_upml<S, CM, VZ>), g, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, r);
         else
-            hipLaunchKernelGGL((k_volt_vec<S, CM, VZ>), g, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.r);
+            hipLaunchKernelGGL((k_volt_vec<S, CM, VZ>), g, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, r);
     }
     if (nshell > 0)
         hipLaunchKernelGGL((k_volt_shell<S, CM, VS>), dim3((unsigned)((nshell + 255) / 256)), b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d,
@@ -547,13 +564,14 @@ void launch_curr_main(fdtd_engine* h, const VecLaunch& L, CoeffPtrs<typename Sto
     const ShellGrid gs = VS == VZ ? L.g : shell_grid_split(L, (int)h->N[2], VZ, VZ / VS);
     const long long nshell = L.shell ? gs.nA0 + gs.nA1 + gs.nB + gs.nC : 0;
     if (L.nx) {
-        const dim3 g((unsigned)((L.nq / VZ + 255) / 256), L.nx, 1);
+        VecRange r = L.r;
+        const dim3 g = vec_grid(h, L.nq / VZ, L.nx, r);
         if constexpr (PML && CM == CF_FULL && sizeof(S) >= 4)
-            hipLaunchKernelGGL((k_curr_vec_upml_b5<S, CM, VZ>), g, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.r);
+            hipLaunchKernelGGL((k_curr_vec_upml_b5<S, CM, VZ>), g, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, r);
         else if constexpr (PML)
-            hipLaunchKernelGGL((k_curr_vec_upml<S, CM, VZ>), g, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.r);
+            hipLaunchKernelGGL((k_curr_vec_upml<S, CM, VZ>), g, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, r);
         else
-            hipLaunchKernelGGL((k_curr_vec<S, CM, VZ>), g, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, L.r);
+            hipLaunchKernelGGL((k_curr_vec<S, CM, VZ>), g, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, r);
     }
     if (nshell > 0)
         hipLaunchKernelGGL((k_curr_shell<S, CM, VS>), dim3((unsigned)((nshell + 255) / 256)), b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d,
@@ -1827,12 +1845,34 @@ int fdtd_read_dft(fdtd_engine* h, int id, double* out) {
     return stage_d2h(h->stream, out, b->acc.p, b->acc.bytes);
 }
 
-int fdtd_set_tuning(fdtd_engine* h, int variant, int tile_y, int chunk_x) {
+int fdtd_set_shell_align(fdtd_engine* h, int vectors) {
     if (!h) return fail(FDTD_ERR_ARG, "null handle");
-    if (variant != 3) return fail(FDTD_ERR_ARG, "unknown kernel variant %d (3: vector kernels)", variant);
-    if (tile_y != 1 && tile_y != 2 && tile_y != 4) return fail(FDTD_ERR_ARG, "tile_y must be 1, 2 or 4");
-    if (chunk_x < 1) return fail(FDTD_ERR_ARG, "chunk_x must be >= 1");
+    if (vectors != -1 && vectors != 1 && vectors != 2 && vectors != 4 && vectors != 8)
+        return fail(FDTD_ERR_ARG, "vectors must be -1 (auto), 1, 2, 4 or 8");
+    h->shell_zalign = vectors;
     invalidate_graph(h);
+    return FDTD_OK;
+}
+
+int fdtd_set_tile_order(fdtd_engine* h, int band_tiles, int xcd) {
+    if (!h) return fail(FDTD_ERR_ARG, "null handle");
+    if (band_tiles < -1) return fail(FDTD_ERR_ARG, "band_tiles must be -1 (auto), 0 (whole planes) or a tile count");
+    if (xcd < -1 || xcd > 1) return fail(FDTD_ERR_ARG, "xcd must be -1 (auto), 0 or 1");
+    h->tile_band = band_tiles;
+    h->tile_xcd = xcd;
+    invalidate_graph(h);
+    return FDTD_OK;
+}
+
+int fdtd_tile_order(fdtd_engine* h, int* band_tiles, int* xcd) {
+    if (!h || !band_tiles || !xcd) return fail(FDTD_ERR_ARG, "null argument");
+    const int vz = (int)(16 / h->elem);
+    const VecLaunch L = vec_launch(h, vz, false);
+    VecRange r = L.r;
+    (void)vec_grid(h, L.nq / vz, L.nx, r);
+    const long long tiles = (L.nq / vz + 255) / 256;
+    *band_tiles = r.bpp && r.tpb < tiles ? r.tpb : 0;
+    *xcd = r.xb;
     return FDTD_OK;
 }
 

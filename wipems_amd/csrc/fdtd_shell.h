@@ -58,7 +58,11 @@ struct VecLaunch {
     ShellGrid g;
 };
 
-inline VecLaunch plan_vec_launch(const uint32_t N[3], long long sy, int pml_count, bool shell_ok, const int ibox[3][2], int vz, bool curr) {
+// za: the interior launch's z range starts and ends at multiples of za
+// vectors (rounded inwards), so that it and the shell launch share fewer
+// partial cache lines at the ends of the interior rows.
+inline VecLaunch plan_vec_launch(const uint32_t N[3], long long sy, int pml_count, bool shell_ok, const int ibox[3][2], int vz, bool curr,
+                                 int za = 1) {
     VecLaunch L{};
     const int nx = (int)N[0], ny = (int)N[1], nz = (int)N[2];
     const int xe = curr ? std::max(1, nx - 1) : nx;
@@ -66,7 +70,7 @@ inline VecLaunch plan_vec_launch(const uint32_t N[3], long long sy, int pml_coun
     L.pml_main = pml_count > 0 && !shell_ok;
     L.shell = pml_count > 0 && shell_ok;
     if (!L.shell) {
-        L.r = VecRange{0, 0, ny, 0, 1 << 30};
+        L.r = VecRange{0, 0, ny, 0, 1 << 30, 0, 0, 0, 0};
         L.nx = (unsigned)xe;
         L.nq = (long long)std::max(ye, 0) * sy;
         return L;
@@ -74,9 +78,10 @@ inline VecLaunch plan_vec_launch(const uint32_t N[3], long long sy, int pml_coun
     const int x0 = std::min(ibox[0][0], xe), x1 = std::min(std::max(ibox[0][1], x0), xe);
     const int y0 = ibox[1][0], y1 = std::max(ibox[1][1], y0);
     const int vn = (nz + vz - 1) / vz;
-    const int vlo = std::min((ibox[2][0] + vz - 1) / vz, vn);
-    const int vhi = std::max(std::min(ibox[2][1] / vz, vn), vlo);
-    L.r = VecRange{x0, y0, y1, vlo * vz, vhi * vz};
+    za = std::max(za, 1);
+    const int vlo = std::min(((ibox[2][0] + vz - 1) / vz + za - 1) / za * za, vn);
+    const int vhi = std::max(std::min(ibox[2][1] / vz / za * za, vn), vlo);
+    L.r = VecRange{x0, y0, y1, vlo * vz, vhi * vz, 0, 0, 0, 0};
     L.nx = (unsigned)(x1 - x0);
     L.nq = (long long)(y1 - y0) * sy;
     if (y1 == y0 || vhi == vlo) L.nx = 0;

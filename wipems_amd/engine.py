@@ -309,8 +309,20 @@ class Engine:
         self._chk(self._lib.fdtd_pml_split(self._h, byref(a)))
         return bool(a.value)
 
-    def set_tuning(self, variant=3, tile_y=4, chunk_x=1):
-        self._chk(self._lib.fdtd_set_tuning(self._h, int(variant), int(tile_y), int(chunk_x)))
+    def set_shell_align(self, vectors=-1):
+        """With the UPML split: the interior launch's z range aligned to `vectors` 16-byte vectors (1, 2, 4, 8; -1 auto)."""
+        self._chk(self._lib.fdtd_set_shell_align(self._h, int(vectors)))
+
+    def set_tile_order(self, band_tiles=-1, xcd=-1):
+        """Tile order of the update launches: band_tiles = tiles per y band (0: whole planes, -1: the
+        engine's choice), xcd = 1/0/-1: a sub-band of every band-plane per XCD (on / off / the engine's choice)."""
+        self._chk(self._lib.fdtd_set_tile_order(self._h, int(band_tiles), int(xcd)))
+
+    def tile_order(self):
+        """(tiles per y band, 0 = whole planes; XCD sub-bands on) of the voltage update."""
+        a, b = c_int(), c_int()
+        self._chk(self._lib.fdtd_tile_order(self._h, byref(a), byref(b)))
+        return a.value, bool(b.value)
 
     def set_kernel_timing(self, on):
         self._chk(self._lib.fdtd_set_kernel_timing(self._h, int(bool(on))))
@@ -541,6 +553,10 @@ class Batch:
 
     def set_use_graph(self, on):
         self._chk(self._lib.fdtd_batch_set_use_graph(self._h, int(bool(on))))
+
+    def set_split(self, mode):
+        """1/-1: interior and shell launches for members whose UPML forms a shell, 0: one UPML-form launch."""
+        self._chk(self._lib.fdtd_batch_set_split(self._h, int(mode)))
 
     def iterate(self, n):
         self._chk(self._lib.fdtd_batch_iterate(self._h, int(n)))
