@@ -274,11 +274,17 @@ int compress_pml(fdtd_engine* h);
 // coefficients 0.475 -> 0.498 ms.
 inline bool pml_split_in_use(const fdtd_engine* h) {
     if (!h->shell_ok || h->pml_split_mode == 0) return false;
-    if (h->pml_split_mode == 1) return true;
+    if (h->pml_split_mode >= 1) return true;
     if (h->elem == 8 || (h->elem == 4 && h->cf_mode == CF_FULL)) return false;
     long long inner = 1;
     for (int n = 0; n < 3; ++n) inner *= h->ibox[n][1] - h->ibox[n][0];
     return (double)inner >= 0.5 * (double)h->N[0] * h->N[1] * h->N[2];
+}
+
+// Rows form of the split (the main launch sweeps whole interior rows and
+// applies the z slabs at their ends): mode 2.
+inline bool pml_rows_form(const fdtd_engine* h) {
+    return pml_split_in_use(h) && h->pml_split_mode == 2;
 }
 
 // UPML coefficient rows: by default with full main coefficients, and with
@@ -296,9 +302,17 @@ inline bool pml_rows_in_use(const fdtd_engine* h) {
 // update); whole planes in block order otherwise (1537x1537x769 dipole:
 // every band size from 584 down to 97 tiles, with and without XCD
 // sub-bands, 1-12% slower; 512^3 with 2-byte storage 2-3% slower).
-// z alignment (in vectors) of the interior launch of the shell split
+// z alignment (in vectors) of the interior launch of the shell split.
+// Auto, measured on MI355X: 4 vectors (64 bytes) for 4-byte storage
+// (1537x1537x769 dipole 15.07 -> 14.41 ms per voltage update, 14.63 ->
+// 13.97 ms per current update; dispersive 385x385x193 0.364 -> 0.340 and
+// 0.354 -> 0.319 ms; 8 vectors is slower than 4), none for 2-byte storage,
+// whose half-width shell vectors make every cell moved into the shell launch
+// dearer (dispersive FP16 0.259 -> 0.298 ms at 4 vectors, graded 257^3 FP16
+// 0.157 -> 0.179 ms).
 inline int shell_zalign_of(const fdtd_engine* h) {
-    return h->shell_zalign > 0 ? h->shell_zalign : 1;
+    if (h->shell_zalign > 0) return h->shell_zalign;
+    return h->elem == 4 ? 4 : 1;
 }
 inline bool tile_auto_banded(const fdtd_engine* h, unsigned tiles) {
     return h->elem == 4 && h->pml.count == 0 && tiles >= 256;
