@@ -283,14 +283,10 @@ int  fdtd_pml_compressed(fdtd_engine* h, int* active);
  * interior box (as the operator's UPML does), the interior can run in a
  * launch without UPML code and the shell in a launch of its own, instead of
  * one launch whose wavefronts branch into the UPML form.  mode 1 = split,
- * 2 = split in the rows form (the main launch sweeps whole interior rows
- * and applies the z slabs at the row ends in the UPML form; the shell launch
- * keeps the x-slab planes and y-slab rows), 0 = one launch, -1 (default) =
- * the engine's choice: split when the interior holds at least half of the
- * cells, for FP16/BF16 storage and for FP32 with compressed coefficients;
- * one launch for FP32 with full coefficient arrays and for FP64 (measured
- * slower split).  Results are identical in every form.  fdtd_pml_split
- * reports whether the split launches are in use. */
+ * 0 = one launch, -1 (default) = the engine's choice: split when the
+ * interior holds at least half of the cells, for FP16/BF16 and FP32 storage;
+ * one launch for FP64.  Results are identical in every form.
+ * fdtd_pml_split reports whether the split launches are in use. */
 int  fdtd_set_pml_split(fdtd_engine* h, int mode);
 int  fdtd_pml_split(fdtd_engine* h, int* active);
 /* With the split: the interior launch's z range starts and ends at

@@ -2,9 +2,7 @@
 // sets read from stdin, every cell the update bodies write is covered
 // exactly once by the interior launch plus the shell launch (the shell in
 // vectors of the interior's width, half or a quarter of it; the interior's z
-// range aligned to 1, 4 or 8 vectors; and in the rows form, where the main
-// launch sweeps whole interior rows and may meet only the slabs of its own
-// set), the interior launch
+// range aligned to 1, 4 or 8 vectors), the interior launch
 // touches no slab cell, and the shell enumeration stays inside the field
 // arrays.  Input: cases of "N0 N1 N2 count" followed by count lines
 // "s0 s1 s2 n0 n1 n2".  Prints one line per case and vector width; exit 1
@@ -14,22 +12,11 @@
 #include <vector>
 #include "fdtd_shell.h"
 
-static int check(const uint32_t N[3], const std::vector<PmlSlab>& slabs, int vz, bool curr, int f, int za, bool rows) {
+static int check(const uint32_t N[3], const std::vector<PmlSlab>& slabs, int vz, bool curr, int f, int za) {
     int ibox[3][2] = {{0, 0}, {0, 0}, {0, 0}};
     const bool ok = find_interior(N, slabs.data(), (int)slabs.size(), ibox);
     const long long sy = ((long long)N[2] + 15) / 16 * 16;
-    const VecLaunch L = plan_vec_launch(N, sy, (int)slabs.size(), ok, ibox, vz, curr, za, rows);
-    PmlSet all{};
-    for (const PmlSlab& b : slabs) all.slab[all.count++] = b;
-    const PmlSet rps = rows_pml_set(all, L);
-    // a slab cell the rows-form main launch writes must lie in a slab of its set
-    auto in_set = [&](const PmlSet& ps, int x, int y, int z) {
-        for (int s = 0; s < ps.count; ++s) {
-            const PmlSlab& b = ps.slab[s];
-            if (x >= b.s[0] && x < b.s[0] + b.n[0] && y >= b.s[1] && y < b.s[1] + b.n[1] && z >= b.s[2] && z < b.s[2] + b.n[2]) return true;
-        }
-        return false;
-    };
+    const VecLaunch L = plan_vec_launch(N, sy, (int)slabs.size(), ok, ibox, vz, curr, za);
     const ShellGrid G = f == 1 ? L.g : shell_grid_split(L, (int)N[2], vz, f);
     const int w = vz / f;
     const int nx = (int)N[0], ny = (int)N[1], nz = (int)N[2];
@@ -59,13 +46,7 @@ static int check(const uint32_t N[3], const std::vector<PmlSlab>& slabs, int vz,
             if (y >= L.r.y1 || z0 < L.r.zlo || z0 + vz > L.r.zhi) continue;
             const int x = L.r.x0 + (int)bx;
             if (x >= xe || y >= ye || z0 >= ze) continue;
-            if (L.shell && !L.rows && in_slab(x, y, z0)) { printf("interior launch reaches a slab cell at %d %d %d\n", x, y, z0); return 1; }
-            if (L.rows)
-                for (int z = z0; z < std::min(z0 + vz, ze); ++z)
-                    if (in_set(all, x, y, z) && !in_set(rps, x, y, z)) {
-                        printf("rows launch meets a slab cell outside its slab set at %d %d %d\n", x, y, z);
-                        return 1;
-                    }
+            if (L.shell && in_slab(x, y, z0)) { printf("interior launch reaches a slab cell at %d %d %d\n", x, y, z0); return 1; }
             main_vecs += cover(x, y, z0, vz);
         }
     }
@@ -96,8 +77,8 @@ static int check(const uint32_t N[3], const std::vector<PmlSlab>& slabs, int vz,
                     return 1;
                 }
             }
-    printf("N %u %u %u slabs %zu shell %d vz %d/%d za %d rows %d %s: interior %lld shell %lld\n", N[0], N[1], N[2], slabs.size(), (int)ok, vz, w,
-           za, (int)L.rows, curr ? "curr" : "volt", main_vecs, shell_vecs);
+    printf("N %u %u %u slabs %zu shell %d vz %d/%d za %d %s: interior %lld shell %lld\n", N[0], N[1], N[2], slabs.size(), (int)ok, vz, w,
+           za, curr ? "curr" : "volt", main_vecs, shell_vecs);
     return 0;
 }
 
@@ -114,9 +95,8 @@ int main() {
         for (int vz : {2, 4, 8})
             for (int f : {1, 2, 4})
                 for (int za : {1, 4, 8})
-                    for (bool rows : {false, true})
-                        for (bool curr : {false, true})
-                            if (f <= vz && (!rows || za == 1) && check(N, slabs, vz, curr, f, za, rows)) return 1;
+                    for (bool curr : {false, true})
+                        if (f <= vz && check(N, slabs, vz, curr, f, za)) return 1;
     }
     return 0;
 }

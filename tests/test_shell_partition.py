@@ -3,9 +3,7 @@ launch over the UPML shell, wipems_amd/csrc/fdtd_shell.h) cover every
 vector the update bodies process exactly once, keep the interior launch off
 slab cells, and index inside the field arrays; also with the shell in
 vectors of a half or a quarter of the interior's width, and with the
-interior's z range aligned to 4 or 8 vectors, and in the rows form (the
-main launch sweeping whole interior rows and meeting only the slabs of its
-own set).  Checked on the host with the
+interior's z range aligned to 4 or 8 vectors.  Checked on the host with the
 engine's own planning code (tests/native/shell_partition.cpp) for the slab
 sets the operator builds, for partial shells, for a z extent so short that
 one vector touches both z slabs, and for slabs that form no shell (one
@@ -66,7 +64,7 @@ def test_operator_shells(harness):
         assert slabs
         cases.append((N, slabs))
     lines = run(harness, cases)
-    assert len(lines) == 64 * len(cases)
+    assert len(lines) == 48 * len(cases)
     assert all(" shell 1 " in l for l in lines)
 
 
@@ -81,4 +79,4 @@ def test_non_shell_slabs_use_one_launch(harness):
 def test_no_upml_single_launch(harness):
     # no slabs: one launch over everything, XCD-contiguous block order
     lines = run(harness, [((37, 29, 45), []), ((9, 300, 17), [])])
-    assert len(lines) == 128 and all(" shell 0 " in l for l in lines)
+    assert len(lines) == 96 and all(" shell 0 " in l for l in lines)

@@ -501,8 +501,7 @@ int build_hooks(fdtd_engine* h) {
 }
 
 VecLaunch vec_launch(const fdtd_engine* h, int vz, bool curr) {
-    VecLaunch L = plan_vec_launch(h->N, h->d.sy, h->pml.count, pml_split_in_use(h), h->ibox, vz, curr, shell_zalign_of(h),
-                                  pml_rows_form(h));
+    VecLaunch L = plan_vec_launch(h->N, h->d.sy, h->pml.count, pml_split_in_use(h), h->ibox, vz, curr, shell_zalign_of(h));
     // shell vectors are numbered in 32 bits, at up to 3x the count in
     // half-width vectors: larger shells run as one UPML launch
     if (L.shell && 3 * (L.g.nA0 + L.g.nA1 + L.g.nB + L.g.nC) >= (1LL << 31))
@@ -543,15 +542,14 @@ void launch_volt_main(fdtd_engine* h, const VecLaunch& L, CoeffPtrs<typename Sto
     const ShellGrid gs = VS == VZ ? L.g : shell_grid_split(L, (int)h->N[2], VZ, VZ / VS);
     const long long nshell = L.shell ? gs.nA0 + gs.nA1 + gs.nB + gs.nC : 0;
     if (L.nx) {
-        const PmlSet pm = L.rows ? rows_pml_set(h->pml, L) : h->pml;
         VecRange r = L.r;
         const dim3 g = vec_grid(h, L.nq / VZ, L.nx, r);
         if constexpr (PML && CM == CF_FULL && sizeof(S) >= 4)
-            hipLaunchKernelGGL((k_volt_vec_upml_b5<S, CM, VZ>), g, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), pm, pp, r);
+            hipLaunchKernelGGL((k_volt_vec_upml_b5<S, CM, VZ>), g, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, r);
         else if constexpr (PML)
-            hipLaunchKernelGGL((k_volt_vec_upml<S, CM, VZ>), g, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), pm, pp, r);
+            hipLaunchKernelGGL((k_volt_vec_upml<S, CM, VZ>), g, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, r);
         else
-            hipLaunchKernelGGL((k_volt_vec<S, CM, VZ>), g, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), pm, pp, r);
+            hipLaunchKernelGGL((k_volt_vec<S, CM, VZ>), g, b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, r);
     }
     if (nshell > 0)
         hipLaunchKernelGGL((k_volt_shell<S, CM, VS>), dim3((unsigned)((nshell + 255) / 256)), b, 0, h->stream, h->V.as<S>(), h->I.as<S>(), cf, h->d,
@@ -566,15 +564,14 @@ void launch_curr_main(fdtd_engine* h, const VecLaunch& L, CoeffPtrs<typename Sto
     const ShellGrid gs = VS == VZ ? L.g : shell_grid_split(L, (int)h->N[2], VZ, VZ / VS);
     const long long nshell = L.shell ? gs.nA0 + gs.nA1 + gs.nB + gs.nC : 0;
     if (L.nx) {
-        const PmlSet pm = L.rows ? rows_pml_set(h->pml, L) : h->pml;
         VecRange r = L.r;
         const dim3 g = vec_grid(h, L.nq / VZ, L.nx, r);
         if constexpr (PML && CM == CF_FULL && sizeof(S) >= 4)
-            hipLaunchKernelGGL((k_curr_vec_upml_b5<S, CM, VZ>), g, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), pm, pp, r);
+            hipLaunchKernelGGL((k_curr_vec_upml_b5<S, CM, VZ>), g, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, r);
         else if constexpr (PML)
-            hipLaunchKernelGGL((k_curr_vec_upml<S, CM, VZ>), g, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), pm, pp, r);
+            hipLaunchKernelGGL((k_curr_vec_upml<S, CM, VZ>), g, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, r);
         else
-            hipLaunchKernelGGL((k_curr_vec<S, CM, VZ>), g, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), pm, pp, r);
+            hipLaunchKernelGGL((k_curr_vec<S, CM, VZ>), g, b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d, make_scales<typename StoreTraits<S>::A>(h), h->pml, pp, r);
     }
     if (nshell > 0)
         hipLaunchKernelGGL((k_curr_shell<S, CM, VS>), dim3((unsigned)((nshell + 255) / 256)), b, 0, h->stream, h->I.as<S>(), h->V.as<S>(), cf, h->d,
@@ -1897,7 +1894,7 @@ int fdtd_pml_compressed(fdtd_engine* h, int* active) {
 
 int fdtd_set_pml_split(fdtd_engine* h, int mode) {
     if (!h) return fail(FDTD_ERR_ARG, "null handle");
-    if (mode < -1 || mode > 2) return fail(FDTD_ERR_ARG, "mode must be -1 (auto), 0, 1 or 2");
+    if (mode < -1 || mode > 1) return fail(FDTD_ERR_ARG, "mode must be -1 (auto), 0 or 1");
     h->pml_split_mode = mode;
     invalidate_graph(h);
     return FDTD_OK;

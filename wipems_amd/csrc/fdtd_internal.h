@@ -266,25 +266,18 @@ void find_shell(fdtd_engine* h);
 int compress_pml(fdtd_engine* h);
 // split interior / shell launches of the vector kernels (see fdtd_set_pml_split):
 // by default where the interior holds at least half of the cells, for
-// 2-byte storage and for FP32 with compressed coefficients; FP32 with full
-// coefficient arrays and FP64 keep one launch.  Measured per step on
-// MI355X: 1537x1537x769 dipole 34.1 -> 30.5 ms, 385x385x193 dipole 0.763 ->
-// 0.721 ms, dispersive 385x385x193 0.910 -> 0.891 ms (FP16: 0.780 ->
-// 0.667), graded 257^3 FP16 0.448 -> 0.437 ms; graded 257^3 FP32 with full
-// coefficients 0.475 -> 0.498 ms.
+// 2-byte and 4-byte storage; FP64 keeps one launch.  Measured per step on
+// MI355X with the interior's z range aligned (shell_zalign_of): 1537x1537x769
+// dipole FP32 37.8 (one launch) -> 28.3 ms, dispersive 385x385x193 FP32
+// 0.957 -> 0.828 ms (FP16 0.793 -> 0.669), graded 257^3 FP32 with full
+// coefficients 0.490 -> 0.473 ms (FP16 0.465 -> 0.326).
 inline bool pml_split_in_use(const fdtd_engine* h) {
     if (!h->shell_ok || h->pml_split_mode == 0) return false;
-    if (h->pml_split_mode >= 1) return true;
-    if (h->elem == 8 || (h->elem == 4 && h->cf_mode == CF_FULL)) return false;
+    if (h->pml_split_mode == 1) return true;
+    if (h->elem == 8) return false;
     long long inner = 1;
     for (int n = 0; n < 3; ++n) inner *= h->ibox[n][1] - h->ibox[n][0];
     return (double)inner >= 0.5 * (double)h->N[0] * h->N[1] * h->N[2];
-}
-
-// Rows form of the split (the main launch sweeps whole interior rows and
-// applies the z slabs at their ends): mode 2.
-inline bool pml_rows_form(const fdtd_engine* h) {
-    return pml_split_in_use(h) && h->pml_split_mode == 2;
 }
 
 // UPML coefficient rows: by default with full main coefficients, and with

@@ -49,17 +49,12 @@ inline bool find_interior(const uint32_t N[3], const PmlSlab* slab, int count, i
 // slabs that do not form a shell) one launch covers planes [0, xe) and rows
 // [0, ye); with a shell, the main launch covers the vectors of the interior
 // box only and the shell launch the rest.
-// rows = true: the main launch sweeps the interior rows in full and applies
-// the z slabs at their ends itself (in the UPML form, with only the slabs
-// that cross the interior rows in its slab set, rows_ps); the shell launch
-// then holds the x-slab planes and y-slab rows, contiguous runs of memory.
 struct VecLaunch {
     VecRange r;
     unsigned nx;               // planes of the main launch (0: none)
     long long nq;              // flat (y, z) extent of the main launch per plane
     bool pml_main;             // main launch runs the UPML form
     bool shell;
-    bool rows;                 // main launch is the rows form
     ShellGrid g;
 };
 
@@ -67,7 +62,7 @@ struct VecLaunch {
 // vectors (rounded inwards), so that it and the shell launch share fewer
 // partial cache lines at the ends of the interior rows.
 inline VecLaunch plan_vec_launch(const uint32_t N[3], long long sy, int pml_count, bool shell_ok, const int ibox[3][2], int vz, bool curr,
-                                 int za = 1, bool rows = false) {
+                                 int za = 1) {
     VecLaunch L{};
     const int nx = (int)N[0], ny = (int)N[1], nz = (int)N[2];
     const int xe = curr ? std::max(1, nx - 1) : nx;
@@ -86,12 +81,10 @@ inline VecLaunch plan_vec_launch(const uint32_t N[3], long long sy, int pml_coun
     za = std::max(za, 1);
     const int vlo = std::min(((ibox[2][0] + vz - 1) / vz + za - 1) / za * za, vn);
     const int vhi = std::max(std::min(ibox[2][1] / vz / za * za, vn), vlo);
-    L.rows = rows;
-    L.pml_main = rows;
-    L.r = rows ? VecRange{x0, y0, y1, 0, 1 << 30, 0, 0, 0, 0} : VecRange{x0, y0, y1, vlo * vz, vhi * vz, 0, 0, 0, 0};
+    L.r = VecRange{x0, y0, y1, vlo * vz, vhi * vz, 0, 0, 0, 0};
     L.nx = (unsigned)(x1 - x0);
     L.nq = (long long)(y1 - y0) * sy;
-    if (y1 == y0 || (!rows && vhi == vlo)) L.nx = 0;
+    if (y1 == y0 || vhi == vlo) L.nx = 0;
     ShellGrid& g = L.g;
     g.x0 = x0; g.x1 = x1; g.xe = xe;
     g.y0 = y0; g.y1 = y1; g.ny = ny;
@@ -99,20 +92,8 @@ inline VecLaunch plan_vec_launch(const uint32_t N[3], long long sy, int pml_coun
     g.nA0 = (long long)x0 * ny * vn;
     g.nA1 = (long long)(xe - x1) * ny * vn;
     g.nB = (long long)(x1 - x0) * (y0 + ny - y1) * vn;
-    g.nC = rows ? 0 : (long long)(x1 - x0) * (y1 - y0) * (vlo + vn - vhi);
+    g.nC = (long long)(x1 - x0) * (y1 - y0) * (vlo + vn - vhi);
     return L;
-}
-
-// The slabs that cross the rows of the interior box (planes [x0, x1), rows
-// [y0, y1)): the only ones the rows-form main launch can meet.
-inline PmlSet rows_pml_set(const PmlSet& all, const VecLaunch& L) {
-    PmlSet r{};
-    const int x0 = L.r.x0, x1 = L.r.x0 + (int)L.nx;
-    for (int s = 0; s < all.count; ++s) {
-        const PmlSlab& b = all.slab[s];
-        if (b.s[0] < x1 && b.s[0] + b.n[0] > x0 && b.s[1] < L.r.y1 && b.s[1] + b.n[1] > L.r.y0) r.slab[r.count++] = b;
-    }
-    return r;
 }
 
 // The shell of L (planned with vectors of vz cells) in vectors of vz / f
@@ -127,6 +108,6 @@ inline ShellGrid shell_grid_split(const VecLaunch& L, int nz, int vz, int f) {
     g.nA0 = (long long)g.x0 * g.ny * g.vn;
     g.nA1 = (long long)(g.xe - g.x1) * g.ny * g.vn;
     g.nB = (long long)(g.x1 - g.x0) * (g.y0 + g.ny - g.y1) * g.vn;
-    g.nC = L.rows ? 0 : (long long)(g.x1 - g.x0) * (g.y1 - g.y0) * (g.vlo + g.vn - g.vhi);
+    g.nC = (long long)(g.x1 - g.x0) * (g.y1 - g.y0) * (g.vlo + g.vn - g.vhi);
     return g;
 }
