@@ -372,8 +372,8 @@ def test_three_slabs_in_one_vector_are_rejected(gpu):
 
 def test_upml_split_auto_policy(gpu):
     """Default: split launches when the interior holds at least half of the
-    cells, for 2-byte storage and for FP32 with compressed coefficients; one
-    launch for FP32 with full coefficient arrays and for FP64."""
+    cells, for 2-byte storage and for FP32 (compressed or full coefficient
+    arrays); one launch for FP64."""
     from wipems_amd.operator import BC_PML, Operator
 
     def domain(pml, n=129, prec=0):
@@ -387,7 +387,7 @@ def test_upml_split_auto_policy(gpu):
     thin, thick, fat = domain(1), domain(8), domain(24)     # interior 91%, 76%, 37% of the cells
     thin64 = domain(1, 129, 1)
     cases = [(thin, "fp16", None, True), (thin, "bf16", None, True), (thin, "fp32", True, True), (thick, "fp32", True, True),
-             (thick, "fp16", None, True), (thin, "fp32", False, False), (fat, "fp32", True, False), (fat, "fp16", None, False),
+             (thick, "fp16", None, True), (thin, "fp32", False, True), (fat, "fp32", True, False), (fat, "fp16", None, False),
              (thin64, "fp64", False, False)]
     for d, precision, compressed, want in cases:
         eng = engine_from_operator(d, precision=precision, compressed=compressed)
