@@ -261,12 +261,14 @@ int  fdtd_get_num_timesteps(fdtd_engine* h, uint32_t* numTS);
  * whole planes; n > 0: the tiles of a plane are cut into y bands of n tiles
  * and the launch sweeps band by band, each over all its planes, so the
  * neighbour plane a tile reads again is still in L2 where a whole plane
- * would not fit; -1 (default) = the engine's choice.  xcd = 1: each of the
- * eight XCDs takes one contiguous sub-band of every band-plane, so the
- * neighbour rows its blocks read again were loaded by the same XCD; 0 off;
- * -1 the engine's choice.  Results do not depend on the order, only speed
- * does.  fdtd_tile_order reports the order in use for the voltage update
- * (band_tiles 0 = whole planes). */
+ * would not fit; -1 (default) = the engine's choice.  xcd = n > 0: the
+ * eight XCDs take chunks of n consecutive tiles of every band-plane in turn
+ * (XCD j chunks j, j + 8, ...; n >= band_tiles / 8 gives each XCD one
+ * contiguous sub-band), so the neighbour rows and planes its blocks read
+ * again were loaded by the same XCD; 0 off; -1 the engine's choice.
+ * Results do not depend on the order, only speed does.  fdtd_tile_order
+ * reports the order in use for the voltage update (band_tiles 0 = whole
+ * planes, xcd = chunk width in use). */
 int  fdtd_set_tile_order(fdtd_engine* h, int band_tiles, int xcd);
 int  fdtd_tile_order(fdtd_engine* h, int* band_tiles, int* xcd);
 /* UPML coefficients: the vector kernels can read per-cell uint16 row

@@ -518,24 +518,25 @@ def test_fused_hooks_equal_separate_launches(gpu, precision, prec, case):
     assert np.abs(out[0][0]).max() > 0
 
 
-@pytest.mark.parametrize("precision,compressed,split,band,xcd", [
-    ("fp32", False, 0, 0, 1), ("fp32", True, 1, 0, 1), ("fp32", True, 0, 3, 1), ("fp32", False, 0, 5, 0),
-    ("fp32", True, 1, 2, 0), ("fp64", False, 0, 3, 1),
+@pytest.mark.parametrize("precision,compressed,split,band,xcd,want_xcd", [
+    ("fp32", False, 0, 0, 1, 1), ("fp32", True, 1, 0, 1, 1), ("fp32", True, 0, 3, 1, 1), ("fp32", False, 0, 5, 0, 0),
+    ("fp32", True, 1, 2, 0, 0), ("fp64", False, 0, 3, 1, 1), ("fp64", False, 0, 0, 1, 1), ("fp64", False, 1, 0, 8, 2),
 ])
-def test_tile_orders_bitwise(gpu, precision, compressed, split, band, xcd):
+def test_tile_orders_bitwise(gpu, precision, compressed, split, band, xcd, want_xcd):
     """Banded tile orders (y bands of `band` tiles swept plane by plane, the
-    last band partial; with xcd, tile (k % 8) * (bpp / 8) + k / 8 of each
-    band-plane, padding blocks idle) give the oracle's bits, with one UPML
-    launch and with the interior/shell split, for the ascending voltage and
-    the descending current sweep; 64 cells per axis give planes of 6 tiles
-    (FP32; 11 for FP64), no multiple of 8 or of the band."""
+    last band partial; with xcd, chunks of xcd tiles of each band-plane to
+    the eight XCDs in turn, capped at one sub-band per XCD, padding blocks
+    idle) give the oracle's bits, with one UPML launch and with the
+    interior/shell split, for the ascending voltage and the descending
+    current sweep; 64 cells per axis give planes of 6 tiles (FP32; 11 for
+    FP64), no multiple of 8 or of the band."""
     sc = scenarios.graded_port_scenario(n_cells=64, steps=160, prec=1 if precision == "fp64" else 0, n_boxes=5, pml=6)
     d = sc.data
     eng = engine_from_operator(d, precision=precision, compressed=compressed)
     eng.set_tile_order(band, xcd)
     eng.set_pml_split(split)
     got_band, got_xcd = eng.tile_order()
-    assert (got_band, got_xcd) == (band, bool(xcd))
+    assert (got_band, got_xcd) == (band, want_xcd)
     _kinds[id(eng)] = [p.kind for p in d.probes]
     orc = oracle_from_operator(d, ftz=True)
     eng.iterate(160)

@@ -448,11 +448,14 @@ __device__ __forceinline__ void volt_vec_body(S* __restrict__ V, const S* __rest
 // of tpb tiles, and the launch sweeps band by band, each band over all its
 // planes, so the x-1 (x+1) plane of a band is still in L2 when the next
 // plane reads it again even where a whole plane would not fit.  A band-plane
-// takes bpp blocks; with xb set bpp is a multiple of 8 and block k of a
-// band-plane takes tile (k % 8) * (bpp / 8) + k / 8: workgroups go to the
-// eight XCDs in turn, so each XCD sweeps one contiguous sub-band and the y-1
-// (y+1) rows its blocks read again were loaded by the same XCD.  bpp = 0:
-// a 2D grid, tile = blockIdx.x, plane = blockIdx.y.
+// takes bpp blocks.  xb > 0 (XCD chunks of xb tiles): workgroups go to the
+// eight XCDs in turn (block k to XCD k % 8), and block k of a band-plane
+// takes tile (c * 8 + k % 8) * xb + o with s = k / 8, c = s / xb, o = s % xb:
+// XCD j sweeps chunks j, j + 8, j + 16, ... of xb consecutive tiles, the
+// same chunks in every plane, so the y-1 (y+1) rows and the x-1 (x+1) plane
+// its blocks read again were loaded by the same XCD; bpp is then a multiple
+// of 8 * xb.  xb >= tpb / 8 gives each XCD one contiguous sub-band.
+// bpp = 0: a 2D grid, tile = blockIdx.x, plane = blockIdx.y.
 struct VecRange {
     int x0, y0, y1, zlo, zhi;
     int bpp, tpb, nx, xb;
@@ -466,7 +469,11 @@ __device__ __forceinline__ bool band_tile(const VecRange& r, unsigned b, long lo
     const unsigned rem = b - band * per;
     px = (int)(rem / (unsigned)r.bpp);
     const unsigned k = rem - (unsigned)px * (unsigned)r.bpp;
-    const unsigned t = r.xb ? (k & 7u) * ((unsigned)r.bpp >> 3) + (k >> 3) : k;
+    unsigned t = k;
+    if (r.xb) {
+        const unsigned s = k >> 3, c = s / (unsigned)r.xb;
+        t = (c * 8u + (k & 7u)) * (unsigned)r.xb + (s - c * (unsigned)r.xb);
+    }
     if (t >= (unsigned)r.tpb) return false;
     tile = (long long)band * r.tpb + t;
     return true;

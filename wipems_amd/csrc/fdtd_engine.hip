@@ -524,13 +524,14 @@ static dim3 vec_grid(const fdtd_engine* h, long long nvec, unsigned nx, VecRange
     const unsigned tiles = (unsigned)((nvec + 255) / 256);
     r.bpp = 0; r.tpb = 0; r.nx = (int)nx; r.xb = 0;
     int tpb = tile_band_of(h, tiles);
-    const bool xb = tile_xcd_of(h, tiles);
+    int xb = tile_xcd_of(h, tiles);
     if (!tpb && !xb) return dim3(tiles, nx, 1);
     if (!tpb) tpb = (int)tiles;
-    const unsigned bpp = xb ? 8u * (((unsigned)tpb + 7u) / 8u) : (unsigned)tpb;
+    xb = std::min(xb, (tpb + 7) / 8);     // at most one contiguous sub-band per XCD
+    const unsigned bpp = xb ? 8u * (unsigned)xb * (((unsigned)tpb + 8u * xb - 1u) / (8u * xb)) : (unsigned)tpb;
     const unsigned long long nb = (tiles + (unsigned)tpb - 1) / (unsigned)tpb;
     if (nb * bpp * nx >= (1ull << 31)) return dim3(tiles, nx, 1);
-    r.tpb = tpb; r.xb = xb ? 1 : 0; r.bpp = (int)bpp;
+    r.tpb = tpb; r.xb = xb; r.bpp = (int)bpp;
     return dim3((unsigned)(nb * bpp * nx), 1, 1);
 }
 
@@ -1857,7 +1858,7 @@ int fdtd_set_shell_align(fdtd_engine* h, int vectors) {
 int fdtd_set_tile_order(fdtd_engine* h, int band_tiles, int xcd) {
     if (!h) return fail(FDTD_ERR_ARG, "null handle");
     if (band_tiles < -1) return fail(FDTD_ERR_ARG, "band_tiles must be -1 (auto), 0 (whole planes) or a tile count");
-    if (xcd < -1 || xcd > 1) return fail(FDTD_ERR_ARG, "xcd must be -1 (auto), 0 or 1");
+    if (xcd < -1) return fail(FDTD_ERR_ARG, "xcd must be -1 (auto), 0 (off) or a chunk width in tiles");
     h->tile_band = band_tiles;
     h->tile_xcd = xcd;
     invalidate_graph(h);

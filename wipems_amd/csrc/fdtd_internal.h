@@ -176,7 +176,7 @@ struct fdtd_engine {
     int ibox[3][2] = {{0, 0}, {0, 0}, {0, 0}};
     int shell_zalign = -1;        // interior z range aligned to this many vectors: -1 auto (fdtd_set_shell_align)
     int tile_band = -1;           // tiles per y band of the update launches: -1 auto, 0 whole planes (fdtd_set_tile_order)
-    int tile_xcd = -1;            // XCD sub-bands: -1 auto, 0 off, 1 on
+    int tile_xcd = -1;            // XCD chunk width in tiles: -1 auto, 0 off
     int pml_split_mode = -1;      // -1 auto (FP16/BF16 storage), 0 off, 1 on
 
     // excitation
@@ -287,14 +287,6 @@ inline bool pml_split_in_use(const fdtd_engine* h) {
 inline bool pml_rows_in_use(const fdtd_engine* h) {
     return h->pml_cmp && (h->pml_cmp_mode == 1 || (h->pml_cmp_mode < 0 && (h->cf_mode == CF_FULL || pml_split_in_use(h))));
 }
-// Tile order of the main update launches (see VecRange): tiles per y band
-// (0: whole planes) and whether each XCD takes a sub-band of every
-// band-plane.  Auto, from measurements on MI355X: bands of 128 tiles with
-// XCD sub-bands for 4-byte storage without UPML on planes of at least 256
-// tiles (512^3: 0.863 -> 0.844 ms per voltage, 0.907 -> 0.881 ms per current
-// update); whole planes in block order otherwise (1537x1537x769 dipole:
-// every band size from 584 down to 97 tiles, with and without XCD
-// sub-bands, 1-12% slower; 512^3 with 2-byte storage 2-3% slower).
 // z alignment (in vectors) of the interior launch of the shell split.
 // Auto, measured on MI355X: 4 vectors (64 bytes) for 4-byte storage
 // (1537x1537x769 dipole 15.07 -> 14.41 ms per voltage update, 14.63 ->
@@ -307,6 +299,16 @@ inline int shell_zalign_of(const fdtd_engine* h) {
     if (h->shell_zalign > 0) return h->shell_zalign;
     return h->elem == 4 ? 4 : 1;
 }
+
+// Tile order of the main update launches (see VecRange): tiles per y band
+// (0: whole planes) and the XCD chunk width in tiles (0: off).  Auto, from
+// measurements on MI355X: bands of 128 tiles with one contiguous sub-band
+// per XCD (chunks of 16 tiles) for 4-byte storage without UPML on planes of
+// at least 256 tiles (512^3: 0.863 -> 0.844 ms per voltage, 0.907 -> 0.881
+// ms per current update, fetched bytes 1.10 -> 1.006 of the algorithmic);
+// whole planes in block order otherwise (1537x1537x769 dipole: every band
+// size from 584 down to 97 tiles, with and without XCD sub-bands, 1-12%
+// slower; 512^3 with 2-byte storage 2-3% slower).
 inline bool tile_auto_banded(const fdtd_engine* h, unsigned tiles) {
     return h->elem == 4 && h->pml.count == 0 && tiles >= 256;
 }
@@ -315,9 +317,9 @@ inline int tile_band_of(const fdtd_engine* h, unsigned tiles) {
     if (h->tile_band < 0 && tile_auto_banded(h, tiles)) return 128;
     return 0;
 }
-inline bool tile_xcd_of(const fdtd_engine* h, unsigned tiles) {
-    if (h->tile_xcd >= 0) return h->tile_xcd == 1;
-    return h->tile_band < 0 && tile_auto_banded(h, tiles);
+inline int tile_xcd_of(const fdtd_engine* h, unsigned tiles) {
+    if (h->tile_xcd >= 0) return h->tile_xcd;
+    return h->tile_band < 0 && tile_auto_banded(h, tiles) ? 16 : 0;
 }
 bool probes_need_realloc(const fdtd_engine* h, uint32_t upto_ts);
 int finalize_probes(fdtd_engine* h, uint32_t upto_ts);

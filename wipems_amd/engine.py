@@ -315,14 +315,14 @@ class Engine:
 
     def set_tile_order(self, band_tiles=-1, xcd=-1):
         """Tile order of the update launches: band_tiles = tiles per y band (0: whole planes, -1: the
-        engine's choice), xcd = 1/0/-1: a sub-band of every band-plane per XCD (on / off / the engine's choice)."""
+        engine's choice), xcd = n > 0: the XCDs take chunks of n tiles in turn, 0: off, -1: the engine's choice."""
         self._chk(self._lib.fdtd_set_tile_order(self._h, int(band_tiles), int(xcd)))
 
     def tile_order(self):
-        """(tiles per y band, 0 = whole planes; XCD sub-bands on) of the voltage update."""
+        """(tiles per y band, 0 = whole planes; XCD chunk width, 0 = off) of the voltage update."""
         a, b = c_int(), c_int()
         self._chk(self._lib.fdtd_tile_order(self._h, byref(a), byref(b)))
-        return a.value, bool(b.value)
+        return a.value, b.value
 
     def set_kernel_timing(self, on):
         self._chk(self._lib.fdtd_set_kernel_timing(self._h, int(bool(on))))
