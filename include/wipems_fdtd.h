@@ -42,6 +42,11 @@
  *                                   (FDTD/engine_interface_fdtd.cpp:63-158), FP64 values
  *                                   and accumulators; six of them make the nf2ff box
  *                                   (python/openEMS/nf2ff.py:61-77)
+ *   fdtd_add_td_box                 ProcessFieldsTD (Common/processfields_td.cpp:54-91)
+ *   fdtd_add_dft_dump_box,          the same for the dump types E/H/J/rotH/D/B and the
+ *   fdtd_add_td_dump_box,           three interpolations (Common/processfields.cpp:280-400,
+ *   fdtd_set_material               FDTD/engine_interface_fdtd.cpp:60-288) with the
+ *                                   operator's stored materials (FDTD/operator.cpp:1176-1188)
  *
  * Conventions: every function returns 0 on success and a negative FDTD_ERR_*
  * code on failure; fdtd_last_error() returns the message of the last failure
@@ -251,6 +256,46 @@ typedef void (*fdtd_td_sink)(void* user, int id, uint32_t ts, double time, const
 int  fdtd_add_td_box(fdtd_engine* h, int field, const uint32_t start[3], const uint32_t stop[3], uint32_t interval,
                      fdtd_td_sink sink, void* user, int* id);
 int  fdtd_td_num_samples(fdtd_engine* h, int id, uint64_t* n);
+
+/* ---- dump types and interpolations of the field boxes ---- */
+/* Dump types of ProcessFields (Common/processfields.h:43): E, H, current
+ * density J = kappa E, rot H, flux densities D = epsilon E and B = mu H.  The
+ * values are those of Engine_Interface_FDTD::GetRawField /
+ * GetRawDualField (FDTD/engine_interface_fdtd.cpp:132-142,257-288). */
+#define FDTD_DUMP_E    0
+#define FDTD_DUMP_H    1
+#define FDTD_DUMP_J    2
+#define FDTD_DUMP_ROTH 3
+#define FDTD_DUMP_D    4
+#define FDTD_DUMP_B    5
+/* Engine_Interface_Base::InterpolationType (Common/engine_interface_base.h:33):
+ * NO_INTERPOLATION, NODE_INTERPOLATE, CELL_INTERPOLATE; the primary-grid
+ * types (E, J, rot H, D) and the dual-grid types (H, B) each follow
+ * GetRawInterpolatedField / GetRawInterpolatedDualField
+ * (FDTD/engine_interface_fdtd.cpp:60-116,143-198). */
+#define FDTD_INTERP_NONE 0
+#define FDTD_INTERP_NODE 1
+#define FDTD_INTERP_CELL 2
+/* Averaged material of every edge as the operator stores it for dumps
+ * (Operator::Calc_ECPos, FDTD/operator.cpp:1176-1188, m_epsR / m_kappa /
+ * m_mueR): kind FDTD_MAT_EPS (absolute permittivity, F/m), FDTD_MAT_KAPPA
+ * (S/m) or FDTD_MAT_MU (absolute permeability, H/m), float [3][Nx][Ny][Nz].
+ * J dumps need kappa, D dumps epsilon and B dumps mu, set before the box is
+ * added (the reference's SetMaterialStoreFlags, openems.cpp:617-624). */
+#define FDTD_MAT_EPS   0
+#define FDTD_MAT_KAPPA 1
+#define FDTD_MAT_MU    2
+int  fdtd_set_material(fdtd_engine* h, int kind, const float* values);
+/* fdtd_add_dft_box / fdtd_add_td_box for any dump type and interpolation:
+ * fdtd_add_dft_box(h, FDTD_VOLT, ...) is dump type E with node
+ * interpolation and FDTD_CURR dump type H with node interpolation.  Only
+ * H boxes use the dual time (ts + 1/2) dT (openems.cpp:595-600); J, rot H,
+ * D and B are sampled at ts dT like E.  Values outside the ranges, or a
+ * missing material, fail with FDTD_ERR_ARG / FDTD_ERR_STATE. */
+int  fdtd_add_dft_dump_box(fdtd_engine* h, int dump_type, int interp, const uint32_t start[3], const uint32_t stop[3],
+                           const double* freqs, uint32_t n_freq, uint32_t fd_interval, int* id);
+int  fdtd_add_td_dump_box(fdtd_engine* h, int dump_type, int interp, const uint32_t start[3], const uint32_t stop[3],
+                          uint32_t interval, fdtd_td_sink sink, void* user, int* id);
 
 /* ---- time stepping ---- */
 int  fdtd_iterate(fdtd_engine* h, uint32_t nsteps);

@@ -118,6 +118,14 @@ int  wop_get_signal_period(wop_operator* op, double* period, uint32_t* period_ts
 int  wop_get_coeffs(wop_operator* op, int dtype, void* vv, void* vi, void* ii, void* iv);
 /* EC values after Calc_EC (before the operator build), [3][N] each, double */
 int  wop_get_ec(wop_operator* op, double* C, double* G, double* L, double* R);
+/* Material storage for field dumps (Operator_Base::SetMaterialStoreFlags,
+ * Common/operator_base.cpp:83, with Operator::Calc_ECPos storing the
+ * averaged EffMat[type] of every edge, FDTD/operator.cpp:1176-1188): type 0
+ * absolute permittivity, 1 conductivity kappa, 2 absolute permeability.
+ * Set before the build (full or streamed); wop_get_material then copies the
+ * values, rounded to float as the reference stores them, [3][Nx][Ny][Nz]. */
+int  wop_set_material_store_flags(wop_operator* op, int type, int val);
+int  wop_get_material(wop_operator* op, int type, float* out);
 /* compression of the coefficients: returns the number of unique 12-tuples and
  * (if index/table are non-null) the per-cell index and the table [n][12] */
 int  wop_compress(wop_operator* op, uint32_t* n_entries, uint32_t* index, double* table, uint32_t max_entries);

@@ -75,6 +75,11 @@ _ENGINE_SIGS = {
     "nf2ff_destroy": [c_void_p],
     "nf2ff_set_mirror": [c_void_p, c_int, c_int, c_double],
     "fdtd_add_td_box": [c_void_p, c_int, POINTER(c_uint32), POINTER(c_uint32), c_uint32, c_void_p, c_void_p, POINTER(c_int)],
+    "fdtd_set_material": [c_void_p, c_int, POINTER(c_float)],
+    "fdtd_add_dft_dump_box": [c_void_p, c_int, c_int, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_double), c_uint32,
+                              c_uint32, POINTER(c_int)],
+    "fdtd_add_td_dump_box": [c_void_p, c_int, c_int, POINTER(c_uint32), POINTER(c_uint32), c_uint32, c_void_p, c_void_p,
+                             POINTER(c_int)],
     "fdtd_td_num_samples": [c_void_p, c_int, POINTER(c_uint64)],
     "fdtd_set_steady_state": [c_void_p, c_double, c_uint32, POINTER(c_uint32), POINTER(c_uint32)],
     "fdtd_get_steady_state_period": [c_void_p, POINTER(c_uint32)],
@@ -137,6 +142,8 @@ _OPERATOR_SIGS = {
     "wop_get_signal": [c_void_p, POINTER(c_double), POINTER(c_double)],
     "wop_get_coeffs": [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p],
     "wop_get_ec": [c_void_p, POINTER(c_double), POINTER(c_double), POINTER(c_double), POINTER(c_double)],
+    "wop_set_material_store_flags": [c_void_p, c_int, c_int],
+    "wop_get_material": [c_void_p, c_int, POINTER(c_float)],
     "wop_compress": [c_void_p, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_double), c_uint32],
     "wop_factor": [c_void_p, c_int, c_uint32, POINTER(c_uint32), POINTER(c_float), POINTER(c_uint32), POINTER(c_float),
                    POINTER(c_float), POINTER(c_double)],

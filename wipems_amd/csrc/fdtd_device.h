@@ -1337,14 +1337,30 @@ __global__ void k_step_end(const S* __restrict__ V, const S* __restrict__ I, uns
 // ---------------------------------------------------------------------------
 #define MAX_DFT_FREQ 64
 
+// Field values of dump boxes (Engine_Interface_FDTD,
+// FDTD/engine_interface_fdtd.cpp:60-198,257-288), for the dump types of
+// ProcessFields (Common/processfields.h:43): 0 E, 1 H, 2 J, 3 rot H, 4 D,
+// 5 B, and the interpolation types of Engine_Interface_Base: 0 none, 1 node,
+// 2 cell.  Values are FP64; the material arrays hold the operator's averaged
+// absolute epsilon, kappa and mu per edge as float, [3][Nx][Ny][Nzp] in the
+// padded field layout (null when not set: those types then give 0, as the
+// reference does without stored materials).
 struct DftAxes {
-    const double* len;     // [Nx+Ny+Nz] edge lengths in metres: primary for E, dual for H
+    const double* len;     // [Nx+Ny+Nz] primary edge lengths in metres
+    const double* dlen;    // [Nx+Ny+Nz] dual edge lengths
     int off[3];
     int N[3];
 };
 
+struct DumpMat {
+    const float* eps;
+    const float* kappa;
+    const float* mu;
+};
+
 struct DftDev {
-    int field;             // 0: E from V, 1: H from I
+    int type;              // dump type 0..5
+    int interp;            // 0 none, 1 node, 2 cell
     int s[3], n[3];        // first node and node count per axis
     long long nodes;
     int n_freq;
@@ -1352,62 +1368,136 @@ struct DftDev {
     double2* acc;          // [n_freq][3][nodes]
 };
 
+__device__ __forceinline__ bool dump_is_dual(int type) { return type == 1 || type == 5; }
+
 template <typename S>
-__device__ __forceinline__ double dft_raw(const S* __restrict__ F, const Dims& d, const DftAxes& ax, int n, int x, int y, int z,
-                                          typename StoreTraits<S>::A scale) {
-    typedef StoreTraits<S> T;
-    const int p = n == 0 ? x : (n == 1 ? y : z);
-    const double delta = ax.len[ax.off[n] + p];
-    if (delta == 0.0) return 0.0;
-    const double v = (double)T::load(F + n * d.sc + x * d.sx + y * d.sy + z, scale);
-    return v / delta;
+__device__ __forceinline__ double fval(const S* __restrict__ F, const Dims& d, int n, int x, int y, int z, typename StoreTraits<S>::A scale) {
+    return (double)StoreTraits<S>::load(F + n * d.sc + x * d.sx + y * d.sy + z, scale);
 }
 
-// the interpolated E (field 0, from V) or H (field 1, from I) at node pos
+// GetRawField (type: 0 E, 1 J, 2 rot H, 3 D)
 template <typename S>
-__device__ __forceinline__ void node_field(const S* __restrict__ F, const Dims& d, const Scales<typename StoreTraits<S>::A>& sc,
-                                           const DftAxes& ax, int field, const int pos[3], double out[3]) {
-    if (field == 0) {
+__device__ __forceinline__ double raw_primary(const S* __restrict__ V, const S* __restrict__ I, const Dims& d,
+                                              const Scales<typename StoreTraits<S>::A>& sc, const DftAxes& ax, const DumpMat& m,
+                                              int rt, int n, const int q[3]) {
+    const long long c = (long long)n * d.sc + q[0] * d.sx + q[1] * d.sy + q[2];
+    if (rt == 2) {
+        const int nP = (n + 1) % 3, nPP = (n + 2) % 3;
+        const double area = ax.dlen[ax.off[nP] + q[nP]] * ax.dlen[ax.off[nPP] + q[nPP]];
+        double value = fval<S>(I, d, nPP, q[0], q[1], q[2], sc.c[nPP]);
+        value -= fval<S>(I, d, nP, q[0], q[1], q[2], sc.c[nP]);
+        int l[3] = {q[0], q[1], q[2]};
+        if (q[nPP] > 0) {
+            --l[nPP];
+            value += fval<S>(I, d, nP, l[0], l[1], l[2], sc.c[nP]);
+            ++l[nPP];
+        }
+        if (q[nP] > 0) {
+            --l[nP];
+            value -= fval<S>(I, d, nPP, l[0], l[1], l[2], sc.c[nPP]);
+        }
+        return value / area;
+    }
+    const double value = fval<S>(V, d, n, q[0], q[1], q[2], sc.v[n]);
+    const double delta = ax.len[ax.off[n] + q[n]];
+    if (rt == 0 && delta != 0.0) return value / delta;
+    if (rt == 1 && m.kappa && delta != 0.0) return value * (double)m.kappa[c] / delta;
+    if (rt == 3 && m.eps && delta != 0.0) return value * (double)m.eps[c] / delta;
+    return 0.0;
+}
+
+// GetRawDualField (type: 0 H, 1 B)
+template <typename S>
+__device__ __forceinline__ double raw_dual(const S* __restrict__ I, const Dims& d, const Scales<typename StoreTraits<S>::A>& sc,
+                                           const DftAxes& ax, const DumpMat& m, int rt, int n, const int q[3]) {
+    const long long c = (long long)n * d.sc + q[0] * d.sx + q[1] * d.sy + q[2];
+    const double value = fval<S>(I, d, n, q[0], q[1], q[2], sc.c[n]);
+    const double delta = ax.dlen[ax.off[n] + q[n]];
+    if (rt == 0 && delta != 0.0) return value / delta;
+    if (rt == 1 && m.mu && delta != 0.0) return value * (double)m.mu[c] / delta;
+    return 0.0;
+}
+
+// the dump value of `type` with interpolation `interp` at node pos
+// (GetRawInterpolatedField / GetRawInterpolatedDualField)
+template <typename S>
+__device__ __forceinline__ void dump_value(const S* __restrict__ V, const S* __restrict__ I, const Dims& d,
+                                           const Scales<typename StoreTraits<S>::A>& sc, const DftAxes& ax, const DumpMat& m,
+                                           int type, int interp, const int pos[3], double out[3]) {
+    if (!dump_is_dual(type)) {
+        const int rt = type == 0 ? 0 : (type == 2 ? 1 : (type == 3 ? 2 : 3));
+        if (interp == 2) {
+            const bool edge = pos[0] == ax.N[0] - 1 || pos[1] == ax.N[1] - 1 || pos[2] == ax.N[2] - 1;
+            for (int n = 0; n < 3; ++n) {
+                const int nP = (n + 1) % 3, nPP = (n + 2) % 3;
+                if (edge) { out[n] = 0.0; continue; }
+                int q[3] = {pos[0], pos[1], pos[2]};
+                double o = raw_primary<S>(V, I, d, sc, ax, m, rt, n, q);
+                ++q[nP];
+                o += raw_primary<S>(V, I, d, sc, ax, m, rt, n, q);
+                ++q[nPP];
+                o += raw_primary<S>(V, I, d, sc, ax, m, rt, n, q);
+                --q[nP];
+                o += raw_primary<S>(V, I, d, sc, ax, m, rt, n, q);
+                out[n] = o / 4.0;
+            }
+            return;
+        }
         for (int n = 0; n < 3; ++n) {
-            const typename StoreTraits<S>::A scale = sc.v[n];
             int q[3] = {pos[0], pos[1], pos[2]};
+            if (interp == 0) { out[n] = raw_primary<S>(V, I, d, sc, ax, m, rt, n, q); continue; }
             if (ax.N[n] < 2) { out[n] = 0.0; continue; }
             if (pos[n] == ax.N[n] - 1) {           // upper bound: the lower value only
                 --q[n];
-                out[n] = dft_raw<S>(F, d, ax, n, q[0], q[1], q[2], scale);
+                out[n] = raw_primary<S>(V, I, d, sc, ax, m, rt, n, q);
                 continue;
             }
             const double delta = ax.len[ax.off[n] + pos[n]];
-            out[n] = dft_raw<S>(F, d, ax, n, q[0], q[1], q[2], scale);
+            out[n] = raw_primary<S>(V, I, d, sc, ax, m, rt, n, q);
             if (delta == 0.0) { out[n] = 0.0; continue; }
             if (pos[n] == 0) continue;             // lower bound: the upper value only
             --q[n];
             const double deltaDown = ax.len[ax.off[n] + q[n]];
             const double rel = delta / (delta + deltaDown);
-            out[n] = out[n] * (1.0 - rel) + dft_raw<S>(F, d, ax, n, q[0], q[1], q[2], scale) * rel;
+            out[n] = out[n] * (1.0 - rel) + raw_primary<S>(V, I, d, sc, ax, m, rt, n, q) * rel;
         }
-    } else {
+        return;
+    }
+    const int rt = type == 1 ? 0 : 1;
+    if (interp == 1) {
         const bool edge = pos[0] == ax.N[0] - 1 || pos[1] == ax.N[1] - 1 || pos[2] == ax.N[2] - 1;
         for (int n = 0; n < 3; ++n) {
             const int nP = (n + 1) % 3, nPP = (n + 2) % 3;
-            const typename StoreTraits<S>::A scale = sc.c[n];
             if (edge || pos[nP] == 0 || pos[nPP] == 0) { out[n] = 0.0; continue; }
             int q[3] = {pos[0], pos[1], pos[2]};
-            double o = dft_raw<S>(F, d, ax, n, q[0], q[1], q[2], scale);
+            double o = raw_dual<S>(I, d, sc, ax, m, rt, n, q);
             --q[nP];
-            o += dft_raw<S>(F, d, ax, n, q[0], q[1], q[2], scale);
+            o += raw_dual<S>(I, d, sc, ax, m, rt, n, q);
             --q[nPP];
-            o += dft_raw<S>(F, d, ax, n, q[0], q[1], q[2], scale);
+            o += raw_dual<S>(I, d, sc, ax, m, rt, n, q);
             ++q[nP];
-            o += dft_raw<S>(F, d, ax, n, q[0], q[1], q[2], scale);
+            o += raw_dual<S>(I, d, sc, ax, m, rt, n, q);
             out[n] = o / 4.0;
         }
+        return;
+    }
+    for (int n = 0; n < 3; ++n) {
+        int q[3] = {pos[0], pos[1], pos[2]};
+        out[n] = raw_dual<S>(I, d, sc, ax, m, rt, n, q);
+        if (interp == 0) continue;
+        if (pos[n] >= ax.N[n] - 1) { out[n] = 0.0; continue; }     // outer boundary
+        const double delta = ax.dlen[ax.off[n] + pos[n]];
+        ++q[n];
+        const double deltaUp = ax.dlen[ax.off[n] + q[n]];
+        const double rel = delta / (delta + deltaUp);
+        out[n] = out[n] * (1.0 - rel) + raw_dual<S>(I, d, sc, ax, m, rt, n, q) * rel;
     }
 }
 
 template <typename S>
 __global__ void __launch_bounds__(256)
-k_dft(const S* __restrict__ F, Dims d, Scales<typename StoreTraits<S>::A> sc, DftAxes ax, DftDev b, double t, double g) {
+k_dft(const S* __restrict__ V, const S* __restrict__ I, Dims d, Scales<typename StoreTraits<S>::A> sc, DftAxes ax, DumpMat m, DftDev b,
+      double t, double g) {
     __shared__ double2 ph[MAX_DFT_FREQ];
     for (int k = threadIdx.x; k < b.n_freq; k += blockDim.x) {
         double sn, cs;
@@ -1424,7 +1514,7 @@ k_dft(const S* __restrict__ F, Dims d, Scales<typename StoreTraits<S>::A> sc, Df
     const int i = (int)(r / b.n[1]);
     const int pos[3] = {b.s[0] + i, b.s[1] + j, b.s[2] + k};
     double out[3];
-    node_field<S>(F, d, sc, ax, b.field, pos, out);
+    dump_value<S>(V, I, d, sc, ax, m, b.type, b.interp, pos, out);
     for (int f = 0; f < b.n_freq; ++f) {
         const double2 p = ph[f];
         for (int n = 0; n < 3; ++n) {
@@ -1438,11 +1528,12 @@ k_dft(const S* __restrict__ F, Dims d, Scales<typename StoreTraits<S>::A> sc, Df
 }
 
 // Time-domain field dump of a node box (ProcessFieldsTD::Process via
-// ProcessFields::CalcField, Common/processfields.cpp:280-325): the
-// interpolated field rounded to float, [3][nodes] with nodes z fastest.
+// ProcessFields::CalcField, Common/processfields.cpp:280-400): the dump
+// value rounded to float, [3][nodes] with nodes z fastest.
 template <typename S>
 __global__ void __launch_bounds__(256)
-k_snapshot(const S* __restrict__ F, Dims d, Scales<typename StoreTraits<S>::A> sc, DftAxes ax, DftDev b, float* __restrict__ out) {
+k_snapshot(const S* __restrict__ V, const S* __restrict__ I, Dims d, Scales<typename StoreTraits<S>::A> sc, DftAxes ax, DumpMat m, DftDev b,
+           float* __restrict__ out) {
     const long long node = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (node >= b.nodes) return;
     const int k = (int)(node % b.n[2]);
@@ -1451,7 +1542,7 @@ k_snapshot(const S* __restrict__ F, Dims d, Scales<typename StoreTraits<S>::A> s
     const int i = (int)(r / b.n[1]);
     const int pos[3] = {b.s[0] + i, b.s[1] + j, b.s[2] + k};
     double v[3];
-    node_field<S>(F, d, sc, ax, b.field, pos, v);
+    dump_value<S>(V, I, d, sc, ax, m, b.type, b.interp, pos, v);
     for (int n = 0; n < 3; ++n) out[n * b.nodes + node] = (float)v[n];
 }
 

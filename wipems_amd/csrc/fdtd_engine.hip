@@ -883,28 +883,43 @@ int energy_impl(fdtd_engine* h, double* out) {
     return FDTD_OK;
 }
 
+// edge lengths and stored materials of the dump kernels
+DftAxes dump_axes(const fdtd_engine* h) {
+    const int tot = (int)(h->N[0] + h->N[1] + h->N[2]);
+    DftAxes ax;
+    ax.len = h->edge_len.as<double>();
+    ax.dlen = h->edge_len.as<double>() + tot;
+    ax.off[0] = 0; ax.off[1] = (int)h->N[0]; ax.off[2] = (int)(h->N[0] + h->N[1]);
+    for (int n = 0; n < 3; ++n) ax.N[n] = (int)h->N[n];
+    return ax;
+}
+
+DumpMat dump_mat(const fdtd_engine* h) {
+    DumpMat m;
+    m.eps = h->mat[FDTD_MAT_EPS].p ? h->mat[FDTD_MAT_EPS].as<float>() : nullptr;
+    m.kappa = h->mat[FDTD_MAT_KAPPA].p ? h->mat[FDTD_MAT_KAPPA].as<float>() : nullptr;
+    m.mu = h->mat[FDTD_MAT_MU].p ? h->mat[FDTD_MAT_MU].as<float>() : nullptr;
+    return m;
+}
+
 template <typename S>
 int launch_dft(fdtd_engine* h, DftHost* b, uint32_t ts) {
     typedef typename StoreTraits<S>::A A;
-    const int tot = (int)(h->N[0] + h->N[1] + h->N[2]);
-    DftAxes ax;
-    ax.len = h->edge_len.as<double>() + (b->field ? tot : 0);
-    ax.off[0] = 0; ax.off[1] = (int)h->N[0]; ax.off[2] = (int)(h->N[0] + h->N[1]);
-    for (int n = 0; n < 3; ++n) ax.N[n] = (int)h->N[n];
     DftDev dd;
-    dd.field = b->field;
+    dd.type = b->type;
+    dd.interp = b->interp;
     for (int n = 0; n < 3; ++n) { dd.s[n] = (int)b->s[n]; dd.n[n] = (int)b->n[n]; }
     dd.nodes = b->nodes;
     dd.n_freq = (int)b->freqs.size();
     dd.freqs = b->dfreq.as<double>();
     dd.acc = b->acc.as<double2>();
     // Engine_Interface_FDTD::GetTime (engine_interface_fdtd.h:52); H uses the dual time
-    const double t = ((double)ts + (b->field ? 0.5 : 0.0)) * h->dT;
+    const double t = ((double)ts + (b->type == FDTD_DUMP_H ? 0.5 : 0.0)) * h->dT;
     const double g = h->dT * (double)b->interval;
     const long long blocks = (b->nodes + 255) / 256;
     if (blocks > 0x7fffffffLL) return fail(FDTD_ERR_RANGE, "DFT box too large");
-    const S* F = b->field ? h->I.as<S>() : h->V.as<S>();
-    hipLaunchKernelGGL((k_dft<S>), dim3((unsigned)blocks), dim3(256), 0, h->stream, F, h->d, make_scales<A>(h), ax, dd, t, g);
+    hipLaunchKernelGGL((k_dft<S>), dim3((unsigned)blocks), dim3(256), 0, h->stream, h->V.as<S>(), h->I.as<S>(), h->d, make_scales<A>(h),
+                       dump_axes(h), dump_mat(h), dd, t, g);
     ++b->samples;
     return FDTD_OK;
 }
@@ -914,7 +929,7 @@ int drain_td_slot(fdtd_engine* h, TdHost* b, int slot) {
     if (!b->pending[slot]) return FDTD_OK;
     HIPCHK(hipEventSynchronize(b->ev[slot]));
     b->pending[slot] = false;
-    const double t = ((double)b->pts[slot] + (b->field ? 0.5 : 0.0)) * h->dT;
+    const double t = ((double)b->pts[slot] + (b->type == FDTD_DUMP_H ? 0.5 : 0.0)) * h->dT;
     b->sink(b->user, b->id, b->pts[slot], t, b->host[slot], (uint64_t)3 * b->nodes);
     return FDTD_OK;
 }
@@ -934,18 +949,13 @@ int launch_td(fdtd_engine* h, TdHost* b, uint32_t ts) {
     const int slot = (int)(b->count & 1);
     int rc;
     if ((rc = drain_td_slot(h, b, slot))) return rc;
-    const int tot = (int)(h->N[0] + h->N[1] + h->N[2]);
-    DftAxes ax;
-    ax.len = h->edge_len.as<double>() + (b->field ? tot : 0);
-    ax.off[0] = 0; ax.off[1] = (int)h->N[0]; ax.off[2] = (int)(h->N[0] + h->N[1]);
-    for (int n = 0; n < 3; ++n) ax.N[n] = (int)h->N[n];
     DftDev dd{};
-    dd.field = b->field;
+    dd.type = b->type;
+    dd.interp = b->interp;
     for (int n = 0; n < 3; ++n) { dd.s[n] = (int)b->s[n]; dd.n[n] = (int)b->n[n]; }
     dd.nodes = b->nodes;
-    const S* F = b->field ? h->I.as<S>() : h->V.as<S>();
-    hipLaunchKernelGGL((k_snapshot<S>), dim3((unsigned)((b->nodes + 255) / 256)), dim3(256), 0, h->stream, F, h->d, make_scales<A>(h), ax, dd,
-                       b->buf.as<float>());
+    hipLaunchKernelGGL((k_snapshot<S>), dim3((unsigned)((b->nodes + 255) / 256)), dim3(256), 0, h->stream, h->V.as<S>(), h->I.as<S>(), h->d,
+                       make_scales<A>(h), dump_axes(h), dump_mat(h), dd, b->buf.as<float>());
     HIPCHK(hipMemcpyAsync(b->host[slot], b->buf.p, (size_t)3 * b->nodes * sizeof(float), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipEventRecord(b->ev[slot], h->stream));
     b->pending[slot] = true;
@@ -1763,15 +1773,54 @@ int fdtd_set_edge_lengths(fdtd_engine* h, const double* primary, const double* d
     return FDTD_OK;
 }
 
+// dump type and interpolation of a new field box, and the state it needs
+static int check_dump(const fdtd_engine* h, int dump_type, int interp, const char* what) {
+    if (dump_type < FDTD_DUMP_E || dump_type > FDTD_DUMP_B) return fail(FDTD_ERR_ARG, "dump_type must be FDTD_DUMP_E .. FDTD_DUMP_B");
+    if (interp < FDTD_INTERP_NONE || interp > FDTD_INTERP_CELL) return fail(FDTD_ERR_ARG, "interp must be FDTD_INTERP_NONE, _NODE or _CELL");
+    if (!h->has_edges) return fail(FDTD_ERR_STATE, "fdtd_set_edge_lengths must be called before %s", what);
+    const int need = dump_type == FDTD_DUMP_J ? FDTD_MAT_KAPPA : dump_type == FDTD_DUMP_D ? FDTD_MAT_EPS : dump_type == FDTD_DUMP_B ? FDTD_MAT_MU : -1;
+    static const char* const names[3] = {"FDTD_MAT_EPS", "FDTD_MAT_KAPPA", "FDTD_MAT_MU"};
+    if (need >= 0 && !h->mat[need].p) return fail(FDTD_ERR_STATE, "this dump type needs fdtd_set_material(%s) before %s", names[need], what);
+    return FDTD_OK;
+}
+
+int fdtd_set_material(fdtd_engine* h, int kind, const float* values) {
+    if (!h || !values) return fail(FDTD_ERR_ARG, "null argument");
+    if (kind < FDTD_MAT_EPS || kind > FDTD_MAT_MU) return fail(FDTD_ERR_ARG, "kind must be FDTD_MAT_EPS, FDTD_MAT_KAPPA or FDTD_MAT_MU");
+    const size_t nd = (size_t)h->N[0] * h->N[1] * h->N[2];
+    DevBuf& buf = h->mat[kind];
+    int rc;
+    if (!buf.p && (rc = buf.alloc((size_t)3 * h->d.sc * sizeof(float)))) return rc;
+    HIPCHK(hipMemsetAsync(buf.p, 0, buf.bytes, h->stream));
+    DevBuf dense;
+    if ((rc = dense.alloc(nd * sizeof(float)))) return rc;
+    for (int n = 0; n < 3; ++n) {
+        if ((rc = stage_h2d(h->stream, dense.p, values + (size_t)n * nd, nd * sizeof(float)))) return rc;
+        HIPCHK(hipMemcpy2DAsync((char*)buf.p + (size_t)n * h->d.sc * sizeof(float), (size_t)h->d.sy * sizeof(float), dense.p,
+                                (size_t)h->N[2] * sizeof(float), (size_t)h->N[2] * sizeof(float), (size_t)h->N[0] * h->N[1],
+                                hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+    }
+    return FDTD_OK;
+}
+
 int fdtd_add_dft_box(fdtd_engine* h, int field, const uint32_t start[3], const uint32_t stop[3],
                      const double* freqs, uint32_t n_freq, uint32_t fd_interval, int* id) {
-    if (!h || !start || !stop || !freqs || !id) return fail(FDTD_ERR_ARG, "null argument");
+    if (!h) return fail(FDTD_ERR_ARG, "null handle");
     if (field != FDTD_VOLT && field != FDTD_CURR) return fail(FDTD_ERR_ARG, "field must be FDTD_VOLT (E) or FDTD_CURR (H)");
-    if (!h->has_edges) return fail(FDTD_ERR_STATE, "fdtd_set_edge_lengths must be called before fdtd_add_dft_box");
+    return fdtd_add_dft_dump_box(h, field == FDTD_CURR ? FDTD_DUMP_H : FDTD_DUMP_E, FDTD_INTERP_NODE, start, stop, freqs, n_freq, fd_interval, id);
+}
+
+int fdtd_add_dft_dump_box(fdtd_engine* h, int dump_type, int interp, const uint32_t start[3], const uint32_t stop[3],
+                          const double* freqs, uint32_t n_freq, uint32_t fd_interval, int* id) {
+    if (!h || !start || !stop || !freqs || !id) return fail(FDTD_ERR_ARG, "null argument");
+    int rc;
+    if ((rc = check_dump(h, dump_type, interp, "a DFT box is added"))) return rc;
     if (n_freq < 1 || n_freq > MAX_DFT_FREQ) return fail(FDTD_ERR_RANGE, "n_freq must be 1..%d", MAX_DFT_FREQ);
     if (fd_interval < 1) return fail(FDTD_ERR_ARG, "fd_interval must be >= 1");
     auto b = std::unique_ptr<DftHost>(new DftHost());
-    b->field = field == FDTD_CURR ? 1 : 0;
+    b->type = dump_type;
+    b->interp = interp;
     b->nodes = 1;
     for (int n = 0; n < 3; ++n) {
         if (start[n] > stop[n] || stop[n] >= h->N[n]) return fail(FDTD_ERR_RANGE, "DFT box out of the mesh on axis %d", n);
@@ -1781,7 +1830,6 @@ int fdtd_add_dft_box(fdtd_engine* h, int field, const uint32_t start[3], const u
     }
     b->interval = fd_interval;
     b->freqs.assign(freqs, freqs + n_freq);
-    int rc;
     const size_t bytes = (size_t)n_freq * 3 * b->nodes * 2 * sizeof(double);
     if ((rc = b->acc.alloc(bytes))) return rc;
     if ((rc = b->dfreq.alloc(n_freq * sizeof(double)))) return rc;
@@ -1795,12 +1843,20 @@ int fdtd_add_dft_box(fdtd_engine* h, int field, const uint32_t start[3], const u
 
 int fdtd_add_td_box(fdtd_engine* h, int field, const uint32_t start[3], const uint32_t stop[3], uint32_t interval,
                     fdtd_td_sink sink, void* user, int* id) {
-    if (!h || !start || !stop || !sink || !id) return fail(FDTD_ERR_ARG, "null argument");
+    if (!h) return fail(FDTD_ERR_ARG, "null handle");
     if (field != FDTD_VOLT && field != FDTD_CURR) return fail(FDTD_ERR_ARG, "field must be FDTD_VOLT (E) or FDTD_CURR (H)");
-    if (!h->has_edges) return fail(FDTD_ERR_STATE, "fdtd_set_edge_lengths must be called before fdtd_add_td_box");
+    return fdtd_add_td_dump_box(h, field == FDTD_CURR ? FDTD_DUMP_H : FDTD_DUMP_E, FDTD_INTERP_NODE, start, stop, interval, sink, user, id);
+}
+
+int fdtd_add_td_dump_box(fdtd_engine* h, int dump_type, int interp, const uint32_t start[3], const uint32_t stop[3],
+                         uint32_t interval, fdtd_td_sink sink, void* user, int* id) {
+    if (!h || !start || !stop || !sink || !id) return fail(FDTD_ERR_ARG, "null argument");
+    int rc;
+    if ((rc = check_dump(h, dump_type, interp, "a dump box is added"))) return rc;
     if (interval < 1) return fail(FDTD_ERR_ARG, "interval must be >= 1");
     auto b = std::unique_ptr<TdHost>(new TdHost());
-    b->field = field == FDTD_CURR ? 1 : 0;
+    b->type = dump_type;
+    b->interp = interp;
     b->nodes = 1;
     for (int n = 0; n < 3; ++n) {
         if (start[n] > stop[n] || stop[n] >= h->N[n]) return fail(FDTD_ERR_RANGE, "dump box out of the mesh on axis %d", n);
@@ -1814,7 +1870,6 @@ int fdtd_add_td_box(fdtd_engine* h, int field, const uint32_t start[3], const ui
     b->user = user;
     b->id = (int)h->td.size();
     const size_t bytes = (size_t)3 * b->nodes * sizeof(float);
-    int rc;
     if ((rc = b->buf.alloc(bytes))) return rc;
     for (int k = 0; k < 2; ++k) {
         HIPCHK(hipHostMalloc((void**)&b->host[k], bytes, hipHostMallocDefault));

@@ -58,7 +58,8 @@ struct ProbeHost {
 };
 
 struct DftHost {
-    int field = 0;                 // 0: E, 1: H
+    int type = 0;                  // dump type FDTD_DUMP_E .. FDTD_DUMP_B
+    int interp = 1;                // FDTD_INTERP_NONE / NODE / CELL
     uint32_t s[3] = {0, 0, 0}, n[3] = {0, 0, 0};
     uint32_t interval = 0;
     long long nodes = 0;
@@ -71,7 +72,8 @@ struct DftHost {
 // is copied to one of two pinned host slots; a slot is handed to the sink
 // when it is needed again or when fdtd_iterate returns
 struct TdHost {
-    int field = 0;                 // 0: E, 1: H
+    int type = 0;                  // dump type FDTD_DUMP_E .. FDTD_DUMP_B
+    int interp = 1;                // FDTD_INTERP_NONE / NODE / CELL
     uint32_t s[3] = {0, 0, 0}, n[3] = {0, 0, 0};
     uint32_t interval = 0;
     long long nodes = 0;
@@ -222,6 +224,7 @@ struct fdtd_engine {
     std::vector<TdHost*> td;
     DevBuf edge_len;              // [2][Nx+Ny+Nz]: primary, then dual edge lengths (metres)
     bool has_edges = false;
+    DevBuf mat[3];                // averaged absolute epsilon, kappa, mu per edge: float [3] padded arrays
 
     // energy
     DevBuf e_part, e_out;

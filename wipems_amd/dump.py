@@ -3,8 +3,8 @@
 * ``H5Dump``: one dump file (ProcessFields::InitProcess writes the mesh of the
   box and the version attribute, Common/processfields.cpp:126-158).
 * ``TdDump``: a time-domain dump box on an Engine (ProcessFieldsTD): the engine
-  snapshots the node-interpolated field on the device and hands every sample
-  to this writer while stepping continues.
+  evaluates the dump type (E, H, J, rot H, D, B) with its interpolation on the
+  device and hands every sample to this writer while stepping continues.
 * ``write_fd_dump``: the frequency-domain result of a field box
   (ProcessFieldsFD::DumpFDData) after the run.
 """
@@ -71,18 +71,45 @@ class H5Dump:
             pass
 
 
-def box_lines(lines, start, stop):
-    return [np.asarray(lines[n], dtype=np.float64)[start[n]:stop[n] + 1] for n in range(3)]
+def box_lines(lines, start, stop, dual=False):
+    """The dump file's mesh of nodes start..stop: the primary lines, or with
+    dual=True the dual nodes of Operator::GetDiscLine (FDTD/operator.cpp:136-149),
+    half way to the next line and half a cell past the last one."""
+    out = []
+    for n in range(3):
+        l = np.asarray(lines[n], dtype=np.float64)
+        if not dual:
+            out.append(l[start[n]:stop[n] + 1].copy())
+            continue
+        idx = np.arange(start[n], stop[n] + 1)
+        nxt = np.where(idx + 1 < len(l), l[np.minimum(idx + 1, len(l) - 1)], 0.0)
+        last = l[idx] + 0.5 * (l[idx] - l[np.maximum(idx - 1, 0)])
+        out.append(np.where(idx + 1 < len(l), 0.5 * (l[idx] + nxt), last))
+    return out
+
+
+def dump_on_dual_mesh(dump_type, interp):
+    """Whether a dump's file mesh is the dual one: H dumps start on the dual
+    mesh (openems.cpp:595-600), cell interpolation moves any dump there and
+    node interpolation to the primary mesh (ProcessFields::SetDumpMode,
+    Common/processfields.cpp:53-61)."""
+    if interp == 2:
+        return True
+    if interp == 1:
+        return False
+    return dump_type == 1
 
 
 class TdDump:
     """Time-domain dump box writing every sample of `engine` to `path`
-    (field 0 = E, 1 = H; nodes start..stop; every `interval` steps)."""
+    (dump type `field`: 0 E, 1 H, 2 J, 3 rot H, 4 D, 5 B; interpolation
+    `interp`: 0 none, 1 node, 2 cell; nodes start..stop; every `interval`
+    steps).  `lines` is the main mesh; the file holds the mesh the dump is on."""
 
-    def __init__(self, engine, path, field, start, stop, interval, lines, unit):
-        self.file = H5Dump(path, box_lines(lines, start, stop), unit)
+    def __init__(self, engine, path, field, start, stop, interval, lines, unit, interp=1):
+        self.file = H5Dump(path, box_lines(lines, start, stop, dump_on_dual_mesh(field, interp)), unit)
         self.samples = 0
-        self.id = engine.add_td_box(field, start, stop, interval, self._sink)
+        self.id = engine.add_td_box(field, start, stop, interval, self._sink, interp=interp)
 
     def _sink(self, ts, time, ptr, count):
         self.file.write_td(ts, time, ptr)
@@ -96,7 +123,7 @@ def write_fd_dump(path, engine, box_id, lines, unit):
     """The accumulators of frequency-domain box `box_id` (fdtd_read_dft) as a
     /FieldData/FD dump file."""
     b = engine.dft_boxes[box_id]
-    f = H5Dump(path, box_lines(lines, b.start, b.stop), unit)
+    f = H5Dump(path, box_lines(lines, b.start, b.stop, dump_on_dual_mesh(b.field, getattr(b, "interp", 1))), unit)
     try:
         f.write_fd(b.freqs, engine.read_dft(box_id))
     finally:

@@ -208,19 +208,33 @@ class Engine:
         dual = np.ascontiguousarray(dual, dtype=np.float64)
         self._chk(self._lib.fdtd_set_edge_lengths(self._h, _dp(primary), _dp(dual)))
 
+    def set_material(self, kind, values):
+        """Averaged material per edge for J (kind 1, kappa), D (kind 0,
+        epsilon) and B (kind 2, mu) dumps (fdtd_set_material): float
+        [3][Nx][Ny][Nz], as Operator.materials holds them."""
+        v = np.ascontiguousarray(values, dtype=np.float32)
+        if v.shape != (3,) + tuple(self.num_lines):
+            raise ValueError("material array must be [3][Nx][Ny][Nz], got %s" % (v.shape,))
+        self._chk(self._lib.fdtd_set_material(self._h, int(kind), v.ctypes.data_as(POINTER(c_float))))
+
     def add_dft_box(self, b):
+        """Frequency-domain box b (DftBoxDef): dump type b.field, interpolation
+        b.interp (fdtd_add_dft_dump_box)."""
         fr = np.ascontiguousarray(b.freqs, dtype=np.float64)
         st = (c_uint32 * 3)(*b.start)
         sp = (c_uint32 * 3)(*b.stop)
         i = c_int()
-        self._chk(self._lib.fdtd_add_dft_box(self._h, int(b.field), st, sp, _dp(fr), len(fr), int(b.fd_interval), byref(i)))
+        self._chk(self._lib.fdtd_add_dft_dump_box(self._h, int(b.field), int(getattr(b, "interp", 1)), st, sp, _dp(fr), len(fr),
+                                                  int(b.fd_interval), byref(i)))
         self.dft_boxes.append(b)
         return i.value
 
-    def add_td_box(self, field, start, stop, interval, fn):
-        """Time-domain dump box (fdtd_add_td_box): fn(ts, time, ptr, count) gets
-        every sample, ptr a ctypes float pointer to [3][nx][ny][nz] valid
-        during the call.  Errors raised by fn surface from the next IterateTS."""
+    def add_td_box(self, field, start, stop, interval, fn, interp=1):
+        """Time-domain dump box (fdtd_add_td_dump_box) of dump type `field`
+        (0 E, 1 H, 2 J, 3 rot H, 4 D, 5 B) with interpolation `interp` (0 none,
+        1 node, 2 cell): fn(ts, time, ptr, count) gets every sample, ptr a
+        ctypes float pointer to [3][nx][ny][nz] valid during the call.  Errors
+        raised by fn surface from the next IterateTS."""
         from .dump import TD_SINK
 
         def sink(user, bid, ts, t, ptr, n):
@@ -233,14 +247,14 @@ class Engine:
         st = (c_uint32 * 3)(*start)
         sp = (c_uint32 * 3)(*stop)
         i = c_int()
-        self._chk(self._lib.fdtd_add_td_box(self._h, int(field), st, sp, int(interval), ctypes.cast(cb, c_void_p), None,
-                                            byref(i)))
+        self._chk(self._lib.fdtd_add_td_dump_box(self._h, int(field), int(interp), st, sp, int(interval), ctypes.cast(cb, c_void_p),
+                                                 None, byref(i)))
         return i.value
 
-    def add_td_dump(self, path, field, start, stop, interval, lines, unit):
+    def add_td_dump(self, path, field, start, stop, interval, lines, unit, interp=1):
         """A TD field dump written to the HDF5 file `path` (wipems_amd.dump.TdDump)."""
         from .dump import TdDump
-        d = TdDump(self, path, field, start, stop, interval, lines, unit)
+        d = TdDump(self, path, field, start, stop, interval, lines, unit, interp=interp)
         self._td_dumps.append(d)
         return d
 
@@ -483,6 +497,8 @@ def engine_from_operator(data: OperatorData, precision="fp32", compressed=None, 
             eng.set_tfsf(which, t["pos"], t["comp"], t["delay"], t["delta"], t["amp"], data.tfsf["max_delay"])
     for p in data.probes:
         eng.add_probe(p)
+    for kind, values in sorted(data.materials.items()):
+        eng.set_material(kind, values)
     if data.dft_boxes:
         eng.set_edge_lengths(data.edge_primary, data.edge_dual)
         for b in data.dft_boxes:

@@ -112,6 +112,8 @@ struct wop_operator {
     uint32_t ring = 0;              // plane slots of the streamed build (0: full arrays)
     std::vector<uint32_t> ring_slot;    // x % ring
     bool streamed = false;          // last build kept no full coefficient arrays
+    bool store_mat[3] = {false, false, false};  // epsilon, kappa, mu kept for dumps
+    std::vector<float> mat[3][3];   // [type][n] averaged material per edge, x-major cell order
     std::vector<double> stream_table;   // its unique coefficient rows [n][12]
 
     // ---- results ----
@@ -174,6 +176,15 @@ struct wop_operator {
     // `ring` plane slots
     size_t idx(uint32_t x, uint32_t y, uint32_t z) const { return ((size_t)(ring ? ring_slot[x] : x) * N[1] + y) * N[2] + z; }
     size_t nd() const { return (size_t)N[0] * N[1] * N[2]; }
+    // material arrays of the flagged types (Operator::InitDataStorage,
+    // FDTD/operator.cpp:850-880); the others are released
+    void InitMaterialStorage() {
+        for (int k = 0; k < 3; ++k)
+            for (int n = 0; n < 3; ++n) {
+                if (store_mat[k]) mat[k][n].assign(nd(), 0.0f);
+                else std::vector<float>().swap(mat[k][n]);
+            }
+    }
 
     double DiscLine(int n, uint32_t pos, bool dual) const {
         if (pos >= N[n]) return 0.0;
@@ -516,8 +527,11 @@ struct Builder {
                 std::vector<int> cand = op->RowCandidates((int)pos[0], (int)pos[1], P_MATERIAL);
                 for (pos[2] = 0; pos[2] < op->N[2]; ++pos[2]) {
                     const size_t i = op->idx(pos[0], pos[1], pos[2]);
+                    const size_t li = ((size_t)pos[0] * op->N[1] + pos[1]) * op->N[2] + pos[2];
                     for (int n = 0; n < 3; ++n) {
                         op->AverageMatQuarterCell(n, pos, EffMat, cand);
+                        for (int k = 0; k < 3; ++k)
+                            if (op->store_mat[k]) op->mat[k][n][li] = (float)EffMat[k];
                         double delta = op->EdgeLength(n, pos);
                         double area = op->EdgeArea(n, pos);
                         double e0, e1, e2, e3;
@@ -1607,6 +1621,7 @@ int wop_operator::build(uint32_t nr_ts) {
     sf.clear();
     sd.clear();
     FillEdgeTables();
+    InitMaterialStorage();
     ring = 0;
     streamed = false;
     stream_table.clear();
@@ -1937,6 +1952,23 @@ int wop_get_edge_lengths(wop_operator* op, int dual, double* out) {
     return 0;
 }
 
+int wop_set_material_store_flags(wop_operator* op, int type, int val) {
+    if (!op) return fail(-1, "bad argument");
+    if (type < 0 || type > 2) return fail(-1, "material type must be 0 (epsilon), 1 (kappa) or 2 (mu)");
+    op->store_mat[type] = val != 0;
+    return 0;
+}
+
+int wop_get_material(wop_operator* op, int type, float* out) {
+    if (!op || !out) return fail(-1, "bad argument");
+    if (type < 0 || type > 2) return fail(-1, "material type must be 0 (epsilon), 1 (kappa) or 2 (mu)");
+    if (!op->built) return fail(-1, "operator not built");
+    if (op->mat[type][0].size() != op->nd()) return fail(-1, "material type %d was not stored by the last build", type);
+    const size_t n = op->nd();
+    for (int d = 0; d < 3; ++d) memcpy(out + d * n, op->mat[type][d].data(), n * sizeof(float));
+    return 0;
+}
+
 int wop_get_timestep(wop_operator* op, double* dT) {
     if (!op || !dT || !op->built) return fail(-1, "operator not built");
     *dT = op->dT;
@@ -2142,6 +2174,7 @@ int stream_build(wop_operator* op, uint32_t nr_ts, uint32_t* index, uint32_t max
     op->sf.clear();
     op->sd.clear();
     op->FillEdgeTables();
+    op->InitMaterialStorage();
     op->built = false;
     op->prec = 0;
     op->ring_slot.resize(N[0]);

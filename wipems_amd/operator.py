@@ -147,14 +147,17 @@ class ProbeDef:
 
 @dataclass
 class DftBoxDef:
-    """Frequency-domain field dump over a box of mesh nodes (ProcessFieldsFD,
-    NODE_INTERPOLATE).  field: 0 = E, 1 = H."""
+    """Frequency-domain field dump over a box of mesh nodes (ProcessFieldsFD).
+    field: the dump type of ProcessFields (Common/processfields.h:43), 0 E,
+    1 H, 2 J, 3 rot H, 4 D, 5 B; interp: 0 none, 1 node (the nf2ff boxes'
+    mode), 2 cell (Common/engine_interface_base.h:33)."""
     field: int
     start: tuple
     stop: tuple
     freqs: tuple
     fd_interval: int
     name: str = ""
+    interp: int = 1
 
 
 def fd_interval_for(freqs, dT, nyquist):
@@ -201,6 +204,9 @@ class OperatorData:
     rlc: Optional[RlcCells] = None
     signal_period: float = 0.0             # Excitation::GetSignalPeriod (s), 0 when not periodic
     steady_state: Optional[SteadyState] = None
+    # averaged material per edge kept for dumps (Operator::Calc_ECPos, FDTD/operator.cpp:1176-1188):
+    # kind 0 epsilon, 1 kappa, 2 mu -> float [3][Nx][Ny][Nz]
+    materials: dict = field(default_factory=dict)
 
     @property
     def cells(self):
@@ -279,6 +285,7 @@ class Operator:
         self._chk(self._lib.wop_create(byref(h)))
         self._h = h
         self.num_lines = None
+        self._store_mat = [False, False, False]
 
     def __del__(self):
         try:
@@ -479,6 +486,12 @@ class Operator:
             edges.append(ebuf)
         self._edges = edges
         nl = tuple(int(x) for x in n)
+        materials = {}
+        for kind in range(3):
+            if self._store_mat[kind]:
+                m = np.empty((3,) + nl, dtype=np.float32)
+                self._chk(lib.wop_get_material(h, kind, _fp(m)))
+                materials[kind] = m
         # openems.cpp:1060-1091: a periodic excitation gets a steady-state detection
         ss = None
         if per.value > 0:
@@ -486,7 +499,7 @@ class Operator:
             ss = SteadyState(per.value, spos, sdir)
         return OperatorData(nl, dT.value, prec, *arrs, exc, slabs, ades, nyq.value, mstep.value,
                             tfsf=tfsf, source=self, edge_primary=edges[0], edge_dual=edges[1], mur=mur, rlc=rlc,
-                            signal_period=per.value, steady_state=ss)
+                            signal_period=per.value, steady_state=ss, materials=materials)
 
     def _rlc_cells(self, prec):
         cnt = c_uint32()
@@ -553,6 +566,15 @@ class Operator:
             return None
         self._chk(rc)
         return FactoredCoeffs(index, table[:nu.value].copy(), fv, fc, int(bits), err.value)
+
+    def set_material_store_flags(self, kind, val=True):
+        """Keep the averaged material of every edge for J, D or B dumps
+        (Operator_Base::SetMaterialStoreFlags): kind 0 epsilon, 1 kappa,
+        2 mu.  The next build fills OperatorData.materials[kind]."""
+        if kind not in (0, 1, 2):
+            raise ValueError("material kind must be 0 (epsilon), 1 (kappa) or 2 (mu)")
+        self._chk(self._lib.wop_set_material_store_flags(self._h, int(kind), 1 if val else 0))
+        self._store_mat[kind] = bool(val)
 
     def get_ec(self):
         n = self.num_lines
