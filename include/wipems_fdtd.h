@@ -326,6 +326,16 @@ int  fdtd_tile_order(fdtd_engine* h, int* band_tiles, int* xcd);
  * use. */
 int  fdtd_set_pml_compression(fdtd_engine* h, int mode);
 int  fdtd_pml_compressed(fdtd_engine* h, int* active);
+/* ADE coefficients (Lorentz/Drude orders): the ADE pre-update can read a
+ * uint16 row index per entry into a table of the distinct coefficient
+ * 9-tuples {int, ext, Lorentz} x 3 components of one field instead of the
+ * per-entry lists (the entries of one material share a few tuples).  mode
+ * 1 / -1 (default) = tables where a field has at most 65536 distinct
+ * tuples, 0 = lists.  Applies to the orders added after the call.
+ * fdtd_ade_compressed reports the form of order `order` per field.  Results
+ * are identical. */
+int  fdtd_set_ade_compression(fdtd_engine* h, int mode);
+int  fdtd_ade_compressed(fdtd_engine* h, int order, int* volt, int* curr);
 /* UPML shell launch (vector kernels): when the slabs form a shell around an
  * interior box (as the operator's UPML does), the interior can run in a
  * launch without UPML code and the shell in a launch of its own, instead of

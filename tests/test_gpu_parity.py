@@ -179,6 +179,29 @@ def test_dispersive_ade_tfsf_bitwise(gpu, prec, precision):
     compare(eng, orc)
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_dispersive_ade_coefficient_forms(gpu, mode):
+    """ADE coefficients from per-entry lists (mode 0) and from tables of
+    distinct tuples (mode 1) give the oracle's bits, fused hooks and one
+    launch per hook."""
+    sc = scenarios.dispersive_scenario(num_lines=(41, 37, 45), steps=240, prec=0, n_spheres=5,
+                                       material_scale=1e-3, ts_factor=0.9)
+    d = sc.data
+    print("case: dispersive 41x37x45 ADE coefficient mode %d" % mode, flush=True)
+    for fuse in (1, 0):
+        eng = engine_from_operator(d, precision="fp32", compressed=None, ade_compression=mode)
+        eng.set_fuse_hooks(fuse)
+        for o in range(len(d.ade)):
+            v, c = eng.ade_compressed(o)
+            assert v == (mode == 1 and bool(d.ade[o].flags & 1)) and c == (mode == 1 and bool(d.ade[o].flags & 2))
+        _kinds[id(eng)] = [p.kind for p in d.probes]
+        orc = oracle_from_operator(d, ftz=True)
+        eng.iterate(240)
+        orc.iterate(240)
+        compare(eng, orc)
+        eng.close()
+
+
 def test_lorentz_filled_cavity_bitwise(gpu):
     """Undamped Lorentz pole filling a PEC cavity (the ADE dispersion pin)
     on the GPU, 3000 steps, against the oracle."""

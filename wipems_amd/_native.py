@@ -65,6 +65,8 @@ _ENGINE_SIGS = {
     "fdtd_set_pml_compression": [c_void_p, c_int],
     "fdtd_pml_compressed": [c_void_p, POINTER(c_int)],
     "fdtd_set_pml_split": [c_void_p, c_int],
+    "fdtd_set_ade_compression": [c_void_p, c_int],
+    "fdtd_ade_compressed": [c_void_p, c_int, POINTER(c_int), POINTER(c_int)],
     "fdtd_add_mur": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_uint32],
     "fdtd_set_rlc": [c_void_p, c_uint32, POINTER(c_uint32), POINTER(c_uint32), c_int, c_void_p],
     "nf2ff_create": [c_double, POINTER(c_double), c_uint32, POINTER(c_double), c_uint32, POINTER(c_double), c_double,

@@ -1026,6 +1026,10 @@ struct AdeDev {
     const void* i_int; const void* i_ext; const void* i_lor;
     void* volt_ade; void* volt_lor; void* curr_ade; void* curr_lor;  // state [3][count]
     int volt_on, curr_on, volt_lor_on, curr_lor_on;
+    // coefficient tables (volt, curr; null: the lists above): per entry a
+    // uint16 row of distinct {int0..2, _, ext0..2, _, lor0..2, _}
+    const unsigned short* cidx[2];
+    const void* ctab[2];
     // fused hooks: per entry, bit n set when component n shares its location
     // with a TFSF or excitation entry of the half step (volt, curr); such
     // components are applied through the location's hook program
@@ -1148,22 +1152,25 @@ __device__ __forceinline__ void ade_pre_entry(const S* __restrict__ F, const Ade
     A* J = (A*)(is_curr ? a.curr_ade : a.volt_ade);
     A* JL = (A*)(is_curr ? a.curr_lor : a.volt_lor);
     const int lor_on = is_curr ? a.curr_lor_on : a.volt_lor_on;
+    const A* row = a.ctab[is_curr] ? (const A*)a.ctab[is_curr] + 12 * (size_t)a.cidx[is_curr][i] : nullptr;
 #pragma unroll
     for (int n = 0; n < 3; ++n) {
         const long long q = (long long)n * a.count + i;
         const A f = T::load(F + n * sc_field + a.off[i], is_curr ? sc.c[n] : sc.v[n]);
+        const A ki = row ? row[n] : kint[q];
+        const A ke = row ? row[4 + n] : kext[q];
         if (lor_on) {
             A jl = JL[q];
             A j = J[q];
-            jl = jl + klor[q] * j;
-            j = j * kint[q];
-            j = j + kext[q] * (f - jl);
+            jl = jl + (row ? row[8 + n] : klor[q]) * j;
+            j = j * ki;
+            j = j + ke * (f - jl);
             JL[q] = jl;
             J[q] = j;
         } else {
             A j = J[q];
-            j = j * kint[q];
-            j = j + kext[q] * f;
+            j = j * ki;
+            j = j + ke * f;
             J[q] = j;
         }
     }

@@ -30,6 +30,7 @@
 #include <mutex>
 #include <unordered_map>
 #include <unordered_set>
+#include <string>
 
 thread_local std::string g_last_error;
 
@@ -653,8 +654,15 @@ int launch_step_kernels(fdtd_engine* h, bool timed, bool first, bool last) {
         A* st = a->state.as<A>();
         ad.count = (int)a->count;
         ad.off = a->off.as<long long>();
-        ad.v_int = cf; ad.v_ext = cf + n3; ad.v_lor = cf + 2 * n3;
-        ad.i_int = cf + 3 * n3; ad.i_ext = cf + 4 * n3; ad.i_lor = cf + 5 * n3;
+        if (cf) {
+            ad.v_int = cf; ad.v_ext = cf + n3; ad.v_lor = cf + 2 * n3;
+            ad.i_int = cf + 3 * n3; ad.i_ext = cf + 4 * n3; ad.i_lor = cf + 5 * n3;
+        }
+        for (int w = 0; w < 2; ++w)
+            if (a->ctab[w].p) {
+                ad.cidx[w] = a->cidx[w].as<unsigned short>();
+                ad.ctab[w] = a->ctab[w].as<A>();
+            }
         ad.volt_ade = st; ad.volt_lor = st + n3; ad.curr_ade = st + 2 * n3; ad.curr_lor = st + 3 * n3;
         ad.volt_on = (a->flags & 1) != 0; ad.curr_on = (a->flags & 2) != 0;
         ad.volt_lor_on = (a->flags & 4) != 0; ad.curr_lor_on = (a->flags & 8) != 0;
@@ -1392,7 +1400,7 @@ int fdtd_add_ade(fdtd_engine* h, uint32_t count, const uint32_t* pos, int dtype,
     a->flags = flags;
     const size_t n3 = (size_t)3 * count, ab = h->acc;
     int rc;
-    if ((rc = a->off.alloc(count * sizeof(long long))) || (rc = a->coef.alloc(6 * n3 * ab)) || (rc = a->state.alloc(4 * n3 * ab))) { delete a; return rc; }
+    if ((rc = a->off.alloc(count * sizeof(long long))) || (rc = a->state.alloc(4 * n3 * ab))) { delete a; return rc; }
     std::vector<char> all(6 * n3 * ab, 0), tmp;
     const void* lists[6] = {v_int, v_ext, v_lor, i_int, i_ext, i_lor};
     for (int k = 0; k < 6; ++k) {
@@ -1400,8 +1408,44 @@ int fdtd_add_ade(fdtd_engine* h, uint32_t count, const uint32_t* pos, int dtype,
         if ((rc = convert_host(h, dtype, lists[k], n3, tmp))) { delete a; return rc; }
         memcpy(all.data() + k * n3 * ab, tmp.data(), n3 * ab);
     }
+    // per field, the distinct 9-tuples of the entries' coefficients (the
+    // entries of one material share a few); a uint16 row index per entry
+    // replaces 36 bytes of list reads per ADE pre-update (FP32)
+    bool lists_needed = false;
+    for (int w = 0; w < 2; ++w) {
+        if (!(flags & (1u << w))) continue;
+        std::vector<unsigned short> idx;
+        std::vector<char> tab;
+        if (h->ade_cmp_mode != 0) {
+            std::unordered_map<std::string, unsigned> rows;
+            idx.resize(count);
+            std::string key(9 * ab, '\0');
+            for (uint32_t i = 0; i < count && rows.size() <= 65536; ++i) {
+                for (int k = 0; k < 3; ++k)
+                    for (int n = 0; n < 3; ++n)
+                        memcpy(&key[(k * 3 + n) * ab], all.data() + ((3 * w + k) * n3 + (size_t)n * count + i) * ab, ab);
+                auto it = rows.find(key);
+                if (it == rows.end()) {
+                    if (rows.size() == 65536) { rows.clear(); idx.clear(); break; }
+                    it = rows.emplace(key, (unsigned)rows.size()).first;
+                    tab.resize(rows.size() * 12 * ab, 0);
+                    for (int k = 0; k < 3; ++k) memcpy(&tab[((rows.size() - 1) * 12 + 4 * k) * ab], &key[3 * k * ab], 3 * ab);
+                }
+                idx[i] = (unsigned short)it->second;
+            }
+        }
+        if (idx.empty()) { lists_needed = true; continue; }
+        if ((rc = a->cidx[w].alloc(count * sizeof(unsigned short))) || (rc = a->ctab[w].alloc(tab.size()))) { delete a; return rc; }
+        if (hipMemcpyAsync(a->cidx[w].p, idx.data(), count * sizeof(unsigned short), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+            hipMemcpyAsync(a->ctab[w].p, tab.data(), tab.size(), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+            hipStreamSynchronize(h->stream) != hipSuccess) {
+            delete a;
+            return fail(FDTD_ERR_HIP, "ADE upload failed");
+        }
+    }
+    if (lists_needed && (rc = a->coef.alloc(6 * n3 * ab))) { delete a; return rc; }
     if (hipMemcpyAsync(a->off.p, off.data(), count * sizeof(long long), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
-        hipMemcpyAsync(a->coef.p, all.data(), all.size(), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+        (lists_needed && hipMemcpyAsync(a->coef.p, all.data(), all.size(), hipMemcpyHostToDevice, h->stream) != hipSuccess) ||
         hipMemsetAsync(a->state.p, 0, a->state.bytes, h->stream) != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess) {
         delete a;
         return fail(FDTD_ERR_HIP, "ADE upload failed");
@@ -1929,6 +1973,21 @@ int fdtd_tile_order(fdtd_engine* h, int* band_tiles, int* xcd) {
     const long long tiles = (L.nq / vz + 255) / 256;
     *band_tiles = r.bpp && r.tpb < tiles ? r.tpb : 0;
     *xcd = r.xb;
+    return FDTD_OK;
+}
+
+int fdtd_set_ade_compression(fdtd_engine* h, int mode) {
+    if (!h) return fail(FDTD_ERR_ARG, "null handle");
+    if (mode < -1 || mode > 1) return fail(FDTD_ERR_ARG, "mode must be -1 (auto), 0 or 1");
+    h->ade_cmp_mode = mode;
+    return FDTD_OK;
+}
+
+int fdtd_ade_compressed(fdtd_engine* h, int order, int* volt, int* curr) {
+    if (!h || !volt || !curr) return fail(FDTD_ERR_ARG, "null argument");
+    if (order < 0 || order >= (int)h->ade.size()) return fail(FDTD_ERR_RANGE, "bad ADE order");
+    *volt = h->ade[order]->ctab[0].p ? 1 : 0;
+    *curr = h->ade[order]->ctab[1].p ? 1 : 0;
     return FDTD_OK;
 }
 

@@ -99,6 +99,7 @@ struct AdeHost {
     uint32_t count = 0;
     uint32_t flags = 0;
     DevBuf off, coef, state;   // coef: 6 lists [3][count]; state: 4 lists [3][count]
+    DevBuf cidx[2], ctab[2];   // coefficient tables of the volt / curr lists (AdeDev::cidx, ctab)
     std::vector<long long> host_off;   // the offsets, kept for the fused hook programs
     DevBuf shared[2];                  // fused hooks: AdeDev::shared masks (volt, curr)
 };
@@ -177,6 +178,7 @@ struct fdtd_engine {
     bool curr_hook_faces = false; // a current-side hook writes a cell of those planes
     int ibox[3][2] = {{0, 0}, {0, 0}, {0, 0}};
     int shell_zalign = -1;        // interior z range aligned to this many vectors: -1 auto (fdtd_set_shell_align)
+    int ade_cmp_mode = -1;        // ADE coefficient tables: 0 lists, 1 / -1 tables where they fit (fdtd_set_ade_compression)
     int tile_band = -1;           // tiles per y band of the update launches: -1 auto, 0 whole planes (fdtd_set_tile_order)
     int tile_xcd = -1;            // XCD chunk width in tiles: -1 auto, 0 off
     int pml_split_mode = -1;      // -1 auto (FP16/BF16 storage), 0 off, 1 on

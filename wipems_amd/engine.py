@@ -327,6 +327,16 @@ class Engine:
         """With the UPML split: the interior launch's z range aligned to `vectors` 16-byte vectors (1, 2, 4, 8; -1 auto)."""
         self._chk(self._lib.fdtd_set_shell_align(self._h, int(vectors)))
 
+    def set_ade_compression(self, mode=-1):
+        """ADE coefficient tables for the orders added afterwards: 0 lists, 1 / -1 tables where they fit."""
+        self._chk(self._lib.fdtd_set_ade_compression(self._h, int(mode)))
+
+    def ade_compressed(self, order):
+        """(volt, curr): whether ADE order `order` reads its coefficients from tables."""
+        v, c = c_int(), c_int()
+        self._chk(self._lib.fdtd_ade_compressed(self._h, int(order), byref(v), byref(c)))
+        return bool(v.value), bool(c.value)
+
     def set_tile_order(self, band_tiles=-1, xcd=-1):
         """Tile order of the update launches: band_tiles = tiles per y band (0: whole planes, -1: the
         engine's choice), xcd = n > 0: the XCDs take chunks of n tiles in turn, 0: off, -1: the engine's choice."""
@@ -442,7 +452,7 @@ def pilot_scales(data: OperatorData, steps=None, headroom=1024.0, compressed=Non
 
 
 def engine_from_operator(data: OperatorData, precision="fp32", compressed=None, max_entries=65536, scales=None,
-                         factored=None):
+                         factored=None, ade_compression=-1):
     """Create an Engine and load the operator, extensions and probes into it.
 
     compressed: True -> per-cell index + table (fails if more than max_entries
@@ -451,7 +461,8 @@ def engine_from_operator(data: OperatorData, precision="fp32", compressed=None, 
     factored: True -> factored coefficients (float handles; fails if they do
     not fit 65536 rows), None -> factored for FP16/BF16 handles whose plain
     table does not fit (when the Operator's full build is at hand), False ->
-    never."""
+    never.
+    ade_compression: fdtd_set_ade_compression mode for the ADE orders."""
     eng = Engine(data.num_lines, data.dT, precision)
     use_cmp = compressed
     fac = None
@@ -483,6 +494,8 @@ def engine_from_operator(data: OperatorData, precision="fp32", compressed=None, 
         eng.set_excitation(data.excitation)
     for s in data.upml:
         eng.add_upml(s)
+    if data.ade:
+        eng.set_ade_compression(ade_compression)
     for a in data.ade:
         eng.add_ade(a)
     for m in data.mur:
