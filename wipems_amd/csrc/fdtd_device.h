@@ -66,8 +66,9 @@ struct Scales {            // per-component storage scales (1 for FP32/FP64)
 
 #define MAX_PML_SLABS 6
 // A slab's arrays are stored [3][nx][ny][nzp] with the z range widened to
-// start at za = s[2] rounded down to a multiple of 16 and padded to nzp, so
-// a 16-byte-aligned vector of consecutive z cells of the main arrays maps to
+// start at za = s[2] rounded down to a multiple of the main arrays' vector
+// width (16 bytes of storage) and padded to nzp, a multiple of it, so a
+// 16-byte-aligned vector of consecutive z cells of the main arrays maps to
 // an aligned vector of the slab arrays.  Padding cells hold zeros.
 struct PmlSlab {
     int s[3];              // first line per axis

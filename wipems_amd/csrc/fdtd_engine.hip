@@ -1351,8 +1351,12 @@ int fdtd_add_upml(fdtd_engine* h, const uint32_t start[3], const uint32_t num_li
     }
     PmlSlab b{};
     for (int n = 0; n < 3; ++n) { b.s[n] = (int)start[n]; b.n[n] = (int)num_lines[n]; }
-    b.za = b.s[2] / 16 * 16;
-    b.nzp = (b.s[2] + b.n[2] - b.za + 15) / 16 * 16;
+    // z range aligned to one vector of the main arrays (16 bytes of storage:
+    // 4 cells FP32, 8 FP16/BF16, 2 FP64), so slab rows are as short as the
+    // vectors allow (a z slab of 8 cells: 8 or 12 cells per row, not 16 or 32)
+    const int w = (int)(16 / h->elem);
+    b.za = b.s[2] / w * w;
+    b.nzp = (b.s[2] + b.n[2] - b.za + w - 1) / w * w;
     b.sc = (long long)num_lines[0] * num_lines[1] * b.nzp;
     b.off = h->pml_total;
     b.ioff = h->pml_cells;
