@@ -202,6 +202,40 @@ def test_dispersive_ade_coefficient_forms(gpu, mode):
         eng.close()
 
 
+def test_fused_hooks_early_ade_pre_update(gpu):
+    """The fused hooks run the ADE pre-update of an unshared component right
+    after its apply; shared components (TFSF locations) keep theirs in the
+    other field's hook launch, and a launch sequence's first step redoes all
+    of them.  Electric and magnetic poles (the magnetic lists copied from the
+    electric ones), TFSF, uneven chunks that cross graph and eager steps,
+    and fields set between calls: the oracle's bits, fused and unfused."""
+    import dataclasses
+    sc = scenarios.dispersive_scenario(num_lines=(41, 37, 45), steps=200, prec=0, n_spheres=5, material_scale=1e-3, ts_factor=0.9)
+    d = sc.data
+    a = d.ade[0]
+    lists = list(a.lists)
+    lists[3], lists[4], lists[5] = (np.ascontiguousarray(lists[k] * np.float32(0.5)) if k != 1 else np.ascontiguousarray(lists[k])
+                                    for k in (0, 1, 2))
+    d = dataclasses.replace(d, ade=[dataclasses.replace(a, lists=lists, flags=a.flags | 2 | (8 if a.flags & 4 else 0))])
+    print("case: dispersive 41x37x45, electric + magnetic ADE", flush=True)
+    for fuse in (1, 0):
+        eng = engine_from_operator(d, precision="fp32", compressed=None)
+        eng.set_fuse_hooks(fuse)
+        _kinds[id(eng)] = [p.kind for p in d.probes]
+        orc = oracle_from_operator(d, ftz=True)
+        for c in (1, 19, 2, 33, 17):
+            eng.iterate(c)
+            orc.iterate(c)
+        V = orc.get_field(0) * np.float32(0.75)
+        eng.set_field(0, V)
+        orc.set_field(0, V)
+        for c in (16, 5):
+            eng.iterate(c)
+            orc.iterate(c)
+        compare(eng, orc)
+        eng.close()
+
+
 def test_lorentz_filled_cavity_bitwise(gpu):
     """Undamped Lorentz pole filling a PEC cavity (the ADE dispersion pin)
     on the GPU, 3000 steps, against the oracle."""

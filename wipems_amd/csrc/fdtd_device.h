@@ -1142,11 +1142,13 @@ __device__ __forceinline__ typename StoreTraits<S>::A storage_round(typename Sto
     }
 }
 
+// Engine_Ext_LorentzMaterial::DoPre{Voltage,Current}UpdatesImpl for
+// component n of entry i: J (and J_L) from the field value f and the
+// current J value j
 template <typename S>
-__device__ __forceinline__ void ade_pre_entry(const S* __restrict__ F, const AdeDev& a, int i, long long sc_field,
-                                              const Scales<typename StoreTraits<S>::A>& sc, int is_curr) {
-    typedef StoreTraits<S> T;
-    typedef typename T::A A;
+__device__ __forceinline__ void ade_pre_comp(const AdeDev& a, int i, int n, typename StoreTraits<S>::A f, typename StoreTraits<S>::A j,
+                                             int is_curr) {
+    typedef typename StoreTraits<S>::A A;
     const A* kint = (const A*)(is_curr ? a.i_int : a.v_int);
     const A* kext = (const A*)(is_curr ? a.i_ext : a.v_ext);
     const A* klor = (const A*)(is_curr ? a.i_lor : a.v_lor);
@@ -1154,33 +1156,46 @@ __device__ __forceinline__ void ade_pre_entry(const S* __restrict__ F, const Ade
     A* JL = (A*)(is_curr ? a.curr_lor : a.volt_lor);
     const int lor_on = is_curr ? a.curr_lor_on : a.volt_lor_on;
     const A* row = a.ctab[is_curr] ? (const A*)a.ctab[is_curr] + 12 * (size_t)a.cidx[is_curr][i] : nullptr;
+    const long long q = (long long)n * a.count + i;
+    const A ki = row ? row[n] : kint[q];
+    const A ke = row ? row[4 + n] : kext[q];
+    if (lor_on) {
+        A jl = JL[q];
+        jl = jl + (row ? row[8 + n] : klor[q]) * j;
+        j = j * ki;
+        j = j + ke * (f - jl);
+        JL[q] = jl;
+        J[q] = j;
+    } else {
+        j = j * ki;
+        j = j + ke * f;
+        J[q] = j;
+    }
+}
+
+// the pre-update of entry i from the field F: every component (mode 1) or
+// only those that share their location with another hook (mode 2: the
+// others were updated right after their apply, see k_hooks)
+template <typename S>
+__device__ __forceinline__ void ade_pre_entry(const S* __restrict__ F, const AdeDev& a, int i, long long sc_field,
+                                              const Scales<typename StoreTraits<S>::A>& sc, int is_curr, int mode = 1) {
+    typedef StoreTraits<S> T;
+    typedef typename T::A A;
+    const unsigned m = mode == 2 ? (a.shared[is_curr] ? a.shared[is_curr][i] : 0u) : 7u;
+    if (!m) return;
+    const A* J = (const A*)(is_curr ? a.curr_ade : a.volt_ade);
 #pragma unroll
     for (int n = 0; n < 3; ++n) {
-        const long long q = (long long)n * a.count + i;
+        if (!(m & (1u << n))) continue;
         const A f = T::load(F + n * sc_field + a.off[i], is_curr ? sc.c[n] : sc.v[n]);
-        const A ki = row ? row[n] : kint[q];
-        const A ke = row ? row[4 + n] : kext[q];
-        if (lor_on) {
-            A jl = JL[q];
-            A j = J[q];
-            jl = jl + (row ? row[8 + n] : klor[q]) * j;
-            j = j * ki;
-            j = j + ke * (f - jl);
-            JL[q] = jl;
-            J[q] = j;
-        } else {
-            A j = J[q];
-            j = j * ki;
-            j = j + ke * f;
-            J[q] = j;
-        }
+        ade_pre_comp<S>(a, i, n, f, J[(long long)n * a.count + i], is_curr);
     }
 }
 
 template <typename S>
 __global__ void __launch_bounds__(256)
 k_hooks(S* __restrict__ F, const S* __restrict__ G, const unsigned* __restrict__ step, HookProg hp, TfsfDev t, ExcDev e, AdeSet ade,
-        Scales<typename StoreTraits<S>::A> sc, long long sc_field, int is_curr, int pre_next) {
+        Scales<typename StoreTraits<S>::A> sc, long long sc_field, int is_curr, int pre_next, int pre_self) {
     typedef StoreTraits<S> T;
     typedef typename T::A A;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1234,7 +1249,11 @@ k_hooks(S* __restrict__ F, const S* __restrict__ G, const unsigned* __restrict__
         }
         T::store(F + off, v, is);
     }
-    // dispersive apply of the locations no other hook touches, per entry
+    // dispersive apply of the locations no other hook touches, per entry;
+    // with pre_self, the same thread then runs the pre-update of this field's
+    // next half step for them: no other hook of this or the other field's
+    // half step touches these components again before it, so the value just
+    // stored (rounded to the storage type) and J are its inputs
     for (int o = 0; o < ade.n; ++o) {
         const AdeDev& a = ade.o[o];
         if (i >= a.count || !(is_curr ? a.curr_on : a.volt_on)) continue;
@@ -1244,15 +1263,19 @@ k_hooks(S* __restrict__ F, const S* __restrict__ G, const unsigned* __restrict__
         for (int n = 0; n < 3; ++n) {
             if (m & (1u << n)) continue;
             S* fp = F + n * sc_field + a.off[i];
-            A f = T::load(fp, is_curr ? sc.c[n] : sc.v[n]);
-            f = f - J[(long long)n * a.count + i];
-            T::store(fp, f, is_curr ? sc.ic[n] : sc.iv[n]);
+            const A s = is_curr ? sc.c[n] : sc.v[n];
+            const A is = is_curr ? sc.ic[n] : sc.iv[n];
+            A f = T::load(fp, s);
+            const A j = J[(long long)n * a.count + i];
+            f = f - j;
+            T::store(fp, f, is);
+            if (pre_self) ade_pre_comp<S>(a, i, n, storage_round<S>(f, s, is), j, is_curr);
         }
     }
     if (pre_next) {
         for (int o = 0; o < ade.n; ++o) {
             const AdeDev& a = ade.o[o];
-            if (i < a.count && (is_curr ? a.volt_on : a.curr_on)) ade_pre_entry<S>(G, a, i, sc_field, sc, !is_curr);
+            if (i < a.count && (is_curr ? a.volt_on : a.curr_on)) ade_pre_entry<S>(G, a, i, sc_field, sc, !is_curr, pre_next);
         }
     }
 }

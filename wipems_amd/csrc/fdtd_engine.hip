@@ -692,16 +692,26 @@ int launch_step_kernels(fdtd_engine* h, bool timed, bool first, bool last) {
         curr_ade |= (a->flags & 2) && a->count;
         ade_max = std::max(ade_max, (int)a->count);
     }
-    // one fused hook launch: the post-update hooks of `which` and, with
-    // pre_next, the other field's ADE pre-update
-    auto hooks = [&](int which, bool pre_next) {
+    // one fused hook launch: the post-update hooks of `which`; with
+    // pre_self the ADE pre-update of which's next half step for the
+    // components no other hook touches (see k_hooks); pre_next: the other
+    // field's ADE pre-update, 1 every component, 2 only the shared ones
+    // (the others had theirs in the launch after their own apply)
+    auto shared_any = [&](int w) {
+        for (auto a : h->ade)
+            if ((a->flags & (w == 0 ? 1u : 2u)) && a->count && a->shared[w].p) return true;
+        return false;
+    };
+    auto hooks = [&](int which, int pre_next, bool pre_self) {
         const HookProg& hp = h->hook[which];
+        if (pre_next == 2 && !shared_any(1 - which)) pre_next = 0;
         const int n = std::max(std::max(hp.ntarget, pre_next ? ade_max : 0), (which == 0 ? volt_ade : curr_ade) ? ade_max : 0);
         if (!n) return;
         S* F = which == 0 ? h->V.as<S>() : h->I.as<S>();
         const S* G = which == 0 ? h->I.as<S>() : h->V.as<S>();
         hipLaunchKernelGGL((k_hooks<S>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, F, G, stp, hp,
-                           which == 0 ? h->tfsf_v : h->tfsf_c, which == 0 ? h->exc_v : h->exc_c, aset, sc, h->d.sc, which, pre_next ? 1 : 0);
+                           which == 0 ? h->tfsf_v : h->tfsf_c, which == 0 ? h->exc_v : h->exc_c, aset, sc, h->d.sc, which, pre_next,
+                           pre_self ? 1 : 0);
     };
     mur_phase(0);
     if (!fh || first)
@@ -718,7 +728,10 @@ int launch_step_kernels(fdtd_engine* h, bool timed, bool first, bool last) {
     if (timed) HIPCHK(hipEventRecord(h->ev[evi], h->stream));
     launch_volt_vec<S>(h, cfe, pp);
     if (timed) HIPCHK(hipEventRecord(h->ev[evi + 1], h->stream));
-    if (fh) hooks(0, curr_ade);
+    // the current pre-update of this step: every component at the first step
+    // of a launch sequence, else the shared ones (the others were done right
+    // after the previous step's current apply)
+    if (fh) hooks(0, curr_ade ? (first ? 1 : 2) : 0, volt_ade && !last);
     if (!fh && h->tfsf_v.count)
         launch_tfsf<S>(h, h->V.as<S>(), stp, h->tfsf_v, sc, 0);
     mur_phase(1);
@@ -754,7 +767,10 @@ int launch_step_kernels(fdtd_engine* h, bool timed, bool first, bool last) {
     }
     if (timed) HIPCHK(hipEventRecord(h->ev[evi + 3], h->stream));
     if (fh) {
-        hooks(1, volt_ade && !last);
+        // the next step's voltage pre-update: none after the last step of a
+        // launch sequence (the next one starts with k_ade_pre), else the
+        // shared components (the others ran after this step's voltage apply)
+        hooks(1, volt_ade && !last ? 2 : 0, curr_ade && !last);
         return launch_step_end<S>(h, 1);
     }
     if (h->tfsf_c.count)
