@@ -225,6 +225,16 @@ def dump_value(V, I, ep, ed, mats, dump_type, interp, pos):
     return out
 
 
+def float_ftz(x):
+    """Values rounded to float as the reference stores dump fields
+    (ProcessFields::CalcField, Common/processfields.cpp:280-400, FDTD_FLOAT
+    arrays) on a thread with flush-to-zero set: Engine_sse's constructor
+    calls Denormal::Disable (FDTD/engine_sse.cpp:38-39, tools/denormal.h),
+    so a double that rounds to a float denormal becomes a zero of its sign."""
+    y = np.asarray(x, dtype=np.float64).astype(np.float32)
+    return np.where(np.abs(y) < np.finfo(np.float32).tiny, np.copysign(np.float32(0), y), y)
+
+
 def split_edges(edges, num_lines):
     e = np.asarray(edges, dtype=np.float64)
     N = num_lines

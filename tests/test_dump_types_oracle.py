@@ -15,7 +15,7 @@ nf2ff tests pin.  Runs without a GPU."""
 import numpy as np
 import pytest
 
-from oracle.dft import dump_box, dump_time, interp_e, interp_h, split_edges
+from oracle.dft import dump_box, dump_time, float_ftz, interp_e, interp_h, split_edges
 from wipems_amd.dump import box_lines, dump_on_dual_mesh
 from wipems_amd.operator import BC_PEC, Operator
 
@@ -297,3 +297,15 @@ def test_materials_not_stored_unless_flagged():
     assert set(d.materials) == {1}
     with pytest.raises(ValueError):
         op.set_material_store_flags(3)
+
+
+def test_float_rounding_flushes_denormals():
+    """The reference rounds dump values to float with flush-to-zero set: a
+    double that rounds to a float denormal is 0, FLT_MIN and larger keep
+    their float value, signs stay."""
+    tiny = float(np.finfo(np.float32).tiny)
+    x = np.array([1e-39, -3e-39, tiny, -tiny, 2.5e-38, 1.0 / 3.0, 0.0])
+    y = float_ftz(x)
+    assert y.dtype == np.float32
+    np.testing.assert_array_equal(y, np.array([0.0, -0.0, tiny, -tiny, 2.5e-38, 1.0 / 3.0, 0.0], dtype=np.float32))
+    assert np.signbit(y[1])

@@ -76,7 +76,7 @@ def _case(prec=0):
 @pytest.mark.gpu
 @pytest.mark.parametrize("use_graph", [True, False])
 def test_gpu_td_dump_matches_oracle_interpolation(gpu, tmp_path, use_graph):
-    from oracle.dft import interp_e, interp_h
+    from oracle.dft import float_ftz, interp_e, interp_h
     from oracle.oracle import oracle_from_operator
     from wipems_amd.engine import engine_from_operator
     print("case: TD dump vs oracle interpolation, graph %s" % use_graph, flush=True)
@@ -113,7 +113,7 @@ def test_gpu_td_dump_matches_oracle_interpolation(gpu, tmp_path, use_graph):
             if ts % interval:
                 continue
             F = o.get_field(field)
-            ref = (interp_e(F, lens_e, a, b) if field == 0 else interp_h(F, lens_h, a, b)).astype(np.float32)
+            ref = float_ftz(interp_e(F, lens_e, a, b) if field == 0 else interp_h(F, lens_h, a, b))
             t, got = captured[field][ts]
             np.testing.assert_array_equal(got, ref)
             assert t == (ts + 0.5 * field) * d.dT
@@ -195,7 +195,7 @@ def test_gpu_iterate_splits_combined(gpu):
     frequency-domain samples and time-domain dump samples on one run, with
     chunks that straddle all three: fields bitwise, the steady-state value,
     FD accumulators and TD samples as the oracle gives them."""
-    from oracle.dft import DftBox, interp_e, run_with_boxes
+    from oracle.dft import DftBox, float_ftz, interp_e, run_with_boxes
     from oracle.oracle import oracle_from_operator
     from tests.test_steady_state import line_op
     from wipems_amd.engine import engine_from_operator
@@ -225,7 +225,7 @@ def test_gpu_iterate_splits_combined(gpu):
             sampled0 = True
             ts += 1
             if ts % 7 == 0:
-                ref = interp_e(o.get_field(0), lens, (20, 0, 0), (40, 4, 4)).astype(np.float32)
+                ref = float_ftz(interp_e(o.get_field(0), lens, (20, 0, 0), (40, 4, 4)))
                 np.testing.assert_array_equal(td[ts], ref)
         assert eng.GetLastDiff() == pytest.approx(o.GetLastDiff(), rel=1e-9, abs=2e-12)
     for which in (0, 1):

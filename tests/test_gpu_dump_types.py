@@ -5,13 +5,14 @@ node-by-node restatement in oracle/dft.py evaluated on the oracle's fields.
 The scenario holds a lossy dielectric, magnetic box, so the stored epsilon,
 kappa and mu differ between edges.  The fields are bit-identical to the
 oracle's; the dump values are FP64 on both sides with the same operation
-order, so the float TD samples must match bit for bit.  The FD accumulators
+order, so the float TD samples must match bit for bit (rounded with flush to zero,
+as the reference's main thread rounds them).  The FD accumulators
 use the device's sincos and numpy's cos/sin for the phase, which may differ
 in the last bit: they are compared at max|gpu - oracle| <= 1e-11 * max|oracle|."""
 import numpy as np
 import pytest
 
-from oracle.dft import DftBox, dump_box, dump_time
+from oracle.dft import DftBox, dump_box, dump_time, float_ftz
 from oracle.oracle import oracle_from_operator
 from wipems_amd.engine import engine_from_operator
 from wipems_amd.operator import BC_PEC, DftBoxDef, Operator
@@ -22,7 +23,7 @@ TOL = 1e-11
 STEPS = 24
 
 
-def material_case():
+def material_case(prec=0):
     rng = np.random.default_rng(41)
     lines = [np.concatenate([[0.0], np.cumsum(rng.uniform(0.6, 1.5, n - 1))]) for n in (14, 12, 11)]
     op = Operator()
@@ -35,7 +36,7 @@ def material_case():
     op.add_box(e, (lines[0][6], lines[1][5], lines[2][4]), (lines[0][6], lines[1][5], lines[2][6]), priority=10)
     for k in range(3):
         op.set_material_store_flags(k)
-    d = op.build(0, nr_ts=STEPS)
+    d = op.build(prec, nr_ts=STEPS)
     return op, d
 
 
@@ -75,7 +76,7 @@ def test_td_dump_types_match_oracle():
             if ts % interval:
                 continue
             t, interp = key
-            ref = dump_box(V, I, d.edge_primary, d.edge_dual, d.materials, t, interp, a, b).astype(np.float32)
+            ref = float_ftz(dump_box(V, I, d.edge_primary, d.edge_dual, d.materials, t, interp, a, b))
             tt, got = captured[key][ts]
             np.testing.assert_array_equal(got, ref, err_msg="dump type %d interp %d ts %d" % (t, interp, ts))
             assert tt == dump_time(t, ts, d.dT)
@@ -84,6 +85,47 @@ def test_td_dump_types_match_oracle():
     for key, a, b, interval in boxes:
         assert sorted(captured[key]) == list(range(0, STEPS + 1, interval))
     assert len(seen_nonzero) == 18, sorted(set(captured) - seen_nonzero)
+    eng.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp16", "bf16"])
+def test_td_dump_types_other_storage(precision):
+    """FP64 and FP16/BF16 storage: every dump type and interpolation equals
+    the restatement evaluated on the engine's own fields at the sample step
+    (fdtd_get_field gives the stored values, scaled, in FP64)."""
+    from wipems_amd.engine import pilot_scales
+    op, d = material_case(1 if precision == "fp64" else 0)
+    N = d.num_lines
+    print("case: TD dump types, %s storage" % precision, flush=True)
+    scales = pilot_scales(d, steps=16) if precision in ("fp16", "bf16") else None
+    eng = engine_from_operator(d, precision=precision, compressed=False, scales=scales)
+    eng.set_edge_lengths(d.edge_primary, d.edge_dual)
+    a, b = (0, 0, 0), tuple(k - 1 for k in N)
+    captured = {}
+    for t in range(6):
+        for interp in range(3):
+            captured[(t, interp)] = {}
+
+            def keep(ts, tt, ptr, n, key=(t, interp)):
+                captured[key][ts] = np.ctypeslib.as_array(ptr, shape=(int(n),)).reshape((3,) + N).copy()
+            eng.add_td_box(t, a, b, 4, keep, interp=interp)
+    fields = {}
+    for ts in range(0, STEPS + 1, 4):
+        if ts:
+            eng.IterateTS(4)
+        else:
+            eng.IterateTS(0)
+        fields[ts] = (eng.get_field(0), eng.get_field(1))
+    eng.IterateTS(0)
+    nonzero = 0
+    for (t, interp), got in captured.items():
+        assert sorted(got) == list(range(0, STEPS + 1, 4))
+        for ts, arr in got.items():
+            V, I = fields[ts]
+            ref = float_ftz(dump_box(V, I, d.edge_primary, d.edge_dual, d.materials, t, interp, a, b))
+            np.testing.assert_array_equal(arr, ref, err_msg="%s type %d interp %d ts %d" % (precision, t, interp, ts))
+            nonzero += bool(np.abs(ref).max() > 0)
+    assert nonzero > 50
     eng.close()
 
 
