@@ -11,6 +11,8 @@
 #include <cstdio>
 #include <string>
 #include <vector>
+#include <cmath>
+#include <cfloat>
 
 struct wdump_file {
     hid_t file = -1;
@@ -74,6 +76,14 @@ int write_float_set(hid_t grp, const std::string& name, int rank, const hsize_t*
     return 0;
 }
 
+// a value rounded to float as the reference stores dump fields: on a thread
+// with flush-to-zero set (Engine_sse's constructor calls Denormal::Disable,
+// FDTD/engine_sse.cpp:38-39), so a float denormal becomes a zero of its sign
+inline float to_float_ftz(double v) {
+    const float f = (float)v;
+    return std::fabs(f) < FLT_MIN ? std::copysign(0.0f, f) : f;
+}
+
 // [3][nx][ny][nz] (z fastest) -> [3][nz][ny][nx] (x fastest), the order of
 // HDF5_File_Writer::WriteVectorField's buffer fill
 template <typename F>
@@ -84,7 +94,7 @@ void transpose(const uint32_t n[3], F get, std::vector<float>& out) {
     for (int c = 0; c < 3; ++c)
         for (size_t k = 0; k < nz; ++k)
             for (size_t j = 0; j < ny; ++j)
-                for (size_t i = 0; i < nx; ++i) out[pos++] = (float)get(((c * nx + i) * ny + j) * nz + k);
+                for (size_t i = 0; i < nx; ++i) out[pos++] = to_float_ftz(get(((c * nx + i) * ny + j) * nz + k));
 }
 
 }  // namespace
