@@ -1173,6 +1173,63 @@ __device__ __forceinline__ void ade_pre_comp(const AdeDev& a, int i, int n, type
     }
 }
 
+// Apply (V -= J) of entry i's unshared components and, with pre, their
+// pre-update of the next half step (the value just stored, rounded to the
+// storage type, and J are its inputs).  Every load of the entry comes
+// before its first store, so the components' memory round trips overlap
+// (J and the field are separate arrays the compiler cannot tell apart).
+template <typename S>
+__device__ __forceinline__ void ade_apply_entry(S* __restrict__ F, const AdeDev& a, int i, long long sc_field,
+                                                const Scales<typename StoreTraits<S>::A>& sc, int is_curr, bool pre) {
+    typedef StoreTraits<S> T;
+    typedef typename T::A A;
+    const unsigned m = a.shared[is_curr] ? a.shared[is_curr][i] : 0u;
+    if (m == 7u) return;
+    A* J = (A*)(is_curr ? a.curr_ade : a.volt_ade);
+    A* JL = (A*)(is_curr ? a.curr_lor : a.volt_lor);
+    const int lor_on = pre && (is_curr ? a.curr_lor_on : a.volt_lor_on);
+    const long long off = a.off[i];
+    const A* row = pre && a.ctab[is_curr] ? (const A*)a.ctab[is_curr] + 12 * (size_t)a.cidx[is_curr][i] : nullptr;
+    A f[3], j[3], jl[3], ki[3], ke[3], kl[3];
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+        if (m & (1u << n)) continue;
+        const long long q = (long long)n * a.count + i;
+        f[n] = T::load(F + n * sc_field + off, is_curr ? sc.c[n] : sc.v[n]);
+        j[n] = J[q];
+        if (lor_on) jl[n] = JL[q];
+        if (pre) {
+            ki[n] = row ? row[n] : ((const A*)(is_curr ? a.i_int : a.v_int))[q];
+            ke[n] = row ? row[4 + n] : ((const A*)(is_curr ? a.i_ext : a.v_ext))[q];
+            if (lor_on) kl[n] = row ? row[8 + n] : ((const A*)(is_curr ? a.i_lor : a.v_lor))[q];
+        }
+    }
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+        if (m & (1u << n)) continue;
+        const long long q = (long long)n * a.count + i;
+        const A s = is_curr ? sc.c[n] : sc.v[n];
+        const A is = is_curr ? sc.ic[n] : sc.iv[n];
+        A v = f[n] - j[n];
+        T::store(F + n * sc_field + off, v, is);
+        if (!pre) continue;
+        v = storage_round<S>(v, s, is);
+        A jj = j[n];
+        if (lor_on) {
+            A l = jl[n];
+            l = l + kl[n] * jj;
+            jj = jj * ki[n];
+            jj = jj + ke[n] * (v - l);
+            JL[q] = l;
+            J[q] = jj;
+        } else {
+            jj = jj * ki[n];
+            jj = jj + ke[n] * v;
+            J[q] = jj;
+        }
+    }
+}
+
 // the pre-update of entry i from the field F: every component (mode 1) or
 // only those that share their location with another hook (mode 2: the
 // others were updated right after their apply, see k_hooks)
@@ -1184,12 +1241,17 @@ __device__ __forceinline__ void ade_pre_entry(const S* __restrict__ F, const Ade
     const unsigned m = mode == 2 ? (a.shared[is_curr] ? a.shared[is_curr][i] : 0u) : 7u;
     if (!m) return;
     const A* J = (const A*)(is_curr ? a.curr_ade : a.volt_ade);
+    const long long off = a.off[i];
+    A f[3], j[3];
 #pragma unroll
     for (int n = 0; n < 3; ++n) {
         if (!(m & (1u << n))) continue;
-        const A f = T::load(F + n * sc_field + a.off[i], is_curr ? sc.c[n] : sc.v[n]);
-        ade_pre_comp<S>(a, i, n, f, J[(long long)n * a.count + i], is_curr);
+        f[n] = T::load(F + n * sc_field + off, is_curr ? sc.c[n] : sc.v[n]);
+        j[n] = J[(long long)n * a.count + i];
     }
+#pragma unroll
+    for (int n = 0; n < 3; ++n)
+        if (m & (1u << n)) ade_pre_comp<S>(a, i, n, f[n], j[n], is_curr);
 }
 
 template <typename S>
@@ -1256,21 +1318,7 @@ k_hooks(S* __restrict__ F, const S* __restrict__ G, const unsigned* __restrict__
     // stored (rounded to the storage type) and J are its inputs
     for (int o = 0; o < ade.n; ++o) {
         const AdeDev& a = ade.o[o];
-        if (i >= a.count || !(is_curr ? a.curr_on : a.volt_on)) continue;
-        const A* J = (const A*)(is_curr ? a.curr_ade : a.volt_ade);
-        const unsigned m = a.shared[is_curr] ? a.shared[is_curr][i] : 0u;
-#pragma unroll
-        for (int n = 0; n < 3; ++n) {
-            if (m & (1u << n)) continue;
-            S* fp = F + n * sc_field + a.off[i];
-            const A s = is_curr ? sc.c[n] : sc.v[n];
-            const A is = is_curr ? sc.ic[n] : sc.iv[n];
-            A f = T::load(fp, s);
-            const A j = J[(long long)n * a.count + i];
-            f = f - j;
-            T::store(fp, f, is);
-            if (pre_self) ade_pre_comp<S>(a, i, n, storage_round<S>(f, s, is), j, is_curr);
-        }
+        if (i < a.count && (is_curr ? a.curr_on : a.volt_on)) ade_apply_entry<S>(F, a, i, sc_field, sc, is_curr, pre_self != 0);
     }
     if (pre_next) {
         for (int o = 0; o < ade.n; ++o) {
