@@ -59,6 +59,20 @@ int wdump_write_td(wdump_file* w, uint32_t ts, double time, const float* field);
  * written as float like the reference's complex<float> accumulators. */
 int wdump_write_fd(wdump_file* w, uint32_t n_freq, const double* freqs, const double* acc);
 
+/* SAR dumps of an E-field box with cell interpolation (ProcessFieldsSAR,
+ * Common/processfields_sar.cpp:179-335): kappa, density and volume are the
+ * cells' centre conductivity, mass density and volume [nx][ny][nz]
+ * (wop_get_cell_materials).  raw = 0: local SAR (dump type 20), per
+ * frequency /FieldData/FD/f<k> float [nz][ny][nx] with P = 0.5 * sum_n kappa
+ * |E_n|^2, SAR = P / density (0 where the density is 0), attributes
+ * frequency and power = sum P * volume (SAR_Calculation::CalcLocalSAR /
+ * CalcSARPower, tools/sar_calculation.cpp:162-230).  raw = 1 (type 29): the
+ * E accumulators as f<k>_real / f<k>_imag and /CellData/Conductivity,
+ * /Density, /Volume.  The /FieldData/FD frequency attribute either way.
+ * Mass-averaged SAR (types 21, 22) is not built. */
+int wdump_write_sar(wdump_file* w, uint32_t n_freq, const double* freqs, const double* acc, const float* kappa,
+                    const float* density, const float* volume, int raw);
+
 int wdump_close(wdump_file* w);
 
 #ifdef __cplusplus

@@ -125,6 +125,16 @@ int  wop_get_ec(wop_operator* op, double* C, double* G, double* L, double* R);
  * Set before the build (full or streamed); wop_get_material then copies the
  * values, rounded to float as the reference stores them, [3][Nx][Ny][Nz]. */
 int  wop_set_material_store_flags(wop_operator* op, int type, int val);
+/* Mass density of a material property (kg/m^3; CSPropMaterial's Density),
+ * for SAR. */
+int  wop_set_density(wop_operator* op, int prop, double density);
+/* Per mesh node of the box start..stop (inclusive), [nx][ny][nz] float: the
+ * conductivity and density of the material at the cell's centre (the cell
+ * from that node to the next lines; no material: 0) and the cell volume in
+ * m^3, as ProcessFieldsSAR::DumpFDData gathers them
+ * (Common/processfields_sar.cpp:200-230).  Any output may be null. */
+int  wop_get_cell_materials(wop_operator* op, const uint32_t start[3], const uint32_t stop[3], float* kappa, float* density,
+                            float* volume);
 int  wop_get_material(wop_operator* op, int type, float* out);
 /* compression of the coefficients: returns the number of unique 12-tuples and
  * (if index/table are non-null) the per-cell index and the table [n][12] */

@@ -146,6 +146,9 @@ _OPERATOR_SIGS = {
     "wop_get_ec": [c_void_p, POINTER(c_double), POINTER(c_double), POINTER(c_double), POINTER(c_double)],
     "wop_set_material_store_flags": [c_void_p, c_int, c_int],
     "wop_get_material": [c_void_p, c_int, POINTER(c_float)],
+    "wop_set_density": [c_void_p, c_int, c_double],
+    "wop_get_cell_materials": [c_void_p, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_float), POINTER(c_float),
+                               POINTER(c_float)],
     "wop_compress": [c_void_p, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_double), c_uint32],
     "wop_factor": [c_void_p, c_int, c_uint32, POINTER(c_uint32), POINTER(c_float), POINTER(c_uint32), POINTER(c_float),
                    POINTER(c_float), POINTER(c_double)],
@@ -175,6 +178,8 @@ _DUMP_SIGS = {
                      c_double, POINTER(c_void_p)],
     "wdump_write_td": [c_void_p, c_uint32, c_double, c_void_p],
     "wdump_write_fd": [c_void_p, c_uint32, POINTER(c_double), POINTER(c_double)],
+    "wdump_write_sar": [c_void_p, c_uint32, POINTER(c_double), POINTER(c_double), POINTER(c_float), POINTER(c_float),
+                        POINTER(c_float), c_int],
     "wdump_close": [c_void_p],
 }
 

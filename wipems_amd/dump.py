@@ -7,6 +7,8 @@
   device and hands every sample to this writer while stepping continues.
 * ``write_fd_dump``: the frequency-domain result of a field box
   (ProcessFieldsFD::DumpFDData) after the run.
+* ``write_sar_dump``: local SAR or SAR raw data of an E box with cell
+  interpolation (ProcessFieldsSAR::DumpFDData).
 """
 from __future__ import annotations
 
@@ -58,6 +60,21 @@ class H5Dump:
         assert a.shape == (len(fr), 3) + self.shape, (a.shape, self.shape)
         pairs = a.view(np.float64)
         self._chk(self._lib.wdump_write_fd(self._h, len(fr), _dp(fr), _dp(pairs)))
+
+    def write_sar(self, freqs, acc, kappa, density, volume, raw=False):
+        """Local SAR (raw=False, dump type 20) or SAR raw data (raw=True,
+        type 29) of the cell-interpolated E accumulators acc (complex
+        [n_freq][3][nx][ny][nz]) with the cell materials of
+        Operator.cell_materials (include/wipems_dump.h, wdump_write_sar)."""
+        fr = np.ascontiguousarray(freqs, dtype=np.float64)
+        a = np.ascontiguousarray(acc, dtype=np.complex128)
+        assert a.shape == (len(fr), 3) + self.shape, (a.shape, self.shape)
+        cells = [np.ascontiguousarray(c, dtype=np.float32) for c in (kappa, density, volume)]
+        for c in cells:
+            assert c.shape == self.shape, (c.shape, self.shape)
+        fp = lambda c: c.ctypes.data_as(POINTER(c_float))
+        self._chk(self._lib.wdump_write_sar(self._h, len(fr), _dp(fr), _dp(a.view(np.float64)), fp(cells[0]), fp(cells[1]),
+                                            fp(cells[2]), 1 if raw else 0))
 
     def close(self):
         if self._h:
@@ -126,5 +143,22 @@ def write_fd_dump(path, engine, box_id, lines, unit):
     f = H5Dump(path, box_lines(lines, b.start, b.stop, dump_on_dual_mesh(b.field, getattr(b, "interp", 1))), unit)
     try:
         f.write_fd(b.freqs, engine.read_dft(box_id))
+    finally:
+        f.close()
+
+
+def write_sar_dump(path, engine, box_id, op, raw=False):
+    """Local SAR (or, with raw=True, SAR raw data) of frequency-domain box
+    `box_id`, an E box with cell interpolation, after the run
+    (ProcessFieldsSAR::DumpFDData, Common/processfields_sar.cpp:179-335).
+    op: the Operator the engine was built from, holding the materials'
+    conductivity and density.  Mass-averaged SAR is not built."""
+    b = engine.dft_boxes[box_id]
+    if b.field != 0 or getattr(b, "interp", 1) != 2:
+        raise ValueError("a SAR dump needs an E box with cell interpolation")
+    kappa, density, volume = op.cell_materials(b.start, b.stop)
+    f = H5Dump(path, box_lines(op.lines, b.start, b.stop, True), op.unit)
+    try:
+        f.write_sar(b.freqs, engine.read_dft(box_id), kappa, density, volume, raw=raw)
     finally:
         f.close()

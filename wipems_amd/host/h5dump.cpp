@@ -194,6 +194,78 @@ int wdump_write_fd(wdump_file* w, uint32_t n_freq, const double* freqs, const do
     return rc;
 }
 
+int wdump_write_sar(wdump_file* w, uint32_t n_freq, const double* freqs, const double* acc, const float* kappa, const float* density,
+                    const float* volume, int raw) {
+    if (!w || w->file < 0 || !kappa || !density || !volume || (n_freq && (!freqs || !acc))) return fail("bad argument");
+    hid_t g = open_group(w->file, "/FieldData/FD");
+    if (g < 0) return fail("opening /FieldData/FD failed");
+    int rc = write_attr(g, "frequency", H5T_NATIVE_DOUBLE, freqs, n_freq);
+    const size_t nx = w->n[0], ny = w->n[1], nz = w->n[2], cells = nx * ny * nz, per = 3 * cells;
+    const hsize_t dims3[3] = {nz, ny, nx};
+    const hsize_t dims4[4] = {3, nz, ny, nx};
+    // scalar [nx][ny][nz] -> [nz][ny][nx] (HDF5_File_Writer::WriteScalarField, tools/hdf5_file_writer.cpp:198-213)
+    auto scalar = [&](const std::string& name, hid_t grp, auto get) -> hid_t {
+        w->buf.resize(cells);
+        size_t pos = 0;
+        for (size_t k = 0; k < nz; ++k)
+            for (size_t j = 0; j < ny; ++j)
+                for (size_t i = 0; i < nx; ++i) w->buf[pos++] = to_float_ftz(get((i * ny + j) * nz + k));
+        hid_t ds = -1;
+        rc = write_float_set(grp, name, 3, dims3, w->buf.data(), &ds);
+        return rc ? -1 : ds;
+    };
+    for (uint32_t f = 0; f < n_freq && !rc; ++f) {
+        const double* a = acc + 2 * per * f;
+        char name[40];
+        snprintf(name, sizeof name, "f%u", f);
+        if (raw) {
+            // SAR_RAW_DATA: the E accumulators (Common/processfields_sar.cpp:245-256)
+            for (int part = 0; part < 2 && !rc; ++part) {
+                transpose(w->n, [&](size_t q) { return a[2 * q + part]; }, w->buf);
+                rc = write_float_set(g, std::string(name) + (part ? "_imag" : "_real"), 4, dims4, w->buf.data(), nullptr);
+            }
+            continue;
+        }
+        // local SAR (SAR_Calculation::CalcLocalSAR / CalcLocalPowerDensity with cell
+        // conductivity, tools/sar_calculation.cpp:162-230): per cell
+        // P = 0.5 * sum_n kappa |E_n| |E_n|, SAR = P / density (0 where density is 0),
+        // total power sum(P * volume) in the reference's loop order
+        std::vector<double> sar(cells);
+        double power = 0;
+        for (size_t q = 0; q < cells; ++q) {
+            double l = 0;
+            for (int n = 0; n < 3; ++n) {
+                const double* e = a + 2 * (n * cells + q);
+                const double m = std::hypot(e[0], e[1]);
+                l += (double)kappa[q] * m * m;
+            }
+            l *= 0.5;
+            power += l * (double)volume[q];
+            sar[q] = density[q] > 0 ? l / (double)density[q] : 0.0;
+        }
+        hid_t ds = scalar(name, g, [&](size_t q) { return sar[q]; });
+        if (ds >= 0) {
+            const float fr = (float)freqs[f], pw = to_float_ftz(power);
+            rc = write_attr(ds, "frequency", H5T_NATIVE_FLOAT, &fr, 1);
+            if (!rc) rc = write_attr(ds, "power", H5T_NATIVE_FLOAT, &pw, 1);
+            H5Dclose(ds);
+        }
+    }
+    H5Gclose(g);
+    if (raw && !rc) {
+        hid_t c = open_group(w->file, "/CellData");
+        if (c < 0) return fail("opening /CellData failed");
+        const float* cell[3] = {kappa, density, volume};
+        const char* names[3] = {"Conductivity", "Density", "Volume"};
+        for (int t = 0; t < 3 && !rc; ++t) {
+            hid_t ds = scalar(names[t], c, [&](size_t q) { return (double)cell[t][q]; });
+            if (ds >= 0) H5Dclose(ds);
+        }
+        H5Gclose(c);
+    }
+    return rc;
+}
+
 int wdump_close(wdump_file* w) {
     if (!w) return fail("null handle");
     int rc = 0;

@@ -50,6 +50,7 @@ struct Property {
     int kind = 0;
     bool lorentz = false;
     double eps = 1, kappa = 0, mue = 1, sigma = 0;
+    double density = 0;       // kg/m^3, for SAR (CSPropMaterial's Density)
     double eps_plasma_f = 0, eps_relax_t = 0, eps_lor_f = 0;
     int ny = 0;
     double R = -1, C = -1;
@@ -1790,6 +1791,49 @@ int wop_add_material(wop_operator* op, double eps, double kappa, double mue, dou
     Property p;
     p.kind = P_MATERIAL; p.eps = eps; p.kappa = kappa; p.mue = mue; p.sigma = sigma;
     return add_prop(op, p, id);
+}
+
+int wop_set_density(wop_operator* op, int prop, double density) {
+    if (!op) return fail(-1, "bad argument");
+    if (prop < 0 || prop >= (int)op->props.size() || !(op->props[prop].kind & P_MATERIAL))
+        return fail(-1, "property %d is not a material", prop);
+    if (!(density >= 0)) return fail(-1, "density must be >= 0");
+    op->props[prop].density = density;
+    return 0;
+}
+
+int wop_get_cell_materials(wop_operator* op, const uint32_t start[3], const uint32_t stop[3], float* kappa, float* density, float* volume) {
+    if (!op || !start || !stop) return fail(-1, "bad argument");
+    if (op->lines[0].empty() || op->lines[1].empty() || op->lines[2].empty()) return fail(-1, "mesh not set");
+    for (int n = 0; n < 3; ++n)
+        if (start[n] > stop[n] || stop[n] >= op->N[n]) return fail(-1, "box out of the mesh on axis %d", n);
+    const uint32_t nx = stop[0] - start[0] + 1, ny = stop[1] - start[1] + 1, nz = stop[2] - start[2] + 1;
+    op->parallel_x(nx, [&](uint32_t i) {
+        uint32_t pos[3];
+        pos[0] = start[0] + i;
+        for (uint32_t j = 0; j < ny; ++j) {
+            pos[1] = start[1] + j;
+            for (uint32_t k = 0; k < nz; ++k) {
+                pos[2] = start[2] + k;
+                const size_t q = ((size_t)i * ny + j) * nz + k;
+                // the material at the cell centre (Operator::GetCellCenterMaterialAvgCoord,
+                // FDTD/operator.cpp:1238-1249, then GetPropertyByCoordPriority over the
+                // material primitives, Common/processfields_sar.cpp:200-230)
+                double c[3];
+                for (int n = 0; n < 3; ++n) c[n] = op->DiscLine(n, pos[n], true);
+                const int pr = op->PropertyAt(c, P_MATERIAL);
+                if (kappa) kappa[q] = pr >= 0 ? (float)op->props[pr].kappa : 0.0f;
+                if (density) density[q] = pr >= 0 ? (float)op->props[pr].density : 0.0f;
+                // Operator::GetCellVolume (FDTD/operator.cpp:209-215)
+                if (volume) {
+                    double v = 1;
+                    for (int n = 0; n < 3; ++n) v *= op->EdgeLength(n, pos, false);
+                    volume[q] = (float)v;
+                }
+            }
+        }
+    });
+    return 0;
 }
 
 int wop_add_metal(wop_operator* op, int* id) {

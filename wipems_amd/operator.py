@@ -332,6 +332,25 @@ class Operator:
         self._chk(self._lib.wop_add_material(self._h, eps_r, kappa, mue_r, sigma, byref(i)))
         return i.value
 
+    def set_density(self, prop, density):
+        """Mass density of material `prop` in kg/m^3 for SAR dumps
+        (CSPropMaterial's Density; 0, the default, marks cells SAR skips)."""
+        self._chk(self._lib.wop_set_density(self._h, int(prop), float(density)))
+
+    def cell_materials(self, start, stop):
+        """(kappa, density, volume) of every cell of the nodes start..stop
+        (inclusive), each float32 [nx][ny][nz]: the conductivity and density of
+        the material at the cell's centre and the cell volume in m^3, as
+        ProcessFieldsSAR::DumpFDData gathers them (Common/processfields_sar.cpp:200-230)."""
+        shape = tuple(int(stop[n]) - int(start[n]) + 1 for n in range(3))
+        if min(shape) < 1:
+            raise ValueError("stop must not be below start")
+        out = [np.empty(shape, np.float32) for _ in range(3)]
+        s, e = (c_uint32 * 3)(*start), (c_uint32 * 3)(*stop)
+        fp = lambda a: a.ctypes.data_as(POINTER(c_float))
+        self._chk(self._lib.wop_get_cell_materials(self._h, s, e, fp(out[0]), fp(out[1]), fp(out[2])))
+        return tuple(out)
+
     def add_metal(self):
         i = c_int()
         self._chk(self._lib.wop_add_metal(self._h, byref(i)))
