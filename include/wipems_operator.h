@@ -69,6 +69,13 @@ int  wop_set_threads(wop_operator* op, int nthreads);
 /* 1: the streamed build prints a progress line to stderr every 64 x planes
  * (openEMS' verbose level, openems.h:120 SetVerboseLevel) */
 int  wop_set_verbose(wop_operator* op, int level);
+/* 1 (default): the builds copy a cell's EC, Var3 timestep estimate and
+ * coefficients from the cell below it in z where they are provably the same
+ * (no primitive face near either cell, the mesh around them shifted bit for
+ * bit); 0: every cell is evaluated.  The result is identical either way.  No
+ * counterpart in the reference, whose Calc_EC_Range evaluates every cell
+ * (FDTD/operator.cpp:1784-1809). */
+int  wop_set_row_reuse(wop_operator* op, int on);
 
 /* properties (CSXCAD stand-ins); each returns its id */
 int  wop_add_material(wop_operator* op, double eps_r, double kappa, double mue_r, double sigma, int* id);

@@ -130,3 +130,82 @@ def test_mur_faces_with_upml_and_boundary_excitation():
     full = op.build(0, nr_ts=40)
     assert [m.face for m in full.mur] == [0, 1, 4] and full.mur[0].start_ts > 0
     compare(op, full, 40)
+
+
+def _stepped_lines(rng):
+    """Mesh lines with uniform runs (where cells can be copied along z) and
+    graded stretches between them."""
+    z = [0.0]
+    for step, count in ((1.0, 9), (None, 5), (0.5, 11), (None, 4), (1.25, 8), (None, 2), (0.75, 16)):
+        for _ in range(count):
+            z.append(z[-1] + (step if step else rng.uniform(0.4, 1.3)))
+    return np.array(z)
+
+
+def _reuse_case(kind, reuse):
+    rng = np.random.default_rng(21)
+    if kind == "dipole":
+        sc = scenarios.large_dipole_scenario(num_lines=(37, 33, 41), steps=30, n_boxes=40, pml=4, n_freq=2,
+                                             box_margin=3, dipole_cells=6, threads=3, streamed=False)
+        return sc
+    op = Operator()
+    op.set_row_reuse(reuse)
+    zl = _stepped_lines(rng)
+    lines = [np.cumsum(rng.uniform(0.7, 1.3, 15)), np.arange(13, dtype=np.float64) * 0.8, zl]
+    op.set_mesh(lines, 1e-3)
+    op.set_boundary([BC_PML, BC_PEC, BC_PMC, BC_PEC, BC_PML, BC_PML], [3, 0, 0, 0, 3, 3])
+    op.set_threads(3)
+    # z faces on lines, on the cells' quarter and half points (where sample
+    # points sit), and between them
+    zf = [zl[4], zl[6] + 0.25 * (zl[7] - zl[6]), zl[14] + 0.5 * (zl[15] - zl[14]), zl[22] - 0.125 * (zl[22] - zl[21]),
+          zl[30], zl[33] + 0.75 * (zl[34] - zl[33])]
+    for k in range(len(zf) - 1):
+        m = op.add_material(eps_r=2.0 + k, kappa=0.01 * k, mue_r=1.0 + 0.5 * (k % 2), sigma=100.0 * (k % 3))
+        op.add_box(m, (lines[0][2 + k], lines[1][1], zf[k]), (lines[0][9 + k % 3], lines[1][10 - k], zf[k + 1]), priority=k)
+    if kind == "sphere":
+        s = op.add_material(eps_r=6.0, kappa=0.2)
+        # curved in z inside a uniform run with no box faces
+        op.add_sphere(s, (lines[0][7], lines[1][6], zl[47]), 3.2, priority=20)
+        metal = op.add_metal()
+        op.add_box(metal, (lines[0][3], lines[1][8], zl[26]), (lines[0][5], lines[1][9], zl[29]), priority=30)
+    e = op.add_excitation(0, (0.0, 0.0, 1.0))
+    op.add_box(e, (lines[0][8], lines[1][6], zl[10]), (lines[0][8], lines[1][6], zl[13]), priority=40)
+    le = op.add_lumped_element(2, R=50.0, caps=True)
+    op.add_box(le, (lines[0][11], lines[1][3], zl[17]), (lines[0][11], lines[1][3], zl[19]), priority=41)
+    for k in range(3):
+        op.set_material_store_flags(k)
+    return op
+
+
+@pytest.mark.parametrize("kind", ["graded", "sphere", "dipole"])
+def test_row_reuse_changes_no_bit(kind):
+    """Copying provably equal cells along z (the default) gives the same
+    full build (EC, coefficients, stored materials, timestep) and the same
+    streamed build as evaluating every cell."""
+    out = {}
+    for reuse in (True, False):
+        if kind == "dipole":
+            sc = _reuse_case(kind, reuse)
+            op = sc.op
+            op.set_row_reuse(reuse)
+            full = op.build(0, nr_ts=30)
+        else:
+            op = _reuse_case(kind, reuse)
+            full = op.build(0, nr_ts=30)
+        ec = op.get_ec()
+        st = op.build_streamed(nr_ts=30, max_entries=0)
+        out[reuse] = (full, ec, st)
+    (f1, e1, s1), (f0, e0, s0) = out[True], out[False]
+    assert f1.dT == f0.dT and s1.dT == s0.dT == f0.dT
+    for a, b in zip(e1, e0):
+        np.testing.assert_array_equal(a, b)
+    for name in ("vv", "vi", "ii", "iv"):
+        np.testing.assert_array_equal(getattr(f1, name), getattr(f0, name), err_msg=name)
+    if kind != "dipole":
+        for k in range(3):
+            np.testing.assert_array_equal(f1.materials[k], f0.materials[k])
+    np.testing.assert_array_equal(s1.index, s0.index)
+    np.testing.assert_array_equal(s1.table, s0.table)
+    for a, b in zip(s1.upml, s0.upml):
+        for x, y in zip(a.arrays, b.arrays):
+            np.testing.assert_array_equal(x, y)

@@ -147,6 +147,7 @@ _OPERATOR_SIGS = {
     "wop_set_material_store_flags": [c_void_p, c_int, c_int],
     "wop_get_material": [c_void_p, c_int, POINTER(c_float)],
     "wop_set_density": [c_void_p, c_int, c_double],
+    "wop_set_row_reuse": [c_void_p, c_int],
     "wop_get_cell_materials": [c_void_p, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_float), POINTER(c_float),
                                POINTER(c_float)],
     "wop_compress": [c_void_p, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_double), c_uint32],

@@ -215,7 +215,28 @@ struct wop_operator {
                 edge_len[dl][n].resize(N[n]);
                 for (uint32_t i = 0; i < N[n]; ++i) edge_len[dl][n][i] = DiscDelta(n, i, dl != 0) * gridDelta;
             }
+        // z_shift[z]: the mesh around line z is the mesh around z-1 moved by one
+        // line, bit for bit (raw deltas and both edge-length tables over z-3..z+2),
+        // so every mesh quantity a cell's EC reads is the same for z and z-1
+        const uint32_t nz = N[2];
+        z_shift.assign(nz, 0);
+        if (!row_reuse) return;
+        auto same = [](double a, double b) { return memcmp(&a, &b, sizeof a) == 0; };
+        for (uint32_t z = 3; z + 3 <= nz; ++z) {
+            bool ok = true;
+            for (int k = -2; k <= 2 && ok; ++k) {
+                const int a = (int)z + k, b = a - 1;
+                ok = same(RawDiscDelta(2, a), RawDiscDelta(2, b)) && same(edge_len[0][2][a], edge_len[0][2][b]) &&
+                     same(edge_len[1][2][a], edge_len[1][2][b]);
+            }
+            z_shift[z] = ok;
+        }
     }
+    // reuse of a row's previous cell where the result is provably the same
+    // (EC in ECRow, the Var3 estimate, the coefficients); off = every cell
+    // evaluated, for checking
+    bool row_reuse = true;
+    std::vector<uint8_t> z_shift;
     double EdgeLength(int n, const uint32_t* pos, bool dual = false) const {
         const std::vector<double>& t = edge_len[dual ? 1 : 0][n];
         if (pos[n] < t.size()) return t[pos[n]];
@@ -474,11 +495,45 @@ struct wop_operator {
 namespace {
 
 // ---- coefficient helpers with F rounding at the reference's storage points ----
+// Along one row of cells (x, y fixed, z ascending): whether
+// AverageMatQuarterCell of cell z gives what it gave for cell z-1.  Its
+// sample points for cell z lie in [line z-1, line z+1] and share x and y
+// with those of cell z-1, and a box's inside test along the row changes only
+// at its z faces: with no face of a candidate box in [line z-2, line z+1] and
+// the mesh around z a bit-exact shift of the mesh around z-1 (z_shift), the
+// results are the same.  Spheres are not interval-shaped in z: rows that can
+// see one evaluate every cell.
+struct ZRunReuse {
+    const wop_operator* op;
+    std::vector<double> faces;
+    size_t fp = 0;
+    bool on;
+    ZRunReuse(const wop_operator* o, const std::vector<int>& cand, bool enable) : op(o), on(enable && !o->z_shift.empty()) {
+        for (int c : cand) {
+            const Primitive& p = op->prims[c];
+            if (p.shape != 0) on = false;
+            faces.push_back(p.a[2]);
+            faces.push_back(p.b[2]);
+        }
+        std::sort(faces.begin(), faces.end());
+    }
+    // z ascending over the calls
+    bool same(uint32_t z) {
+        if (!on || !op->z_shift[z]) return false;
+        const double lo = op->lines[2][z - 2], hi = op->lines[2][z + 1];
+        while (fp < faces.size() && faces[fp] < lo) ++fp;
+        return fp == faces.size() || faces[fp] > hi;
+    }
+};
+
 template <typename F>
 struct Builder {
     wop_operator* op;
     std::vector<F> (&EC)[4][3];   // C,G,L,R per direction
     std::vector<F> (&co)[4][3];   // vv, vi, ii, iv per direction
+    // same_prev[i] = 1: ECRow copied cell i's EC (all directions) from the
+    // cell below it in z, whose EC is then bitwise the same
+    std::vector<uint8_t> same_prev;
 
     explicit Builder(wop_operator* o) : op(o), EC(o->store<F>().EC), co(o->store<F>().co) {}
 
@@ -513,6 +568,7 @@ struct Builder {
         const size_t nd = op->nd();
         for (int k = 0; k < 4; ++k)
             for (int n = 0; n < 3; ++n) EC[k][n].assign(nd, (F)0);
+        same_prev.assign(nd, 0);
         op->parallel_x(op->N[0], [&](uint32_t x) {
             for (uint32_t y = 0; y < op->N[1]; ++y) ECRow(x, y);
         });
@@ -526,9 +582,23 @@ struct Builder {
             double EffMat[4];
             {
                 std::vector<int> cand = op->RowCandidates((int)pos[0], (int)pos[1], P_MATERIAL);
+                // a cell's EC is a function of its AverageMatQuarterCell results and
+                // of the mesh around it: where both repeat, copy cell z-1's EC
+                ZRunReuse run(op, cand, !same_prev.empty());
                 for (pos[2] = 0; pos[2] < op->N[2]; ++pos[2]) {
                     const size_t i = op->idx(pos[0], pos[1], pos[2]);
                     const size_t li = ((size_t)pos[0] * op->N[1] + pos[1]) * op->N[2] + pos[2];
+                    if (run.same(pos[2])) {
+                        const size_t ip = op->idx(pos[0], pos[1], pos[2] - 1);
+                        for (int k = 0; k < 4; ++k)
+                            for (int n = 0; n < 3; ++n) EC[k][n][i] = EC[k][n][ip];
+                        for (int k = 0; k < 3; ++k)
+                            if (op->store_mat[k])
+                                for (int n = 0; n < 3; ++n) op->mat[k][n][li] = op->mat[k][n][li - 1];
+                        same_prev[i] = 1;
+                        continue;
+                    }
+                    if (!same_prev.empty()) same_prev[i] = 0;
                     for (int n = 0; n < 3; ++n) {
                         op->AverageMatQuarterCell(n, pos, EffMat, cand);
                         for (int k = 0; k < 3; ++k)
@@ -585,12 +655,30 @@ struct Builder {
         std::vector<double> ymin(N[1], 1e200);
         op->parallel_x(N[1], [&](uint32_t y) {
             uint32_t pos[3] = {x, y, 0};
+            // Var3Cell(n, z) reads EC within one cell of (x, y, z) on each axis
+            // (x and y reflected at the ends, z at most one away).  Where the 3x3
+            // columns around the row all have EC(z-2) = EC(z-1) = EC(z) = EC(z+1),
+            // the estimate at z is the one at z-1, already in the minimum.
+            std::vector<uint8_t> col(N[2], 0);
+            if (!same_prev.empty()) {
+                for (uint32_t z = 0; z < N[2]; ++z) col[z] = 1;
+                auto refl = [](int q, int n) { return q < 0 ? -q - 1 : (q > n - 1 ? 2 * (n - 1) - q + 1 : q); };
+                for (int dx = -1; dx <= 1; ++dx)
+                    for (int dy = -1; dy <= 1; ++dy) {
+                        const int qx = refl((int)x + dx, (int)N[0]), qy = refl((int)y + dy, (int)N[1]);
+                        const size_t i0 = op->idx(qx, qy, 0);
+                        for (uint32_t z = 0; z < N[2]; ++z) col[z] &= same_prev[i0 + z];
+                    }
+            }
             double best = 1e200;
-            for (int n = 0; n < 3; ++n)
-                for (pos[2] = 0; pos[2] < N[2]; ++pos[2]) {
+            for (pos[2] = 0; pos[2] < N[2]; ++pos[2]) {
+                const uint32_t z = pos[2];
+                if (z >= 1 && z + 1 < N[2] && col[z - 1] && col[z] && col[z + 1]) continue;
+                for (int n = 0; n < 3; ++n) {
                     const double newT = Var3Cell(n, pos);
                     if ((newT < best) && (newT > 0.0)) best = newT;
                 }
+            }
             ymin[y] = best;
         });
         double dT = 1e200;
@@ -1015,14 +1103,18 @@ struct Builder {
         loc[1] = ly;
         pos[0] = lx + u.start[0];
         pos[1] = ly + u.start[1];
-        double eff[4], kv[3], ki[3];
+        double effs[3][4], kv[3], ki[3];
         std::vector<int> cand = op->RowCandidates((int)pos[0], (int)pos[1], P_MATERIAL);
+        ZRunReuse run(op, cand, true);
         for (loc[2] = 0; loc[2] < u.n[2]; ++loc[2]) {
             pos[2] = loc[2] + u.start[2];
             const size_t i = op->idx(pos[0], pos[1], pos[2]);
+            // the averaged materials repeat where cell z-1's were computed here
+            const bool again = run.same(pos[2]) && loc[2] > 0;
             for (int n = 0; n < 3; ++n) {
                 const size_t li = (size_t)n * cnt + ((size_t)loc[0] * u.n[1] + loc[1]) * u.n[2] + loc[2];
-                op->AverageMatQuarterCell(n, pos, eff, cand);
+                double* eff = effs[n];
+                if (!again) op->AverageMatQuarterCell(n, pos, eff, cand);
                 GradingKappaTab(n, pos, kv, ki);
                 const int nP = (n + 1) % 3, nPP = (n + 2) % 3;
                 if (((kv[0] + kv[1] + kv[2]) != 0) && (eff[1] < 1e3)) {
@@ -1780,6 +1872,12 @@ int wop_set_verbose(wop_operator* op, int level) {
     return 0;
 }
 
+int wop_set_row_reuse(wop_operator* op, int on) {
+    if (!op) return fail(-1, "bad argument");
+    op->row_reuse = on != 0;
+    return 0;
+}
+
 static int add_prop(wop_operator* op, const Property& p, int* id) {
     op->props.push_back(p);
     if (id) *id = (int)op->props.size() - 1;
@@ -2230,6 +2328,7 @@ int stream_build(wop_operator* op, uint32_t nr_ts, uint32_t* index, uint32_t max
     Builder<float> b(op);
     for (int k = 0; k < 4; ++k)
         for (int d = 0; d < 3; ++d) { b.EC[k][d].assign(3 * plane, 0.0f); b.co[k][d].assign(3 * plane, 0.0f); }
+    b.same_prev.assign(3 * plane, 0);
     // EC planes kept as ids (4 bytes per cell) into their distinct tuples
     std::vector<uint32_t> ec_ids((size_t)N[0] * plane);
     PlaneDedup<std::array<uint32_t, 12>> ec_tab;
@@ -2283,8 +2382,17 @@ int stream_build(wop_operator* op, uint32_t nr_ts, uint32_t* index, uint32_t max
             }
         });
         op->parallel_x(N[1], [&](uint32_t y) {
-            for (int d = 0; d < 3; ++d)
-                for (uint32_t z = 0; z < N[2]; ++z) b.ECOperatorPos(d, op->idx(x, y, z));
+            // the coefficients are a function of the cell's EC tuple and dT alone
+            const uint32_t* id = ec_ids.data() + (size_t)x * plane + (size_t)y * N[2];
+            for (uint32_t z = 0; z < N[2]; ++z) {
+                const size_t i = op->idx(x, y, z);
+                if (op->row_reuse && z > 0 && id[z] == id[z - 1]) {
+                    for (int k = 0; k < 4; ++k)
+                        for (int d = 0; d < 3; ++d) b.co[k][d][i] = b.co[k][d][i - 1];
+                    continue;
+                }
+                for (int d = 0; d < 3; ++d) b.ECOperatorPos(d, i);
+            }
         });
         b.ApplyElectricBC((int)x);
         b.PECPlane(x);

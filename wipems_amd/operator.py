@@ -327,6 +327,11 @@ class Operator:
     def set_verbose(self, level):
         self._chk(self._lib.wop_set_verbose(self._h, int(level)))
 
+    def set_row_reuse(self, on=True):
+        """Copy provably equal cells along z in the builds (default) or
+        evaluate every cell; the result is identical (wop_set_row_reuse)."""
+        self._chk(self._lib.wop_set_row_reuse(self._h, 1 if on else 0))
+
     def add_material(self, eps_r=1.0, kappa=0.0, mue_r=1.0, sigma=0.0):
         i = c_int()
         self._chk(self._lib.wop_add_material(self._h, eps_r, kappa, mue_r, sigma, byref(i)))
