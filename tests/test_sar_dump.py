@@ -169,7 +169,7 @@ def test_cell_material_arguments_are_checked():
 
 
 @pytest.mark.gpu
-def test_gpu_sar_dump_matches_oracle(tmp_path):
+def test_gpu_sar_dump_matches_oracle(gpu, tmp_path):
     """A lossy, dense box excited by a dipole: the SAR file written from the
     engine's E box (cell interpolation) equals, to the FD accumulators'
     tolerance, the SAR of the oracle's accumulators of the same box."""

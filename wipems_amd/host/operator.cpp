@@ -13,15 +13,22 @@
 #include <algorithm>
 #include <array>
 #include <atomic>
+#include <condition_variable>
+#include <functional>
+#include <memory>
+#include <mutex>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <limits>
 #include <string>
 #include <thread>
 #include <unordered_map>
 #include <vector>
+
+#include <unistd.h>
 
 #include "wipems_operator.h"
 
@@ -60,6 +67,89 @@ struct Property {
     int exc_type = 0;
     double exc_val[3] = {0, 0, 0};
     double delay = 0;
+};
+
+// Worker threads kept for the length of one build (PoolScope): a build
+// calls parallel_x several times per x plane, and starting threads per call
+// cost a third of the streamed build.  run(n, fn) calls fn(0..n-1) on the
+// workers and the calling thread and returns when all calls are done; the
+// first exception of any call is rethrown there once every worker has
+// stopped.  A call made from inside fn runs serially on that thread.
+class WorkerPool {
+public:
+    explicit WorkerPool(int workers) {
+        for (int t = 0; t < workers; ++t) th_.emplace_back([this] { loop(); });
+    }
+    ~WorkerPool() {
+        {
+            std::lock_guard<std::mutex> l(m_);
+            stop_ = true;
+        }
+        work_.notify_all();
+        for (auto& t : th_) t.join();
+    }
+    size_t workers() const { return th_.size(); }
+    static bool inside() { return in_pool(); }
+    void run(uint32_t n, const std::function<void(uint32_t)>& fn) {
+        {
+            std::lock_guard<std::mutex> l(m_);
+            job_ = &fn;
+            count_ = n;
+            next_.store(0);
+            busy_ = (int)th_.size();
+            ++gen_;
+        }
+        work_.notify_all();
+        drain();
+        std::unique_lock<std::mutex> l(m_);
+        done_.wait(l, [&] { return busy_ == 0; });
+        job_ = nullptr;
+        if (err_) {
+            std::exception_ptr e = err_;
+            err_ = nullptr;
+            std::rethrow_exception(e);
+        }
+    }
+
+private:
+    static bool& in_pool() {
+        static thread_local bool v = false;
+        return v;
+    }
+    void drain() {
+        in_pool() = true;
+        try {
+            for (uint32_t x; (x = next_.fetch_add(1)) < count_;) (*job_)(x);
+        } catch (...) {
+            std::lock_guard<std::mutex> l(m_);
+            if (!err_) err_ = std::current_exception();
+            next_.store(count_);
+        }
+        in_pool() = false;
+    }
+    void loop() {
+        uint64_t seen = 0;
+        for (;;) {
+            std::unique_lock<std::mutex> l(m_);
+            work_.wait(l, [&] { return stop_ || gen_ != seen; });
+            if (stop_) return;
+            seen = gen_;
+            l.unlock();
+            drain();
+            l.lock();
+            if (--busy_ == 0) done_.notify_one();
+        }
+    }
+    std::vector<std::thread> th_;
+    std::mutex m_;
+    std::condition_variable work_, done_;
+    const std::function<void(uint32_t)>* job_ = nullptr;
+    uint32_t count_ = 0;
+    std::atomic<uint32_t> next_{0};
+    int busy_ = 0;
+    uint64_t gen_ = 0;
+    bool stop_ = false;
+    std::exception_ptr err_;
 };
 
 struct Primitive {
@@ -468,24 +558,37 @@ struct wop_operator {
         return std::max(1, std::min(t, 64));
     }
 
+    // the build's workers (PoolScope); none are kept between calls, so an
+    // idle operator holds no threads
+    mutable std::unique_ptr<WorkerPool> pool;
+    mutable pid_t pool_pid = 0;
+    mutable int pool_depth = 0;
     template <typename Fn>
     void parallel_x(uint32_t nx, Fn fn) const {
         const int T = threads();
-        if (T <= 1 || nx <= 1) {
+        if (T <= 1 || nx <= 1 || WorkerPool::inside()) {
             for (uint32_t x = 0; x < nx; ++x) fn(x);
             return;
         }
-        std::atomic<uint32_t> next(0);
-        std::vector<std::thread> th;
-        for (int t = 0; t < T; ++t)
-            th.emplace_back([&]() {
-                for (;;) {
-                    uint32_t x = next.fetch_add(1);
-                    if (x >= nx) break;
-                    fn(x);
-                }
-            });
-        for (auto& t : th) t.join();
+        if (pool_depth == 0) {
+            std::atomic<uint32_t> next(0);
+            std::vector<std::thread> th;
+            for (int t = 0; t < T; ++t)
+                th.emplace_back([&]() {
+                    for (uint32_t x; (x = next.fetch_add(1)) < nx;) fn(x);
+                });
+            for (auto& t : th) t.join();
+            return;
+        }
+        // a forked child has none of the parent's workers: leave that pool alone
+        if (pool && pool_pid != getpid()) (void)pool.release();
+        if (!pool || (int)pool->workers() != T - 1) {
+            pool_pid = getpid();
+            pool.reset();
+            pool.reset(new WorkerPool(T - 1));
+        }
+        const std::function<void(uint32_t)> f = [&](uint32_t x) { fn(x); };
+        pool->run(nx, f);
     }
 
     template <typename F>
@@ -2066,11 +2169,23 @@ int wop_set_excite_signal(wop_operator* op, int type, double f0, double fc) {
     return 0;
 }
 
+namespace {
+// workers for the length of one build call
+struct PoolScope {
+    const wop_operator* op;
+    explicit PoolScope(const wop_operator* o) : op(o) { ++op->pool_depth; }
+    ~PoolScope() {
+        if (--op->pool_depth == 0) op->pool.reset();
+    }
+};
+}  // namespace
+
 int wop_build(wop_operator* op, int prec, uint32_t nr_ts) {
     if (!op) return fail(-1, "bad argument");
     for (int n = 0; n < 3; ++n)
         if (op->N[n] < 2) return fail(-1, "mesh not set on axis %d", n);
     op->prec = prec;
+    PoolScope scope(op);
     if (prec == 0) return op->build<float>(nr_ts);
     return op->build<double>(nr_ts);
 }
@@ -2446,6 +2561,7 @@ int stream_build(wop_operator* op, uint32_t nr_ts, uint32_t* index, uint32_t max
 extern "C" int wop_build_streamed(wop_operator* op, uint32_t nr_ts, uint32_t* index, uint32_t max_entries, uint32_t* n_entries) {
     if (!op || !index || !n_entries) return fail(-1, "bad argument");
     if (op->lines[0].empty() || op->lines[1].empty() || op->lines[2].empty()) return fail(-1, "mesh not set");
+    PoolScope scope(op);
     try {
         return stream_build(op, nr_ts, index, max_entries, n_entries);
     } catch (const std::bad_alloc&) {
