@@ -21,6 +21,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <type_traits>
 #include <exception>
 #include <limits>
 #include <string>
@@ -231,7 +232,10 @@ struct wop_operator {
 
     struct Upml {
         uint32_t start[3], n[3];
-        std::vector<double> a[6];          // vv, vvfo, vvfn, ii, iifo, iifn [3][n]
+        // vv, vvfo, vvfn, ii, iifo, iifn [3][n], in the build's precision
+        // (a float build keeps af, a double build a)
+        std::vector<double> a[6];
+        std::vector<float> af[6];
     };
     std::vector<Upml> upml;
 
@@ -308,6 +312,7 @@ struct wop_operator {
         // z_shift[z]: the mesh around line z is the mesh around z-1 moved by one
         // line, bit for bit (raw deltas and both edge-length tables over z-3..z+2),
         // so every mesh quantity a cell's EC reads is the same for z and z-1
+        FillXCandidates();      // both builds start here
         const uint32_t nz = N[2];
         z_shift.assign(nz, 0);
         if (!row_reuse) return;
@@ -398,19 +403,46 @@ struct wop_operator {
         }
         return r;
     }
+    // the x (n = 0) or y (n = 1) window of the rows at line p: one cell either
+    // side, widened by its length (quarter-cell and dual sample points stay
+    // within one cell of the row)
+    void RowWindow(int n, int p, double& lo, double& hi) const {
+        lo = lines[n][std::max(0, p - 1)];
+        hi = lines[n][std::min((int)N[n] - 1, p + 1)];
+        double d = (hi - lo);
+        lo -= d; hi += d;
+    }
+    // xcand[x]: the primitives whose bounding box meets the x window of line
+    // x, in ascending index (priority) order; filled at build start, cleared
+    // when the primitives or the mesh change
+    std::vector<std::vector<int>> xcand;
+    void FillXCandidates() {
+        xcand.assign(N[0], {});
+        for (uint32_t x = 0; x < N[0]; ++x) {
+            double lo, hi;
+            RowWindow(0, (int)x, lo, hi);
+            for (size_t i = 0; i < prims.size(); ++i)
+                if (!(prims[i].bb1[0] < lo || prims[i].bb0[0] > hi)) xcand[x].push_back((int)i);
+        }
+    }
     std::vector<int> RowCandidates(int x, int y, int kinds) const {
         double lo[3], hi[3];
         int pos[2] = {x, y};
-        for (int n = 0; n < 2; ++n) {
-            lo[n] = lines[n][std::max(0, pos[n] - 1)];
-            hi[n] = lines[n][std::min((int)N[n] - 1, pos[n] + 1)];
-            // quarter-cell and dual sample points stay within one cell of the row
-            double d = (hi[n] - lo[n]);
-            lo[n] -= d; hi[n] += d;
-        }
+        for (int n = 0; n < 2; ++n) RowWindow(n, pos[n], lo[n], hi[n]);
         lo[2] = -std::numeric_limits<double>::infinity();
         hi[2] = std::numeric_limits<double>::infinity();
-        return Candidates(lo, hi, kinds);
+        if (xcand.size() != N[0]) return Candidates(lo, hi, kinds);
+        // the same test as Candidates over the primitives that pass its x test
+        std::vector<int> r;
+        for (int i : xcand[x]) {
+            const Primitive& p = prims[i];
+            if (!(props[p.prop].kind & kinds)) continue;
+            bool ok = true;
+            for (int n = 0; n < 3; ++n)
+                if (p.bb1[n] < lo[n] || p.bb0[n] > hi[n]) ok = false;
+            if (ok) r.push_back(i);
+        }
+        return r;
     }
 
     // GetMaterial (FDTD/operator.cpp:1251-1298)
@@ -866,6 +898,7 @@ struct Builder {
             int nP = (n + 1) % 3, nPP = (n + 2) % 3;
             const bool in_lo = n != 0 || only_x < 0 || only_x == 0;
             const bool in_hi = n != 0 || only_x < 0 || only_x == (int)N[0] - 1;
+            if (!(dirs[2 * n] && in_lo) && !(dirs[2 * n + 1] && in_hi)) continue;   // nothing on this plane
             for (pos[nP] = lo[nP]; pos[nP] < hi[nP]; ++pos[nP])
                 for (pos[nPP] = lo[nPP]; pos[nPP] < hi[nPP]; ++pos[nPP]) {
                     if (dirs[2 * n] && in_lo) {
@@ -893,6 +926,7 @@ struct Builder {
         for (int n = 0; n < 3; ++n) {
             int nP = (n + 1) % 3, nPP = (n + 2) % 3;
             const bool at = n != 0 || only_x < 0;
+            if (!at && only_x != 0 && only_x != (int)N[0] - 2 && only_x != (int)N[0] - 1) continue;   // nothing on this plane
             for (pos[nP] = lo[nP]; pos[nP] < hi[nP]; ++pos[nP])
                 for (pos[nPP] = lo[nPP]; pos[nPP] < hi[nPP]; ++pos[nPP]) {
                     if (dirs[2 * n] && (at || only_x == 0)) {
@@ -1256,9 +1290,11 @@ struct Builder {
         for (size_t si = 0; si < op->upml.size(); ++si) {
             wop_operator::Upml& u = op->upml[si];
             for (int k = 0; k < 6; ++k) {
-                const std::vector<F>& e = upml_ext[si * 6 + k];
-                u.a[k].assign(e.size(), 0.0);
-                for (size_t i = 0; i < e.size(); ++i) u.a[k][i] = (double)e[i];
+                std::vector<F>& e = upml_ext[si * 6 + k];
+                std::vector<double>().swap(u.a[k]);
+                std::vector<float>().swap(u.af[k]);
+                if constexpr (std::is_same<F, float>::value) u.af[k].swap(e);
+                else u.a[k].swap(e);
             }
         }
         upml_ext.clear();
@@ -1900,6 +1936,7 @@ int wop_set_mesh(wop_operator* op, int axis, const double* lines, uint32_t n) {
     op->lines[axis].assign(lines, lines + n);
     op->N[axis] = n;
     op->built = false;
+    op->xcand.clear();
     for (int dl = 0; dl < 2; ++dl) op->edge_len[dl][axis].clear();
     return 0;
 }
@@ -2134,6 +2171,7 @@ static int add_prim(wop_operator* op, Primitive pr) {
     auto it = std::upper_bound(op->prims.begin(), op->prims.end(), pr,
                                [](const Primitive& a, const Primitive& b) { return a.priority > b.priority; });
     op->prims.insert(it, pr);
+    op->xcand.clear();
     return 0;
 }
 
@@ -2266,6 +2304,13 @@ static void put(const std::vector<double>& src, int dtype, void* dst) {
         for (size_t i = 0; i < src.size(); ++i) o[i] = (float)src[i];
     }
 }
+static void put_f(const std::vector<float>& src, int dtype, void* dst) {
+    if (!dst) return;
+    if (dtype == 1) {
+        double* o = (double*)dst;
+        for (size_t i = 0; i < src.size(); ++i) o[i] = (double)src[i];
+    } else memcpy(dst, src.data(), src.size() * 4);
+}
 
 int wop_get_coeffs(wop_operator* op, int dtype, void* vv, void* vi, void* ii, void* iv) {
     if (!op || !op->built) return fail(-1, "operator not built");
@@ -2340,9 +2385,15 @@ struct PlaneDedup {
     std::vector<Tuple> uniq;
 
     // get(y, z, tuple&) fills the tuple of cell (y, z); out has Ny*Nz ids.
-    // Returns false once the table holds more than max_entries (0 = no limit).
+    // same(y, z) may say that cell (y, z) holds the tuple of (y, z-1), which
+    // is then not built.  Returns false once the table holds more than
+    // max_entries (0 = no limit).
     template <typename Get>
     bool plane(const wop_operator* op, uint32_t* out, size_t max_entries, Get get) {
+        return plane(op, out, max_entries, get, [](uint32_t, uint32_t) { return false; });
+    }
+    template <typename Get, typename Same>
+    bool plane(const wop_operator* op, uint32_t* out, size_t max_entries, Get get, Same same) {
         const uint32_t Ny = op->N[1], Nz = op->N[2];
         rows.resize(Ny);
         op->parallel_x(Ny, [&](uint32_t y) {
@@ -2353,6 +2404,10 @@ struct PlaneDedup {
             Tuple t;
             TupleHash H;
             for (uint32_t z = 0; z < Nz; ++z) {
+                if (prev != UINT32_MAX && same(y, z)) {
+                    o[z] = prev;
+                    continue;
+                }
                 get(y, z, t);
                 uint32_t id = UINT32_MAX;
                 size_t h = 0;
@@ -2452,7 +2507,7 @@ int stream_build(wop_operator* op, uint32_t nr_ts, uint32_t* index, uint32_t max
             const size_t i = op->idx(x, y, z);
             for (int k = 0; k < 4; ++k)
                 for (int d = 0; d < 3; ++d) memcpy(&t[k * 3 + d], &b.EC[k][d][i], 4);
-        });
+        }, [&](uint32_t y, uint32_t z) { return b.same_prev[op->idx(x, y, z)] != 0; });
     };
     // pass 1: timestep
     double dT = 1e200;
@@ -2733,7 +2788,11 @@ int wop_get_upml_range(wop_operator* op, uint32_t k, uint32_t start[3], uint32_t
 int wop_get_upml(wop_operator* op, uint32_t k, int dtype, void* vv, void* vvfo, void* vvfn, void* ii, void* iifo, void* iifn) {
     if (!op || !op->built || k >= op->upml.size()) return fail(-1, "bad UPML index");
     void* dst[6] = {vv, vvfo, vvfn, ii, iifo, iifn};
-    for (int j = 0; j < 6; ++j) put(op->upml[k].a[j], dtype, dst[j]);
+    for (int j = 0; j < 6; ++j) {
+        const wop_operator::Upml& u = op->upml[k];
+        if (!u.af[j].empty() || u.a[j].empty()) put_f(u.af[j], dtype, dst[j]);
+        else put(u.a[j], dtype, dst[j]);
+    }
     return 0;
 }
 
