@@ -2578,6 +2578,13 @@ int stream_build(wop_operator* op, uint32_t nr_ts, uint32_t* index, uint32_t max
                     const double v = (double)b.co[k][d][i];
                     memcpy(&t[k * 3 + d], &v, 8);
                 }
+        }, [&](uint32_t y, uint32_t z) {
+            // the tuple widens the 12 floats, so equal float bits give equal tuples
+            const size_t i = op->idx(x, y, z);
+            for (int k = 0; k < 4; ++k)
+                for (int d = 0; d < 3; ++d)
+                    if (memcmp(&b.co[k][d][i], &b.co[k][d][i - 1], sizeof(float))) return false;
+            return true;
         });
         if (!ok) {
             op->ring = 0;
